@@ -1,0 +1,119 @@
+/*
+ * vt.h -- C ABI of the MI355X (gfx950) ray-traversal and path-tracing core.
+ *
+ * This is the drop-in boundary for metal-vibe-tracer's scene-upload / trace / render-frame path.
+ * Handles are opaque; the caller owns its host arrays (copied at the call); the library owns
+ * device memory.  Ray and hit buffers passed to the trace calls are DEVICE pointers (HIP device
+ * memory on the context's GPU), laid out as structure-of-arrays so every lane of a wavefront
+ * reads consecutive words.  Errors: a vt_status plus a thread-local message (vt_last_error).
+ * Threading: one context per thread; all work of a context is serialised on the stream given
+ * to each call (NULL = the default stream).
+ */
+#ifndef VT_H
+#define VT_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum vt_status {
+    VT_OK = 0,
+    VT_ERROR_INVALID_ARGUMENT = 1,
+    /* The reference throws "The imported meshes exceed the safe GPU memory budget."
+     * (Sources/Acceleration.swift:728-733) for uploads over its budget. */
+    VT_ERROR_OUT_OF_MEMORY = 2,
+    VT_ERROR_DEVICE = 3,
+    VT_ERROR_UNSUPPORTED = 4
+} vt_status;
+
+typedef struct vt_context vt_context;
+typedef struct vt_scene vt_scene;
+
+typedef struct vt_context_desc {
+    int32_t device;         /* HIP device ordinal */
+    uint64_t memory_budget; /* bytes; 0 = max(256 MiB, 1/4 of device memory), as
+                               MaterialLibrary.meshBudget (Sources/Acceleration.swift:610-612) */
+} vt_context_desc;
+
+/* Replaces PathTracerRenderer.init(device:shaders:sharing:) (main.swift:10487) and the
+ * MaterialLibrary's device binding: one HIP device per context. */
+vt_status vt_context_create(const vt_context_desc *desc, vt_context **out);
+void vt_context_destroy(vt_context *ctx);
+/* Message of the last failed call on this thread ("" if none); replaces the NSError built by
+ * MaterialLibrary.error (main.swift:9128). */
+const char *vt_last_error(void);
+
+typedef struct vt_build_options {
+    uint32_t leaf_size; /* triangles per leaf, 1..3 (0: 3) */
+    uint32_t bins;      /* SAH bins per axis, 4..64 (0: 16, Sources/Acceleration.swift:109) */
+    uint32_t max_depth; /* tree depth limit before a median rebuild, <= 24 (0: 24) */
+} vt_build_options;
+
+/* Replaces MaterialLibrary.setMesh(_:) for an indexed mesh (Sources/StudioModel.swift:1480-1504,
+ * Sources/Acceleration.swift:652-782 publishTwoLevel): vertices are MeshVertex records (8 floats:
+ * position, normal, uv; Sources/MeshGeometry.swift:4-13), faces are MeshFace records (4 u32:
+ * three vertex indices and the subset/slot tag).  Builds the 8-wide BVH on the host and uploads
+ * it.  opts may be NULL.  Hit face IDs returned by the trace calls are DOCUMENT face indices
+ * (position in `faces`). */
+vt_status vt_scene_upload_indexed(vt_context *ctx, const float *vertices, uint32_t vertex_count,
+                                  const uint32_t *faces, uint32_t face_count,
+                                  const vt_build_options *opts, vt_scene **out);
+void vt_scene_destroy(vt_scene *scene);
+
+typedef struct vt_scene_info {
+    uint32_t face_count, vertex_count, node_count, depth;
+    uint32_t leaf_size, width, node_bytes, tri_bytes; /* per record */
+    uint64_t device_bytes;                            /* nodes + triangle records */
+    float root_max;     /* max |coordinate| of the root bounds (the traversal enlargement) */
+    float build_seconds;
+} vt_scene_info;
+vt_status vt_scene_get_info(const vt_scene *scene, vt_scene_info *info);
+
+/* Device-resident rays, one array per component (n entries each). */
+typedef struct vt_ray_soa {
+    const float *ox, *oy, *oz;
+    const float *dx, *dy, *dz;
+    const float *tmin, *tmax;
+} vt_ray_soa;
+
+/* Device-resident closest hits.  face = -1 on a miss (then t = tmax, b1 = b2 = 0).  b1, b2
+ * weight vertices b and c, as intersect_mesh_triangle (main.swift:1564-1582).  instance may be
+ * NULL (single-instance scenes write 0). */
+typedef struct vt_hit_soa {
+    float *t, *b1, *b2;
+    int32_t *face;
+    uint32_t *instance;
+} vt_hit_soa;
+
+/* Replaces the software branch of mesh_closest / blas_closest (main.swift:1788-1812,
+ * 1878-1900) over a ray buffer, as the reference's ray-query kernels call trace_scene
+ * (tests/Fix_accel.swift:70-85).  Closest hit in [tmin, tmax); among hits at exactly equal t the
+ * smallest document face index wins.  Asynchronous on `stream` (a hipStream_t, NULL = default). */
+vt_status vt_trace_closest(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, const vt_hit_soa *hits,
+                           void *stream);
+
+/* Replaces mesh_occluded / blas_any (main.swift:1815-1838, 1968-1983): occluded[i] = 1 when any
+ * triangle is hit in [tmin, tmax). */
+vt_status vt_trace_any(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, uint8_t *occluded, void *stream);
+
+/* Work counters of the same traversal (not timed): counters[0] = node records fetched,
+ * counters[1] = triangle records tested, summed over the n rays; any != 0 counts the any-hit
+ * traversal.  counters is a HOST array; the call synchronises the stream. */
+vt_status vt_trace_stats(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, int any, uint64_t counters[2],
+                         void *stream);
+
+/* Host-only build of the same 8-wide BVH the upload builds (no device needed), for inspection:
+ * *nodes receives node_count 80-byte node records, *tris face_count 48-byte triangle records
+ * (layout: metal-vibe-tracer_amd/csrc/bvh8.h).  Free both with vt_host_free.  Mirrors the
+ * reference's host builder entry WideBVH.build (Sources/Acceleration.swift:296-300). */
+vt_status vt_bvh8_build_host(const float *vertices, uint32_t vertex_count, const uint32_t *faces, uint32_t face_count,
+                             const vt_build_options *opts, void **nodes, uint32_t *node_count, void **tris,
+                             uint32_t *depth, float *root_max);
+void vt_host_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,299 @@
+// capi.cpp -- the C ABI (include/vt.h) over the host runtime: device context, scene publish
+// (MaterialLibrary::setMesh) and the trace entry points.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "bvh8.h"
+#include "trace_kernels.h"
+#include "vt.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+vt_status fail(vt_status s, const std::string &msg) {
+    g_error = msg;
+    return s;
+}
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct BudgetError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+vt_status guarded(F &&f) {
+    try {
+        g_error.clear();
+        return f();
+    } catch (const BudgetError &e) {
+        return fail(VT_ERROR_OUT_OF_MEMORY, e.what());
+    } catch (const DeviceError &e) {
+        return fail(VT_ERROR_DEVICE, e.what());
+    } catch (const std::invalid_argument &e) {
+        return fail(VT_ERROR_INVALID_ARGUMENT, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(VT_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(VT_ERROR_DEVICE, e.what());
+    }
+}
+
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() { reset(); }
+    void reset() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    void upload(const void *src, size_t n) {
+        reset();
+        if (n == 0) return;
+        check(hipMalloc(&ptr, n), "hipMalloc");
+        bytes = n;
+        check(hipMemcpy(ptr, src, n, hipMemcpyHostToDevice), "hipMemcpy");
+    }
+};
+
+}  // namespace
+
+struct vt_context {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t resident = 0;  // bytes of live scenes
+};
+
+// One published mesh: the reference's MeshStreams + BLAS nodes (Sources/Acceleration.swift:59-64).
+struct vt_scene {
+    vt_context *ctx = nullptr;
+    DeviceBuffer nodes, tris, counters;
+    vt::TraceScene view{};
+    vt_scene_info info{};
+    ~vt_scene() {
+        if (ctx) ctx->resident -= nodes.bytes + tris.bytes;
+    }
+};
+
+namespace vt {
+
+// Mirrors MaterialLibrary.setMesh for an indexed mesh (Sources/StudioModel.swift:1480-1504 with
+// Sources/Acceleration.swift:652-782): validate, build the hierarchy on the host, check the GPU
+// budget with the reference's message, then copy the streams to device memory.
+std::unique_ptr<vt_scene> set_mesh(vt_context *ctx, const float *vertices, uint32_t vertex_count,
+                                   const uint32_t *faces, uint32_t face_count, const BuildOptions &opt) {
+    if (face_count > 0 && (!vertices || !faces)) throw std::invalid_argument("vertices and faces must not be NULL");
+    if (face_count >= (1u << 30)) throw std::invalid_argument("Stored mesh assets exceed the 1,073,741,824 triangle limit.");
+    Bvh8 bvh = build_bvh8(vertices, vertex_count, faces, face_count, opt);
+    const uint64_t bytes = bvh.nodes.size() * sizeof(Node8) + bvh.tris.size() * sizeof(TriRecord);
+    if (bvh.nodes.size() >= (1u << 26)) throw std::invalid_argument("mesh exceeds the node index range");
+    if (ctx->resident + bytes > ctx->budget)
+        throw BudgetError("The imported meshes exceed the safe GPU memory budget.");
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    auto s = std::make_unique<vt_scene>();
+    s->nodes.upload(bvh.nodes.data(), bvh.nodes.size() * sizeof(Node8));
+    s->tris.upload(bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord));
+    s->ctx = ctx;
+    ctx->resident += s->nodes.bytes + s->tris.bytes;
+    s->view.nodes = s->nodes.ptr;
+    s->view.tris = s->tris.ptr;
+    s->view.node_count = (uint32_t)bvh.nodes.size();
+    s->view.root_max = bvh.root_max;
+    vt_scene_info &i = s->info;
+    i.face_count = face_count;
+    i.vertex_count = vertex_count;
+    i.node_count = (uint32_t)bvh.nodes.size();
+    i.depth = (uint32_t)bvh.depth;
+    i.leaf_size = (uint32_t)opt.leaf_size;
+    i.width = 8;
+    i.node_bytes = sizeof(Node8);
+    i.tri_bytes = sizeof(TriRecord);
+    i.device_bytes = bytes;
+    i.root_max = bvh.root_max;
+    i.build_seconds = (float)bvh.build_seconds;
+    return s;
+}
+
+}  // namespace vt
+
+extern "C" {
+
+const char *vt_last_error(void) { return g_error.c_str(); }
+
+vt_status vt_context_create(const vt_context_desc *desc, vt_context **out) {
+    return guarded([&]() -> vt_status {
+        if (!out) return fail(VT_ERROR_INVALID_ARGUMENT, "out must not be NULL");
+        *out = nullptr;
+        int device = desc ? desc->device : 0;
+        int count = 0;
+        check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (device < 0 || device >= count) return fail(VT_ERROR_INVALID_ARGUMENT, "no such HIP device");
+        check(hipSetDevice(device), "hipSetDevice");
+        size_t free_b = 0, total = 0;
+        check(hipMemGetInfo(&free_b, &total), "hipMemGetInfo");
+        auto c = std::make_unique<vt_context>();
+        c->device = device;
+        c->budget = desc && desc->memory_budget ? desc->memory_budget
+                                                : std::max<uint64_t>(256ull << 20, (uint64_t)total / 4);
+        *out = c.release();
+        return VT_OK;
+    });
+}
+
+void vt_context_destroy(vt_context *ctx) { delete ctx; }
+
+static vt::BuildOptions build_options(const vt_build_options *opts) {
+    vt::BuildOptions o;
+    if (opts) {
+        if (opts->leaf_size) o.leaf_size = (int)opts->leaf_size;
+        if (opts->bins) o.bins = (int)opts->bins;
+        if (opts->max_depth) o.max_depth = (int)opts->max_depth;
+    }
+    return o;
+}
+
+vt_status vt_bvh8_build_host(const float *vertices, uint32_t vertex_count, const uint32_t *faces, uint32_t face_count,
+                             const vt_build_options *opts, void **nodes, uint32_t *node_count, void **tris,
+                             uint32_t *depth, float *root_max) {
+    return guarded([&]() -> vt_status {
+        if (!nodes || !node_count || !tris || !depth || !root_max)
+            return fail(VT_ERROR_INVALID_ARGUMENT, "outputs must not be NULL");
+        if (face_count > 0 && (!vertices || !faces)) return fail(VT_ERROR_INVALID_ARGUMENT, "vertices and faces must not be NULL");
+        *nodes = nullptr;
+        *tris = nullptr;
+        vt::Bvh8 b = vt::build_bvh8(vertices, vertex_count, faces, face_count, build_options(opts));
+        void *n = std::malloc(std::max<size_t>(1, b.nodes.size() * sizeof(vt::Node8)));
+        void *t = std::malloc(std::max<size_t>(1, b.tris.size() * sizeof(vt::TriRecord)));
+        if (!n || !t) { std::free(n); std::free(t); throw std::bad_alloc(); }
+        if (!b.nodes.empty()) std::memcpy(n, b.nodes.data(), b.nodes.size() * sizeof(vt::Node8));
+        if (!b.tris.empty()) std::memcpy(t, b.tris.data(), b.tris.size() * sizeof(vt::TriRecord));
+        *nodes = n;
+        *tris = t;
+        *node_count = (uint32_t)b.nodes.size();
+        *depth = (uint32_t)b.depth;
+        *root_max = b.root_max;
+        return VT_OK;
+    });
+}
+
+void vt_host_free(void *p) { std::free(p); }
+
+vt_status vt_scene_upload_indexed(vt_context *ctx, const float *vertices, uint32_t vertex_count, const uint32_t *faces,
+                                  uint32_t face_count, const vt_build_options *opts, vt_scene **out) {
+    return guarded([&]() -> vt_status {
+        if (!ctx || !out) return fail(VT_ERROR_INVALID_ARGUMENT, "context and out must not be NULL");
+        *out = nullptr;
+        *out = vt::set_mesh(ctx, vertices, vertex_count, faces, face_count, build_options(opts)).release();
+        return VT_OK;
+    });
+}
+
+void vt_scene_destroy(vt_scene *scene) { delete scene; }
+
+vt_status vt_scene_get_info(const vt_scene *scene, vt_scene_info *info) {
+    if (!scene || !info) return fail(VT_ERROR_INVALID_ARGUMENT, "scene and info must not be NULL");
+    *info = scene->info;
+    return VT_OK;
+}
+
+static bool rays_ok(const vt_ray_soa *r) {
+    return r && r->ox && r->oy && r->oz && r->dx && r->dy && r->dz && r->tmin && r->tmax;
+}
+
+static vt::RayArgs ray_args(const vt_ray_soa *r) {
+    return vt::RayArgs{r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, r->tmin, r->tmax};
+}
+
+vt_status vt_trace_closest(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, const vt_hit_soa *hits, void *stream) {
+    return guarded([&]() -> vt_status {
+        if (!scene) return fail(VT_ERROR_INVALID_ARGUMENT, "scene must not be NULL");
+        if (n == 0) return VT_OK;
+        if (!rays_ok(rays)) return fail(VT_ERROR_INVALID_ARGUMENT, "ray arrays must not be NULL");
+        if (!hits || !hits->t || !hits->b1 || !hits->b2 || !hits->face)
+            return fail(VT_ERROR_INVALID_ARGUMENT, "hit arrays must not be NULL");
+        check(hipSetDevice(scene->ctx->device), "hipSetDevice");
+        vt::HitArgs h{hits->t, hits->b1, hits->b2, hits->face, hits->instance};
+        check(vt::launch_trace(scene->view, ray_args(rays), h, nullptr, n, false, nullptr, (hipStream_t)stream),
+              "trace_closest launch");
+        return VT_OK;
+    });
+}
+
+vt_status vt_trace_any(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, uint8_t *occluded, void *stream) {
+    return guarded([&]() -> vt_status {
+        if (!scene) return fail(VT_ERROR_INVALID_ARGUMENT, "scene must not be NULL");
+        if (n == 0) return VT_OK;
+        if (!rays_ok(rays) || !occluded) return fail(VT_ERROR_INVALID_ARGUMENT, "ray and result arrays must not be NULL");
+        check(hipSetDevice(scene->ctx->device), "hipSetDevice");
+        vt::HitArgs h{};
+        check(vt::launch_trace(scene->view, ray_args(rays), h, occluded, n, true, nullptr, (hipStream_t)stream),
+              "trace_any launch");
+        return VT_OK;
+    });
+}
+
+vt_status vt_trace_stats(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, int any, uint64_t counters[2],
+                         void *stream) {
+    return guarded([&]() -> vt_status {
+        if (!scene || !counters) return fail(VT_ERROR_INVALID_ARGUMENT, "scene and counters must not be NULL");
+        counters[0] = counters[1] = 0;
+        if (n == 0) return VT_OK;
+        if (!rays_ok(rays)) return fail(VT_ERROR_INVALID_ARGUMENT, "ray arrays must not be NULL");
+        check(hipSetDevice(scene->ctx->device), "hipSetDevice");
+        const size_t cbytes = 128 * sizeof(unsigned long long);
+        if (!scene->counters.ptr) {
+            check(hipMalloc(&scene->counters.ptr, cbytes), "hipMalloc");
+            scene->counters.bytes = cbytes;
+        }
+        hipStream_t s = (hipStream_t)stream;
+        check(hipMemsetAsync(scene->counters.ptr, 0, cbytes, s), "hipMemsetAsync");
+        // Scratch outputs: the counting variant writes results like the timed kernels do.
+        DeviceBuffer out;
+        const size_t per = any ? 1 : (4 * sizeof(float) + sizeof(int32_t));
+        check(hipMalloc(&out.ptr, per * (size_t)n + 64), "hipMalloc");
+        out.bytes = per * (size_t)n + 64;
+        char *base = (char *)out.ptr;
+        vt::HitArgs h{};
+        uint8_t *occ = nullptr;
+        if (any) {
+            occ = (uint8_t *)base;
+        } else {
+            h.t = (float *)base;
+            h.b1 = h.t + n;
+            h.b2 = h.b1 + n;
+            h.face = (int32_t *)(h.b2 + n);
+            h.instance = nullptr;
+        }
+        check(vt::launch_trace(scene->view, ray_args(rays), h, occ, n, any != 0,
+                               (unsigned long long *)scene->counters.ptr, s),
+              "trace_stats launch");
+        unsigned long long host[128];
+        check(hipMemcpyAsync(host, scene->counters.ptr, cbytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        for (int k = 0; k < 64; ++k) {
+            counters[0] += host[2 * k];
+            counters[1] += host[2 * k + 1];
+        }
+        return VT_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// trace.hip -- gfx950 closest-hit and any-hit traversal kernels over SoA ray buffers.
+//
+// One lane per ray, 64-lane wavefronts, 128-lane blocks.  Ray components are read and hits
+// written as separate arrays, so every wavefront access is one coalesced 256-byte line.  The
+// per-ray traversal (traverse8.h) keeps its node-group stack in LDS, laid out [depth][lane] so
+// every lane's push/pop touches its own bank.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "trace_kernels.h"
+#include "traverse8.h"
+
+namespace vt {
+namespace {
+
+constexpr int kBlock = 128;
+constexpr int kStack = 24;  // the builder bounds the tree depth by 24, so <= 23 groups are pending
+
+struct LdsStack {
+    uint32_t (*base)[kBlock];
+    uint32_t (*bits)[kBlock];
+    int lane;
+    int sp;
+    __device__ __forceinline__ void push(uint32_t b, uint32_t m) {
+        base[sp][lane] = b;
+        bits[sp][lane] = m;
+        ++sp;
+    }
+    __device__ __forceinline__ bool full() const { return sp >= kStack; }
+    __device__ __forceinline__ void pop(uint32_t &b, uint32_t &m) {
+        --sp;
+        b = base[sp][lane];
+        m = bits[sp][lane];
+    }
+    __device__ __forceinline__ bool empty() const { return sp == 0; }
+};
+
+template <bool ANY, bool STATS>
+__global__ __launch_bounds__(kBlock) void trace_kernel(const U4 *__restrict__ nodes, const F4 *__restrict__ tris,
+                                                       RayArgs rays, HitArgs hits, uint8_t *__restrict__ occluded,
+                                                       uint32_t n, float root_max,
+                                                       unsigned long long *__restrict__ counters) {
+    __shared__ uint32_t stack_base[kStack][kBlock];
+    __shared__ uint32_t stack_bits[kStack][kBlock];
+    const int lane = threadIdx.x;
+    const uint32_t i = blockIdx.x * kBlock + lane;
+    if (i >= n) return;
+    TraceRay r{rays.ox[i], rays.oy[i], rays.oz[i], rays.dx[i], rays.dy[i], rays.dz[i], rays.tmin[i], rays.tmax[i]};
+    LdsStack st{stack_base, stack_bits, lane, 0};
+    TraceResult res = traverse<ANY, STATS>(nodes, tris, root_max, r, st);
+    if (ANY) {
+        occluded[i] = res.occluded ? 1 : 0;
+    } else {
+        hits.t[i] = res.t;
+        hits.b1[i] = res.b1;
+        hits.b2[i] = res.b2;
+        hits.face[i] = res.face;
+        if (hits.instance) hits.instance[i] = 0u;
+    }
+    if (STATS) {
+        unsigned long long *c = counters + 2 * (blockIdx.x & 63u);
+        atomicAdd(c, (unsigned long long)res.nodes);
+        atomicAdd(c + 1, (unsigned long long)res.tris);
+    }
+}
+
+template <bool ANY>
+__global__ __launch_bounds__(256) void miss_kernel(RayArgs rays, HitArgs hits, uint8_t *occluded, uint32_t n) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (ANY) {
+        occluded[i] = 0;
+    } else {
+        hits.t[i] = rays.tmax[i];
+        hits.b1[i] = 0.0f;
+        hits.b2[i] = 0.0f;
+        hits.face[i] = -1;
+        if (hits.instance) hits.instance[i] = 0u;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitArgs &hits, uint8_t *occluded,
+                        uint32_t n, bool any, unsigned long long *counters, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    if (scene.node_count == 0) {
+        dim3 grid((n + 255) / 256);
+        if (any) hipLaunchKernelGGL(miss_kernel<true>, grid, dim3(256), 0, stream, rays, hits, occluded, n);
+        else hipLaunchKernelGGL(miss_kernel<false>, grid, dim3(256), 0, stream, rays, hits, occluded, n);
+        return hipGetLastError();
+    }
+    dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+    const U4 *nd = reinterpret_cast<const U4 *>(scene.nodes);
+    const F4 *tr = reinterpret_cast<const F4 *>(scene.tris);
+    if (counters) {
+        if (any) hipLaunchKernelGGL((trace_kernel<true, true>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, counters);
+        else hipLaunchKernelGGL((trace_kernel<false, true>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, counters);
+    } else {
+        if (any) hipLaunchKernelGGL((trace_kernel<true, false>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, nullptr);
+        else hipLaunchKernelGGL((trace_kernel<false, false>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, nullptr);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace vt

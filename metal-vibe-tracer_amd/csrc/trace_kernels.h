@@ -1,0 +1,28 @@
+// trace_kernels.h -- launch interface between the host runtime and the traversal kernels.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+namespace vt {
+
+struct RayArgs {
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+};
+struct HitArgs {
+    float *t, *b1, *b2;
+    int32_t *face;
+    uint32_t *instance;
+};
+struct TraceScene {
+    const void *nodes;  // Node8 records (bvh8.h)
+    const void *tris;   // TriRecord records
+    uint32_t node_count;
+    float root_max;
+};
+
+// Closest hit (any = false) into `hits`, or any hit into `occluded`.  `counters` (device, 128
+// u64, zeroed by the caller) selects the counting variant: 64 pairs (node fetches, triangle tests).
+hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitArgs &hits, uint8_t *occluded,
+                        uint32_t n, bool any, unsigned long long *counters, hipStream_t stream);
+
+}  // namespace vt

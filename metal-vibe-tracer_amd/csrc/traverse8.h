@@ -1,0 +1,231 @@
+// traverse8.h -- per-ray traversal of the 8-wide compressed BVH (layout in bvh8.h), shared by
+// every kernel that traces (closest-hit, any-hit, and the integrator's extend/shadow stages).
+//
+// The node-group stack is abstracted (Stack::push/pop/size) so a kernel keeps it in LDS.
+// Arithmetic follows the reference exactly where results depend on it:
+//   * boxes: conservative slab test of wide_node_hit (main.swift:1766-1779) on the exactly
+//     dequantised child boxes, grown by e = 1.1920929e-6 (|o|max + root max) (main.swift:1792);
+//   * triangles: watertight Woop test (main.swift:1537-1582), products rounded separately,
+//     correctly rounded divisions, FMA tie-break of equal rounded edge products.
+// Among hits at exactly equal t the smaller document face index wins.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define VT_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#include <cstring>
+#define VT_HD inline
+#endif
+
+namespace vt {
+
+struct U4 {
+    uint32_t x, y, z, w;
+};
+struct F4 {
+    float x, y, z, w;
+};
+
+VT_HD float bits_to_float(uint32_t u) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __uint_as_float(u);
+#else
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+#endif
+}
+VT_HD uint32_t float_to_bits(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __float_as_uint(f);
+#else
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    return u;
+#endif
+}
+
+struct WoopRay {
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+VT_HD float sel3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
+
+// main.swift:1537-1550 watertight_ray.
+VT_HD WoopRay woop_ray(float dx, float dy, float dz) {
+    float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    int kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+    float dkz = sel3(dx, dy, dz, kz);
+    if (dkz < 0.0f) {
+        int s = kx;
+        kx = ky;
+        ky = s;
+    }
+    WoopRay w;
+    w.kx = kx;
+    w.ky = ky;
+    w.kz = kz;
+    w.sx = sel3(dx, dy, dz, kx) / dkz;
+    w.sy = sel3(dx, dy, dz, ky) / dkz;
+    w.sz = 1.0f / dkz;
+    return w;
+}
+
+// main.swift:1557-1562 watertight_edge.
+VT_HD float watertight_edge(float px, float py, float qx, float qy) {
+#pragma clang fp contract(off)
+    float x = px * qy, y = py * qx, e = x - y;
+    if (e == 0.0f) e = __builtin_fmaf(px, qy, -x) - __builtin_fmaf(py, qx, -y);
+    return e;
+}
+
+// main.swift:1564-1578 without the [tmin, tmax) check (the caller applies it).
+VT_HD bool woop_triangle(const WoopRay &w, float ox, float oy, float oz, F4 p0, F4 p1, F4 p2, float &t, float &b1,
+                         float &b2) {
+#pragma clang fp contract(off)
+    float Ax = p0.x - ox, Ay = p0.y - oy, Az = p0.z - oz;
+    float Bx = p1.x - ox, By = p1.y - oy, Bz = p1.z - oz;
+    float Cx = p2.x - ox, Cy = p2.y - oy, Cz = p2.z - oz;
+    float akz = sel3(Ax, Ay, Az, w.kz), bkz = sel3(Bx, By, Bz, w.kz), ckz = sel3(Cx, Cy, Cz, w.kz);
+    float ax = sel3(Ax, Ay, Az, w.kx) - w.sx * akz, ay = sel3(Ax, Ay, Az, w.ky) - w.sy * akz;
+    float bx = sel3(Bx, By, Bz, w.kx) - w.sx * bkz, by = sel3(Bx, By, Bz, w.ky) - w.sy * bkz;
+    float cx = sel3(Cx, Cy, Cz, w.kx) - w.sx * ckz, cy = sel3(Cx, Cy, Cz, w.ky) - w.sy * ckz;
+    float U = watertight_edge(cx, cy, bx, by);
+    float V = watertight_edge(ax, ay, cx, cy);
+    float W = watertight_edge(bx, by, ax, ay);
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float T = U * (w.sz * akz) + V * (w.sz * bkz) + W * (w.sz * ckz);
+    float inv = 1.0f / det;
+    t = T * inv;
+    b1 = V * inv;
+    b2 = W * inv;
+    return true;
+}
+
+VT_HD uint32_t byte_of(uint32_t lo4, uint32_t hi4, int s) { return ((s < 4 ? lo4 : hi4) >> (8 * (s & 3))) & 0xffu; }
+
+struct TraceRay {
+    float ox, oy, oz, dx, dy, dz, tmin, tmax;
+};
+
+struct TraceResult {
+    float t, b1, b2;
+    int32_t face;    // -1: miss
+    bool occluded;   // any-hit result
+    uint32_t nodes;  // node records fetched
+    uint32_t tris;   // triangle records tested
+};
+
+// Traverses one ray.  nodes: Node8 records as 5 U4 each; tris: TriRecord as 3 F4 each.
+template <bool ANY, bool STATS, class Stack>
+VT_HD TraceResult traverse(const U4 *__restrict__ nodes, const F4 *__restrict__ tris, float root_max,
+                           const TraceRay &r, Stack &stack) {
+    TraceResult res;
+    res.t = r.tmax;
+    res.b1 = res.b2 = 0.0f;
+    res.face = -1;
+    res.occluded = false;
+    res.nodes = res.tris = 0;
+
+    const WoopRay w = woop_ray(r.dx, r.dy, r.dz);
+    const float e = 1.1920929e-6f * (fmaxf(fabsf(r.ox), fmaxf(fabsf(r.oy), fabsf(r.oz))) + root_max);
+    const float lowx = r.ox + e, lowy = r.oy + e, lowz = r.oz + e;
+    const float highx = r.ox - e, highy = r.oy - e, highz = r.oz - e;
+    const bool parx = fabsf(r.dx) < 1e-8f, pary = fabsf(r.dy) < 1e-8f, parz = fabsf(r.dz) < 1e-8f;
+    const float invx = 1.0f / (parx ? 1.0f : r.dx), invy = 1.0f / (pary ? 1.0f : r.dy),
+                invz = 1.0f / (parz ? 1.0f : r.dz);
+    const uint32_t oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
+
+    float closest = r.tmax;
+    // Current node group: first child index and (imask << 8 | visit-order bits still to visit).
+    uint32_t cur_base = 0, cur_bits = 0;
+    uint32_t node = 0;
+    for (;;) {
+        const U4 *np = nodes + 5 * (size_t)node;
+        const U4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+        if (STATS) ++res.nodes;
+        const float px = bits_to_float(w0.x), py = bits_to_float(w0.y), pz = bits_to_float(w0.z);
+        const float sx = bits_to_float((w0.w & 0xffu) << 23);
+        const float sy = bits_to_float(((w0.w >> 8) & 0xffu) << 23);
+        const float sz = bits_to_float(((w0.w >> 16) & 0xffu) << 23);
+        const uint32_t imask = w0.w >> 24;
+        uint32_t hitbits = 0, trimask = 0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const uint32_t meta = byte_of(w1.z, w1.w, s);
+            const float lox = __builtin_fmaf((float)byte_of(w2.x, w2.y, s), sx, px);
+            const float loy = __builtin_fmaf((float)byte_of(w2.z, w2.w, s), sy, py);
+            const float loz = __builtin_fmaf((float)byte_of(w3.x, w3.y, s), sz, pz);
+            const float hix = __builtin_fmaf((float)byte_of(w3.z, w3.w, s), sx, px);
+            const float hiy = __builtin_fmaf((float)byte_of(w4.x, w4.y, s), sy, py);
+            const float hiz = __builtin_fmaf((float)byte_of(w4.z, w4.w, s), sz, pz);
+            const float ax = (lox - lowx) * invx, bx = (hix - highx) * invx;
+            const float ay = (loy - lowy) * invy, by = (hiy - highy) * invy;
+            const float az = (loz - lowz) * invz, bz = (hiz - highz) * invz;
+            const float lx = parx ? -1e30f : fminf(ax, bx), hx = parx ? 1e30f : fmaxf(ax, bx);
+            const float ly = pary ? -1e30f : fminf(ay, by), hy = pary ? 1e30f : fmaxf(ay, by);
+            const float lz = parz ? -1e30f : fminf(az, bz), hz = parz ? 1e30f : fmaxf(az, bz);
+            const float tn = fmaxf(fmaxf(lx, ly), fmaxf(lz, r.tmin));
+            const float tf = fminf(fminf(hx, hy), fminf(hz, closest));
+            bool hit = tn <= tf && meta != 0u;
+            if (parx) hit = hit && !(lowx < lox || highx > hix);
+            if (pary) hit = hit && !(lowy < loy || highy > hiy);
+            if (parz) hit = hit && !(lowz < loz || highz > hiz);
+            if (hit) {
+                if (meta & 0x80u) hitbits |= 1u << (s ^ oct);
+                else trimask |= ((1u << (meta & 3u)) - 1u) << (meta >> 2);
+            }
+        }
+        if (hitbits) {
+            if (cur_bits & 0xffu) {
+                // The builder bounds the depth so a full stack never occurs; if it did, stop
+                // this ray (a wrong result, never an out-of-bounds access).
+                if (stack.full()) break;
+                stack.push(cur_base, cur_bits);
+            }
+            cur_base = w1.x;
+            cur_bits = (imask << 8) | hitbits;
+        }
+        const uint32_t tri_base = w1.y;
+        while (trimask) {
+            const uint32_t b = __builtin_ctz(trimask);
+            trimask &= trimask - 1u;
+            const F4 *tp = tris + 3 * (size_t)(tri_base + b);
+            const F4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+            if (STATS) ++res.tris;
+            float t, b1, b2;
+            if (!woop_triangle(w, r.ox, r.oy, r.oz, p0, p1, p2, t, b1, b2)) continue;
+            const int32_t face = (int32_t)float_to_bits(p0.w);
+            if (ANY) {
+                if (t >= r.tmin && t < r.tmax) {
+                    res.occluded = true;
+                    return res;
+                }
+            } else if (t >= r.tmin && (t < closest || (t == closest && res.face >= 0 && face < res.face))) {
+                closest = t;
+                res.t = t;
+                res.b1 = b1;
+                res.b2 = b2;
+                res.face = face;
+            }
+        }
+        if ((cur_bits & 0xffu) == 0u) {
+            if (stack.empty()) break;
+            stack.pop(cur_base, cur_bits);
+        }
+        const uint32_t v = __builtin_ctz(cur_bits & 0xffu);
+        cur_bits &= ~(1u << v);
+        const uint32_t slot = v ^ oct;
+        node = cur_base + __builtin_popcount((cur_bits >> 8) & ((1u << slot) - 1u));
+    }
+    return res;
+}
+
+}  // namespace vt

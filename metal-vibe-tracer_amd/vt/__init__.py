@@ -1,0 +1,325 @@
+"""Python binding of libvt.so (include/vt.h, include/vt_workloads.h) over ctypes.
+
+This is a thin host-side binding used by the tests and the benchmark; the runtime itself is C++
+and HIP.  Device buffers are passed as torch tensors on the HIP device (only their data pointers
+cross the ABI).  Nothing here computes results: every trace call runs the gfx950 kernels, and a
+missing or stale libvt.so raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PACKAGE_DIR = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PACKAGE_DIR, "libvt.so")
+
+VT_OK = 0
+STATUS_NAMES = {0: "VT_OK", 1: "VT_ERROR_INVALID_ARGUMENT", 2: "VT_ERROR_OUT_OF_MEMORY", 3: "VT_ERROR_DEVICE",
+                4: "VT_ERROR_UNSUPPORTED"}
+
+# Every symbol include/vt.h and include/vt_workloads.h declare.
+EXPORTED = [
+    "vt_context_create", "vt_context_destroy", "vt_last_error", "vt_scene_upload_indexed", "vt_scene_destroy",
+    "vt_scene_get_info", "vt_trace_closest", "vt_trace_any", "vt_trace_stats", "vt_bvh8_build_host", "vt_host_free",
+    "vt_gen_icosphere_field", "vt_gen_heightfield", "vt_gen_camera_rays", "vt_gen_bounce_rays", "vt_gen_random_rays",
+]
+
+
+class VTError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+class _ContextDesc(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("memory_budget", ctypes.c_uint64)]
+
+
+class _BuildOptions(ctypes.Structure):
+    _fields_ = [("leaf_size", ctypes.c_uint32), ("bins", ctypes.c_uint32), ("max_depth", ctypes.c_uint32)]
+
+
+class _SceneInfo(ctypes.Structure):
+    _fields_ = [("face_count", ctypes.c_uint32), ("vertex_count", ctypes.c_uint32), ("node_count", ctypes.c_uint32),
+                ("depth", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32), ("width", ctypes.c_uint32),
+                ("node_bytes", ctypes.c_uint32), ("tri_bytes", ctypes.c_uint32), ("device_bytes", ctypes.c_uint64),
+                ("root_max", ctypes.c_float), ("build_seconds", ctypes.c_float)]
+
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+class _RaySoA(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")]
+
+
+class _HitSoA(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("t", "b1", "b2", "face", "instance")]
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads libvt.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(path)
+    c = ctypes
+    vp, u32, i32, f32 = c.c_void_p, c.c_uint32, c.c_int32, c.c_float
+    P = c.POINTER
+    sig = {
+        "vt_context_create": (i32, [P(_ContextDesc), P(vp)]),
+        "vt_context_destroy": (None, [vp]),
+        "vt_last_error": (c.c_char_p, []),
+        "vt_scene_upload_indexed": (i32, [vp, vp, u32, vp, u32, P(_BuildOptions), P(vp)]),
+        "vt_scene_destroy": (None, [vp]),
+        "vt_scene_get_info": (i32, [vp, P(_SceneInfo)]),
+        "vt_trace_closest": (i32, [vp, P(_RaySoA), u32, P(_HitSoA), vp]),
+        "vt_trace_any": (i32, [vp, P(_RaySoA), u32, vp, vp]),
+        "vt_trace_stats": (i32, [vp, P(_RaySoA), u32, c.c_int, P(c.c_uint64), vp]),
+        "vt_bvh8_build_host": (i32, [vp, u32, vp, u32, P(_BuildOptions), P(vp), P(u32), P(vp), P(u32), P(f32)]),
+        "vt_host_free": (None, [vp]),
+        "vt_gen_icosphere_field": (c.c_int, [u32, u32, f32, f32, f32, u32, vp, P(u32), vp, P(u32)]),
+        "vt_gen_heightfield": (c.c_int, [u32, u32, f32, f32, f32, u32, vp, P(u32), vp, P(u32)]),
+        "vt_gen_camera_rays": (None, [P(f32), P(f32), f32, u32, u32, u32] + [vp] * 8),
+        "vt_gen_bounce_rays": (u32, [vp, vp, u32, vp, vp, vp, vp, vp, vp, u32] + [vp] * 8),
+        "vt_gen_random_rays": (None, [P(f32), P(f32), u32, u32] + [vp] * 8),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(status: int):
+    if status != VT_OK:
+        raise VTError(status, load().vt_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# --------------------------------------------------------------------------------------------
+# Workloads (host arrays)
+# --------------------------------------------------------------------------------------------
+
+@dataclass
+class Mesh:
+    vertices: np.ndarray  # (V, 8) float32: position, normal, uv
+    faces: np.ndarray     # (F, 4) uint32: a, b, c, tag
+
+
+def icosphere_field(spheres=1024, subdivisions=3, mu=0.0, sigma=0.5, half_extent=50.0, seed=7) -> Mesh:
+    lib = load()
+    nv, nf = ctypes.c_uint32(), ctypes.c_uint32()
+    lib.vt_gen_icosphere_field(spheres, subdivisions, mu, sigma, half_extent, seed, None, ctypes.byref(nv), None,
+                               ctypes.byref(nf))
+    v = np.empty((nv.value, 8), np.float32)
+    f = np.empty((nf.value, 4), np.uint32)
+    rc = lib.vt_gen_icosphere_field(spheres, subdivisions, mu, sigma, half_extent, seed, _ptr(v), ctypes.byref(nv),
+                                    _ptr(f), ctypes.byref(nf))
+    if rc != 0:
+        raise ValueError("invalid icosphere field parameters")
+    return Mesh(v, f)
+
+
+def heightfield(grid=2237, octaves=8, gain=0.5, half_extent=100.0, height=12.0, seed=3) -> Mesh:
+    lib = load()
+    nv, nf = ctypes.c_uint32(), ctypes.c_uint32()
+    if lib.vt_gen_heightfield(grid, octaves, gain, half_extent, height, seed, None, ctypes.byref(nv), None,
+                              ctypes.byref(nf)) != 0:
+        raise ValueError("invalid heightfield parameters")
+    v = np.empty((nv.value, 8), np.float32)
+    f = np.empty((nf.value, 4), np.uint32)
+    lib.vt_gen_heightfield(grid, octaves, gain, half_extent, height, seed, _ptr(v), ctypes.byref(nv), _ptr(f),
+                           ctypes.byref(nf))
+    return Mesh(v, f)
+
+
+RAY_FIELDS = ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")
+
+
+def _ray_args(r: np.ndarray):
+    return [_ptr(r[k]) for k in range(8)]
+
+
+def camera_rays(eye, target, fov, width, height, seed) -> np.ndarray:
+    """(8, width*height) float32 SoA rays."""
+    r = np.empty((8, width * height), np.float32)
+    e = (ctypes.c_float * 3)(*eye)
+    t = (ctypes.c_float * 3)(*target)
+    load().vt_gen_camera_rays(e, t, fov, width, height, seed, *_ray_args(r))
+    return r
+
+
+def bounce_rays(mesh: Mesh, rays: np.ndarray, face: np.ndarray, b1: np.ndarray, b2: np.ndarray, seed: int):
+    n = rays.shape[1]
+    out = np.empty((8, n), np.float32)
+    face = np.ascontiguousarray(face, np.int32)
+    b1 = np.ascontiguousarray(b1, np.float32)
+    b2 = np.ascontiguousarray(b2, np.float32)
+    d = [np.ascontiguousarray(rays[k]) for k in (3, 4, 5)]
+    hits = load().vt_gen_bounce_rays(_ptr(mesh.vertices), _ptr(mesh.faces), n, _ptr(d[0]), _ptr(d[1]), _ptr(d[2]),
+                                     _ptr(face), _ptr(b1), _ptr(b2), seed, *_ray_args(out))
+    if hits == 0:
+        raise ValueError("no hits to bounce from")
+    return out
+
+
+def random_rays(lo, hi, n, seed) -> np.ndarray:
+    r = np.empty((8, n), np.float32)
+    load().vt_gen_random_rays((ctypes.c_float * 3)(*lo), (ctypes.c_float * 3)(*hi), n, seed, *_ray_args(r))
+    return r
+
+
+# --------------------------------------------------------------------------------------------
+# Host-only BVH inspection
+# --------------------------------------------------------------------------------------------
+
+NODE_DTYPE = np.dtype([("p", "<f4", 3), ("e", "u1", 3), ("imask", "u1"), ("child_base", "<u4"), ("tri_base", "<u4"),
+                       ("meta", "u1", 8), ("qlo", "u1", (3, 8)), ("qhi", "u1", (3, 8))])
+TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("face", "<u4"), ("v1", "<f4", 3), ("pad1", "<u4"), ("v2", "<f4", 3),
+                      ("pad2", "<u4")])
+assert NODE_DTYPE.itemsize == 80 and TRI_DTYPE.itemsize == 48
+
+
+def _opts(leaf_size=0, bins=0, max_depth=0):
+    return _BuildOptions(leaf_size, bins, max_depth)
+
+
+def build_bvh8_host(mesh: Mesh, leaf_size=0, bins=0, max_depth=0):
+    """Returns (nodes structured array, tris structured array, depth, root_max)."""
+    lib = load()
+    nodes, tris = ctypes.c_void_p(), ctypes.c_void_p()
+    count, depth, root_max = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_float()
+    o = _opts(leaf_size, bins, max_depth)
+    _check(lib.vt_bvh8_build_host(_ptr(mesh.vertices), len(mesh.vertices), _ptr(mesh.faces), len(mesh.faces),
+                                  ctypes.byref(o), ctypes.byref(nodes), ctypes.byref(count), ctypes.byref(tris),
+                                  ctypes.byref(depth), ctypes.byref(root_max)))
+    try:
+        nb = count.value * 80
+        tb = len(mesh.faces) * 48
+        n = np.frombuffer(ctypes.string_at(nodes, nb), NODE_DTYPE).copy() if nb else np.zeros(0, NODE_DTYPE)
+        t = np.frombuffer(ctypes.string_at(tris, tb), TRI_DTYPE).copy() if tb else np.zeros(0, TRI_DTYPE)
+    finally:
+        lib.vt_host_free(nodes)
+        lib.vt_host_free(tris)
+    return n, t, depth.value, root_max.value
+
+
+# --------------------------------------------------------------------------------------------
+# Device context, scenes and traces
+# --------------------------------------------------------------------------------------------
+
+class Context:
+    """One HIP device (vt_context)."""
+
+    def __init__(self, device: int = 0, memory_budget: int = 0):
+        lib = load()
+        h = ctypes.c_void_p()
+        desc = _ContextDesc(device, memory_budget)
+        _check(lib.vt_context_create(ctypes.byref(desc), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            load().vt_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mesh(self, mesh: Mesh, leaf_size=0, bins=0, max_depth=0) -> "Scene":
+        """MaterialLibrary.setMesh for an indexed mesh: builds and uploads the BVH."""
+        lib = load()
+        v = np.ascontiguousarray(mesh.vertices, np.float32)
+        f = np.ascontiguousarray(mesh.faces, np.uint32)
+        if v.ndim != 2 or v.shape[1] != 8 or f.ndim != 2 or f.shape[1] != 4:
+            raise ValueError("vertices must be (V, 8) float32 and faces (F, 4) uint32")
+        h = ctypes.c_void_p()
+        o = _opts(leaf_size, bins, max_depth)
+        _check(lib.vt_scene_upload_indexed(self._h, _ptr(v), len(v), _ptr(f), len(f), ctypes.byref(o),
+                                           ctypes.byref(h)))
+        return Scene(self, h)
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return ctypes.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+def _rays_struct(rays):
+    import torch
+    if not (isinstance(rays, torch.Tensor) and rays.is_cuda and rays.dtype == torch.float32 and rays.dim() == 2
+            and rays.shape[0] == 8 and rays.is_contiguous()):
+        raise ValueError("rays must be a contiguous (8, n) float32 device tensor")
+    n = rays.shape[1]
+    base = rays.data_ptr()
+    return _RaySoA(*[base + 4 * n * k for k in range(8)]), n
+
+
+class Scene:
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self._h = handle
+
+    def close(self):
+        if self._h:
+            load().vt_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def info(self) -> dict:
+        i = _SceneInfo()
+        _check(load().vt_scene_get_info(self._h, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in _SceneInfo._fields_}
+
+    def trace_closest(self, rays, out=None, stream=None):
+        """rays: (8, n) float32 device tensor.  Returns dict of device tensors t, b1, b2, face."""
+        import torch
+        rs, n = _rays_struct(rays)
+        if out is None:
+            out = {"t": torch.empty(n, dtype=torch.float32, device=rays.device),
+                   "b1": torch.empty(n, dtype=torch.float32, device=rays.device),
+                   "b2": torch.empty(n, dtype=torch.float32, device=rays.device),
+                   "face": torch.empty(n, dtype=torch.int32, device=rays.device)}
+        hs = _HitSoA(out["t"].data_ptr(), out["b1"].data_ptr(), out["b2"].data_ptr(), out["face"].data_ptr(), None)
+        _check(load().vt_trace_closest(self._h, ctypes.byref(rs), n, ctypes.byref(hs), _stream_ptr(stream)))
+        return out
+
+    def trace_any(self, rays, out=None, stream=None):
+        import torch
+        rs, n = _rays_struct(rays)
+        if out is None:
+            out = torch.empty(n, dtype=torch.uint8, device=rays.device)
+        _check(load().vt_trace_any(self._h, ctypes.byref(rs), n, ctypes.c_void_p(out.data_ptr()),
+                                   _stream_ptr(stream)))
+        return out
+
+    def trace_stats(self, rays, any_hit=False, stream=None):
+        rs, n = _rays_struct(rays)
+        c = (ctypes.c_uint64 * 2)()
+        _check(load().vt_trace_stats(self._h, ctypes.byref(rs), n, 1 if any_hit else 0, c, _stream_ptr(stream)))
+        return int(c[0]), int(c[1])

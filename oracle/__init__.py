@@ -1,0 +1,189 @@
+"""ctypes wrapper of the oracle (oracle/liboracle.so): TEST INFRASTRUCTURE ONLY.
+
+The oracle restates the reference's traversal arithmetic on the CPU (see oracle.h for the
+reference file:line of each function and what pins it).  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module, and only as the checker or the timed CPU
+baseline; the shipped library never calls it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+class WideNode(ctypes.Structure):
+    _fields_ = [("lox", ctypes.c_float * 4), ("loy", ctypes.c_float * 4), ("loz", ctypes.c_float * 4),
+                ("hix", ctypes.c_float * 4), ("hiy", ctypes.c_float * 4), ("hiz", ctypes.c_float * 4),
+                ("child", ctypes.c_int32 * 4), ("count", ctypes.c_int32 * 4)]
+
+
+WIDE_NODE_DTYPE = np.dtype([("lo", "<f4", (3, 4)), ("hi", "<f4", (3, 4)), ("child", "<i4", 4), ("count", "<i4", 4)])
+assert ctypes.sizeof(WideNode) == 128 and WIDE_NODE_DTYPE.itemsize == 128
+
+
+class Hit(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_float), ("b1", ctypes.c_float), ("b2", ctypes.c_float), ("face", ctypes.c_int32)]
+
+
+class SceneView(ctypes.Structure):
+    _fields_ = [("vertices", ctypes.c_void_p), ("faces", ctypes.c_void_p), ("face_count", ctypes.c_uint32),
+                ("nodes", ctypes.c_void_p), ("order", ctypes.c_void_p), ("root_max", ctypes.c_float)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} missing: run make -C oracle")
+    lib = ctypes.CDLL(LIB_PATH)
+    c = ctypes
+    vp, f32, u32, i32 = c.c_void_p, c.c_float, c.c_uint32, c.c_int32
+    P = c.POINTER
+    sig = {
+        "vo_pcg_hash": (u32, [u32]),
+        "vo_rand_f": (f32, [P(u32)]),
+        "vo_watertight_edge": (f32, [f32, f32, f32, f32]),
+        "vo_intersect_triangle": (c.c_int, [P(f32), P(f32), P(f32), P(f32), P(f32), f32, f32, P(f32), P(f32), P(f32)]),
+        "vo_mesh_bounds": (None, [vp, vp, u32, vp]),
+        "vo_widebvh_build": (i32, [vp, i32, i32, i32, P(vp), vp, P(i32)]),
+        "vo_free": (None, [vp]),
+        "vo_blas_closest": (Hit, [P(SceneView), P(f32), P(f32), f32, f32, P(c.c_uint64), P(c.c_uint64)]),
+        "vo_blas_any": (c.c_int, [P(SceneView), P(f32), P(f32), f32, f32, P(c.c_uint64), P(c.c_uint64)]),
+        "vo_brute_closest": (Hit, [P(SceneView), P(f32), P(f32), f32, f32]),
+        "vo_brute_any": (c.c_int, [P(SceneView), P(f32), P(f32), f32, f32]),
+        "vo_trace_closest_batch": (None, [P(SceneView)] + [vp] * 8 + [c.c_int64, c.c_int, c.c_int] + [vp] * 5),
+        "vo_trace_any_batch": (None, [P(SceneView)] + [vp] * 8 + [c.c_int64, c.c_int, c.c_int, vp, vp]),
+        "vo_set_fault": (None, [c.c_int]),
+        "vo_max_threads": (c.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f3(x):
+    return (ctypes.c_float * 3)(*[float(v) for v in x])
+
+
+def pcg_hash(x: int) -> int:
+    return load().vo_pcg_hash(x & 0xffffffff)
+
+
+def rand_f(seed: int):
+    s = ctypes.c_uint32(seed)
+    v = load().vo_rand_f(ctypes.byref(s))
+    return v, s.value
+
+
+def watertight_edge(p, q) -> float:
+    return load().vo_watertight_edge(p[0], p[1], q[0], q[1])
+
+
+def intersect_triangle(o, d, a, b, c, tmin=0.0, tmax=1e20):
+    t, b1, b2 = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    hit = load().vo_intersect_triangle(_f3(o), _f3(d), _f3(a), _f3(b), _f3(c), tmin, tmax, ctypes.byref(t),
+                                       ctypes.byref(b1), ctypes.byref(b2))
+    return (t.value, b1.value, b2.value) if hit else None
+
+
+def set_fault(f: int):
+    load().vo_set_fault(f)
+
+
+def max_threads() -> int:
+    return load().vo_max_threads()
+
+
+def mesh_bounds(vertices: np.ndarray, faces: np.ndarray) -> np.ndarray:
+    v = np.ascontiguousarray(vertices, np.float32)
+    f = np.ascontiguousarray(faces, np.uint32)
+    b = np.empty((len(f), 6), np.float32)
+    load().vo_mesh_bounds(_p(v), _p(f), len(f), _p(b))
+    return b
+
+
+def widebvh_build(bounds: np.ndarray, leaf_size=2, max_depth=20):
+    """Sources/Acceleration.swift:296-351: returns (nodes structured array, order, depth)."""
+    lib = load()
+    b = np.ascontiguousarray(bounds, np.float32)
+    n = len(b)
+    order = np.empty(n, np.int32)
+    nodes = ctypes.c_void_p()
+    depth = ctypes.c_int32()
+    count = lib.vo_widebvh_build(_p(b), n, leaf_size, max_depth, ctypes.byref(nodes), _p(order), ctypes.byref(depth))
+    try:
+        arr = np.frombuffer(ctypes.string_at(nodes, 128 * count), WIDE_NODE_DTYPE).copy() if count else \
+            np.zeros(0, WIDE_NODE_DTYPE)
+    finally:
+        lib.vo_free(nodes)
+    return arr, order, depth.value
+
+
+class Scene:
+    """Document-order mesh plus the reference 4-wide BVH over it."""
+
+    def __init__(self, vertices: np.ndarray, faces: np.ndarray, with_bvh=True, leaf_size=2, max_depth=20):
+        self.vertices = np.ascontiguousarray(vertices, np.float32)
+        self.faces = np.ascontiguousarray(faces, np.uint32)
+        self.nodes = None
+        self.order = None
+        self.depth = 0
+        self.root_max = 0.0
+        if with_bvh and len(self.faces):
+            self.nodes, self.order, self.depth = widebvh_build(mesh_bounds(self.vertices, self.faces), leaf_size,
+                                                               max_depth)
+            root = self.nodes[0]
+            used = root["child"] != 0
+            lo, hi = root["lo"][:, used], root["hi"][:, used]
+            self.root_max = float(max(np.abs(lo).max(), np.abs(hi).max()))
+        self._view = SceneView(_p(self.vertices), _p(self.faces), len(self.faces),
+                               _p(self.nodes) if self.nodes is not None else None,
+                               _p(self.order) if self.order is not None else None, self.root_max)
+
+    def closest(self, rays: np.ndarray, mode=0, threads=0, counters=None):
+        """rays (8, n) float32 host SoA.  mode 0: reference BVH order; 1: brute force."""
+        if mode == 0 and self.nodes is None:
+            raise ValueError("scene built without a BVH")
+        r = np.ascontiguousarray(rays, np.float32)
+        n = r.shape[1]
+        t, b1, b2 = (np.empty(n, np.float32) for _ in range(3))
+        face = np.empty(n, np.int32)
+        c = (ctypes.c_uint64 * 2)()
+        load().vo_trace_closest_batch(ctypes.byref(self._view), *[_p(r[k]) for k in range(8)], n, mode, threads,
+                                      _p(t), _p(b1), _p(b2), _p(face), c)
+        if counters is not None:
+            counters[0] += c[0]
+            counters[1] += c[1]
+        return {"t": t, "b1": b1, "b2": b2, "face": face}
+
+    def any(self, rays: np.ndarray, mode=0, threads=0, counters=None):
+        r = np.ascontiguousarray(rays, np.float32)
+        n = r.shape[1]
+        occ = np.empty(n, np.uint8)
+        c = (ctypes.c_uint64 * 2)()
+        load().vo_trace_any_batch(ctypes.byref(self._view), *[_p(r[k]) for k in range(8)], n, mode, threads, _p(occ),
+                                  c)
+        if counters is not None:
+            counters[0] += c[0]
+            counters[1] += c[1]
+        return occ

@@ -1,0 +1,112 @@
+/*
+ * oracle.h -- CPU restatement of metal-vibe-tracer's ray-traversal hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, loads or calls this code:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, and only as the
+ * checker (or as the timed CPU baseline).  Every function names the reference file:line whose
+ * algorithm it restates.  Build: oracle/Makefile (gcc, -ffp-contract=off, OpenMP).
+ *
+ * Pinning (what checks each function against something other than itself):
+ *   vo_pcg_hash / vo_rand_f      closed-form hash; tests/GPUChecks.swift:25-30 (seed 890625759:
+ *                                second draw is exactly 0) and the published PCG-RXS-M-XS constants.
+ *   vo_watertight_edge           golden tie vector tests/Fix_renderer-followups.swift:167-168,234
+ *                                (+2^-24 / -2^-24) and exact antisymmetry (:170-176).
+ *   vo_intersect_triangle        hand-derived closed-form cases (axis rays through known triangles,
+ *                                barycentrics of the hit point), and the brute-force identity
+ *                                "every ray aimed at a shared edge/vertex of a closed mesh hits"
+ *                                (tests/Fix_accel.swift:302-337).
+ *   vo_widebvh_build             structural invariants (every item in exactly one leaf, child
+ *                                boxes enclose their items, depth limit), exhaustive SAH check on
+ *                                small inputs, and traversal == brute force.
+ *   vo_trace_*                   closest hit == brute force (t bit-exact, id by the tie rule),
+ *                                any hit == (closest hit exists), tests/Fix_geometry.swift:141-161.
+ */
+#ifndef VT_ORACLE_H
+#define VT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* main.swift:91-101 (pcg_hash, rand_f) */
+uint32_t vo_pcg_hash(uint32_t input);
+float vo_rand_f(uint32_t *seed);
+
+/* main.swift:1557-1562 (watertight_edge), contraction off, FMA tie-break */
+float vo_watertight_edge(float px, float py, float qx, float qy);
+
+/* main.swift:1537-1582 (watertight_ray + intersect_mesh_triangle): two-sided hit in [tmin, tmax).
+ * o, d: ray; a, b, c: vertices (3 floats each).  Returns 1 on hit with t, b1, b2 written. */
+int vo_intersect_triangle(const float *o, const float *d, const float *a, const float *b, const float *c,
+                          float tmin, float tmax, float *t, float *b1, float *b2);
+
+/* Sources/Acceleration.swift:445-458 (MeshSceneLayout.bounds): 6 floats per face. */
+void vo_mesh_bounds(const float *vertices /* 8 floats each */, const uint32_t *faces /* 4 u32 each */,
+                    uint32_t face_count, float *bounds /* 6 * face_count */);
+
+/* Sources/Acceleration.swift:23-30 (MeshWideNode), 128 bytes. */
+typedef struct vo_wide_node {
+    float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
+    int32_t child[4];
+    int32_t count[4];
+} vo_wide_node;
+
+/* Sources/Acceleration.swift:106-351 (WideBVH.build): binned-SAH 4-wide build, leaf size
+ * `leaf_size`, median rebuild when deeper than `max_depth`.  nodes_out is malloc'ed (free with
+ * vo_free); order_out[n] receives the item order.  Returns the node count, depth in *depth. */
+int32_t vo_widebvh_build(const float *bounds, int32_t n, int32_t leaf_size, int32_t max_depth,
+                         vo_wide_node **nodes_out, int32_t *order_out, int32_t *depth);
+void vo_free(void *p);
+
+/* A closest hit; face is the DOCUMENT face index (input order). */
+typedef struct vo_hit {
+    float t, b1, b2;
+    int32_t face; /* -1: miss */
+} vo_hit;
+
+/* Scene view used by the traversal functions: faces and vertices in document order plus a BVH
+ * over them (order maps BLAS position -> document face). */
+typedef struct vo_scene {
+    const float *vertices;    /* 8 floats per vertex */
+    const uint32_t *faces;    /* 4 u32 per face (a, b, c, tag) */
+    uint32_t face_count;
+    const vo_wide_node *nodes; /* may be NULL for brute force */
+    const int32_t *order;
+    float root_max;           /* MeshSceneLayout.Asset.maxAbs */
+} vo_scene;
+
+/* main.swift:1788-1812 (blas_closest): reference visit order, first found wins among equal t.
+ * Optional counters (may be NULL): node fetches and triangle tests. */
+vo_hit vo_blas_closest(const vo_scene *s, const float *o, const float *d, float tmin, float tmax,
+                       uint64_t *nodes_visited, uint64_t *tris_tested);
+/* main.swift:1815-1838 (blas_any), slack off. */
+int vo_blas_any(const vo_scene *s, const float *o, const float *d, float tmin, float tmax,
+                uint64_t *nodes_visited, uint64_t *tris_tested);
+
+/* Brute force over every face (tests/Fix_accel.swift:131-141): the closest hit under the
+ * canonical rule "smallest t, then smallest document face index". */
+vo_hit vo_brute_closest(const vo_scene *s, const float *o, const float *d, float tmin, float tmax);
+int vo_brute_any(const vo_scene *s, const float *o, const float *d, float tmin, float tmax);
+
+/* Batched forms over SoA rays, OpenMP-parallel over rays (threads <= 0: OpenMP default).
+ * mode 0: reference BVH order (vo_blas_closest), 1: brute force.
+ * counters[0] += node fetches, counters[1] += triangle tests (mode 0 only; may be NULL). */
+void vo_trace_closest_batch(const vo_scene *s, const float *ox, const float *oy, const float *oz,
+                            const float *dx, const float *dy, const float *dz, const float *tmin,
+                            const float *tmax, int64_t n, int mode, int threads, float *t, float *b1,
+                            float *b2, int32_t *face, uint64_t *counters);
+void vo_trace_any_batch(const vo_scene *s, const float *ox, const float *oy, const float *oz,
+                        const float *dx, const float *dy, const float *dz, const float *tmin,
+                        const float *tmax, int64_t n, int mode, int threads, uint8_t *occluded,
+                        uint64_t *counters);
+
+/* For tests that plant a mistake in the oracle (never the product): 0 = faithful,
+ * 1 = drop the FMA tie-break of watertight_edge, 2 = swap b1/b2. */
+void vo_set_fault(int fault);
+int vo_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,113 @@
+"""Small deterministic test scenes and ray sets shared by the CPU and GPU tests."""
+import numpy as np
+
+
+def mesh_from_triangles(tris: np.ndarray):
+    """(F, 3, 3) float32 corner positions -> (vertices (3F, 8), faces (F, 4)), unwelded."""
+    f = len(tris)
+    v = np.zeros((3 * f, 8), np.float32)
+    v[:, :3] = tris.reshape(-1, 3)
+    v[:, 4] = 1.0
+    faces = np.zeros((f, 4), np.uint32)
+    faces[:, :3] = np.arange(3 * f, dtype=np.uint32).reshape(f, 3)
+    return v, faces
+
+
+def patch(cells=16):
+    """The curved height-field patch of tests/Fix_renderer-followups.swift:14-35: a welded grid
+    with alternating diagonals.  Returns (vertices (V, 8), faces (F, 4), grid positions)."""
+    idx = np.arange(cells + 1, dtype=np.float32)
+    x = idx[:, None] / np.float32(cells) * np.float32(2) - np.float32(1)
+    z = idx[None, :] / np.float32(cells) * np.float32(2) - np.float32(1)
+    x, z = np.broadcast_arrays(x, z)
+    y = (np.float32(0.15) * np.sin(np.float32(1.7) * x + np.float32(0.3)) * np.cos(np.float32(1.3) * z)
+         + np.float32(0.05) * x).astype(np.float32)
+    grid = np.stack([x, y, z], -1).astype(np.float32)
+    verts = np.zeros(((cells + 1) ** 2, 8), np.float32)
+    verts[:, :3] = grid.reshape(-1, 3)
+    verts[:, 4] = 1.0
+
+    def vid(i, j):
+        return i * (cells + 1) + j
+
+    faces = []
+    for i in range(cells):
+        for j in range(cells):
+            a, b, c, d = vid(i, j), vid(i + 1, j), vid(i + 1, j + 1), vid(i, j + 1)
+            if (i + j) % 2 == 0:
+                faces += [(a, b, c, 0), (a, c, d, 0)]
+            else:
+                faces += [(a, b, d, 0), (b, c, d, 0)]
+    return verts, np.array(faces, np.uint32), grid
+
+
+def edge_targets(grid, cells=16):
+    """Interior vertices and the 1/4, 1/2, 3/4 points of every shared edge
+    (tests/Fix_renderer-followups.swift:208-218)."""
+    t = [grid[i, j] for i in range(1, cells) for j in range(1, cells)]
+
+    def edge(a, b):
+        for f in (0.25, 0.5, 0.75):
+            t.append(a + (b - a) * np.float32(f))
+
+    for i in range(cells):
+        for j in range(cells):
+            if j > 0:
+                edge(grid[i, j], grid[i + 1, j])
+            if i > 0:
+                edge(grid[i, j], grid[i, j + 1])
+            if (i + j) % 2 == 0:
+                edge(grid[i, j], grid[i + 1, j + 1])
+            else:
+                edge(grid[i + 1, j], grid[i, j + 1])
+    return np.array(t, np.float32)
+
+
+EYES = np.array([[0.3, 3, 0.2], [4, 3.5, -3], [-7, 6, 5], [0.05, 20, 0.01], [2.5, 1.8, 1.7], [-0.4, 9, -12]],
+                np.float32)
+
+
+def aimed_rays(eyes, targets, tmin=0.0, tmax=1e20):
+    """(8, len(eyes) * len(targets)) SoA rays from every eye through every target."""
+    o = np.repeat(eyes, len(targets), axis=0)
+    tg = np.tile(targets, (len(eyes), 1))
+    d = tg - o
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    r = np.empty((8, len(o)), np.float32)
+    r[0:3] = o.T
+    r[3:6] = d.T
+    r[6] = tmin
+    r[7] = tmax
+    return r
+
+
+def scattered_triangles(n=3000, seed=5):
+    """Random small triangles in [-1.2, 1.2]^3 (tests/Fix_geometry.swift:141-161 style)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1.2, 1.2, (n, 1, 3))
+    e = rng.uniform(-0.08, 0.08, (n, 3, 3))
+    return mesh_from_triangles((c + e).astype(np.float32))
+
+
+def random_rays(n, seed, box=1.2, tmin=0.0, tmax=1e20):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(-box, box, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    r = np.empty((8, n), np.float32)
+    r[0:3] = o.T
+    r[3:6] = d.T
+    r[6] = tmin
+    r[7] = tmax
+    return r
+
+
+def axis_rays(n, seed, box=1.2):
+    """Rays with exactly axis-aligned directions (the parallel-slab branches)."""
+    r = random_rays(n, seed, box)
+    rng = np.random.default_rng(seed + 1)
+    axis = rng.integers(0, 3, n)
+    sign = rng.choice([-1.0, 1.0], n).astype(np.float32)
+    r[3:6] = 0
+    r[3 + axis, np.arange(n)] = sign
+    return r

@@ -1,0 +1,186 @@
+"""CPU tests of libvt.so: exported symbols, the host BVH builder's layout invariants, error paths
+and the workload generators.  No compute on a GPU."""
+import ctypes
+import glob
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+import scenes
+import vt
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(vt_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = vt.load()
+    declared = _declared_functions()
+    assert declared, "no declarations found"
+    missing = [n for n in sorted(declared) if not hasattr(lib, n)]
+    assert not missing
+    assert declared == set(vt.EXPORTED)
+
+
+def _dequant(q, e, p):
+    # fma(q, 2^(e-127), p): the product is exact, so float64 sum then one rounding to float32.
+    s = np.ldexp(1.0, e.astype(np.int64) - 127)
+    return (p.astype(np.float64) + q.astype(np.float64) * s).astype(np.float32)
+
+
+def _check_bvh8(mesh, nodes, tris, depth, leaf_size):
+    f = len(mesh.faces)
+    assert len(tris) == f
+    assert sorted(tris["face"].tolist()) == list(range(f))
+    # Triangle records are exact copies of the document's corner positions.
+    fc = mesh.faces[tris["face"]]
+    for k, key in enumerate(("v0", "v1", "v2")):
+        assert np.array_equal(tris[key], mesh.vertices[fc[:, k], :3])
+    assert depth <= 24
+    seen_tris = np.zeros(f, np.int64)
+    stack = [(0, 1)]
+    visited = 0
+    while stack:
+        i, level = stack.pop()
+        visited += 1
+        assert level <= depth
+        nd = nodes[i]
+        rank = 0
+        for s in range(8):
+            meta = int(nd["meta"][s])
+            if meta == 0:
+                continue
+            lo = _dequant(nd["qlo"][:, s], nd["e"], nd["p"])
+            hi = _dequant(nd["qhi"][:, s], nd["e"], nd["p"])
+            if meta & 0x80:
+                assert nd["imask"] >> s & 1
+                child = int(nd["child_base"]) + rank
+                rank += 1
+                stack.append((child, level + 1))
+                sub = _subtree_tris(nodes, child)
+            else:
+                assert not (nd["imask"] >> s & 1)
+                off, cnt = meta >> 2, meta & 3
+                assert 1 <= cnt <= leaf_size
+                sub = np.arange(int(nd["tri_base"]) + off, int(nd["tri_base"]) + off + cnt)
+                seen_tris[sub] += 1
+            pts = np.concatenate([tris["v0"][sub], tris["v1"][sub], tris["v2"][sub]])
+            assert np.all(pts >= lo) and np.all(pts <= hi), "child box must enclose its triangles"
+        assert rank == bin(int(nd["imask"])).count("1")
+    assert visited == len(nodes)
+    assert np.all(seen_tris == 1)
+
+
+def _subtree_tris(nodes, i):
+    out = []
+    stack = [i]
+    while stack:
+        nd = nodes[stack.pop()]
+        rank = 0
+        for s in range(8):
+            meta = int(nd["meta"][s])
+            if meta & 0x80:
+                stack.append(int(nd["child_base"]) + rank)
+                rank += 1
+            elif meta:
+                out.extend(range(int(nd["tri_base"]) + (meta >> 2), int(nd["tri_base"]) + (meta >> 2) + (meta & 3)))
+    return np.array(out, np.int64)
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 3])
+def test_bvh8_layout_invariants(leaf):
+    mesh = vt.icosphere_field(spheres=24, subdivisions=2, seed=3, half_extent=6)
+    nodes, tris, depth, root_max = vt.build_bvh8_host(mesh, leaf_size=leaf)
+    _check_bvh8(mesh, nodes, tris, depth, leaf)
+    assert root_max == np.abs(mesh.vertices[:, :3]).max()
+
+
+def test_bvh8_degenerate_inputs():
+    # Coincident triangles (no SAH split; median fallback), a single triangle, tiny and huge scales.
+    tri = np.array([[[0, 0, 0], [1, 0, 0], [0, 1, 0]]], np.float32)
+    for tris_in in (np.repeat(tri, 50, 0), tri, tri * np.float32(1e-20), tri * np.float32(1e20) + np.float32(3e20)):
+        v, f = scenes.mesh_from_triangles(tris_in)
+        mesh = vt.Mesh(v, f)
+        nodes, t, depth, _ = vt.build_bvh8_host(mesh)
+        _check_bvh8(mesh, nodes, t, depth, 3)
+
+
+def test_bvh8_deterministic():
+    mesh = vt.icosphere_field(spheres=80, subdivisions=3, seed=4, half_extent=30)
+    a = vt.build_bvh8_host(mesh)
+    os.environ["VT_BUILD_THREADS"] = "1"
+    try:
+        b = vt.build_bvh8_host(mesh)
+    finally:
+        del os.environ["VT_BUILD_THREADS"]
+    assert a[0].tobytes() == b[0].tobytes() and a[1].tobytes() == b[1].tobytes()
+
+
+def test_bvh8_empty_scene():
+    mesh = vt.Mesh(np.zeros((0, 8), np.float32), np.zeros((0, 4), np.uint32))
+    nodes, tris, depth, _ = vt.build_bvh8_host(mesh)
+    assert len(nodes) == 0 and len(tris) == 0
+
+
+def test_error_paths():
+    mesh = vt.Mesh(np.zeros((3, 8), np.float32), np.array([[0, 1, 5, 0]], np.uint32))
+    with pytest.raises(vt.VTError) as e:
+        vt.build_bvh8_host(mesh)
+    assert e.value.status == 1 and "out of range" in e.value.message
+    ok = vt.Mesh(np.zeros((3, 8), np.float32), np.array([[0, 1, 2, 0]], np.uint32))
+    with pytest.raises(vt.VTError) as e:
+        vt.build_bvh8_host(ok, leaf_size=4)
+    assert e.value.status == 1
+    bad = ok.vertices.copy()
+    bad[0, 0] = np.nan
+    with pytest.raises(vt.VTError):
+        vt.build_bvh8_host(vt.Mesh(bad, ok.faces))
+
+
+def test_context_without_device_reports_error():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(vt.VTError) as e:
+        vt.Context(0)
+    assert e.value.message
+
+
+def test_workload_counts_and_determinism():
+    lib = vt.load()
+    nv, nf = ctypes.c_uint32(), ctypes.c_uint32()
+    lib.vt_gen_icosphere_field(1024, 3, 0.0, 0.5, 50.0, 7, None, ctypes.byref(nv), None, ctypes.byref(nf))
+    assert nf.value == 1_310_720 and nv.value == 1024 * 642
+    lib.vt_gen_heightfield(2237, 8, 0.5, 100.0, 12.0, 3, None, ctypes.byref(nv), None, ctypes.byref(nf))
+    assert nf.value == 9_999_392
+    a = vt.icosphere_field(spheres=16, seed=7)
+    b = vt.icosphere_field(spheres=16, seed=7)
+    c = vt.icosphere_field(spheres=16, seed=8)
+    assert a.vertices.tobytes() == b.vertices.tobytes()
+    assert a.vertices.tobytes() != c.vertices.tobytes()
+    r1 = vt.camera_rays((0, 1, -5), (0, 0, 0), 40, 64, 32, 1)
+    r2 = vt.camera_rays((0, 1, -5), (0, 0, 0), 40, 64, 32, 1)
+    assert r1.tobytes() == r2.tobytes()
+    assert np.allclose(np.linalg.norm(r1[3:6], axis=0), 1, atol=1e-6)
+
+
+def test_icosphere_radius_distribution():
+    m = vt.icosphere_field(spheres=1024, subdivisions=0, seed=7)
+    p = m.vertices[:, :3].reshape(1024, 12, 3).astype(np.float64)
+    c = p.mean(1)
+    r = np.linalg.norm(p - c[:, None], axis=2).mean(1)
+    lr = np.log(r)
+    assert abs(lr.mean()) < 0.05 and abs(lr.std() - 0.5) < 0.05
+    assert np.all(np.abs(c) <= 50.0 + 1e-3)
