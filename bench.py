@@ -31,6 +31,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILES = {"icosphere-field-1024x1280": "profiles/trace_closest_icosphere_field_1M3_traffic.json"}
+
+
+def measured_traffic(name):
+    """HBM bytes per traversal launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of
+    this workload (scaled by the copy-kernel calibration recorded beside them), or None."""
+    path = TRAFFIC_PROFILES.get(name)
+    if not path or not os.path.exists(os.path.join(ROOT, path)):
+        return None, None
+    with open(os.path.join(ROOT, path)) as f:
+        t = json.load(f)
+    return int(t["traffic_bytes_per_launch"]), path
+
+
 def workload_stress():
     import vt
     mesh = vt.icosphere_field(spheres=1024, subdivisions=3, mu=0.0, sigma=0.5, half_extent=50.0, seed=7)
@@ -53,6 +67,9 @@ def main():
     ap.add_argument("--workload", choices=["stress", "heightfield"], default="stress")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="rays in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device-to-device 3 times with a plain copy kernel (the "
+                         "reference launch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -125,6 +142,8 @@ def main():
     hits_p = int((out_p["face"] >= 0).sum().item())
     hits_b = int((out_b["face"] >= 0).sum().item())
 
+    traffic, traffic_src = measured_traffic(name)
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -161,11 +180,19 @@ def main():
         "kernel_ms": {"primary": round(t_prim, 4), "bounce": round(t_bnc, 4)},
         "mrays_per_s": {"primary": round(n_p / t_prim / 1e3, 1), "bounce": round(n_b / t_bnc / 1e3, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": (bytes_p + bytes_b) // 2,
                      "work": {"node_fetches": np_ + nb_, "triangle_tests": tp_ + tb_, "bytes_per_step": bytes_p + bytes_b}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {args.copy_calibration} MiB device-to-device")
     scene.close()
     ctx.close()
 

@@ -116,6 +116,7 @@ std::unique_ptr<vt_scene> set_mesh(vt_context *ctx, const float *vertices, uint3
     s->view.nodes = s->nodes.ptr;
     s->view.tris = s->tris.ptr;
     s->view.node_count = (uint32_t)bvh.nodes.size();
+    s->view.depth = (uint32_t)bvh.depth;
     s->view.root_max = bvh.root_max;
     vt_scene_info &i = s->info;
     i.face_count = face_count;

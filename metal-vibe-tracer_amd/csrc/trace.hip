@@ -14,8 +14,11 @@ namespace vt {
 namespace {
 
 constexpr int kBlock = 128;
-constexpr int kStack = 24;  // the builder bounds the tree depth by 24, so <= 23 groups are pending
 
+// A tree of depth D leaves at most D - 1 node groups pending; the launcher picks the smallest
+// instantiated depth that covers the scene's tree (the builder bounds it by 24), so LDS per block,
+// and with it the number of resident waves, follows the tree actually traced.
+template <int kStack>
 struct LdsStack {
     uint32_t (*base)[kBlock];
     uint32_t (*bits)[kBlock];
@@ -35,7 +38,7 @@ struct LdsStack {
     __device__ __forceinline__ bool empty() const { return sp == 0; }
 };
 
-template <bool ANY, bool STATS>
+template <bool ANY, bool STATS, int kStack>
 __global__ __launch_bounds__(kBlock) void trace_kernel(const U4 *__restrict__ nodes, const F4 *__restrict__ tris,
                                                        RayArgs rays, HitArgs hits, uint8_t *__restrict__ occluded,
                                                        uint32_t n, float root_max,
@@ -46,7 +49,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(const U4 *__restrict__ no
     const uint32_t i = blockIdx.x * kBlock + lane;
     if (i >= n) return;
     TraceRay r{rays.ox[i], rays.oy[i], rays.oz[i], rays.dx[i], rays.dy[i], rays.dz[i], rays.tmin[i], rays.tmax[i]};
-    LdsStack st{stack_base, stack_bits, lane, 0};
+    LdsStack<kStack> st{stack_base, stack_bits, lane, 0};
     TraceResult res = traverse<ANY, STATS>(nodes, tris, root_max, r, st);
     if (ANY) {
         occluded[i] = res.occluded ? 1 : 0;
@@ -93,13 +96,23 @@ hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitA
     dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
     const U4 *nd = reinterpret_cast<const U4 *>(scene.nodes);
     const F4 *tr = reinterpret_cast<const F4 *>(scene.tris);
-    if (counters) {
-        if (any) hipLaunchKernelGGL((trace_kernel<true, true>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, counters);
-        else hipLaunchKernelGGL((trace_kernel<false, true>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, counters);
-    } else {
-        if (any) hipLaunchKernelGGL((trace_kernel<true, false>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, nullptr);
-        else hipLaunchKernelGGL((trace_kernel<false, false>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, nullptr);
-    }
+    const int need = scene.depth > 1 ? (int)scene.depth - 1 : 1;
+    if (need > 24) return hipErrorInvalidValue;
+#define VT_LAUNCH(S)                                                                                                   \
+    do {                                                                                                               \
+        if (counters) {                                                                                                \
+            if (any) hipLaunchKernelGGL((trace_kernel<true, true, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, counters); \
+            else hipLaunchKernelGGL((trace_kernel<false, true, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, counters); \
+        } else {                                                                                                       \
+            if (any) hipLaunchKernelGGL((trace_kernel<true, false, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, nullptr); \
+            else hipLaunchKernelGGL((trace_kernel<false, false, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, nullptr); \
+        }                                                                                                              \
+    } while (0)
+    if (need <= 8) VT_LAUNCH(8);
+    else if (need <= 12) VT_LAUNCH(12);
+    else if (need <= 16) VT_LAUNCH(16);
+    else VT_LAUNCH(24);
+#undef VT_LAUNCH
     return hipGetLastError();
 }
 

@@ -17,6 +17,7 @@ struct TraceScene {
     const void *nodes;  // Node8 records (bvh8.h)
     const void *tris;   // TriRecord records
     uint32_t node_count;
+    uint32_t depth;  // tree depth (root = 1); the traversal stack holds depth - 1 groups
     float root_max;
 };
 
