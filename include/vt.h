@@ -104,6 +104,61 @@ vt_status vt_trace_any(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, uint
 vt_status vt_trace_stats(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, int any, uint64_t counters[2],
                          void *stream);
 
+/* ---------------------------------------------------------------- materials and rendering */
+
+typedef struct vt_renderer vt_renderer;
+
+/* One material slot, selected by the face tag (MeshFace.w).  Replaces the per-slot
+ * SurfaceSettings / emissions of MaterialLibrary (main.swift:2253-2263, 8926, 9002). */
+typedef struct vt_material {
+    uint32_t type;          /* MaterialType (main.swift:316-322): 0 DIFFUSE, 3 EMISSIVE */
+    float albedo[3];        /* linear sRGB reflectance, clamped to [0, 1] */
+    float emission[3];      /* linear sRGB radiance under the D65 illuminant; any > 0 makes the slot an emitter */
+    float roughness, ior, metalness, specular_weight, transmission;
+    float reserved[4];
+} vt_material;              /* 64 bytes */
+
+/* Replaces MaterialLibrary's settings/emissions and its emitter list (main.swift:9296-9346):
+ * emitting faces are chosen by area x luminance for next-event estimation.  Copies the materials;
+ * builds the spectral tables (D65 emitters) and the material reflectance spectra. */
+vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, uint32_t count);
+
+/* Per-call frame parameters: the subset of Uniforms (main.swift:393-420) this path reads. */
+typedef struct vt_frame_params {
+    uint32_t width, height;
+    float eye[3];            /* cameraPos.xyz */
+    float target[3];         /* cameraTarget.xyz (up is +y) */
+    float fov_degrees;       /* cameraPos.w, vertical */
+    uint32_t max_depth;      /* cameraTarget.w: path vertices, 1 = direct light only */
+    uint32_t sampling_mode;  /* samplingMode: 1 = MIS (NEE + BSDF, power heuristic) */
+    uint32_t sample_index;   /* sampleIndex of the first sample of this call (>= 1) */
+} vt_frame_params;
+
+/* Replaces PathTracerRenderer (main.swift:9514) for this path: owns the accumulation buffer. */
+vt_status vt_renderer_create(vt_context *ctx, vt_renderer **out);
+void vt_renderer_destroy(vt_renderer *r);
+/* Replaces renderFrame(output:samples:) (main.swift:10978): renders `samples` samples (sample
+ * indices sample_index .. sample_index + samples - 1) and folds them into the running mean
+ * accum = prev + (x - prev) / frameIndex (main.swift:7908-7913).  A size change resets the
+ * accumulation.  Asynchronous on `stream`. */
+vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params *params, uint32_t samples,
+                          void *stream);
+/* Replaces resetAccumulation() (main.swift:10761). */
+vt_status vt_reset_accumulation(vt_renderer *r);
+/* Copies the rgba32f running mean (width * height * 4 floats) to host memory; synchronises. */
+vt_status vt_accum_readback(vt_renderer *r, float *rgba, size_t bytes);
+/* Samples folded into the accumulation so far (frameIndex). */
+uint32_t vt_accum_frames(const vt_renderer *r);
+
+/* The spectral tables the integrator reads (D65 scenes), computed on the host: icdf[1025] (wavelength
+ * inverse CDF), cmf[471 * 4] (linear-sRGB CMF and cos of the warped phase per nm), basis[471 * 8]
+ * (emission basis R G B D65 | C M Y 0 per nm).  Replaces spectral_icdf_kernel's table
+ * (main.swift:8117-8185).  Any pointer may be NULL. */
+vt_status vt_spectral_tables(float *icdf, float *cmf, float *basis);
+/* Reflectance record of a linear-sRGB colour (spectral_lagrange, main.swift:889-896):
+ * (L0, L1, L2, 1) bounded-MESE multipliers, or (grey, 0, 0, 2). */
+vt_status vt_spectral_reflectance(const float rgb[3], float out[4]);
+
 /* Host-only build of the same 8-wide BVH the upload builds (no device needed), for inspection:
  * *nodes receives node_count 80-byte node records, *tris face_count 48-byte triangle records
  * (layout: metal-vibe-tracer_amd/csrc/bvh8.h).  Free both with vt_host_free.  Mirrors the

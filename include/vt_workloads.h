@@ -28,6 +28,14 @@ int vt_gen_icosphere_field(uint32_t spheres, uint32_t subdivisions, float mu, fl
 int vt_gen_heightfield(uint32_t grid, uint32_t octaves, float gain, float half_extent, float height, uint32_t seed,
                        float *vertices, uint32_t *vertex_count, uint32_t *faces, uint32_t *face_count);
 
+/* The Cornell box of the reference's procedural scene 1 as triangles (main.swift:1421-1438): five
+ * walls of [-1, 1]^3 (floor, ceiling, back: tag 0 white; left x = -1: tag 1 red; right x = 1: tag 2
+ * green), a tall box (centre (-0.33, -0.4, 0.35), half extents (0.3, 0.6, 0.3), yaw 0.32) and a
+ * short box (centre (0.33, -0.7, -0.25), half extents 0.3, yaw -0.30), tag 0, and a square ceiling
+ * light of side `light_size` at y = 0.999 emitting downwards, tag 3.  Flat-shaded, unwelded:
+ * 36 faces, 108 vertices. */
+int vt_gen_cornell(float light_size, float *vertices, uint32_t *vertex_count, uint32_t *faces, uint32_t *face_count);
+
 /* Pinhole camera rays through a jittered width x height grid (one per pixel, row-major):
  * eye, target, vertical fov in degrees.  tmin = 0, tmax = 1e20. */
 void vt_gen_camera_rays(const float eye[3], const float target[3], float fov_degrees, uint32_t width, uint32_t height,

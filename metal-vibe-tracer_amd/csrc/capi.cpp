@@ -11,7 +11,10 @@
 #include <stdexcept>
 #include <string>
 
+#include <functional>
+
 #include "bvh8.h"
+#include "scene.h"
 #include "trace_kernels.h"
 #include "vt.h"
 
@@ -19,24 +22,21 @@ namespace {
 
 thread_local std::string g_error;
 
+using vt::BudgetError;
+using vt::check;
+using vt::DeviceBuffer;
+using vt::DeviceError;
+
+}  // namespace
+
+namespace vt {
+
 vt_status fail(vt_status s, const std::string &msg) {
     g_error = msg;
     return s;
 }
 
-struct DeviceError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-struct BudgetError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-void check(hipError_t e, const char *what) {
-    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <class F>
-vt_status guarded(F &&f) {
+vt_status guarded_call(const std::function<vt_status()> &f) {
     try {
         g_error.clear();
         return f();
@@ -53,45 +53,13 @@ vt_status guarded(F &&f) {
     }
 }
 
-struct DeviceBuffer {
-    void *ptr = nullptr;
-    size_t bytes = 0;
-    DeviceBuffer() = default;
-    DeviceBuffer(const DeviceBuffer &) = delete;
-    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
-    ~DeviceBuffer() { reset(); }
-    void reset() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-    void upload(const void *src, size_t n) {
-        reset();
-        if (n == 0) return;
-        check(hipMalloc(&ptr, n), "hipMalloc");
-        bytes = n;
-        check(hipMemcpy(ptr, src, n, hipMemcpyHostToDevice), "hipMemcpy");
-    }
-};
+}  // namespace vt
 
-}  // namespace
-
-struct vt_context {
-    int device = 0;
-    uint64_t budget = 0;
-    uint64_t resident = 0;  // bytes of live scenes
-};
-
-// One published mesh: the reference's MeshStreams + BLAS nodes (Sources/Acceleration.swift:59-64).
-struct vt_scene {
-    vt_context *ctx = nullptr;
-    DeviceBuffer nodes, tris, counters;
-    vt::TraceScene view{};
-    vt_scene_info info{};
-    ~vt_scene() {
-        if (ctx) ctx->resident -= nodes.bytes + tris.bytes;
-    }
-};
+using vt::fail;
+template <class F>
+static vt_status guarded(F &&f) {
+    return vt::guarded_call(std::function<vt_status()>(std::forward<F>(f)));
+}
 
 namespace vt {
 
@@ -103,7 +71,8 @@ std::unique_ptr<vt_scene> set_mesh(vt_context *ctx, const float *vertices, uint3
     if (face_count > 0 && (!vertices || !faces)) throw std::invalid_argument("vertices and faces must not be NULL");
     if (face_count >= (1u << 30)) throw std::invalid_argument("Stored mesh assets exceed the 1,073,741,824 triangle limit.");
     Bvh8 bvh = build_bvh8(vertices, vertex_count, faces, face_count, opt);
-    const uint64_t bytes = bvh.nodes.size() * sizeof(Node8) + bvh.tris.size() * sizeof(TriRecord);
+    const uint64_t bytes = bvh.nodes.size() * sizeof(Node8) + bvh.tris.size() * sizeof(TriRecord) +
+                           (uint64_t)vertex_count * 32 + (uint64_t)face_count * 16;
     if (bvh.nodes.size() >= (1u << 26)) throw std::invalid_argument("mesh exceeds the node index range");
     if (ctx->resident + bytes > ctx->budget)
         throw BudgetError("The imported meshes exceed the safe GPU memory budget.");
@@ -111,8 +80,14 @@ std::unique_ptr<vt_scene> set_mesh(vt_context *ctx, const float *vertices, uint3
     auto s = std::make_unique<vt_scene>();
     s->nodes.upload(bvh.nodes.data(), bvh.nodes.size() * sizeof(Node8));
     s->tris.upload(bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord));
+    // Document-order geometry for hit attributes and emitter sampling (the reference's MeshStreams).
+    s->vertices.upload(vertices, (size_t)vertex_count * 8 * sizeof(float));
+    s->faces.upload(faces, (size_t)face_count * 4 * sizeof(uint32_t));
+    s->host_vertices.assign(vertices, vertices + (size_t)vertex_count * 8);
+    s->host_faces.assign(faces, faces + (size_t)face_count * 4);
     s->ctx = ctx;
-    ctx->resident += s->nodes.bytes + s->tris.bytes;
+    s->accounted = s->nodes.bytes + s->tris.bytes + s->vertices.bytes + s->faces.bytes;
+    ctx->resident += s->accounted;
     s->view.nodes = s->nodes.ptr;
     s->view.tris = s->tris.ptr;
     s->view.node_count = (uint32_t)bvh.nodes.size();

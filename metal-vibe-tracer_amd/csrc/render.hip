@@ -1,0 +1,563 @@
+// render.hip -- gfx950 kernels of the wavefront spectral path tracer (render_kernels.h).
+//
+// Estimator: the reference's pass-2 path loop in MIS mode (main.swift:7370-7914) for Lambertian
+// surfaces and triangle emitters: camera rays see emitters directly; at every non-delta vertex a
+// next-event sample of an emitter chosen by area x luminance (sample_direct_light, imported branch,
+// main.swift:2888-2917) is weighted by the power heuristic against the BSDF pdf; BSDF continuations
+// that reach an emitter are weighted by emission_weight (main.swift:3462-3466); Russian roulette after
+// the third bounce (main.swift:7819-7827); four stratified wavelengths per path (main.swift:795-833).
+// Random numbers follow the reference's per-pixel PCG streams draw for draw.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "render_kernels.h"
+#include "traverse8.h"
+
+namespace vt {
+namespace {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTwoPi = 6.28318530717958647692f;
+constexpr int kIcdfSegments = 1024;
+constexpr int kSamples = 471;
+
+// ---------------------------------------------------------------- RNG (main.swift:91-117)
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t input) {
+    uint32_t state = input * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+__device__ __forceinline__ float rand_f(uint32_t &seed) {
+    seed = pcg_hash(seed);
+    return (float)(seed >> 8u) * (1.0f / 16777216.0f);
+}
+
+// ---------------------------------------------------------------- small vector helpers
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ V3 normalize(V3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+__device__ __forceinline__ float max_abs(V3 p) { return fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))); }
+__device__ __forceinline__ V3 load_v3(const float *p) { return v3(p[0], p[1], p[2]); }
+
+struct S4 {
+    float v[4];
+};
+__device__ __forceinline__ S4 s4(float a) { return S4{{a, a, a, a}}; }
+__device__ __forceinline__ S4 mul(S4 a, S4 b) {
+    return S4{{a.v[0] * b.v[0], a.v[1] * b.v[1], a.v[2] * b.v[2], a.v[3] * b.v[3]}};
+}
+__device__ __forceinline__ S4 scale(S4 a, float s) { return S4{{a.v[0] * s, a.v[1] * s, a.v[2] * s, a.v[3] * s}}; }
+__device__ __forceinline__ float smax(S4 a) { return fmaxf(fmaxf(a.v[0], a.v[1]), fmaxf(a.v[2], a.v[3])); }
+
+// ---------------------------------------------------------------- tolerances (main.swift:1340-1369)
+__device__ __forceinline__ float ray_epsilon(V3 p) { return fmaxf(1e-7f, 9.536743e-7f * max_abs(p)); }
+__device__ __forceinline__ float ray_t_min(V3 o) { return ray_epsilon(o) * 0.25f; }
+__device__ __forceinline__ V3 ray_origin(V3 p, V3 ng, V3 dir, float error) {
+    float offset = fmaxf(error, ray_epsilon(p));
+    return p + ng * (dot(dir, ng) >= 0.0f ? offset : -offset);
+}
+__device__ __forceinline__ float endpoint_tolerance(V3 origin, V3 endpoint, float d) {
+    float legacy = 2.0f * ray_epsilon(endpoint);
+    return fmaxf(legacy, 1.907349e-6f * (max_abs(origin) + max_abs(endpoint) + d));
+}
+
+// ---------------------------------------------------------------- spectral (main.swift:795-944)
+struct Spectral {
+    const float *icdf;
+    const float4 *cmf;
+    const float4 *basis;
+};
+__device__ __forceinline__ float spectral_lambda(const Spectral &sp, float u, int k, float &weight) {
+    float x = (u + (float)k) * 0.25f * (float)kIcdfSegments;
+    uint32_t i = min((uint32_t)x, (uint32_t)kIcdfSegments - 1u);
+    float low = sp.icdf[i], width = sp.icdf[i + 1] - low;
+    weight = 0.25f * (float)kIcdfSegments * width;
+    return low + (x - (float)i) * width;
+}
+__device__ __forceinline__ uint32_t spectral_bin(float lambda) {
+    return min((uint32_t)fminf(fmaxf(lambda - 360.0f + 0.5f, 0.0f), (float)(kSamples - 1)), (uint32_t)kSamples - 1u);
+}
+struct Lanes {
+    uint32_t bin[4];
+    float weight[4];
+};
+__device__ __forceinline__ Lanes lanes_of(const Spectral &sp, float u) {
+    Lanes l;
+    for (int k = 0; k < 4; ++k) l.bin[k] = spectral_bin(spectral_lambda(sp, u, k, l.weight[k]));
+    return l;
+}
+// spectrum_rgb: sum_k s_k / (4 p(lambda_k)) * rgb CMF(lambda_k).
+__device__ __forceinline__ V3 spectrum_rgb(const Spectral &sp, const Lanes &l, S4 s) {
+    V3 rgb = v3(0.0f, 0.0f, 0.0f);
+    for (int k = 0; k < 4; ++k) {
+        float4 c = sp.cmf[l.bin[k]];
+        float w = s.v[k] * l.weight[k];
+        rgb = rgb + v3(c.x, c.y, c.z) * w;
+    }
+    return rgb;
+}
+// spectral_from_lagrange (main.swift:898-903): bounded reflectance at each lane.
+__device__ __forceinline__ S4 reflectance(const Spectral &sp, const Lanes &l, float4 L) {
+    if (L.w == 2.0f) return s4(L.x);
+    S4 r;
+    for (int k = 0; k < 4; ++k) {
+        float x = sp.cmf[l.bin[k]].w;
+        float series = L.x + 2.0f * L.y * x + 2.0f * L.z * (2.0f * x * x - 1.0f);
+        r.v[k] = atanf(series) * 0.318309886f + 0.5f;
+    }
+    return r;
+}
+// spectral_emission with the D65 illuminant (main.swift:918-944): white + secondary + primary.
+__device__ __forceinline__ S4 emission_spectrum(const Spectral &sp, const Lanes &l, V3 e) {
+    e = v3(fmaxf(e.x, 0.0f), fmaxf(e.y, 0.0f), fmaxf(e.z, 0.0f));
+    float hi = fmaxf(e.x, fmaxf(e.y, e.z));
+    if (!(hi > 0.0f)) return s4(0.0f);
+    float lo = fminf(e.x, fminf(e.y, e.z)), mid = e.x + e.y + e.z - hi - lo;
+    uint32_t pi = e.x == hi ? 0u : (e.y == hi ? 1u : 2u);
+    uint32_t si = pi != 2u && e.z == lo ? 2u : (pi != 1u && e.y == lo ? 1u : 0u);
+    float ws = fmaxf(mid - lo, 0.0f), wp = fmaxf(hi - mid, 0.0f);
+    S4 r;
+    for (int k = 0; k < 4; ++k) {
+        float4 a = sp.basis[2 * l.bin[k]], b = sp.basis[2 * l.bin[k] + 1];
+        float prim = pi == 0u ? a.x : (pi == 1u ? a.y : a.z);
+        float sec = si == 0u ? b.x : (si == 1u ? b.y : b.z);
+        r.v[k] = (lo + ws * sec + wp * prim) * a.w;
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------- hit attributes (main.swift:1587-1602, 2000-2014)
+struct Surface {
+    V3 p, n, ng;
+    float error;
+    bool front;
+    uint32_t slot;
+};
+__device__ __forceinline__ void face_corners(const RenderScene &sc, uint32_t face, V3 &a, V3 &b, V3 &c, uint32_t &tag,
+                                             const float **va, const float **vb, const float **vc) {
+    const uint32_t *f = sc.faces + 4 * (size_t)face;
+    *va = sc.vertices + 8 * (size_t)f[0];
+    *vb = sc.vertices + 8 * (size_t)f[1];
+    *vc = sc.vertices + 8 * (size_t)f[2];
+    a = load_v3(*va);
+    b = load_v3(*vb);
+    c = load_v3(*vc);
+    tag = f[3];
+}
+__device__ __forceinline__ Surface surface_at(const RenderScene &sc, uint32_t face, float b1, float b2, V3 dir) {
+    V3 a, b, c;
+    uint32_t tag;
+    const float *va, *vb, *vc;
+    face_corners(sc, face, a, b, c, tag, &va, &vb, &vc);
+    Surface s;
+    float b0 = 1.0f - b1 - b2;
+    s.p = a * b0 + b * b1 + c * b2;
+    V3 ng = normalize(cross(b - a, c - a));
+    s.front = dot(ng, dir) < 0.0f;
+    s.ng = s.front ? ng : -ng;
+    V3 smooth = load_v3(va + 3) * b0 + load_v3(vb + 3) * b1 + load_v3(vc + 3) * b2;
+    s.n = dot(smooth, smooth) > 1e-12f ? normalize(smooth) : s.ng;
+    if (dot(s.n, s.ng) < 0.0f) s.n = -s.n;
+    s.error = fmaxf(1e-7f, 1.907349e-6f * fmaxf(max_abs(a), fmaxf(max_abs(b), max_abs(c))));
+    s.slot = tag;
+    return s;
+}
+struct Mat {
+    uint32_t type;
+    V3 albedo;
+    float4 lagrange;
+    V3 emission;
+    float weight;
+};
+__device__ __forceinline__ Mat material(const RenderScene &sc, uint32_t slot) {
+    Mat m;
+    if (slot >= sc.material_count) {
+        m.type = kDiffuse;
+        m.albedo = v3(0.7f, 0.7f, 0.7f);
+        m.lagrange = make_float4(0.7f, 0.0f, 0.0f, 2.0f);
+        m.emission = v3(0.0f, 0.0f, 0.0f);
+        m.weight = 0.0f;
+        return m;
+    }
+    const float4 *r = reinterpret_cast<const float4 *>(sc.materials) + 4 * (size_t)slot;
+    float4 r0 = r[0], r2 = r[2];
+    m.type = __float_as_uint(r0.x);
+    m.albedo = v3(r0.y, r0.z, r0.w);
+    m.lagrange = r[1];
+    m.emission = v3(r2.x, r2.y, r2.z);
+    m.weight = r2.w;
+    if (m.emission.x > 0.0f || m.emission.y > 0.0f || m.emission.z > 0.0f) m.type = kEmissive;
+    return m;
+}
+
+// ---------------------------------------------------------------- MIS helpers (main.swift:2454-2476, 3462-3473)
+__device__ __forceinline__ float power_heuristic(float pf, float pg) {
+    float s = fmaxf(pf, pg);
+    if (s <= 0.0f) return 0.0f;
+    float f = pf / s, g = pg / s;
+    return f * f / (f * f + g * g);
+}
+__device__ __forceinline__ float emission_weight(bool prev_delta, bool prev_nee, bool prev_mis, float bsdf_pdf,
+                                                 float light_pdf) {
+    if (prev_delta || !prev_nee || light_pdf <= 0.0f) return 1.0f;
+    return prev_mis ? power_heuristic(bsdf_pdf, light_pdf) : 0.0f;
+}
+__device__ __forceinline__ V3 sample_cosine_hemisphere(V3 n, float r0, float r1) {
+    float phi = kTwoPi * r0;
+    float cos_th = sqrtf(r1 + (0.5f / 16777216.0f));
+    float sin_th = sqrtf(fmaxf(0.0f, 1.0f - cos_th * cos_th));
+    V3 up = fabsf(n.z) < 0.999f ? v3(0.0f, 0.0f, 1.0f) : v3(1.0f, 0.0f, 0.0f);
+    V3 u = normalize(cross(up, n));
+    V3 v = cross(n, u);
+    return normalize(u * (cosf(phi) * sin_th) + v * (sinf(phi) * sin_th) + n * cos_th);
+}
+
+// ---------------------------------------------------------------- emitters (main.swift:2720-2724, 2853-2917, 3070-3075)
+__device__ __forceinline__ float imported_geometry(V3 p, V3 position, V3 a, V3 b, V3 c) {
+    V3 d = position - p;
+    float d2 = dot(d, d);
+    V3 n = cross(b - a, c - a);
+    return d2 > 1e-12f && dot(n, n) > 1e-20f ? fmaxf(0.0f, dot(normalize(n), -d * (1.0f / sqrtf(d2)))) / d2 : 0.0f;
+}
+struct LightSample {
+    V3 position, wi, emission;
+    float dist, pdf;
+    uint32_t slot;
+};
+__device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, V3 p, uint32_t &seed) {
+    LightSample ls;
+    ls.pdf = 0.0f;
+    ls.slot = 0;
+    ls.emission = v3(0.0f, 0.0f, 0.0f);
+    ls.position = p;
+    ls.wi = v3(0.0f, 0.0f, 1.0f);
+    ls.dist = 0.0f;
+    if (sc.emitter_count == 0) return ls;
+    // importedProbability = 1 (no environment): the draw is still consumed.
+    if (!(rand_f(seed) < 1.0f)) return ls;
+    float value = rand_f(seed), ry = rand_f(seed), rz = rand_f(seed);
+    uint32_t low = 0, high = sc.emitter_count - 1;
+    while (low < high) {
+        uint32_t mid = (low + high) / 2;
+        if (sc.emitter_cdf[mid] >= value) high = mid;
+        else low = mid + 1;
+    }
+    uint32_t face = sc.emitter_face[low];
+    V3 a, b, c;
+    uint32_t tag;
+    const float *va, *vb, *vc;
+    face_corners(sc, face, a, b, c, tag, &va, &vb, &vc);
+    float root = sqrtf(ry);
+    ls.position = a * (1.0f - root) + b * (root * (1.0f - rz)) + c * (root * rz);
+    V3 delta = ls.position - p;
+    ls.dist = length(delta);
+    ls.wi = delta * (1.0f / fmaxf(ls.dist, 1e-8f));
+    float geometry = imported_geometry(p, ls.position, a, b, c);
+    Mat m = material(sc, tag);
+    ls.emission = m.emission;
+    ls.slot = tag;
+    float area_pdf = sc.emitter_total > 0.0f ? m.weight / sc.emitter_total : 0.0f;
+    ls.pdf = geometry > 1e-12f ? area_pdf / geometry : 0.0f;
+    return ls;
+}
+__device__ __forceinline__ float emitter_pdf(const RenderScene &sc, V3 p, V3 position, uint32_t face, float weight) {
+    if (sc.emitter_count == 0 || !(sc.emitter_total > 0.0f) || !(weight > 0.0f)) return 0.0f;
+    V3 a, b, c;
+    uint32_t tag;
+    const float *va, *vb, *vc;
+    face_corners(sc, face, a, b, c, tag, &va, &vb, &vc);
+    float g = imported_geometry(p, position, a, b, c);
+    return g > 1e-12f ? (weight / sc.emitter_total) / g : 0.0f;
+}
+
+// ---------------------------------------------------------------- wave-aggregated queue append
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    const uint32_t lane = __lane_id();
+    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+    }
+    return base + rank;
+}
+
+constexpr uint32_t kPrevDelta = 1u << 16, kPrevNee = 1u << 17, kPrevMis = 1u << 18;
+
+// ---------------------------------------------------------------- kernels
+__global__ __launch_bounds__(256) void generate_kernel(FrameArgs f, PathQueue q, float *radiance, uint32_t *count) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n = f.npix * f.samples;
+    if (i == 0) *count = n;
+    if (i >= n) return;
+    const uint32_t s = i / f.npix, pix = i - s * f.npix;
+    const uint32_t gx = pix % f.width, gy = pix / f.width;
+    const uint32_t sample_index = f.sample_index0 + s;
+    // pixel_sampler (main.swift:508-515) and wavelength_numbers (main.swift:2218-2230).
+    uint32_t seed = (gy * f.width + gx) ^ (sample_index * 1999999973u);
+    const uint32_t h = pcg_hash(seed ^ 0x5bd1e995u);
+    const float wl = (float)(h >> 8u) * (1.0f / 16777216.0f);
+    // frameJitter: Halton(2, 3) - 0.5 with a 1024-frame period (main.swift:8773-8786).
+    uint32_t jn = (max(1u, sample_index) - 1u) % 1024u + 1u;
+    float jx = 0.0f, jy = 0.0f;
+    {
+        uint32_t m = jn;
+        float frac = 1.0f;
+        while (m > 0u) { frac /= 2.0f; jx += frac * (float)(m % 2u); m /= 2u; }
+        m = jn;
+        frac = 1.0f;
+        while (m > 0u) { frac /= 3.0f; jy += frac * (float)(m % 3u); m /= 3u; }
+        jx -= 0.5f;
+        jy -= 0.5f;
+    }
+    // Pass-2 camera ray (main.swift:7420-7436).
+    float u = (((float)gx + (jx + 0.5f)) / (float)f.width) * 2.0f - 1.0f;
+    float v = (((float)gy + (jy + 0.5f)) / (float)f.height) * 2.0f - 1.0f;
+    v = -v;
+    u *= f.aspect * f.fov_scale;
+    v *= f.fov_scale;
+    V3 fw = load_v3(f.forward), rt = load_v3(f.right), up = load_v3(f.up), eye = load_v3(f.eye);
+    V3 d = normalize(fw + rt * u + up * v);
+    // decorrelate_shading_seed (main.swift:106-108): pass 2 draws from a rehashed stream.
+    seed = pcg_hash(seed ^ 0x27d4eb2fu);
+    q.ox[i] = eye.x; q.oy[i] = eye.y; q.oz[i] = eye.z;
+    q.dx[i] = d.x; q.dy[i] = d.y; q.dz[i] = d.z;
+    q.tmin[i] = ray_t_min(eye);
+    q.tmax[i] = 1e20f;
+    q.id[i] = i;
+    q.seed[i] = seed;
+    q.wl[i] = wl;
+    q.t0[i] = q.t1[i] = q.t2[i] = q.t3[i] = 1.0f;
+    q.state[i] = 0u;
+    q.pdf[i] = 0.0f;
+    q.px[i] = eye.x; q.py[i] = eye.y; q.pz[i] = eye.z;
+    radiance[3 * (size_t)i] = 0.0f;
+    radiance[3 * (size_t)i + 1] = 0.0f;
+    radiance[3 * (size_t)i + 2] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f, PathQueue in, HitArgs hits,
+                                                    PathQueue out, ShadowQueue sh, Counters c, uint32_t capacity,
+                                                    float *radiance) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n = min(*c.in, capacity);
+    const bool active = i < n;
+    bool cont = false, shadow = false;
+    // Continuation and shadow-ray values produced below (written after the wave-wide appends).
+    V3 new_o = v3(0.0f, 0.0f, 0.0f), new_d = v3(0.0f, 0.0f, 1.0f), prev_p = v3(0.0f, 0.0f, 0.0f);
+    V3 sh_o = new_o, sh_d = new_d, sh_c = new_o;
+    float sh_tmin = 0.0f, sh_tmax = 0.0f, new_pdf = 0.0f;
+    uint32_t id = 0, seed = 0, state = 0;
+    float wl = 0.0f;
+    S4 thr = s4(0.0f);
+    if (active) {
+        id = in.id[i];
+        seed = in.seed[i];
+        wl = in.wl[i];
+        thr = S4{{in.t0[i], in.t1[i], in.t2[i], in.t3[i]}};
+        state = in.state[i];
+        const uint32_t bounce = state & 0xffu;
+        int sdepth = (int)((state >> 8) & 0xffu);
+        const int face = hits.face[i];
+        const V3 dir = v3(in.dx[i], in.dy[i], in.dz[i]);
+        const Spectral sp{sc.icdf, reinterpret_cast<const float4 *>(sc.cmf), reinterpret_cast<const float4 *>(sc.basis)};
+        const Lanes lanes = lanes_of(sp, wl);
+        V3 add = v3(0.0f, 0.0f, 0.0f);
+        bool alive = face >= 0;
+        Surface s;
+        Mat m;
+        if (alive) {
+            s = surface_at(sc, (uint32_t)face, hits.b1[i], hits.b2[i], dir);
+            m = material(sc, s.slot);
+            if (m.type == kEmissive) {
+                V3 e = s.front ? m.emission : v3(0.0f, 0.0f, 0.0f);
+                if (bounce == 0u) {
+                    add = e;  // camera rays see the exact RGB (emitter_rgb, D65)
+                } else {
+                    const V3 prev = v3(in.px[i], in.py[i], in.pz[i]);
+                    float light_pdf = emitter_pdf(sc, prev, s.p, (uint32_t)face, m.weight);
+                    float w = emission_weight((state & kPrevDelta) != 0u, (state & kPrevNee) != 0u,
+                                              (state & kPrevMis) != 0u, in.pdf[i], light_pdf);
+                    add = spectrum_rgb(sp, lanes, scale(mul(thr, emission_spectrum(sp, lanes, e)), w));
+                }
+                alive = false;
+            }
+        }
+        if (alive) {
+            // Next-event estimation at this vertex (Lambert: never delta).
+            const bool use_mis = f.sampling_mode <= 1u && (bounce == 0u ? f.scattering_limit > 0 : sdepth < f.scattering_limit);
+            LightSample ls = sample_emitter(sc, s.p, seed);
+            if (ls.pdf > 0.0f) {
+                V3 o = ray_origin(s.p, s.ng, ls.wi, s.error);
+                V3 delta = ls.position - o;
+                float d = length(delta);
+                float tol = endpoint_tolerance(o, ls.position, d);
+                if (d > tol) {
+                    // eval_bsdf_with_pdf for DIFFUSE (main.swift:2659-2684).
+                    const V3 wo = -dir;
+                    float pdf = dot(s.n, wo) <= 0.0f ? 0.0f : fmaxf(0.0f, dot(s.n, ls.wi)) / kPi;
+                    float cosine = fabsf(dot(s.n, ls.wi));
+                    bool f_ok = dot(ls.wi, s.ng) > 0.0f && cosine >= 1e-7f && dot(s.n, wo) > 0.0f && dot(s.n, ls.wi) > 0.0f;
+                    if (f_ok) {
+                        S4 fbsdf = scale(reflectance(sp, lanes, m.lagrange), 1.0f / kPi);
+                        float w = use_mis ? power_heuristic(ls.pdf, pdf) : 1.0f;
+                        S4 le = emission_spectrum(sp, lanes, ls.emission);
+                        sh_c = spectrum_rgb(sp, lanes, scale(mul(mul(thr, fbsdf), le), cosine * w / ls.pdf));
+                        sh_o = o;
+                        sh_d = normalize(delta);
+                        sh_tmin = ray_t_min(o);
+                        sh_tmax = d - tol;
+                        shadow = true;
+                    }
+                }
+            }
+            // Russian roulette after the third bounce (main.swift:7819-7827).
+            if (bounce > 3u) {
+                float survival = fminf(fmaxf(smax(thr), 0.05f), 0.95f);
+                if (rand_f(seed) >= survival) alive = false;
+                else thr = scale(thr, 1.0f / survival);
+            }
+        }
+        if (alive) {
+            // Departure: the scattering budget, then the BSDF sample (main.swift:7712-7742).
+            if (sdepth >= f.scattering_limit) alive = false;
+        }
+        if (alive) {
+            ++sdepth;
+            float r0 = rand_f(seed);
+            (void)r0;  // the lobe-choice draw the generic sampler keeps (main.swift:3354)
+            float r1 = rand_f(seed), r2 = rand_f(seed);
+            V3 nd = sample_cosine_hemisphere(s.n, r1, r2);
+            float pdf = fmaxf(0.0f, dot(s.n, nd)) / kPi;
+            if (pdf > 0.0f && dot(nd, s.ng) > 0.0f) {
+                thr = mul(thr, reflectance(sp, lanes, m.lagrange));
+                bool finite = isfinite(thr.v[0]) && isfinite(thr.v[1]) && isfinite(thr.v[2]) && isfinite(thr.v[3]);
+                if (finite && smax(thr) > 0.0f) {
+                    new_o = ray_origin(s.p, s.ng, nd, s.error);
+                    new_d = nd;
+                    new_pdf = pdf;
+                    prev_p = s.p;
+                    uint32_t flags = kPrevNee | (f.sampling_mode == 1u ? kPrevMis : 0u);
+                    state = (bounce + 1u) | ((uint32_t)sdepth << 8) | flags;
+                    cont = true;
+                }
+            }
+        }
+        if (add.x != 0.0f || add.y != 0.0f || add.z != 0.0f) {
+            radiance[3 * (size_t)id] += add.x;
+            radiance[3 * (size_t)id + 1] += add.y;
+            radiance[3 * (size_t)id + 2] += add.z;
+        }
+    }
+    const uint32_t slot = wave_append(c.out, cont);
+    const uint32_t sslot = wave_append(c.shadow, shadow);
+    if (cont && slot < capacity) {
+        out.ox[slot] = new_o.x; out.oy[slot] = new_o.y; out.oz[slot] = new_o.z;
+        out.dx[slot] = new_d.x; out.dy[slot] = new_d.y; out.dz[slot] = new_d.z;
+        out.tmin[slot] = ray_t_min(new_o);
+        out.tmax[slot] = 1e20f;
+        out.id[slot] = id;
+        out.seed[slot] = seed;
+        out.wl[slot] = wl;
+        out.t0[slot] = thr.v[0]; out.t1[slot] = thr.v[1]; out.t2[slot] = thr.v[2]; out.t3[slot] = thr.v[3];
+        out.state[slot] = state;
+        out.pdf[slot] = new_pdf;
+        out.px[slot] = prev_p.x; out.py[slot] = prev_p.y; out.pz[slot] = prev_p.z;
+    }
+    if (shadow && sslot < capacity) {
+        sh.ox[sslot] = sh_o.x; sh.oy[sslot] = sh_o.y; sh.oz[sslot] = sh_o.z;
+        sh.dx[sslot] = sh_d.x; sh.dy[sslot] = sh_d.y; sh.dz[sslot] = sh_d.z;
+        sh.tmin[sslot] = sh_tmin;
+        sh.tmax[sslot] = sh_tmax;
+        sh.id[sslot] = id;
+        sh.r[sslot] = sh_c.x; sh.g[sslot] = sh_c.y; sh.b[sslot] = sh_c.z;
+    }
+}
+
+__global__ __launch_bounds__(256) void resolve_kernel(ShadowQueue sh, const uint8_t *occluded, const uint32_t *count,
+                                                      uint32_t capacity, float *radiance) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= min(*count, capacity)) return;
+    if (occluded[i]) return;
+    const uint32_t id = sh.id[i];
+    radiance[3 * (size_t)id] += sh.r[i];
+    radiance[3 * (size_t)id + 1] += sh.g[i];
+    radiance[3 * (size_t)id + 2] += sh.b[i];
+}
+
+__global__ void reset_kernel(uint32_t *counters, uint32_t n) {
+    if (threadIdx.x < n) counters[threadIdx.x] = 0u;
+}
+
+// Running mean per pixel over this wavefront's samples in sample order (main.swift:7862-7913):
+// NaN/Inf samples count as 0; negative channels stay (spectral transport).
+__global__ __launch_bounds__(256) void accumulate_kernel(FrameArgs f, const float *radiance, float *accum,
+                                                         uint32_t frame_index) {
+    const uint32_t pix = blockIdx.x * 256 + threadIdx.x;
+    if (pix >= f.npix) return;
+    float4 a = reinterpret_cast<float4 *>(accum)[pix];
+    for (uint32_t s = 0; s < f.samples; ++s) {
+        const size_t id = (size_t)s * f.npix + pix;
+        float r = radiance[3 * id], g = radiance[3 * id + 1], b = radiance[3 * id + 2];
+        if (!(isfinite(r) && isfinite(g) && isfinite(b))) r = g = b = 0.0f;
+        const uint32_t k = frame_index + s;
+        if (k <= 1u) {
+            a = make_float4(r, g, b, 1.0f);
+        } else {
+            a.x = a.x + (r - a.x) / (float)k;
+            a.y = a.y + (g - a.y) / (float)k;
+            a.z = a.z + (b - a.z) / (float)k;
+            a.w = 1.0f;
+        }
+    }
+    reinterpret_cast<float4 *>(accum)[pix] = a;
+}
+
+}  // namespace
+
+hipError_t launch_generate(const FrameArgs &f, const PathQueue &q, float *radiance, uint32_t *count, hipStream_t s) {
+    const uint32_t n = f.npix * f.samples;
+    hipLaunchKernelGGL(generate_kernel, dim3((n + 255) / 256), dim3(256), 0, s, f, q, radiance, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
+                        const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
+                        float *radiance, hipStream_t s) {
+    hipLaunchKernelGGL(shade_kernel, dim3((capacity + 255) / 256), dim3(256), 0, s, sc, f, in, hits, out, sh, c,
+                       capacity, radiance);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(const ShadowQueue &sh, const uint8_t *occluded, const uint32_t *count, uint32_t capacity,
+                          float *radiance, hipStream_t s) {
+    hipLaunchKernelGGL(resolve_kernel, dim3((capacity + 255) / 256), dim3(256), 0, s, sh, occluded, count, capacity,
+                       radiance);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_counters(uint32_t *counters, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, counters, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate(const FrameArgs &f, const float *radiance, float *accum, uint32_t frame_index,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(accumulate_kernel, dim3((f.npix + 255) / 256), dim3(256), 0, s, f, radiance, accum, frame_index);
+    return hipGetLastError();
+}
+
+}  // namespace vt

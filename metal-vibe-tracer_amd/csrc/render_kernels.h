@@ -1,0 +1,79 @@
+// render_kernels.h -- device data of the wavefront path tracer and its launch interface.
+//
+// The frame runs as a wavefront: generate (camera rays) -> [extend (closest hit) -> shade
+// (emission with MIS, next-event sample, Russian roulette, BSDF sample, compaction of surviving
+// paths) -> shadow (any hit) -> resolve (add unoccluded next-event contributions)] per bounce ->
+// accumulate.  Every queue is structure-of-arrays and compacted with one atomic per wavefront.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "trace_kernels.h"
+
+namespace vt {
+
+// Material slot record: 4 x float4 per slot (indexed by the face tag).
+//   [0] type (as float bits), albedo rgb
+//   [1] reflectance Lagrange multipliers (L0, L1, L2, w): w = 1 MESE, w = 2 flat grey in x
+//   [2] emission rgb, light-selection weight (luminance; 0 = not an emitter)
+//   [3] roughness, ior, metalness, specular weight (OpenPBR inputs)
+enum MaterialType : uint32_t { kDiffuse = 0, kGlossy = 1, kDielectric = 2, kEmissive = 3, kOpenPbr = 4 };
+
+struct RenderScene {
+    TraceScene trace;
+    const float *vertices;     // 8 floats per vertex (document order)
+    const uint32_t *faces;     // 4 u32 per face (document order)
+    const float *materials;    // 16 floats per slot
+    uint32_t material_count;
+    const uint32_t *emitter_face;  // emitting faces (power sampling), ascending face index
+    const float *emitter_cdf;      // cumulative area x weight / total; last = 1
+    uint32_t emitter_count;
+    float emitter_total;           // sum of area x weight
+    const float *icdf;             // 1025 wavelength inverse-CDF nodes
+    const float *cmf;              // 471 x (r, g, b, cos phase)
+    const float *basis;            // 471 x 2 x 4 emission basis (w of the first: D65)
+};
+
+struct FrameArgs {
+    uint32_t width, height, npix;
+    uint32_t samples;          // samples in this wavefront (path id = sample * npix + pixel)
+    uint32_t sample_index0;    // Uniforms.sampleIndex of sample 0
+    float eye[3], forward[3], right[3], up[3];
+    float fov_scale, aspect;
+    int32_t scattering_limit;  // max(0, depth - 1)
+    uint32_t sampling_mode;    // 1 = MIS (NEE + BSDF, power heuristic)
+};
+
+// One queue of paths: the ray arrays first (RayArgs layout), then the path state.
+struct PathQueue {
+    float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+    uint32_t *id;       // path id (sample * npix + pixel)
+    uint32_t *seed;     // PCG state
+    float *wl;          // wavelength number u
+    float *t0, *t1, *t2, *t3;  // throughput per wavelength lane
+    uint32_t *state;    // bounce | scattering depth << 8 | flags << 16
+    float *pdf;         // BSDF pdf of the departure that produced the ray
+    float *px, *py, *pz;  // position the ray left
+};
+
+struct ShadowQueue {
+    float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+    uint32_t *id;
+    float *r, *g, *b;   // contribution if unoccluded
+};
+
+struct Counters {
+    uint32_t *in, *out, *shadow;
+};
+
+hipError_t launch_generate(const FrameArgs &f, const PathQueue &q, float *radiance, uint32_t *count, hipStream_t s);
+hipError_t launch_shade(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
+                        const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
+                        float *radiance, hipStream_t s);
+hipError_t launch_resolve(const ShadowQueue &sh, const uint8_t *occluded, const uint32_t *count, uint32_t capacity,
+                          float *radiance, hipStream_t s);
+hipError_t launch_reset_counters(uint32_t *counters, uint32_t n, hipStream_t s);
+hipError_t launch_accumulate(const FrameArgs &f, const float *radiance, float *accum, uint32_t frame_index,
+                             hipStream_t s);
+
+}  // namespace vt

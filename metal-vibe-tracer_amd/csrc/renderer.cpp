@@ -1,0 +1,298 @@
+// renderer.cpp -- materials, emitter list and the wavefront frame driver behind vt_render_frame.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "render_kernels.h"
+#include "scene.h"
+#include "spectral.h"
+#include "vt.h"
+
+using vt::check;
+using vt::DeviceBuffer;
+using vt::fail;
+
+namespace {
+
+// Path-state arrays of one queue: 8 ray arrays, then id, seed, wl, t0-t3, state, pdf, px, py, pz.
+constexpr int kQueueArrays = 8 + 12;
+constexpr int kShadowArrays = 8 + 4;
+
+struct QueueStorage {
+    DeviceBuffer buf;
+    vt::PathQueue view(uint32_t cap) const {
+        char *b = buf.as<char>();
+        auto f = [&](int k) { return reinterpret_cast<float *>(b + (size_t)k * cap * 4); };
+        auto u = [&](int k) { return reinterpret_cast<uint32_t *>(b + (size_t)k * cap * 4); };
+        vt::PathQueue q;
+        q.ox = f(0); q.oy = f(1); q.oz = f(2); q.dx = f(3); q.dy = f(4); q.dz = f(5); q.tmin = f(6); q.tmax = f(7);
+        q.id = u(8); q.seed = u(9); q.wl = f(10);
+        q.t0 = f(11); q.t1 = f(12); q.t2 = f(13); q.t3 = f(14);
+        q.state = u(15); q.pdf = f(16); q.px = f(17); q.py = f(18); q.pz = f(19);
+        return q;
+    }
+};
+
+}  // namespace
+
+struct vt_renderer {
+    vt_context *ctx = nullptr;
+    uint32_t width = 0, height = 0, frame_index = 0;
+    uint32_t capacity = 0;  // paths per wavefront
+    DeviceBuffer accum, radiance, hits, occluded, counters, shadow;
+    QueueStorage q[2];
+};
+
+namespace {
+
+void ensure_capacity(vt_renderer *r, uint32_t cap) {
+    if (r->capacity >= cap) return;
+    r->q[0].buf.alloc((size_t)kQueueArrays * cap * 4);
+    r->q[1].buf.alloc((size_t)kQueueArrays * cap * 4);
+    r->shadow.alloc((size_t)kShadowArrays * cap * 4);
+    r->hits.alloc((size_t)4 * cap * 4);
+    r->occluded.alloc(cap);
+    r->radiance.alloc((size_t)3 * cap * 4);
+    r->capacity = cap;
+}
+
+vt::ShadowQueue shadow_view(const vt_renderer *r) {
+    char *b = r->shadow.as<char>();
+    const size_t cap = r->capacity;
+    auto f = [&](int k) { return reinterpret_cast<float *>(b + (size_t)k * cap * 4); };
+    vt::ShadowQueue s;
+    s.ox = f(0); s.oy = f(1); s.oz = f(2); s.dx = f(3); s.dy = f(4); s.dz = f(5); s.tmin = f(6); s.tmax = f(7);
+    s.id = reinterpret_cast<uint32_t *>(f(8));
+    s.r = f(9); s.g = f(10); s.b = f(11);
+    return s;
+}
+
+void normalize3(float *v) {
+    float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int k = 0; k < 3; ++k) v[k] /= l;
+}
+
+}  // namespace
+
+extern "C" {
+
+vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, uint32_t count) {
+    return vt::guarded_call([&]() -> vt_status {
+        if (!scene || (count && !materials)) return fail(VT_ERROR_INVALID_ARGUMENT, "scene and materials must not be NULL");
+        check(hipSetDevice(scene->ctx->device), "hipSetDevice");
+        if (!scene->spectral) {
+            scene->spectral = std::make_unique<vt::SpectralTables>();
+            vt::build_spectral_tables(*scene->spectral);
+            const vt::SpectralTables &t = *scene->spectral;
+            scene->icdf.upload(t.icdf, sizeof t.icdf);
+            scene->cmf.upload(t.cmf, sizeof t.cmf);
+            scene->basis.upload(t.basis, sizeof t.basis);
+        }
+        const vt::SpectralTables &t = *scene->spectral;
+        // Slot records (render_kernels.h); an emitter's selection weight is its luminance
+        // (main.swift:9301-9309).
+        std::vector<float> rec((size_t)std::max<uint32_t>(count, 1) * 16, 0.0f);
+        std::vector<double> weight(count, 0.0);
+        for (uint32_t k = 0; k < count; ++k) {
+            const vt_material &m = materials[k];
+            if (m.type != 0 && m.type != 3) return fail(VT_ERROR_UNSUPPORTED, "material type not supported yet");
+            for (int c = 0; c < 3; ++c)
+                if (!std::isfinite(m.albedo[c]) || !std::isfinite(m.emission[c]) || m.emission[c] < 0.0f)
+                    return fail(VT_ERROR_INVALID_ARGUMENT, "material colours must be finite, emission >= 0");
+            float *r = rec.data() + 16 * (size_t)k;
+            uint32_t type = m.type;
+            std::memcpy(&r[0], &type, 4);
+            float alb[3];
+            for (int c = 0; c < 3; ++c) alb[c] = std::min(std::max(m.albedo[c], 0.0f), 1.0f);
+            r[1] = alb[0]; r[2] = alb[1]; r[3] = alb[2];
+            vt::reflectance_lagrange(t, alb, r + 4);
+            r[8] = m.emission[0]; r[9] = m.emission[1]; r[10] = m.emission[2];
+            if (m.emission[0] > 0 || m.emission[1] > 0 || m.emission[2] > 0)
+                weight[k] = std::max(1e-8, 0.2126 * m.emission[0] + 0.7152 * m.emission[1] + 0.0722 * m.emission[2]);
+            r[11] = (float)weight[k];
+            r[12] = m.roughness; r[13] = m.ior; r[14] = m.metalness; r[15] = m.specular_weight;
+        }
+        // Power-sampling emitter list: every face of an emitting slot, weighted by world area x
+        // weight, in document order; CDF normalised with the last entry exactly 1 (main.swift:9312-9346).
+        std::vector<uint32_t> ids;
+        std::vector<double> power;
+        const size_t nf = scene->host_faces.size() / 4;
+        for (size_t f = 0; f < nf; ++f) {
+            uint32_t slot = scene->host_faces[4 * f + 3];
+            if (slot >= count || weight[slot] <= 0.0) continue;
+            const float *a = &scene->host_vertices[8 * (size_t)scene->host_faces[4 * f]];
+            const float *b = &scene->host_vertices[8 * (size_t)scene->host_faces[4 * f + 1]];
+            const float *c = &scene->host_vertices[8 * (size_t)scene->host_faces[4 * f + 2]];
+            double e1[3], e2[3];
+            for (int q = 0; q < 3; ++q) { e1[q] = (double)(b[q] - a[q]); e2[q] = (double)(c[q] - a[q]); }
+            double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+            ids.push_back((uint32_t)f);
+            power.push_back(0.5 * std::sqrt(cx * cx + cy * cy + cz * cz) * (double)(float)weight[slot]);
+        }
+        double total = 0;
+        std::vector<float> cdf(ids.size());
+        for (double p : power) total += p;
+        if (!(total > 0) || !std::isfinite(total)) { ids.clear(); cdf.clear(); total = 0; }
+        double run = 0;
+        for (size_t k = 0; k < cdf.size(); ++k) {
+            run += power[k];
+            cdf[k] = k + 1 == cdf.size() ? 1.0f : (float)(run / total);
+        }
+        scene->materials.upload(rec.data(), rec.size() * 4);
+        scene->emitter_face.upload(ids.data(), ids.size() * 4);
+        scene->emitter_cdf.upload(cdf.data(), cdf.size() * 4);
+        scene->material_count = count;
+        scene->emitter_count = (uint32_t)ids.size();
+        scene->emitter_total = (float)total;
+        scene->shaded = true;
+        return VT_OK;
+    });
+}
+
+static const vt::SpectralTables &host_tables() {
+    static vt::SpectralTables *t = [] {
+        auto *p = new vt::SpectralTables();
+        vt::build_spectral_tables(*p);
+        return p;
+    }();
+    return *t;
+}
+
+vt_status vt_spectral_tables(float *icdf, float *cmf, float *basis) {
+    return vt::guarded_call([&]() -> vt_status {
+        const vt::SpectralTables &t = host_tables();
+        if (icdf) std::memcpy(icdf, t.icdf, sizeof t.icdf);
+        if (cmf) std::memcpy(cmf, t.cmf, sizeof t.cmf);
+        if (basis) std::memcpy(basis, t.basis, sizeof t.basis);
+        return VT_OK;
+    });
+}
+
+vt_status vt_spectral_reflectance(const float rgb[3], float out[4]) {
+    return vt::guarded_call([&]() -> vt_status {
+        if (!rgb || !out) return fail(VT_ERROR_INVALID_ARGUMENT, "rgb and out must not be NULL");
+        vt::reflectance_lagrange(host_tables(), rgb, out);
+        return VT_OK;
+    });
+}
+
+vt_status vt_renderer_create(vt_context *ctx, vt_renderer **out) {
+    return vt::guarded_call([&]() -> vt_status {
+        if (!ctx || !out) return fail(VT_ERROR_INVALID_ARGUMENT, "context and out must not be NULL");
+        auto r = std::make_unique<vt_renderer>();
+        r->ctx = ctx;
+        r->counters.alloc(16 * sizeof(uint32_t));
+        *out = r.release();
+        return VT_OK;
+    });
+}
+
+void vt_renderer_destroy(vt_renderer *r) { delete r; }
+
+vt_status vt_reset_accumulation(vt_renderer *r) {
+    if (!r) return fail(VT_ERROR_INVALID_ARGUMENT, "renderer must not be NULL");
+    r->frame_index = 0;
+    return VT_OK;
+}
+
+uint32_t vt_accum_frames(const vt_renderer *r) { return r ? r->frame_index : 0; }
+
+vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params *p, uint32_t samples, void *stream) {
+    return vt::guarded_call([&]() -> vt_status {
+        if (!r || !scene || !p) return fail(VT_ERROR_INVALID_ARGUMENT, "renderer, scene and params must not be NULL");
+        if (!scene->shaded) return fail(VT_ERROR_INVALID_ARGUMENT, "scene has no materials (vt_scene_set_materials)");
+        if (p->width == 0 || p->height == 0 || (uint64_t)p->width * p->height > (1u << 26))
+            return fail(VT_ERROR_INVALID_ARGUMENT, "render size out of range");
+        if (p->max_depth < 1 || p->max_depth > 64) return fail(VT_ERROR_INVALID_ARGUMENT, "max_depth must be 1..64");
+        if (p->sampling_mode != 1) return fail(VT_ERROR_UNSUPPORTED, "only sampling_mode 1 (MIS) is implemented");
+        if (p->sample_index == 0) return fail(VT_ERROR_INVALID_ARGUMENT, "sample_index starts at 1");
+        if (samples == 0) return VT_OK;
+        check(hipSetDevice(r->ctx->device), "hipSetDevice");
+        hipStream_t s = (hipStream_t)stream;
+        const uint32_t npix = p->width * p->height;
+        if (p->width != r->width || p->height != r->height) {
+            r->accum.alloc((size_t)npix * 16);
+            check(hipMemsetAsync(r->accum.ptr, 0, r->accum.bytes, s), "hipMemsetAsync");
+            r->width = p->width;
+            r->height = p->height;
+            r->frame_index = 0;
+        }
+        // Paths per wavefront: whole images, up to ~4M paths.
+        const uint32_t per = std::max<uint32_t>(1, std::min<uint32_t>(samples, (1u << 22) / npix));
+        ensure_capacity(r, per * npix);
+
+        vt::FrameArgs f{};
+        f.width = p->width;
+        f.height = p->height;
+        f.npix = npix;
+        float fw[3] = {p->target[0] - p->eye[0], p->target[1] - p->eye[1], p->target[2] - p->eye[2]};
+        normalize3(fw);
+        float rt[3] = {fw[1] * 0.0f - fw[2] * 1.0f, fw[2] * 0.0f - fw[0] * 0.0f, fw[0] * 1.0f - fw[1] * 0.0f};
+        normalize3(rt);
+        float up[3] = {rt[1] * fw[2] - rt[2] * fw[1], rt[2] * fw[0] - rt[0] * fw[2], rt[0] * fw[1] - rt[1] * fw[0]};
+        for (int k = 0; k < 3; ++k) {
+            f.eye[k] = p->eye[k];
+            f.forward[k] = fw[k];
+            f.right[k] = rt[k];
+            f.up[k] = up[k];
+        }
+        f.fov_scale = std::tan((p->fov_degrees * 0.5f) * 3.14159265358979323846f / 180.0f);
+        f.aspect = (float)p->width / (float)p->height;
+        f.scattering_limit = std::max(0, (int)p->max_depth - 1);
+        f.sampling_mode = p->sampling_mode;
+
+        const vt::RenderScene sc = scene->render_view();
+        uint32_t *cnt = r->counters.as<uint32_t>();
+        const vt::ShadowQueue sh = shadow_view(r);
+        char *hb = r->hits.as<char>();
+        const size_t cap = r->capacity;
+        vt::HitArgs hits{reinterpret_cast<float *>(hb), reinterpret_cast<float *>(hb + 4 * cap),
+                         reinterpret_cast<float *>(hb + 8 * cap), reinterpret_cast<int32_t *>(hb + 12 * cap), nullptr};
+        for (uint32_t done = 0; done < samples; done += per) {
+            f.samples = std::min(per, samples - done);
+            f.sample_index0 = p->sample_index + done;
+            const uint32_t n = f.samples * npix;
+            check(vt::launch_reset_counters(cnt, 16, s), "reset");
+            int cur = 0;
+            check(vt::launch_generate(f, r->q[0].view(r->capacity), r->radiance.as<float>(), cnt + 0, s), "generate");
+            for (int b = 0; b <= f.scattering_limit; ++b) {
+                uint32_t *cin = cnt + cur, *cout = cnt + (1 - cur), *csh = cnt + 2;
+                check(vt::launch_reset_counters(cout, 1, s), "reset");
+                check(vt::launch_reset_counters(csh, 1, s), "reset");
+                const vt::PathQueue qin = r->q[cur].view(r->capacity), qout = r->q[1 - cur].view(r->capacity);
+                vt::RayArgs rays{qin.ox, qin.oy, qin.oz, qin.dx, qin.dy, qin.dz, qin.tmin, qin.tmax};
+                check(vt::launch_trace(scene->view, rays, hits, nullptr, n, false, nullptr, s, cin), "extend");
+                vt::Counters c{cin, cout, csh};
+                check(vt::launch_shade(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), s), "shade");
+                vt::RayArgs srays{sh.ox, sh.oy, sh.oz, sh.dx, sh.dy, sh.dz, sh.tmin, sh.tmax};
+                check(vt::launch_trace(scene->view, srays, vt::HitArgs{}, r->occluded.as<uint8_t>(), n, true, nullptr, s, csh),
+                      "shadow");
+                check(vt::launch_resolve(sh, r->occluded.as<uint8_t>(), csh, n, r->radiance.as<float>(), s), "resolve");
+                cur = 1 - cur;
+            }
+            check(vt::launch_accumulate(f, r->radiance.as<float>(), r->accum.as<float>(), r->frame_index + 1, s),
+                  "accumulate");
+            r->frame_index += f.samples;
+        }
+        return VT_OK;
+    });
+}
+
+vt_status vt_accum_readback(vt_renderer *r, float *rgba, size_t bytes) {
+    return vt::guarded_call([&]() -> vt_status {
+        if (!r || !rgba) return fail(VT_ERROR_INVALID_ARGUMENT, "renderer and output must not be NULL");
+        const size_t need = (size_t)r->width * r->height * 16;
+        if (bytes < need) return fail(VT_ERROR_INVALID_ARGUMENT, "output buffer too small");
+        if (need == 0) return VT_OK;
+        check(hipSetDevice(r->ctx->device), "hipSetDevice");
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        check(hipMemcpy(rgba, r->accum.ptr, need, hipMemcpyDeviceToHost), "hipMemcpy");
+        return VT_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// scene.h -- the published scene (vt_scene) and the context, shared by capi.cpp and renderer.cpp.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "render_kernels.h"
+#include "spectral.h"
+#include "trace_kernels.h"
+#include "vt.h"
+
+namespace vt {
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct BudgetError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// Error plumbing of the C ABI (capi.cpp): set the thread's message and return the status; run a
+// body, mapping exceptions to statuses.
+vt_status fail(vt_status s, const std::string &msg);
+vt_status guarded_call(const std::function<vt_status()> &f);
+
+inline void check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() { reset(); }
+    void reset() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        reset();
+        if (n == 0) return;
+        check(hipMalloc(&ptr, n), "hipMalloc");
+        bytes = n;
+    }
+    void upload(const void *src, size_t n) {
+        alloc(n);
+        if (n) check(hipMemcpy(ptr, src, n, hipMemcpyHostToDevice), "hipMemcpy");
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(ptr); }
+};
+
+}  // namespace vt
+
+struct vt_context {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t resident = 0;  // bytes of live scenes
+};
+
+// One published mesh: the reference's MeshStreams + BLAS nodes (Sources/Acceleration.swift:59-64),
+// the document-order geometry the shading stage reads, and the material / emitter tables.
+struct vt_scene {
+    vt_context *ctx = nullptr;
+    vt::DeviceBuffer nodes, tris, counters;
+    vt::DeviceBuffer vertices, faces;             // document order
+    vt::DeviceBuffer materials, emitter_face, emitter_cdf, icdf, cmf, basis;
+    vt::TraceScene view{};
+    vt_scene_info info{};
+    uint32_t material_count = 0, emitter_count = 0;
+    float emitter_total = 0.0f;
+    bool shaded = false;  // materials set
+    std::vector<float> host_vertices;
+    std::vector<uint32_t> host_faces;
+    std::unique_ptr<vt::SpectralTables> spectral;
+    uint64_t accounted = 0;
+    ~vt_scene() {
+        if (ctx) ctx->resident -= accounted;
+    }
+    vt::RenderScene render_view() const {
+        vt::RenderScene r{};
+        r.trace = view;
+        r.vertices = vertices.as<const float>();
+        r.faces = faces.as<const uint32_t>();
+        r.materials = materials.as<const float>();
+        r.material_count = material_count;
+        r.emitter_face = emitter_face.as<const uint32_t>();
+        r.emitter_cdf = emitter_cdf.as<const float>();
+        r.emitter_count = emitter_count;
+        r.emitter_total = emitter_total;
+        r.icdf = icdf.as<const float>();
+        r.cmf = cmf.as<const float>();
+        r.basis = basis.as<const float>();
+        return r;
+    }
+};

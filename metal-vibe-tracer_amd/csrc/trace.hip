@@ -41,12 +41,13 @@ struct LdsStack {
 template <bool ANY, bool STATS, int kStack>
 __global__ __launch_bounds__(kBlock) void trace_kernel(const U4 *__restrict__ nodes, const F4 *__restrict__ tris,
                                                        RayArgs rays, HitArgs hits, uint8_t *__restrict__ occluded,
-                                                       uint32_t n, float root_max,
-                                                       unsigned long long *__restrict__ counters) {
+                                                       uint32_t n, const uint32_t *__restrict__ count,
+                                                       float root_max, unsigned long long *__restrict__ counters) {
     __shared__ uint32_t stack_base[kStack][kBlock];
     __shared__ uint32_t stack_bits[kStack][kBlock];
     const int lane = threadIdx.x;
     const uint32_t i = blockIdx.x * kBlock + lane;
+    if (count) n = min(n, *count);
     if (i >= n) return;
     TraceRay r{rays.ox[i], rays.oy[i], rays.oz[i], rays.dx[i], rays.dy[i], rays.dz[i], rays.tmin[i], rays.tmax[i]};
     LdsStack<kStack> st{stack_base, stack_bits, lane, 0};
@@ -68,8 +69,10 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(const U4 *__restrict__ no
 }
 
 template <bool ANY>
-__global__ __launch_bounds__(256) void miss_kernel(RayArgs rays, HitArgs hits, uint8_t *occluded, uint32_t n) {
+__global__ __launch_bounds__(256) void miss_kernel(RayArgs rays, HitArgs hits, uint8_t *occluded, uint32_t n,
+                                                   const uint32_t *count) {
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (count) n = min(n, *count);
     if (i >= n) return;
     if (ANY) {
         occluded[i] = 0;
@@ -85,12 +88,12 @@ __global__ __launch_bounds__(256) void miss_kernel(RayArgs rays, HitArgs hits, u
 }  // namespace
 
 hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitArgs &hits, uint8_t *occluded,
-                        uint32_t n, bool any, unsigned long long *counters, hipStream_t stream) {
+                        uint32_t n, bool any, unsigned long long *counters, hipStream_t stream, const uint32_t *count) {
     if (n == 0) return hipSuccess;
     if (scene.node_count == 0) {
         dim3 grid((n + 255) / 256);
-        if (any) hipLaunchKernelGGL(miss_kernel<true>, grid, dim3(256), 0, stream, rays, hits, occluded, n);
-        else hipLaunchKernelGGL(miss_kernel<false>, grid, dim3(256), 0, stream, rays, hits, occluded, n);
+        if (any) hipLaunchKernelGGL(miss_kernel<true>, grid, dim3(256), 0, stream, rays, hits, occluded, n, count);
+        else hipLaunchKernelGGL(miss_kernel<false>, grid, dim3(256), 0, stream, rays, hits, occluded, n, count);
         return hipGetLastError();
     }
     dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
@@ -101,11 +104,11 @@ hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitA
 #define VT_LAUNCH(S)                                                                                                   \
     do {                                                                                                               \
         if (counters) {                                                                                                \
-            if (any) hipLaunchKernelGGL((trace_kernel<true, true, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, counters); \
-            else hipLaunchKernelGGL((trace_kernel<false, true, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, counters); \
+            if (any) hipLaunchKernelGGL((trace_kernel<true, true, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, counters); \
+            else hipLaunchKernelGGL((trace_kernel<false, true, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, counters); \
         } else {                                                                                                       \
-            if (any) hipLaunchKernelGGL((trace_kernel<true, false, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, nullptr); \
-            else hipLaunchKernelGGL((trace_kernel<false, false, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, scene.root_max, nullptr); \
+            if (any) hipLaunchKernelGGL((trace_kernel<true, false, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, nullptr); \
+            else hipLaunchKernelGGL((trace_kernel<false, false, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, nullptr); \
         }                                                                                                              \
     } while (0)
     if (need <= 8) VT_LAUNCH(8);

@@ -23,7 +23,9 @@ struct TraceScene {
 
 // Closest hit (any = false) into `hits`, or any hit into `occluded`.  `counters` (device, 128
 // u64, zeroed by the caller) selects the counting variant: 64 pairs (node fetches, triangle tests).
+// `count` (device, optional) caps n at launch time: the wavefront integrator's queue length.
 hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitArgs &hits, uint8_t *occluded,
-                        uint32_t n, bool any, unsigned long long *counters, hipStream_t stream);
+                        uint32_t n, bool any, unsigned long long *counters, hipStream_t stream,
+                        const uint32_t *count = nullptr);
 
 }  // namespace vt

@@ -174,6 +174,65 @@ int vt_gen_heightfield(uint32_t grid, uint32_t octaves, float gain, float half_e
     return 0;
 }
 
+int vt_gen_cornell(float light_size, float *vertices, uint32_t *vertex_count, uint32_t *faces, uint32_t *face_count) {
+    struct P { float x, y, z; };
+    std::vector<P> tri;
+    std::vector<uint32_t> tags;
+    auto quad = [&](P a, P b, P c, P d, uint32_t tag) {  // a b c d around the quad
+        tri.insert(tri.end(), {a, b, c, a, c, d});
+        tags.insert(tags.end(), {tag, tag});
+    };
+    // Walls of the [-1, 1]^3 box, open towards -z (main.swift:1428-1432).
+    quad({-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}, 0);   // floor
+    quad({-1, 1, -1}, {-1, 1, 1}, {1, 1, 1}, {1, 1, -1}, 0);       // ceiling
+    quad({1, 1, 1}, {-1, 1, 1}, {-1, -1, 1}, {1, -1, 1}, 0);       // back
+    quad({-1, 1, 1}, {-1, 1, -1}, {-1, -1, -1}, {-1, -1, 1}, 1);   // left, red
+    quad({1, 1, -1}, {1, 1, 1}, {1, -1, 1}, {1, -1, -1}, 2);       // right, green
+    // Boxes rotated by yaw about y: world = centre + R(yaw) local (main.swift:1214-1219 inverted).
+    auto box = [&](P c, P h, float yaw) {
+        double cs = std::cos((double)yaw), sn = std::sin((double)yaw);
+        auto w = [&](float sx, float sy, float sz) {
+            double lx = sx * h.x, ly = sy * h.y, lz = sz * h.z;
+            return P{(float)(c.x + cs * lx - sn * lz), (float)(c.y + ly), (float)(c.z + sn * lx + cs * lz)};
+        };
+        quad(w(-1, -1, -1), w(1, -1, -1), w(1, 1, -1), w(-1, 1, -1), 0);
+        quad(w(1, -1, 1), w(-1, -1, 1), w(-1, 1, 1), w(1, 1, 1), 0);
+        quad(w(-1, -1, 1), w(-1, -1, -1), w(-1, 1, -1), w(-1, 1, 1), 0);
+        quad(w(1, -1, -1), w(1, -1, 1), w(1, 1, 1), w(1, 1, -1), 0);
+        quad(w(-1, 1, -1), w(1, 1, -1), w(1, 1, 1), w(-1, 1, 1), 0);
+        quad(w(-1, -1, 1), w(1, -1, 1), w(1, -1, -1), w(-1, -1, -1), 0);
+    };
+    box({-0.33f, -0.4f, 0.35f}, {0.3f, 0.6f, 0.3f}, 0.32f);
+    box({0.33f, -0.7f, -0.25f}, {0.3f, 0.3f, 0.3f}, -0.30f);
+    // Ceiling light: geometric normal cross(b - a, c - a) points down (-y), so it emits downwards.
+    const float h = 0.5f * light_size, y = 0.999f;
+    tri.insert(tri.end(), {P{-h, y, -h}, P{h, y, h}, P{-h, y, h}, P{-h, y, -h}, P{h, y, -h}, P{h, y, h}});
+    tags.insert(tags.end(), {3u, 3u});
+    const uint32_t nf = (uint32_t)tags.size();
+    if (vertex_count) *vertex_count = 3 * nf;
+    if (face_count) *face_count = nf;
+    if (!vertices || !faces) return 0;
+    for (uint32_t f = 0; f < nf; ++f) {
+        P a = tri[3 * f], b = tri[3 * f + 1], c = tri[3 * f + 2];
+        double e1[3] = {(double)b.x - a.x, (double)b.y - a.y, (double)b.z - a.z};
+        double e2[3] = {(double)c.x - a.x, (double)c.y - a.y, (double)c.z - a.z};
+        double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; ++k) {
+            P p = tri[3 * f + k];
+            float *o = vertices + 8 * (3 * (size_t)f + k);
+            o[0] = p.x; o[1] = p.y; o[2] = p.z;
+            o[3] = (float)(n[0] / l); o[4] = (float)(n[1] / l); o[5] = (float)(n[2] / l);
+            o[6] = 0.0f; o[7] = 0.0f;
+        }
+        faces[4 * f] = 3 * f;
+        faces[4 * f + 1] = 3 * f + 1;
+        faces[4 * f + 2] = 3 * f + 2;
+        faces[4 * f + 3] = tags[f];
+    }
+    return 0;
+}
+
 void vt_gen_camera_rays(const float eye[3], const float target[3], float fov_degrees, uint32_t width, uint32_t height,
                         uint32_t seed, float *ox, float *oy, float *oz, float *dx, float *dy, float *dz, float *tmin,
                         float *tmax) {

@@ -26,6 +26,8 @@ EXPORTED = [
     "vt_context_create", "vt_context_destroy", "vt_last_error", "vt_scene_upload_indexed", "vt_scene_destroy",
     "vt_scene_get_info", "vt_trace_closest", "vt_trace_any", "vt_trace_stats", "vt_bvh8_build_host", "vt_host_free",
     "vt_gen_icosphere_field", "vt_gen_heightfield", "vt_gen_camera_rays", "vt_gen_bounce_rays", "vt_gen_random_rays",
+    "vt_scene_set_materials", "vt_renderer_create", "vt_renderer_destroy", "vt_render_frame", "vt_reset_accumulation",
+    "vt_accum_readback", "vt_accum_frames", "vt_gen_cornell", "vt_spectral_tables", "vt_spectral_reflectance",
 ]
 
 
@@ -52,6 +54,25 @@ class _SceneInfo(ctypes.Structure):
 
 
 _fp = ctypes.POINTER(ctypes.c_float)
+
+
+class Material(ctypes.Structure):
+    """vt_material (64 bytes): MaterialType 0 DIFFUSE / 3 EMISSIVE, linear-sRGB albedo and emission."""
+    _fields_ = [("type", ctypes.c_uint32), ("albedo", ctypes.c_float * 3), ("emission", ctypes.c_float * 3),
+                ("roughness", ctypes.c_float), ("ior", ctypes.c_float), ("metalness", ctypes.c_float),
+                ("specular_weight", ctypes.c_float), ("transmission", ctypes.c_float), ("reserved", ctypes.c_float * 4)]
+
+
+def material(type=0, albedo=(0.7, 0.7, 0.7), emission=(0.0, 0.0, 0.0), roughness=0.5, ior=1.5, metalness=0.0,
+             specular_weight=1.0, transmission=0.0) -> Material:
+    return Material(type, (ctypes.c_float * 3)(*albedo), (ctypes.c_float * 3)(*emission), roughness, ior, metalness,
+                    specular_weight, transmission)
+
+
+class _FrameParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("eye", ctypes.c_float * 3),
+                ("target", ctypes.c_float * 3), ("fov_degrees", ctypes.c_float), ("max_depth", ctypes.c_uint32),
+                ("sampling_mode", ctypes.c_uint32), ("sample_index", ctypes.c_uint32)]
 
 
 class _RaySoA(ctypes.Structure):
@@ -93,6 +114,16 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_gen_camera_rays": (None, [P(f32), P(f32), f32, u32, u32, u32] + [vp] * 8),
         "vt_gen_bounce_rays": (u32, [vp, vp, u32, vp, vp, vp, vp, vp, vp, u32] + [vp] * 8),
         "vt_gen_random_rays": (None, [P(f32), P(f32), u32, u32] + [vp] * 8),
+        "vt_gen_cornell": (c.c_int, [f32, vp, P(u32), vp, P(u32)]),
+        "vt_scene_set_materials": (i32, [vp, vp, u32]),
+        "vt_renderer_create": (i32, [vp, P(vp)]),
+        "vt_renderer_destroy": (None, [vp]),
+        "vt_render_frame": (i32, [vp, vp, P(_FrameParams), u32, vp]),
+        "vt_reset_accumulation": (i32, [vp]),
+        "vt_accum_readback": (i32, [vp, vp, c.c_size_t]),
+        "vt_accum_frames": (u32, [vp]),
+        "vt_spectral_tables": (i32, [vp, vp, vp]),
+        "vt_spectral_reflectance": (i32, [P(f32), P(f32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -146,6 +177,46 @@ def heightfield(grid=2237, octaves=8, gain=0.5, half_extent=100.0, height=12.0, 
     lib.vt_gen_heightfield(grid, octaves, gain, half_extent, height, seed, _ptr(v), ctypes.byref(nv), _ptr(f),
                            ctypes.byref(nf))
     return Mesh(v, f)
+
+
+def cornell(light_size=0.25) -> Mesh:
+    """Config-1 Cornell box (tags: 0 white, 1 red, 2 green, 3 light)."""
+    lib = load()
+    nv, nf = ctypes.c_uint32(), ctypes.c_uint32()
+    lib.vt_gen_cornell(light_size, None, ctypes.byref(nv), None, ctypes.byref(nf))
+    v = np.empty((nv.value, 8), np.float32)
+    f = np.empty((nf.value, 4), np.uint32)
+    lib.vt_gen_cornell(light_size, _ptr(v), ctypes.byref(nv), _ptr(f), ctypes.byref(nf))
+    return Mesh(v, f)
+
+
+def cornell_materials(light_scale=17.0):
+    """White 0.73, red (0.65, 0.05, 0.05), green (0.12, 0.45, 0.15) (main.swift:1422-1424) and a D65 light of
+    RGB radiance light_scale."""
+    return [material(0, (0.73, 0.73, 0.73)), material(0, (0.65, 0.05, 0.05)), material(0, (0.12, 0.45, 0.15)),
+            material(3, (0.0, 0.0, 0.0), (light_scale, light_scale, light_scale))]
+
+
+def orbit_eye(yaw=0.45, pitch=0.25, distance=3.4, target=(0.0, -0.1, 0.0)):
+    """The reference's orbit camera position (main.swift:11303-11305), Cornell default view (:10755)."""
+    import math
+    return (target[0] + distance * math.cos(pitch) * math.sin(yaw), target[1] + distance * math.sin(pitch),
+            target[2] - distance * math.cos(pitch) * math.cos(yaw))
+
+
+def spectral_tables():
+    """(icdf (1025,), cmf (471, 4), basis (471, 2, 4)) float32, as the integrator reads them."""
+    icdf = np.empty(1025, np.float32)
+    cmf = np.empty((471, 4), np.float32)
+    basis = np.empty((471, 2, 4), np.float32)
+    _check(load().vt_spectral_tables(_ptr(icdf), _ptr(cmf), _ptr(basis)))
+    return icdf, cmf, basis
+
+
+def spectral_reflectance(rgb):
+    out = (ctypes.c_float * 4)()
+    _check(load().vt_spectral_reflectance((ctypes.c_float * 3)(*rgb), out))
+    return np.array(out[:], np.float32)
 
 
 RAY_FIELDS = ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")
@@ -318,8 +389,54 @@ class Scene:
                                    _stream_ptr(stream)))
         return out
 
+    def set_materials(self, materials):
+        arr = (Material * len(materials))(*materials)
+        _check(load().vt_scene_set_materials(self._h, arr, len(materials)))
+
     def trace_stats(self, rays, any_hit=False, stream=None):
         rs, n = _rays_struct(rays)
         c = (ctypes.c_uint64 * 2)()
         _check(load().vt_trace_stats(self._h, ctypes.byref(rs), n, 1 if any_hit else 0, c, _stream_ptr(stream)))
         return int(c[0]), int(c[1])
+
+
+class Renderer:
+    """vt_renderer: the wavefront path tracer and its running-mean accumulation."""
+
+    def __init__(self, ctx: Context):
+        h = ctypes.c_void_p()
+        _check(load().vt_renderer_create(ctx._h, ctypes.byref(h)))
+        self.ctx = ctx
+        self._h = h
+        self.size = (0, 0)
+
+    def close(self):
+        if self._h:
+            load().vt_renderer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, scene: Scene, width, height, eye, target, fov, max_depth=8, samples=1, sample_index=1,
+               sampling_mode=1, stream=None):
+        p = _FrameParams(width, height, (ctypes.c_float * 3)(*eye), (ctypes.c_float * 3)(*target), fov, max_depth,
+                         sampling_mode, sample_index)
+        _check(load().vt_render_frame(self._h, scene._h, ctypes.byref(p), samples, _stream_ptr(stream)))
+        self.size = (width, height)
+
+    def reset(self):
+        _check(load().vt_reset_accumulation(self._h))
+
+    @property
+    def frames(self) -> int:
+        return int(load().vt_accum_frames(self._h))
+
+    def readback(self) -> np.ndarray:
+        w, h = self.size
+        out = np.empty((h, w, 4), np.float32)
+        _check(load().vt_accum_readback(self._h, _ptr(out), out.nbytes))
+        return out

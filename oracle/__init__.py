@@ -67,6 +67,8 @@ def load():
         "vo_trace_closest_batch": (None, [P(SceneView)] + [vp] * 8 + [c.c_int64, c.c_int, c.c_int] + [vp] * 5),
         "vo_trace_any_batch": (None, [P(SceneView)] + [vp] * 8 + [c.c_int64, c.c_int, c.c_int, vp, vp]),
         "vo_set_fault": (None, [c.c_int]),
+        "vo_render_f32": (None, [vp, vp, u32, u32, vp, u32, c.c_int, vp]),
+        "vo_render_f64": (None, [vp, vp, u32, u32, vp, u32, c.c_int, vp]),
         "vo_max_threads": (c.c_int, []),
     }
     for name, (res, args) in sig.items():
@@ -187,3 +189,87 @@ class Scene:
             counters[0] += c[0]
             counters[1] += c[1]
         return occ
+
+
+# ------------------------------------------------------------------------------------------------
+# Integrator (integrator.c): the reference's MIS path loop, with tables from oracle/spectral.py
+# ------------------------------------------------------------------------------------------------
+
+class RenderScene(ctypes.Structure):
+    _fields_ = [("geo", ctypes.c_void_p), ("materials", ctypes.c_void_p), ("material_count", ctypes.c_uint32),
+                ("emitter_face", ctypes.c_void_p), ("emitter_cdf", ctypes.c_void_p),
+                ("emitter_count", ctypes.c_uint32), ("emitter_total", ctypes.c_float), ("icdf", ctypes.c_void_p),
+                ("cmf", ctypes.c_void_p), ("basis", ctypes.c_void_p)]
+
+
+class Frame(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("eye", ctypes.c_float * 3),
+                ("target", ctypes.c_float * 3), ("fov", ctypes.c_float), ("max_depth", ctypes.c_int32),
+                ("sampling_mode", ctypes.c_uint32)]
+
+
+_SPECTRAL = None
+
+
+def spectral_data():
+    global _SPECTRAL
+    if _SPECTRAL is None:
+        from oracle import spectral
+        d = spectral.Data()
+        _SPECTRAL = (d, d.tables())
+    return _SPECTRAL
+
+
+def material_records(materials):
+    """materials: list of dicts with type, albedo, emission -> (records (n, 16) float32, weights)."""
+    d, _ = spectral_data()
+    rec = np.zeros((len(materials), 16), np.float32)
+    weights = np.zeros(len(materials))
+    for k, m in enumerate(materials):
+        rec[k, 0] = np.frombuffer(np.uint32(m["type"]).tobytes(), np.float32)[0]
+        alb = np.clip(np.asarray(m["albedo"], np.float32), 0, 1)
+        rec[k, 1:4] = alb
+        rec[k, 4:8] = d.reflectance_lagrange(alb)
+        e = np.asarray(m.get("emission", (0, 0, 0)), np.float32)
+        rec[k, 8:11] = e
+        if (e > 0).any():
+            weights[k] = max(1e-8, 0.2126 * float(e[0]) + 0.7152 * float(e[1]) + 0.0722 * float(e[2]))
+        rec[k, 11] = np.float32(weights[k])
+    return rec, weights
+
+
+def emitter_list(vertices, faces, weights):
+    """main.swift:9312-9346: emitting faces in document order, power = area (double) x weight."""
+    slot = faces[:, 3].astype(np.int64)
+    ok = (slot < len(weights))
+    ok[ok] = weights[slot[ok]] > 0
+    ids = np.nonzero(ok)[0].astype(np.uint32)
+    a, b, c = (vertices[faces[ids, k], :3].astype(np.float32) for k in range(3))
+    e1 = (b - a).astype(np.float64)
+    e2 = (c - a).astype(np.float64)
+    area = 0.5 * np.linalg.norm(np.cross(e1, e2), axis=1)
+    power = area * weights[slot[ids]].astype(np.float32).astype(np.float64)
+    total = power.sum()
+    cdf = (np.cumsum(power) / total).astype(np.float32) if total > 0 else np.zeros(0, np.float32)
+    if len(cdf):
+        cdf[-1] = 1.0
+    return ids, cdf, float(total)
+
+
+def render(scene: "Scene", materials, frame: dict, sample_index0: int, samples: int, pixels=None, precision="f32",
+           threads=0):
+    """Mean radiance per pixel (pixels: flat indices, default all) -> (len(pixels), 3) float64."""
+    lib = load()
+    _, (icdf, cmf, basis) = spectral_data()
+    rec, weights = material_records(materials)
+    ids, cdf, total = emitter_list(scene.vertices, scene.faces, weights)
+    w, h = frame["width"], frame["height"]
+    pix = np.arange(w * h, dtype=np.uint32) if pixels is None else np.ascontiguousarray(pixels, np.uint32)
+    rs = RenderScene(ctypes.cast(ctypes.pointer(scene._view), ctypes.c_void_p), _p(rec), len(rec), _p(ids), _p(cdf),
+                     len(ids), total, _p(icdf), _p(cmf), _p(basis))
+    fr = Frame(w, h, (ctypes.c_float * 3)(*frame["eye"]), (ctypes.c_float * 3)(*frame["target"]), frame["fov"],
+               frame["max_depth"], frame.get("sampling_mode", 1))
+    out = np.zeros((len(pix), 3), np.float64)
+    fn = lib.vo_render_f64 if precision == "f64" else lib.vo_render_f32
+    fn(ctypes.byref(rs), ctypes.byref(fr), sample_index0, samples, _p(pix), len(pix), threads, _p(out))
+    return out

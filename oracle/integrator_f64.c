@@ -1,0 +1,12 @@
+/* integrator_f64.c -- oracle (test infrastructure only): the FP64 build of integrator.c. */
+#define REAL double
+#define SUFFIX(name) name##_f64
+#define RSQRT sqrt
+#define RATAN atan
+#define RCOS cos
+#define RSIN sin
+#define RTAN tan
+#define RFABS fabs
+#define RFMAX fmax
+#define RFMIN fmin
+#include "integrator.c"

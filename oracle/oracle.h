@@ -101,6 +101,39 @@ void vo_trace_any_batch(const vo_scene *s, const float *ox, const float *oy, con
                         const float *tmax, int64_t n, int mode, int threads, uint8_t *occluded,
                         uint64_t *counters);
 
+/* ------------------------------------------------------------------ integrator (integrator.c) */
+
+/* Scene as the integrators read it; material records are 16 floats per slot: type (u32 bits),
+ * albedo rgb | Lagrange multipliers (L0, L1, L2, w: 1 MESE, 2 flat grey) | emission rgb, selection
+ * weight | roughness, ior, metalness, specular weight. */
+typedef struct vo_render_scene {
+    const vo_scene *geo;
+    const float *materials;
+    uint32_t material_count;
+    const uint32_t *emitter_face;   /* emitting faces, power CDF order */
+    const float *emitter_cdf;       /* cumulative area x weight / total, last = 1 */
+    uint32_t emitter_count;
+    float emitter_total;
+    const float *icdf;              /* 1025 */
+    const float *cmf;               /* 471 x 4: rgb CMF, cos phase */
+    const float *basis;             /* 471 x 2 x 4: R G B D65 | C M Y 0 */
+} vo_render_scene;
+
+typedef struct vo_frame {
+    uint32_t width, height;
+    float eye[3], target[3], fov;
+    int32_t max_depth;
+    uint32_t sampling_mode;
+} vo_frame;
+
+/* Mean linear-sRGB radiance of `samples` samples (sample indices sample_index0...) for each listed
+ * pixel (index y * width + x): the reference's MIS path loop (see integrator.c), float arithmetic
+ * (_f32: the reference integrator) or double (_f64: the FP64 oracle). */
+void vo_render_f32(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
+                   const uint32_t *pixels, uint32_t npixels, int threads, double *out_rgb);
+void vo_render_f64(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
+                   const uint32_t *pixels, uint32_t npixels, int threads, double *out_rgb);
+
 /* For tests that plant a mistake in the oracle (never the product): 0 = faithful,
  * 1 = drop the FMA tie-break of watertight_edge, 2 = swap b1/b2. */
 void vo_set_fault(int fault);
