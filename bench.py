@@ -59,6 +59,67 @@ def workload_heightfield():
     return "fbm-heightfield-2237^2", mesh, prim, 3
 
 
+CORNELL_MATERIALS = [dict(type=0, albedo=(0.73, 0.73, 0.73)), dict(type=0, albedo=(0.65, 0.05, 0.05)),
+                     dict(type=0, albedo=(0.12, 0.45, 0.15)), dict(type=3, albedo=(0, 0, 0), emission=(17, 17, 17))]
+
+
+def relmse(x, ref):
+    """Mean over pixels and channels of (x - ref)^2 / (ref^2 + 0.01)."""
+    import numpy as np
+    return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def bench_integrator(args, ctx):
+    """Config 1: Cornell box (36 triangles), 512 x 512, 64 spp, depth 8, MIS, 4 wavelengths, PCG seed 1.
+
+    GPU: time of the 64-sample render (wavefronts of 16 samples).  Error: relMSE of the 64-spp image
+    against the FP64 oracle at `--reference-spp` on a fixed 16 x 16 stratified pixel subset.  CPU
+    baseline: the reference integrator restated in C (oracle, float) on 4096 pixels x 64 spp."""
+    import numpy as np
+    import torch
+
+    import vt
+    w = h = 512
+    spp = 64
+    mesh = vt.cornell()
+    scene = ctx.set_mesh(mesh)
+    scene.set_materials(vt.cornell_materials(17.0))
+    r = vt.Renderer(ctx)
+    eye, target = vt.orbit_eye(), (0.0, -0.1, 0.0)
+    r.render(scene, w, h, eye, target, 42.0, max_depth=8, samples=16, sample_index=1)  # warm-up
+    torch.cuda.synchronize()
+    r.reset()
+    t0 = time.perf_counter()
+    r.render(scene, w, h, eye, target, 42.0, max_depth=8, samples=spp, sample_index=1)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    img = r.readback()[..., :3].astype(np.float64)
+    ys, xs = np.meshgrid((np.arange(16) * 32 + 16), (np.arange(16) * 32 + 16), indexing="ij")
+    pix = (ys * w + xs).ravel().astype(np.uint32)
+    out = {"workload": "cornell-36tri-512x512-64spp-depth8", "gpu_seconds": round(gpu_s, 4),
+           "gpu_msamples_per_s": round(w * h * spp / gpu_s / 1e6, 2), "mean_radiance": float(img.mean())}
+    if not args.no_cpu:
+        import oracle
+        sc = oracle.Scene(mesh.vertices, mesh.faces)
+        frame = dict(width=w, height=h, eye=eye, target=target, fov=42.0, max_depth=8)
+        c0 = time.perf_counter()
+        ref = oracle.render(sc, CORNELL_MATERIALS, frame, 1, args.reference_spp, pixels=pix, precision="f64")
+        ref_s = time.perf_counter() - c0
+        sub = img.reshape(-1, 3)[pix]
+        out["relmse_vs_fp64"] = relmse(sub, ref)
+        out["fp64_reference"] = {"spp": args.reference_spp, "pixels": int(len(pix)), "seconds": round(ref_s, 2)}
+        cpix = np.arange(0, w * h, 64, dtype=np.uint32)
+        c0 = time.perf_counter()
+        oracle.render(sc, CORNELL_MATERIALS, frame, 1, spp, pixels=cpix, precision="f32")
+        cs = time.perf_counter() - c0
+        out["cpu_baseline"] = {"value": round(len(cpix) * spp / cs / 1e6, 3), "unit": "Msamples/s",
+                               "cores": oracle.max_threads(), "kind": "port",
+                               "sample": f"every 64th pixel ({len(cpix)} pixels) x {spp} spp, reference integrator in C"}
+    r.close()
+    scene.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +128,8 @@ def main():
     ap.add_argument("--workload", choices=["stress", "heightfield"], default="stress")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="rays in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-integrator", action="store_true", help="skip the path-tracer (Cornell) measurement")
+    ap.add_argument("--reference-spp", type=int, default=65536, help="FP64 oracle samples per pixel for relMSE")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device 3 times with a plain copy kernel (the "
                          "reference launch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -161,6 +224,8 @@ def main():
                "sample": f"every {stride}th primary and bounce ray ({sample.shape[1]} rays), reference 4-wide "
                          f"binned-SAH BVH (leaf 2) and traversal restated in C, median of 3"}
 
+    integrator = None if args.no_integrator else bench_integrator(args, ctx)
+
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -184,6 +249,7 @@ def main():
                      "algorithmic_bytes_per_launch": (bytes_p + bytes_b) // 2,
                      "work": {"node_fetches": np_ + nb_, "triangle_tests": tp_ + tb_, "bytes_per_step": bytes_p + bytes_b}},
         "cpu_baseline": cpu,
+        "integrator": integrator,
     }
     print(json.dumps(line), flush=True)
     if args.copy_calibration:

@@ -123,6 +123,18 @@ typedef struct vt_material {
  * builds the spectral tables (D65 emitters) and the material reflectance spectra. */
 vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, uint32_t count);
 
+/* Emitter selection for next-event estimation (MaterialLibrary.lightTree, main.swift:9006): 1 (default)
+ * builds a light BVH over the emitting faces (Sources/LightTree.swift) when there are at least two and
+ * the tree has more than one node; 0 keeps power sampling.  Takes effect at vt_scene_set_materials. */
+vt_status vt_scene_set_light_tree(vt_scene *scene, int enabled);
+
+/* Host build of the light BVH over `count` emitter triangles (9 floats each: a, b, c) with powers
+ * (area x weight): *words (8 per node), *order (leaf position -> emitter), *leaf_node (leaf position ->
+ * node) are malloc'ed (free with vt_host_free).  Mirrors LightTree.build (Sources/LightTree.swift:112). */
+vt_status vt_light_tree_build_host(const float *tris, const double *power, uint32_t count, uint32_t **words,
+                                   uint32_t *node_count, uint32_t **order, uint32_t **leaf_node, uint32_t *depth,
+                                   float grid_lo[3], float grid_hi[3]);
+
 /* Per-call frame parameters: the subset of Uniforms (main.swift:393-420) this path reads. */
 typedef struct vt_frame_params {
     uint32_t width, height;

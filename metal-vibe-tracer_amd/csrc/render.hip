@@ -230,6 +230,120 @@ __device__ __forceinline__ float imported_geometry(V3 p, V3 position, V3 a, V3 b
     V3 n = cross(b - a, c - a);
     return d2 > 1e-12f && dot(n, n) > 1e-20f ? fmaxf(0.0f, dot(normalize(n), -d * (1.0f / sqrtf(d2)))) / d2 : 0.0f;
 }
+// ---------------------------------------------------------------- light BVH walk (main.swift:2743-2844)
+struct LtNode {
+    V3 lo, hi, w;
+    float phi, cos_o, cos_e;
+    uint32_t child, parent, count;
+};
+constexpr uint32_t kNoParent = 0x7ffffffu;
+__device__ __forceinline__ LtNode lt_node(const RenderScene &sc, uint32_t i) {
+    const uint4 a = reinterpret_cast<const uint4 *>(sc.tree_nodes)[2 * i];
+    const uint4 b = reinterpret_cast<const uint4 *>(sc.tree_nodes)[2 * i + 1];
+    LtNode n;
+    float fx = (float)(a.x & 0xffffu) / 65535.0f * 2.0f - 1.0f, fy = (float)(a.x >> 16) / 65535.0f * 2.0f - 1.0f;
+    V3 v = v3(fx, fy, 1.0f - fabsf(fx) - fabsf(fy));
+    if (v.z < 0.0f) {
+        float nx = (1.0f - fabsf(fy)) * (fx >= 0.0f ? 1.0f : -1.0f), ny = (1.0f - fabsf(fx)) * (fy >= 0.0f ? 1.0f : -1.0f);
+        v.x = nx;
+        v.y = ny;
+    }
+    n.w = v * (1.0f / sqrtf(dot(v, v)));
+    n.phi = __uint_as_float(a.y);
+    n.cos_o = (float)(a.z & 0xffffu) / 65535.0f * 2.0f - 1.0f;
+    n.cos_e = (float)(a.z >> 16) / 65535.0f * 2.0f - 1.0f;
+    const V3 lo = v3(sc.tree_lo[0], sc.tree_lo[1], sc.tree_lo[2]);
+    n.lo = v3(lo.x + (float)(a.w & 0xffffu) * sc.tree_scale[0], lo.y + (float)(a.w >> 16) * sc.tree_scale[1],
+              lo.z + (float)(b.x & 0xffffu) * sc.tree_scale[2]);
+    n.hi = v3(lo.x + (float)(b.x >> 16) * sc.tree_scale[0], lo.y + (float)(b.y & 0xffffu) * sc.tree_scale[1],
+              lo.z + (float)(b.y >> 16) * sc.tree_scale[2]);
+    n.child = b.z;
+    n.parent = b.w & kNoParent;
+    n.count = b.w >> 27;
+    return n;
+}
+__device__ __forceinline__ float lt_power(const RenderScene &sc, uint32_t position) {
+    return __uint_as_float(sc.tree_leaves[3 * (size_t)position + 1]);
+}
+__device__ __forceinline__ float lt_leaf_sum(const RenderScene &sc, const LtNode &n) {
+    float s = 0.0f;
+    for (uint32_t i = 0; i < n.count; ++i) s += lt_power(sc, n.child + i);
+    return s;
+}
+__device__ __forceinline__ float cos_sub(float sa, float ca, float sb, float cb) { return ca > cb ? 1.0f : ca * cb + sa * sb; }
+__device__ __forceinline__ float sin_sub(float sa, float ca, float sb, float cb) { return ca > cb ? 0.0f : sa * cb - ca * sb; }
+__device__ __forceinline__ float lt_importance(const LtNode &n, V3 p) {
+    if (!(n.phi > 0.0f)) return 0.0f;
+    V3 pc = (n.lo + n.hi) * 0.5f;
+    V3 dpc = p - pc;
+    float d2 = fmaxf(dot(dpc, dpc), fmaxf(0.5f * length(n.hi - n.lo), 1e-20f));
+    if (p.x >= n.lo.x && p.y >= n.lo.y && p.z >= n.lo.z && p.x <= n.hi.x && p.y <= n.hi.y && p.z <= n.hi.z)
+        return n.phi / d2;
+    V3 wi = dpc * (1.0f / sqrtf(dot(dpc, dpc)));
+    float cw = dot(n.w, wi), sw = sqrtf(fmaxf(0.0f, 1.0f - cw * cw));
+    V3 dh = pc - n.hi;
+    float s2 = dot(dh, dh) / dot(dpc, dpc);
+    float cb = s2 >= 1.0f ? -1.0f : sqrtf(fmaxf(0.0f, 1.0f - s2)), sb = sqrtf(fmaxf(0.0f, 1.0f - cb * cb));
+    float so = sqrtf(fmaxf(0.0f, 1.0f - n.cos_o * n.cos_o));
+    float cx = cos_sub(sw, cw, so, n.cos_o), sx = sin_sub(sw, cw, so, n.cos_o);
+    float cp = cos_sub(sx, cx, sb, cb);
+    return cp <= n.cos_e ? 0.0f : n.phi * cp / d2;
+}
+__device__ __forceinline__ uint32_t lt_sample(const RenderScene &sc, V3 p, float u, float &pmf) {
+    pmf = 1.0f;
+    uint32_t index = 0;
+    for (uint32_t depth = 0; depth <= 64u; ++depth) {
+        LtNode n = lt_node(sc, index);
+        if (n.count != 0u) {
+            float sum = lt_leaf_sum(sc, n), target = u * sum, cum = 0.0f, chosen = 0.0f;
+            uint32_t position = 0xffffffffu;
+            for (uint32_t i = 0; i < n.count && sum > 0.0f; ++i) {
+                float power = lt_power(sc, n.child + i);
+                if (!(power > 0.0f)) continue;
+                position = n.child + i;
+                chosen = power;
+                cum += power;
+                if (target < cum) break;
+            }
+            if (position == 0xffffffffu) break;
+            pmf *= chosen / sum;
+            return position;
+        }
+        float l = lt_importance(lt_node(sc, index + 1u), p), r = lt_importance(lt_node(sc, n.child), p);
+        if (!(l + r > 0.0f)) break;
+        float pl = l / (l + r);
+        if (u < pl) {
+            u = fminf(u / pl, 0x1.fffffep-1f);
+            index = index + 1u;
+            pmf *= pl;
+        } else {
+            u = fminf((u - pl) / (1.0f - pl), 0x1.fffffep-1f);
+            index = n.child;
+            pmf *= r / (l + r);
+        }
+    }
+    pmf = 0.0f;
+    return 0xffffffffu;
+}
+__device__ __forceinline__ float lt_pmf(const RenderScene &sc, V3 p, uint32_t position) {
+    uint32_t index = sc.tree_leaves[3 * (size_t)position + 2];
+    LtNode node = lt_node(sc, index);
+    float sum = lt_leaf_sum(sc, node);
+    float pmf = sum > 0.0f ? lt_power(sc, position) / sum : 0.0f;
+    for (uint32_t depth = 0; depth <= 64u && pmf > 0.0f; ++depth) {
+        uint32_t parent = node.parent;
+        if (parent == kNoParent) return pmf;
+        LtNode up = lt_node(sc, parent);
+        float l = lt_importance(lt_node(sc, parent + 1u), p), r = lt_importance(lt_node(sc, up.child), p);
+        float mine = index == parent + 1u ? l : r;
+        if (!(l + r > 0.0f) || !(mine > 0.0f)) return 0.0f;
+        pmf *= mine / (l + r);
+        index = parent;
+        node = up;
+    }
+    return 0.0f;
+}
+
 struct LightSample {
     V3 position, wi, emission;
     float dist, pdf;
@@ -247,13 +361,21 @@ __device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, V3 
     // importedProbability = 1 (no environment): the draw is still consumed.
     if (!(rand_f(seed) < 1.0f)) return ls;
     float value = rand_f(seed), ry = rand_f(seed), rz = rand_f(seed);
-    uint32_t low = 0, high = sc.emitter_count - 1;
-    while (low < high) {
-        uint32_t mid = (low + high) / 2;
-        if (sc.emitter_cdf[mid] >= value) high = mid;
-        else low = mid + 1;
+    uint32_t face;
+    float tree_pmf = 0.0f;
+    if (sc.tree_node_count) {
+        uint32_t position = lt_sample(sc, p, value, tree_pmf);
+        if (position == 0xffffffffu) return ls;
+        face = sc.tree_leaves[3 * (size_t)position];
+    } else {
+        uint32_t low = 0, high = sc.emitter_count - 1;
+        while (low < high) {
+            uint32_t mid = (low + high) / 2;
+            if (sc.emitter_cdf[mid] >= value) high = mid;
+            else low = mid + 1;
+        }
+        face = sc.emitter_face[low];
     }
-    uint32_t face = sc.emitter_face[low];
     V3 a, b, c;
     uint32_t tag;
     const float *va, *vb, *vc;
@@ -267,18 +389,38 @@ __device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, V3 
     Mat m = material(sc, tag);
     ls.emission = m.emission;
     ls.slot = tag;
-    float area_pdf = sc.emitter_total > 0.0f ? m.weight / sc.emitter_total : 0.0f;
+    float area_pdf;
+    if (sc.tree_node_count) {
+        float area = 0.5f * length(cross(b - a, c - a));
+        area_pdf = area > 0.0f ? tree_pmf / area : 0.0f;
+    } else {
+        area_pdf = sc.emitter_total > 0.0f ? m.weight / sc.emitter_total : 0.0f;
+    }
     ls.pdf = geometry > 1e-12f ? area_pdf / geometry : 0.0f;
     return ls;
 }
 __device__ __forceinline__ float emitter_pdf(const RenderScene &sc, V3 p, V3 position, uint32_t face, float weight) {
-    if (sc.emitter_count == 0 || !(sc.emitter_total > 0.0f) || !(weight > 0.0f)) return 0.0f;
+    if (sc.emitter_count == 0) return 0.0f;
     V3 a, b, c;
     uint32_t tag;
     const float *va, *vb, *vc;
     face_corners(sc, face, a, b, c, tag, &va, &vb, &vc);
     float g = imported_geometry(p, position, a, b, c);
-    return g > 1e-12f ? (weight / sc.emitter_total) / g : 0.0f;
+    if (!(g > 1e-12f)) return 0.0f;
+    if (sc.tree_node_count) {
+        // imported_emitter_position: binary search of the ascending emitter faces (main.swift:2846-2850).
+        uint32_t lo = 0, hi = sc.emitter_count;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) / 2;
+            if (sc.tree_ids[mid] < face) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo >= sc.emitter_count || sc.tree_ids[lo] != face) return 0.0f;
+        float area = 0.5f * length(cross(b - a, c - a));
+        return area > 0.0f ? lt_pmf(sc, p, sc.tree_pos[lo]) / area / g : 0.0f;
+    }
+    if (!(sc.emitter_total > 0.0f) || !(weight > 0.0f)) return 0.0f;
+    return (weight / sc.emitter_total) / g;
 }
 
 // ---------------------------------------------------------------- wave-aggregated queue append

@@ -32,6 +32,13 @@ struct RenderScene {
     const float *icdf;             // 1025 wavelength inverse-CDF nodes
     const float *cmf;              // 471 x (r, g, b, cos phase)
     const float *basis;            // 471 x 2 x 4 emission basis (w of the first: D65)
+    // Light BVH (tree_node_count 0: power sampling through emitter_cdf).
+    const uint32_t *tree_nodes;    // 8 words per node (Sources/LightTree.swift:192-205)
+    const uint32_t *tree_leaves;   // 3 words per leaf position: face, power bits, leaf node
+    const uint32_t *tree_ids;      // emitting faces, ascending (emitter_count entries)
+    const uint32_t *tree_pos;      // leaf position of tree_ids[i]
+    uint32_t tree_node_count;
+    float tree_lo[3], tree_scale[3];
 };
 
 struct FrameArgs {

@@ -3,10 +3,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <vector>
 
+#include "lighttree.h"
 #include "render_kernels.h"
 #include "scene.h"
 #include "spectral.h"
@@ -133,6 +135,45 @@ vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, 
             ids.push_back((uint32_t)f);
             power.push_back(0.5 * std::sqrt(cx * cx + cy * cy + cz * cz) * (double)(float)weight[slot]);
         }
+        // Light BVH over the same list (MaterialLibrary.lightTreeList, main.swift:9374-9401): kept when
+        // there are two or more emitters, more than one node, and a walkable depth.
+        scene->tree_node_count = 0;
+        if (scene->light_tree && ids.size() > 1) {
+            std::vector<vt::LightEmitter> em(ids.size());
+            for (size_t k = 0; k < ids.size(); ++k) {
+                const uint32_t *fc = &scene->host_faces[4 * (size_t)ids[k]];
+                for (int q = 0; q < 3; ++q) {
+                    em[k].a[q] = scene->host_vertices[8 * (size_t)fc[0] + q];
+                    em[k].b[q] = scene->host_vertices[8 * (size_t)fc[1] + q];
+                    em[k].c[q] = scene->host_vertices[8 * (size_t)fc[2] + q];
+                }
+                em[k].power = power[k];
+                em[k].cos_emission = 0.0;
+            }
+            vt::LightTreeBuilt tree = vt::build_light_tree(em);
+            if (tree.node_count > 1 && tree.depth <= vt::kLightTreeMaxDepth && tree.node_count < vt::kLightTreeMaxNodes) {
+                std::vector<uint32_t> leaves(3 * ids.size()), pos(ids.size());
+                for (size_t leaf = 0; leaf < tree.order.size(); ++leaf) {
+                    uint32_t input = tree.order[leaf];
+                    pos[input] = (uint32_t)leaf;
+                    float pw = (float)power[input];
+                    uint32_t bits;
+                    std::memcpy(&bits, &pw, 4);
+                    leaves[3 * leaf] = ids[input];
+                    leaves[3 * leaf + 1] = bits;
+                    leaves[3 * leaf + 2] = tree.leaf_node[leaf];
+                }
+                scene->tree_nodes.upload(tree.words.data(), tree.words.size() * 4);
+                scene->tree_leaves.upload(leaves.data(), leaves.size() * 4);
+                scene->tree_ids.upload(ids.data(), ids.size() * 4);
+                scene->tree_pos.upload(pos.data(), pos.size() * 4);
+                scene->tree_node_count = tree.node_count;
+                for (int k = 0; k < 3; ++k) {
+                    scene->tree_lo[k] = tree.grid_lo[k];
+                    scene->tree_scale[k] = (tree.grid_hi[k] - tree.grid_lo[k]) / 65535.0f;
+                }
+            }
+        }
         double total = 0;
         std::vector<float> cdf(ids.size());
         for (double p : power) total += p;
@@ -176,6 +217,46 @@ vt_status vt_spectral_reflectance(const float rgb[3], float out[4]) {
     return vt::guarded_call([&]() -> vt_status {
         if (!rgb || !out) return fail(VT_ERROR_INVALID_ARGUMENT, "rgb and out must not be NULL");
         vt::reflectance_lagrange(host_tables(), rgb, out);
+        return VT_OK;
+    });
+}
+
+vt_status vt_scene_set_light_tree(vt_scene *scene, int enabled) {
+    if (!scene) return fail(VT_ERROR_INVALID_ARGUMENT, "scene must not be NULL");
+    scene->light_tree = enabled != 0;
+    return VT_OK;
+}
+
+vt_status vt_light_tree_build_host(const float *tris, const double *power, uint32_t count, uint32_t **words,
+                                   uint32_t *node_count, uint32_t **order, uint32_t **leaf_node, uint32_t *depth,
+                                   float grid_lo[3], float grid_hi[3]) {
+    return vt::guarded_call([&]() -> vt_status {
+        if ((count && (!tris || !power)) || !words || !node_count || !order || !leaf_node || !depth || !grid_lo || !grid_hi)
+            return fail(VT_ERROR_INVALID_ARGUMENT, "arguments must not be NULL");
+        std::vector<vt::LightEmitter> em(count);
+        for (uint32_t k = 0; k < count; ++k) {
+            std::memcpy(em[k].a, tris + 9 * (size_t)k, 12);
+            std::memcpy(em[k].b, tris + 9 * (size_t)k + 3, 12);
+            std::memcpy(em[k].c, tris + 9 * (size_t)k + 6, 12);
+            em[k].power = power[k];
+            em[k].cos_emission = 0.0;
+        }
+        vt::LightTreeBuilt t = vt::build_light_tree(em);
+        auto copy = [](const std::vector<uint32_t> &v) {
+            uint32_t *p = (uint32_t *)std::malloc(std::max<size_t>(1, v.size()) * 4);
+            if (!p) throw std::bad_alloc();
+            if (!v.empty()) std::memcpy(p, v.data(), v.size() * 4);
+            return p;
+        };
+        *words = copy(t.words);
+        *order = copy(t.order);
+        *leaf_node = copy(t.leaf_node);
+        *node_count = t.node_count;
+        *depth = t.depth;
+        for (int k = 0; k < 3; ++k) {
+            grid_lo[k] = t.grid_lo[k];
+            grid_hi[k] = t.grid_hi[k];
+        }
         return VT_OK;
     });
 }

@@ -72,6 +72,10 @@ struct vt_scene {
     vt::DeviceBuffer nodes, tris, counters;
     vt::DeviceBuffer vertices, faces;             // document order
     vt::DeviceBuffer materials, emitter_face, emitter_cdf, icdf, cmf, basis;
+    vt::DeviceBuffer tree_nodes, tree_leaves, tree_ids, tree_pos;
+    uint32_t tree_node_count = 0;
+    float tree_lo[3] = {0, 0, 0}, tree_scale[3] = {0, 0, 0};
+    bool light_tree = true;
     vt::TraceScene view{};
     vt_scene_info info{};
     uint32_t material_count = 0, emitter_count = 0;
@@ -98,6 +102,15 @@ struct vt_scene {
         r.icdf = icdf.as<const float>();
         r.cmf = cmf.as<const float>();
         r.basis = basis.as<const float>();
+        r.tree_nodes = tree_nodes.as<const uint32_t>();
+        r.tree_leaves = tree_leaves.as<const uint32_t>();
+        r.tree_ids = tree_ids.as<const uint32_t>();
+        r.tree_pos = tree_pos.as<const uint32_t>();
+        r.tree_node_count = tree_node_count;
+        for (int k = 0; k < 3; ++k) {
+            r.tree_lo[k] = tree_lo[k];
+            r.tree_scale[k] = tree_scale[k];
+        }
         return r;
     }
 };

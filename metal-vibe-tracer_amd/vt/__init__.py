@@ -28,6 +28,7 @@ EXPORTED = [
     "vt_gen_icosphere_field", "vt_gen_heightfield", "vt_gen_camera_rays", "vt_gen_bounce_rays", "vt_gen_random_rays",
     "vt_scene_set_materials", "vt_renderer_create", "vt_renderer_destroy", "vt_render_frame", "vt_reset_accumulation",
     "vt_accum_readback", "vt_accum_frames", "vt_gen_cornell", "vt_spectral_tables", "vt_spectral_reflectance",
+    "vt_scene_set_light_tree", "vt_light_tree_build_host",
 ]
 
 
@@ -124,6 +125,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_accum_frames": (u32, [vp]),
         "vt_spectral_tables": (i32, [vp, vp, vp]),
         "vt_spectral_reflectance": (i32, [P(f32), P(f32)]),
+        "vt_scene_set_light_tree": (i32, [vp, c.c_int]),
+        "vt_light_tree_build_host": (i32, [vp, vp, u32, P(vp), P(u32), P(vp), P(vp), P(u32), P(f32), P(f32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -202,6 +205,28 @@ def orbit_eye(yaw=0.45, pitch=0.25, distance=3.4, target=(0.0, -0.1, 0.0)):
     import math
     return (target[0] + distance * math.cos(pitch) * math.sin(yaw), target[1] + distance * math.sin(pitch),
             target[2] - distance * math.cos(pitch) * math.cos(yaw))
+
+
+def light_tree_build_host(tris: np.ndarray, power: np.ndarray):
+    """(n, 9) float32 emitter corners, (n,) float64 powers -> dict(words (m, 8), order, leaf_node, depth, grid)."""
+    lib = load()
+    t = np.ascontiguousarray(tris, np.float32)
+    pw = np.ascontiguousarray(power, np.float64)
+    n = len(t)
+    words, order, leaf = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    count, depth = ctypes.c_uint32(), ctypes.c_uint32()
+    lo, hi = (ctypes.c_float * 3)(), (ctypes.c_float * 3)()
+    _check(lib.vt_light_tree_build_host(_ptr(t), _ptr(pw), n, ctypes.byref(words), ctypes.byref(count),
+                                        ctypes.byref(order), ctypes.byref(leaf), ctypes.byref(depth), lo, hi))
+    try:
+        w = np.frombuffer(ctypes.string_at(words, 32 * count.value), np.uint32).reshape(-1, 8).copy()
+        o = np.frombuffer(ctypes.string_at(order, 4 * n), np.uint32).copy() if n else np.zeros(0, np.uint32)
+        ln = np.frombuffer(ctypes.string_at(leaf, 4 * n), np.uint32).copy() if n else np.zeros(0, np.uint32)
+    finally:
+        for ptr in (words, order, leaf):
+            lib.vt_host_free(ptr)
+    return {"words": w, "order": o, "leaf_node": ln, "depth": depth.value, "grid_lo": np.array(lo[:], np.float32),
+            "grid_hi": np.array(hi[:], np.float32)}
 
 
 def spectral_tables():
@@ -388,6 +413,9 @@ class Scene:
         _check(load().vt_trace_any(self._h, ctypes.byref(rs), n, ctypes.c_void_p(out.data_ptr()),
                                    _stream_ptr(stream)))
         return out
+
+    def set_light_tree(self, enabled: bool):
+        _check(load().vt_scene_set_light_tree(self._h, 1 if enabled else 0))
 
     def set_materials(self, materials):
         arr = (Material * len(materials))(*materials)

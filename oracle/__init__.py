@@ -70,6 +70,11 @@ def load():
         "vo_render_f32": (None, [vp, vp, u32, u32, vp, u32, c.c_int, vp]),
         "vo_render_f64": (None, [vp, vp, u32, u32, vp, u32, c.c_int, vp]),
         "vo_max_threads": (c.c_int, []),
+        "vo_light_tree_build": (i32, [vp, vp, u32, c.c_int, c.c_double, vp]),
+        "vo_light_tree_free": (None, [vp]),
+        "vo_light_tree_importance": (f32, [vp, u32, P(f32)]),
+        "vo_light_tree_sample": (u32, [vp, P(f32), f32, P(f32)]),
+        "vo_light_tree_pmf": (f32, [vp, P(f32), u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -195,11 +200,63 @@ class Scene:
 # Integrator (integrator.c): the reference's MIS path loop, with tables from oracle/spectral.py
 # ------------------------------------------------------------------------------------------------
 
+class _LightTreeC(ctypes.Structure):
+    _fields_ = [("words", ctypes.c_void_p), ("node_count", ctypes.c_uint32), ("order", ctypes.c_void_p),
+                ("leaf_node", ctypes.c_void_p), ("depth", ctypes.c_uint32), ("grid_lo", ctypes.c_float * 3),
+                ("grid_hi", ctypes.c_float * 3)]
+
+
+class LightView(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_void_p), ("leaves", ctypes.c_void_p), ("lo", ctypes.c_float * 3),
+                ("scale", ctypes.c_float * 3), ("count", ctypes.c_uint32), ("ids", ctypes.c_void_p),
+                ("positions", ctypes.c_void_p)]
+
+
+class LightTree:
+    """Sources/LightTree.swift build (oracle/lighttree.c) and the GPU walk's float restatement."""
+
+    def __init__(self, tris: np.ndarray, power: np.ndarray, ids: np.ndarray = None):
+        lib = load()
+        t = np.ascontiguousarray(tris, np.float32)
+        self.power = np.ascontiguousarray(power, np.float64)
+        n = len(t)
+        c = _LightTreeC()
+        lib.vo_light_tree_build(_p(t), _p(self.power), n, 8, 0.75, ctypes.byref(c))
+        try:
+            self.words = np.frombuffer(ctypes.string_at(c.words, 32 * c.node_count), np.uint32).reshape(-1, 8).copy()
+            self.order = np.frombuffer(ctypes.string_at(c.order, 4 * n), np.uint32).copy() if n else np.zeros(0, np.uint32)
+            self.leaf_node = np.frombuffer(ctypes.string_at(c.leaf_node, 4 * n), np.uint32).copy() if n else \
+                np.zeros(0, np.uint32)
+            self.depth = c.depth
+            self.grid_lo = np.array(c.grid_lo[:], np.float32)
+            self.grid_hi = np.array(c.grid_hi[:], np.float32)
+        finally:
+            lib.vo_light_tree_free(ctypes.byref(c))
+        self.ids = np.arange(n, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, np.uint32)
+        leaves = np.zeros((n, 3), np.uint32)
+        self.positions = np.zeros(n, np.uint32)
+        for leaf, inp in enumerate(self.order):
+            self.positions[inp] = leaf
+            leaves[leaf] = (self.ids[inp], np.float32(self.power[inp]).view(np.uint32), self.leaf_node[leaf])
+        self.leaves = leaves
+        scale = ((self.grid_hi - self.grid_lo) / np.float32(65535.0)).astype(np.float32)
+        self.view = LightView(_p(self.words), _p(self.leaves), (ctypes.c_float * 3)(*self.grid_lo),
+                              (ctypes.c_float * 3)(*scale), n, _p(self.ids), _p(self.positions))
+
+    def sample(self, p, u):
+        pmf = ctypes.c_float()
+        pos = load().vo_light_tree_sample(ctypes.byref(self.view), _f3(p), u, ctypes.byref(pmf))
+        return (None if pos == 0xffffffff else int(pos)), pmf.value
+
+    def pmf(self, p, position):
+        return load().vo_light_tree_pmf(ctypes.byref(self.view), _f3(p), position)
+
+
 class RenderScene(ctypes.Structure):
     _fields_ = [("geo", ctypes.c_void_p), ("materials", ctypes.c_void_p), ("material_count", ctypes.c_uint32),
                 ("emitter_face", ctypes.c_void_p), ("emitter_cdf", ctypes.c_void_p),
                 ("emitter_count", ctypes.c_uint32), ("emitter_total", ctypes.c_float), ("icdf", ctypes.c_void_p),
-                ("cmf", ctypes.c_void_p), ("basis", ctypes.c_void_p)]
+                ("cmf", ctypes.c_void_p), ("basis", ctypes.c_void_p), ("tree", ctypes.c_void_p)]
 
 
 class Frame(ctypes.Structure):
@@ -256,17 +313,37 @@ def emitter_list(vertices, faces, weights):
     return ids, cdf, float(total)
 
 
+def emitter_power(vertices, faces, ids, weights):
+    slot = faces[ids, 3].astype(np.int64)
+    a, b, c = (vertices[faces[ids, k], :3].astype(np.float32) for k in range(3))
+    area = 0.5 * np.linalg.norm(np.cross((b - a).astype(np.float64), (c - a).astype(np.float64)), axis=1)
+    return area * weights[slot].astype(np.float32).astype(np.float64)
+
+
+def build_scene_tree(vertices, faces, ids, weights):
+    """MaterialLibrary.lightTreeList (main.swift:9374-9401): a tree when >= 2 emitters and > 1 node."""
+    if len(ids) < 2:
+        return None
+    tris = np.concatenate([vertices[faces[ids, k], :3] for k in range(3)], axis=1).astype(np.float32)
+    tree = LightTree(tris, emitter_power(vertices, faces, ids, weights), ids=ids)
+    if len(tree.words) <= 1 or tree.depth > 64:
+        return None
+    return tree
+
+
 def render(scene: "Scene", materials, frame: dict, sample_index0: int, samples: int, pixels=None, precision="f32",
-           threads=0):
+           threads=0, light_tree=True):
     """Mean radiance per pixel (pixels: flat indices, default all) -> (len(pixels), 3) float64."""
     lib = load()
     _, (icdf, cmf, basis) = spectral_data()
     rec, weights = material_records(materials)
     ids, cdf, total = emitter_list(scene.vertices, scene.faces, weights)
+    tree = build_scene_tree(scene.vertices, scene.faces, ids, weights) if light_tree else None
     w, h = frame["width"], frame["height"]
     pix = np.arange(w * h, dtype=np.uint32) if pixels is None else np.ascontiguousarray(pixels, np.uint32)
     rs = RenderScene(ctypes.cast(ctypes.pointer(scene._view), ctypes.c_void_p), _p(rec), len(rec), _p(ids), _p(cdf),
-                     len(ids), total, _p(icdf), _p(cmf), _p(basis))
+                     len(ids), total, _p(icdf), _p(cmf), _p(basis),
+                     ctypes.cast(ctypes.pointer(tree.view), ctypes.c_void_p) if tree else None)
     fr = Frame(w, h, (ctypes.c_float * 3)(*frame["eye"]), (ctypes.c_float * 3)(*frame["target"]), frame["fov"],
                frame["max_depth"], frame.get("sampling_mode", 1))
     out = np.zeros((len(pix), 3), np.float64)

@@ -212,13 +212,23 @@ static v3 nee(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, cons
     if (sc->emitter_count == 0) return zero;
     if (!(vo_rand_f(seed) < 1.0f)) return zero;  /* importedProbability = 1 */
     real value = (real)vo_rand_f(seed), ry = (real)vo_rand_f(seed), rz = (real)vo_rand_f(seed);
-    uint32_t low = 0, high = sc->emitter_count - 1;
-    while (low < high) {
-        uint32_t mid = (low + high) / 2;
-        if ((real)sc->emitter_cdf[mid] >= value) high = mid;
-        else low = mid + 1;
+    uint32_t face;
+    float tree_pmf = 0.0f;
+    if (sc->tree) {
+        float fp[3] = {f_of(s->p.x), f_of(s->p.y), f_of(s->p.z)};
+        uint32_t position = vo_light_tree_sample(sc->tree, fp, f_of(value), &tree_pmf);
+        if (position == 0xffffffffu) return zero;
+        face = sc->tree->leaves[3 * (size_t)position];
+    } else {
+        uint32_t low = 0, high = sc->emitter_count - 1;
+        while (low < high) {
+            uint32_t mid = (low + high) / 2;
+            if ((real)sc->emitter_cdf[mid] >= value) high = mid;
+            else low = mid + 1;
+        }
+        face = sc->emitter_face[low];
     }
-    tri_t t = tri_of(sc, sc->emitter_face[low]);
+    tri_t t = tri_of(sc, face);
     real root = RSQRT(ry);
     v3 pos = add(add(mulv(t.a, (real)1 - root), mulv(t.b, root * ((real)1 - rz))), mulv(t.c, root * rz));
     v3 delta = sub(pos, s->p);
@@ -226,7 +236,13 @@ static v3 nee(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, cons
     v3 wi = mulv(delta, (real)1 / RFMAX(dist, (real)1e-8));
     real geometry = imported_geometry(s->p, pos, &t);
     mat_t em = mat_of(sc, t.tag);
-    real area_pdf = sc->emitter_total > 0 ? em.weight / (real)sc->emitter_total : 0;
+    real area_pdf;
+    if (sc->tree) {
+        real area = (real)0.5 * len(cross(sub(t.b, t.a), sub(t.c, t.a)));
+        area_pdf = area > 0 ? (real)tree_pmf / area : 0;
+    } else {
+        area_pdf = sc->emitter_total > 0 ? em.weight / (real)sc->emitter_total : 0;
+    }
     real pdf = geometry > (real)1e-12 ? area_pdf / geometry : 0;
     if (!(pdf > 0)) return zero;
     /* light_visible */
@@ -306,8 +322,24 @@ static v3 sample_pixel(const vo_render_scene *sc, const vo_frame *fr, const real
         if (rm.type == 3) {
             tri_t t = tri_of(sc, face);
             real g = imported_geometry(prev, r.p, &t);
-            real light_pdf = sc->emitter_count && sc->emitter_total > 0 && rm.weight > 0 && g > (real)1e-12
-                                 ? (rm.weight / (real)sc->emitter_total) / g : 0;
+            real light_pdf = 0;
+            if (sc->tree) {
+                /* imported_emitter_area_pdf with a tree (main.swift:2853-2860) */
+                uint32_t lo = 0, hi = sc->tree->count;
+                while (lo < hi) {
+                    uint32_t mid = (lo + hi) / 2;
+                    if (sc->tree->ids[mid] < face) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo < sc->tree->count && sc->tree->ids[lo] == face && g > (real)1e-12) {
+                    float fp[3] = {f_of(prev.x), f_of(prev.y), f_of(prev.z)};
+                    real area = (real)0.5 * len(cross(sub(t.b, t.a), sub(t.c, t.a)));
+                    real pmf = (real)vo_light_tree_pmf(sc->tree, fp, sc->tree->positions[lo]);
+                    light_pdf = area > 0 ? pmf / area / g : 0;
+                }
+            } else if (sc->emitter_count && sc->emitter_total > 0 && rm.weight > 0 && g > (real)1e-12) {
+                light_pdf = (rm.weight / (real)sc->emitter_total) / g;
+            }
             real w = emission_weight(0, fr->sampling_mode != 3u, fr->sampling_mode == 1u, bpdf, light_pdf);
             v3 e = r.front ? rm.emission : V(0, 0, 0);
             radiance = add(radiance, spectrum_rgb(sc, &l, sscale(smul(thr, emission_spectrum(sc, &l, e)), w)));

@@ -101,6 +101,36 @@ void vo_trace_any_batch(const vo_scene *s, const float *ox, const float *oy, con
                         const float *tmax, int64_t n, int mode, int threads, uint8_t *occluded,
                         uint64_t *counters);
 
+/* ------------------------------------------------------------------ light tree (lighttree.c) */
+
+typedef struct vo_light_tree {
+    uint32_t *words;      /* 8 per node */
+    uint32_t node_count;
+    uint32_t *order;      /* leaf position -> input emitter */
+    uint32_t *leaf_node;  /* leaf position -> node */
+    uint32_t depth;
+    float grid_lo[3], grid_hi[3];
+} vo_light_tree;
+
+/* Sources/LightTree.swift:112-190: tris = 9 floats per emitter (a, b, c), power = area x weight. */
+int32_t vo_light_tree_build(const float *tris, const double *power, uint32_t count, int max_leaf, double leaf_ratio,
+                            vo_light_tree *out);
+void vo_light_tree_free(vo_light_tree *t);
+
+/* The tree as the walk reads it (MaterialLibrary.lightTreeList, main.swift:9374-9401). */
+typedef struct vo_light_view {
+    const uint32_t *nodes;      /* 8 words per node */
+    const uint32_t *leaves;     /* 3 words per leaf position: face, power bits, leaf node */
+    float lo[3], scale[3];      /* grid lo, (hi - lo) / 65535 */
+    uint32_t count;             /* emitters */
+    const uint32_t *ids;        /* emitter faces, ascending */
+    const uint32_t *positions;  /* leaf position of ids[i] */
+} vo_light_view;
+
+float vo_light_tree_importance(const vo_light_view *t, uint32_t node, const float *p);
+uint32_t vo_light_tree_sample(const vo_light_view *t, const float *p, float u, float *pmf);
+float vo_light_tree_pmf(const vo_light_view *t, const float *p, uint32_t position);
+
 /* ------------------------------------------------------------------ integrator (integrator.c) */
 
 /* Scene as the integrators read it; material records are 16 floats per slot: type (u32 bits),
@@ -117,6 +147,7 @@ typedef struct vo_render_scene {
     const float *icdf;              /* 1025 */
     const float *cmf;               /* 471 x 4: rgb CMF, cos phase */
     const float *basis;             /* 471 x 2 x 4: R G B D65 | C M Y 0 */
+    const vo_light_view *tree;      /* light tree, or NULL for power sampling */
 } vo_render_scene;
 
 typedef struct vo_frame {

@@ -87,3 +87,49 @@ def test_cornell_config1_full_size_mean_radiance(gpu_ctx):
     rel = abs(got.mean() / ref.mean() - 1)
     print(f"config1 mean gpu {got.mean():.7f} reference integrator {ref.mean():.7f} rel {rel:.2e}", flush=True)
     assert rel < 1e-3
+
+
+def _multi_light_cornell():
+    """The Cornell box plus a 4 x 4 grid of small ceiling lights of different intensities (tags 4-19)."""
+    base = vt.cornell(0.25)
+    verts, faces = [base.vertices], [base.faces]
+    nv = len(base.vertices)
+    mats = list(CORNELL_MATS)
+    k = 0
+    for i in range(4):
+        for j in range(4):
+            x0, z0, s, y = -0.8 + 0.45 * i, -0.8 + 0.45 * j, 0.12, 0.998
+            quad = np.array([[x0, y, z0], [x0 + s, y, z0 + s], [x0, y, z0 + s],
+                             [x0, y, z0], [x0 + s, y, z0], [x0 + s, y, z0 + s]], np.float32)
+            v = np.zeros((6, 8), np.float32)
+            v[:, :3] = quad
+            v[:, 4] = -1.0
+            verts.append(v)
+            faces.append(np.array([[nv, nv + 1, nv + 2, 4 + k], [nv + 3, nv + 4, nv + 5, 4 + k]], np.uint32))
+            nv += 6
+            e = 2.0 + 3.0 * ((i * 4 + j) % 5)
+            mats.append(dict(type=3, albedo=(0, 0, 0), emission=(e, e, e)))
+            k += 1
+    return vt.Mesh(np.concatenate(verts), np.concatenate(faces)), mats
+
+
+@pytest.mark.parametrize("tree", [True, False])
+def test_multi_light_scene_matches_reference_integrator(gpu_ctx, tree):
+    mesh, mats = _multi_light_cornell()
+    w, h, spp = 96, 96, 32
+    scene = gpu_ctx.set_mesh(mesh)
+    scene.set_light_tree(tree)
+    scene.set_materials([vt.material(m["type"], m["albedo"], m.get("emission", (0, 0, 0))) for m in mats])
+    r = vt.Renderer(gpu_ctx)
+    r.render(scene, w, h, vt.orbit_eye(), (0.0, -0.1, 0.0), 42.0, max_depth=8, samples=spp, sample_index=1)
+    got = r.readback()[..., :3].astype(np.float64)
+    r.close()
+    scene.close()
+    sc = oracle.Scene(mesh.vertices, mesh.faces)
+    frame = dict(width=w, height=h, eye=vt.orbit_eye(), target=(0.0, -0.1, 0.0), fov=42.0, max_depth=8)
+    ref = oracle.render(sc, mats, frame, 1, spp, light_tree=tree).reshape(h, w, 3)
+    rel = abs(got.mean() / ref.mean() - 1)
+    print(f"multi-light tree={tree}: gpu {got.mean():.6f} oracle {ref.mean():.6f} rel {rel:.2e}", flush=True)
+    assert rel < 1e-3
+    close = np.isclose(got, ref, rtol=1e-3, atol=1e-4).all(axis=2)
+    assert close.mean() > 0.97, close.mean()
