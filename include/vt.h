@@ -142,8 +142,11 @@ typedef struct vt_frame_params {
     float target[3];         /* cameraTarget.xyz (up is +y) */
     float fov_degrees;       /* cameraPos.w, vertical */
     uint32_t max_depth;      /* cameraTarget.w: path vertices, 1 = direct light only */
-    uint32_t sampling_mode;  /* samplingMode: 1 = MIS (NEE + BSDF, power heuristic) */
+    uint32_t sampling_mode;  /* samplingMode: 0 = ReSTIR DI at the primary vertex (RIS, temporal and
+                                spatial reuse) + MIS suffix, 1 = MIS (NEE + BSDF, power heuristic) */
     uint32_t sample_index;   /* sampleIndex of the first sample of this call (>= 1) */
+    uint32_t restir_candidates; /* RIS light candidates per pixel in mode 0 (restir_di_initial's
+                                   M = 4, main.swift:4649); 0 means 4, at most 256 */
 } vt_frame_params;
 
 /* Replaces PathTracerRenderer (main.swift:9514) for this path: owns the accumulation buffer. */
@@ -152,10 +155,13 @@ void vt_renderer_destroy(vt_renderer *r);
 /* Replaces renderFrame(output:samples:) (main.swift:10978): renders `samples` samples (sample
  * indices sample_index .. sample_index + samples - 1) and folds them into the running mean
  * accum = prev + (x - prev) / frameIndex (main.swift:7908-7913).  A size change resets the
- * accumulation.  Asynchronous on `stream`. */
+ * accumulation.  In sampling mode 0 every sample is one ReSTIR frame (encodeSample,
+ * main.swift:11214-11708): its reservoirs become the next sample's temporal history, across calls,
+ * until a reset, a size change or a sample in another mode.  Asynchronous on `stream`. */
 vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params *params, uint32_t samples,
                           void *stream);
-/* Replaces resetAccumulation() (main.swift:10761). */
+/* Replaces resetAccumulation() (main.swift:10761); also drops the ReSTIR reservoir history
+ * (reservoirHistory = 0). */
 vt_status vt_reset_accumulation(vt_renderer *r);
 /* Copies the rgba32f running mean (width * height * 4 floats) to host memory; synchronises. */
 vt_status vt_accum_readback(vt_renderer *r, float *rgba, size_t bytes);

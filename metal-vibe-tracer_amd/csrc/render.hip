@@ -7,6 +7,7 @@
 // that reach an emitter are weighted by emission_weight (main.swift:3462-3466); Russian roulette after
 // the third bounce (main.swift:7819-7827); four stratified wavelengths per path (main.swift:795-833).
 // Random numbers follow the reference's per-pixel PCG streams draw for draw.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -346,13 +347,15 @@ __device__ __forceinline__ float lt_pmf(const RenderScene &sc, V3 p, uint32_t po
 
 struct LightSample {
     V3 position, wi, emission;
-    float dist, pdf;
-    uint32_t slot;
+    float dist, pdf, geometry;
+    uint32_t slot, light;  // light: face + 2 (LightSample.isDirectional), 0 = none
 };
 __device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, V3 p, uint32_t &seed) {
     LightSample ls;
     ls.pdf = 0.0f;
     ls.slot = 0;
+    ls.light = 0;
+    ls.geometry = 0.0f;
     ls.emission = v3(0.0f, 0.0f, 0.0f);
     ls.position = p;
     ls.wi = v3(0.0f, 0.0f, 1.0f);
@@ -389,6 +392,8 @@ __device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, V3 
     Mat m = material(sc, tag);
     ls.emission = m.emission;
     ls.slot = tag;
+    ls.light = face + 2u;
+    ls.geometry = geometry;
     float area_pdf;
     if (sc.tree_node_count) {
         float area = 0.5f * length(cross(b - a, c - a));
@@ -421,6 +426,82 @@ __device__ __forceinline__ float emitter_pdf(const RenderScene &sc, V3 p, V3 pos
     }
     if (!(sc.emitter_total > 0.0f) || !(weight > 0.0f)) return 0.0f;
     return (weight / sc.emitter_total) / g;
+}
+
+// ---------------------------------------------------------------- ReSTIR DI (main.swift:3165-3196, 4618-4716, 4812-4835, 5015-5033)
+struct DiSample {
+    V3 position, emission;
+    uint32_t light;  // face + 2, 0 = none
+    float pdf, u;
+};
+__device__ __forceinline__ DiSample di_empty() {
+    DiSample d;
+    d.position = v3(0.0f, 0.0f, 0.0f);
+    d.emission = d.position;
+    d.light = 0;
+    d.pdf = 0.0f;
+    d.u = 0.0f;
+    return d;
+}
+// restir_di_pack_light / unpack (main.swift:4674-4675): id bits offset into the normal floats.
+__device__ __forceinline__ float pack_light(uint32_t id) { return id == 0u ? 0.0f : __uint_as_float(id + 0x00800000u); }
+__device__ __forceinline__ uint32_t unpack_light(float w) {
+    uint32_t bits = __float_as_uint(w);
+    return bits == 0u ? 0u : bits - 0x00800000u;
+}
+__device__ __forceinline__ uint2 pack_half4(float x, float y, float z, float w) {
+    uint32_t a = __half_as_ushort(__float2half_rn(x)), b = __half_as_ushort(__float2half_rn(y));
+    uint32_t c = __half_as_ushort(__float2half_rn(z)), d = __half_as_ushort(__float2half_rn(w));
+    return make_uint2(a | (b << 16), c | (d << 16));
+}
+__device__ __forceinline__ float4 unpack_half4(uint2 v) {
+    return make_float4(__half2float(__ushort_as_half((unsigned short)(v.x & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short)(v.x >> 16))),
+                       __half2float(__ushort_as_half((unsigned short)(v.y & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short)(v.y >> 16))));
+}
+// restir_di_stored_sample (main.swift:4679-4688).
+__device__ __forceinline__ DiSample di_stored(float4 posdir, float4 emitpdf, float4 weights) {
+    DiSample d;
+    d.position = v3(posdir.x, posdir.y, posdir.z);
+    d.light = unpack_light(posdir.w);
+    d.emission = v3(emitpdf.x, emitpdf.y, emitpdf.z);
+    d.pdf = emitpdf.w;
+    d.u = weights.w;
+    return d;
+}
+// light_geometry of an imported emitter (main.swift:3165-3166).
+__device__ __forceinline__ float light_geometry(const RenderScene &sc, V3 p, const DiSample &d) {
+    if (d.light < 2u || d.light - 2u >= sc.face_count) return 0.0f;
+    V3 a, b, c;
+    uint32_t tag;
+    const float *va, *vb, *vc;
+    face_corners(sc, d.light - 2u, a, b, c, tag, &va, &vb, &vc);
+    return imported_geometry(p, d.position, a, b, c);
+}
+// eval_bsdf for DIFFUSE (main.swift:2659-2665), given rho/pi at the lanes.
+__device__ __forceinline__ S4 diffuse_f(const Surface &s, S4 rho_pi, V3 wo, V3 wi) {
+    if (dot(wi, s.ng) <= 0.0f) return s4(0.0f);
+    float c = fabsf(dot(s.n, wi));
+    if (c < 1e-7f || dot(s.n, wo) <= 0.0f) return s4(0.0f);
+    return dot(s.n, wi) > 0.0f ? rho_pi : s4(0.0f);
+}
+// eval_restir_target_pdf (main.swift:3187-3196): |rgb(f L cos)| G at the sample's wavelengths
+// (lanes and rho_pi belong to d.u; g is light_geometry(p, d)).
+__device__ __forceinline__ float target_pdf(const Spectral &sp, const Lanes &l, S4 rho_pi, const Surface &s, V3 raydir,
+                                            const DiSample &d, float g) {
+    if (d.pdf <= 0.0f) return 0.0f;
+    V3 dir = normalize(d.position - s.p);
+    float ct = fmaxf(0.0f, dot(s.n, dir));
+    if (ct <= 0.0f) return 0.0f;
+    S4 f = diffuse_f(s, rho_pi, -raydir, dir);
+    return length(spectrum_rgb(sp, l, scale(mul(f, emission_spectrum(sp, l, d.emission)), ct))) * g;
+}
+__device__ __forceinline__ float target_pdf_at(const RenderScene &sc, const Spectral &sp, const Mat &m, const Surface &s,
+                                               V3 raydir, const DiSample &d) {
+    if (d.pdf <= 0.0f) return 0.0f;
+    const Lanes l = lanes_of(sp, d.u);
+    return target_pdf(sp, l, scale(reflectance(sp, l, m.lagrange), 1.0f / kPi), s, raydir, d, light_geometry(sc, s.p, d));
 }
 
 // ---------------------------------------------------------------- wave-aggregated queue append
@@ -491,9 +572,103 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameArgs f, PathQueue q,
     radiance[3 * (size_t)i + 2] = 0.0f;
 }
 
+// Pass 1 of ReSTIR DI for one sample (restir_temporal_kernel, main.swift:4842-4940): one thread per
+// pixel over the camera wavefront (queue slot = pixel).  Writes the G-buffer and the pixel's
+// reservoir: RIS over ra.candidates light samples on the pixel's un-decorrelated PCG stream
+// (restir_di_initial :4637-4668), then the temporal merge of the backprojected history reservoir
+// (restir_di_merge :4692-4704) and the encode (restir_di_encode :4708-4716).
+__global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, FrameArgs f, PathQueue q, HitArgs hits,
+                                                             RestirArgs ra) {
+    const uint32_t pix = blockIdx.x * 256 + threadIdx.x;
+    if (pix >= f.npix) return;
+    float4 gpos = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    uint2 gnorm = pack_half4(0.0f, 0.0f, 0.0f, -1.0f);
+    float4 posdir = make_float4(0.0f, 0.0f, 0.0f, 0.0f), emitpdf = posdir, weights = posdir;
+    const int face = hits.face[pix];
+    if (face >= 0) {
+        const V3 dir = v3(q.dx[pix], q.dy[pix], q.dz[pix]);
+        const Surface s = surface_at(sc, (uint32_t)face, hits.b1[pix], hits.b2[pix], dir);
+        const Mat m = material(sc, s.slot);
+        const float t = hits.t[pix];
+        gpos = make_float4(s.p.x, s.p.y, s.p.z, t);
+        gnorm = pack_half4(s.n.x, s.n.y, s.n.z, (float)m.type);
+        if (m.type == kDiffuse) {
+            const Spectral sp{sc.icdf, reinterpret_cast<const float4 *>(sc.cmf), reinterpret_cast<const float4 *>(sc.basis)};
+            uint32_t seed = pix ^ (f.sample_index0 * 1999999973u);
+            const float u = q.wl[pix];
+            const Lanes lanes = lanes_of(sp, u);
+            const S4 rho_pi = scale(reflectance(sp, lanes, m.lagrange), 1.0f / kPi);
+            DiSample sel = di_empty();
+            float wsum = 0.0f, M = 0.0f;
+            for (uint32_t i = 0; i < ra.candidates; ++i) {
+                const LightSample ls = sample_emitter(sc, s.p, seed);
+                DiSample c;
+                c.position = ls.position;
+                c.emission = ls.emission;
+                c.light = ls.light;
+                c.pdf = ls.pdf;
+                c.u = u;
+                M += 1.0f;
+                if (c.pdf > 0.0f) {
+                    const float ph = target_pdf(sp, lanes, rho_pi, s, dir, c, ls.geometry);
+                    const float proposal = c.pdf * ls.geometry;
+                    const float w = proposal > 0.0f ? ph / proposal : 0.0f;
+                    wsum += w;
+                    if (rand_f(seed) * wsum < w) sel = c;
+                }
+            }
+            if (ra.temporal) {
+                // restir_backproject through the previous sample's camera.
+                const V3 d = s.p - load_v3(ra.prev_eye);
+                const float w = dot(d, load_v3(ra.prev_forward));
+                if (w > 0.0f) {
+                    const float x = dot(d, load_v3(ra.prev_right)) / (w * ra.prev_aspect * ra.prev_fov_scale);
+                    const float y = dot(d, load_v3(ra.prev_up)) / (w * ra.prev_fov_scale);
+                    const float ux = x * 0.5f + 0.5f, uy = y * -0.5f + 0.5f;
+                    if (ux >= 0.0f && uy >= 0.0f && ux < 1.0f && uy < 1.0f) {
+                        const int px = (int)(ux * (float)f.width), py = (int)(uy * (float)f.height);
+                        if (px >= 0 && px < (int)f.width && py >= 0 && py < (int)f.height) {
+                            const uint32_t hp = (uint32_t)py * f.width + (uint32_t)px;
+                            const float4 hw = ra.hist.weights[hp];
+                            const float4 og = ra.hist.gpos[hp];
+                            const float4 on = unpack_half4(ra.hist.gnorm[hp]);
+                            // restir_same_surface (main.swift:4816-4823).
+                            const bool same = og.w > 0.0f && on.w == (float)m.type &&
+                                              dot(v3(on.x, on.y, on.z), s.n) > 0.95f &&
+                                              length(v3(og.x, og.y, og.z) - s.p) < fmaxf(0.01f, t * 0.01f);
+                            if (same && hw.y > 0.0f && hw.z > 0.0f) {
+                                const DiSample hs = di_stored(ra.hist.posdir[hp], ra.hist.emitpdf[hp], hw);
+                                const float ph = target_pdf_at(sc, sp, m, s, dir, hs);
+                                const float cm = fminf(hw.y, 20.0f);
+                                const float wt = ph * hw.z * cm;
+                                M += cm;
+                                wsum += wt;
+                                if (rand_f(seed) * wsum < wt) sel = hs;
+                            }
+                        }
+                    }
+                }
+            }
+            const float ph = target_pdf_at(sc, sp, m, s, dir, sel);
+            const float W = (M > 0.0f && ph > 0.0f) ? wsum / (M * ph) : 0.0f;
+            posdir = make_float4(sel.position.x, sel.position.y, sel.position.z, pack_light(sel.light));
+            emitpdf = make_float4(sel.emission.x, sel.emission.y, sel.emission.z, sel.pdf);
+            weights = make_float4(wsum, M, W, sel.u);
+        }
+    }
+    ra.cur.gpos[pix] = gpos;
+    ra.cur.gnorm[pix] = gnorm;
+    ra.cur.posdir[pix] = posdir;
+    ra.cur.emitpdf[pix] = emitpdf;
+    ra.cur.weights[pix] = weights;
+}
+
+// Shading of one wavefront vertex per path.  kRestir: sampling mode 0, where a diffuse primary vertex
+// takes its direct light from ReSTIR DI's spatial reuse (main.swift:7484-7580) instead of NEE.
+template <bool kRestir>
 __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f, PathQueue in, HitArgs hits,
                                                     PathQueue out, ShadowQueue sh, Counters c, uint32_t capacity,
-                                                    float *radiance) {
+                                                    float *radiance, RestirArgs ra) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const uint32_t n = min(*c.in, capacity);
     const bool active = i < n;
@@ -538,7 +713,58 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
                 alive = false;
             }
         }
-        if (alive) {
+        if (kRestir && alive && bounce == 0u && m.type == kDiffuse) {
+            // Spatial reuse: the pixel's reservoir and four uniform taps in a 16-pixel box
+            // (uniform_neighbor, restir2020_compatible), combined with w = p_hat W M.
+            const uint32_t pix = id;
+            const uint32_t gx = pix % f.width, gy = pix / f.width;
+            const float4 cg = ra.cur.gpos[pix], cw = ra.cur.weights[pix];
+            DiSample sel = di_stored(ra.cur.posdir[pix], ra.cur.emitpdf[pix], cw);
+            float wsum = target_pdf_at(sc, sp, m, s, dir, sel) * cw.z * cw.y;
+            float M = cw.y;
+            for (int k = 0; k < 4; ++k) {
+                const float rx = rand_f(seed), ry = rand_f(seed);
+                const int nx = (int)gx + (int)((rx * 2.0f - 1.0f) * 16.0f);
+                const int ny = (int)gy + (int)((ry * 2.0f - 1.0f) * 16.0f);
+                if (nx < 0 || nx >= (int)f.width || ny < 0 || ny >= (int)f.height) continue;
+                const uint32_t np = (uint32_t)ny * f.width + (uint32_t)nx;
+                const float4 ng = ra.cur.gpos[np];
+                const float4 nn = unpack_half4(ra.cur.gnorm[np]);
+                const bool compatible = ng.w > 0.0f && nn.w == (float)m.type && dot(s.n, v3(nn.x, nn.y, nn.z)) > 0.95f &&
+                                        fabsf(cg.w - ng.w) < 0.05f * cg.w;
+                if (!compatible) continue;
+                const float4 nw = ra.cur.weights[np];
+                if (!(nw.y > 0.0f && nw.z > 0.0f)) continue;
+                const DiSample ns = di_stored(ra.cur.posdir[np], ra.cur.emitpdf[np], nw);
+                const float w = target_pdf_at(sc, sp, m, s, dir, ns) * nw.z * nw.y;
+                wsum += w;
+                M += nw.y;
+                if (rand_f(seed) * wsum < w) sel = ns;
+            }
+            const float fp = target_pdf_at(sc, sp, m, s, dir, sel);
+            const float W = (M > 0.0f && fp > 0.0f) ? wsum / (M * fp) : 0.0f;
+            if (W > 0.0f) {
+                // Shading with visibility (light_visible): bsdf cos L W G at the sample's wavelengths.
+                const V3 wi = normalize(sel.position - s.p);
+                const V3 o = ray_origin(s.p, s.ng, wi, s.error);
+                const V3 delta = sel.position - o;
+                const float d = length(delta);
+                const float tol = endpoint_tolerance(o, sel.position, d);
+                if (d > tol) {
+                    const float ct = fmaxf(0.0f, dot(s.n, wi));
+                    const Lanes sl = lanes_of(sp, sel.u);
+                    const S4 fb = diffuse_f(s, scale(reflectance(sp, sl, m.lagrange), 1.0f / kPi), -dir, wi);
+                    const S4 v = scale(scale(mul(scale(fb, ct), emission_spectrum(sp, sl, sel.emission)), W),
+                                       light_geometry(sc, s.p, sel));
+                    sh_c = spectrum_rgb(sp, sl, v);
+                    sh_o = o;
+                    sh_d = normalize(delta);
+                    sh_tmin = ray_t_min(o);
+                    sh_tmax = d - tol;
+                    shadow = true;
+                }
+            }
+        } else if (alive) {
             // Next-event estimation at this vertex (Lambert: never delta).
             const bool use_mis = f.sampling_mode <= 1u && (bounce == 0u ? f.scattering_limit > 0 : sdepth < f.scattering_limit);
             LightSample ls = sample_emitter(sc, s.p, seed);
@@ -592,7 +818,10 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
                     new_d = nd;
                     new_pdf = pdf;
                     prev_p = s.p;
-                    uint32_t flags = kPrevNee | (f.sampling_mode == 1u ? kPrevMis : 0u);
+                    // previousMIS (main.swift:7725-7728): a diffuse primary's DI reservoir estimated the
+                    // whole direct integral, so in ReSTIR mode its continuation is not MIS-weighted.
+                    const bool mis = f.sampling_mode == 1u || (f.sampling_mode == 0u && (bounce > 0u || m.type != kDiffuse));
+                    uint32_t flags = kPrevNee | (mis ? kPrevMis : 0u);
                     state = (bounce + 1u) | ((uint32_t)sdepth << 8) | flags;
                     cont = true;
                 }
@@ -679,8 +908,22 @@ hipError_t launch_generate(const FrameArgs &f, const PathQueue &q, float *radian
 hipError_t launch_shade(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
                         const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
                         float *radiance, hipStream_t s) {
-    hipLaunchKernelGGL(shade_kernel, dim3((capacity + 255) / 256), dim3(256), 0, s, sc, f, in, hits, out, sh, c,
-                       capacity, radiance);
+    hipLaunchKernelGGL(shade_kernel<false>, dim3((capacity + 255) / 256), dim3(256), 0, s, sc, f, in, hits, out, sh, c,
+                       capacity, radiance, RestirArgs{});
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_restir(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
+                               const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
+                               float *radiance, const RestirArgs &ra, hipStream_t s) {
+    hipLaunchKernelGGL(shade_kernel<true>, dim3((capacity + 255) / 256), dim3(256), 0, s, sc, f, in, hits, out, sh, c,
+                       capacity, radiance, ra);
+    return hipGetLastError();
+}
+
+hipError_t launch_restir_initial(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
+                                 const RestirArgs &ra, hipStream_t s) {
+    hipLaunchKernelGGL(restir_initial_kernel, dim3((f.npix + 255) / 256), dim3(256), 0, s, sc, f, q, hits, ra);
     return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@ struct RenderScene {
     TraceScene trace;
     const float *vertices;     // 8 floats per vertex (document order)
     const uint32_t *faces;     // 4 u32 per face (document order)
+    uint32_t face_count;
     const float *materials;    // 16 floats per slot
     uint32_t material_count;
     const uint32_t *emitter_face;  // emitting faces (power sampling), ascending face index
@@ -41,6 +42,24 @@ struct RenderScene {
     float tree_lo[3], tree_scale[3];
 };
 
+// ReSTIR DI state per pixel, the reference's G-buffer and reservoir textures (main.swift:4842-4940):
+// gpos = position, camera distance (w = -1: no hit; rgba32f), gnorm = normal, material type as four
+// halves (rgba16f), then the reservoir: posdir = sample position + light id bits, emitpdf =
+// emission rgb + solid-angle pdf, weights = weight sum, M, W, wavelength number (rgba32f each).
+struct RestirBuffers {
+    float4 *gpos;
+    uint2 *gnorm;
+    float4 *posdir, *emitpdf, *weights;
+};
+struct RestirArgs {
+    RestirBuffers cur, hist;  // written by this sample's pass 1 / the previous sample's
+    uint32_t candidates;      // RIS candidates per pixel (the reference uses 4)
+    uint32_t temporal;        // history holds the previous sample of the same image
+    // Previous sample's camera for backprojection (prevViewProj, main.swift:4828-4835).
+    float prev_eye[3], prev_forward[3], prev_right[3], prev_up[3];
+    float prev_fov_scale, prev_aspect;
+};
+
 struct FrameArgs {
     uint32_t width, height, npix;
     uint32_t samples;          // samples in this wavefront (path id = sample * npix + pixel)
@@ -48,7 +67,7 @@ struct FrameArgs {
     float eye[3], forward[3], right[3], up[3];
     float fov_scale, aspect;
     int32_t scattering_limit;  // max(0, depth - 1)
-    uint32_t sampling_mode;    // 1 = MIS (NEE + BSDF, power heuristic)
+    uint32_t sampling_mode;    // 0 = ReSTIR DI at the primary vertex, 1 = MIS (NEE + BSDF, power heuristic)
 };
 
 // One queue of paths: the ray arrays first (RayArgs layout), then the path state.
@@ -77,6 +96,15 @@ hipError_t launch_generate(const FrameArgs &f, const PathQueue &q, float *radian
 hipError_t launch_shade(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
                         const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
                         float *radiance, hipStream_t s);
+// Pass 1 of ReSTIR DI (restir_temporal_kernel): G-buffer, RIS candidates, temporal merge, encode,
+// for the wavefront's single sample (q holds the camera paths in pixel order, hits their hits).
+hipError_t launch_restir_initial(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
+                                 const RestirArgs &ra, hipStream_t s);
+// launch_shade with ReSTIR DI at the primary vertex (sampling mode 0): spatial reuse of ra.cur,
+// shading with one visibility ray, then the path suffix.
+hipError_t launch_shade_restir(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
+                               const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
+                               float *radiance, const RestirArgs &ra, hipStream_t s);
 hipError_t launch_resolve(const ShadowQueue &sh, const uint8_t *occluded, const uint32_t *count, uint32_t capacity,
                           float *radiance, hipStream_t s);
 hipError_t launch_reset_counters(uint32_t *counters, uint32_t n, hipStream_t s);

@@ -47,6 +47,13 @@ struct vt_renderer {
     uint32_t capacity = 0;  // paths per wavefront
     DeviceBuffer accum, radiance, hits, occluded, counters, shadow;
     QueueStorage q[2];
+    // ReSTIR DI: two sets of G-buffer + reservoir arrays (72 B per pixel each), ping-ponged per
+    // sample; restir_history counts consecutive mode-0 samples of this image (reservoirHistory).
+    DeviceBuffer restir[2];
+    int restir_cur = 0;
+    uint32_t restir_history = 0;
+    uint64_t restir_generation = 0;  // vt_scene::generation the history was rendered with
+    float prev_camera[14] = {};  // eye, forward, right, up, fov_scale, aspect of the last sample
 };
 
 namespace {
@@ -60,6 +67,17 @@ void ensure_capacity(vt_renderer *r, uint32_t cap) {
     r->occluded.alloc(cap);
     r->radiance.alloc((size_t)3 * cap * 4);
     r->capacity = cap;
+}
+
+vt::RestirBuffers restir_view(const DeviceBuffer &b, uint32_t npix) {
+    char *p = b.as<char>();
+    vt::RestirBuffers v;
+    v.gpos = reinterpret_cast<float4 *>(p);
+    v.gnorm = reinterpret_cast<uint2 *>(p + (size_t)16 * npix);
+    v.posdir = reinterpret_cast<float4 *>(p + (size_t)24 * npix);
+    v.emitpdf = reinterpret_cast<float4 *>(p + (size_t)40 * npix);
+    v.weights = reinterpret_cast<float4 *>(p + (size_t)56 * npix);
+    return v;
 }
 
 vt::ShadowQueue shadow_view(const vt_renderer *r) {
@@ -190,6 +208,7 @@ vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, 
         scene->emitter_count = (uint32_t)ids.size();
         scene->emitter_total = (float)total;
         scene->shaded = true;
+        scene->generation = vt_scene::next_generation();
         return VT_OK;
     });
 }
@@ -224,6 +243,7 @@ vt_status vt_spectral_reflectance(const float rgb[3], float out[4]) {
 vt_status vt_scene_set_light_tree(vt_scene *scene, int enabled) {
     if (!scene) return fail(VT_ERROR_INVALID_ARGUMENT, "scene must not be NULL");
     scene->light_tree = enabled != 0;
+    scene->generation = vt_scene::next_generation();
     return VT_OK;
 }
 
@@ -277,6 +297,7 @@ void vt_renderer_destroy(vt_renderer *r) { delete r; }
 vt_status vt_reset_accumulation(vt_renderer *r) {
     if (!r) return fail(VT_ERROR_INVALID_ARGUMENT, "renderer must not be NULL");
     r->frame_index = 0;
+    r->restir_history = 0;
     return VT_OK;
 }
 
@@ -289,7 +310,10 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
         if (p->width == 0 || p->height == 0 || (uint64_t)p->width * p->height > (1u << 26))
             return fail(VT_ERROR_INVALID_ARGUMENT, "render size out of range");
         if (p->max_depth < 1 || p->max_depth > 64) return fail(VT_ERROR_INVALID_ARGUMENT, "max_depth must be 1..64");
-        if (p->sampling_mode != 1) return fail(VT_ERROR_UNSUPPORTED, "only sampling_mode 1 (MIS) is implemented");
+        if (p->sampling_mode > 1) return fail(VT_ERROR_UNSUPPORTED, "sampling_mode must be 0 (ReSTIR DI) or 1 (MIS)");
+        const uint32_t candidates = p->restir_candidates ? p->restir_candidates : 4u;
+        if (candidates > 256) return fail(VT_ERROR_INVALID_ARGUMENT, "restir_candidates must be at most 256");
+        if (!(p->fov_degrees > 0.0f && p->fov_degrees < 180.0f)) return fail(VT_ERROR_INVALID_ARGUMENT, "fov must be in (0, 180)");
         if (p->sample_index == 0) return fail(VT_ERROR_INVALID_ARGUMENT, "sample_index starts at 1");
         if (samples == 0) return VT_OK;
         check(hipSetDevice(r->ctx->device), "hipSetDevice");
@@ -301,9 +325,21 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
             r->width = p->width;
             r->height = p->height;
             r->frame_index = 0;
+            r->restir_history = 0;
+            r->restir[0].reset();
+            r->restir[1].reset();
         }
-        // Paths per wavefront: whole images, up to ~4M paths.
-        const uint32_t per = std::max<uint32_t>(1, std::min<uint32_t>(samples, (1u << 22) / npix));
+        const bool restir = p->sampling_mode == 0;
+        if (restir && !r->restir[0].ptr) {
+            r->restir[0].alloc((size_t)72 * npix);
+            r->restir[1].alloc((size_t)72 * npix);
+            r->restir_history = 0;
+        }
+        if (!restir || r->restir_generation != scene->generation) r->restir_history = 0;
+        r->restir_generation = scene->generation;
+        // Paths per wavefront: whole images, up to ~4M paths; ReSTIR frames run one sample at a time
+        // (each reads the previous sample's reservoirs).
+        const uint32_t per = restir ? 1u : std::max<uint32_t>(1, std::min<uint32_t>(samples, (1u << 22) / npix));
         ensure_capacity(r, per * npix);
 
         vt::FrameArgs f{};
@@ -340,6 +376,19 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
             check(vt::launch_reset_counters(cnt, 16, s), "reset");
             int cur = 0;
             check(vt::launch_generate(f, r->q[0].view(r->capacity), r->radiance.as<float>(), cnt + 0, s), "generate");
+            vt::RestirArgs ra{};
+            if (restir) {
+                ra.cur = restir_view(r->restir[r->restir_cur], npix);
+                ra.hist = restir_view(r->restir[1 - r->restir_cur], npix);
+                ra.candidates = candidates;
+                ra.temporal = r->restir_history >= 1 ? 1u : 0u;
+                std::memcpy(ra.prev_eye, r->prev_camera, 3 * sizeof(float));
+                std::memcpy(ra.prev_forward, r->prev_camera + 3, 3 * sizeof(float));
+                std::memcpy(ra.prev_right, r->prev_camera + 6, 3 * sizeof(float));
+                std::memcpy(ra.prev_up, r->prev_camera + 9, 3 * sizeof(float));
+                ra.prev_fov_scale = r->prev_camera[12];
+                ra.prev_aspect = r->prev_camera[13];
+            }
             for (int b = 0; b <= f.scattering_limit; ++b) {
                 uint32_t *cin = cnt + cur, *cout = cnt + (1 - cur), *csh = cnt + 2;
                 check(vt::launch_reset_counters(cout, 1, s), "reset");
@@ -348,7 +397,13 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
                 vt::RayArgs rays{qin.ox, qin.oy, qin.oz, qin.dx, qin.dy, qin.dz, qin.tmin, qin.tmax};
                 check(vt::launch_trace(scene->view, rays, hits, nullptr, n, false, nullptr, s, cin), "extend");
                 vt::Counters c{cin, cout, csh};
-                check(vt::launch_shade(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), s), "shade");
+                if (restir && b == 0) {
+                    check(vt::launch_restir_initial(sc, f, qin, hits, ra, s), "restir pass 1");
+                    check(vt::launch_shade_restir(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), ra, s),
+                          "shade");
+                } else {
+                    check(vt::launch_shade(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), s), "shade");
+                }
                 vt::RayArgs srays{sh.ox, sh.oy, sh.oz, sh.dx, sh.dy, sh.dz, sh.tmin, sh.tmax};
                 check(vt::launch_trace(scene->view, srays, vt::HitArgs{}, r->occluded.as<uint8_t>(), n, true, nullptr, s, csh),
                       "shadow");
@@ -358,6 +413,16 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
             check(vt::launch_accumulate(f, r->radiance.as<float>(), r->accum.as<float>(), r->frame_index + 1, s),
                   "accumulate");
             r->frame_index += f.samples;
+            if (restir) {
+                r->restir_cur = 1 - r->restir_cur;
+                r->restir_history = std::min<uint32_t>(r->restir_history, 1u << 24) + 1u;
+                std::memcpy(r->prev_camera, f.eye, 3 * sizeof(float));
+                std::memcpy(r->prev_camera + 3, f.forward, 3 * sizeof(float));
+                std::memcpy(r->prev_camera + 6, f.right, 3 * sizeof(float));
+                std::memcpy(r->prev_camera + 9, f.up, 3 * sizeof(float));
+                r->prev_camera[12] = f.fov_scale;
+                r->prev_camera[13] = f.aspect;
+            }
         }
         return VT_OK;
     });

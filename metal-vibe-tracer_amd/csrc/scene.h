@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <functional>
 #include <memory>
 #include <stdexcept>
@@ -81,6 +82,13 @@ struct vt_scene {
     uint32_t material_count = 0, emitter_count = 0;
     float emitter_total = 0.0f;
     bool shaded = false;  // materials set
+    // Identity of the scene's current contents: a process-unique id, bumped by every material or
+    // light-tree change (renderers drop ReSTIR history rendered with other contents).
+    uint64_t generation = next_generation();
+    static uint64_t next_generation() {
+        static std::atomic<uint64_t> counter{1};
+        return counter.fetch_add(1);
+    }
     std::vector<float> host_vertices;
     std::vector<uint32_t> host_faces;
     std::unique_ptr<vt::SpectralTables> spectral;
@@ -95,6 +103,7 @@ struct vt_scene {
         r.faces = faces.as<const uint32_t>();
         r.materials = materials.as<const float>();
         r.material_count = material_count;
+        r.face_count = (uint32_t)(host_faces.size() / 4);
         r.emitter_face = emitter_face.as<const uint32_t>();
         r.emitter_cdf = emitter_cdf.as<const float>();
         r.emitter_count = emitter_count;

@@ -73,7 +73,8 @@ def material(type=0, albedo=(0.7, 0.7, 0.7), emission=(0.0, 0.0, 0.0), roughness
 class _FrameParams(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("eye", ctypes.c_float * 3),
                 ("target", ctypes.c_float * 3), ("fov_degrees", ctypes.c_float), ("max_depth", ctypes.c_uint32),
-                ("sampling_mode", ctypes.c_uint32), ("sample_index", ctypes.c_uint32)]
+                ("sampling_mode", ctypes.c_uint32), ("sample_index", ctypes.c_uint32),
+                ("restir_candidates", ctypes.c_uint32)]
 
 
 class _RaySoA(ctypes.Structure):
@@ -450,9 +451,9 @@ class Renderer:
             pass
 
     def render(self, scene: Scene, width, height, eye, target, fov, max_depth=8, samples=1, sample_index=1,
-               sampling_mode=1, stream=None):
+               sampling_mode=1, candidates=4, stream=None):
         p = _FrameParams(width, height, (ctypes.c_float * 3)(*eye), (ctypes.c_float * 3)(*target), fov, max_depth,
-                         sampling_mode, sample_index)
+                         sampling_mode, sample_index, candidates)
         _check(load().vt_render_frame(self._h, scene._h, ctypes.byref(p), samples, _stream_ptr(stream)))
         self.size = (width, height)
 

@@ -67,8 +67,11 @@ def load():
         "vo_trace_closest_batch": (None, [P(SceneView)] + [vp] * 8 + [c.c_int64, c.c_int, c.c_int] + [vp] * 5),
         "vo_trace_any_batch": (None, [P(SceneView)] + [vp] * 8 + [c.c_int64, c.c_int, c.c_int, vp, vp]),
         "vo_set_fault": (None, [c.c_int]),
+        "vo_half_round": (c.c_float, [c.c_float]),
         "vo_render_f32": (None, [vp, vp, u32, u32, vp, u32, c.c_int, vp]),
         "vo_render_f64": (None, [vp, vp, u32, u32, vp, u32, c.c_int, vp]),
+        "vo_render_restir_f32": (None, [vp, vp, u32, u32, u32, vp, c.c_int, vp]),
+        "vo_render_restir_f64": (None, [vp, vp, u32, u32, u32, vp, c.c_int, vp]),
         "vo_max_threads": (c.c_int, []),
         "vo_light_tree_build": (i32, [vp, vp, u32, c.c_int, c.c_double, vp]),
         "vo_light_tree_free": (None, [vp]),
@@ -331,22 +334,47 @@ def build_scene_tree(vertices, faces, ids, weights):
     return tree
 
 
-def render(scene: "Scene", materials, frame: dict, sample_index0: int, samples: int, pixels=None, precision="f32",
-           threads=0, light_tree=True):
-    """Mean radiance per pixel (pixels: flat indices, default all) -> (len(pixels), 3) float64."""
-    lib = load()
+def _render_inputs(scene, materials, frame, light_tree):
     _, (icdf, cmf, basis) = spectral_data()
     rec, weights = material_records(materials)
     ids, cdf, total = emitter_list(scene.vertices, scene.faces, weights)
     tree = build_scene_tree(scene.vertices, scene.faces, ids, weights) if light_tree else None
-    w, h = frame["width"], frame["height"]
-    pix = np.arange(w * h, dtype=np.uint32) if pixels is None else np.ascontiguousarray(pixels, np.uint32)
+    keep = (rec, ids, cdf, icdf, cmf, basis, tree)
     rs = RenderScene(ctypes.cast(ctypes.pointer(scene._view), ctypes.c_void_p), _p(rec), len(rec), _p(ids), _p(cdf),
                      len(ids), total, _p(icdf), _p(cmf), _p(basis),
                      ctypes.cast(ctypes.pointer(tree.view), ctypes.c_void_p) if tree else None)
-    fr = Frame(w, h, (ctypes.c_float * 3)(*frame["eye"]), (ctypes.c_float * 3)(*frame["target"]), frame["fov"],
-               frame["max_depth"], frame.get("sampling_mode", 1))
+    fr = Frame(frame["width"], frame["height"], (ctypes.c_float * 3)(*frame["eye"]),
+               (ctypes.c_float * 3)(*frame["target"]), frame["fov"], frame["max_depth"], frame.get("sampling_mode", 1))
+    return rs, fr, keep
+
+
+def render(scene: "Scene", materials, frame: dict, sample_index0: int, samples: int, pixels=None, precision="f32",
+           threads=0, light_tree=True):
+    """Mean radiance per pixel (pixels: flat indices, default all) -> (len(pixels), 3) float64."""
+    lib = load()
+    rs, fr, keep = _render_inputs(scene, materials, frame, light_tree)
+    w, h = frame["width"], frame["height"]
+    pix = np.arange(w * h, dtype=np.uint32) if pixels is None else np.ascontiguousarray(pixels, np.uint32)
     out = np.zeros((len(pix), 3), np.float64)
     fn = lib.vo_render_f64 if precision == "f64" else lib.vo_render_f32
     fn(ctypes.byref(rs), ctypes.byref(fr), sample_index0, samples, _p(pix), len(pix), threads, _p(out))
+    del keep
     return out
+
+
+def render_restir(scene: "Scene", materials, frame: dict, sample_index0: int, samples: int, candidates=4, eyes=None,
+                  precision="f32", threads=0, light_tree=True):
+    """ReSTIR DI frames (sampling mode 0): per-pixel mean over `samples` consecutive samples, the
+    first without history -> (height, width, 3) float64.  eyes: None or (samples, 3) cameras."""
+    lib = load()
+    rs, fr, keep = _render_inputs(scene, materials, dict(frame, sampling_mode=0), light_tree)
+    w, h = frame["width"], frame["height"]
+    ep = None
+    if eyes is not None:
+        eyes = np.ascontiguousarray(eyes, np.float32).reshape(samples, 3)
+        ep = _p(eyes)
+    out = np.zeros((h * w, 3), np.float64)
+    fn = lib.vo_render_restir_f64 if precision == "f64" else lib.vo_render_restir_f32
+    fn(ctypes.byref(rs), ctypes.byref(fr), sample_index0, samples, candidates, ep, threads, _p(out))
+    del keep
+    return out.reshape(h, w, 3)

@@ -20,11 +20,13 @@
 #include <math.h>
 #include <omp.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "oracle.h"
 
 #ifndef REAL
+#define VO_F32_UNIT 1
 #define REAL float
 #define SUFFIX(name) name##_f32
 #define RSQRT sqrtf
@@ -178,7 +180,7 @@ static real imported_geometry(v3 p, v3 pos, const tri_t *t) {
 }
 static float f_of(real x) { return (float)x; }
 
-typedef struct { v3 p, n, ng; real error; int front; uint32_t slot; } surf_t;
+typedef struct { v3 p, n, ng; real error, t; int front; uint32_t slot; } surf_t;
 /* Closest hit through the oracle's reference traversal; returns 0 on a miss. */
 static int trace(const vo_render_scene *sc, v3 o, v3 d, real tmin, surf_t *s, uint32_t *face_out) {
     float fo[3] = {f_of(o.x), f_of(o.y), f_of(o.z)}, fd[3] = {f_of(d.x), f_of(d.y), f_of(d.z)};
@@ -196,6 +198,7 @@ static int trace(const vo_render_scene *sc, v3 o, v3 d, real tmin, surf_t *s, ui
     if (dot(s->n, s->ng) < 0) s->n = neg(s->n);
     s->error = RFMAX((real)1e-7, (real)1.907349e-6 * RFMAX(maxabs(t.a), RFMAX(maxabs(t.b), maxabs(t.c))));
     s->slot = t.tag;
+    s->t = (real)h.t;
     *face_out = (uint32_t)h.face;
     return 1;
 }
@@ -205,19 +208,21 @@ static int occluded(const vo_render_scene *sc, v3 o, v3 d, real tmin, real tmax)
                           : vo_brute_any(sc->geo, fo, fd, f_of(tmin), f_of(tmax));
 }
 
-/* Next-event estimate at surface s: draws 4 numbers, returns the contribution if visible. */
-static v3 nee(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, const mat_t *m, v3 wo, s4 thr,
-              int use_mis, uint32_t *seed) {
-    v3 zero = V(0, 0, 0);
-    if (sc->emitter_count == 0) return zero;
-    if (!(vo_rand_f(seed) < 1.0f)) return zero;  /* importedProbability = 1 */
+/* sample_direct_light, imported branch (main.swift:2888-2917): draws 4 numbers; pdf is the
+ * solid-angle pdf (0: no sample).  light = face + 2 (LightSample.isDirectional). */
+typedef struct { v3 position, wi, emission; real pdf; uint32_t light; } light_sample;
+static light_sample sample_direct_light(const vo_render_scene *sc, v3 p, uint32_t *seed) {
+    light_sample ls;
+    memset(&ls, 0, sizeof ls);
+    if (sc->emitter_count == 0) return ls;
+    if (!(vo_rand_f(seed) < 1.0f)) return ls;  /* importedProbability = 1 */
     real value = (real)vo_rand_f(seed), ry = (real)vo_rand_f(seed), rz = (real)vo_rand_f(seed);
     uint32_t face;
     float tree_pmf = 0.0f;
     if (sc->tree) {
-        float fp[3] = {f_of(s->p.x), f_of(s->p.y), f_of(s->p.z)};
+        float fp[3] = {f_of(p.x), f_of(p.y), f_of(p.z)};
         uint32_t position = vo_light_tree_sample(sc->tree, fp, f_of(value), &tree_pmf);
-        if (position == 0xffffffffu) return zero;
+        if (position == 0xffffffffu) return ls;
         face = sc->tree->leaves[3 * (size_t)position];
     } else {
         uint32_t low = 0, high = sc->emitter_count - 1;
@@ -230,12 +235,14 @@ static v3 nee(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, cons
     }
     tri_t t = tri_of(sc, face);
     real root = RSQRT(ry);
-    v3 pos = add(add(mulv(t.a, (real)1 - root), mulv(t.b, root * ((real)1 - rz))), mulv(t.c, root * rz));
-    v3 delta = sub(pos, s->p);
+    ls.position = add(add(mulv(t.a, (real)1 - root), mulv(t.b, root * ((real)1 - rz))), mulv(t.c, root * rz));
+    v3 delta = sub(ls.position, p);
     real dist = len(delta);
-    v3 wi = mulv(delta, (real)1 / RFMAX(dist, (real)1e-8));
-    real geometry = imported_geometry(s->p, pos, &t);
+    ls.wi = mulv(delta, (real)1 / RFMAX(dist, (real)1e-8));
+    real geometry = imported_geometry(p, ls.position, &t);
     mat_t em = mat_of(sc, t.tag);
+    ls.emission = em.emission;
+    ls.light = face + 2u;
     real area_pdf;
     if (sc->tree) {
         real area = (real)0.5 * len(cross(sub(t.b, t.a), sub(t.c, t.a)));
@@ -243,33 +250,62 @@ static v3 nee(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, cons
     } else {
         area_pdf = sc->emitter_total > 0 ? em.weight / (real)sc->emitter_total : 0;
     }
-    real pdf = geometry > (real)1e-12 ? area_pdf / geometry : 0;
-    if (!(pdf > 0)) return zero;
-    /* light_visible */
+    ls.pdf = geometry > (real)1e-12 ? area_pdf / geometry : 0;
+    return ls;
+}
+
+/* light_visible (main.swift:3254-3264): shadow ray from the offset origin to the point, stopped
+ * short by the endpoint tolerance. */
+static int light_visible(const vo_render_scene *sc, const surf_t *s, v3 wi, v3 pos) {
     v3 o = ray_origin(s->p, s->ng, wi, s->error);
     v3 dd = sub(pos, o);
     real d = len(dd), tol = endpoint_tolerance(o, pos, d);
-    if (d <= tol) return zero;
-    if (occluded(sc, o, norm(dd), ray_t_min(o), d - tol)) return zero;
-    /* eval_bsdf_with_pdf, DIFFUSE */
-    real bpdf = dot(s->n, wo) <= 0 ? 0 : RFMAX((real)0, dot(s->n, wi)) / PI_R;
-    real cosine = RFABS(dot(s->n, wi));
-    if (!(dot(wi, s->ng) > 0) || cosine < (real)1e-7 || dot(s->n, wo) <= 0 || !(dot(s->n, wi) > 0)) return zero;
-    s4 f = sscale(reflectance(sc, l, m->lagrange), (real)1 / PI_R);
-    real w = use_mis ? power_heuristic(pdf, bpdf) : 1;
-    s4 le = emission_spectrum(sc, l, em.emission);
-    return spectrum_rgb(sc, l, sscale(smul(smul(thr, f), le), cosine * w / pdf));
+    if (d <= tol) return 0;
+    return !occluded(sc, o, norm(dd), ray_t_min(o), d - tol);
 }
 
-/* One sample of one pixel: the reference's shading_kernel in MIS mode (samplingMode 1). */
-static v3 sample_pixel(const vo_render_scene *sc, const vo_frame *fr, const real *camera, uint32_t gx, uint32_t gy,
-                       uint32_t sample_index) {
-    const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
-    uint32_t seed = (gy * fr->width + gx) ^ (sample_index * 1999999973u);
+/* Next-event estimate at surface s: draws 4 numbers, returns the contribution if visible. */
+static v3 nee(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, const mat_t *m, v3 wo, s4 thr,
+              int use_mis, uint32_t *seed) {
+    v3 zero = V(0, 0, 0);
+    light_sample ls = sample_direct_light(sc, s->p, seed);
+    if (!(ls.pdf > 0)) return zero;
+    if (!light_visible(sc, s, ls.wi, ls.position)) return zero;
+    /* eval_bsdf_with_pdf, DIFFUSE */
+    real bpdf = dot(s->n, wo) <= 0 ? 0 : RFMAX((real)0, dot(s->n, ls.wi)) / PI_R;
+    real cosine = RFABS(dot(s->n, ls.wi));
+    if (!(dot(ls.wi, s->ng) > 0) || cosine < (real)1e-7 || dot(s->n, wo) <= 0 || !(dot(s->n, ls.wi) > 0)) return zero;
+    s4 f = sscale(reflectance(sc, l, m->lagrange), (real)1 / PI_R);
+    real w = use_mis ? power_heuristic(ls.pdf, bpdf) : 1;
+    s4 le = emission_spectrum(sc, l, ls.emission);
+    return spectrum_rgb(sc, l, sscale(smul(smul(thr, f), le), cosine * w / ls.pdf));
+}
+
+/* ------------------------------------------------------------------ camera and pixel streams */
+typedef struct { v3 eye, fw, rt, up; real aspect, fov_scale; } cam_t;
+
+/* Camera basis (host side of the reference, main.swift:11303-11346, and pass 2's frame). */
+static cam_t camera_of(const vo_frame *fr, const float *eye) {
+    cam_t c;
+    c.eye = V(eye[0], eye[1], eye[2]);
+    c.fw = norm(sub(V(fr->target[0], fr->target[1], fr->target[2]), c.eye));
+    c.rt = norm(cross(c.fw, V(0, 1, 0)));
+    c.up = cross(c.rt, c.fw);
+    c.aspect = (real)fr->width / (real)fr->height;
+    c.fov_scale = RTAN((real)(fr->fov * 0.5f) * PI_R / (real)180);
+    return c;
+}
+/* pixel_sampler (main.swift:508-515). */
+static uint32_t pixel_seed(const vo_frame *fr, uint32_t gx, uint32_t gy, uint32_t sample_index) {
+    return (gy * fr->width + gx) ^ (sample_index * 1999999973u);
+}
+/* wavelength_numbers (main.swift:2218-2230): the pixel's wavelength number u, no draw. */
+static real pixel_u(uint32_t seed) {
     uint32_t h = vo_pcg_hash(seed ^ 0x5bd1e995u);
-    real wl = (real)(h >> 8u) * (real)(1.0 / 16777216.0);
-    lanes_t l = lanes_of(sc, wl);
-    /* frameJitter */
+    return (real)(h >> 8u) * (real)(1.0 / 16777216.0);
+}
+/* Camera ray through the jittered pixel (main.swift:7420-7436; frameJitter :8773-8786). */
+static v3 camera_dir(const cam_t *c, const vo_frame *fr, uint32_t gx, uint32_t gy, uint32_t sample_index) {
     uint32_t jn = ((sample_index > 1u ? sample_index : 1u) - 1u) % 1024u + 1u;
     float jx = 0, jy = 0;
     {
@@ -285,33 +321,34 @@ static v3 sample_pixel(const vo_render_scene *sc, const vo_frame *fr, const real
     real u = (((real)gx + (real)(jx + 0.5f)) / (real)fr->width) * (real)2 - (real)1;
     real v = (((real)gy + (real)(jy + 0.5f)) / (real)fr->height) * (real)2 - (real)1;
     v = -v;
-    u *= camera[12] * camera[13];
-    v *= camera[13];
-    v3 eye = V(camera[0], camera[1], camera[2]), fw = V(camera[3], camera[4], camera[5]),
-       rt = V(camera[6], camera[7], camera[8]), up = V(camera[9], camera[10], camera[11]);
-    v3 dir = norm(add(add(fw, mulv(rt, u)), mulv(up, v)));
-    seed = vo_pcg_hash(seed ^ 0x27d4eb2fu);
+    u *= c->aspect * c->fov_scale;
+    v *= c->fov_scale;
+    return norm(add(add(c->fw, mulv(c->rt, u)), mulv(c->up, v)));
+}
 
+/* The path after its primary vertex s (main.swift:7712-7828): BSDF continuation, emission with
+ * emission_weight, NEE with MIS at every later vertex, roulette after bounce 3.  In ReSTIR mode
+ * (samplingMode 0) the departure from a diffuse primary is not MIS-weighted: its DI reservoir
+ * estimated the whole direct integral (main.swift:7725-7728). */
+static v3 continue_path(const vo_render_scene *sc, const vo_frame *fr, const lanes_t *l, surf_t s, mat_t m, v3 dir,
+                        uint32_t *seed) {
+    const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
     v3 radiance = V(0, 0, 0);
-    surf_t s;
-    uint32_t face;
-    if (!trace(sc, eye, dir, ray_t_min(eye), &s, &face)) return radiance;
-    mat_t m = mat_of(sc, s.slot);
-    if (m.type == 3) return s.front ? m.emission : V(0, 0, 0);
     s4 thr = S4(1);
-    radiance = nee(sc, &l, &s, &m, neg(dir), thr, fr->sampling_mode <= 1u && limit > 0, &seed);
     v3 cur_dir = dir;
     int sdepth = 0;
+    uint32_t face;
     for (int bounce = 1;; ++bounce) {
         /* Departure from s (Lambert: never delta). */
         if (sdepth >= limit) break;
         ++sdepth;
-        vo_rand_f(&seed);
-        real r1 = (real)vo_rand_f(&seed), r2 = (real)vo_rand_f(&seed);
+        int prev_mis = fr->sampling_mode == 1u || (fr->sampling_mode == 0u && (bounce > 1 || m.type != 0u));
+        vo_rand_f(seed);
+        real r1 = (real)vo_rand_f(seed), r2 = (real)vo_rand_f(seed);
         v3 nd = cosine_hemisphere(s.n, r1, r2);
         real bpdf = RFMAX((real)0, dot(s.n, nd)) / PI_R;
         if (!(bpdf > 0 && dot(nd, s.ng) > 0)) break;
-        thr = smul(thr, reflectance(sc, &l, m.lagrange));
+        thr = smul(thr, reflectance(sc, l, m.lagrange));
         if (!(isfinite(thr.v[0]) && isfinite(thr.v[1]) && isfinite(thr.v[2]) && isfinite(thr.v[3])) || smax(thr) <= 0) break;
         v3 prev = s.p;
         v3 o = ray_origin(s.p, s.ng, nd, s.error);
@@ -340,48 +377,298 @@ static v3 sample_pixel(const vo_render_scene *sc, const vo_frame *fr, const real
             } else if (sc->emitter_count && sc->emitter_total > 0 && rm.weight > 0 && g > (real)1e-12) {
                 light_pdf = (rm.weight / (real)sc->emitter_total) / g;
             }
-            real w = emission_weight(0, fr->sampling_mode != 3u, fr->sampling_mode == 1u, bpdf, light_pdf);
+            real w = emission_weight(0, fr->sampling_mode != 3u, prev_mis, bpdf, light_pdf);
             v3 e = r.front ? rm.emission : V(0, 0, 0);
-            radiance = add(radiance, spectrum_rgb(sc, &l, sscale(smul(thr, emission_spectrum(sc, &l, e)), w)));
+            radiance = add(radiance, spectrum_rgb(sc, l, sscale(smul(thr, emission_spectrum(sc, l, e)), w)));
             break;
         }
-        radiance = add(radiance, nee(sc, &l, &r, &rm, neg(cur_dir), thr, fr->sampling_mode <= 1u && sdepth < limit, &seed));
+        radiance = add(radiance, nee(sc, l, &r, &rm, neg(cur_dir), thr, fr->sampling_mode <= 1u && sdepth < limit, seed));
         s = r;
         m = rm;
         if (bounce > 3) {
             real survival = RFMIN(RFMAX(smax(thr), (real)0.05), (real)0.95);
-            if ((real)vo_rand_f(&seed) >= survival) break;
+            if ((real)vo_rand_f(seed) >= survival) break;
             thr = sscale(thr, (real)1 / survival);
         }
     }
+    return radiance;
+}
+
+/* One sample of one pixel: the reference's shading_kernel in MIS mode (samplingMode 1). */
+static v3 sample_pixel(const vo_render_scene *sc, const vo_frame *fr, const cam_t *cam, uint32_t gx, uint32_t gy,
+                       uint32_t sample_index) {
+    const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
+    uint32_t seed = pixel_seed(fr, gx, gy, sample_index);
+    lanes_t l = lanes_of(sc, pixel_u(seed));
+    v3 dir = camera_dir(cam, fr, gx, gy, sample_index);
+    seed = vo_pcg_hash(seed ^ 0x27d4eb2fu);  /* decorrelate_shading_seed (main.swift:106-108) */
+
+    v3 radiance = V(0, 0, 0);
+    surf_t s;
+    uint32_t face;
+    if (!trace(sc, cam->eye, dir, ray_t_min(cam->eye), &s, &face)) return radiance;
+    mat_t m = mat_of(sc, s.slot);
+    if (m.type == 3) return s.front ? m.emission : V(0, 0, 0);
+    radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), fr->sampling_mode <= 1u && limit > 0, &seed);
+    radiance = add(radiance, continue_path(sc, fr, &l, s, m, dir, &seed));
     if (!(isfinite(radiance.x) && isfinite(radiance.y) && isfinite(radiance.z))) radiance = V(0, 0, 0);
     return radiance;
 }
 
 void SUFFIX(vo_render)(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
                        const uint32_t *pixels, uint32_t npixels, int threads, double *out_rgb) {
-    /* Camera basis (host side of the reference, main.swift:11303-11346, and pass 2's frame). */
-    real camera[14];
-    v3 eye = V(fr->eye[0], fr->eye[1], fr->eye[2]);
-    v3 fw = norm(sub(V(fr->target[0], fr->target[1], fr->target[2]), eye));
-    v3 rt = norm(cross(fw, V(0, 1, 0)));
-    v3 up = cross(rt, fw);
-    camera[0] = eye.x; camera[1] = eye.y; camera[2] = eye.z;
-    camera[3] = fw.x; camera[4] = fw.y; camera[5] = fw.z;
-    camera[6] = rt.x; camera[7] = rt.y; camera[8] = rt.z;
-    camera[9] = up.x; camera[10] = up.y; camera[11] = up.z;
-    camera[12] = (real)fr->width / (real)fr->height;
-    camera[13] = RTAN((real)(fr->fov * 0.5f) * PI_R / (real)180);
+    const cam_t cam = camera_of(fr, fr->eye);
 #pragma omp parallel for schedule(dynamic, 4) num_threads(threads > 0 ? threads : omp_get_max_threads())
     for (int64_t k = 0; k < (int64_t)npixels; ++k) {
         uint32_t pix = pixels[k], gx = pix % fr->width, gy = pix / fr->width;
         double acc[3] = {0, 0, 0};
         for (uint32_t s = 0; s < samples; ++s) {
-            v3 c = sample_pixel(sc, fr, camera, gx, gy, sample_index0 + s);
+            v3 c = sample_pixel(sc, fr, &cam, gx, gy, sample_index0 + s);
             acc[0] += (double)c.x;
             acc[1] += (double)c.y;
             acc[2] += (double)c.z;
         }
         for (int q = 0; q < 3; ++q) out_rgb[3 * k + q] = acc[q] / (double)samples;
     }
+}
+
+/* ------------------------------------------------------------------ ReSTIR DI (samplingMode 0)
+ * Pass 1 (restir_temporal_kernel, main.swift:4842-4940): G-buffer, RIS over `candidates` light
+ * samples (restir_di_initial :4637-4668), temporal merge through backprojection
+ * (restir_backproject :4828-4835, restir_same_surface :4816-4823, restir_di_merge :4692-4704),
+ * encode (restir_di_encode :4708-4716).  Pass 2 (shading_kernel :7484-7580): four uniform spatial
+ * taps in a 16-pixel box (uniform_neighbor :5025-5027, restir2020_compatible :5030-5033), final W,
+ * shading with one visibility ray; then the path suffix.  The stored texels hold what the
+ * reference's textures hold: rgba32f G-buffer position, rgba16f normal/material type, rgba32f
+ * reservoirs. */
+typedef struct {
+    float gpos[4];      /* position, camera distance (w = -1: no hit) */
+    float gnorm[4];     /* normal, material type, each rounded to half precision */
+    float pos[3];       /* sample position */
+    uint32_t light;     /* face + 2; 0 = no sample */
+    float emit_pdf[4];  /* emission rgb, solid-angle pdf */
+    float weights[4];   /* weight sum, M, W, wavelength number */
+} di_texel;
+typedef struct { surf_t s; v3 dir; int hit; } primary_t;
+typedef struct { v3 position, emission; uint32_t light; real pdf, u; } di_sample;
+
+/* Round to the nearest half-precision value (ties to even), as an rgba16f store does. */
+float vo_half_round(float x);
+static float half_round(float x) { return vo_half_round(x); }
+#ifdef VO_F32_UNIT
+float vo_half_round(float x) {
+    if (x == 0.0f || !isfinite(x)) return x;
+    float ax = fabsf(x);
+    if (ax >= 65520.0f) return copysignf(INFINITY, x);
+    int e;
+    frexpf(ax, &e);
+    int lead = e - 1;
+    if (lead < -14) lead = -14;
+    float q = ldexpf(1.0f, lead - 10);
+    return rintf(x / q) * q;
+}
+#endif
+
+/* light_geometry for an imported emitter (main.swift:3165-3166). */
+static real light_geometry(const vo_render_scene *sc, v3 p, const di_sample *c) {
+    if (c->light < 2u) return 0;
+    tri_t t = tri_of(sc, c->light - 2u);
+    return imported_geometry(p, c->position, &t);
+}
+/* eval_bsdf for DIFFUSE (main.swift:2659-2665). */
+static s4 diffuse_f(const vo_render_scene *sc, const lanes_t *l, const mat_t *m, const surf_t *s, v3 wo, v3 wi) {
+    if (dot(wi, s->ng) <= 0) return S4(0);
+    real c = RFABS(dot(s->n, wi));
+    if (c < (real)1e-7 || dot(s->n, wo) <= 0) return S4(0);
+    return dot(s->n, wi) > 0 ? sscale(reflectance(sc, l, m->lagrange), (real)1 / PI_R) : S4(0);
+}
+/* eval_restir_target_pdf (main.swift:3187-3196): |rgb(f L cos)| G at the sample's own wavelengths. */
+static real target_pdf(const vo_render_scene *sc, const surf_t *s, const mat_t *m, v3 raydir, const di_sample *c) {
+    if (c->pdf <= 0) return 0;
+    v3 dir = norm(sub(c->position, s->p));
+    real ct = RFMAX((real)0, dot(s->n, dir));
+    if (ct <= 0) return 0;
+    lanes_t l = lanes_of(sc, c->u);
+    s4 f = diffuse_f(sc, &l, m, s, neg(raydir), dir);
+    v3 rgb = spectrum_rgb(sc, &l, sscale(smul(f, emission_spectrum(sc, &l, c->emission)), ct));
+    return len(rgb) * light_geometry(sc, s->p, c);
+}
+static di_sample stored_sample(const di_texel *t) {
+    di_sample c;
+    c.position = ld(t->pos);
+    c.light = t->light;
+    c.emission = ld(t->emit_pdf);
+    c.pdf = (real)t->emit_pdf[3];
+    c.u = (real)t->weights[3];
+    return c;
+}
+/* restir_backproject with the previous sample's camera: the pixel whose unjittered frustum holds p. */
+static int backproject(const cam_t *prev, const vo_frame *fr, v3 p, int *px, int *py) {
+    v3 d = sub(p, prev->eye);
+    real w = dot(d, prev->fw);
+    if (!(w > 0)) return 0;
+    real x = dot(d, prev->rt) / (w * prev->aspect * prev->fov_scale);
+    real y = dot(d, prev->up) / (w * prev->fov_scale);
+    real ux = x * (real)0.5 + (real)0.5, uy = y * (real)-0.5 + (real)0.5;
+    if (!(ux >= 0 && uy >= 0 && ux < 1 && uy < 1)) return 0;
+    *px = (int)(ux * (real)fr->width);
+    *py = (int)(uy * (real)fr->height);
+    return *px >= 0 && *px < (int)fr->width && *py >= 0 && *py < (int)fr->height;
+}
+
+static void restir_pass1(const vo_render_scene *sc, const vo_frame *fr, const cam_t *cam, const cam_t *prev,
+                         int temporal, uint32_t candidates, uint32_t gx, uint32_t gy, uint32_t sample_index,
+                         const di_texel *hist, di_texel *cur, primary_t *prim) {
+    const uint32_t pix = gy * fr->width + gx;
+    uint32_t seed = pixel_seed(fr, gx, gy, sample_index);
+    const real u = pixel_u(seed);
+    const v3 dir = camera_dir(cam, fr, gx, gy, sample_index);
+    di_texel *o = cur + pix;
+    memset(o, 0, sizeof *o);
+    primary_t *pr = prim + pix;
+    uint32_t face;
+    pr->dir = dir;
+    pr->hit = trace(sc, cam->eye, dir, ray_t_min(cam->eye), &pr->s, &face);
+    if (!pr->hit) {
+        o->gpos[3] = -1.0f;
+        o->gnorm[3] = -1.0f;
+        return;
+    }
+    const surf_t *s = &pr->s;
+    mat_t m = mat_of(sc, s->slot);
+    o->gpos[0] = (float)s->p.x; o->gpos[1] = (float)s->p.y; o->gpos[2] = (float)s->p.z; o->gpos[3] = (float)s->t;
+    o->gnorm[0] = half_round((float)s->n.x);
+    o->gnorm[1] = half_round((float)s->n.y);
+    o->gnorm[2] = half_round((float)s->n.z);
+    o->gnorm[3] = half_round((float)m.type);
+    if (m.type != 0u) return;
+    di_sample sel;
+    memset(&sel, 0, sizeof sel);
+    real wsum = 0, M = 0;
+    for (uint32_t i = 0; i < candidates; ++i) {
+        light_sample ls = sample_direct_light(sc, s->p, &seed);
+        di_sample c = {ls.position, ls.emission, ls.light, ls.pdf, u};
+        if (vo_get_fault() != 4 || c.pdf > 0) M += 1;
+        if (c.pdf > 0) {
+            real ph = target_pdf(sc, s, &m, dir, &c);
+            real proposal = c.pdf * light_geometry(sc, s->p, &c);
+            real w = proposal > 0 ? ph / proposal : 0;
+            wsum += w;
+            if ((real)vo_rand_f(&seed) * wsum < w) sel = c;
+        }
+    }
+    int px, py;
+    if (temporal && backproject(prev, fr, s->p, &px, &py)) {
+        const di_texel *h = hist + (size_t)py * fr->width + px;
+        real hM = (real)h->weights[1], hW = (real)h->weights[2];
+        v3 hp = V(h->gpos[0], h->gpos[1], h->gpos[2]);
+        v3 hn = V(h->gnorm[0], h->gnorm[1], h->gnorm[2]);
+        int same = h->gpos[3] > 0.0f && h->gnorm[3] == (float)m.type && dot(hn, s->n) > (real)0.95 &&
+                   len(sub(hp, s->p)) < RFMAX((real)0.01, s->t * (real)0.01);
+        if (same && hM > 0 && hW > 0) {
+            di_sample hs = stored_sample(h);
+            real ph = target_pdf(sc, s, &m, dir, &hs);
+            real cm = RFMIN(hM, (real)20);
+            real w = ph * hW * cm;
+            M += cm;
+            wsum += w;
+            if ((real)vo_rand_f(&seed) * wsum < w) sel = hs;
+        }
+    }
+    real ph = target_pdf(sc, s, &m, dir, &sel);
+    real W = (M > 0 && ph > 0) ? wsum / (M * ph) : 0;
+    o->pos[0] = (float)sel.position.x; o->pos[1] = (float)sel.position.y; o->pos[2] = (float)sel.position.z;
+    o->light = sel.light;
+    o->emit_pdf[0] = (float)sel.emission.x; o->emit_pdf[1] = (float)sel.emission.y;
+    o->emit_pdf[2] = (float)sel.emission.z; o->emit_pdf[3] = (float)sel.pdf;
+    o->weights[0] = (float)wsum; o->weights[1] = (float)M; o->weights[2] = (float)W; o->weights[3] = (float)sel.u;
+}
+
+static v3 restir_pass2(const vo_render_scene *sc, const vo_frame *fr, uint32_t gx, uint32_t gy, uint32_t sample_index,
+                       const di_texel *cur, const primary_t *prim) {
+    const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
+    const uint32_t pix = gy * fr->width + gx;
+    uint32_t seed = pixel_seed(fr, gx, gy, sample_index);
+    lanes_t l = lanes_of(sc, pixel_u(seed));
+    seed = vo_pcg_hash(seed ^ 0x27d4eb2fu);
+    const di_texel *c = cur + pix;
+    v3 radiance = V(0, 0, 0);
+    if (c->gpos[3] <= 0.0f) return radiance;
+    const surf_t s = prim[pix].s;
+    const v3 dir = prim[pix].dir;
+    mat_t m = mat_of(sc, s.slot);
+    if (m.type == 3) return s.front ? m.emission : V(0, 0, 0);
+    if (m.type == 0u) {
+        di_sample sel = stored_sample(c);
+        real wsum = target_pdf(sc, &s, &m, dir, &sel) * (real)c->weights[2] * (real)c->weights[1];
+        real M = (real)c->weights[1];
+        for (int i = 0; i < 4; ++i) {
+            real rx = (real)vo_rand_f(&seed), ry = (real)vo_rand_f(&seed);
+            int nx = (int)gx + (int)((rx * (real)2 - (real)1) * (real)16);
+            int ny = (int)gy + (int)((ry * (real)2 - (real)1) * (real)16);
+            if (nx < 0 || nx >= (int)fr->width || ny < 0 || ny >= (int)fr->height) continue;
+            const di_texel *n = cur + (size_t)ny * fr->width + nx;
+            v3 nn = V(n->gnorm[0], n->gnorm[1], n->gnorm[2]);
+            int compatible = n->gpos[3] > 0.0f && n->gnorm[3] == (float)m.type && dot(s.n, nn) > (real)0.95 &&
+                             RFABS((real)c->gpos[3] - (real)n->gpos[3]) < (real)0.05 * (real)c->gpos[3];
+            if (!compatible || !(n->weights[1] > 0.0f && n->weights[2] > 0.0f)) continue;
+            di_sample ns = stored_sample(n);
+            real w = target_pdf(sc, &s, &m, dir, &ns) * (real)n->weights[2] * (real)n->weights[1];
+            wsum += w;
+            if (vo_get_fault() != 3) M += (real)n->weights[1];
+            if ((real)vo_rand_f(&seed) * wsum < w) sel = ns;
+        }
+        real fp = target_pdf(sc, &s, &m, dir, &sel);
+        real W = (M > 0 && fp > 0) ? wsum / (M * fp) : 0;
+        if (W > 0) {
+            v3 wi = norm(sub(sel.position, s.p));
+            if (light_visible(sc, &s, wi, sel.position)) {
+                real ct = RFMAX((real)0, dot(s.n, wi));
+                lanes_t sl = lanes_of(sc, sel.u);
+                s4 f = diffuse_f(sc, &sl, &m, &s, neg(dir), wi);
+                s4 v = sscale(sscale(smul(sscale(f, ct), emission_spectrum(sc, &sl, sel.emission)), W),
+                              light_geometry(sc, s.p, &sel));
+                radiance = add(radiance, spectrum_rgb(sc, &sl, v));
+            }
+        }
+    } else {
+        radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), limit > 0, &seed);
+    }
+    radiance = add(radiance, continue_path(sc, fr, &l, s, m, dir, &seed));
+    if (!(isfinite(radiance.x) && isfinite(radiance.y) && isfinite(radiance.z))) radiance = V(0, 0, 0);
+    return radiance;
+}
+
+void SUFFIX(vo_render_restir)(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
+                              uint32_t candidates, const float *eyes, int threads, double *out_rgb) {
+    const size_t npix = (size_t)fr->width * fr->height;
+    di_texel *buf[2] = {calloc(npix, sizeof(di_texel)), calloc(npix, sizeof(di_texel))};
+    primary_t *prim = calloc(npix, sizeof(primary_t));
+    double *acc = calloc(3 * npix, sizeof(double));
+    if (!buf[0] || !buf[1] || !prim || !acc) {
+        free(buf[0]); free(buf[1]); free(prim); free(acc);
+        return;
+    }
+    const int nt = threads > 0 ? threads : omp_get_max_threads();
+    cam_t prev = camera_of(fr, fr->eye);
+    for (uint32_t k = 0; k < samples; ++k) {
+        const uint32_t si = sample_index0 + k;
+        const cam_t cam = camera_of(fr, eyes ? eyes + 3 * (size_t)k : fr->eye);
+        di_texel *cur = buf[k & 1u];
+        const di_texel *hist = buf[(k + 1u) & 1u];
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+        for (int64_t p = 0; p < (int64_t)npix; ++p)
+            restir_pass1(sc, fr, &cam, &prev, k > 0, candidates, (uint32_t)(p % fr->width), (uint32_t)(p / fr->width),
+                         si, hist, cur, prim);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+        for (int64_t p = 0; p < (int64_t)npix; ++p) {
+            v3 c = restir_pass2(sc, fr, (uint32_t)(p % fr->width), (uint32_t)(p / fr->width), si, cur, prim);
+            acc[3 * p] += (double)c.x;
+            acc[3 * p + 1] += (double)c.y;
+            acc[3 * p + 2] += (double)c.z;
+        }
+        prev = cam;
+    }
+    for (size_t q = 0; q < 3 * npix; ++q) out_rgb[q] = samples ? acc[q] / (double)samples : 0.0;
+    free(buf[0]); free(buf[1]); free(prim); free(acc);
 }

@@ -165,9 +165,22 @@ void vo_render_f32(const vo_render_scene *sc, const vo_frame *fr, uint32_t sampl
 void vo_render_f64(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
                    const uint32_t *pixels, uint32_t npixels, int threads, double *out_rgb);
 
+/* ReSTIR DI (samplingMode 0, see integrator.c): `samples` consecutive samples rendered as frames, each
+ * pass 1 (RIS over `candidates` light samples + temporal reuse of the previous sample's reservoirs)
+ * then pass 2 (spatial reuse, shading, path suffix); the first sample has no history.  eyes: NULL
+ * (fixed camera) or 3 floats per sample.  out_rgb: width * height * 3 per-pixel means. */
+void vo_render_restir_f32(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
+                          uint32_t candidates, const float *eyes, int threads, double *out_rgb);
+void vo_render_restir_f64(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
+                          uint32_t candidates, const float *eyes, int threads, double *out_rgb);
+
 /* For tests that plant a mistake in the oracle (never the product): 0 = faithful,
- * 1 = drop the FMA tie-break of watertight_edge, 2 = swap b1/b2. */
+ * 1 = drop the FMA tie-break of watertight_edge, 2 = swap b1/b2, 3 = ReSTIR spatial reuse leaves
+ * the neighbours' confidence out of M, 4 = ReSTIR RIS counts only candidates with pdf > 0 in M. */
 void vo_set_fault(int fault);
+int vo_get_fault(void);
+/* Nearest half-precision value of x (ties to even): what an rgba16f texel store keeps. */
+float vo_half_round(float x);
 int vo_max_threads(void);
 
 #ifdef __cplusplus

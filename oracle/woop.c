@@ -8,6 +8,7 @@
 
 static int g_fault = 0;
 void vo_set_fault(int fault) { g_fault = fault; }
+int vo_get_fault(void) { return g_fault; }
 
 /* main.swift:91-95 pcg_hash: PCG-RXS-M-XS output of one LCG step. */
 uint32_t vo_pcg_hash(uint32_t input) {

@@ -111,3 +111,55 @@ def axis_rays(n, seed, box=1.2):
     r[3:6] = 0
     r[3 + axis, np.arange(n)] = sign
     return r
+
+
+CORNELL_MATS = [dict(type=0, albedo=(0.73, 0.73, 0.73)), dict(type=0, albedo=(0.65, 0.05, 0.05)),
+                dict(type=0, albedo=(0.12, 0.45, 0.15)), dict(type=3, albedo=(0, 0, 0), emission=(17, 17, 17))]
+
+
+def _ceiling_quad(x0, z0, s, y, tag, flip=False):
+    """Two triangles of an s x s square at height y; normal -y (facing down) unless flip."""
+    quad = np.array([[x0, y, z0], [x0 + s, y, z0 + s], [x0, y, z0 + s],
+                     [x0, y, z0], [x0 + s, y, z0], [x0 + s, y, z0 + s]], np.float32)
+    if flip:
+        quad = quad[[0, 2, 1, 3, 5, 4]]
+    v = np.zeros((6, 8), np.float32)
+    v[:, :3] = quad
+    v[:, 4] = 1.0 if flip else -1.0
+    return v, np.array([[0, 1, 2, tag], [3, 4, 5, tag]], np.uint32)
+
+
+def _append(base, parts, mats, extra_mats):
+    import vt
+    verts, faces = [base.vertices], [base.faces]
+    nv = len(base.vertices)
+    for v, f in parts:
+        f = f.copy()
+        f[:, :3] += nv
+        verts.append(v)
+        faces.append(f)
+        nv += len(v)
+    return vt.Mesh(np.concatenate(verts), np.concatenate(faces)), list(mats) + list(extra_mats)
+
+
+def multi_light_cornell():
+    """The Cornell box with a 0.25 ceiling light plus a 4 x 4 grid of small ceiling lights of
+    different intensities (tags 4-19)."""
+    import vt
+    parts, mats = [], []
+    k = 0
+    for i in range(4):
+        for j in range(4):
+            parts.append(_ceiling_quad(-0.8 + 0.45 * i, -0.8 + 0.45 * j, 0.12, 0.998, 4 + k))
+            e = 2.0 + 3.0 * ((i * 4 + j) % 5)
+            mats.append(dict(type=3, albedo=(0, 0, 0), emission=(e, e, e)))
+            k += 1
+    return _append(vt.cornell(0.25), parts, CORNELL_MATS, mats)
+
+
+def backfacing_light_cornell():
+    """The Cornell box plus a 0.25 emitter lying on the floor facing down (tag 4): it lights nothing
+    inside the box, so half of the power-sampled light candidates have pdf 0."""
+    import vt
+    return _append(vt.cornell(0.25), [_ceiling_quad(0.3, 0.3, 0.25, -0.998, 4)], CORNELL_MATS,
+                   [dict(type=3, albedo=(0, 0, 0), emission=(17, 17, 17))])

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
+import scenes
 import vt
 
 pytestmark = pytest.mark.gpu
@@ -90,27 +91,7 @@ def test_cornell_config1_full_size_mean_radiance(gpu_ctx):
 
 
 def _multi_light_cornell():
-    """The Cornell box plus a 4 x 4 grid of small ceiling lights of different intensities (tags 4-19)."""
-    base = vt.cornell(0.25)
-    verts, faces = [base.vertices], [base.faces]
-    nv = len(base.vertices)
-    mats = list(CORNELL_MATS)
-    k = 0
-    for i in range(4):
-        for j in range(4):
-            x0, z0, s, y = -0.8 + 0.45 * i, -0.8 + 0.45 * j, 0.12, 0.998
-            quad = np.array([[x0, y, z0], [x0 + s, y, z0 + s], [x0, y, z0 + s],
-                             [x0, y, z0], [x0 + s, y, z0], [x0 + s, y, z0 + s]], np.float32)
-            v = np.zeros((6, 8), np.float32)
-            v[:, :3] = quad
-            v[:, 4] = -1.0
-            verts.append(v)
-            faces.append(np.array([[nv, nv + 1, nv + 2, 4 + k], [nv + 3, nv + 4, nv + 5, 4 + k]], np.uint32))
-            nv += 6
-            e = 2.0 + 3.0 * ((i * 4 + j) % 5)
-            mats.append(dict(type=3, albedo=(0, 0, 0), emission=(e, e, e)))
-            k += 1
-    return vt.Mesh(np.concatenate(verts), np.concatenate(faces)), mats
+    return scenes.multi_light_cornell()
 
 
 @pytest.mark.parametrize("tree", [True, False])

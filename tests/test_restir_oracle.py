@@ -1,0 +1,129 @@
+"""CPU tests that pin the ReSTIR DI oracle (oracle/integrator.c, sampling mode 0): the half-precision
+G-buffer rounding against numpy, unbiasedness against the MIS tracer (itself pinned by closed forms)
+on scenes whose light sampling differs (one light, a light tree, power sampling with lights that
+face away), a closed-form direct-light value, camera motion through backprojection, and planted
+mistakes that those checks must catch."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import scenes
+import vt
+
+CORNELL = scenes.CORNELL_MATS
+
+
+def _frame(w, depth, **kw):
+    return dict(width=w, height=w, eye=vt.orbit_eye(), target=(0.0, -0.1, 0.0), fov=42.0, max_depth=depth, **kw)
+
+
+@pytest.fixture(autouse=True)
+def _faithful():
+    oracle.set_fault(0)
+    yield
+    oracle.set_fault(0)
+
+
+def test_half_round_matches_numpy_float16():
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.uniform(-1, 1, 20000), rng.normal(0, 1e-5, 5000), np.arange(-6, 7),
+                         # exact ties between neighbouring halves (ties to even)
+                         (np.arange(1024, 1100) + 0.5) / 1024.0, (np.arange(1, 60) + 0.5) * 2.0 ** -24,
+                         [0.95, -0.95, 65504.0, 65519.0, 1e-8, -0.0]]).astype(np.float32)
+    lib = oracle.load()
+    got = np.array([lib.vo_half_round(float(x)) for x in xs], np.float32)
+    want = xs.astype(np.float16).astype(np.float32)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("depth", [1, 3, 8])
+def test_restir_mean_equals_mis_single_light(depth):
+    # Same pixels, sample indices and primary jitter: the two unbiased estimators differ only in how
+    # the primary vertex's direct light is estimated.
+    mesh = vt.cornell()
+    sc = oracle.Scene(mesh.vertices, mesh.faces)
+    a = oracle.render(sc, CORNELL, _frame(48, depth), 1, 64).mean()
+    b = oracle.render_restir(sc, CORNELL, _frame(48, depth), 1, 64).mean()
+    assert abs(b / a - 1) < 2e-3, (a, b)
+
+
+@pytest.mark.parametrize("tree,candidates", [(True, 4), (False, 4), (True, 1), (True, 32)])
+def test_restir_mean_equals_mis_many_lights(tree, candidates):
+    mesh, mats = scenes.multi_light_cornell()
+    sc = oracle.Scene(mesh.vertices, mesh.faces)
+    a = oracle.render(sc, mats, _frame(48, 1), 1, 128, light_tree=tree).mean()
+    b = oracle.render_restir(sc, mats, _frame(48, 1), 1, 128, candidates=candidates, light_tree=tree).mean()
+    assert abs(b / a - 1) < 5e-3, (a, b)
+
+
+def test_zero_pdf_candidates_count_in_m():
+    # Power sampling picks the floor emitter (facing away from everything) half of the time; those
+    # candidates have pdf 0 yet count in M (main.swift:4654).  Reuse skips neighbour and history
+    # reservoirs whose W is 0 without adding their M (main.swift:4922, 7529-7530), as the reference
+    # does; with one reservoir in sixteen empty here, that raises the mean by about 1% over MIS.
+    mesh, mats = scenes.backfacing_light_cornell()
+    sc = oracle.Scene(mesh.vertices, mesh.faces)
+    a = oracle.render(sc, mats, _frame(48, 1), 1, 64, light_tree=False).mean()
+    b = oracle.render_restir(sc, mats, _frame(48, 1), 1, 64, light_tree=False).mean()
+    assert -2e-3 < b / a - 1 < 0.02, (a, b)
+    oracle.set_fault(4)                       # planted: M counts only candidates with pdf > 0
+    c = oracle.render_restir(sc, mats, _frame(48, 1), 1, 64, light_tree=False).mean()
+    assert c / b - 1 > 0.05, (b, c)
+
+
+def test_spatial_reuse_confidence_fault_is_caught():
+    mesh = vt.cornell()
+    sc = oracle.Scene(mesh.vertices, mesh.faces)
+    a = oracle.render(sc, CORNELL, _frame(48, 2), 1, 32).mean()
+    oracle.set_fault(3)                       # planted: neighbours' M left out of the normalisation
+    c = oracle.render_restir(sc, CORNELL, _frame(48, 2), 1, 32).mean()
+    assert c / a - 1 > 0.05, (a, c)
+
+
+def _rect_form_factor(X, Y):
+    a, b = X / math.sqrt(1 + X * X), Y / math.sqrt(1 + Y * Y)
+    return (a * math.atan(Y / math.sqrt(1 + X * X)) + b * math.atan(X / math.sqrt(1 + Y * Y))) / (2 * math.pi)
+
+
+def test_restir_direct_light_matches_form_factor_closed_form():
+    # One pixel seeing the floor point under a square D65 light (depth 1): every frame's reservoir
+    # reuses the previous one (temporal) and itself (the only in-bounds spatial tap), and the mean
+    # over frames is rho * L * F.
+    h, a = 1.0, 0.5
+    c, u, v = (np.float32(x) for x in ((-2, 0, -2), (0, 0, 4), (4, 0, 0)))
+    tris = [[c, c + u, c + u + v], [c, c + u + v, c + v]]
+    tris += [[np.float32([-a / 2, h, -a / 2]), np.float32([a / 2, h, a / 2]), np.float32([-a / 2, h, a / 2])],
+             [np.float32([-a / 2, h, -a / 2]), np.float32([a / 2, h, -a / 2]), np.float32([a / 2, h, a / 2])]]
+    vtx, f = scenes.mesh_from_triangles(np.array(tris, np.float32))
+    f[2:, 3] = 1
+    sc = oracle.Scene(vtx, f)
+    mats = [dict(type=0, albedo=(0.5, 0.5, 0.5)), dict(type=3, albedo=(0, 0, 0), emission=(17, 17, 17))]
+    frame = dict(width=1, height=1, eye=(0.0, 0.6, 0.0), target=(0.0, -1.0, 1e-4), fov=0.05, max_depth=1)
+    got = oracle.render_restir(sc, mats, frame, 1, 60000)[0, 0]
+    F = 4 * _rect_form_factor(a / 2 / h, a / 2 / h)
+    want = 0.5 * 17.0 * F
+    assert np.all(np.abs(got / want - 1) < 0.01), (got, want)
+
+
+def test_temporal_history_is_used_and_moving_camera_stays_unbiased():
+    mesh = vt.cornell()
+    sc = oracle.Scene(mesh.vertices, mesh.faces)
+    fr = _frame(32, 3)
+    # Sample 2 with sample 1 as history differs from sample 2 rendered alone.
+    two = oracle.render_restir(sc, CORNELL, fr, 1, 2)
+    first = oracle.render_restir(sc, CORNELL, fr, 1, 1)
+    alone = oracle.render_restir(sc, CORNELL, fr, 2, 1)
+    assert not np.allclose(2 * two - first, alone)
+    # A camera orbiting by 0.3 degrees per sample: backprojected history keeps the mean unbiased.
+    n = 48
+    base = np.asarray(vt.orbit_eye(), np.float64)
+    r = math.hypot(base[0], base[2])
+    ang0 = math.atan2(base[2], base[0])
+    eyes = np.array([[r * math.cos(ang0 + math.radians(0.3 * k)), base[1], r * math.sin(ang0 + math.radians(0.3 * k))]
+                     for k in range(n)], np.float32)
+    b = oracle.render_restir(sc, CORNELL, fr, 1, n, eyes=eyes).mean()
+    a = np.mean([oracle.render(sc, CORNELL, dict(fr, eye=tuple(map(float, eyes[k]))), 1 + k, 1).mean()
+                 for k in range(n)])
+    assert abs(b / a - 1) < 3e-3, (a, b)
