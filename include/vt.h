@@ -111,10 +111,16 @@ typedef struct vt_renderer vt_renderer;
 /* One material slot, selected by the face tag (MeshFace.w).  Replaces the per-slot
  * SurfaceSettings / emissions of MaterialLibrary (main.swift:2253-2263, 8926, 9002). */
 typedef struct vt_material {
-    uint32_t type;          /* MaterialType (main.swift:316-322): 0 DIFFUSE, 3 EMISSIVE */
-    float albedo[3];        /* linear sRGB reflectance, clamped to [0, 1] */
+    uint32_t type;          /* MaterialType (main.swift:316-322): 0 DIFFUSE (Lambert), 1 GLOSSY (metal:
+                               delta mirror below roughness 0.02, else OpenPBR with metalness 1),
+                               2 DIELECTRIC (delta glass, Schlick), 3 EMISSIVE, 4 OPENPBR */
+    float albedo[3];        /* linear sRGB reflectance (base colour / metal f0 / glass tint), clamped to [0, 1] */
     float emission[3];      /* linear sRGB radiance under the D65 illuminant; any > 0 makes the slot an emitter */
-    float roughness, ior, metalness, specular_weight, transmission;
+    float roughness;        /* specular roughness (OpenPBR clamps it to >= 0.03) */
+    float ior;              /* specular IOR (>= 1.01 for OpenPBR) */
+    float metalness;        /* OPENPBR base_metalness in [0, 1] */
+    float specular_weight;  /* MaterialX materials only; ignored (1 for GLOSSY / OPENPBR, main.swift:2525) */
+    float transmission;     /* OPENPBR transmission_weight in [0, 1] */
     float reserved[4];
 } vt_material;              /* 64 bytes */
 
@@ -143,7 +149,8 @@ typedef struct vt_frame_params {
     float fov_degrees;       /* cameraPos.w, vertical */
     uint32_t max_depth;      /* cameraTarget.w: path vertices, 1 = direct light only */
     uint32_t sampling_mode;  /* samplingMode: 0 = ReSTIR DI at the primary vertex (RIS, temporal and
-                                spatial reuse) + MIS suffix, 1 = MIS (NEE + BSDF, power heuristic) */
+                                spatial reuse) + MIS suffix, 1 = MIS (NEE + BSDF, power heuristic),
+                                2 = NEE only, 3 = BSDF sampling only */
     uint32_t sample_index;   /* sampleIndex of the first sample of this call (>= 1) */
     uint32_t restir_candidates; /* RIS light candidates per pixel in mode 0 (restir_di_initial's
                                    M = 4, main.swift:4649); 0 means 4, at most 256 */
@@ -176,6 +183,14 @@ vt_status vt_spectral_tables(float *icdf, float *cmf, float *basis);
 /* Reflectance record of a linear-sRGB colour (spectral_lagrange, main.swift:889-896):
  * (L0, L1, L2, 1) bounded-MESE multipliers, or (grey, 0, 0, 2). */
 vt_status vt_spectral_reflectance(const float rgb[3], float out[4]);
+
+/* The layered BSDF's energy-compensation tables as the context bakes them on its device (baking them
+ * first if needed): metal = 129 x 129 ideal-metal complements then 129 averages (the table
+ * scripts/generate_metal_energy.py writes into Shaders/MetalEnergy.metal), luts = 5 x 32 x 32 x 32
+ * floats (ideal dielectric complement; its average, ratio and the opaque-dielectric complement and
+ * average use the first 32 x 32 of their block), the tables of
+ * Vendor/OpenPBR/impl/data/openpbr_energy_arrays.h.  Either pointer may be NULL; synchronises. */
+vt_status vt_energy_tables(vt_context *ctx, float *metal, float *luts);
 
 /* Host-only build of the same 8-wide BVH the upload builds (no device needed), for inspection:
  * *nodes receives node_count 80-byte node records, *tris face_count 48-byte triangle records

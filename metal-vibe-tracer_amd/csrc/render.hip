@@ -180,6 +180,7 @@ struct Mat {
     float4 lagrange;
     V3 emission;
     float weight;
+    float roughness, ior, metalness, transmission;
 };
 __device__ __forceinline__ Mat material(const RenderScene &sc, uint32_t slot) {
     Mat m;
@@ -189,10 +190,18 @@ __device__ __forceinline__ Mat material(const RenderScene &sc, uint32_t slot) {
         m.lagrange = make_float4(0.7f, 0.0f, 0.0f, 2.0f);
         m.emission = v3(0.0f, 0.0f, 0.0f);
         m.weight = 0.0f;
+        m.roughness = 0.5f;
+        m.ior = 1.5f;
+        m.metalness = 0.0f;
+        m.transmission = 0.0f;
         return m;
     }
     const float4 *r = reinterpret_cast<const float4 *>(sc.materials) + 4 * (size_t)slot;
-    float4 r0 = r[0], r2 = r[2];
+    float4 r0 = r[0], r2 = r[2], r3 = r[3];
+    m.roughness = r3.x;
+    m.ior = r3.y;
+    m.metalness = r3.z;
+    m.transmission = r3.w;
     m.type = __float_as_uint(r0.x);
     m.albedo = v3(r0.y, r0.z, r0.w);
     m.lagrange = r[1];
@@ -222,6 +231,100 @@ __device__ __forceinline__ V3 sample_cosine_hemisphere(V3 n, float r0, float r1)
     V3 u = normalize(cross(up, n));
     V3 v = cross(n, u);
     return normalize(u * (cosf(phi) * sin_th) + v * (sinf(phi) * sin_th) + n * cos_th);
+}
+
+#include "openpbr_device.h"
+
+// is_delta (main.swift:3285-3287).
+__device__ __forceinline__ bool is_delta(const Mat &m) { return m.type == kDielectric || (m.type == kGlossy && m.roughness < 0.02f); }
+
+// eval_bsdf_with_pdf (main.swift:2659-2691): f (not cosine-weighted) at the path's lanes, and the pdf.
+// kLayered = false compiles the Lambert branch alone (scenes without GLOSSY / OPENPBR slots).
+template <bool kLayered>
+__device__ __forceinline__ S4 eval_bsdf_pdf(const RenderScene &sc, const Spectral &sp, const Lanes &lanes, const Surface &s,
+                                            const Mat &m, V3 wo, V3 wi, float &pdf) {
+    if (m.type == kDiffuse) {
+        pdf = dot(s.n, wo) <= 0.0f ? 0.0f : fmaxf(0.0f, dot(s.n, wi)) / kPi;
+        float cosine = fabsf(dot(s.n, wi));
+        bool ok = dot(wi, s.ng) > 0.0f && cosine >= 1e-7f && dot(s.n, wo) > 0.0f && dot(s.n, wi) > 0.0f;
+        return ok ? scale(reflectance(sp, lanes, m.lagrange), 1.0f / kPi) : s4(0.0f);
+    }
+    pdf = 0.0f;
+    if constexpr (!kLayered) return s4(0.0f);
+    float cosine = fabsf(dot(s.n, wi));
+    if (cosine < 1e-7f || dot(s.n, wo) <= 0.0f) return s4(0.0f);
+    const EnergyTables E{sc.energy_metal, sc.energy_luts};
+    Pbr p;
+    pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, s.n, wo, !s.front,
+                reflectance(sp, lanes, m.lagrange));
+    pdf = pbr_pdf(p, wi);
+    if (!(m.transmission > 0.0f) && dot(wi, s.ng) <= 0.0f) return s4(0.0f);
+    return scale(pbr_eval(p, E, wi), 1.0f / cosine);
+}
+
+// sample_bsdf (main.swift:3304-3368): direction, weight = f |cos| / pdf per lane, pdf (0 for the delta
+// branches); false ends the path.
+template <bool kLayered>
+__device__ __forceinline__ bool sample_bsdf(const RenderScene &sc, const Spectral &sp, const Lanes &lanes, const Surface &s,
+                                           const Mat &m, V3 in, uint32_t &seed, V3 &dir, S4 &weight, float &pdf) {
+    pdf = 0.0f;
+    if (m.type == kDielectric) {
+        const float ior = m.ior;
+        const S4 tint = reflectance(sp, lanes, m.lagrange);
+        const float eta = s.front ? 1.0f / ior : ior;
+        const float r0 = (1.0f - ior) / (1.0f + ior);
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            const V3 mm = attempt == 0 ? s.n : s.ng;
+            const float ci = fminf(fmaxf(dot(-in, mm), 0.0f), 1.0f);
+            const float s2t = eta * eta * (1.0f - ci * ci);
+            const float c = eta > 1.0f && s2t < 1.0f ? sqrtf(1.0f - s2t) : ci;
+            const float omc = 1.0f - c;
+            const float fr = r0 * r0 + (1.0f - r0 * r0) * (omc * omc * omc * omc * omc);
+            const bool reflected = s2t >= 1.0f || rand_f(seed) < fr;
+            if (reflected) {
+                dir = normalize(in - mm * (2.0f * dot(mm, in)));
+                weight = tint;
+            } else {
+                dir = normalize(in * eta + mm * (eta * ci - sqrtf(1.0f - s2t)));
+                weight = scale(tint, eta * eta);
+            }
+            if ((dot(dir, s.ng) > 0.0f) == reflected) return true;
+        }
+        return false;
+    }
+    if (is_delta(m)) {
+        const V3 rn = in - s.n * (2.0f * dot(s.n, in));
+        const V3 mm = dot(rn, s.ng) > 0.0f ? s.n : s.ng;
+        dir = normalize(in - mm * (2.0f * dot(mm, in)));
+        const float ci = fminf(fmaxf(dot(-in, mm), 0.0f), 1.0f);
+        const S4 f0 = reflectance(sp, lanes, m.lagrange);
+        const float omc = 1.0f - ci, p5 = omc * omc * omc * omc * omc;
+        for (int k = 0; k < 4; ++k) weight.v[k] = f0.v[k] + (1.0f - f0.v[k]) * p5;
+        return dot(dir, s.ng) > 0.0f;
+    }
+    if (m.type == kDiffuse) {
+        (void)rand_f(seed);  // the lobe-choice draw the generic sampler keeps (main.swift:3354)
+        const float r1 = rand_f(seed), r2 = rand_f(seed);
+        dir = sample_cosine_hemisphere(s.n, r1, r2);
+        pdf = fmaxf(0.0f, dot(s.n, dir)) / kPi;
+        weight = reflectance(sp, lanes, m.lagrange);
+        return pdf > 0.0f && dot(dir, s.ng) > 0.0f;
+    }
+    if constexpr (!kLayered) return false;
+    const EnergyTables E{sc.energy_metal, sc.energy_luts};
+    Pbr p;
+    pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, s.n, -in, !s.front,
+                reflectance(sp, lanes, m.lagrange));
+    const float r0 = rand_f(seed), r1 = rand_f(seed), r2 = rand_f(seed);
+    if (!pbr_sample(p, E, r0, r1, r2, dir, weight, pdf)) {
+        pdf = 0.0f;
+        return false;
+    }
+    if (!(pdf > 0.0f)) return false;
+    if (!(m.transmission > 0.0f) && dot(dir, s.ng) <= 0.0f) return false;
+    for (int k = 0; k < 4; ++k)
+        if (!isfinite(weight.v[k]) || weight.v[k] < 0.0f) return false;
+    return true;
 }
 
 // ---------------------------------------------------------------- emitters (main.swift:2720-2724, 2853-2917, 3070-3075)
@@ -519,6 +622,9 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
 }
 
 constexpr uint32_t kPrevDelta = 1u << 16, kPrevNee = 1u << 17, kPrevMis = 1u << 18;
+// BSDF-only paths (mode 3) past the scattering budget: one last continuation that may only reach
+// emission (main.swift:7716-7722, 7785).
+constexpr uint32_t kEmissionOnly = 1u << 19;
 
 // ---------------------------------------------------------------- kernels
 __global__ __launch_bounds__(256) void generate_kernel(FrameArgs f, PathQueue q, float *radiance, uint32_t *count) {
@@ -665,7 +771,7 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
 
 // Shading of one wavefront vertex per path.  kRestir: sampling mode 0, where a diffuse primary vertex
 // takes its direct light from ReSTIR DI's spatial reuse (main.swift:7484-7580) instead of NEE.
-template <bool kRestir>
+template <bool kRestir, bool kLayered>
 __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f, PathQueue in, HitArgs hits,
                                                     PathQueue out, ShadowQueue sh, Counters c, uint32_t capacity,
                                                     float *radiance, RestirArgs ra) {
@@ -713,6 +819,7 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
                 alive = false;
             }
         }
+        if (alive && (state & kEmissionOnly) != 0u) alive = false;
         if (kRestir && alive && bounce == 0u && m.type == kDiffuse) {
             // Spatial reuse: the pixel's reservoir and four uniform taps in a 16-pixel box
             // (uniform_neighbor, restir2020_compatible), combined with w = p_hat W M.
@@ -764,8 +871,8 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
                     shadow = true;
                 }
             }
-        } else if (alive) {
-            // Next-event estimation at this vertex (Lambert: never delta).
+        } else if (alive && !is_delta(m) && f.sampling_mode != 3u) {
+            // Next-event estimation at this non-delta vertex (main.swift:7655-7672, 7790-7805).
             const bool use_mis = f.sampling_mode <= 1u && (bounce == 0u ? f.scattering_limit > 0 : sdepth < f.scattering_limit);
             LightSample ls = sample_emitter(sc, s.p, seed);
             if (ls.pdf > 0.0f) {
@@ -774,15 +881,12 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
                 float d = length(delta);
                 float tol = endpoint_tolerance(o, ls.position, d);
                 if (d > tol) {
-                    // eval_bsdf_with_pdf for DIFFUSE (main.swift:2659-2684).
-                    const V3 wo = -dir;
-                    float pdf = dot(s.n, wo) <= 0.0f ? 0.0f : fmaxf(0.0f, dot(s.n, ls.wi)) / kPi;
-                    float cosine = fabsf(dot(s.n, ls.wi));
-                    bool f_ok = dot(ls.wi, s.ng) > 0.0f && cosine >= 1e-7f && dot(s.n, wo) > 0.0f && dot(s.n, ls.wi) > 0.0f;
-                    if (f_ok) {
-                        S4 fbsdf = scale(reflectance(sp, lanes, m.lagrange), 1.0f / kPi);
-                        float w = use_mis ? power_heuristic(ls.pdf, pdf) : 1.0f;
-                        S4 le = emission_spectrum(sp, lanes, ls.emission);
+                    float pdf;
+                    const S4 fbsdf = eval_bsdf_pdf<kLayered>(sc, sp, lanes, s, m, -dir, ls.wi, pdf);
+                    if (smax(fbsdf) > 0.0f) {
+                        const float cosine = fabsf(dot(s.n, ls.wi));
+                        const float w = use_mis ? power_heuristic(ls.pdf, pdf) : 1.0f;
+                        const S4 le = emission_spectrum(sp, lanes, ls.emission);
                         sh_c = spectrum_rgb(sp, lanes, scale(mul(mul(thr, fbsdf), le), cosine * w / ls.pdf));
                         sh_o = o;
                         sh_d = normalize(delta);
@@ -792,26 +896,29 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
                     }
                 }
             }
+        }
+        if (alive && bounce > 3u) {
             // Russian roulette after the third bounce (main.swift:7819-7827).
-            if (bounce > 3u) {
-                float survival = fminf(fmaxf(smax(thr), 0.05f), 0.95f);
-                if (rand_f(seed) >= survival) alive = false;
-                else thr = scale(thr, 1.0f / survival);
+            const float survival = fminf(fmaxf(smax(thr), 0.05f), is_delta(m) ? 0.99f : 0.95f);
+            if (rand_f(seed) >= survival) alive = false;
+            else thr = scale(thr, 1.0f / survival);
+        }
+        const bool prev_delta = alive && is_delta(m);
+        bool emission_only = false;
+        if (alive) {
+            // Departure: the scattering budget counts non-delta events (main.swift:7712-7742).
+            if (!prev_delta && sdepth >= f.scattering_limit) {
+                if (f.sampling_mode != 3u) alive = false;
+                else emission_only = true;
             }
         }
         if (alive) {
-            // Departure: the scattering budget, then the BSDF sample (main.swift:7712-7742).
-            if (sdepth >= f.scattering_limit) alive = false;
-        }
-        if (alive) {
-            ++sdepth;
-            float r0 = rand_f(seed);
-            (void)r0;  // the lobe-choice draw the generic sampler keeps (main.swift:3354)
-            float r1 = rand_f(seed), r2 = rand_f(seed);
-            V3 nd = sample_cosine_hemisphere(s.n, r1, r2);
-            float pdf = fmaxf(0.0f, dot(s.n, nd)) / kPi;
-            if (pdf > 0.0f && dot(nd, s.ng) > 0.0f) {
-                thr = mul(thr, reflectance(sp, lanes, m.lagrange));
+            if (!prev_delta) ++sdepth;
+            V3 nd;
+            S4 bw;
+            float pdf;
+            if (sample_bsdf<kLayered>(sc, sp, lanes, s, m, dir, seed, nd, bw, pdf)) {
+                thr = mul(thr, bw);
                 bool finite = isfinite(thr.v[0]) && isfinite(thr.v[1]) && isfinite(thr.v[2]) && isfinite(thr.v[3]);
                 if (finite && smax(thr) > 0.0f) {
                     new_o = ray_origin(s.p, s.ng, nd, s.error);
@@ -821,8 +928,10 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
                     // previousMIS (main.swift:7725-7728): a diffuse primary's DI reservoir estimated the
                     // whole direct integral, so in ReSTIR mode its continuation is not MIS-weighted.
                     const bool mis = f.sampling_mode == 1u || (f.sampling_mode == 0u && (bounce > 0u || m.type != kDiffuse));
-                    uint32_t flags = kPrevNee | (mis ? kPrevMis : 0u);
-                    state = (bounce + 1u) | ((uint32_t)sdepth << 8) | flags;
+                    const bool nee = !prev_delta && f.sampling_mode != 3u;
+                    uint32_t flags = (prev_delta ? kPrevDelta : 0u) | (nee ? kPrevNee : 0u) | (mis ? kPrevMis : 0u) |
+                                     (emission_only ? kEmissionOnly : 0u);
+                    state = min(bounce + 1u, 255u) | ((uint32_t)min(sdepth, 255) << 8) | flags;
                     cont = true;
                 }
             }
@@ -869,6 +978,63 @@ __global__ __launch_bounds__(256) void resolve_kernel(ShadowQueue sh, const uint
     radiance[3 * (size_t)id + 2] += sh.b[i];
 }
 
+// ---------------------------------------------------------------- energy-table bakes
+__device__ __forceinline__ uint32_t reverse_bits32(uint32_t v) { return __builtin_bitreverse32(v); }
+// scripts/generate_metal_energy.py:12-33: node (roughness i/128, cosine max(1e-5, (j/128)^2)), mean of
+// 1 - G1(l) over 16384 Hammersley VNDF samples; one thread per node, sequential float sum.
+__global__ __launch_bounds__(64) void bake_metal_kernel(float *out) {
+    const uint32_t node = blockIdx.x * 64 + threadIdx.x;
+    if (node >= 129u * 129u) return;
+    const uint32_t i = node / 129u, j = node % 129u;
+    if (i == 0u) {
+        out[node] = 0.0f;
+        return;
+    }
+    const float roughness = (float)i / 128.0f;
+    const float cosine = fmaxf(1e-5f, powf((float)j / 128.0f, 2.0f));
+    const V3 v = v3(sqrtf(fmaxf(0.0f, 1.0f - cosine * cosine)), 0.0f, cosine);
+    const float alpha = roughness * roughness;
+    float sum = 0.0f;
+    for (uint32_t k = 0; k < 16384u; ++k) {
+        const float r0 = ((float)k + 0.5f) / 16384.0f, r1 = (float)reverse_bits32(k) * 2.3283064365386963e-10f;
+        const V3 h = sample_vndf(alpha, v, r0, r1);
+        const float d = dot(v, h);
+        const V3 l = -v + h * (2.0f * d);
+        const float e = l.z > 0.0f ? smith_g1(l, alpha) : 0.0f;
+        sum += 1.0f - e;
+    }
+    out[node] = sum / 16384.0f;
+}
+// Dielectric single scattering per node (relative IOR, roughness j/31, cosine k/31): E_R = <F G1(l_r)>,
+// E_T = <(1 - F) G1(l_t)> over 1024 Hammersley VNDF samples.
+__global__ __launch_bounds__(64) void bake_dielectric_kernel(float *er, float *et) {
+    const uint32_t node = blockIdx.x * 64 + threadIdx.x;
+    if (node >= 32768u) return;
+    const uint32_t i = node >> 10, j = (node >> 5) & 31u, k = node & 31u;
+    const float eta = i >= 16u ? 1.0f + (float)(i - 16u) / 15.0f * 1.5f : 1.0f / (1.0f + (float)(15u - i) / 15.0f * 1.5f);
+    const float roughness = (float)j / 31.0f, alpha = roughness * roughness;
+    const float c = fmaxf(1e-5f, (float)k / 31.0f);
+    const V3 v = v3(sqrtf(fmaxf(0.0f, 1.0f - c * c)), 0.0f, c);
+    float R = 0.0f, T = 0.0f;
+    for (uint32_t q = 0; q < 1024u; ++q) {
+        const float r0 = ((float)q + 0.5f) / 1024.0f, r1 = (float)reverse_bits32(q) * 2.3283064365386963e-10f;
+        const V3 h = sample_vndf(alpha, v, r0, r1);
+        const float d = dot(v, h);
+        if (!(d > 0.0f)) continue;
+        const float F = fresnel(eta, d);
+        const V3 l = -v + h * (2.0f * d);
+        if (l.z > 0.0f) R += F * smith_g1(l, alpha);
+        const float inv = 1.0f / eta, s2t = inv * inv * fmaxf(0.0f, 1.0f - d * d);
+        if (s2t < 1.0f) {
+            const float ct = sqrtf(1.0f - s2t);
+            const V3 t = normalize(v * -inv + h * (inv * d - ct));
+            if (t.z < 0.0f) T += (1.0f - F) * smith_g1(t, alpha);
+        }
+    }
+    er[node] = R / 1024.0f;
+    et[node] = T / 1024.0f;
+}
+
 __global__ void reset_kernel(uint32_t *counters, uint32_t n) {
     if (threadIdx.x < n) counters[threadIdx.x] = 0u;
 }
@@ -908,16 +1074,26 @@ hipError_t launch_generate(const FrameArgs &f, const PathQueue &q, float *radian
 hipError_t launch_shade(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
                         const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
                         float *radiance, hipStream_t s) {
-    hipLaunchKernelGGL(shade_kernel<false>, dim3((capacity + 255) / 256), dim3(256), 0, s, sc, f, in, hits, out, sh, c,
-                       capacity, radiance, RestirArgs{});
+    const dim3 grid((capacity + 255) / 256);
+    if (sc.layered)
+        hipLaunchKernelGGL((shade_kernel<false, true>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
+                           radiance, RestirArgs{});
+    else
+        hipLaunchKernelGGL((shade_kernel<false, false>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
+                           radiance, RestirArgs{});
     return hipGetLastError();
 }
 
 hipError_t launch_shade_restir(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
                                const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
                                float *radiance, const RestirArgs &ra, hipStream_t s) {
-    hipLaunchKernelGGL(shade_kernel<true>, dim3((capacity + 255) / 256), dim3(256), 0, s, sc, f, in, hits, out, sh, c,
-                       capacity, radiance, ra);
+    const dim3 grid((capacity + 255) / 256);
+    if (sc.layered)
+        hipLaunchKernelGGL((shade_kernel<true, true>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
+                           radiance, ra);
+    else
+        hipLaunchKernelGGL((shade_kernel<true, false>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
+                           radiance, ra);
     return hipGetLastError();
 }
 
@@ -931,6 +1107,16 @@ hipError_t launch_resolve(const ShadowQueue &sh, const uint8_t *occluded, const 
                           float *radiance, hipStream_t s) {
     hipLaunchKernelGGL(resolve_kernel, dim3((capacity + 255) / 256), dim3(256), 0, s, sh, occluded, count, capacity,
                        radiance);
+    return hipGetLastError();
+}
+
+hipError_t launch_bake_metal(float *out, hipStream_t s) {
+    hipLaunchKernelGGL(bake_metal_kernel, dim3((129 * 129 + 63) / 64), dim3(64), 0, s, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bake_dielectric(float *er, float *et, hipStream_t s) {
+    hipLaunchKernelGGL(bake_dielectric_kernel, dim3(32768 / 64), dim3(64), 0, s, er, et);
     return hipGetLastError();
 }
 

@@ -40,6 +40,10 @@ struct RenderScene {
     const uint32_t *tree_pos;      // leaf position of tree_ids[i]
     uint32_t tree_node_count;
     float tree_lo[3], tree_scale[3];
+    // Energy-compensation tables of the layered BSDF (129 x 129 + 129 ideal metal; 5 x 32768 others).
+    const float *energy_metal;
+    const float *energy_luts;
+    uint32_t layered;  // some slot is GLOSSY or OPENPBR (the shading kernels then carry the layered BSDF)
 };
 
 // ReSTIR DI state per pixel, the reference's G-buffer and reservoir textures (main.swift:4842-4940):
@@ -67,7 +71,7 @@ struct FrameArgs {
     float eye[3], forward[3], right[3], up[3];
     float fov_scale, aspect;
     int32_t scattering_limit;  // max(0, depth - 1)
-    uint32_t sampling_mode;    // 0 = ReSTIR DI at the primary vertex, 1 = MIS (NEE + BSDF, power heuristic)
+    uint32_t sampling_mode;    // 0 ReSTIR DI at the primary vertex, 1 MIS, 2 NEE only, 3 BSDF only
 };
 
 // One queue of paths: the ray arrays first (RayArgs layout), then the path state.
@@ -107,6 +111,10 @@ hipError_t launch_shade_restir(const RenderScene &sc, const FrameArgs &f, const 
                                float *radiance, const RestirArgs &ra, hipStream_t s);
 hipError_t launch_resolve(const ShadowQueue &sh, const uint8_t *occluded, const uint32_t *count, uint32_t capacity,
                           float *radiance, hipStream_t s);
+// Energy-table bakes: ideal-metal complements (129 x 129, Hammersley 16384 per node) and the
+// dielectric single-scattering albedos E_R, E_T (32 x 32 x 32, Hammersley 1024 per node).
+hipError_t launch_bake_metal(float *out, hipStream_t s);
+hipError_t launch_bake_dielectric(float *er, float *et, hipStream_t s);
 hipError_t launch_reset_counters(uint32_t *counters, uint32_t n, hipStream_t s);
 hipError_t launch_accumulate(const FrameArgs &f, const float *radiance, float *accum, uint32_t frame_index,
                              hipStream_t s);

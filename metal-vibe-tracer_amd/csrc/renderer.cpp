@@ -53,6 +53,10 @@ struct vt_renderer {
     int restir_cur = 0;
     uint32_t restir_history = 0;
     uint64_t restir_generation = 0;  // vt_scene::generation the history was rendered with
+    uint32_t *host_count = nullptr;  // pinned: live paths, read between vertices of delta scenes
+    ~vt_renderer() {
+        if (host_count) (void)hipHostFree(host_count);
+    }
     float prev_camera[14] = {};  // eye, forward, right, up, fov_scale, aspect of the last sample
 };
 
@@ -117,12 +121,17 @@ vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, 
         // (main.swift:9301-9309).
         std::vector<float> rec((size_t)std::max<uint32_t>(count, 1) * 16, 0.0f);
         std::vector<double> weight(count, 0.0);
+        bool layered = false, delta = false;
         for (uint32_t k = 0; k < count; ++k) {
             const vt_material &m = materials[k];
-            if (m.type != 0 && m.type != 3) return fail(VT_ERROR_UNSUPPORTED, "material type not supported yet");
+            if (m.type > 4) return fail(VT_ERROR_INVALID_ARGUMENT, "material type must be 0..4");
             for (int c = 0; c < 3; ++c)
                 if (!std::isfinite(m.albedo[c]) || !std::isfinite(m.emission[c]) || m.emission[c] < 0.0f)
                     return fail(VT_ERROR_INVALID_ARGUMENT, "material colours must be finite, emission >= 0");
+            if (!(m.roughness >= 0.0f && m.roughness <= 1.0f) || !(m.ior >= 1.0f && m.ior <= 4.0f) ||
+                !(m.metalness >= 0.0f && m.metalness <= 1.0f) || !(m.transmission >= 0.0f && m.transmission <= 1.0f))
+                return fail(VT_ERROR_INVALID_ARGUMENT, "roughness, metalness, transmission must be in [0, 1], ior in [1, 4]");
+            if (m.type == 2 && !(m.ior > 1.0f)) return fail(VT_ERROR_INVALID_ARGUMENT, "a dielectric needs ior > 1");
             float *r = rec.data() + 16 * (size_t)k;
             uint32_t type = m.type;
             std::memcpy(&r[0], &type, 4);
@@ -134,8 +143,11 @@ vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, 
             if (m.emission[0] > 0 || m.emission[1] > 0 || m.emission[2] > 0)
                 weight[k] = std::max(1e-8, 0.2126 * m.emission[0] + 0.7152 * m.emission[1] + 0.0722 * m.emission[2]);
             r[11] = (float)weight[k];
-            r[12] = m.roughness; r[13] = m.ior; r[14] = m.metalness; r[15] = m.specular_weight;
+            r[12] = m.roughness; r[13] = m.ior; r[14] = m.metalness; r[15] = m.transmission;
+            if (m.type == 1 || m.type == 4) layered = true;
+            if (m.type == 2 || (m.type == 1 && m.roughness < 0.02f)) delta = true;
         }
+        if (layered) vt::ensure_energy_tables(scene->ctx);
         // Power-sampling emitter list: every face of an emitting slot, weighted by world area x
         // weight, in document order; CDF normalised with the last entry exactly 1 (main.swift:9312-9346).
         std::vector<uint32_t> ids;
@@ -208,6 +220,8 @@ vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, 
         scene->emitter_count = (uint32_t)ids.size();
         scene->emitter_total = (float)total;
         scene->shaded = true;
+        scene->has_delta = delta;
+        scene->has_layered = layered;
         scene->generation = vt_scene::next_generation();
         return VT_OK;
     });
@@ -236,6 +250,17 @@ vt_status vt_spectral_reflectance(const float rgb[3], float out[4]) {
     return vt::guarded_call([&]() -> vt_status {
         if (!rgb || !out) return fail(VT_ERROR_INVALID_ARGUMENT, "rgb and out must not be NULL");
         vt::reflectance_lagrange(host_tables(), rgb, out);
+        return VT_OK;
+    });
+}
+
+vt_status vt_energy_tables(vt_context *ctx, float *metal, float *luts) {
+    return vt::guarded_call([&]() -> vt_status {
+        if (!ctx) return fail(VT_ERROR_INVALID_ARGUMENT, "context must not be NULL");
+        vt::ensure_energy_tables(ctx);
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (metal) check(hipMemcpy(metal, ctx->energy_metal.ptr, ctx->energy_metal.bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (luts) check(hipMemcpy(luts, ctx->energy_luts.ptr, ctx->energy_luts.bytes, hipMemcpyDeviceToHost), "hipMemcpy");
         return VT_OK;
     });
 }
@@ -287,6 +312,7 @@ vt_status vt_renderer_create(vt_context *ctx, vt_renderer **out) {
         auto r = std::make_unique<vt_renderer>();
         r->ctx = ctx;
         r->counters.alloc(16 * sizeof(uint32_t));
+        check(hipHostMalloc((void **)&r->host_count, sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
         *out = r.release();
         return VT_OK;
     });
@@ -310,7 +336,7 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
         if (p->width == 0 || p->height == 0 || (uint64_t)p->width * p->height > (1u << 26))
             return fail(VT_ERROR_INVALID_ARGUMENT, "render size out of range");
         if (p->max_depth < 1 || p->max_depth > 64) return fail(VT_ERROR_INVALID_ARGUMENT, "max_depth must be 1..64");
-        if (p->sampling_mode > 1) return fail(VT_ERROR_UNSUPPORTED, "sampling_mode must be 0 (ReSTIR DI) or 1 (MIS)");
+        if (p->sampling_mode > 3) return fail(VT_ERROR_INVALID_ARGUMENT, "sampling_mode must be 0..3");
         const uint32_t candidates = p->restir_candidates ? p->restir_candidates : 4u;
         if (candidates > 256) return fail(VT_ERROR_INVALID_ARGUMENT, "restir_candidates must be at most 256");
         if (!(p->fov_degrees > 0.0f && p->fov_degrees < 180.0f)) return fail(VT_ERROR_INVALID_ARGUMENT, "fov must be in (0, 180)");
@@ -389,7 +415,16 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
                 ra.prev_fov_scale = r->prev_camera[12];
                 ra.prev_aspect = r->prev_camera[13];
             }
-            for (int b = 0; b <= f.scattering_limit; ++b) {
+            // Vertices: the scattering budget bounds non-delta events only; with delta materials in the
+            // scene, keep extending (checking the live count every 8 vertices) until no path is left.
+            const int base = f.scattering_limit + 1 + (f.sampling_mode == 3u ? 1 : 0);
+            const int cap = scene->has_delta ? 255 : base;
+            for (int b = 0; b < cap; ++b) {
+                if (b >= base && (b - base) % 8 == 0) {
+                    check(hipMemcpyAsync(r->host_count, cnt + cur, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+                    check(hipStreamSynchronize(s), "hipStreamSynchronize");
+                    if (*r->host_count == 0) break;
+                }
                 uint32_t *cin = cnt + cur, *cout = cnt + (1 - cur), *csh = cnt + 2;
                 check(vt::launch_reset_counters(cout, 1, s), "reset");
                 check(vt::launch_reset_counters(csh, 1, s), "reset");

@@ -64,7 +64,15 @@ struct vt_context {
     int device = 0;
     uint64_t budget = 0;
     uint64_t resident = 0;  // bytes of live scenes
+    // Energy-compensation tables of the layered BSDF, baked on the device on first use.
+    vt::DeviceBuffer energy_metal, energy_luts;
+    bool energy_ready = false;
 };
+
+namespace vt {
+// Bakes ctx->energy_metal / energy_luts once (GPU bakes, host averages and unorm16 quantisation).
+void ensure_energy_tables(vt_context *ctx);
+}  // namespace vt
 
 // One published mesh: the reference's MeshStreams + BLAS nodes (Sources/Acceleration.swift:59-64),
 // the document-order geometry the shading stage reads, and the material / emitter tables.
@@ -82,6 +90,8 @@ struct vt_scene {
     uint32_t material_count = 0, emitter_count = 0;
     float emitter_total = 0.0f;
     bool shaded = false;  // materials set
+    bool has_delta = false;  // some material is a delta (mirror or smooth glass)
+    bool has_layered = false;  // some material is GLOSSY or OPENPBR
     // Identity of the scene's current contents: a process-unique id, bumped by every material or
     // light-tree change (renderers drop ReSTIR history rendered with other contents).
     uint64_t generation = next_generation();
@@ -116,6 +126,9 @@ struct vt_scene {
         r.tree_ids = tree_ids.as<const uint32_t>();
         r.tree_pos = tree_pos.as<const uint32_t>();
         r.tree_node_count = tree_node_count;
+        r.energy_metal = ctx->energy_metal.as<const float>();
+        r.energy_luts = ctx->energy_luts.as<const float>();
+        r.layered = has_layered ? 1u : 0u;
         for (int k = 0; k < 3; ++k) {
             r.tree_lo[k] = tree_lo[k];
             r.tree_scale[k] = tree_scale[k];

@@ -28,7 +28,7 @@ EXPORTED = [
     "vt_gen_icosphere_field", "vt_gen_heightfield", "vt_gen_camera_rays", "vt_gen_bounce_rays", "vt_gen_random_rays",
     "vt_scene_set_materials", "vt_renderer_create", "vt_renderer_destroy", "vt_render_frame", "vt_reset_accumulation",
     "vt_accum_readback", "vt_accum_frames", "vt_gen_cornell", "vt_spectral_tables", "vt_spectral_reflectance",
-    "vt_scene_set_light_tree", "vt_light_tree_build_host",
+    "vt_scene_set_light_tree", "vt_light_tree_build_host", "vt_energy_tables",
 ]
 
 
@@ -58,7 +58,7 @@ _fp = ctypes.POINTER(ctypes.c_float)
 
 
 class Material(ctypes.Structure):
-    """vt_material (64 bytes): MaterialType 0 DIFFUSE / 3 EMISSIVE, linear-sRGB albedo and emission."""
+    """vt_material (64 bytes): MaterialType 0 DIFFUSE, 1 GLOSSY, 2 DIELECTRIC, 3 EMISSIVE, 4 OPENPBR."""
     _fields_ = [("type", ctypes.c_uint32), ("albedo", ctypes.c_float * 3), ("emission", ctypes.c_float * 3),
                 ("roughness", ctypes.c_float), ("ior", ctypes.c_float), ("metalness", ctypes.c_float),
                 ("specular_weight", ctypes.c_float), ("transmission", ctypes.c_float), ("reserved", ctypes.c_float * 4)]
@@ -122,6 +122,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_renderer_destroy": (None, [vp]),
         "vt_render_frame": (i32, [vp, vp, P(_FrameParams), u32, vp]),
         "vt_reset_accumulation": (i32, [vp]),
+        "vt_energy_tables": (i32, [vp, vp, vp]),
         "vt_accum_readback": (i32, [vp, vp, c.c_size_t]),
         "vt_accum_frames": (u32, [vp]),
         "vt_spectral_tables": (i32, [vp, vp, vp]),
@@ -340,6 +341,13 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def energy_tables(self):
+        """(metal (129, 129), metal averages (129,), luts (5, 32, 32, 32)) as baked on the device."""
+        metal = np.empty(129 * 129 + 129, np.float32)
+        luts = np.empty((5, 32, 32, 32), np.float32)
+        _check(load().vt_energy_tables(self._h, _ptr(metal), _ptr(luts)))
+        return metal[:129 * 129].reshape(129, 129), metal[129 * 129:], luts
 
     def set_mesh(self, mesh: Mesh, leaf_size=0, bins=0, max_depth=0) -> "Scene":
         """MaterialLibrary.setMesh for an indexed mesh: builds and uploads the BVH."""

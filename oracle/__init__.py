@@ -68,6 +68,13 @@ def load():
         "vo_trace_any_batch": (None, [P(SceneView)] + [vp] * 8 + [c.c_int64, c.c_int, c.c_int, vp, vp]),
         "vo_set_fault": (None, [c.c_int]),
         "vo_half_round": (c.c_float, [c.c_float]),
+        "vo_bsdf_eval_f32": (None, [vp, vp, vp, vp, vp, c.c_int, vp, vp]),
+        "vo_bsdf_sample_f32": (c.c_int, [vp, vp, vp, vp, c.c_int, vp, vp, vp, vp]),
+        "vo_bsdf_eval_f64": (None, [vp, vp, vp, vp, vp, c.c_int, vp, vp]),
+        "vo_bsdf_sample_f64": (c.c_int, [vp, vp, vp, vp, c.c_int, vp, vp, vp, vp]),
+        "vo_energy_table": (c.POINTER(c.c_uint16), [c.c_int]),
+        "vo_metal_energy_table": (c.POINTER(c.c_float), []),
+        "vo_energy_ior_node": (c.c_float, [c.c_int]),
         "vo_render_f32": (None, [vp, vp, u32, u32, vp, u32, c.c_int, vp]),
         "vo_render_f64": (None, [vp, vp, u32, u32, vp, u32, c.c_int, vp]),
         "vo_render_restir_f32": (None, [vp, vp, u32, u32, u32, vp, c.c_int, vp]),
@@ -295,6 +302,7 @@ def material_records(materials):
         if (e > 0).any():
             weights[k] = max(1e-8, 0.2126 * float(e[0]) + 0.7152 * float(e[1]) + 0.0722 * float(e[2]))
         rec[k, 11] = np.float32(weights[k])
+        rec[k, 12:16] = (m.get("roughness", 0.5), m.get("ior", 1.5), m.get("metalness", 0.0), m.get("transmission", 0.0))
     return rec, weights
 
 
@@ -332,6 +340,47 @@ def build_scene_tree(vertices, faces, ids, weights):
     if len(tree.words) <= 1 or tree.depth > 64:
         return None
     return tree
+
+
+class BSDF:
+    """The oracle's layered BSDF subset for one material: type 1 (GLOSSY: metal) or 4 (OPENPBR)."""
+
+    def __init__(self, type, roughness, ior=1.5, metalness=0.0, transmission=0.0, rho=(0.8, 0.8, 0.8, 0.8),
+                 precision="f32"):
+        self.params = np.array([type, roughness, ior, metalness, transmission], np.float32)
+        self.rho = np.asarray(rho, np.float32)
+        self.sfx = precision
+
+    def eval(self, n, wo, wi, inside=False):
+        f = np.zeros(4, np.float32)
+        pdf = np.zeros(1, np.float32)
+        getattr(load(), "vo_bsdf_eval_" + self.sfx)(_p(self.params), _p(np.asarray(n, np.float32)),
+                                                     _p(np.asarray(wo, np.float32)), _p(np.asarray(wi, np.float32)),
+                                                     _p(self.rho), int(inside), _p(f), _p(pdf))
+        return f.astype(np.float64), float(pdf[0])
+
+    def sample(self, n, wo, u3, inside=False):
+        d = np.zeros(3, np.float32)
+        w = np.zeros(4, np.float32)
+        pdf = np.zeros(1, np.float32)
+        ok = getattr(load(), "vo_bsdf_sample_" + self.sfx)(_p(self.params), _p(np.asarray(n, np.float32)),
+                                                            _p(np.asarray(wo, np.float32)), _p(self.rho), int(inside),
+                                                            _p(np.asarray(u3, np.float32)), _p(d), _p(w), _p(pdf))
+        return (d.astype(np.float64), w.astype(np.float64), float(pdf[0])) if ok else None
+
+
+def energy_tables():
+    """(metal complements (129, 129), metal averages (129,), dict of the 32-node unorm tables as floats)."""
+    lib = load()
+    m = np.ctypeslib.as_array(lib.vo_metal_energy_table(), shape=(129 * 129 + 129,)).copy()
+    names = ["dielectric", "dielectric_avg", "dielectric_ratio", "opaque", "opaque_avg"]
+    shapes = [(32, 32, 32), (32, 32), (32, 32), (32, 32, 32), (32, 32)]
+    luts = {}
+    for i, (nm, sh) in enumerate(zip(names, shapes)):
+        a = np.ctypeslib.as_array(lib.vo_energy_table(i), shape=(int(np.prod(sh)),)).copy()
+        luts[nm] = a.reshape(sh).astype(np.float64) / 65535.0
+    iors = np.array([lib.vo_energy_ior_node(i) for i in range(32)], np.float64)
+    return m[:129 * 129].reshape(129, 129), m[129 * 129:], luts, iors
 
 
 def _render_inputs(scene, materials, frame, light_tree):

@@ -37,6 +37,7 @@
 #define RFABS fabsf
 #define RFMAX fmaxf
 #define RFMIN fminf
+#define RFLOOR floorf
 #endif
 
 typedef REAL real;
@@ -156,12 +157,18 @@ static tri_t tri_of(const vo_render_scene *sc, uint32_t f) {
     t.tag = fc[3];
     return t;
 }
-typedef struct { uint32_t type; const float *lagrange; v3 emission; real weight; } mat_t;
+typedef struct {
+    uint32_t type;
+    const float *lagrange;
+    v3 emission;
+    real weight, roughness, ior, metalness, transmission;
+} mat_t;
 static mat_t mat_of(const vo_render_scene *sc, uint32_t slot) {
     static const float grey[4] = {0.7f, 0, 0, 2.0f};
     mat_t m;
     if (slot >= sc->material_count) {
         m.type = 0; m.lagrange = grey; m.emission = V(0, 0, 0); m.weight = 0;
+        m.roughness = (real)0.5; m.ior = (real)1.5; m.metalness = 0; m.transmission = 0;
         return m;
     }
     const float *r = sc->materials + 16 * (size_t)slot;
@@ -169,9 +176,15 @@ static mat_t mat_of(const vo_render_scene *sc, uint32_t slot) {
     m.lagrange = r + 4;
     m.emission = ld(r + 8);
     m.weight = (real)r[11];
+    m.roughness = (real)r[12];
+    m.ior = (real)r[13];
+    m.metalness = (real)r[14];
+    m.transmission = (real)r[15];
     if (r[8] > 0 || r[9] > 0 || r[10] > 0) m.type = 3;
     return m;
 }
+/* is_delta (main.swift:3285-3287). */
+static int is_delta(const mat_t *m) { return m->type == 2u || (m->type == 1u && m->roughness < (real)0.02); }
 static real imported_geometry(v3 p, v3 pos, const tri_t *t) {
     v3 d = sub(pos, p);
     real d2 = dot(d, d);
@@ -181,6 +194,9 @@ static real imported_geometry(v3 p, v3 pos, const tri_t *t) {
 static float f_of(real x) { return (float)x; }
 
 typedef struct { v3 p, n, ng; real error, t; int front; uint32_t slot; } surf_t;
+
+#include "openpbr_impl.h"
+
 /* Closest hit through the oracle's reference traversal; returns 0 on a miss. */
 static int trace(const vo_render_scene *sc, v3 o, v3 d, real tmin, surf_t *s, uint32_t *face_out) {
     float fo[3] = {f_of(o.x), f_of(o.y), f_of(o.z)}, fd[3] = {f_of(d.x), f_of(d.y), f_of(d.z)};
@@ -264,6 +280,85 @@ static int light_visible(const vo_render_scene *sc, const surf_t *s, v3 wi, v3 p
     return !occluded(sc, o, norm(dd), ray_t_min(o), d - tol);
 }
 
+/* eval_bsdf_with_pdf (main.swift:2659-2691): f (not cosine-weighted) and the BSDF pdf of wi. */
+static s4 eval_bsdf_pdf(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, const mat_t *m, v3 wo, v3 wi,
+                        real *pdf) {
+    if (m->type == 0u) {
+        *pdf = dot(s->n, wo) <= 0 ? 0 : RFMAX((real)0, dot(s->n, wi)) / PI_R;
+        real cosine = RFABS(dot(s->n, wi));
+        if (!(dot(wi, s->ng) > 0) || cosine < (real)1e-7 || dot(s->n, wo) <= 0 || !(dot(s->n, wi) > 0)) return S4(0);
+        return sscale(reflectance(sc, l, m->lagrange), (real)1 / PI_R);
+    }
+    *pdf = 0;
+    real cosine = RFABS(dot(s->n, wi));
+    if (cosine < (real)1e-7 || dot(s->n, wo) <= 0) return S4(0);
+    pbr_t p;
+    pbr_inputs in = {(real)m->type, m->roughness, m->ior, m->metalness, m->transmission};
+    pbr_prepare(&p, &in, s->n, wo, !s->front, reflectance(sc, l, m->lagrange));
+    *pdf = pbr_pdf(&p, wi);
+    if (!(m->transmission > 0) && dot(wi, s->ng) <= 0) return S4(0);
+    return sscale(pbr_eval(&p, wi), (real)1 / cosine);
+}
+
+/* sample_bsdf (main.swift:3304-3368): direction, weight = f |cos| / pdf per lane, pdf (0 for the
+ * delta branches).  Returns 0 when the path ends here. */
+static int sample_bsdf(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, const mat_t *m, v3 in,
+                       uint32_t *seed, v3 *dir, s4 *weight, real *pdf) {
+    *pdf = 0;
+    if (m->type == 2u) {
+        real ior = m->ior;
+        s4 tint = reflectance(sc, l, m->lagrange);
+        real eta = s->front ? (real)1 / ior : ior;
+        real r0 = ((real)1 - ior) / ((real)1 + ior);
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            v3 mm = attempt == 0 ? s->n : s->ng;
+            real ci = RFMIN(RFMAX(dot(neg(in), mm), (real)0), (real)1);
+            real s2t = eta * eta * ((real)1 - ci * ci);
+            real c = eta > 1 && s2t < 1 ? RSQRT((real)1 - s2t) : ci;
+            real omc = (real)1 - c;
+            real fr = r0 * r0 + ((real)1 - r0 * r0) * (omc * omc * omc * omc * omc);
+            int reflected = s2t >= 1 || (real)vo_rand_f(seed) < fr;
+            if (reflected) {
+                *dir = norm(sub(in, mulv(mm, (real)2 * dot(mm, in))));
+                *weight = tint;
+            } else {
+                *dir = norm(add(mulv(in, eta), mulv(mm, eta * ci - RSQRT((real)1 - s2t))));
+                *weight = sscale(tint, eta * eta);
+            }
+            if ((dot(*dir, s->ng) > 0) == reflected) return 1;
+        }
+        return 0;
+    }
+    if (is_delta(m)) {
+        v3 rn = sub(in, mulv(s->n, (real)2 * dot(s->n, in)));
+        v3 mm = dot(rn, s->ng) > 0 ? s->n : s->ng;
+        *dir = norm(sub(in, mulv(mm, (real)2 * dot(mm, in))));
+        real ci = RFMIN(RFMAX(dot(neg(in), mm), (real)0), (real)1);
+        s4 f0 = reflectance(sc, l, m->lagrange);
+        real omc = (real)1 - ci, p5 = omc * omc * omc * omc * omc;
+        for (int k = 0; k < 4; ++k) weight->v[k] = f0.v[k] + ((real)1 - f0.v[k]) * p5;
+        return dot(*dir, s->ng) > 0;
+    }
+    if (m->type == 0u) {
+        vo_rand_f(seed);
+        real r1 = (real)vo_rand_f(seed), r2 = (real)vo_rand_f(seed);
+        *dir = cosine_hemisphere(s->n, r1, r2);
+        *pdf = RFMAX((real)0, dot(s->n, *dir)) / PI_R;
+        *weight = reflectance(sc, l, m->lagrange);
+        return *pdf > 0 && dot(*dir, s->ng) > 0;
+    }
+    pbr_t p;
+    pbr_inputs pin = {(real)m->type, m->roughness, m->ior, m->metalness, m->transmission};
+    pbr_prepare(&p, &pin, s->n, neg(in), !s->front, reflectance(sc, l, m->lagrange));
+    real r0 = (real)vo_rand_f(seed), r1 = (real)vo_rand_f(seed), r2 = (real)vo_rand_f(seed);
+    if (!pbr_sample(&p, r0, r1, r2, dir, weight, pdf)) { *pdf = 0; return 0; }
+    if (!(*pdf > 0)) return 0;
+    if (!(m->transmission > 0) && dot(*dir, s->ng) <= 0) return 0;
+    for (int k = 0; k < 4; ++k)
+        if (!isfinite(weight->v[k]) || weight->v[k] < 0) return 0;
+    return 1;
+}
+
 /* Next-event estimate at surface s: draws 4 numbers, returns the contribution if visible. */
 static v3 nee(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, const mat_t *m, v3 wo, s4 thr,
               int use_mis, uint32_t *seed) {
@@ -271,14 +366,42 @@ static v3 nee(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, cons
     light_sample ls = sample_direct_light(sc, s->p, seed);
     if (!(ls.pdf > 0)) return zero;
     if (!light_visible(sc, s, ls.wi, ls.position)) return zero;
-    /* eval_bsdf_with_pdf, DIFFUSE */
-    real bpdf = dot(s->n, wo) <= 0 ? 0 : RFMAX((real)0, dot(s->n, ls.wi)) / PI_R;
+    real bpdf;
+    s4 f = eval_bsdf_pdf(sc, l, s, m, wo, ls.wi, &bpdf);
     real cosine = RFABS(dot(s->n, ls.wi));
-    if (!(dot(ls.wi, s->ng) > 0) || cosine < (real)1e-7 || dot(s->n, wo) <= 0 || !(dot(s->n, ls.wi) > 0)) return zero;
-    s4 f = sscale(reflectance(sc, l, m->lagrange), (real)1 / PI_R);
     real w = use_mis ? power_heuristic(ls.pdf, bpdf) : 1;
     s4 le = emission_spectrum(sc, l, ls.emission);
     return spectrum_rgb(sc, l, sscale(smul(smul(thr, f), le), cosine * w / ls.pdf));
+}
+
+/* The layered BSDF on its own, for tests: params = type, roughness, ior, metalness, transmission;
+ * n is the shading normal on the arriving side, wo the direction toward the viewer, rho the base
+ * colour per lane.  eval: f |cos| per lane and the aggregate pdf of wi. */
+void SUFFIX(vo_bsdf_eval)(const float *params, const float *n, const float *wo, const float *wi, const float *rho,
+                          int inside, float *fcos, float *pdf) {
+    pbr_t p;
+    pbr_inputs in = {params[0], params[1], params[2], params[3], params[4]};
+    s4 r = {{rho[0], rho[1], rho[2], rho[3]}};
+    pbr_prepare(&p, &in, ld(n), ld(wo), inside, r);
+    s4 f = pbr_eval(&p, ld(wi));
+    for (int k = 0; k < 4; ++k) fcos[k] = (float)f.v[k];
+    *pdf = (float)pbr_pdf(&p, ld(wi));
+}
+/* sample: returns 1 and (direction, f |cos| / pdf per lane, pdf), or 0. */
+int SUFFIX(vo_bsdf_sample)(const float *params, const float *n, const float *wo, const float *rho, int inside,
+                           const float *u3, float *dir, float *weight, float *pdf) {
+    pbr_t p;
+    pbr_inputs in = {params[0], params[1], params[2], params[3], params[4]};
+    s4 r = {{rho[0], rho[1], rho[2], rho[3]}};
+    pbr_prepare(&p, &in, ld(n), ld(wo), inside, r);
+    v3 d;
+    s4 w;
+    real pd;
+    if (!pbr_sample(&p, u3[0], u3[1], u3[2], &d, &w, &pd)) return 0;
+    dir[0] = (float)d.x; dir[1] = (float)d.y; dir[2] = (float)d.z;
+    for (int k = 0; k < 4; ++k) weight[k] = (float)w.v[k];
+    *pdf = (float)pd;
+    return 1;
 }
 
 /* ------------------------------------------------------------------ camera and pixel streams */
@@ -336,19 +459,24 @@ static v3 continue_path(const vo_render_scene *sc, const vo_frame *fr, const lan
     v3 radiance = V(0, 0, 0);
     s4 thr = S4(1);
     v3 cur_dir = dir;
-    int sdepth = 0;
+    int sdepth = 0, emission_only = 0;
     uint32_t face;
     for (int bounce = 1;; ++bounce) {
-        /* Departure from s (Lambert: never delta). */
-        if (sdepth >= limit) break;
-        ++sdepth;
+        /* Departure from s: the scattering budget counts non-delta events only; BSDF-only paths
+         * (mode 3) take one final continuation that may only reach emission. */
+        const int prev_delta = is_delta(&m);
+        if (!prev_delta && sdepth >= limit) {
+            if (fr->sampling_mode != 3u || emission_only) break;
+            emission_only = 1;
+        }
+        if (!prev_delta) ++sdepth;
+        const int prev_nee = !prev_delta && fr->sampling_mode != 3u;
         int prev_mis = fr->sampling_mode == 1u || (fr->sampling_mode == 0u && (bounce > 1 || m.type != 0u));
-        vo_rand_f(seed);
-        real r1 = (real)vo_rand_f(seed), r2 = (real)vo_rand_f(seed);
-        v3 nd = cosine_hemisphere(s.n, r1, r2);
-        real bpdf = RFMAX((real)0, dot(s.n, nd)) / PI_R;
-        if (!(bpdf > 0 && dot(nd, s.ng) > 0)) break;
-        thr = smul(thr, reflectance(sc, l, m.lagrange));
+        v3 nd;
+        s4 bw;
+        real bpdf;
+        if (!sample_bsdf(sc, l, &s, &m, cur_dir, seed, &nd, &bw, &bpdf)) break;
+        thr = smul(thr, bw);
         if (!(isfinite(thr.v[0]) && isfinite(thr.v[1]) && isfinite(thr.v[2]) && isfinite(thr.v[3])) || smax(thr) <= 0) break;
         v3 prev = s.p;
         v3 o = ray_origin(s.p, s.ng, nd, s.error);
@@ -377,16 +505,18 @@ static v3 continue_path(const vo_render_scene *sc, const vo_frame *fr, const lan
             } else if (sc->emitter_count && sc->emitter_total > 0 && rm.weight > 0 && g > (real)1e-12) {
                 light_pdf = (rm.weight / (real)sc->emitter_total) / g;
             }
-            real w = emission_weight(0, fr->sampling_mode != 3u, prev_mis, bpdf, light_pdf);
+            real w = emission_weight(prev_delta, prev_nee, prev_mis, bpdf, light_pdf);
             v3 e = r.front ? rm.emission : V(0, 0, 0);
             radiance = add(radiance, spectrum_rgb(sc, l, sscale(smul(thr, emission_spectrum(sc, l, e)), w)));
             break;
         }
-        radiance = add(radiance, nee(sc, l, &r, &rm, neg(cur_dir), thr, fr->sampling_mode <= 1u && sdepth < limit, seed));
+        if (emission_only) break;
+        if (!is_delta(&rm) && fr->sampling_mode != 3u)
+            radiance = add(radiance, nee(sc, l, &r, &rm, neg(cur_dir), thr, fr->sampling_mode <= 1u && sdepth < limit, seed));
         s = r;
         m = rm;
         if (bounce > 3) {
-            real survival = RFMIN(RFMAX(smax(thr), (real)0.05), (real)0.95);
+            real survival = RFMIN(RFMAX(smax(thr), (real)0.05), is_delta(&m) ? (real)0.99 : (real)0.95);
             if ((real)vo_rand_f(seed) >= survival) break;
             thr = sscale(thr, (real)1 / survival);
         }
@@ -409,7 +539,8 @@ static v3 sample_pixel(const vo_render_scene *sc, const vo_frame *fr, const cam_
     if (!trace(sc, cam->eye, dir, ray_t_min(cam->eye), &s, &face)) return radiance;
     mat_t m = mat_of(sc, s.slot);
     if (m.type == 3) return s.front ? m.emission : V(0, 0, 0);
-    radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), fr->sampling_mode <= 1u && limit > 0, &seed);
+    if (!is_delta(&m) && fr->sampling_mode != 3u)
+        radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), fr->sampling_mode <= 1u && limit > 0, &seed);
     radiance = add(radiance, continue_path(sc, fr, &l, s, m, dir, &seed));
     if (!(isfinite(radiance.x) && isfinite(radiance.y) && isfinite(radiance.z))) radiance = V(0, 0, 0);
     return radiance;
@@ -418,6 +549,7 @@ static v3 sample_pixel(const vo_render_scene *sc, const vo_frame *fr, const cam_
 void SUFFIX(vo_render)(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
                        const uint32_t *pixels, uint32_t npixels, int threads, double *out_rgb) {
     const cam_t cam = camera_of(fr, fr->eye);
+    (void)vo_energy_table(0);  /* bake the BSDF tables outside the parallel region */
 #pragma omp parallel for schedule(dynamic, 4) num_threads(threads > 0 ? threads : omp_get_max_threads())
     for (int64_t k = 0; k < (int64_t)npixels; ++k) {
         uint32_t pix = pixels[k], gx = pix % fr->width, gy = pix / fr->width;
@@ -631,7 +763,7 @@ static v3 restir_pass2(const vo_render_scene *sc, const vo_frame *fr, uint32_t g
                 radiance = add(radiance, spectrum_rgb(sc, &sl, v));
             }
         }
-    } else {
+    } else if (!is_delta(&m)) {
         radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), limit > 0, &seed);
     }
     radiance = add(radiance, continue_path(sc, fr, &l, s, m, dir, &seed));
@@ -651,6 +783,7 @@ void SUFFIX(vo_render_restir)(const vo_render_scene *sc, const vo_frame *fr, uin
     }
     const int nt = threads > 0 ? threads : omp_get_max_threads();
     cam_t prev = camera_of(fr, fr->eye);
+    (void)vo_energy_table(0);
     for (uint32_t k = 0; k < samples; ++k) {
         const uint32_t si = sample_index0 + k;
         const cam_t cam = camera_of(fr, eyes ? eyes + 3 * (size_t)k : fr->eye);

@@ -9,4 +9,5 @@
 #define RFABS fabs
 #define RFMAX fmax
 #define RFMIN fmin
+#define RFLOOR floor
 #include "integrator.c"

@@ -174,9 +174,32 @@ void vo_render_restir_f32(const vo_render_scene *sc, const vo_frame *fr, uint32_
 void vo_render_restir_f64(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
                           uint32_t candidates, const float *eyes, int threads, double *out_rgb);
 
+/* Energy-compensation tables of the layered BSDF (energy.c), baked on first use.
+ * vo_energy_table(id): unorm16, 32 x 32 x 32 (IOR, roughness, cosine) or 32 x 32 (IOR, roughness).
+ * vo_metal_energy_table(): 129 x 129 ideal-metal complements (roughness, sqrt cosine), then the
+ * 129 average complements. */
+enum { VO_LUT_DIELECTRIC = 0, VO_LUT_DIELECTRIC_AVG, VO_LUT_DIELECTRIC_RATIO, VO_LUT_OPAQUE, VO_LUT_OPAQUE_AVG,
+       VO_LUT_COUNT };
+const uint16_t *vo_energy_table(int id);
+const float *vo_metal_energy_table(void);
+float vo_energy_ior_node(int i);
+
+/* The layered BSDF subset (openpbr_impl.h) on its own: params = type (1 GLOSSY, 4 OPENPBR),
+ * roughness, ior, metalness, transmission; n shading normal on the arriving side, wo toward the
+ * viewer, rho base colour per lane, inside = hit from the back of the surface. */
+void vo_bsdf_eval_f32(const float *params, const float *n, const float *wo, const float *wi, const float *rho,
+                      int inside, float *fcos, float *pdf);
+int vo_bsdf_sample_f32(const float *params, const float *n, const float *wo, const float *rho, int inside,
+                       const float *u3, float *dir, float *weight, float *pdf);
+void vo_bsdf_eval_f64(const float *params, const float *n, const float *wo, const float *wi, const float *rho,
+                      int inside, float *fcos, float *pdf);
+int vo_bsdf_sample_f64(const float *params, const float *n, const float *wo, const float *rho, int inside,
+                       const float *u3, float *dir, float *weight, float *pdf);
+
 /* For tests that plant a mistake in the oracle (never the product): 0 = faithful,
  * 1 = drop the FMA tie-break of watertight_edge, 2 = swap b1/b2, 3 = ReSTIR spatial reuse leaves
- * the neighbours' confidence out of M, 4 = ReSTIR RIS counts only candidates with pdf > 0 in M. */
+ * the neighbours' confidence out of M, 4 = ReSTIR RIS counts only candidates with pdf > 0 in M,
+ * 5 = the GGX reflection lobe drops the shadowing term G1(wi). */
 void vo_set_fault(int fault);
 int vo_get_fault(void);
 /* Nearest half-precision value of x (ties to even): what an rgba16f texel store keeps. */

@@ -163,3 +163,39 @@ def backfacing_light_cornell():
     import vt
     return _append(vt.cornell(0.25), [_ceiling_quad(0.3, 0.3, 0.25, -0.998, 4)], CORNELL_MATS,
                    [dict(type=3, albedo=(0, 0, 0), emission=(17, 17, 17))])
+
+
+def materials_cornell(variant="layered", light_size=0.25):
+    """The Cornell box with its tall box (tag 4), short box (tag 5) and floor (tag 6) given other
+    materials.  'layered': rough gold-like metal (GLOSSY, roughness 0.3), rough glass (OPENPBR,
+    transmission 1, roughness 0.2, IOR 1.5) and a glossy plastic floor (OPENPBR, roughness 0.4);
+    'delta': a mirror (GLOSSY, roughness 0) and smooth glass (DIELECTRIC, IOR 1.5), Lambert floor;
+    'opaque': the metal, a half-metallic blue OPENPBR box (roughness 0.25, IOR 1.6) and the plastic floor."""
+    import vt
+    base = vt.cornell(light_size)
+    faces = base.faces.copy()
+    faces[10:22, 3] = 4
+    faces[22:34, 3] = 5
+    ys = base.vertices[faces[:, :3], 1]
+    floor = np.all(ys == ys.min(), axis=1) & (faces[:, 3] == 0)
+    faces[floor, 3] = 6
+    mats = list(CORNELL_MATS)
+    if variant == "opaque":
+        mats += [dict(type=1, albedo=(0.95, 0.64, 0.54), roughness=0.3),
+                 dict(type=4, albedo=(0.2, 0.4, 0.8), roughness=0.25, ior=1.6, metalness=0.5),
+                 dict(type=4, albedo=(0.73, 0.73, 0.73), roughness=0.4, ior=1.5)]
+    elif variant == "layered":
+        mats += [dict(type=1, albedo=(0.95, 0.64, 0.54), roughness=0.3),
+                 dict(type=4, albedo=(1.0, 1.0, 1.0), roughness=0.2, ior=1.5, transmission=1.0),
+                 dict(type=4, albedo=(0.73, 0.73, 0.73), roughness=0.4, ior=1.5)]
+    else:
+        mats += [dict(type=1, albedo=(0.9, 0.9, 0.9), roughness=0.0),
+                 dict(type=2, albedo=(1.0, 1.0, 1.0), ior=1.5),
+                 dict(type=0, albedo=(0.73, 0.73, 0.73))]
+    return vt.Mesh(base.vertices, faces), mats
+
+
+def vt_materials(mats):
+    import vt
+    return [vt.material(m["type"], m["albedo"], m.get("emission", (0, 0, 0)), m.get("roughness", 0.5),
+                        m.get("ior", 1.5), m.get("metalness", 0.0), 1.0, m.get("transmission", 0.0)) for m in mats]
