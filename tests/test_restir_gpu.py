@@ -132,7 +132,7 @@ def test_parameter_errors(gpu_ctx):
     with pytest.raises(vt.VTError):
         r.render(scene, 16, 16, vt.orbit_eye(), TARGET, 42.0, sampling_mode=0, candidates=257)
     with pytest.raises(vt.VTError):
-        r.render(scene, 16, 16, vt.orbit_eye(), TARGET, 42.0, sampling_mode=2)
+        r.render(scene, 16, 16, vt.orbit_eye(), TARGET, 42.0, sampling_mode=4)
     r.render(scene, 16, 16, vt.orbit_eye(), TARGET, 42.0, sampling_mode=0, candidates=0, samples=2)
     assert r.frames == 2
     r.close()
