@@ -107,6 +107,7 @@ vt_status vt_trace_stats(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, in
 /* ---------------------------------------------------------------- materials and rendering */
 
 typedef struct vt_renderer vt_renderer;
+typedef struct vt_nrc vt_nrc;
 
 /* One material slot, selected by the face tag (MeshFace.w).  Replaces the per-slot
  * SurfaceSettings / emissions of MaterialLibrary (main.swift:2253-2263, 8926, 9002). */
@@ -154,6 +155,13 @@ typedef struct vt_frame_params {
     uint32_t sample_index;   /* sampleIndex of the first sample of this call (>= 1) */
     uint32_t restir_candidates; /* RIS light candidates per pixel in mode 0 (restir_di_initial's
                                    M = 4, main.swift:4649); 0 means 4, at most 256 */
+    uint32_t radiance_cache;    /* 1: use the renderer's learned radiance cache (vt_renderer_set_cache):
+                                   paths end at their first vertex after a non-delta scatter with the
+                                   cache's answer, 1 in cache_train_period is traced in full and trains
+                                   the cache on its suffixes (no counterpart in the reference);
+                                   2: the same estimator with the cache frozen (no training) */
+    uint32_t cache_train_period; /* 0 means 16 */
+    float cache_learning_rate;   /* 0 means 1e-3 */
 } vt_frame_params;
 
 /* Replaces PathTracerRenderer (main.swift:9514) for this path: owns the accumulation buffer. */
@@ -167,6 +175,9 @@ void vt_renderer_destroy(vt_renderer *r);
  * until a reset, a size change or a sample in another mode.  Asynchronous on `stream`. */
 vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params *params, uint32_t samples,
                           void *stream);
+/* Attaches a learned radiance cache (owned by the caller, which keeps it alive while attached); NULL
+ * detaches.  No counterpart in the reference. */
+vt_status vt_renderer_set_cache(vt_renderer *r, vt_nrc *cache);
 /* Replaces resetAccumulation() (main.swift:10761); also drops the ReSTIR reservoir history
  * (reservoirHistory = 0). */
 vt_status vt_reset_accumulation(vt_renderer *r);
@@ -191,6 +202,26 @@ vt_status vt_spectral_reflectance(const float rgb[3], float out[4]);
  * average use the first 32 x 32 of their block), the tables of
  * Vendor/OpenPBR/impl/data/openpbr_energy_arrays.h.  Either pointer may be NULL; synchronises. */
 vt_status vt_energy_tables(vt_context *ctx, float *metal, float *luts);
+
+/* ---------------------------------------------------------------- learned radiance cache
+ * No counterpart in the reference renderer: a small MLP (BF16 weights on the matrix cores, FP32
+ * accumulation) that maps a path vertex -- 11 numbers in [0, 1]: position in the scene box,
+ * direction (d + 1) / 2, normal (n + 1) / 2, roughness, wavelength number -- to the radiance leaving it
+ * at the path's four wavelengths.  Encoding 12 frequency bands, 4 x 64 ReLU layers, relative-L2 loss,
+ * Adam (beta 0.9 / 0.99).  Parameter layout: metal-vibe-tracer_amd/csrc/nrc.h (30720 floats). */
+#define VT_NRC_PARAMS 30720
+vt_status vt_nrc_create(vt_context *ctx, uint32_t seed, vt_nrc **out);   /* He-uniform initialisation */
+void vt_nrc_destroy(vt_nrc *nrc);
+vt_status vt_nrc_get_params(vt_nrc *nrc, float *params);                 /* host, VT_NRC_PARAMS floats */
+vt_status vt_nrc_set_params(vt_nrc *nrc, const float *params);           /* also resets Adam */
+/* outputs[4 q + c] = radiance of query q at lane c; inputs (count x 11) and outputs are device memory. */
+vt_status vt_nrc_infer(vt_nrc *nrc, const float *inputs, uint32_t count, float *outputs, void *stream);
+/* One Adam step on the batch (inputs count x 11, targets count x 4, device memory). */
+vt_status vt_nrc_train(vt_nrc *nrc, const float *inputs, const float *targets, uint32_t count, float learning_rate,
+                       void *stream);
+/* The batch's loss gradient (host, VT_NRC_PARAMS floats) and mean loss, without updating; synchronises. */
+vt_status vt_nrc_gradients(vt_nrc *nrc, const float *inputs, const float *targets, uint32_t count, float *grads,
+                           float *loss);
 
 /* Host-only build of the same 8-wide BVH the upload builds (no device needed), for inspection:
  * *nodes receives node_count 80-byte node records, *tris face_count 48-byte triangle records

@@ -1,0 +1,341 @@
+// nrc.hip -- gfx950 kernels of the learned radiance cache (nrc.h): fused encode + MLP inference,
+// fused forward + backward with gradient accumulation, Adam.
+//
+// All matrix products run on the matrix cores (v_mfma_f32_16x16x16_bf16: bf16 operands, fp32
+// accumulation).  A wave owns 16 queries at a time.  For D = A (16 x K) . B (K x 16) the operands are
+// read from LDS as lane l: A[l % 16][k0 + 4 (l / 16) .. + 3] and B[k0 + 4 (l / 16) .. + 3][l % 16], and
+// lane l receives D[4 (l / 16) + j][l % 16], j = 0..3.  Weights are stored transposed ([out][in]), so a
+// B operand of a forward product is four consecutive bf16.  The whole bf16 network (60 KB) sits in
+// LDS for the life of a block; inference keeps one 16 x 272 input tile and two 16 x 64 hidden tiles
+// per wave, training keeps a block's 64 samples' activations for the backward pass (144 KB of LDS).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nrc.h"
+
+namespace vt {
+namespace {
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+constexpr int kOffW2 = kNrcW1, kOffW3 = kOffW2 + kNrcWh, kOffW4 = kOffW3 + kNrcWh, kOffW5 = kOffW4 + kNrcWh;
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ f4v mfma(s4v a, s4v b, f4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ s4v load4(const uint16_t *p) { return *reinterpret_cast<const s4v *>(p); }
+__device__ __forceinline__ s4v gather4(const uint16_t *p, int stride) {
+    s4v r;
+    r.x = (short)p[0];
+    r.y = (short)p[stride];
+    r.z = (short)p[2 * stride];
+    r.w = (short)p[3 * stride];
+    return r;
+}
+
+// Copies the bf16 network into LDS (all threads of the block).
+__device__ __forceinline__ void load_weights(uint16_t *w, const uint16_t *src) {
+    const uint4 *s = reinterpret_cast<const uint4 *>(src);
+    uint4 *d = reinterpret_cast<uint4 *>(w);
+    for (int i = threadIdx.x; i < kNrcParams / 8; i += blockDim.x) d[i] = s[i];
+}
+
+// Encodes 16 queries (rows of x, stride 272) from inputs[q0..q0 + 15] (rows >= n are zeros).
+__device__ __forceinline__ void encode_tile(uint16_t *x, const float *inputs, uint32_t q0, uint32_t n, int lane) {
+    for (int idx = lane; idx < 16 * kNrcInputs * kNrcBands; idx += 64) {
+        const int r = idx / (kNrcInputs * kNrcBands), rem = idx - r * (kNrcInputs * kNrcBands);
+        const int i = rem / kNrcBands, b = rem - i * kNrcBands;
+        const uint32_t q = q0 + (uint32_t)r;
+        float s = 0.0f, c = 0.0f;
+        if (q < n) sincospif(inputs[(size_t)q * kNrcInputs + i] * (float)(1 << b), &s, &c);
+        x[r * kNrcFeatures + 24 * i + 2 * b] = to_bf16(s);
+        x[r * kNrcFeatures + 24 * i + 2 * b + 1] = to_bf16(c);
+    }
+    for (int idx = lane; idx < 16 * 8; idx += 64) {
+        const int r = idx >> 3, j = idx & 7;
+        x[r * kNrcFeatures + 264 + j] = (j == 0 && q0 + (uint32_t)r < n) ? to_bf16(1.0f) : (uint16_t)0;
+    }
+}
+
+// h_out (16 x 64, bf16) = relu(x (16 x K) . W^T), W stored [64][K].
+template <int K>
+__device__ __forceinline__ void dense_relu(const uint16_t *x, const uint16_t *w, uint16_t *h_out, int lane) {
+    f4v acc[4] = {};
+    const int r = lane & 15, kq = (lane >> 4) * 4;
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const s4v a = load4(x + r * K + k0 + kq);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[nt] = mfma(a, load4(w + (nt * 16 + r) * K + k0 + kq), acc[nt]);
+    }
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) h_out[(kq + j) * kNrcHidden + nt * 16 + r] = to_bf16(fmaxf(acc[nt][j], 0.0f));
+}
+// y (16 x 16, fp32 in the D layout) = h (16 x 64) . W5^T, W5 stored [16][64].
+__device__ __forceinline__ f4v dense_out(const uint16_t *h, const uint16_t *w5, int lane) {
+    f4v acc = {};
+    const int r = lane & 15, kq = (lane >> 4) * 4;
+#pragma unroll
+    for (int k0 = 0; k0 < kNrcHidden; k0 += 16) acc = mfma(load4(h + r * kNrcHidden + k0 + kq), load4(w5 + r * kNrcHidden + k0 + kq), acc);
+    return acc;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------- inference
+constexpr int kInferLds = kNrcParams * 2 + 4 * (16 * kNrcFeatures + 2 * 16 * kNrcHidden) * 2;
+
+__global__ __launch_bounds__(256) void nrc_infer_kernel(const uint16_t *wsrc, const float *inputs, uint32_t n, float *out,
+                                                        const uint32_t *count) {
+    extern __shared__ __align__(16) uint16_t lds[];
+    if (count) n = min(n, *count);
+    if (n == 0) return;
+    uint16_t *w = lds;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint16_t *x = lds + kNrcParams + wave * (16 * kNrcFeatures + 2 * 16 * kNrcHidden);
+    uint16_t *h0 = x + 16 * kNrcFeatures, *h1 = h0 + 16 * kNrcHidden;
+    load_weights(w, wsrc);
+    __syncthreads();
+    const uint32_t tiles = (n + 15) / 16;
+    for (uint32_t t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
+        const uint32_t q0 = t * 16;
+        encode_tile(x, inputs, q0, n, lane);
+        wave_sync();
+        dense_relu<kNrcFeatures>(x, w, h0, lane);
+        wave_sync();
+        dense_relu<kNrcHidden>(h0, w + kOffW2, h1, lane);
+        wave_sync();
+        dense_relu<kNrcHidden>(h1, w + kOffW3, h0, lane);
+        wave_sync();
+        dense_relu<kNrcHidden>(h0, w + kOffW4, h1, lane);
+        wave_sync();
+        const f4v y = dense_out(h1, w + kOffW5, lane);
+        const int c = lane & 15;
+        if (c < kNrcOut) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t q = q0 + (uint32_t)((lane >> 4) * 4 + j);
+                if (q < n) out[(size_t)q * kNrcOut + c] = y[j];
+            }
+        }
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------- training (forward + backward)
+// LDS: weights, a0 [64][272], a1..a4 [64][64] each, da ping-pong [2][64][64], dy [64][16] (bf16).
+constexpr int kTrainLds = (kNrcParams + 64 * kNrcFeatures + 4 * 64 * kNrcHidden + 2 * 64 * kNrcHidden + 64 * kNrcOutPad) * 2;
+
+// grads[off + o * in + i] += sum over the block's 64 samples of da[q][o] * a[q][i] for one 16 x 16
+// tile (o0, i0); da has `out` columns, a has `in` columns.
+__device__ __forceinline__ void weight_grad_tile(const uint16_t *da, int out, const uint16_t *a, int in, int o0, int i0,
+                                                 float *grads, int lane) {
+    f4v acc = {};
+    const int r = lane & 15, kq = (lane >> 4) * 4;
+#pragma unroll
+    for (int k0 = 0; k0 < 64; k0 += 16)
+        acc = mfma(gather4(da + (k0 + kq) * out + o0 + r, out), gather4(a + (k0 + kq) * in + i0 + r, in), acc);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) atomicAdd(grads + (size_t)(o0 + kq + j) * in + i0 + r, acc[j]);
+}
+// da_out (the wave's 16 rows x 64) = (da_in (16 x K) . W) masked by a_prev > 0, W stored [K][64].
+template <int K>
+__device__ __forceinline__ void back_dense(const uint16_t *da_in, const uint16_t *w, const uint16_t *a_prev, uint16_t *da_out,
+                                           int lane) {
+    f4v acc[4] = {};
+    const int r = lane & 15, kq = (lane >> 4) * 4;
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const s4v a = load4(da_in + r * K + k0 + kq);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[nt] = mfma(a, gather4(w + (k0 + kq) * kNrcHidden + nt * 16 + r, kNrcHidden), acc[nt]);
+    }
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = kq + j, col = nt * 16 + r;
+            const float g = from_bf16(a_prev[row * kNrcHidden + col]) > 0.0f ? acc[nt][j] : 0.0f;
+            da_out[row * kNrcHidden + col] = to_bf16(g);
+        }
+}
+
+__global__ __launch_bounds__(256) void nrc_backward_kernel(const uint16_t *wsrc, const float *inputs, const float *targets,
+                                                           uint32_t n, float norm, float *grads, float *loss_sum,
+                                                           const uint32_t *count, uint32_t offset) {
+    extern __shared__ __align__(16) uint16_t lds[];
+    if (count) {
+        const uint32_t c = *count;
+        n = c > offset ? min(n, c - offset) : 0u;
+    }
+    if (n == 0) return;
+    inputs += (size_t)offset * kNrcInputs;
+    targets += (size_t)offset * kNrcOut;
+    if (!(norm > 0.0f)) norm = (float)n * kNrcOut;
+    uint16_t *w = lds;
+    uint16_t *a0 = w + kNrcParams;
+    uint16_t *a1 = a0 + 64 * kNrcFeatures, *a2 = a1 + 64 * kNrcHidden, *a3 = a2 + 64 * kNrcHidden, *a4 = a3 + 64 * kNrcHidden;
+    uint16_t *da0 = a4 + 64 * kNrcHidden, *da1 = da0 + 64 * kNrcHidden;
+    uint16_t *dy = da1 + 64 * kNrcHidden;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    load_weights(w, wsrc);
+    __syncthreads();
+    const uint32_t groups = (n + 63) / 64;
+    for (uint32_t g = blockIdx.x; g < groups; g += gridDim.x) {
+        const uint32_t q0 = g * 64 + wave * 16;
+        const int rows = wave * 16;
+        // Forward for this wave's 16 samples, keeping every layer's input.
+        encode_tile(a0 + rows * kNrcFeatures, inputs, q0, n, lane);
+        wave_sync();
+        dense_relu<kNrcFeatures>(a0 + rows * kNrcFeatures, w, a1 + rows * kNrcHidden, lane);
+        wave_sync();
+        dense_relu<kNrcHidden>(a1 + rows * kNrcHidden, w + kOffW2, a2 + rows * kNrcHidden, lane);
+        wave_sync();
+        dense_relu<kNrcHidden>(a2 + rows * kNrcHidden, w + kOffW3, a3 + rows * kNrcHidden, lane);
+        wave_sync();
+        dense_relu<kNrcHidden>(a3 + rows * kNrcHidden, w + kOffW4, a4 + rows * kNrcHidden, lane);
+        wave_sync();
+        const f4v y = dense_out(a4 + rows * kNrcHidden, w + kOffW5, lane);
+        // d loss / d y (relative L2, y constant in the denominator), zero for padded outputs and rows.
+        const int c = lane & 15;
+        float lsum = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = (lane >> 4) * 4 + j;
+            const uint32_t q = q0 + (uint32_t)row;
+            float d = 0.0f;
+            if (c < kNrcOut && q < n) {
+                const float t = targets[(size_t)q * kNrcOut + c], den = y[j] * y[j] + 0.01f;
+                d = 2.0f * (y[j] - t) / den / norm;
+                lsum += (y[j] - t) * (y[j] - t) / den;
+            }
+            dy[(rows + row) * kNrcOutPad + c] = to_bf16(d);
+        }
+        if (loss_sum) {
+            for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+            if (lane == 0) atomicAdd(loss_sum, lsum);
+        }
+        __syncthreads();
+        // Output layer: dW5 (16 x 64, one i-tile per wave) and da4 = dy . W5, masked.
+        weight_grad_tile(dy, kNrcOutPad, a4, kNrcHidden, 0, wave * 16, grads + kOffW5, lane);
+        back_dense<kNrcOutPad>(dy + rows * kNrcOutPad, w + kOffW5, a4 + rows * kNrcHidden, da0 + rows * kNrcHidden, lane);
+        __syncthreads();
+        // Hidden layers 4, 3, 2: dW (64 x 64: the wave's o-tile, every i-tile), then the next da.
+        const uint16_t *acts[3] = {a3, a2, a1};
+        const int offs[3] = {kOffW4, kOffW3, kOffW2};
+        uint16_t *cur = da0, *nxt = da1;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+#pragma unroll
+            for (int it = 0; it < 4; ++it) weight_grad_tile(cur, kNrcHidden, acts[l], kNrcHidden, wave * 16, it * 16, grads + offs[l], lane);
+            back_dense<kNrcHidden>(cur + rows * kNrcHidden, w + offs[l], acts[l] + rows * kNrcHidden, nxt + rows * kNrcHidden, lane);
+            __syncthreads();
+            uint16_t *tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+        }
+        // First layer: dW1 (64 x 272): the wave's o-tile, all 17 i-tiles.
+        for (int it = 0; it < kNrcFeatures / 16; ++it)
+            weight_grad_tile(cur, kNrcHidden, a0, kNrcFeatures, wave * 16, it * 16, grads, lane);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void nrc_adam_kernel(float *params, float *grads, float *m, float *v, uint16_t *wbf16,
+                                                       float lr, float c1, float c2, const uint32_t *count, uint32_t offset) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kNrcParams) return;
+    if (count && *count <= offset) {
+        grads[i] = 0.0f;
+        return;
+    }
+    const float g = grads[i];
+    const float mi = 0.9f * m[i] + 0.1f * g;
+    const float vi = 0.99f * v[i] + 0.01f * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float p = params[i] - lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+    params[i] = p;
+    wbf16[i] = to_bf16(p);
+    grads[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void nrc_sync_kernel(const float *params, uint16_t *wbf16) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < kNrcParams) wbf16[i] = to_bf16(params[i]);
+}
+
+__global__ __launch_bounds__(256) void nrc_zero_kernel(float *p, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+int grid_for(uint32_t work, int per_block) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint32_t blocks = (work + per_block - 1) / per_block;
+    return (int)(blocks < (uint32_t)cus ? (blocks > 0 ? blocks : 1) : (uint32_t)cus);
+}
+
+}  // namespace
+
+hipError_t launch_nrc_infer(const NrcDevice &d, const float *inputs, uint32_t n, float *out, hipStream_t s,
+                            const uint32_t *count) {
+    if (n == 0) return hipSuccess;
+    static bool configured = false;
+    if (!configured) {
+        hipError_t e = hipFuncSetAttribute((const void *)nrc_infer_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, kInferLds);
+        if (e != hipSuccess) return e;
+        configured = true;
+    }
+    hipLaunchKernelGGL(nrc_infer_kernel, dim3(grid_for(n, 64)), dim3(256), kInferLds, s, d.wbf16, inputs, n, out, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_nrc_backward(const NrcDevice &d, const float *inputs, const float *targets, uint32_t n, float norm,
+                               float *loss_sum, hipStream_t s, const uint32_t *count, uint32_t offset) {
+    if (n == 0) return hipSuccess;
+    static bool configured = false;
+    if (!configured) {
+        hipError_t e = hipFuncSetAttribute((const void *)nrc_backward_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, kTrainLds);
+        if (e != hipSuccess) return e;
+        configured = true;
+    }
+    hipLaunchKernelGGL(nrc_backward_kernel, dim3(grid_for(n, 64)), dim3(256), kTrainLds, s, d.wbf16, inputs, targets, n, norm,
+                       d.grads, loss_sum, count, offset);
+    return hipGetLastError();
+}
+
+hipError_t launch_nrc_adam(const NrcDevice &d, uint32_t step, float lr, hipStream_t s, const uint32_t *count, uint32_t offset) {
+    const float c1 = 1.0f - powf(0.9f, (float)step), c2 = 1.0f - powf(0.99f, (float)step);
+    hipLaunchKernelGGL(nrc_adam_kernel, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d.params, d.grads, d.m, d.v, d.wbf16,
+                       lr, c1, c2, count, offset);
+    return hipGetLastError();
+}
+
+hipError_t launch_nrc_sync(const NrcDevice &d, hipStream_t s) {
+    hipLaunchKernelGGL(nrc_sync_kernel, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d.params, d.wbf16);
+    return hipGetLastError();
+}
+
+hipError_t launch_nrc_zero(float *p, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(nrc_zero_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace vt

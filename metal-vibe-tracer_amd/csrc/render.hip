@@ -14,6 +14,8 @@
 #include "render_kernels.h"
 #include "traverse8.h"
 
+constexpr int kNrcInputCount = 11;
+
 namespace vt {
 namespace {
 
@@ -626,6 +628,32 @@ constexpr uint32_t kPrevDelta = 1u << 16, kPrevNee = 1u << 17, kPrevMis = 1u << 
 // emission (main.swift:7716-7722, 7785).
 constexpr uint32_t kEmissionOnly = 1u << 19;
 
+// ---------------------------------------------------------------- learned radiance cache
+// Training paths: 1 in f.train_period, by a hash of pixel and sample index.
+__device__ __forceinline__ bool nrc_training(const FrameArgs &f, uint32_t id) {
+    const uint32_t smp = id / f.npix, pix = id - smp * f.npix;
+    return pcg_hash((pix * 0x9E3779B9u) ^ ((f.sample_index0 + smp) * 0x85EBCA6Bu)) % f.train_period == 0u;
+}
+// The 11 cache inputs of a vertex: position in the scene box, direction toward the previous vertex,
+// shading normal (both mapped to [0, 1]), roughness (1 for Lambert), wavelength number.
+__device__ __forceinline__ void nrc_inputs(const FrameArgs &f, const Surface &s, const Mat &m, V3 dir, float wl, float *out) {
+    out[0] = fminf(fmaxf((s.p.x - f.box_lo[0]) * f.box_scale[0], 0.0f), 1.0f);
+    out[1] = fminf(fmaxf((s.p.y - f.box_lo[1]) * f.box_scale[1], 0.0f), 1.0f);
+    out[2] = fminf(fmaxf((s.p.z - f.box_lo[2]) * f.box_scale[2], 0.0f), 1.0f);
+    out[3] = (1.0f - dir.x) * 0.5f;
+    out[4] = (1.0f - dir.y) * 0.5f;
+    out[5] = (1.0f - dir.z) * 0.5f;
+    out[6] = (s.n.x + 1.0f) * 0.5f;
+    out[7] = (s.n.y + 1.0f) * 0.5f;
+    out[8] = (s.n.z + 1.0f) * 0.5f;
+    out[9] = m.type == kDiffuse ? 1.0f : fminf(fmaxf(m.roughness, 0.0f), 1.0f);
+    out[10] = wl;
+}
+__device__ __forceinline__ void add_f4(float4 *p, S4 v) {
+    float4 a = *p;
+    *p = make_float4(a.x + v.v[0], a.y + v.v[1], a.z + v.v[2], a.w + v.v[3]);
+}
+
 // ---------------------------------------------------------------- kernels
 __global__ __launch_bounds__(256) void generate_kernel(FrameArgs f, PathQueue q, float *radiance, uint32_t *count) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -774,7 +802,7 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
 template <bool kRestir, bool kLayered>
 __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f, PathQueue in, HitArgs hits,
                                                     PathQueue out, ShadowQueue sh, Counters c, uint32_t capacity,
-                                                    float *radiance, RestirArgs ra) {
+                                                    float *radiance, RestirArgs ra, NrcArgs nrc) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const uint32_t n = min(*c.in, capacity);
     const bool active = i < n;
@@ -782,7 +810,11 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
     // Continuation and shadow-ray values produced below (written after the wave-wide appends).
     V3 new_o = v3(0.0f, 0.0f, 0.0f), new_d = v3(0.0f, 0.0f, 1.0f), prev_p = v3(0.0f, 0.0f, 0.0f);
     V3 sh_o = new_o, sh_d = new_d, sh_c = new_o;
+    S4 sh_s = s4(0.0f);
     float sh_tmin = 0.0f, sh_tmax = 0.0f, new_pdf = 0.0f;
+    // Cache query (rendering paths) or training record (training paths) of this vertex.
+    bool query = false, record = false, training = false;
+    float cin[kNrcInputCount];
     uint32_t id = 0, seed = 0, state = 0;
     float wl = 0.0f;
     S4 thr = s4(0.0f);
@@ -814,12 +846,26 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
                     float light_pdf = emitter_pdf(sc, prev, s.p, (uint32_t)face, m.weight);
                     float w = emission_weight((state & kPrevDelta) != 0u, (state & kPrevNee) != 0u,
                                               (state & kPrevMis) != 0u, in.pdf[i], light_pdf);
-                    add = spectrum_rgb(sp, lanes, scale(mul(thr, emission_spectrum(sp, lanes, e)), w));
+                    const S4 se = scale(mul(thr, emission_spectrum(sp, lanes, e)), w);
+                    add = spectrum_rgb(sp, lanes, se);
+                    if (f.nrc && nrc_training(f, id)) add_f4(nrc.sacc + id, se);
                 }
                 alive = false;
             }
         }
         if (alive && (state & kEmissionOnly) != 0u) alive = false;
+        if (f.nrc && alive && bounce > 0u && (state & kPrevDelta) == 0u) {
+            // The first vertex after a non-delta scatter: rendering paths end here with the cache's
+            // answer; training paths go on and record the vertex.
+            training = nrc_training(f, id);
+            nrc_inputs(f, s, m, dir, wl, cin);
+            if (training) {
+                record = true;
+            } else {
+                query = true;
+                alive = false;
+            }
+        }
         if (kRestir && alive && bounce == 0u && m.type == kDiffuse) {
             // Spatial reuse: the pixel's reservoir and four uniform taps in a 16-pixel box
             // (uniform_neighbor, restir2020_compatible), combined with w = p_hat W M.
@@ -863,7 +909,7 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
                     const S4 fb = diffuse_f(s, scale(reflectance(sp, sl, m.lagrange), 1.0f / kPi), -dir, wi);
                     const S4 v = scale(scale(mul(scale(fb, ct), emission_spectrum(sp, sl, sel.emission)), W),
                                        light_geometry(sc, s.p, sel));
-                    sh_c = spectrum_rgb(sp, sl, v);
+                    sh_c = spectrum_rgb(sp, sl, v);  // at the sample's own lanes; not a cache target (primary)
                     sh_o = o;
                     sh_d = normalize(delta);
                     sh_tmin = ray_t_min(o);
@@ -887,7 +933,8 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
                         const float cosine = fabsf(dot(s.n, ls.wi));
                         const float w = use_mis ? power_heuristic(ls.pdf, pdf) : 1.0f;
                         const S4 le = emission_spectrum(sp, lanes, ls.emission);
-                        sh_c = spectrum_rgb(sp, lanes, scale(mul(mul(thr, fbsdf), le), cosine * w / ls.pdf));
+                        sh_s = scale(mul(mul(thr, fbsdf), le), cosine * w / ls.pdf);
+                        sh_c = spectrum_rgb(sp, lanes, sh_s);
                         sh_o = o;
                         sh_d = normalize(delta);
                         sh_tmin = ray_t_min(o);
@@ -944,6 +991,22 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
     }
     const uint32_t slot = wave_append(c.out, cont);
     const uint32_t sslot = wave_append(c.shadow, shadow);
+    if (f.nrc) {
+        const uint32_t qslot = wave_append(nrc.q_count, query);
+        const uint32_t rslot = wave_append(nrc.r_count, record);
+        if (query && qslot < capacity) {
+            for (int k = 0; k < kNrcInputCount; ++k) nrc.q_in[(size_t)qslot * kNrcInputCount + k] = cin[k];
+            nrc.q_thr[qslot] = make_float4(thr.v[0], thr.v[1], thr.v[2], thr.v[3]);
+            nrc.q_wl[qslot] = wl;
+            nrc.q_id[qslot] = id;
+        }
+        if (record && rslot < nrc.r_cap) {
+            for (int k = 0; k < kNrcInputCount; ++k) nrc.r_in[(size_t)rslot * kNrcInputCount + k] = cin[k];
+            nrc.r_thr[rslot] = make_float4(thr.v[0], thr.v[1], thr.v[2], thr.v[3]);
+            nrc.r_snap[rslot] = nrc.sacc[id];
+            nrc.r_id[rslot] = id;
+        }
+    }
     if (cont && slot < capacity) {
         out.ox[slot] = new_o.x; out.oy[slot] = new_o.y; out.oz[slot] = new_o.z;
         out.dx[slot] = new_d.x; out.dy[slot] = new_d.y; out.dz[slot] = new_d.z;
@@ -964,11 +1027,13 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
         sh.tmax[sslot] = sh_tmax;
         sh.id[sslot] = id;
         sh.r[sslot] = sh_c.x; sh.g[sslot] = sh_c.y; sh.b[sslot] = sh_c.z;
+        sh.s[sslot] = make_float4(sh_s.v[0], sh_s.v[1], sh_s.v[2], sh_s.v[3]);
     }
 }
 
-__global__ __launch_bounds__(256) void resolve_kernel(ShadowQueue sh, const uint8_t *occluded, const uint32_t *count,
-                                                      uint32_t capacity, float *radiance) {
+__global__ __launch_bounds__(256) void resolve_kernel(FrameArgs f, ShadowQueue sh, const uint8_t *occluded,
+                                                      const uint32_t *count, uint32_t capacity, float *radiance,
+                                                      NrcArgs nrc) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= min(*count, capacity)) return;
     if (occluded[i]) return;
@@ -976,6 +1041,47 @@ __global__ __launch_bounds__(256) void resolve_kernel(ShadowQueue sh, const uint
     radiance[3 * (size_t)id] += sh.r[i];
     radiance[3 * (size_t)id + 1] += sh.g[i];
     radiance[3 * (size_t)id + 2] += sh.b[i];
+    if (f.nrc && nrc_training(f, id)) {
+        const float4 v = sh.s[i];
+        add_f4(nrc.sacc + id, S4{{v.x, v.y, v.z, v.w}});
+    }
+}
+
+// radiance[id] += rgb(thr * max(y, 0)) for every cache query (its answer y at the path's lanes).
+__global__ __launch_bounds__(256) void cache_resolve_kernel(RenderScene sc, NrcArgs nrc, uint32_t capacity, float *radiance) {
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= min(*nrc.q_count, capacity)) return;
+    const Spectral sp{sc.icdf, reinterpret_cast<const float4 *>(sc.cmf), reinterpret_cast<const float4 *>(sc.basis)};
+    const Lanes lanes = lanes_of(sp, nrc.q_wl[q]);
+    const float4 t = nrc.q_thr[q];
+    const float *y = nrc.q_out + 4 * (size_t)q;
+    const S4 v{{t.x * fmaxf(y[0], 0.0f), t.y * fmaxf(y[1], 0.0f), t.z * fmaxf(y[2], 0.0f), t.w * fmaxf(y[3], 0.0f)}};
+    const V3 rgb = spectrum_rgb(sp, lanes, v);
+    if (!(isfinite(rgb.x) && isfinite(rgb.y) && isfinite(rgb.z))) return;
+    const uint32_t id = nrc.q_id[q];
+    radiance[3 * (size_t)id] += rgb.x;
+    radiance[3 * (size_t)id + 1] += rgb.y;
+    radiance[3 * (size_t)id + 2] += rgb.z;
+}
+
+// Training target of each record: the spectral radiance its path gathered after reaching the vertex,
+// per unit throughput.
+__global__ __launch_bounds__(256) void build_targets_kernel(NrcArgs nrc) {
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= min(*nrc.r_count, nrc.r_cap)) return;
+    const float4 fin = nrc.sacc[nrc.r_id[r]], snap = nrc.r_snap[r], t = nrc.r_thr[r];
+    float *out = nrc.r_target + 4 * (size_t)r;
+    out[0] = t.x > 0.0f ? (fin.x - snap.x) / t.x : 0.0f;
+    out[1] = t.y > 0.0f ? (fin.y - snap.y) / t.y : 0.0f;
+    out[2] = t.z > 0.0f ? (fin.z - snap.z) / t.z : 0.0f;
+    out[3] = t.w > 0.0f ? (fin.w - snap.w) / t.w : 0.0f;
+    for (int k = 0; k < 4; ++k)
+        if (!isfinite(out[k])) out[k] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void zero_f4_kernel(float4 *p, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------- energy-table bakes
@@ -1073,27 +1179,27 @@ hipError_t launch_generate(const FrameArgs &f, const PathQueue &q, float *radian
 
 hipError_t launch_shade(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
                         const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
-                        float *radiance, hipStream_t s) {
+                        float *radiance, const NrcArgs &nrc, hipStream_t s) {
     const dim3 grid((capacity + 255) / 256);
     if (sc.layered)
         hipLaunchKernelGGL((shade_kernel<false, true>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
-                           radiance, RestirArgs{});
+                           radiance, RestirArgs{}, nrc);
     else
         hipLaunchKernelGGL((shade_kernel<false, false>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
-                           radiance, RestirArgs{});
+                           radiance, RestirArgs{}, nrc);
     return hipGetLastError();
 }
 
 hipError_t launch_shade_restir(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
                                const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
-                               float *radiance, const RestirArgs &ra, hipStream_t s) {
+                               float *radiance, const RestirArgs &ra, const NrcArgs &nrc, hipStream_t s) {
     const dim3 grid((capacity + 255) / 256);
     if (sc.layered)
         hipLaunchKernelGGL((shade_kernel<true, true>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
-                           radiance, ra);
+                           radiance, ra, nrc);
     else
         hipLaunchKernelGGL((shade_kernel<true, false>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
-                           radiance, ra);
+                           radiance, ra, nrc);
     return hipGetLastError();
 }
 
@@ -1103,10 +1209,27 @@ hipError_t launch_restir_initial(const RenderScene &sc, const FrameArgs &f, cons
     return hipGetLastError();
 }
 
-hipError_t launch_resolve(const ShadowQueue &sh, const uint8_t *occluded, const uint32_t *count, uint32_t capacity,
-                          float *radiance, hipStream_t s) {
-    hipLaunchKernelGGL(resolve_kernel, dim3((capacity + 255) / 256), dim3(256), 0, s, sh, occluded, count, capacity,
-                       radiance);
+hipError_t launch_resolve(const FrameArgs &f, const ShadowQueue &sh, const uint8_t *occluded, const uint32_t *count,
+                          uint32_t capacity, float *radiance, const NrcArgs &nrc, hipStream_t s) {
+    hipLaunchKernelGGL(resolve_kernel, dim3((capacity + 255) / 256), dim3(256), 0, s, f, sh, occluded, count, capacity,
+                       radiance, nrc);
+    return hipGetLastError();
+}
+
+hipError_t launch_cache_resolve(const RenderScene &sc, const NrcArgs &nrc, uint32_t capacity, float *radiance,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(cache_resolve_kernel, dim3((capacity + 255) / 256), dim3(256), 0, s, sc, nrc, capacity, radiance);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_targets(const NrcArgs &nrc, hipStream_t s) {
+    hipLaunchKernelGGL(build_targets_kernel, dim3((nrc.r_cap + 255) / 256), dim3(256), 0, s, nrc);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_f4(float4 *p, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_f4_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 

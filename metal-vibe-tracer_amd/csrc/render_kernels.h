@@ -72,6 +72,29 @@ struct FrameArgs {
     float fov_scale, aspect;
     int32_t scattering_limit;  // max(0, depth - 1)
     uint32_t sampling_mode;    // 0 ReSTIR DI at the primary vertex, 1 MIS, 2 NEE only, 3 BSDF only
+    // Learned radiance cache: 1 = paths end at their first vertex reached by a non-delta scatter and
+    // take its outgoing radiance from the cache, except training paths (1 in train_period, by a hash
+    // of pixel and sample), which are traced in full and leave one training record per such vertex.
+    uint32_t nrc, train_period;
+    float box_lo[3], box_scale[3];  // cache inputs: (p - lo) * scale in [0, 1]
+};
+
+// Queues of the learned radiance cache (indexed by queue slot unless stated).
+struct NrcArgs {
+    float *q_in;         // [cap][11] queries: inputs of the vertex where a path ends
+    float4 *q_thr;       // throughput reaching it, per lane
+    float *q_wl;         // wavelength number
+    uint32_t *q_id;      // path id
+    uint32_t *q_count;
+    float *q_out;        // [cap][4] cache outputs
+    float4 *sacc;        // [paths] spectral radiance a training path has gathered so far (by path id)
+    float *r_in;         // [rcap][11] training records: inputs of a vertex of a training path
+    float4 *r_thr;       // throughput reaching it
+    float4 *r_snap;      // sacc of its path when it was reached
+    uint32_t *r_id;
+    uint32_t *r_count;
+    uint32_t r_cap;
+    float *r_target;     // [rcap][4] built after the wavefront: (sacc final - snap) / thr
 };
 
 // One queue of paths: the ray arrays first (RayArgs layout), then the path state.
@@ -90,6 +113,7 @@ struct ShadowQueue {
     float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
     uint32_t *id;
     float *r, *g, *b;   // contribution if unoccluded
+    float4 *s;          // the same contribution per wavelength lane (for cache training)
 };
 
 struct Counters {
@@ -99,7 +123,7 @@ struct Counters {
 hipError_t launch_generate(const FrameArgs &f, const PathQueue &q, float *radiance, uint32_t *count, hipStream_t s);
 hipError_t launch_shade(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
                         const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
-                        float *radiance, hipStream_t s);
+                        float *radiance, const NrcArgs &nrc, hipStream_t s);
 // Pass 1 of ReSTIR DI (restir_temporal_kernel): G-buffer, RIS candidates, temporal merge, encode,
 // for the wavefront's single sample (q holds the camera paths in pixel order, hits their hits).
 hipError_t launch_restir_initial(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
@@ -108,9 +132,15 @@ hipError_t launch_restir_initial(const RenderScene &sc, const FrameArgs &f, cons
 // shading with one visibility ray, then the path suffix.
 hipError_t launch_shade_restir(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
                                const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
-                               float *radiance, const RestirArgs &ra, hipStream_t s);
-hipError_t launch_resolve(const ShadowQueue &sh, const uint8_t *occluded, const uint32_t *count, uint32_t capacity,
-                          float *radiance, hipStream_t s);
+                               float *radiance, const RestirArgs &ra, const NrcArgs &nrc, hipStream_t s);
+hipError_t launch_resolve(const FrameArgs &f, const ShadowQueue &sh, const uint8_t *occluded, const uint32_t *count,
+                          uint32_t capacity, float *radiance, const NrcArgs &nrc, hipStream_t s);
+// Adds each cache query's answer: radiance[id] += rgb(thr * max(y, 0)) at the path's wavelengths.
+hipError_t launch_cache_resolve(const RenderScene &sc, const NrcArgs &nrc, uint32_t capacity, float *radiance,
+                                hipStream_t s);
+// Training targets of the wavefront's records: (sacc[id] - snap) / thr per lane (0 where thr is 0).
+hipError_t launch_build_targets(const NrcArgs &nrc, hipStream_t s);
+hipError_t launch_zero_f4(float4 *p, uint32_t n, hipStream_t s);
 // Energy-table bakes: ideal-metal complements (129 x 129, Hammersley 16384 per node) and the
 // dielectric single-scattering albedos E_R, E_T (32 x 32 x 32, Hammersley 1024 per node).
 hipError_t launch_bake_metal(float *out, hipStream_t s);

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "lighttree.h"
+#include "nrc.h"
 #include "render_kernels.h"
 #include "scene.h"
 #include "spectral.h"
@@ -22,7 +23,7 @@ namespace {
 
 // Path-state arrays of one queue: 8 ray arrays, then id, seed, wl, t0-t3, state, pdf, px, py, pz.
 constexpr int kQueueArrays = 8 + 12;
-constexpr int kShadowArrays = 8 + 4;
+constexpr int kShadowArrays = 8 + 4 + 4;  // rays, id, rgb, lane contributions (float4)
 
 struct QueueStorage {
     DeviceBuffer buf;
@@ -54,6 +55,12 @@ struct vt_renderer {
     uint32_t restir_history = 0;
     uint64_t restir_generation = 0;  // vt_scene::generation the history was rendered with
     uint32_t *host_count = nullptr;  // pinned: live paths, read between vertices of delta scenes
+    // Learned radiance cache: the attached network and its queues (sized with the path capacity).
+    vt_nrc *cache = nullptr;
+    DeviceBuffer nrc_queries, nrc_records, nrc_sacc;
+    uint32_t nrc_capacity = 0, nrc_rcap = 0;
+    vt::NrcDevice cache_device() const { return vt::nrc_device(cache); }
+    uint32_t cache_next_step() { return vt::nrc_next_step(cache); }
     ~vt_renderer() {
         if (host_count) (void)hipHostFree(host_count);
     }
@@ -92,7 +99,44 @@ vt::ShadowQueue shadow_view(const vt_renderer *r) {
     s.ox = f(0); s.oy = f(1); s.oz = f(2); s.dx = f(3); s.dy = f(4); s.dz = f(5); s.tmin = f(6); s.tmax = f(7);
     s.id = reinterpret_cast<uint32_t *>(f(8));
     s.r = f(9); s.g = f(10); s.b = f(11);
+    s.s = reinterpret_cast<float4 *>(f(12));
     return s;
+}
+
+// Cache queues: queries (11 inputs, throughput, wavelength, id, 4 outputs) per path slot; records
+// (11 inputs, throughput, snapshot, id, 4 targets) for 8 vertices of each training path.
+void ensure_nrc_buffers(vt_renderer *r, uint32_t period) {
+    // Rounded to 64 entries so every float4 array of the layout stays 16-byte aligned.
+    const uint32_t cap = (r->capacity + 63u) & ~63u;
+    const uint32_t rcap = ((r->capacity / period * 8 + 1024) + 63u) & ~63u;
+    if (r->nrc_capacity >= cap && r->nrc_rcap >= rcap) return;
+    r->nrc_queries.alloc((size_t)cap * (11 + 4 + 1 + 1 + 4) * 4);
+    r->nrc_records.alloc((size_t)rcap * (11 + 4 + 4 + 1 + 4) * 4);
+    r->nrc_sacc.alloc((size_t)cap * 16);
+    r->nrc_capacity = cap;
+    r->nrc_rcap = rcap;
+}
+
+vt::NrcArgs nrc_view(vt_renderer *r, uint32_t *counters) {
+    vt::NrcArgs a{};
+    const size_t cap = r->nrc_capacity, rcap = r->nrc_rcap;
+    char *q = r->nrc_queries.as<char>();
+    a.q_in = reinterpret_cast<float *>(q);
+    a.q_thr = reinterpret_cast<float4 *>(q + cap * 11 * 4);
+    a.q_wl = reinterpret_cast<float *>(q + cap * 15 * 4);
+    a.q_id = reinterpret_cast<uint32_t *>(q + cap * 16 * 4);
+    a.q_out = reinterpret_cast<float *>(q + cap * 17 * 4);
+    a.q_count = counters + 3;
+    char *rr = r->nrc_records.as<char>();
+    a.r_in = reinterpret_cast<float *>(rr);
+    a.r_thr = reinterpret_cast<float4 *>(rr + rcap * 11 * 4);
+    a.r_snap = reinterpret_cast<float4 *>(rr + rcap * 15 * 4);
+    a.r_id = reinterpret_cast<uint32_t *>(rr + rcap * 19 * 4);
+    a.r_target = reinterpret_cast<float *>(rr + rcap * 20 * 4);
+    a.r_count = counters + 4;
+    a.r_cap = (uint32_t)rcap;
+    a.sacc = r->nrc_sacc.as<float4>();
+    return a;
 }
 
 void normalize3(float *v) {
@@ -329,6 +373,12 @@ vt_status vt_reset_accumulation(vt_renderer *r) {
 
 uint32_t vt_accum_frames(const vt_renderer *r) { return r ? r->frame_index : 0; }
 
+vt_status vt_renderer_set_cache(vt_renderer *r, vt_nrc *cache) {
+    if (!r) return fail(VT_ERROR_INVALID_ARGUMENT, "renderer must not be NULL");
+    r->cache = cache;
+    return VT_OK;
+}
+
 vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params *p, uint32_t samples, void *stream) {
     return vt::guarded_call([&]() -> vt_status {
         if (!r || !scene || !p) return fail(VT_ERROR_INVALID_ARGUMENT, "renderer, scene and params must not be NULL");
@@ -387,6 +437,35 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
         f.aspect = (float)p->width / (float)p->height;
         f.scattering_limit = std::max(0, (int)p->max_depth - 1);
         f.sampling_mode = p->sampling_mode;
+        if (p->radiance_cache > 2) return fail(VT_ERROR_INVALID_ARGUMENT, "radiance_cache must be 0, 1 or 2");
+        const bool use_cache = p->radiance_cache != 0, train_cache = p->radiance_cache == 1;
+        if (use_cache && !r->cache) return fail(VT_ERROR_INVALID_ARGUMENT, "radiance_cache needs vt_renderer_set_cache");
+        const uint32_t period = p->cache_train_period ? p->cache_train_period : 16u;
+        const float lr = p->cache_learning_rate > 0.0f ? p->cache_learning_rate : 1e-3f;
+        if (use_cache && (period < 2 || period > 4096)) return fail(VT_ERROR_INVALID_ARGUMENT, "cache_train_period must be 2..4096");
+        f.nrc = use_cache ? 1u : 0u;
+        f.train_period = period;
+        if (use_cache) {
+            if (!scene->box_valid) {
+                float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+                for (size_t v = 0; v < scene->host_vertices.size() / 8; ++v)
+                    for (int k = 0; k < 3; ++k) {
+                        lo[k] = std::min(lo[k], scene->host_vertices[8 * v + k]);
+                        hi[k] = std::max(hi[k], scene->host_vertices[8 * v + k]);
+                    }
+                for (int k = 0; k < 3; ++k) {
+                    scene->box_lo[k] = lo[k];
+                    scene->box_hi[k] = hi[k];
+                }
+                scene->box_valid = true;
+            }
+            for (int k = 0; k < 3; ++k) {
+                f.box_lo[k] = scene->box_lo[k];
+                const float ext = scene->box_hi[k] - scene->box_lo[k];
+                f.box_scale[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
+            }
+            ensure_nrc_buffers(r, period);
+        }
 
         const vt::RenderScene sc = scene->render_view();
         uint32_t *cnt = r->counters.as<uint32_t>();
@@ -402,6 +481,11 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
             check(vt::launch_reset_counters(cnt, 16, s), "reset");
             int cur = 0;
             check(vt::launch_generate(f, r->q[0].view(r->capacity), r->radiance.as<float>(), cnt + 0, s), "generate");
+            vt::NrcArgs na{};
+            if (use_cache) {
+                na = nrc_view(r, cnt);
+                check(vt::launch_zero_f4(na.sacc, n, s), "zero");
+            }
             vt::RestirArgs ra{};
             if (restir) {
                 ra.cur = restir_view(r->restir[r->restir_cur], npix);
@@ -434,16 +518,30 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
                 vt::Counters c{cin, cout, csh};
                 if (restir && b == 0) {
                     check(vt::launch_restir_initial(sc, f, qin, hits, ra, s), "restir pass 1");
-                    check(vt::launch_shade_restir(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), ra, s),
+                    check(vt::launch_shade_restir(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), ra, na, s),
                           "shade");
                 } else {
-                    check(vt::launch_shade(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), s), "shade");
+                    check(vt::launch_shade(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), na, s), "shade");
                 }
                 vt::RayArgs srays{sh.ox, sh.oy, sh.oz, sh.dx, sh.dy, sh.dz, sh.tmin, sh.tmax};
                 check(vt::launch_trace(scene->view, srays, vt::HitArgs{}, r->occluded.as<uint8_t>(), n, true, nullptr, s, csh),
                       "shadow");
-                check(vt::launch_resolve(sh, r->occluded.as<uint8_t>(), csh, n, r->radiance.as<float>(), s), "resolve");
+                check(vt::launch_resolve(f, sh, r->occluded.as<uint8_t>(), csh, n, r->radiance.as<float>(), na, s), "resolve");
                 cur = 1 - cur;
+            }
+            if (use_cache) {
+                // The cache answers the paths that ended at its vertices, then learns from this
+                // wavefront's training paths: up to 4 Adam steps of 16384 records.
+                const vt::NrcDevice nd = r->cache_device();
+                check(vt::launch_nrc_infer(nd, na.q_in, n, na.q_out, s, na.q_count), "cache infer");
+                check(vt::launch_cache_resolve(sc, na, n, r->radiance.as<float>(), s), "cache resolve");
+                check(vt::launch_build_targets(na, s), "cache targets");
+                for (uint32_t off = 0; train_cache && off < na.r_cap && off < 4u * 16384u; off += 16384u) {
+                    const uint32_t batch = std::min<uint32_t>(16384u, na.r_cap - off);
+                    check(vt::launch_nrc_backward(nd, na.r_in, na.r_target, batch, 0.0f, nullptr, s, na.r_count, off),
+                          "cache backward");
+                    check(vt::launch_nrc_adam(nd, r->cache_next_step(), lr, s, na.r_count, off), "cache adam");
+                }
             }
             check(vt::launch_accumulate(f, r->radiance.as<float>(), r->accum.as<float>(), r->frame_index + 1, s),
                   "accumulate");

@@ -29,6 +29,8 @@ EXPORTED = [
     "vt_scene_set_materials", "vt_renderer_create", "vt_renderer_destroy", "vt_render_frame", "vt_reset_accumulation",
     "vt_accum_readback", "vt_accum_frames", "vt_gen_cornell", "vt_spectral_tables", "vt_spectral_reflectance",
     "vt_scene_set_light_tree", "vt_light_tree_build_host", "vt_energy_tables",
+    "vt_nrc_create", "vt_nrc_destroy", "vt_nrc_get_params", "vt_nrc_set_params", "vt_nrc_infer", "vt_nrc_train",
+    "vt_nrc_gradients", "vt_renderer_set_cache",
 ]
 
 
@@ -74,7 +76,8 @@ class _FrameParams(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("eye", ctypes.c_float * 3),
                 ("target", ctypes.c_float * 3), ("fov_degrees", ctypes.c_float), ("max_depth", ctypes.c_uint32),
                 ("sampling_mode", ctypes.c_uint32), ("sample_index", ctypes.c_uint32),
-                ("restir_candidates", ctypes.c_uint32)]
+                ("restir_candidates", ctypes.c_uint32), ("radiance_cache", ctypes.c_uint32),
+                ("cache_train_period", ctypes.c_uint32), ("cache_learning_rate", ctypes.c_float)]
 
 
 class _RaySoA(ctypes.Structure):
@@ -123,6 +126,14 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_render_frame": (i32, [vp, vp, P(_FrameParams), u32, vp]),
         "vt_reset_accumulation": (i32, [vp]),
         "vt_energy_tables": (i32, [vp, vp, vp]),
+        "vt_nrc_create": (i32, [vp, u32, P(vp)]),
+        "vt_nrc_destroy": (None, [vp]),
+        "vt_nrc_get_params": (i32, [vp, vp]),
+        "vt_nrc_set_params": (i32, [vp, vp]),
+        "vt_nrc_infer": (i32, [vp, vp, u32, vp, vp]),
+        "vt_nrc_train": (i32, [vp, vp, vp, u32, c.c_float, vp]),
+        "vt_nrc_gradients": (i32, [vp, vp, vp, u32, vp, vp]),
+        "vt_renderer_set_cache": (i32, [vp, vp]),
         "vt_accum_readback": (i32, [vp, vp, c.c_size_t]),
         "vt_accum_frames": (u32, [vp]),
         "vt_spectral_tables": (i32, [vp, vp, vp]),
@@ -459,14 +470,19 @@ class Renderer:
             pass
 
     def render(self, scene: Scene, width, height, eye, target, fov, max_depth=8, samples=1, sample_index=1,
-               sampling_mode=1, candidates=4, stream=None):
+               sampling_mode=1, candidates=4, cache=False, train_period=16, learning_rate=1e-3, stream=None):
         p = _FrameParams(width, height, (ctypes.c_float * 3)(*eye), (ctypes.c_float * 3)(*target), fov, max_depth,
-                         sampling_mode, sample_index, candidates)
+                         sampling_mode, sample_index, candidates, int(cache), train_period, learning_rate)
         _check(load().vt_render_frame(self._h, scene._h, ctypes.byref(p), samples, _stream_ptr(stream)))
         self.size = (width, height)
 
     def reset(self):
         _check(load().vt_reset_accumulation(self._h))
+
+    def set_cache(self, cache):
+        """Attach a vt.NRC (None detaches); keep the cache alive while attached."""
+        self._cache = cache
+        _check(load().vt_renderer_set_cache(self._h, cache._h if cache is not None else None))
 
     @property
     def frames(self) -> int:
@@ -477,3 +493,69 @@ class Renderer:
         out = np.empty((h, w, 4), np.float32)
         _check(load().vt_accum_readback(self._h, _ptr(out), out.nbytes))
         return out
+
+
+NRC_PARAMS = 30720
+
+
+class NRC:
+    """The learned radiance cache (vt_nrc_*): inputs (n, 11) and outputs (n, 4) are float32 device
+    tensors; parameters move as host numpy arrays."""
+
+    def __init__(self, ctx: "Context", seed: int = 1):
+        h = ctypes.c_void_p()
+        _check(load().vt_nrc_create(ctx._h, seed, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            load().vt_nrc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.empty(NRC_PARAMS, np.float32)
+        _check(load().vt_nrc_get_params(self._h, _ptr(p)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (NRC_PARAMS,)
+        _check(load().vt_nrc_set_params(self._h, _ptr(p)))
+
+    @staticmethod
+    def _dev(t, cols):
+        import torch
+        assert isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+        assert t.dim() == 2 and t.shape[1] == cols
+        return ctypes.c_void_p(t.data_ptr()), t.shape[0]
+
+    def infer(self, inputs, out=None, stream=None):
+        import torch
+        ip, n = self._dev(inputs, 11)
+        if out is None:
+            out = torch.empty((n, 4), dtype=torch.float32, device=inputs.device)
+        op, _ = self._dev(out, 4)
+        _check(load().vt_nrc_infer(self._h, ip, n, op, _stream_ptr(stream)))
+        return out
+
+    def train(self, inputs, targets, lr=1e-3, stream=None):
+        ip, n = self._dev(inputs, 11)
+        tp, m = self._dev(targets, 4)
+        assert m == n
+        _check(load().vt_nrc_train(self._h, ip, tp, n, lr, _stream_ptr(stream)))
+
+    def gradients(self, inputs, targets):
+        ip, n = self._dev(inputs, 11)
+        tp, _ = self._dev(targets, 4)
+        g = np.empty(NRC_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        _check(load().vt_nrc_gradients(self._h, ip, tp, n, _ptr(g), ctypes.byref(loss)))
+        return g, loss.value

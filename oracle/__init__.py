@@ -68,6 +68,7 @@ def load():
         "vo_trace_any_batch": (None, [P(SceneView)] + [vp] * 8 + [c.c_int64, c.c_int, c.c_int, vp, vp]),
         "vo_set_fault": (None, [c.c_int]),
         "vo_half_round": (c.c_float, [c.c_float]),
+        "vo_render_nrc_f32": (None, [vp, vp, u32, u32, u32, vp, vp, vp, c.c_int, vp]),
         "vo_bsdf_eval_f32": (None, [vp, vp, vp, vp, vp, c.c_int, vp, vp]),
         "vo_bsdf_sample_f32": (c.c_int, [vp, vp, vp, vp, c.c_int, vp, vp, vp, vp]),
         "vo_bsdf_eval_f64": (None, [vp, vp, vp, vp, vp, c.c_int, vp, vp]),
@@ -425,5 +426,23 @@ def render_restir(scene: "Scene", materials, frame: dict, sample_index0: int, sa
     out = np.zeros((h * w, 3), np.float64)
     fn = lib.vo_render_restir_f64 if precision == "f64" else lib.vo_render_restir_f32
     fn(ctypes.byref(rs), ctypes.byref(fr), sample_index0, samples, candidates, ep, threads, _p(out))
+    del keep
+    return out.reshape(h, w, 3)
+
+
+def render_nrc(scene: "Scene", materials, frame: dict, sample_index0: int, samples: int, params, period=16,
+               threads=0, light_tree=True):
+    """The library's cache-terminated estimator with fixed network parameters -> (h, w, 3)."""
+    lib = load()
+    rs, fr, keep = _render_inputs(scene, materials, frame, light_tree)
+    v = np.asarray(scene.vertices, np.float32)[:, :3]
+    lo = v.min(0).astype(np.float32)
+    ext = (v.max(0) - lo).astype(np.float32)
+    scale = np.where(ext > 0, np.float32(1) / np.where(ext > 0, ext, 1), 0).astype(np.float32)
+    p = np.ascontiguousarray(params, np.float32)
+    w, h = frame["width"], frame["height"]
+    out = np.zeros((h * w, 3), np.float64)
+    lib.vo_render_nrc_f32(ctypes.byref(rs), ctypes.byref(fr), sample_index0, samples, period, _p(p), _p(lo), _p(scale),
+                          threads, _p(out))
     del keep
     return out.reshape(h, w, 3)

@@ -564,6 +564,172 @@ void SUFFIX(vo_render)(const vo_render_scene *sc, const vo_frame *fr, uint32_t s
     }
 }
 
+/* ------------------------------------------------------------------ learned radiance cache
+ * The library's cache-terminated estimator (no counterpart in the reference): a path whose pixel and
+ * sample hash to a training path (1 in `period`) is the MIS path above; any other path ends at its
+ * first vertex reached by a non-delta scatter that is not an emitter, adding throughput x max(y, 0)
+ * at its lanes, y the network's answer for the vertex (oracle/nrc.py layout, evaluated with bfloat16
+ * weights and layer inputs as the matrix cores consume them). */
+static float bf16r(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return f;
+}
+/* params: the network already rounded to bfloat16. */
+static void nrc_forward(const float *params, const float *x, float *y) {
+    float e[272], h[64], g[64];
+    for (int i = 0; i < 11; ++i)
+        for (int b = 0; b < 12; ++b) {
+            double a = 3.14159265358979323846 * (double)(1 << b) * (double)x[i];
+            e[24 * i + 2 * b] = bf16r((float)sin(a));
+            e[24 * i + 2 * b + 1] = bf16r((float)cos(a));
+        }
+    e[264] = 1.0f;
+    for (int k = 265; k < 272; ++k) e[k] = 0.0f;
+    const float *w = params;
+    for (int o = 0; o < 64; ++o) {
+        float acc = 0.0f;
+        for (int i = 0; i < 272; ++i) acc += w[o * 272 + i] * e[i];
+        h[o] = bf16r(acc > 0.0f ? acc : 0.0f);
+    }
+    w += 64 * 272;
+    for (int l = 0; l < 3; ++l, w += 64 * 64) {
+        for (int o = 0; o < 64; ++o) {
+            float acc = 0.0f;
+            for (int i = 0; i < 64; ++i) acc += w[o * 64 + i] * h[i];
+            g[o] = bf16r(acc > 0.0f ? acc : 0.0f);
+        }
+        memcpy(h, g, sizeof h);
+    }
+    for (int o = 0; o < 4; ++o) {
+        float acc = 0.0f;
+        for (int i = 0; i < 64; ++i) acc += w[o * 64 + i] * h[i];
+        y[o] = acc;
+    }
+}
+static int nrc_training(uint32_t pix, uint32_t sample_index, uint32_t period) {
+    return vo_pcg_hash((pix * 0x9E3779B9u) ^ (sample_index * 0x85EBCA6Bu)) % period == 0u;
+}
+static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const cam_t *cam, uint32_t gx, uint32_t gy,
+                           uint32_t sample_index, const float *params, const float *box_lo, const float *box_scale) {
+    const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
+    uint32_t seed = pixel_seed(fr, gx, gy, sample_index);
+    const real u = pixel_u(seed);
+    lanes_t l = lanes_of(sc, u);
+    v3 dir = camera_dir(cam, fr, gx, gy, sample_index);
+    seed = vo_pcg_hash(seed ^ 0x27d4eb2fu);
+    v3 radiance = V(0, 0, 0);
+    surf_t s;
+    uint32_t face;
+    if (!trace(sc, cam->eye, dir, ray_t_min(cam->eye), &s, &face)) return radiance;
+    mat_t m = mat_of(sc, s.slot);
+    if (m.type == 3) return s.front ? m.emission : V(0, 0, 0);
+    if (!is_delta(&m)) radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), limit > 0, &seed);
+    s4 thr = S4(1);
+    int sdepth = 0;
+    for (int bounce = 1;; ++bounce) {
+        const int prev_delta = is_delta(&m);
+        if (!prev_delta && sdepth >= limit) break;
+        if (!prev_delta) ++sdepth;
+        v3 nd;
+        s4 bw;
+        real bpdf;
+        if (!sample_bsdf(sc, &l, &s, &m, dir, &seed, &nd, &bw, &bpdf)) break;
+        thr = smul(thr, bw);
+        if (!(isfinite(thr.v[0]) && isfinite(thr.v[1]) && isfinite(thr.v[2]) && isfinite(thr.v[3])) || smax(thr) <= 0) break;
+        v3 prev = s.p;
+        v3 o = ray_origin(s.p, s.ng, nd, s.error);
+        dir = nd;
+        surf_t r;
+        if (!trace(sc, o, nd, ray_t_min(o), &r, &face)) break;
+        mat_t rm = mat_of(sc, r.slot);
+        if (rm.type == 3) {
+            tri_t t = tri_of(sc, face);
+            real g = imported_geometry(prev, r.p, &t);
+            real light_pdf = 0;
+            if (sc->tree) {
+                uint32_t lo = 0, hi = sc->tree->count;
+                while (lo < hi) {
+                    uint32_t mid = (lo + hi) / 2;
+                    if (sc->tree->ids[mid] < face) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo < sc->tree->count && sc->tree->ids[lo] == face && g > (real)1e-12) {
+                    float fp[3] = {f_of(prev.x), f_of(prev.y), f_of(prev.z)};
+                    real area = (real)0.5 * len(cross(sub(t.b, t.a), sub(t.c, t.a)));
+                    light_pdf = area > 0 ? (real)vo_light_tree_pmf(sc->tree, fp, sc->tree->positions[lo]) / area / g : 0;
+                }
+            } else if (sc->emitter_count && sc->emitter_total > 0 && rm.weight > 0 && g > (real)1e-12) {
+                light_pdf = (rm.weight / (real)sc->emitter_total) / g;
+            }
+            real w = emission_weight(prev_delta, !prev_delta, 1, bpdf, light_pdf);
+            v3 e = r.front ? rm.emission : V(0, 0, 0);
+            radiance = add(radiance, spectrum_rgb(sc, &l, sscale(smul(thr, emission_spectrum(sc, &l, e)), w)));
+            break;
+        }
+        if (!prev_delta) {
+            float x[11], y[4];
+            x[0] = fminf(fmaxf(((float)r.p.x - box_lo[0]) * box_scale[0], 0.0f), 1.0f);
+            x[1] = fminf(fmaxf(((float)r.p.y - box_lo[1]) * box_scale[1], 0.0f), 1.0f);
+            x[2] = fminf(fmaxf(((float)r.p.z - box_lo[2]) * box_scale[2], 0.0f), 1.0f);
+            x[3] = (1.0f - (float)dir.x) * 0.5f;
+            x[4] = (1.0f - (float)dir.y) * 0.5f;
+            x[5] = (1.0f - (float)dir.z) * 0.5f;
+            x[6] = ((float)r.n.x + 1.0f) * 0.5f;
+            x[7] = ((float)r.n.y + 1.0f) * 0.5f;
+            x[8] = ((float)r.n.z + 1.0f) * 0.5f;
+            x[9] = rm.type == 0u ? 1.0f : fminf(fmaxf((float)rm.roughness, 0.0f), 1.0f);
+            x[10] = (float)u;
+            nrc_forward(params, x, y);
+            s4 v;
+            for (int k = 0; k < 4; ++k) v.v[k] = thr.v[k] * (real)(y[k] > 0.0f ? y[k] : 0.0f);
+            radiance = add(radiance, spectrum_rgb(sc, &l, v));
+            break;
+        }
+        if (!is_delta(&rm))
+            radiance = add(radiance, nee(sc, &l, &r, &rm, neg(dir), thr, sdepth < limit, &seed));
+        s = r;
+        m = rm;
+        if (bounce > 3) {
+            real survival = RFMIN(RFMAX(smax(thr), (real)0.05), is_delta(&m) ? (real)0.99 : (real)0.95);
+            if ((real)vo_rand_f(&seed) >= survival) break;
+            thr = sscale(thr, (real)1 / survival);
+        }
+    }
+    if (!(isfinite(radiance.x) && isfinite(radiance.y) && isfinite(radiance.z))) radiance = V(0, 0, 0);
+    return radiance;
+}
+
+void SUFFIX(vo_render_nrc)(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
+                           uint32_t period, const float *params, const float *box_lo, const float *box_scale, int threads,
+                           double *out_rgb) {
+    const cam_t cam = camera_of(fr, fr->eye);
+    (void)vo_energy_table(0);
+    const int64_t npix = (int64_t)fr->width * fr->height;
+    float *wb = malloc(30720 * sizeof(float));
+    if (!wb) return;
+    for (int i = 0; i < 30720; ++i) wb[i] = bf16r(params[i]);
+    params = wb;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads > 0 ? threads : omp_get_max_threads())
+    for (int64_t k = 0; k < npix; ++k) {
+        uint32_t gx = (uint32_t)(k % fr->width), gy = (uint32_t)(k / fr->width);
+        double acc[3] = {0, 0, 0};
+        for (uint32_t smp = 0; smp < samples; ++smp) {
+            const uint32_t si = sample_index0 + smp;
+            v3 c = nrc_training((uint32_t)k, si, period) ? sample_pixel(sc, fr, &cam, gx, gy, si)
+                                                          : sample_pixel_nrc(sc, fr, &cam, gx, gy, si, params, box_lo, box_scale);
+            acc[0] += (double)c.x;
+            acc[1] += (double)c.y;
+            acc[2] += (double)c.z;
+        }
+        for (int q = 0; q < 3; ++q) out_rgb[3 * k + q] = acc[q] / (double)samples;
+    }
+    free(wb);
+}
+
 /* ------------------------------------------------------------------ ReSTIR DI (samplingMode 0)
  * Pass 1 (restir_temporal_kernel, main.swift:4842-4940): G-buffer, RIS over `candidates` light
  * samples (restir_di_initial :4637-4668), temporal merge through backprojection

@@ -196,6 +196,16 @@ void vo_bsdf_eval_f64(const float *params, const float *n, const float *wo, cons
 int vo_bsdf_sample_f64(const float *params, const float *n, const float *wo, const float *rho, int inside,
                        const float *u3, float *dir, float *weight, float *pdf);
 
+/* The library's cache-terminated estimator with a fixed network (integrator.c): mean radiance of every
+ * pixel over `samples` samples, MIS mode.  params: 30720 floats (oracle/nrc.py layout); the cache's
+ * position frame is (p - box_lo) * box_scale. */
+void vo_render_nrc_f32(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
+                       uint32_t period, const float *params, const float *box_lo, const float *box_scale, int threads,
+                       double *out_rgb);
+void vo_render_nrc_f64(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
+                       uint32_t period, const float *params, const float *box_lo, const float *box_scale, int threads,
+                       double *out_rgb);
+
 /* For tests that plant a mistake in the oracle (never the product): 0 = faithful,
  * 1 = drop the FMA tie-break of watertight_edge, 2 = swap b1/b2, 3 = ReSTIR spatial reuse leaves
  * the neighbours' confidence out of M, 4 = ReSTIR RIS counts only candidates with pdf > 0 in M,

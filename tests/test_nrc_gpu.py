@@ -1,0 +1,108 @@
+"""GPU tests of the radiance cache's kernels (metal-vibe-tracer_amd/csrc/nrc.hip) against the FP64
+oracle (oracle/nrc.py): MFMA forward outputs, the fused backward's gradients (the gradient check),
+one Adam step, ragged batch sizes, and that online training lowers the loss."""
+import numpy as np
+import pytest
+
+from oracle import nrc as onrc
+import vt
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(seed=0, scale=0.15):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-scale, scale, onrc.N_PARAMS)
+    Ws = onrc.split(p)
+    Ws[0][:, 265:] = 0
+    Ws[4][4:] = 0
+    return np.concatenate([w.reshape(-1) for w in Ws]).astype(np.float32)
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+@pytest.mark.parametrize("n", [1, 17, 1000, 4099])
+def test_forward_matches_oracle(gpu_ctx, n):
+    cache = vt.NRC(gpu_ctx, seed=3)
+    p = _params(1)
+    cache.params = p
+    x = np.random.default_rng(n).random((n, 11)).astype(np.float32)
+    y = cache.infer(_dev(x)).cpu().numpy().astype(np.float64)
+    ref, _ = onrc.forward(p.astype(np.float64), x, bf16=True)
+    exact, _ = onrc.forward(p.astype(np.float64), x)
+    scale = np.abs(exact).max() + 1e-6
+    err_bf16 = np.abs(y - ref).max() / scale
+    err_exact = np.abs(y - exact).max() / scale
+    print(f"n {n}: max |gpu - oracle(bf16)| / scale {err_bf16:.2e}, vs fp64 {err_exact:.2e}", flush=True)
+    assert err_bf16 < 2e-2 and err_exact < 5e-2
+    cache.close()
+
+
+def test_gradient_check(gpu_ctx):
+    cache = vt.NRC(gpu_ctx, seed=3)
+    p = _params(2)
+    cache.params = p
+    rng = np.random.default_rng(5)
+    n = 777
+    x = rng.random((n, 11)).astype(np.float32)
+    t = rng.random((n, 4)).astype(np.float32)
+    g, loss = cache.gradients(_dev(x), _dev(t))
+    ref = onrc.gradients(p.astype(np.float64), x, t, bf16=True)
+    exact = onrc.gradients(p.astype(np.float64), x, t)
+    y, _ = onrc.forward(p.astype(np.float64), x)
+    rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+    rel_exact = np.linalg.norm(g - exact) / np.linalg.norm(exact)
+    per_layer = []
+    offsets = np.cumsum([0] + onrc.SIZES)
+    for k in range(5):
+        a, b = offsets[k], offsets[k + 1]
+        per_layer.append(np.linalg.norm(g[a:b] - exact[a:b]) / np.linalg.norm(exact[a:b]))
+    print(f"gradient rel error vs bf16 oracle {rel:.2e}, vs fp64 {rel_exact:.2e}, per layer "
+          + " ".join(f"{e:.2e}" for e in per_layer), flush=True)
+    assert rel < 3e-2 and rel_exact < 6e-2 and max(per_layer) < 0.1
+    assert abs(loss / onrc.loss(y, t) - 1) < 2e-2
+    # Padded inputs and unused outputs receive nothing.
+    Gs = onrc.split(g)
+    assert np.all(Gs[0][:, 265:] == 0) and np.all(Gs[4][4:] == 0)
+    cache.close()
+
+
+def test_adam_step_matches_oracle(gpu_ctx):
+    cache = vt.NRC(gpu_ctx, seed=3)
+    p = _params(4)
+    cache.params = p
+    rng = np.random.default_rng(6)
+    x = rng.random((256, 11)).astype(np.float32)
+    t = rng.random((256, 4)).astype(np.float32)
+    g, _ = cache.gradients(_dev(x), _dev(t))
+    cache.train(_dev(x), _dev(t), lr=1e-3)
+    q = cache.params
+    want, _, _ = onrc.adam(p.astype(np.float64), 0.0, 0.0, g.astype(np.float64), 1)
+    assert np.abs(q - want).max() < 2e-6
+    cache.close()
+
+
+def test_online_training_lowers_the_loss(gpu_ctx):
+    import torch
+    cache = vt.NRC(gpu_ctx, seed=9)
+    rng = np.random.default_rng(7)
+
+    def target(x):
+        # A smooth radiance-like field of position and direction.
+        r = 0.5 + 0.4 * np.sin(3 * x[:, 0] + 2 * x[:, 1]) * np.cos(2 * x[:, 3])
+        return np.stack([r, 0.8 * r, 0.6 * r + 0.1 * x[:, 2], 0.4 * r], 1).astype(np.float32)
+
+    xv = rng.random((4096, 11)).astype(np.float32)
+    tv = target(xv)
+    _, before = cache.gradients(_dev(xv), _dev(tv))
+    for step in range(300):
+        xb = rng.random((8192, 11)).astype(np.float32)
+        cache.train(_dev(xb), _dev(target(xb)), lr=1e-3)
+    torch.cuda.synchronize()
+    _, after = cache.gradients(_dev(xv), _dev(tv))
+    print(f"relative L2 loss before {before:.4f} after 300 steps {after:.4f}", flush=True)
+    assert after < 0.1 * before
+    cache.close()
