@@ -120,6 +120,84 @@ def bench_integrator(args, ctx):
     return out
 
 
+def bench_learned_cache(args, ctx):
+    """Config 5: the emitter terrain with the OpenPBR glossy / metal / transmissive mix (~10M
+    triangles, 4096 emitters in the light tree), 1920 x 1080, depth 8, MIS, 1 sample per frame.
+
+    Plain: `--cache-frames` frames of the plain tracer, each timed (synchronised), accumulated.
+    Learned cache, equal time: the cache is first trained online over `--cache-warmup` frames (the
+    state of an interactive session that has been running; their time is reported, not charged),
+    then frames are accumulated from a reset until the plain run's time is spent.  Cold start: a
+    fresh cache whose every frame, training included, is accumulated within the same time.  Error:
+    relMSE on the 64 x 36 stratified pixel subset against the committed FP64 oracle image at 65536
+    spp (tests/golden/terrain_openpbr_ref.npz, made by tests/golden/make_terrain_reference.py)."""
+    import numpy as np
+    import torch
+
+    import vt
+    from vt import workloads
+    path = os.path.join(ROOT, "tests", "golden", "terrain_openpbr_ref.npz")
+    with np.load(path) as z:
+        pix, ref, depth = z["pixels"], z["radiance"], int(z["depth"])
+        w, h = int(z["width"]), int(z["height"])
+    mesh, mats = workloads.emitter_terrain("openpbr")
+    scene = ctx.set_mesh(mesh)
+    scene.set_materials(workloads.as_materials(mats))
+    view = (workloads.TERRAIN_EYE, workloads.TERRAIN_TARGET, workloads.TERRAIN_FOV)
+    lr = args.cache_lr
+
+    def frame(r, k, cache=0):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r.render(scene, w, h, *view, max_depth=depth, samples=1, sample_index=k, cache=cache, learning_rate=lr)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    def error(r):
+        return relmse(r.readback()[..., :3].astype(np.float64).reshape(-1, 3)[pix], ref)
+
+    r = vt.Renderer(ctx)
+    for k in range(3):
+        frame(r, 1000 + k)
+    r.reset()
+    t_plain = sum(frame(r, 1 + k) for k in range(args.cache_frames))
+    e_plain = error(r)
+    r.close()
+
+    def learned(warm):
+        cache = vt.NRC(ctx, seed=1)
+        r = vt.Renderer(ctx)
+        r.set_cache(cache)
+        t_warm = sum(frame(r, 5000 + k, cache=1) for k in range(warm))
+        if warm:
+            r.reset()
+        spent, n = 0.0, 0
+        while spent < t_plain:
+            spent += frame(r, 1 + n, cache=1)
+            n += 1
+        e = error(r)
+        r.close()
+        cache.close()
+        return e, n, spent, t_warm
+
+    e_nrc, n_nrc, t_nrc, t_warm = learned(args.cache_warmup)
+    e_cold, n_cold, t_cold, _ = learned(0)
+    scene.close()
+    log(f"learned cache: plain {args.cache_frames} spp relMSE {e_plain:.4e}; cache {n_nrc} spp relMSE {e_nrc:.4e} "
+        f"(cold start {n_cold} spp, {e_cold:.4e})")
+    return {"workload": f"emitter-terrain-openpbr-{len(mesh.faces)}tri-{w}x{h}-depth{depth}",
+            "reference": {"kind": "FP64 oracle", "spp": int(np.load(path)["spp"]), "pixels": int(len(pix))},
+            "learning_rate": lr, "train_period": 16,
+            "plain": {"spp": args.cache_frames, "seconds": round(t_plain, 4), "relmse": e_plain,
+                      "ms_per_frame": round(t_plain / args.cache_frames * 1e3, 3)},
+            "learned": {"spp": n_nrc, "seconds": round(t_nrc, 4), "relmse": e_nrc,
+                        "ms_per_frame": round(t_nrc / n_nrc * 1e3, 3), "warmup_frames": args.cache_warmup,
+                        "warmup_seconds": round(t_warm, 3)},
+            "cold_start": {"spp": n_cold, "seconds": round(t_cold, 4), "relmse": e_cold},
+            "relmse_ratio_plain_over_learned": e_plain / e_nrc,
+            "relmse_ratio_cold_start": e_plain / e_cold}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +208,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-integrator", action="store_true", help="skip the path-tracer (Cornell) measurement")
     ap.add_argument("--reference-spp", type=int, default=65536, help="FP64 oracle samples per pixel for relMSE")
+    ap.add_argument("--no-cache", action="store_true", help="skip the learned-cache equal-time measurement")
+    ap.add_argument("--cache-frames", type=int, default=16, help="plain-tracer frames (1 spp each) setting the time budget")
+    ap.add_argument("--cache-warmup", type=int, default=64, help="online training frames before the equal-time run")
+    ap.add_argument("--cache-lr", type=float, default=1e-3, help="Adam learning rate of the online training")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device 3 times with a plain copy kernel (the "
                          "reference launch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -225,6 +307,7 @@ def main():
                          f"binned-SAH BVH (leaf 2) and traversal restated in C, median of 3"}
 
     integrator = None if args.no_integrator else bench_integrator(args, ctx)
+    learned_cache = None if args.no_cache else bench_learned_cache(args, ctx)
 
     line = {
         "metric": METRIC,
@@ -250,6 +333,7 @@ def main():
                      "work": {"node_fetches": np_ + nb_, "triangle_tests": tp_ + tb_, "bytes_per_step": bytes_p + bytes_b}},
         "cpu_baseline": cpu,
         "integrator": integrator,
+        "learned_cache": learned_cache,
     }
     print(json.dumps(line), flush=True)
     if args.copy_calibration:

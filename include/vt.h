@@ -178,6 +178,13 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
 /* Attaches a learned radiance cache (owned by the caller, which keeps it alive while attached); NULL
  * detaches.  No counterpart in the reference. */
 vt_status vt_renderer_set_cache(vt_renderer *r, vt_nrc *cache);
+/* The learned cache's training records of the renderer's last wavefront (radiance_cache 1 or 2):
+ * *count = records made; up to max_records of them are copied, inputs as [n][11] (the cache's input
+ * layout) and targets as [n][4] (spectral radiance per unit throughput at the 4 wavelength lanes);
+ * either pointer may be NULL.  Synchronises.  No counterpart in the reference (inspection of the
+ * online training data). */
+vt_status vt_renderer_cache_records(vt_renderer *r, float *inputs, float *targets, uint32_t max_records,
+                                    uint32_t *count);
 /* Replaces resetAccumulation() (main.swift:10761); also drops the ReSTIR reservoir history
  * (reservoirHistory = 0). */
 vt_status vt_reset_accumulation(vt_renderer *r);

@@ -16,10 +16,11 @@ using vt::fail;
 
 struct vt_nrc {
     vt_context *ctx = nullptr;
-    DeviceBuffer params, grads, m, v, wbf16, loss;
+    DeviceBuffer params, grads, m, v, ema, wbf16, wema, loss;
     uint32_t step = 0;
     vt::NrcDevice view() const {
-        return vt::NrcDevice{params.as<float>(), grads.as<float>(), m.as<float>(), v.as<float>(), wbf16.as<uint16_t>()};
+        return vt::NrcDevice{params.as<float>(), grads.as<float>(), m.as<float>(), v.as<float>(), ema.as<float>(),
+                             wbf16.as<uint16_t>(), wema.as<uint16_t>()};
     }
 };
 
@@ -57,6 +58,9 @@ std::vector<float> initial_params(uint32_t seed) {
     fill(vt::kNrcHidden, vt::kNrcFeatures, 265, vt::kNrcHidden, 265);
     for (int l = 0; l < 3; ++l) fill(vt::kNrcHidden, vt::kNrcHidden, vt::kNrcHidden, vt::kNrcHidden, vt::kNrcHidden);
     fill(vt::kNrcOutPad, vt::kNrcHidden, vt::kNrcHidden, vt::kNrcOut, vt::kNrcHidden);
+    // The output layer starts at 1/20 of that scale, so the untrained cache answers near zero instead
+    // of at radiances far above a typical scene's, which it would first have to unlearn.
+    for (size_t i = vt::kNrcParams - vt::kNrcW5; i < vt::kNrcParams; ++i) p[i] *= 0.05f;
     return p;
 }
 
@@ -85,7 +89,9 @@ vt_status vt_nrc_create(vt_context *ctx, uint32_t seed, vt_nrc **out) {
         n->grads.alloc(bytes);
         n->m.alloc(bytes);
         n->v.alloc(bytes);
+        n->ema.alloc((size_t)vt::kNrcParams * 4);
         n->wbf16.alloc((size_t)vt::kNrcParams * 2);
+        n->wema.alloc((size_t)vt::kNrcParams * 2);
         n->loss.alloc(4);
         std::vector<float> p = initial_params(seed);
         upload_params(n.get(), p.data());

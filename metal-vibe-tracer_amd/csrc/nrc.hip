@@ -48,16 +48,33 @@ __device__ __forceinline__ void load_weights(uint16_t *w, const uint16_t *src) {
     for (int i = threadIdx.x; i < kNrcParams / 8; i += blockDim.x) d[i] = s[i];
 }
 
+// sin and cos of pi 2^b x and of pi 2^(b+1) x, packed as four bf16 (feature order sin, cos, sin, cos).
+// 2^(b-1) x is exact in fp32 and so is its fractional part, so the hardware sine (v_sin_f32 /
+// v_cos_f32, whose argument is in turns) works on the exactly reduced angle.
+__device__ __forceinline__ s4v band_pair(float x, int b) {
+    const float t1 = __builtin_amdgcn_fractf(ldexpf(x, b - 1)), t2 = __builtin_amdgcn_fractf(ldexpf(x, b));
+    s4v r;
+    r.x = (short)to_bf16(__builtin_amdgcn_sinf(t1));
+    r.y = (short)to_bf16(__builtin_amdgcn_cosf(t1));
+    r.z = (short)to_bf16(__builtin_amdgcn_sinf(t2));
+    r.w = (short)to_bf16(__builtin_amdgcn_cosf(t2));
+    return r;
+}
+// Input k of query q: inputs are [q][11] (stride 0) or [11][stride].
+__device__ __forceinline__ float query_input(const float *inputs, uint32_t stride, uint32_t q, int k) {
+    return stride ? inputs[(size_t)k * stride + q] : inputs[(size_t)q * kNrcInputs + k];
+}
+
 // Encodes 16 queries (rows of x, stride 272) from inputs[q0..q0 + 15] (rows >= n are zeros).
-__device__ __forceinline__ void encode_tile(uint16_t *x, const float *inputs, uint32_t q0, uint32_t n, int lane) {
-    for (int idx = lane; idx < 16 * kNrcInputs * kNrcBands; idx += 64) {
-        const int r = idx / (kNrcInputs * kNrcBands), rem = idx - r * (kNrcInputs * kNrcBands);
-        const int i = rem / kNrcBands, b = rem - i * kNrcBands;
+__device__ __forceinline__ void encode_tile(uint16_t *x, const float *inputs, uint32_t stride, uint32_t q0, uint32_t n,
+                                            int lane) {
+    for (int idx = lane; idx < 16 * kNrcInputs * (kNrcBands / 2); idx += 64) {
+        const int r = idx / (kNrcInputs * kNrcBands / 2), rem = idx - r * (kNrcInputs * kNrcBands / 2);
+        const int i = rem / (kNrcBands / 2), b = 2 * (rem - i * (kNrcBands / 2));
         const uint32_t q = q0 + (uint32_t)r;
-        float s = 0.0f, c = 0.0f;
-        if (q < n) sincospif(inputs[(size_t)q * kNrcInputs + i] * (float)(1 << b), &s, &c);
-        x[r * kNrcFeatures + 24 * i + 2 * b] = to_bf16(s);
-        x[r * kNrcFeatures + 24 * i + 2 * b + 1] = to_bf16(c);
+        s4v v = {0, 0, 0, 0};
+        if (q < n) v = band_pair(query_input(inputs, stride, q, i), b);
+        *reinterpret_cast<s4v *>(x + r * kNrcFeatures + 24 * i + 2 * b) = v;
     }
     for (int idx = lane; idx < 16 * 8; idx += 64) {
         const int r = idx >> 3, j = idx & 7;
@@ -97,42 +114,120 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // ---------------------------------------------------------------- inference
-constexpr int kInferLds = kNrcParams * 2 + 4 * (16 * kNrcFeatures + 2 * 16 * kNrcHidden) * 2;
+// The weights are the A operand and the activations the B operand (D^T = W . X^T), so a layer's
+// output tile -- lane l holds neurons 16 m + 4 (l / 16) + j of query l % 16 -- is exactly the B
+// operand of k-step m of the next layer: activations stay in registers from the encoding to the
+// output.  A wave handles 64 queries (4 tiles of 16) per pass, so every weight fragment read from
+// LDS feeds 4 products.  LDS rows are padded (276 / 68 bf16) so the 16 rows a quarter-wave reads
+// fall in distinct banks.
+constexpr int kW1Pitch = 276, kWhPitch = 68;
+constexpr int kLdsW1 = 0, kLdsW2 = kLdsW1 + kNrcHidden * kW1Pitch, kLdsW3 = kLdsW2 + kNrcHidden * kWhPitch,
+              kLdsW4 = kLdsW3 + kNrcHidden * kWhPitch, kLdsW5 = kLdsW4 + kNrcHidden * kWhPitch;
+constexpr int kInferLds = (kLdsW5 + kNrcOutPad * kWhPitch) * 2;  // 63,616 bytes
 
-__global__ __launch_bounds__(256) void nrc_infer_kernel(const uint16_t *wsrc, const float *inputs, uint32_t n, float *out,
-                                                        const uint32_t *count) {
+__device__ __forceinline__ void load_weights_padded(uint16_t *w, const uint16_t *src) {
+    for (int c = threadIdx.x; c < kNrcParams / 4; c += blockDim.x) {
+        const int e = 4 * c;
+        int dst;
+        if (e < kNrcW1) {
+            const int row = e / kNrcFeatures;
+            dst = kLdsW1 + row * kW1Pitch + (e - row * kNrcFeatures);
+        } else {
+            const int f = e - kNrcW1, row = f / kNrcHidden;  // W2..W4 then W5: rows of 64 back to back
+            dst = kLdsW2 + row * kWhPitch + (f - row * kNrcHidden);
+        }
+        *reinterpret_cast<uint2 *>(w + dst) = *reinterpret_cast<const uint2 *>(src + e);
+    }
+}
+
+// relu and bf16 of a layer's output tiles: the next layer's B operands, h[k-step][query tile].
+__device__ __forceinline__ void relu_pack(const f4v (&acc)[4][4], s4v (&h)[4][4]) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            s4v v;
+            v.x = (short)to_bf16(fmaxf(acc[m][t][0], 0.0f));
+            v.y = (short)to_bf16(fmaxf(acc[m][t][1], 0.0f));
+            v.z = (short)to_bf16(fmaxf(acc[m][t][2], 0.0f));
+            v.w = (short)to_bf16(fmaxf(acc[m][t][3], 0.0f));
+            h[m][t] = v;
+        }
+}
+
+// One hidden layer: h (64 x 64 queries, B layout) -> relu(W h) in place.
+__device__ __forceinline__ void hidden_layer(const uint16_t *w, s4v (&h)[4][4], int r, int g) {
+    f4v acc[4][4] = {};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+        s4v a[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) a[m] = load4(w + (16 * m + r) * kWhPitch + 16 * ks + 4 * g);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) acc[m][t] = mfma(a[m], h[ks][t], acc[m][t]);
+    }
+    relu_pack(acc, h);
+}
+
+__global__ __launch_bounds__(256) void nrc_infer_kernel(const uint16_t *wsrc, const float *inputs, uint32_t stride,
+                                                        uint32_t n, float *out, const uint32_t *count) {
     extern __shared__ __align__(16) uint16_t lds[];
     if (count) n = min(n, *count);
     if (n == 0) return;
-    uint16_t *w = lds;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint16_t *x = lds + kNrcParams + wave * (16 * kNrcFeatures + 2 * 16 * kNrcHidden);
-    uint16_t *h0 = x + 16 * kNrcFeatures, *h1 = h0 + 16 * kNrcHidden;
-    load_weights(w, wsrc);
+    load_weights_padded(lds, wsrc);
     __syncthreads();
-    const uint32_t tiles = (n + 15) / 16;
-    for (uint32_t t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
-        const uint32_t q0 = t * 16;
-        encode_tile(x, inputs, q0, n, lane);
-        wave_sync();
-        dense_relu<kNrcFeatures>(x, w, h0, lane);
-        wave_sync();
-        dense_relu<kNrcHidden>(h0, w + kOffW2, h1, lane);
-        wave_sync();
-        dense_relu<kNrcHidden>(h1, w + kOffW3, h0, lane);
-        wave_sync();
-        dense_relu<kNrcHidden>(h0, w + kOffW4, h1, lane);
-        wave_sync();
-        const f4v y = dense_out(h1, w + kOffW5, lane);
-        const int c = lane & 15;
-        if (c < kNrcOut) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const uint32_t passes = (n + 63) / 64;
+    for (uint32_t p = blockIdx.x * 4 + wave; p < passes; p += gridDim.x * 4) {
+        const uint32_t base = p * 64;
+        bool live[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t q = q0 + (uint32_t)((lane >> 4) * 4 + j);
-                if (q < n) out[(size_t)q * kNrcOut + c] = y[j];
+        for (int t = 0; t < 4; ++t) live[t] = base + 16 * t + r < n;
+        // Layer 1 over the 17 k-steps of the encoding: lane l's features f = 16 ks + 4 g .. + 3 are
+        // bands b, b + 1 of input i = f / 24 (f < 264), or the constant 1 and zero padding.
+        f4v acc[4][4] = {};
+#pragma unroll 1
+        for (int ks = 0; ks < kNrcFeatures / 16; ++ks) {
+            s4v a[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) a[m] = load4(lds + kLdsW1 + (16 * m + r) * kW1Pitch + 16 * ks + 4 * g);
+            const int f = 16 * ks + 4 * g;
+            constexpr int kLast = kNrcInputs * 2 * kNrcBands;  // 264
+            const int i = f < kLast ? f / 24 : 0, b = (f - 24 * i) / 2;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                s4v bv = {0, 0, 0, 0};
+                if (f < kLast) {
+                    const float xv = live[t] ? query_input(inputs, stride, base + 16 * t + r, i) : 0.0f;
+                    bv = band_pair(xv, b);
+                } else if (f == kLast) {
+                    bv.x = (short)to_bf16(1.0f);
+                }
+#pragma unroll
+                for (int m = 0; m < 4; ++m) acc[m][t] = mfma(a[m], bv, acc[m][t]);
             }
         }
-        wave_sync();
+        s4v h[4][4];
+        relu_pack(acc, h);
+        hidden_layer(lds + kLdsW2, h, r, g);
+        hidden_layer(lds + kLdsW3, h, r, g);
+        hidden_layer(lds + kLdsW4, h, r, g);
+        f4v y[4] = {};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const s4v a = load4(lds + kLdsW5 + r * kWhPitch + 16 * ks + 4 * g);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) y[t] = mfma(a, h[ks][t], y[t]);
+        }
+        // Lanes 0..15 hold outputs 0..3 of query 16 t + l.
+        if (g == 0) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (live[t]) *reinterpret_cast<float4 *>(out + 4 * (size_t)(base + 16 * t + r)) = make_float4(y[t][0], y[t][1], y[t][2], y[t][3]);
+        }
     }
 }
 
@@ -174,16 +269,24 @@ __device__ __forceinline__ void back_dense(const uint16_t *da_in, const uint16_t
         }
 }
 
-__global__ __launch_bounds__(256) void nrc_backward_kernel(const uint16_t *wsrc, const float *inputs, const float *targets,
-                                                           uint32_t n, float norm, float *grads, float *loss_sum,
-                                                           const uint32_t *count, uint32_t offset) {
+// The samples [offset, offset + n) of a slice (NrcSlice), n capped by the launch's n.
+__device__ __forceinline__ uint32_t slice_bounds(const NrcSlice &sl, uint32_t n, uint32_t &offset) {
+    offset = 0;
+    if (!sl.count) return n;
+    const uint32_t c = min(*sl.count, sl.limit);
+    const uint32_t per = min((c + sl.parts - 1) / sl.parts, n);
+    offset = sl.part * per;
+    return offset < c ? min(per, c - offset) : 0u;
+}
+
+__global__ __launch_bounds__(256) void nrc_backward_kernel(const uint16_t *wsrc, const float *inputs, uint32_t stride,
+                                                           const float *targets, uint32_t n, float norm, float *grads,
+                                                           float *loss_sum, NrcSlice slice) {
     extern __shared__ __align__(16) uint16_t lds[];
-    if (count) {
-        const uint32_t c = *count;
-        n = c > offset ? min(n, c - offset) : 0u;
-    }
+    uint32_t offset;
+    n = slice_bounds(slice, n, offset);
     if (n == 0) return;
-    inputs += (size_t)offset * kNrcInputs;
+    inputs += stride ? (size_t)offset : (size_t)offset * kNrcInputs;
     targets += (size_t)offset * kNrcOut;
     if (!(norm > 0.0f)) norm = (float)n * kNrcOut;
     uint16_t *w = lds;
@@ -199,7 +302,7 @@ __global__ __launch_bounds__(256) void nrc_backward_kernel(const uint16_t *wsrc,
         const uint32_t q0 = g * 64 + wave * 16;
         const int rows = wave * 16;
         // Forward for this wave's 16 samples, keeping every layer's input.
-        encode_tile(a0 + rows * kNrcFeatures, inputs, q0, n, lane);
+        encode_tile(a0 + rows * kNrcFeatures, inputs, stride, q0, n, lane);
         wave_sync();
         dense_relu<kNrcFeatures>(a0 + rows * kNrcFeatures, w, a1 + rows * kNrcHidden, lane);
         wave_sync();
@@ -256,10 +359,12 @@ __global__ __launch_bounds__(256) void nrc_backward_kernel(const uint16_t *wsrc,
 }
 
 __global__ __launch_bounds__(256) void nrc_adam_kernel(float *params, float *grads, float *m, float *v, uint16_t *wbf16,
-                                                       float lr, float c1, float c2, const uint32_t *count, uint32_t offset) {
+                                                       float *ema, uint16_t *wema,
+                                                       float lr, float c1, float c2, uint32_t n, NrcSlice slice) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= kNrcParams) return;
-    if (count && *count <= offset) {
+    uint32_t offset;
+    if (slice.count && slice_bounds(slice, n, offset) == 0u) {
         grads[i] = 0.0f;
         return;
     }
@@ -271,12 +376,19 @@ __global__ __launch_bounds__(256) void nrc_adam_kernel(float *params, float *gra
     const float p = params[i] - lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
     params[i] = p;
     wbf16[i] = to_bf16(p);
+    const float e = kNrcEmaDecay * ema[i] + (1.0f - kNrcEmaDecay) * p;
+    ema[i] = e;
+    wema[i] = to_bf16(e);
     grads[i] = 0.0f;
 }
 
-__global__ __launch_bounds__(256) void nrc_sync_kernel(const float *params, uint16_t *wbf16) {
+__global__ __launch_bounds__(256) void nrc_sync_kernel(const float *params, uint16_t *wbf16, float *ema, uint16_t *wema) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < kNrcParams) wbf16[i] = to_bf16(params[i]);
+    if (i >= kNrcParams) return;
+    const float p = params[i];
+    wbf16[i] = to_bf16(p);
+    ema[i] = p;
+    wema[i] = to_bf16(p);
 }
 
 __global__ __launch_bounds__(256) void nrc_zero_kernel(float *p, uint32_t n) {
@@ -284,17 +396,18 @@ __global__ __launch_bounds__(256) void nrc_zero_kernel(float *p, uint32_t n) {
     if (i < n) p[i] = 0.0f;
 }
 
-int grid_for(uint32_t work, int per_block) {
+// Blocks for `work` items at `per_block` a block, at most `per_cu` blocks per compute unit.
+int grid_for(uint32_t work, int per_block, int per_cu = 1) {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const uint32_t blocks = (work + per_block - 1) / per_block;
-    return (int)(blocks < (uint32_t)cus ? (blocks > 0 ? blocks : 1) : (uint32_t)cus);
+    const uint32_t blocks = (work + per_block - 1) / per_block, cap = (uint32_t)(cus * per_cu);
+    return (int)(blocks < cap ? (blocks > 0 ? blocks : 1) : cap);
 }
 
 }  // namespace
 
 hipError_t launch_nrc_infer(const NrcDevice &d, const float *inputs, uint32_t n, float *out, hipStream_t s,
-                            const uint32_t *count) {
+                            const uint32_t *count, uint32_t stride) {
     if (n == 0) return hipSuccess;
     static bool configured = false;
     if (!configured) {
@@ -302,12 +415,13 @@ hipError_t launch_nrc_infer(const NrcDevice &d, const float *inputs, uint32_t n,
         if (e != hipSuccess) return e;
         configured = true;
     }
-    hipLaunchKernelGGL(nrc_infer_kernel, dim3(grid_for(n, 64)), dim3(256), kInferLds, s, d.wbf16, inputs, n, out, count);
+    hipLaunchKernelGGL(nrc_infer_kernel, dim3(grid_for((n + 3) / 4, 64, 2)), dim3(256), kInferLds, s, d.wema, inputs, stride, n,
+                       out, count);
     return hipGetLastError();
 }
 
 hipError_t launch_nrc_backward(const NrcDevice &d, const float *inputs, const float *targets, uint32_t n, float norm,
-                               float *loss_sum, hipStream_t s, const uint32_t *count, uint32_t offset) {
+                               float *loss_sum, hipStream_t s, const NrcSlice &slice, uint32_t stride) {
     if (n == 0) return hipSuccess;
     static bool configured = false;
     if (!configured) {
@@ -315,20 +429,20 @@ hipError_t launch_nrc_backward(const NrcDevice &d, const float *inputs, const fl
         if (e != hipSuccess) return e;
         configured = true;
     }
-    hipLaunchKernelGGL(nrc_backward_kernel, dim3(grid_for(n, 64)), dim3(256), kTrainLds, s, d.wbf16, inputs, targets, n, norm,
-                       d.grads, loss_sum, count, offset);
+    hipLaunchKernelGGL(nrc_backward_kernel, dim3(grid_for(n, 64)), dim3(256), kTrainLds, s, d.wbf16, inputs, stride, targets, n, norm,
+                       d.grads, loss_sum, slice);
     return hipGetLastError();
 }
 
-hipError_t launch_nrc_adam(const NrcDevice &d, uint32_t step, float lr, hipStream_t s, const uint32_t *count, uint32_t offset) {
+hipError_t launch_nrc_adam(const NrcDevice &d, uint32_t step, float lr, hipStream_t s, uint32_t n, const NrcSlice &slice) {
     const float c1 = 1.0f - powf(0.9f, (float)step), c2 = 1.0f - powf(0.99f, (float)step);
-    hipLaunchKernelGGL(nrc_adam_kernel, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d.params, d.grads, d.m, d.v, d.wbf16,
-                       lr, c1, c2, count, offset);
+    hipLaunchKernelGGL(nrc_adam_kernel, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d.params, d.grads, d.m, d.v, d.wbf16, d.ema, d.wema,
+                       lr, c1, c2, n, slice);
     return hipGetLastError();
 }
 
 hipError_t launch_nrc_sync(const NrcDevice &d, hipStream_t s) {
-    hipLaunchKernelGGL(nrc_sync_kernel, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d.params, d.wbf16);
+    hipLaunchKernelGGL(nrc_sync_kernel, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d.params, d.wbf16, d.ema, d.wema);
     return hipGetLastError();
 }
 

@@ -627,6 +627,8 @@ constexpr uint32_t kPrevDelta = 1u << 16, kPrevNee = 1u << 17, kPrevMis = 1u << 
 // BSDF-only paths (mode 3) past the scattering budget: one last continuation that may only reach
 // emission (main.swift:7716-7722, 7785).
 constexpr uint32_t kEmissionOnly = 1u << 19;
+// Training paths of the radiance cache: the path has recorded its cache vertex.
+constexpr uint32_t kRecorded = 1u << 20;
 
 // ---------------------------------------------------------------- learned radiance cache
 // Training paths: 1 in f.train_period, by a hash of pixel and sample index.
@@ -854,9 +856,10 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
             }
         }
         if (alive && (state & kEmissionOnly) != 0u) alive = false;
-        if (f.nrc && alive && bounce > 0u && (state & kPrevDelta) == 0u) {
+        if (f.nrc && alive && bounce > 0u && (state & (kPrevDelta | kRecorded)) == 0u) {
             // The first vertex after a non-delta scatter: rendering paths end here with the cache's
-            // answer; training paths go on and record the vertex.
+            // answer; training paths go on and record this vertex only, so that the records see the
+            // same remaining depth as the queries the cache answers.
             training = nrc_training(f, id);
             nrc_inputs(f, s, m, dir, wl, cin);
             if (training) {
@@ -977,7 +980,7 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
                     const bool mis = f.sampling_mode == 1u || (f.sampling_mode == 0u && (bounce > 0u || m.type != kDiffuse));
                     const bool nee = !prev_delta && f.sampling_mode != 3u;
                     uint32_t flags = (prev_delta ? kPrevDelta : 0u) | (nee ? kPrevNee : 0u) | (mis ? kPrevMis : 0u) |
-                                     (emission_only ? kEmissionOnly : 0u);
+                                     (emission_only ? kEmissionOnly : 0u) | ((record || (state & kRecorded)) ? kRecorded : 0u);
                     state = min(bounce + 1u, 255u) | ((uint32_t)min(sdepth, 255) << 8) | flags;
                     cont = true;
                 }
@@ -995,13 +998,13 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
         const uint32_t qslot = wave_append(nrc.q_count, query);
         const uint32_t rslot = wave_append(nrc.r_count, record);
         if (query && qslot < capacity) {
-            for (int k = 0; k < kNrcInputCount; ++k) nrc.q_in[(size_t)qslot * kNrcInputCount + k] = cin[k];
+            for (int k = 0; k < kNrcInputCount; ++k) nrc.q_in[(size_t)k * nrc.q_stride + qslot] = cin[k];
             nrc.q_thr[qslot] = make_float4(thr.v[0], thr.v[1], thr.v[2], thr.v[3]);
             nrc.q_wl[qslot] = wl;
             nrc.q_id[qslot] = id;
         }
         if (record && rslot < nrc.r_cap) {
-            for (int k = 0; k < kNrcInputCount; ++k) nrc.r_in[(size_t)rslot * kNrcInputCount + k] = cin[k];
+            for (int k = 0; k < kNrcInputCount; ++k) nrc.r_in[(size_t)k * nrc.r_cap + rslot] = cin[k];
             nrc.r_thr[rslot] = make_float4(thr.v[0], thr.v[1], thr.v[2], thr.v[3]);
             nrc.r_snap[rslot] = nrc.sacc[id];
             nrc.r_id[rslot] = id;
@@ -1047,7 +1050,7 @@ __global__ __launch_bounds__(256) void resolve_kernel(FrameArgs f, ShadowQueue s
     }
 }
 
-// radiance[id] += rgb(thr * max(y, 0)) for every cache query (its answer y at the path's lanes).
+// radiance[id] += rgb(thr * max(y, 0) / unit) for every cache query (its answer y at the path's lanes).
 __global__ __launch_bounds__(256) void cache_resolve_kernel(RenderScene sc, NrcArgs nrc, uint32_t capacity, float *radiance) {
     const uint32_t q = blockIdx.x * 256 + threadIdx.x;
     if (q >= min(*nrc.q_count, capacity)) return;
@@ -1055,7 +1058,9 @@ __global__ __launch_bounds__(256) void cache_resolve_kernel(RenderScene sc, NrcA
     const Lanes lanes = lanes_of(sp, nrc.q_wl[q]);
     const float4 t = nrc.q_thr[q];
     const float *y = nrc.q_out + 4 * (size_t)q;
-    const S4 v{{t.x * fmaxf(y[0], 0.0f), t.y * fmaxf(y[1], 0.0f), t.z * fmaxf(y[2], 0.0f), t.w * fmaxf(y[3], 0.0f)}};
+    const float iu = 1.0f / nrc.unit;
+    const S4 v{{t.x * fmaxf(y[0], 0.0f) * iu, t.y * fmaxf(y[1], 0.0f) * iu, t.z * fmaxf(y[2], 0.0f) * iu,
+                t.w * fmaxf(y[3], 0.0f) * iu}};
     const V3 rgb = spectrum_rgb(sp, lanes, v);
     if (!(isfinite(rgb.x) && isfinite(rgb.y) && isfinite(rgb.z))) return;
     const uint32_t id = nrc.q_id[q];
@@ -1065,16 +1070,16 @@ __global__ __launch_bounds__(256) void cache_resolve_kernel(RenderScene sc, NrcA
 }
 
 // Training target of each record: the spectral radiance its path gathered after reaching the vertex,
-// per unit throughput.
+// per unit throughput, in the network's unit.
 __global__ __launch_bounds__(256) void build_targets_kernel(NrcArgs nrc) {
     const uint32_t r = blockIdx.x * 256 + threadIdx.x;
     if (r >= min(*nrc.r_count, nrc.r_cap)) return;
     const float4 fin = nrc.sacc[nrc.r_id[r]], snap = nrc.r_snap[r], t = nrc.r_thr[r];
     float *out = nrc.r_target + 4 * (size_t)r;
-    out[0] = t.x > 0.0f ? (fin.x - snap.x) / t.x : 0.0f;
-    out[1] = t.y > 0.0f ? (fin.y - snap.y) / t.y : 0.0f;
-    out[2] = t.z > 0.0f ? (fin.z - snap.z) / t.z : 0.0f;
-    out[3] = t.w > 0.0f ? (fin.w - snap.w) / t.w : 0.0f;
+    out[0] = t.x > 0.0f ? (fin.x - snap.x) / t.x * nrc.unit : 0.0f;
+    out[1] = t.y > 0.0f ? (fin.y - snap.y) / t.y * nrc.unit : 0.0f;
+    out[2] = t.z > 0.0f ? (fin.z - snap.z) / t.z * nrc.unit : 0.0f;
+    out[3] = t.w > 0.0f ? (fin.w - snap.w) / t.w * nrc.unit : 0.0f;
     for (int k = 0; k < 4; ++k)
         if (!isfinite(out[k])) out[k] = 0.0f;
 }

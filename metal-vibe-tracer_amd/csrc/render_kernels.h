@@ -81,20 +81,25 @@ struct FrameArgs {
 
 // Queues of the learned radiance cache (indexed by queue slot unless stated).
 struct NrcArgs {
-    float *q_in;         // [cap][11] queries: inputs of the vertex where a path ends
+    float *q_in;         // [11][q_stride] queries: inputs of the vertex where a path ends
     float4 *q_thr;       // throughput reaching it, per lane
     float *q_wl;         // wavelength number
     uint32_t *q_id;      // path id
     uint32_t *q_count;
+    uint32_t q_stride;   // query capacity
     float *q_out;        // [cap][4] cache outputs
     float4 *sacc;        // [paths] spectral radiance a training path has gathered so far (by path id)
-    float *r_in;         // [rcap][11] training records: inputs of a vertex of a training path
+    float *r_in;         // [11][r_cap] training records: inputs of the cache vertex of a training path
     float4 *r_thr;       // throughput reaching it
     float4 *r_snap;      // sacc of its path when it was reached
     uint32_t *r_id;
     uint32_t *r_count;
     uint32_t r_cap;
-    float *r_target;     // [rcap][4] built after the wavefront: (sacc final - snap) / thr
+    float *r_target;     // [rcap][4] built after the wavefront: (sacc final - snap) / thr * unit
+    // The network's unit: its outputs and targets are spectral radiance times `unit` (1 / the mean of
+    // the luminance-normalised D65 over the tabulated wavelengths), so that scene radiances of order
+    // one, for which the relative loss's 0.01 is made, stay of order one.
+    float unit;
 };
 
 // One queue of paths: the ray arrays first (RayArgs layout), then the path state.

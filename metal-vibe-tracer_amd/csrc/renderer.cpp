@@ -104,11 +104,12 @@ vt::ShadowQueue shadow_view(const vt_renderer *r) {
 }
 
 // Cache queues: queries (11 inputs, throughput, wavelength, id, 4 outputs) per path slot; records
-// (11 inputs, throughput, snapshot, id, 4 targets) for 8 vertices of each training path.
+// (11 inputs, throughput, snapshot, id, 4 targets) for the one cache vertex of each training path,
+// sized at twice the expected number of training paths (1 in `period`, chosen by a hash).
 void ensure_nrc_buffers(vt_renderer *r, uint32_t period) {
     // Rounded to 64 entries so every float4 array of the layout stays 16-byte aligned.
     const uint32_t cap = (r->capacity + 63u) & ~63u;
-    const uint32_t rcap = ((r->capacity / period * 8 + 1024) + 63u) & ~63u;
+    const uint32_t rcap = ((r->capacity / period * 2 + 1024) + 63u) & ~63u;
     if (r->nrc_capacity >= cap && r->nrc_rcap >= rcap) return;
     r->nrc_queries.alloc((size_t)cap * (11 + 4 + 1 + 1 + 4) * 4);
     r->nrc_records.alloc((size_t)rcap * (11 + 4 + 4 + 1 + 4) * 4);
@@ -127,6 +128,7 @@ vt::NrcArgs nrc_view(vt_renderer *r, uint32_t *counters) {
     a.q_id = reinterpret_cast<uint32_t *>(q + cap * 16 * 4);
     a.q_out = reinterpret_cast<float *>(q + cap * 17 * 4);
     a.q_count = counters + 3;
+    a.q_stride = (uint32_t)cap;
     char *rr = r->nrc_records.as<char>();
     a.r_in = reinterpret_cast<float *>(rr);
     a.r_thr = reinterpret_cast<float4 *>(rr + rcap * 11 * 4);
@@ -136,6 +138,7 @@ vt::NrcArgs nrc_view(vt_renderer *r, uint32_t *counters) {
     a.r_count = counters + 4;
     a.r_cap = (uint32_t)rcap;
     a.sacc = r->nrc_sacc.as<float4>();
+    a.unit = 1.0f;
     return a;
 }
 
@@ -379,6 +382,31 @@ vt_status vt_renderer_set_cache(vt_renderer *r, vt_nrc *cache) {
     return VT_OK;
 }
 
+vt_status vt_renderer_cache_records(vt_renderer *r, float *inputs, float *targets, uint32_t max_records,
+                                    uint32_t *count) {
+    return vt::guarded_call([&]() -> vt_status {
+        if (!r || !count) return fail(VT_ERROR_INVALID_ARGUMENT, "renderer and count must not be NULL");
+        *count = 0;
+        if (r->nrc_rcap == 0 || !r->counters.ptr) return VT_OK;
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        uint32_t made = 0;
+        check(hipMemcpy(&made, r->counters.as<uint32_t>() + 4, 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        *count = made;
+        const uint32_t n = std::min({made, r->nrc_rcap, max_records});
+        const vt::NrcArgs a = nrc_view(r, r->counters.as<uint32_t>());
+        if (inputs && n) {
+            std::vector<float> soa((size_t)11 * n);
+            for (int k = 0; k < 11; ++k)
+                check(hipMemcpy(soa.data() + (size_t)k * n, a.r_in + (size_t)k * a.r_cap, (size_t)n * 4, hipMemcpyDeviceToHost),
+                      "hipMemcpy");
+            for (uint32_t q = 0; q < n; ++q)
+                for (int k = 0; k < 11; ++k) inputs[(size_t)q * 11 + k] = soa[(size_t)k * n + q];
+        }
+        if (targets && n) check(hipMemcpy(targets, a.r_target, (size_t)n * 4 * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        return VT_OK;
+    });
+}
+
 vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params *p, uint32_t samples, void *stream) {
     return vt::guarded_call([&]() -> vt_status {
         if (!r || !scene || !p) return fail(VT_ERROR_INVALID_ARGUMENT, "renderer, scene and params must not be NULL");
@@ -484,6 +512,10 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
             vt::NrcArgs na{};
             if (use_cache) {
                 na = nrc_view(r, cnt);
+                double mean = 0.0;  // of the D65 column of the emission basis (float, as the device reads it)
+                for (int i = 0; i < vt::kSpectralSamples; ++i) mean += scene->spectral->basis[i][0][3];
+                mean /= (double)vt::kSpectralSamples;
+                na.unit = (float)(1.0 / mean);
                 check(vt::launch_zero_f4(na.sacc, n, s), "zero");
             }
             vt::RestirArgs ra{};
@@ -531,16 +563,18 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
             }
             if (use_cache) {
                 // The cache answers the paths that ended at its vertices, then learns from this
-                // wavefront's training paths: up to 4 Adam steps of 16384 records.
+                // wavefront's training records: 4 Adam steps over equal slices of them (at most 16384
+                // records a step), the slices sized on the device from the record count.
                 const vt::NrcDevice nd = r->cache_device();
-                check(vt::launch_nrc_infer(nd, na.q_in, n, na.q_out, s, na.q_count), "cache infer");
+                check(vt::launch_nrc_infer(nd, na.q_in, n, na.q_out, s, na.q_count, na.q_stride), "cache infer");
                 check(vt::launch_cache_resolve(sc, na, n, r->radiance.as<float>(), s), "cache resolve");
                 check(vt::launch_build_targets(na, s), "cache targets");
-                for (uint32_t off = 0; train_cache && off < na.r_cap && off < 4u * 16384u; off += 16384u) {
-                    const uint32_t batch = std::min<uint32_t>(16384u, na.r_cap - off);
-                    check(vt::launch_nrc_backward(nd, na.r_in, na.r_target, batch, 0.0f, nullptr, s, na.r_count, off),
+                for (uint32_t part = 0; train_cache && part < 4u; ++part) {
+                    const uint32_t batch = std::min<uint32_t>(16384u, na.r_cap);
+                    const vt::NrcSlice slice{na.r_count, na.r_cap, 4u, part};
+                    check(vt::launch_nrc_backward(nd, na.r_in, na.r_target, batch, 0.0f, nullptr, s, slice, na.r_cap),
                           "cache backward");
-                    check(vt::launch_nrc_adam(nd, r->cache_next_step(), lr, s, na.r_count, off), "cache adam");
+                    check(vt::launch_nrc_adam(nd, r->cache_next_step(), lr, s, batch, slice), "cache adam");
                 }
             }
             check(vt::launch_accumulate(f, r->radiance.as<float>(), r->accum.as<float>(), r->frame_index + 1, s),

@@ -30,7 +30,7 @@ EXPORTED = [
     "vt_accum_readback", "vt_accum_frames", "vt_gen_cornell", "vt_spectral_tables", "vt_spectral_reflectance",
     "vt_scene_set_light_tree", "vt_light_tree_build_host", "vt_energy_tables",
     "vt_nrc_create", "vt_nrc_destroy", "vt_nrc_get_params", "vt_nrc_set_params", "vt_nrc_infer", "vt_nrc_train",
-    "vt_nrc_gradients", "vt_renderer_set_cache",
+    "vt_nrc_gradients", "vt_renderer_set_cache", "vt_renderer_cache_records",
 ]
 
 
@@ -134,6 +134,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_nrc_train": (i32, [vp, vp, vp, u32, c.c_float, vp]),
         "vt_nrc_gradients": (i32, [vp, vp, vp, u32, vp, vp]),
         "vt_renderer_set_cache": (i32, [vp, vp]),
+        "vt_renderer_cache_records": (i32, [vp, vp, vp, u32, P(u32)]),
         "vt_accum_readback": (i32, [vp, vp, c.c_size_t]),
         "vt_accum_frames": (u32, [vp]),
         "vt_spectral_tables": (i32, [vp, vp, vp]),
@@ -478,6 +479,15 @@ class Renderer:
 
     def reset(self):
         _check(load().vt_reset_accumulation(self._h))
+
+    def cache_records(self, max_records=1 << 22):
+        """(inputs (n, 11), targets (n, 4), made) of the last wavefront's cache training records."""
+        x = np.empty((max_records, 11), np.float32)
+        t = np.empty((max_records, 4), np.float32)
+        made = ctypes.c_uint32()
+        _check(load().vt_renderer_cache_records(self._h, _ptr(x), _ptr(t), max_records, ctypes.byref(made)))
+        n = min(made.value, max_records)
+        return x[:n], t[:n], made.value
 
     def set_cache(self, cache):
         """Attach a vt.NRC (None detaches); keep the cache alive while attached."""

@@ -1,0 +1,89 @@
+"""Seeded scene recipes of the large-scene workloads: an fBm heightfield whose faces carry a few
+dozen surface materials, lit by thousands of small emitting triangles hovering above it.
+
+  'restir'   the ReSTIR DI workload: a 2237^2 grid (~10M triangles, 8 octaves, gain 0.5, seed 3),
+             4096 emitting triangles with powers uniform in [1, 100], half of the 64 surface slots
+             Lambert (albedo uniform in [0.2, 0.8]^3) and half GGX dielectric (OpenPBR, metalness 0,
+             IOR 1.5, roughness uniform in [0.05, 1])
+  'openpbr'  the learned-cache workload: the same terrain and emitters with an OpenPBR mix in the
+             surface slots -- glossy dielectric (IOR U[1.3, 1.7]), metal, and transmissive dielectric
+             (IOR U[1.3, 1.7]) in turn, roughness uniform in [0.02, 0.5]
+
+Materials are dicts (type, albedo, emission, roughness, ior, metalness, transmission), the form the
+oracle takes; `as_materials` turns them into vt_material records.  Each emitter is an equilateral
+triangle of circumradius 0.35 at a random yaw, centred over a uniform point of [-90, 90]^2 and 2 to 6
+units above the terrain's nearest grid vertex, emitting downwards (its front side) with radiance
+power / (pi x area), white under D65.
+"""
+import numpy as np
+
+TERRAIN_EYE = (0.0, 40.0, -20.0)
+TERRAIN_TARGET = (0.0, 0.0, 30.0)
+TERRAIN_FOV = 40.0
+
+
+def emitter_terrain(kind="restir", grid=2237, emitters=4096, slots=64, seed=3, power=(1.0, 100.0)):
+    """Returns (mesh, materials): face tags 0..slots-1 pick a surface slot, slots + k emitter k."""
+    import vt
+    if kind not in ("restir", "openpbr"):
+        raise ValueError("kind must be 'restir' or 'openpbr'")
+    mesh = vt.heightfield(grid=grid, octaves=8, gain=0.5, half_extent=100.0, height=12.0, seed=seed)
+    nf, nv = len(mesh.faces), len(mesh.vertices)
+    rng = np.random.default_rng(seed)
+    tags = rng.integers(0, slots, nf, dtype=np.uint32)
+    # Emitters: unwelded triangles appended after the terrain.
+    cx, cz = rng.uniform(-90.0, 90.0, emitters), rng.uniform(-90.0, 90.0, emitters)
+    step = 200.0 / (grid - 1)
+    gi = np.clip(np.rint((cx + 100.0) / step), 0, grid - 1).astype(np.int64)
+    gj = np.clip(np.rint((cz + 100.0) / step), 0, grid - 1).astype(np.int64)
+    cy = mesh.vertices[gj * grid + gi, 1].astype(np.float64) + rng.uniform(2.0, 6.0, emitters)
+    yaw = rng.uniform(0.0, 2.0 * np.pi, emitters)
+    # Corners at yaw, yaw + 120, yaw + 240 degrees: (b - a) x (c - a) points down (-y).
+    ang = yaw[:, None] + np.array([0.0, 2.0, 4.0]) * np.pi / 3.0
+    corners = np.stack([cx[:, None] + 0.35 * np.cos(ang), np.repeat(cy[:, None], 3, 1), cz[:, None] + 0.35 * np.sin(ang)], -1)
+    ev = np.zeros((3 * emitters, 8), np.float32)
+    ev[:, :3] = corners.reshape(-1, 3)
+    ev[:, 4] = -1.0
+    ef = np.zeros((emitters, 4), np.uint32)
+    ef[:, :3] = (nv + np.arange(3 * emitters)).reshape(emitters, 3)
+    ef[:, 3] = slots + np.arange(emitters, dtype=np.uint32)
+    mesh.faces[:, 3] = tags
+    mesh.vertices = np.concatenate([mesh.vertices, ev])
+    mesh.faces = np.concatenate([mesh.faces, ef])
+    area = 0.75 * np.sqrt(3.0) * 0.35 ** 2
+    watts = rng.uniform(power[0], power[1], emitters)
+    radiance = watts / (np.pi * area)
+    mats = []
+    for k in range(slots):
+        r = rng.random(4)
+        if kind == "restir":
+            if k % 2 == 0:
+                mats.append(dict(type=0, albedo=tuple(0.2 + 0.6 * r[:3])))
+            else:
+                mats.append(dict(type=4, albedo=tuple(0.2 + 0.6 * r[:3]), roughness=0.05 + 0.95 * r[3], ior=1.5))
+        else:
+            rough = 0.02 + 0.48 * r[3]
+            ior = 1.3 + 0.4 * rng.random()
+            if k % 3 == 0:
+                mats.append(dict(type=4, albedo=tuple(0.2 + 0.6 * r[:3]), roughness=rough, ior=ior))
+            elif k % 3 == 1:
+                mats.append(dict(type=4, albedo=tuple(0.5 + 0.45 * r[:3]), roughness=rough, ior=1.5, metalness=1.0))
+            else:
+                mats.append(dict(type=4, albedo=tuple(0.8 + 0.2 * r[:3]), roughness=rough, ior=ior, transmission=1.0))
+    for L in radiance:
+        mats.append(dict(type=3, albedo=(0.0, 0.0, 0.0), emission=(float(L),) * 3))
+    return mesh, mats
+
+
+def as_materials(mats):
+    """Material dicts -> vt_material records."""
+    import vt
+    return [vt.material(m["type"], m["albedo"], m.get("emission", (0.0, 0.0, 0.0)), m.get("roughness", 0.5),
+                        m.get("ior", 1.5), m.get("metalness", 0.0), 1.0, m.get("transmission", 0.0)) for m in mats]
+
+
+def stratified_pixels(width, height, nx, ny):
+    """Flat indices of an nx x ny grid of pixel centres (the oracle's pixel subset)."""
+    xs = (np.arange(nx) * width) // nx + width // (2 * nx)
+    ys = (np.arange(ny) * height) // ny + height // (2 * ny)
+    return (ys[:, None] * width + xs[None, :]).reshape(-1)

@@ -569,7 +569,8 @@ void SUFFIX(vo_render)(const vo_render_scene *sc, const vo_frame *fr, uint32_t s
  * sample hash to a training path (1 in `period`) is the MIS path above; any other path ends at its
  * first vertex reached by a non-delta scatter that is not an emitter, adding throughput x max(y, 0)
  * at its lanes, y the network's answer for the vertex (oracle/nrc.py layout, evaluated with bfloat16
- * weights and layer inputs as the matrix cores consume them). */
+ * weights and layer inputs as the matrix cores consume them) divided by the network's unit: the
+ * reciprocal of the mean of the basis table's D65 column over its 471 wavelengths. */
 static float bf16r(float f) {
     uint32_t u;
     memcpy(&u, &f, 4);
@@ -614,7 +615,8 @@ static int nrc_training(uint32_t pix, uint32_t sample_index, uint32_t period) {
     return vo_pcg_hash((pix * 0x9E3779B9u) ^ (sample_index * 0x85EBCA6Bu)) % period == 0u;
 }
 static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const cam_t *cam, uint32_t gx, uint32_t gy,
-                           uint32_t sample_index, const float *params, const float *box_lo, const float *box_scale) {
+                           uint32_t sample_index, const float *params, const float *box_lo, const float *box_scale,
+                           float inv_unit) {
     const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
     uint32_t seed = pixel_seed(fr, gx, gy, sample_index);
     const real u = pixel_u(seed);
@@ -685,7 +687,7 @@ static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const 
             x[10] = (float)u;
             nrc_forward(params, x, y);
             s4 v;
-            for (int k = 0; k < 4; ++k) v.v[k] = thr.v[k] * (real)(y[k] > 0.0f ? y[k] : 0.0f);
+            for (int k = 0; k < 4; ++k) v.v[k] = thr.v[k] * (real)(y[k] > 0.0f ? y[k] : 0.0f) * (real)inv_unit;
             radiance = add(radiance, spectrum_rgb(sc, &l, v));
             break;
         }
@@ -713,6 +715,9 @@ void SUFFIX(vo_render_nrc)(const vo_render_scene *sc, const vo_frame *fr, uint32
     if (!wb) return;
     for (int i = 0; i < 30720; ++i) wb[i] = bf16r(params[i]);
     params = wb;
+    double mean = 0.0;
+    for (int i = 0; i < 471; ++i) mean += sc->basis[8 * i + 3];
+    const float inv_unit = 1.0f / (float)(1.0 / (mean / 471.0));
 #pragma omp parallel for schedule(dynamic, 4) num_threads(threads > 0 ? threads : omp_get_max_threads())
     for (int64_t k = 0; k < npix; ++k) {
         uint32_t gx = (uint32_t)(k % fr->width), gy = (uint32_t)(k / fr->width);
@@ -720,7 +725,7 @@ void SUFFIX(vo_render_nrc)(const vo_render_scene *sc, const vo_frame *fr, uint32
         for (uint32_t smp = 0; smp < samples; ++smp) {
             const uint32_t si = sample_index0 + smp;
             v3 c = nrc_training((uint32_t)k, si, period) ? sample_pixel(sc, fr, &cam, gx, gy, si)
-                                                          : sample_pixel_nrc(sc, fr, &cam, gx, gy, si, params, box_lo, box_scale);
+                                                          : sample_pixel_nrc(sc, fr, &cam, gx, gy, si, params, box_lo, box_scale, inv_unit);
             acc[0] += (double)c.x;
             acc[1] += (double)c.y;
             acc[2] += (double)c.z;

@@ -13,6 +13,8 @@ definition (metal-vibe-tracer_amd/csrc/nrc.h):
   loss     mean over queries and the 4 outputs of (y - t)^2 / (y^2 + 0.01), y held constant in
            the denominator (the relative L2 of neural radiance caching)
   Adam     beta1 0.9, beta2 0.99, eps 1e-8, bias-corrected
+  EMA      inference weights: ema = 0.95 ema + 0.05 params after each Adam step, ema = params when
+           the parameters are set
 """
 import numpy as np
 
@@ -87,6 +89,14 @@ def adam(params, m, v, grads, step, lr=1e-3, b1=0.9, b2=0.99, eps=1e-8):
     mh = m / (1 - b1 ** step)
     vh = v / (1 - b2 ** step)
     return params - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+EMA_DECAY = 0.95
+
+
+def ema(prev, params, decay=EMA_DECAY):
+    """One update of the inference weights' moving average."""
+    return decay * np.asarray(prev, np.float64) + (1 - decay) * np.asarray(params, np.float64)
 
 
 def to_bf16(a):

@@ -1,6 +1,7 @@
 """GPU tests of the radiance cache's kernels (metal-vibe-tracer_amd/csrc/nrc.hip) against the FP64
 oracle (oracle/nrc.py): MFMA forward outputs, the fused backward's gradients (the gradient check),
-one Adam step, ragged batch sizes, and that online training lowers the loss."""
+one Adam step, the averaged inference weights, ragged batch sizes, and that online training lowers
+the loss."""
 import numpy as np
 import pytest
 
@@ -82,6 +83,28 @@ def test_adam_step_matches_oracle(gpu_ctx):
     q = cache.params
     want, _, _ = onrc.adam(p.astype(np.float64), 0.0, 0.0, g.astype(np.float64), 1)
     assert np.abs(q - want).max() < 2e-6
+    cache.close()
+
+
+def test_inference_uses_averaged_weights(gpu_ctx):
+    # After one large Adam step (every parameter moves by about lr), inference answers with
+    # decay p0 + (1 - decay) p1, far from the network at p1.
+    cache = vt.NRC(gpu_ctx, seed=3)
+    p = _params(4)
+    cache.params = p
+    rng = np.random.default_rng(8)
+    x = rng.random((256, 11)).astype(np.float32)
+    t = rng.random((256, 4)).astype(np.float32)
+    cache.train(_dev(x), _dev(t), lr=0.05)
+    q = cache.params
+    e = onrc.ema(p.astype(np.float64), q.astype(np.float64))
+    y = cache.infer(_dev(x)).cpu().numpy()
+    ref, _ = onrc.forward(e, x, bf16=True)
+    raw, _ = onrc.forward(q.astype(np.float64), x, bf16=True)
+    scale = np.abs(ref).max()
+    err, sep = np.abs(y - ref).max() / scale, np.abs(raw - ref).max() / scale
+    print(f"ema inference error {err:.2e}, distance to the raw weights' answer {sep:.2e}", flush=True)
+    assert err < 1e-2 and sep > 0.2
     cache.close()
 
 
