@@ -198,6 +198,98 @@ def bench_learned_cache(args, ctx):
             "relmse_ratio_cold_start": e_plain / e_cold}
 
 
+def bench_restir(args, ctx):
+    """Config 3: the emitter terrain with Lambert and GGX surfaces (~10M triangles, 4096 emitters in
+    the light tree), 1920 x 1080, depth 8, ReSTIR DI (32 candidates, 1 temporal + 1 spatial reuse)
+    against MIS, `--restir-frames` frames of 1 spp each, both timed per frame (synchronised).  Error:
+    relMSE on the 64 x 36 stratified pixel subset against the committed FP64 oracle image (MIS at
+    65536 spp, tests/golden/terrain_restir_ref.npz); ReSTIR DI converges to the same image."""
+    import numpy as np
+    import torch
+
+    import vt
+    from vt import workloads
+    path = os.path.join(ROOT, "tests", "golden", "terrain_restir_ref.npz")
+    with np.load(path) as z:
+        pix, ref, depth = z["pixels"], z["radiance"], int(z["depth"])
+        w, h, ref_spp = int(z["width"]), int(z["height"]), int(z["spp"])
+    mesh, mats = workloads.emitter_terrain("restir")
+    scene = ctx.set_mesh(mesh)
+    scene.set_materials(workloads.as_materials(mats))
+    view = (workloads.TERRAIN_EYE, workloads.TERRAIN_TARGET, workloads.TERRAIN_FOV)
+    out = {"workload": f"emitter-terrain-restir-{len(mesh.faces)}tri-{w}x{h}-depth{depth}",
+           "reference": {"kind": "FP64 oracle (MIS)", "spp": ref_spp, "pixels": int(len(pix))}}
+    for name, mode in (("mis", 1), ("restir_di", 0)):
+        r = vt.Renderer(ctx)
+        for k in range(2):
+            r.render(scene, w, h, *view, max_depth=depth, samples=1, sample_index=900 + k, sampling_mode=mode,
+                     candidates=32)
+        r.reset()
+        spent = 0.0
+        for k in range(args.restir_frames):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            r.render(scene, w, h, *view, max_depth=depth, samples=1, sample_index=1 + k, sampling_mode=mode,
+                     candidates=32)
+            torch.cuda.synchronize()
+            spent += time.perf_counter() - t
+        img = r.readback()[..., :3].astype(np.float64).reshape(-1, 3)[pix]
+        out[name] = {"frames": args.restir_frames, "ms_per_frame": round(spent / args.restir_frames * 1e3, 3),
+                     "relmse": relmse(img, ref)}
+        r.close()
+    scene.close()
+    out["relmse_ratio_mis_over_restir"] = out["mis"]["relmse"] / out["restir_di"]["relmse"]
+    out["efficiency_ratio"] = (out["mis"]["relmse"] * out["mis"]["ms_per_frame"]) / \
+        (out["restir_di"]["relmse"] * out["restir_di"]["ms_per_frame"])
+    log(f"restir: MIS {out['mis']}, ReSTIR DI {out['restir_di']}")
+    return out
+
+
+def bench_batch(args, ctx):
+    """Config 4: 64 icosphere-field scenes (triangle counts log-uniform over 10k-2M, seed 11) at image
+    sizes drawn from {256^2, 512^2, 1024x576, 1920x1080}, 16 spp, depth 6, MIS, one persistent
+    wavefront per sample captured in a hipGraph.  The autotuner (capacity, persistent grid, traversal
+    work-group, and with --batch-tune-leaf the BVH leaf size) runs first; its time is reported apart.
+    `value` of this part: camera paths (pixel samples) per second over the 16 timed samples."""
+    import numpy as np
+    import torch
+
+    import vt
+    from vt import workloads
+    t0 = time.perf_counter()
+    work = workloads.batch_workload(count=args.batch_scenes)
+    scenes, jobs = [], []
+    for mesh, mats, w, h, eye, target, fov in work:
+        s = ctx.set_mesh(mesh)
+        s.set_materials(workloads.as_materials(mats))
+        scenes.append(s)
+        jobs.append((s, w, h, eye, target, fov))
+    build_s = time.perf_counter() - t0
+    tris = sum(len(m.faces) for m, *_ in work)
+    pixels = sum(w * h for _, _, w, h, *_ in work)
+    b = vt.Batch(ctx, jobs, max_depth=6)
+    t0 = time.perf_counter()
+    best, best_ms = b.autotune(leaf_size=args.batch_tune_leaf)
+    tune_s = time.perf_counter() - t0
+    spp = 16
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.render(spp)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    mean = float(np.mean([b.readback(k)[..., :3].mean() for k in range(len(jobs))]))
+    b.close()
+    for s in scenes:
+        s.close()
+    log(f"batch: {len(jobs)} scenes, {tris} triangles, {pixels} pixels a sample: {spp} spp in {dt * 1e3:.1f} ms "
+        f"({pixels * spp / dt / 1e6:.1f} Mpaths/s), tuning {best}")
+    return {"workload": f"batch-{len(jobs)}-icosphere-scenes-{tris}tri-{pixels}px-16spp-depth6",
+            "scenes": len(jobs), "triangles": int(tris), "pixels_per_sample": int(pixels), "spp": spp,
+            "seconds": round(dt, 4), "mpaths_per_s": round(pixels * spp / dt / 1e6, 1),
+            "ms_per_sample": round(dt / spp * 1e3, 3), "tuning": best, "tuned_ms_per_sample": round(best_ms, 3),
+            "autotune_seconds": round(tune_s, 2), "scene_build_seconds": round(build_s, 2), "mean_radiance": mean}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +304,11 @@ def main():
     ap.add_argument("--cache-frames", type=int, default=16, help="plain-tracer frames (1 spp each) setting the time budget")
     ap.add_argument("--cache-warmup", type=int, default=64, help="online training frames before the equal-time run")
     ap.add_argument("--cache-lr", type=float, default=1e-3, help="Adam learning rate of the online training")
+    ap.add_argument("--no-restir", action="store_true", help="skip the ReSTIR DI large-scene measurement")
+    ap.add_argument("--restir-frames", type=int, default=16, help="frames (1 spp each) of the ReSTIR measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene measurement")
+    ap.add_argument("--batch-scenes", type=int, default=64, help="scenes in the batched measurement")
+    ap.add_argument("--batch-tune-leaf", action="store_true", help="let the batch autotuner sweep the BVH leaf size")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device 3 times with a plain copy kernel (the "
                          "reference launch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -308,6 +405,8 @@ def main():
 
     integrator = None if args.no_integrator else bench_integrator(args, ctx)
     learned_cache = None if args.no_cache else bench_learned_cache(args, ctx)
+    restir = None if args.no_restir else bench_restir(args, ctx)
+    batch = None if args.no_batch else bench_batch(args, ctx)
 
     line = {
         "metric": METRIC,
@@ -334,6 +433,8 @@ def main():
         "cpu_baseline": cpu,
         "integrator": integrator,
         "learned_cache": learned_cache,
+        "restir": restir,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
     if args.copy_calibration:

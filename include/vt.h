@@ -230,6 +230,51 @@ vt_status vt_nrc_train(vt_nrc *nrc, const float *inputs, const float *targets, u
 vt_status vt_nrc_gradients(vt_nrc *nrc, const float *inputs, const float *targets, uint32_t count, float *grads,
                            float *loss);
 
+/* Rebuilds a published scene's hierarchy from its own vertex and face streams with other build
+ * options (materials and light tree kept).  The reference rebuilds only when the mesh changes
+ * (Sources/Acceleration.swift:652-782); this serves the batch autotuner's leaf-size sweep. */
+vt_status vt_scene_rebuild(vt_scene *scene, const vt_build_options *opts);
+
+/* ---- Batched rendering.  No counterpart in the reference, which renders one view at a time: many
+ * images (jobs) of many scenes are advanced together by one wavefront of persistent kernels, and one
+ * sample of every job -- all chunks, all vertices -- is captured once in a hipGraph and replayed per
+ * sample.  Paths are bounded at max_depth vertices (scenes with mirrors or smooth glass are refused:
+ * their delta chains need vt_render_frame's open-ended extension); sampling modes 1-3, no ReSTIR, no
+ * cache.  Each job's image is the same estimator, sample for sample, as vt_render_frame with the same
+ * camera and sample indices 1, 2, ... */
+typedef struct vt_batch vt_batch;
+typedef struct vt_batch_job {
+    vt_scene *scene;           /* with materials; a scene may serve several jobs */
+    uint32_t width, height;
+    float eye[3], target[3];   /* as vt_frame_params */
+    float fov_degrees;
+} vt_batch_job;
+typedef struct vt_batch_tuning {
+    uint32_t capacity;     /* paths per wavefront chunk (0 or more than a sample's pixels: all of them) */
+    uint32_t grid_waves;   /* persistent grids: waves per SIMD, 1..32 */
+    uint32_t trace_block;  /* traversal work-group size: 64, 128 or 256 */
+    uint32_t leaf_size;    /* BVH leaf size of the batch's scenes, 1..3 (0: keep); changing it rebuilds them */
+} vt_batch_tuning;
+vt_status vt_batch_create(vt_context *ctx, const vt_batch_job *jobs, uint32_t count, uint32_t max_depth,
+                          uint32_t sampling_mode, vt_batch **out);
+void vt_batch_destroy(vt_batch *batch);
+/* Replays the captured sample `samples` times after the work already queued on `stream`; `stream`
+ * waits for them. */
+vt_status vt_batch_render(vt_batch *batch, uint32_t samples, void *stream);
+/* Sample indices back to 1 and the accumulation cleared; synchronises. */
+vt_status vt_batch_reset(vt_batch *batch);
+uint32_t vt_batch_frames(const vt_batch *batch);
+/* Job `job`'s rgba32f running mean (width * height * 4 floats); synchronises. */
+vt_status vt_batch_readback(vt_batch *batch, uint32_t job, float *rgba, size_t bytes);
+vt_status vt_batch_get_tuning(const vt_batch *batch, vt_batch_tuning *tuning);
+/* Sets the tuning and captures the sample again (rebuilding the scenes for another leaf size). */
+vt_status vt_batch_set_tuning(vt_batch *batch, const vt_batch_tuning *tuning);
+/* The autotuner: from the current tuning, each parameter in turn (capacity 2^20 .. all, grid waves
+ * 1..16, traversal block 64..256, and with flags & 1 the leaf size 1..3) over its candidates, timing
+ * replays of the captured sample and keeping the fastest.  Ends with the best tuning set and the
+ * accumulation reset; *best_ms = its milliseconds per sample. */
+vt_status vt_batch_autotune(vt_batch *batch, uint32_t flags, vt_batch_tuning *best, double *best_ms);
+
 /* Host-only build of the same 8-wide BVH the upload builds (no device needed), for inspection:
  * *nodes receives node_count 80-byte node records, *tris face_count 48-byte triangle records
  * (layout: metal-vibe-tracer_amd/csrc/bvh8.h).  Free both with vt_host_free.  Mirrors the

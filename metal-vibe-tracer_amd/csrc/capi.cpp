@@ -108,11 +108,40 @@ std::unique_ptr<vt_scene> set_mesh(vt_context *ctx, const float *vertices, uint3
     return s;
 }
 
+// Rebuilds a published scene's hierarchy from its document-order streams with other options (the
+// autotuner's leaf-size sweep); materials and the light tree are kept.
+void rebuild_bvh(vt_scene *s, const BuildOptions &opt) {
+    const uint32_t vertex_count = (uint32_t)(s->host_vertices.size() / 8), face_count = (uint32_t)(s->host_faces.size() / 4);
+    Bvh8 bvh = build_bvh8(s->host_vertices.data(), vertex_count, s->host_faces.data(), face_count, opt);
+    if (bvh.nodes.size() >= (1u << 26)) throw std::invalid_argument("mesh exceeds the node index range");
+    check(hipSetDevice(s->ctx->device), "hipSetDevice");
+    s->ctx->resident -= s->nodes.bytes + s->tris.bytes;
+    s->accounted -= s->nodes.bytes + s->tris.bytes;
+    s->nodes.upload(bvh.nodes.data(), bvh.nodes.size() * sizeof(Node8));
+    s->tris.upload(bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord));
+    s->ctx->resident += s->nodes.bytes + s->tris.bytes;
+    s->accounted += s->nodes.bytes + s->tris.bytes;
+    s->view.nodes = s->nodes.ptr;
+    s->view.tris = s->tris.ptr;
+    s->view.node_count = (uint32_t)bvh.nodes.size();
+    s->view.depth = (uint32_t)bvh.depth;
+    s->view.root_max = bvh.root_max;
+    s->info.node_count = (uint32_t)bvh.nodes.size();
+    s->info.depth = (uint32_t)bvh.depth;
+    s->info.leaf_size = (uint32_t)opt.leaf_size;
+    s->info.device_bytes = bvh.nodes.size() * sizeof(Node8) + bvh.tris.size() * sizeof(TriRecord) +
+                           (uint64_t)vertex_count * 32 + (uint64_t)face_count * 16;
+    s->info.root_max = bvh.root_max;
+    s->info.build_seconds = (float)bvh.build_seconds;
+    s->generation = vt_scene::next_generation();
+}
+
 }  // namespace vt
 
 extern "C" {
 
 const char *vt_last_error(void) { return g_error.c_str(); }
+
 
 vt_status vt_context_create(const vt_context_desc *desc, vt_context **out) {
     return guarded([&]() -> vt_status {
@@ -144,6 +173,14 @@ static vt::BuildOptions build_options(const vt_build_options *opts) {
         if (opts->max_depth) o.max_depth = (int)opts->max_depth;
     }
     return o;
+}
+
+vt_status vt_scene_rebuild(vt_scene *scene, const vt_build_options *opts) {
+    return guarded([&]() -> vt_status {
+        if (!scene) return fail(VT_ERROR_INVALID_ARGUMENT, "scene must not be NULL");
+        vt::rebuild_bvh(scene, build_options(opts));
+        return VT_OK;
+    });
 }
 
 vt_status vt_bvh8_build_host(const float *vertices, uint32_t vertex_count, const uint32_t *faces, uint32_t face_count,

@@ -657,14 +657,10 @@ __device__ __forceinline__ void add_f4(float4 *p, S4 v) {
 }
 
 // ---------------------------------------------------------------- kernels
-__global__ __launch_bounds__(256) void generate_kernel(FrameArgs f, PathQueue q, float *radiance, uint32_t *count) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t n = f.npix * f.samples;
-    if (i == 0) *count = n;
-    if (i >= n) return;
-    const uint32_t s = i / f.npix, pix = i - s * f.npix;
+// The camera path of pixel `pix` for sample `sample_index`, written to queue slot i (path id i).
+__device__ __forceinline__ void generate_path(const FrameArgs &f, uint32_t pix, uint32_t sample_index, const PathQueue &q,
+                                              uint32_t i, float *radiance) {
     const uint32_t gx = pix % f.width, gy = pix / f.width;
-    const uint32_t sample_index = f.sample_index0 + s;
     // pixel_sampler (main.swift:508-515) and wavelength_numbers (main.swift:2218-2230).
     uint32_t seed = (gy * f.width + gx) ^ (sample_index * 1999999973u);
     const uint32_t h = pcg_hash(seed ^ 0x5bd1e995u);
@@ -706,6 +702,15 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameArgs f, PathQueue q,
     radiance[3 * (size_t)i] = 0.0f;
     radiance[3 * (size_t)i + 1] = 0.0f;
     radiance[3 * (size_t)i + 2] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void generate_kernel(FrameArgs f, PathQueue q, float *radiance, uint32_t *count) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n = f.npix * f.samples;
+    if (i == 0) *count = n;
+    if (i >= n) return;
+    const uint32_t s = i / f.npix;
+    generate_path(f, i - s * f.npix, f.sample_index0 + s, q, i, radiance);
 }
 
 // Pass 1 of ReSTIR DI for one sample (restir_temporal_kernel, main.swift:4842-4940): one thread per
@@ -801,13 +806,14 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
 
 // Shading of one wavefront vertex per path.  kRestir: sampling mode 0, where a diffuse primary vertex
 // takes its direct light from ReSTIR DI's spatial reuse (main.swift:7484-7580) instead of NEE.
+// Shading of queue slot i (active: i holds a path) by every lane of the wave together: the queue
+// appends are wave-wide.  `job` is carried to the continuation and shadow queues when they have a job
+// array (batched rendering).
 template <bool kRestir, bool kLayered>
-__global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f, PathQueue in, HitArgs hits,
-                                                    PathQueue out, ShadowQueue sh, Counters c, uint32_t capacity,
-                                                    float *radiance, RestirArgs ra, NrcArgs nrc) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t n = min(*c.in, capacity);
-    const bool active = i < n;
+__device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArgs &f, const PathQueue &in,
+                                           const HitArgs &hits, const PathQueue &out, const ShadowQueue &sh,
+                                           const Counters &c, uint32_t capacity, float *radiance, const RestirArgs &ra,
+                                           const NrcArgs &nrc, uint32_t i, bool active, uint32_t job) {
     bool cont = false, shadow = false;
     // Continuation and shadow-ray values produced below (written after the wave-wide appends).
     V3 new_o = v3(0.0f, 0.0f, 0.0f), new_d = v3(0.0f, 0.0f, 1.0f), prev_p = v3(0.0f, 0.0f, 0.0f);
@@ -1022,6 +1028,7 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
         out.state[slot] = state;
         out.pdf[slot] = new_pdf;
         out.px[slot] = prev_p.x; out.py[slot] = prev_p.y; out.pz[slot] = prev_p.z;
+        if (out.job) out.job[slot] = job;
     }
     if (shadow && sslot < capacity) {
         sh.ox[sslot] = sh_o.x; sh.oy[sslot] = sh_o.y; sh.oz[sslot] = sh_o.z;
@@ -1031,7 +1038,17 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
         sh.id[sslot] = id;
         sh.r[sslot] = sh_c.x; sh.g[sslot] = sh_c.y; sh.b[sslot] = sh_c.z;
         sh.s[sslot] = make_float4(sh_s.v[0], sh_s.v[1], sh_s.v[2], sh_s.v[3]);
+        if (sh.job) sh.job[sslot] = job;
     }
+}
+
+template <bool kRestir, bool kLayered>
+__global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f, PathQueue in, HitArgs hits,
+                                                    PathQueue out, ShadowQueue sh, Counters c, uint32_t capacity,
+                                                    float *radiance, RestirArgs ra, NrcArgs nrc) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n = min(*c.in, capacity);
+    shade_path<kRestir, kLayered>(sc, f, in, hits, out, sh, c, capacity, radiance, ra, nrc, i, i < n, 0u);
 }
 
 __global__ __launch_bounds__(256) void resolve_kernel(FrameArgs f, ShadowQueue sh, const uint8_t *occluded,
@@ -1174,6 +1191,87 @@ __global__ __launch_bounds__(256) void accumulate_kernel(FrameArgs f, const floa
     reinterpret_cast<float4 *>(accum)[pix] = a;
 }
 
+
+// ---------------------------------------------------------------- batched rendering
+// Persistent kernels: a fixed grid (chosen by the autotuner) strides over the device-side count, so
+// a captured graph replays them unchanged whatever the queue lengths.  Work item g of a sample is
+// pixel g - first[j] of job j (first[j] <= g < first[j + 1]); its path id is its slot in the chunk.
+__device__ __forceinline__ uint32_t job_of(const BatchTables &bt, uint32_t g) {
+    uint32_t lo = 0, hi = bt.jobs;  // largest j with first[j] <= g
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bt.first[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void batch_generate_kernel(BatchTables bt, uint32_t chunk_base, uint32_t chunk_n,
+                                                             PathQueue q, float *radiance, uint32_t *count) {
+    const uint32_t stride = gridDim.x * 256;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count = chunk_n;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < chunk_n; i += stride) {
+        const uint32_t g = chunk_base + i, j = job_of(bt, g);
+        const FrameArgs &f = bt.frames[j];
+        generate_path(f, g - bt.first[j], f.sample_index0, q, i, radiance);
+        q.job[i] = j;
+    }
+}
+
+// Waves stride together (the loop bound is wave-uniform) so the queue appends see every lane.
+template <bool kLayered>
+__global__ __launch_bounds__(256) void batch_shade_kernel(BatchTables bt, PathQueue in, HitArgs hits, PathQueue out,
+                                                          ShadowQueue sh, Counters c, uint32_t capacity, float *radiance) {
+    const uint32_t n = min(*c.in, capacity);
+    const uint32_t lane = threadIdx.x & 63u, stride = gridDim.x * 256;
+    for (uint32_t base = blockIdx.x * 256 + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t i = base + lane;
+        const bool active = i < n;
+        const uint32_t j = active ? in.job[i] : 0u;
+        shade_path<false, kLayered>(bt.scenes[bt.job_scene[j]], bt.frames[j], in, hits, out, sh, c, capacity, radiance,
+                                    RestirArgs{}, NrcArgs{}, i, active, j);
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_resolve_kernel(ShadowQueue sh, const uint8_t *occluded, const uint32_t *count,
+                                                            uint32_t capacity, float *radiance) {
+    const uint32_t n = min(*count, capacity), stride = gridDim.x * 256;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        if (occluded[i]) continue;
+        const uint32_t id = sh.id[i];
+        radiance[3 * (size_t)id] += sh.r[i];
+        radiance[3 * (size_t)id + 1] += sh.g[i];
+        radiance[3 * (size_t)id + 2] += sh.b[i];
+    }
+}
+
+// accum[g] (rgba32f, one pixel per work item) folds in the chunk's radiance as sample *frame_count + 1.
+__global__ __launch_bounds__(256) void batch_accumulate_kernel(BatchTables bt, uint32_t chunk_base, uint32_t chunk_n,
+                                                               const float *radiance, float *accum) {
+    const uint32_t stride = gridDim.x * 256, k = *bt.frame_count + 1u;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < chunk_n; i += stride) {
+        float r = radiance[3 * (size_t)i], g = radiance[3 * (size_t)i + 1], b = radiance[3 * (size_t)i + 2];
+        if (!(isfinite(r) && isfinite(g) && isfinite(b))) r = g = b = 0.0f;
+        float4 *px = reinterpret_cast<float4 *>(accum) + chunk_base + i;
+        float4 a = *px;
+        if (k <= 1u) {
+            a = make_float4(r, g, b, 1.0f);
+        } else {
+            a.x = a.x + (r - a.x) / (float)k;
+            a.y = a.y + (g - a.y) / (float)k;
+            a.z = a.z + (b - a.z) / (float)k;
+            a.w = 1.0f;
+        }
+        *px = a;
+    }
+}
+
+// After every chunk of a sample: each job's next sample index, and the accumulated sample count.
+__global__ __launch_bounds__(256) void batch_advance_kernel(BatchTables bt) {
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j < bt.jobs) bt.frames[j].sample_index0 += 1u;
+    if (j == 0) *bt.frame_count += 1u;
+}
 }  // namespace
 
 hipError_t launch_generate(const FrameArgs &f, const PathQueue &q, float *radiance, uint32_t *count, hipStream_t s) {
@@ -1256,6 +1354,39 @@ hipError_t launch_reset_counters(uint32_t *counters, uint32_t n, hipStream_t s) 
 hipError_t launch_accumulate(const FrameArgs &f, const float *radiance, float *accum, uint32_t frame_index,
                              hipStream_t s) {
     hipLaunchKernelGGL(accumulate_kernel, dim3((f.npix + 255) / 256), dim3(256), 0, s, f, radiance, accum, frame_index);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_generate(const BatchTables &bt, uint32_t chunk_base, uint32_t chunk_n, const PathQueue &q,
+                                 float *radiance, uint32_t *count, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(batch_generate_kernel, dim3(grid), dim3(256), 0, s, bt, chunk_base, chunk_n, q, radiance, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_shade(const BatchTables &bt, const PathQueue &in, const HitArgs &hits, const PathQueue &out,
+                              const ShadowQueue &sh, const Counters &c, uint32_t capacity, float *radiance, bool layered,
+                              uint32_t grid, hipStream_t s) {
+    if (layered)
+        hipLaunchKernelGGL((batch_shade_kernel<true>), dim3(grid), dim3(256), 0, s, bt, in, hits, out, sh, c, capacity, radiance);
+    else
+        hipLaunchKernelGGL((batch_shade_kernel<false>), dim3(grid), dim3(256), 0, s, bt, in, hits, out, sh, c, capacity, radiance);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_resolve(const ShadowQueue &sh, const uint8_t *occluded, const uint32_t *count, uint32_t capacity,
+                                float *radiance, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(batch_resolve_kernel, dim3(grid), dim3(256), 0, s, sh, occluded, count, capacity, radiance);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_accumulate(const BatchTables &bt, uint32_t chunk_base, uint32_t chunk_n, const float *radiance,
+                                   float *accum, uint32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(batch_accumulate_kernel, dim3(grid), dim3(256), 0, s, bt, chunk_base, chunk_n, radiance, accum);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_advance(const BatchTables &bt, hipStream_t s) {
+    hipLaunchKernelGGL(batch_advance_kernel, dim3((bt.jobs + 255) / 256), dim3(256), 0, s, bt);
     return hipGetLastError();
 }
 

@@ -112,6 +112,7 @@ struct PathQueue {
     uint32_t *state;    // bounce | scattering depth << 8 | flags << 16
     float *pdf;         // BSDF pdf of the departure that produced the ray
     float *px, *py, *pz;  // position the ray left
+    uint32_t *job = nullptr;  // batched rendering: the path's job (null for a single image)
 };
 
 struct ShadowQueue {
@@ -119,10 +120,23 @@ struct ShadowQueue {
     uint32_t *id;
     float *r, *g, *b;   // contribution if unoccluded
     float4 *s;          // the same contribution per wavelength lane (for cache training)
+    uint32_t *job = nullptr;  // batched rendering: the ray's job
 };
 
 struct Counters {
     uint32_t *in, *out, *shadow;
+};
+
+// Batched rendering: many images (jobs) of many scenes in one wavefront.  A sample's work items are
+// the jobs' pixels back to back: item g is pixel g - first[j] of job j, first[j] <= g < first[j + 1],
+// and also the pixel's place in the accumulation buffer.
+struct BatchTables {
+    const RenderScene *scenes;    // per scene
+    FrameArgs *frames;            // per job: its camera and image; sample_index0 advances on the device
+    const uint32_t *job_scene;    // per job: scene index
+    const uint32_t *first;        // per job, plus the total at [jobs]
+    uint32_t jobs;
+    uint32_t *frame_count;        // samples accumulated (device)
 };
 
 hipError_t launch_generate(const FrameArgs &f, const PathQueue &q, float *radiance, uint32_t *count, hipStream_t s);
@@ -153,5 +167,20 @@ hipError_t launch_bake_dielectric(float *er, float *et, hipStream_t s);
 hipError_t launch_reset_counters(uint32_t *counters, uint32_t n, hipStream_t s);
 hipError_t launch_accumulate(const FrameArgs &f, const float *radiance, float *accum, uint32_t frame_index,
                              hipStream_t s);
+
+// Batched rendering with persistent grids of `grid` blocks of 256 (MIS, no ReSTIR or cache): the
+// camera paths of work items [chunk_base, chunk_base + chunk_n) (one sample of each), a vertex's
+// shading with per-path scene and camera, the shadow resolve, the running mean of the chunk's pixels,
+// and the step to the next sample.
+hipError_t launch_batch_generate(const BatchTables &bt, uint32_t chunk_base, uint32_t chunk_n, const PathQueue &q,
+                                 float *radiance, uint32_t *count, uint32_t grid, hipStream_t s);
+hipError_t launch_batch_shade(const BatchTables &bt, const PathQueue &in, const HitArgs &hits, const PathQueue &out,
+                              const ShadowQueue &sh, const Counters &c, uint32_t capacity, float *radiance, bool layered,
+                              uint32_t grid, hipStream_t s);
+hipError_t launch_batch_resolve(const ShadowQueue &sh, const uint8_t *occluded, const uint32_t *count, uint32_t capacity,
+                                float *radiance, uint32_t grid, hipStream_t s);
+hipError_t launch_batch_accumulate(const BatchTables &bt, uint32_t chunk_base, uint32_t chunk_n, const float *radiance,
+                                   float *accum, uint32_t grid, hipStream_t s);
+hipError_t launch_batch_advance(const BatchTables &bt, hipStream_t s);
 
 }  // namespace vt

@@ -149,6 +149,28 @@ void normalize3(float *v) {
 
 }  // namespace
 
+namespace vt {
+void camera_frame(const float eye[3], const float target[3], float fov_degrees, uint32_t width, uint32_t height,
+                  FrameArgs &f) {
+    f.width = width;
+    f.height = height;
+    f.npix = width * height;
+    float fw[3] = {target[0] - eye[0], target[1] - eye[1], target[2] - eye[2]};
+    normalize3(fw);
+    float rt[3] = {fw[1] * 0.0f - fw[2] * 1.0f, fw[2] * 0.0f - fw[0] * 0.0f, fw[0] * 1.0f - fw[1] * 0.0f};
+    normalize3(rt);
+    float up[3] = {rt[1] * fw[2] - rt[2] * fw[1], rt[2] * fw[0] - rt[0] * fw[2], rt[0] * fw[1] - rt[1] * fw[0]};
+    for (int k = 0; k < 3; ++k) {
+        f.eye[k] = eye[k];
+        f.forward[k] = fw[k];
+        f.right[k] = rt[k];
+        f.up[k] = up[k];
+    }
+    f.fov_scale = std::tan((fov_degrees * 0.5f) * 3.14159265358979323846f / 180.0f);
+    f.aspect = (float)width / (float)height;
+}
+}  // namespace vt
+
 extern "C" {
 
 vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, uint32_t count) {
@@ -447,22 +469,7 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
         ensure_capacity(r, per * npix);
 
         vt::FrameArgs f{};
-        f.width = p->width;
-        f.height = p->height;
-        f.npix = npix;
-        float fw[3] = {p->target[0] - p->eye[0], p->target[1] - p->eye[1], p->target[2] - p->eye[2]};
-        normalize3(fw);
-        float rt[3] = {fw[1] * 0.0f - fw[2] * 1.0f, fw[2] * 0.0f - fw[0] * 0.0f, fw[0] * 1.0f - fw[1] * 0.0f};
-        normalize3(rt);
-        float up[3] = {rt[1] * fw[2] - rt[2] * fw[1], rt[2] * fw[0] - rt[0] * fw[2], rt[0] * fw[1] - rt[1] * fw[0]};
-        for (int k = 0; k < 3; ++k) {
-            f.eye[k] = p->eye[k];
-            f.forward[k] = fw[k];
-            f.right[k] = rt[k];
-            f.up[k] = up[k];
-        }
-        f.fov_scale = std::tan((p->fov_degrees * 0.5f) * 3.14159265358979323846f / 180.0f);
-        f.aspect = (float)p->width / (float)p->height;
+        vt::camera_frame(p->eye, p->target, p->fov_degrees, p->width, p->height, f);
         f.scattering_limit = std::max(0, (int)p->max_depth - 1);
         f.sampling_mode = p->sampling_mode;
         if (p->radiance_cache > 2) return fail(VT_ERROR_INVALID_ARGUMENT, "radiance_cache must be 0, 1 or 2");

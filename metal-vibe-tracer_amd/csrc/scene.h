@@ -72,6 +72,10 @@ struct vt_context {
 namespace vt {
 // Bakes ctx->energy_metal / energy_luts once (GPU bakes, host averages and unorm16 quantisation).
 void ensure_energy_tables(vt_context *ctx);
+// The pinhole camera of the reference's orbit view (eye, target, +y up, vertical fov) and the image
+// size into f (renderer.cpp).
+void camera_frame(const float eye[3], const float target[3], float fov_degrees, uint32_t width, uint32_t height,
+                  FrameArgs &f);
 }  // namespace vt
 
 // One published mesh: the reference's MeshStreams + BLAS nodes (Sources/Acceleration.swift:59-64),

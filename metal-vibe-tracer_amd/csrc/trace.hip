@@ -18,10 +18,10 @@ constexpr int kBlock = 128;
 // A tree of depth D leaves at most D - 1 node groups pending; the launcher picks the smallest
 // instantiated depth that covers the scene's tree (the builder bounds it by 24), so LDS per block,
 // and with it the number of resident waves, follows the tree actually traced.
-template <int kStack>
+template <int kStack, int kBlk = kBlock>
 struct LdsStack {
-    uint32_t (*base)[kBlock];
-    uint32_t (*bits)[kBlock];
+    uint32_t (*base)[kBlk];
+    uint32_t (*bits)[kBlk];
     int lane;
     int sp;
     __device__ __forceinline__ void push(uint32_t b, uint32_t m) {
@@ -85,7 +85,72 @@ __global__ __launch_bounds__(256) void miss_kernel(RayArgs rays, HitArgs hits, u
     }
 }
 
+// Batched rendering: rays of many scenes in one persistent launch.  Ray i belongs to job job[i],
+// whose scene is job_trace[job[i]]; a fixed grid of kBlk-lane blocks strides over the device-side
+// count.  Rays of one job are contiguous in the queues, so a wave rarely spans two scenes.
+template <bool ANY, int kStack, int kBlk>
+__global__ __launch_bounds__(kBlk) void batch_trace_kernel(const TraceScene *__restrict__ job_trace,
+                                                           const uint32_t *__restrict__ job, RayArgs rays, HitArgs hits,
+                                                           uint8_t *__restrict__ occluded, uint32_t n,
+                                                           const uint32_t *__restrict__ count) {
+    __shared__ uint32_t stack_base[kStack][kBlk];
+    __shared__ uint32_t stack_bits[kStack][kBlk];
+    const int lane = threadIdx.x;
+    n = min(n, *count);
+    for (uint32_t i = blockIdx.x * kBlk + lane; i < n; i += gridDim.x * kBlk) {
+        const TraceScene &ts = job_trace[job[i]];
+        TraceRay r{rays.ox[i], rays.oy[i], rays.oz[i], rays.dx[i], rays.dy[i], rays.dz[i], rays.tmin[i], rays.tmax[i]};
+        TraceResult res;
+        if (ts.node_count == 0) {
+            res.t = r.tmax;
+            res.b1 = res.b2 = 0.0f;
+            res.face = -1;
+            res.occluded = false;
+        } else {
+            LdsStack<kStack, kBlk> st{stack_base, stack_bits, lane, 0};
+            res = traverse<ANY, false>(reinterpret_cast<const U4 *>(ts.nodes), reinterpret_cast<const F4 *>(ts.tris),
+                                       ts.root_max, r, st);
+        }
+        if (ANY) {
+            occluded[i] = res.occluded ? 1 : 0;
+        } else {
+            hits.t[i] = res.t;
+            hits.b1[i] = res.b1;
+            hits.b2[i] = res.b2;
+            hits.face[i] = res.face;
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t launch_trace_batch(const TraceScene *job_trace, const uint32_t *job, const RayArgs &rays, const HitArgs &hits,
+                              uint8_t *occluded, uint32_t n, bool any, const uint32_t *count, uint32_t max_depth,
+                              uint32_t block, uint32_t grid, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const int need = max_depth > 1 ? (int)max_depth - 1 : 1;
+    if (need > 24 || (block != 64 && block != 128 && block != 256) || grid == 0) return hipErrorInvalidValue;
+#define VT_BATCH(S, B)                                                                                                 \
+    do {                                                                                                               \
+        if (any) hipLaunchKernelGGL((batch_trace_kernel<true, S, B>), dim3(grid), dim3(B), 0, stream, job_trace, job,   \
+                                    rays, hits, occluded, n, count);                                                   \
+        else hipLaunchKernelGGL((batch_trace_kernel<false, S, B>), dim3(grid), dim3(B), 0, stream, job_trace, job,      \
+                                rays, hits, occluded, n, count);                                                       \
+    } while (0)
+#define VT_BATCH_STACK(S)                                                                                              \
+    do {                                                                                                               \
+        if (block == 64) VT_BATCH(S, 64);                                                                              \
+        else if (block == 128) VT_BATCH(S, 128);                                                                       \
+        else VT_BATCH(S, 256);                                                                                         \
+    } while (0)
+    if (need <= 8) VT_BATCH_STACK(8);
+    else if (need <= 12) VT_BATCH_STACK(12);
+    else if (need <= 16) VT_BATCH_STACK(16);
+    else VT_BATCH_STACK(24);
+#undef VT_BATCH_STACK
+#undef VT_BATCH
+    return hipGetLastError();
+}
 
 hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitArgs &hits, uint8_t *occluded,
                         uint32_t n, bool any, unsigned long long *counters, hipStream_t stream, const uint32_t *count) {

@@ -30,7 +30,9 @@ EXPORTED = [
     "vt_accum_readback", "vt_accum_frames", "vt_gen_cornell", "vt_spectral_tables", "vt_spectral_reflectance",
     "vt_scene_set_light_tree", "vt_light_tree_build_host", "vt_energy_tables",
     "vt_nrc_create", "vt_nrc_destroy", "vt_nrc_get_params", "vt_nrc_set_params", "vt_nrc_infer", "vt_nrc_train",
-    "vt_nrc_gradients", "vt_renderer_set_cache", "vt_renderer_cache_records",
+    "vt_nrc_gradients", "vt_renderer_set_cache", "vt_renderer_cache_records", "vt_scene_rebuild",
+    "vt_batch_create", "vt_batch_destroy", "vt_batch_render", "vt_batch_reset", "vt_batch_frames",
+    "vt_batch_readback", "vt_batch_get_tuning", "vt_batch_set_tuning", "vt_batch_autotune",
 ]
 
 
@@ -70,6 +72,21 @@ def material(type=0, albedo=(0.7, 0.7, 0.7), emission=(0.0, 0.0, 0.0), roughness
              specular_weight=1.0, transmission=0.0) -> Material:
     return Material(type, (ctypes.c_float * 3)(*albedo), (ctypes.c_float * 3)(*emission), roughness, ior, metalness,
                     specular_weight, transmission)
+
+
+class BatchJob(ctypes.Structure):
+    """vt_batch_job."""
+    _fields_ = [("scene", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("eye", ctypes.c_float * 3), ("target", ctypes.c_float * 3), ("fov_degrees", ctypes.c_float)]
+
+
+class BatchTuning(ctypes.Structure):
+    """vt_batch_tuning."""
+    _fields_ = [("capacity", ctypes.c_uint32), ("grid_waves", ctypes.c_uint32), ("trace_block", ctypes.c_uint32),
+                ("leaf_size", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 class _FrameParams(ctypes.Structure):
@@ -134,6 +151,16 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_nrc_train": (i32, [vp, vp, vp, u32, c.c_float, vp]),
         "vt_nrc_gradients": (i32, [vp, vp, vp, u32, vp, vp]),
         "vt_renderer_set_cache": (i32, [vp, vp]),
+        "vt_scene_rebuild": (i32, [vp, P(_BuildOptions)]),
+        "vt_batch_create": (i32, [vp, P(BatchJob), u32, u32, u32, P(vp)]),
+        "vt_batch_destroy": (None, [vp]),
+        "vt_batch_render": (i32, [vp, u32, vp]),
+        "vt_batch_reset": (i32, [vp]),
+        "vt_batch_frames": (u32, [vp]),
+        "vt_batch_readback": (i32, [vp, u32, vp, c.c_size_t]),
+        "vt_batch_get_tuning": (i32, [vp, P(BatchTuning)]),
+        "vt_batch_set_tuning": (i32, [vp, P(BatchTuning)]),
+        "vt_batch_autotune": (i32, [vp, u32, P(BatchTuning), P(c.c_double)]),
         "vt_renderer_cache_records": (i32, [vp, vp, vp, u32, P(u32)]),
         "vt_accum_readback": (i32, [vp, vp, c.c_size_t]),
         "vt_accum_frames": (u32, [vp]),
@@ -506,6 +533,68 @@ class Renderer:
 
 
 NRC_PARAMS = 30720
+
+
+class Batch:
+    """vt_batch: many images of many scenes rendered together by one captured persistent wavefront.
+    jobs: [(scene, width, height, eye, target, fov)]."""
+
+    def __init__(self, ctx: Context, jobs, max_depth=6, sampling_mode=1):
+        arr = (BatchJob * len(jobs))()
+        self.sizes = []
+        for k, (scene, w, h, eye, target, fov) in enumerate(jobs):
+            arr[k] = BatchJob(scene._h.value if hasattr(scene._h, "value") else scene._h, w, h,
+                              (ctypes.c_float * 3)(*eye), (ctypes.c_float * 3)(*target), fov)
+            self.sizes.append((w, h))
+        self._scenes = [j[0] for j in jobs]  # kept alive with the batch
+        h = ctypes.c_void_p()
+        _check(load().vt_batch_create(ctx._h, arr, len(jobs), max_depth, sampling_mode, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            load().vt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, samples=1, stream=None):
+        _check(load().vt_batch_render(self._h, samples, _stream_ptr(stream)))
+
+    def reset(self):
+        _check(load().vt_batch_reset(self._h))
+
+    @property
+    def frames(self) -> int:
+        return int(load().vt_batch_frames(self._h))
+
+    def readback(self, job) -> np.ndarray:
+        w, h = self.sizes[job]
+        out = np.empty((h, w, 4), np.float32)
+        _check(load().vt_batch_readback(self._h, job, _ptr(out), out.nbytes))
+        return out
+
+    @property
+    def tuning(self) -> dict:
+        t = BatchTuning()
+        _check(load().vt_batch_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    @tuning.setter
+    def tuning(self, d):
+        t = BatchTuning(d.get("capacity", 0), d.get("grid_waves", 4), d.get("trace_block", 128), d.get("leaf_size", 0))
+        _check(load().vt_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, leaf_size=False):
+        """Runs the autotuner; returns (best tuning, milliseconds per sample)."""
+        t = BatchTuning()
+        ms = ctypes.c_double()
+        _check(load().vt_batch_autotune(self._h, 1 if leaf_size else 0, ctypes.byref(t), ctypes.byref(ms)))
+        return t.as_dict(), ms.value
 
 
 class NRC:

@@ -87,3 +87,43 @@ def stratified_pixels(width, height, nx, ny):
     xs = (np.arange(nx) * width) // nx + width // (2 * nx)
     ys = (np.arange(ny) * height) // ny + height // (2 * ny)
     return (ys[:, None] * width + xs[None, :]).reshape(-1)
+
+
+BATCH_SIZES = ((256, 256), (512, 512), (1024, 576), (1920, 1080))
+
+
+def icosphere_scene(triangles, seed, emitter_fraction=0.02):
+    """An icosphere field of about `triangles` triangles (subdivision 3, 1280 a sphere), radii
+    lognormal(0, 0.5), centres uniform in a cube whose side keeps the density of 1024 spheres in
+    [-50, 50]^3; spheres take one of 8 Lambert albedos, and a seeded 2% of them (at least 2) emit
+    white light of radiance 4 to 16.  Returns (mesh, materials, half_extent)."""
+    import vt
+    spheres = max(2, int(round(triangles / 1280)))
+    half = 50.0 * (spheres / 1024.0) ** (1.0 / 3.0)
+    mesh = vt.icosphere_field(spheres=spheres, subdivisions=3, mu=0.0, sigma=0.5, half_extent=half, seed=seed)
+    rng = np.random.default_rng(seed)
+    per = len(mesh.faces) // spheres
+    tags = rng.integers(0, 8, spheres).astype(np.uint32)
+    lit = rng.choice(spheres, max(2, int(round(emitter_fraction * spheres))), replace=False)
+    tags[lit] = 8 + np.arange(len(lit), dtype=np.uint32)
+    mesh.faces[:, 3] = np.repeat(tags, per)
+    mats = [dict(type=0, albedo=tuple(0.2 + 0.6 * rng.random(3))) for _ in range(8)]
+    for _ in lit:
+        L = float(rng.uniform(4.0, 16.0))
+        mats.append(dict(type=3, albedo=(0.0, 0.0, 0.0), emission=(L, L, L)))
+    return mesh, mats, half
+
+
+def batch_workload(count=64, seed=11, tri_range=(10_000, 2_000_000)):
+    """The batched workload: `count` icosphere scenes with triangle counts log-uniform over
+    tri_range, each viewed at a size drawn from BATCH_SIZES.  Returns a list of
+    (mesh, materials, width, height, eye, target, fov)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        tris = float(np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]))))
+        w, h = BATCH_SIZES[int(rng.integers(0, len(BATCH_SIZES)))]
+        mesh, mats, half = icosphere_scene(tris, seed * 1000 + k)
+        eye = (0.0, 0.6 * half, -3.4 * half)
+        out.append((mesh, mats, w, h, eye, (0.0, 0.0, 0.0), 45.0))
+    return out
