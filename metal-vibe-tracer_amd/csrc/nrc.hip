@@ -1,13 +1,13 @@
 // nrc.hip -- gfx950 kernels of the learned radiance cache (nrc.h): fused encode + MLP inference,
-// fused forward + backward with gradient accumulation, Adam.
+// fused forward + backward with gradient accumulation, Adam with a moving average of the weights.
 //
 // All matrix products run on the matrix cores (v_mfma_f32_16x16x16_bf16: bf16 operands, fp32
-// accumulation).  A wave owns 16 queries at a time.  For D = A (16 x K) . B (K x 16) the operands are
-// read from LDS as lane l: A[l % 16][k0 + 4 (l / 16) .. + 3] and B[k0 + 4 (l / 16) .. + 3][l % 16], and
-// lane l receives D[4 (l / 16) + j][l % 16], j = 0..3.  Weights are stored transposed ([out][in]), so a
-// B operand of a forward product is four consecutive bf16.  The whole bf16 network (60 KB) sits in
-// LDS for the life of a block; inference keeps one 16 x 272 input tile and two 16 x 64 hidden tiles
-// per wave, training keeps a block's 64 samples' activations for the backward pass (144 KB of LDS).
+// accumulation).  For D = A (16 x K) . B (K x 16) lane l supplies A[l % 16][k0 + 4 (l / 16) .. + 3] and
+// B[k0 + 4 (l / 16) .. + 3][l % 16] and receives D[4 (l / 16) + j][l % 16], j = 0..3.  Weights are
+// stored [out][in], so a weight fragment is four consecutive bf16.  Inference uses the weights as A
+// and the activations as B, so a layer's output is already the next layer's operand and activations
+// never leave registers; training keeps a block's 64 samples' activations in LDS for the backward
+// pass (144 KB).  The bf16 network (60 KB) sits in LDS for the life of a block.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -124,6 +124,13 @@ constexpr int kW1Pitch = 276, kWhPitch = 68;
 constexpr int kLdsW1 = 0, kLdsW2 = kLdsW1 + kNrcHidden * kW1Pitch, kLdsW3 = kLdsW2 + kNrcHidden * kWhPitch,
               kLdsW4 = kLdsW3 + kNrcHidden * kWhPitch, kLdsW5 = kLdsW4 + kNrcHidden * kWhPitch;
 constexpr int kInferLds = (kLdsW5 + kNrcOutPad * kWhPitch) * 2;  // 63,616 bytes
+#ifndef VT_NRC_INFER_TILES
+#define VT_NRC_INFER_TILES 2
+#endif
+#ifndef VT_NRC_RELOAD
+#define VT_NRC_RELOAD 1
+#endif
+constexpr int kInferTiles = VT_NRC_INFER_TILES;
 
 __device__ __forceinline__ void load_weights_padded(uint16_t *w, const uint16_t *src) {
     for (int c = threadIdx.x; c < kNrcParams / 4; c += blockDim.x) {
@@ -141,11 +148,12 @@ __device__ __forceinline__ void load_weights_padded(uint16_t *w, const uint16_t 
 }
 
 // relu and bf16 of a layer's output tiles: the next layer's B operands, h[k-step][query tile].
-__device__ __forceinline__ void relu_pack(const f4v (&acc)[4][4], s4v (&h)[4][4]) {
+template <int T>
+__device__ __forceinline__ void relu_pack(const f4v (&acc)[4][T], s4v (&h)[4][T]) {
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < T; ++t) {
             s4v v;
             v.x = (short)to_bf16(fmaxf(acc[m][t][0], 0.0f));
             v.y = (short)to_bf16(fmaxf(acc[m][t][1], 0.0f));
@@ -155,24 +163,32 @@ __device__ __forceinline__ void relu_pack(const f4v (&acc)[4][4], s4v (&h)[4][4]
         }
 }
 
-// One hidden layer: h (64 x 64 queries, B layout) -> relu(W h) in place.
-__device__ __forceinline__ void hidden_layer(const uint16_t *w, s4v (&h)[4][4], int r, int g) {
-    f4v acc[4][4] = {};
+// One hidden layer: h (64 neurons x 16 T queries, B layout) -> relu(W h) in place.
+template <int T>
+__device__ __forceinline__ void hidden_layer(const uint16_t *w, s4v (&h)[4][T], int r, int g) {
+    f4v acc[4][T] = {};
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
         s4v a[4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) a[m] = load4(w + (16 * m + r) * kWhPitch + 16 * ks + 4 * g);
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < T; ++t)
 #pragma unroll
             for (int m = 0; m < 4; ++m) acc[m][t] = mfma(a[m], h[ks][t], acc[m][t]);
     }
-    relu_pack(acc, h);
+    relu_pack<T>(acc, h);
 }
 
-__global__ __launch_bounds__(256) void nrc_infer_kernel(const uint16_t *wsrc, const float *inputs, uint32_t stride,
-                                                        uint32_t n, float *out, const uint32_t *count) {
+// A wave handles 16 T queries per pass (T tiles of 16); blocks of kInferBlock lanes share one LDS copy
+// of the network.
+#ifndef VT_NRC_INFER_BLOCK
+#define VT_NRC_INFER_BLOCK 512
+#endif
+constexpr int kInferBlock = VT_NRC_INFER_BLOCK, kInferWaves = kInferBlock / 64;
+template <int T>
+__global__ __launch_bounds__(kInferBlock) void nrc_infer_kernel(const uint16_t *wsrc, const float *inputs, uint32_t stride,
+                                                                uint32_t n, float *out, const uint32_t *count) {
     extern __shared__ __align__(16) uint16_t lds[];
     if (count) n = min(n, *count);
     if (n == 0) return;
@@ -180,15 +196,24 @@ __global__ __launch_bounds__(256) void nrc_infer_kernel(const uint16_t *wsrc, co
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
-    const uint32_t passes = (n + 63) / 64;
-    for (uint32_t p = blockIdx.x * 4 + wave; p < passes; p += gridDim.x * 4) {
-        const uint32_t base = p * 64;
-        bool live[4];
+    const uint32_t passes = (n + 16 * T - 1) / (16 * T);
+    for (uint32_t p = blockIdx.x * kInferWaves + wave; p < passes; p += gridDim.x * kInferWaves) {
+        const uint32_t base = p * 16 * T;
+#if VT_NRC_RELOAD
+        // An offset the compiler cannot see through: the hidden layers' weight fragments are read from
+        // LDS each pass instead of being hoisted into (and occupying) 104 registers per lane.
+        int zero = 0;
+        asm volatile("" : "+v"(zero));
+        const uint16_t *wl = lds + zero;
+#else
+        const uint16_t *wl = lds;
+#endif
+        bool live[T];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) live[t] = base + 16 * t + r < n;
+        for (int t = 0; t < T; ++t) live[t] = base + 16 * t + r < n;
         // Layer 1 over the 17 k-steps of the encoding: lane l's features f = 16 ks + 4 g .. + 3 are
         // bands b, b + 1 of input i = f / 24 (f < 264), or the constant 1 and zero padding.
-        f4v acc[4][4] = {};
+        f4v acc[4][T] = {};
 #pragma unroll 1
         for (int ks = 0; ks < kNrcFeatures / 16; ++ks) {
             s4v a[4];
@@ -198,7 +223,7 @@ __global__ __launch_bounds__(256) void nrc_infer_kernel(const uint16_t *wsrc, co
             constexpr int kLast = kNrcInputs * 2 * kNrcBands;  // 264
             const int i = f < kLast ? f / 24 : 0, b = (f - 24 * i) / 2;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < T; ++t) {
                 s4v bv = {0, 0, 0, 0};
                 if (f < kLast) {
                     const float xv = live[t] ? query_input(inputs, stride, base + 16 * t + r, i) : 0.0f;
@@ -210,22 +235,22 @@ __global__ __launch_bounds__(256) void nrc_infer_kernel(const uint16_t *wsrc, co
                 for (int m = 0; m < 4; ++m) acc[m][t] = mfma(a[m], bv, acc[m][t]);
             }
         }
-        s4v h[4][4];
-        relu_pack(acc, h);
-        hidden_layer(lds + kLdsW2, h, r, g);
-        hidden_layer(lds + kLdsW3, h, r, g);
-        hidden_layer(lds + kLdsW4, h, r, g);
-        f4v y[4] = {};
+        s4v h[4][T];
+        relu_pack<T>(acc, h);
+        hidden_layer<T>(wl + kLdsW2, h, r, g);
+        hidden_layer<T>(wl + kLdsW3, h, r, g);
+        hidden_layer<T>(wl + kLdsW4, h, r, g);
+        f4v y[T] = {};
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
-            const s4v a = load4(lds + kLdsW5 + r * kWhPitch + 16 * ks + 4 * g);
+            const s4v a = load4(wl + kLdsW5 + r * kWhPitch + 16 * ks + 4 * g);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) y[t] = mfma(a, h[ks][t], y[t]);
+            for (int t = 0; t < T; ++t) y[t] = mfma(a, h[ks][t], y[t]);
         }
         // Lanes 0..15 hold outputs 0..3 of query 16 t + l.
         if (g == 0) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < T; ++t)
                 if (live[t]) *reinterpret_cast<float4 *>(out + 4 * (size_t)(base + 16 * t + r)) = make_float4(y[t][0], y[t][1], y[t][2], y[t][3]);
         }
     }
@@ -411,12 +436,12 @@ hipError_t launch_nrc_infer(const NrcDevice &d, const float *inputs, uint32_t n,
     if (n == 0) return hipSuccess;
     static bool configured = false;
     if (!configured) {
-        hipError_t e = hipFuncSetAttribute((const void *)nrc_infer_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, kInferLds);
+        hipError_t e = hipFuncSetAttribute((const void *)nrc_infer_kernel<kInferTiles>, hipFuncAttributeMaxDynamicSharedMemorySize, kInferLds);
         if (e != hipSuccess) return e;
         configured = true;
     }
-    hipLaunchKernelGGL(nrc_infer_kernel, dim3(grid_for((n + 3) / 4, 64, 2)), dim3(256), kInferLds, s, d.wema, inputs, stride, n,
-                       out, count);
+    hipLaunchKernelGGL(nrc_infer_kernel<kInferTiles>, dim3(grid_for(n, 16 * kInferTiles * kInferWaves, 2)), dim3(kInferBlock),
+                       kInferLds, s, d.wema, inputs, stride, n, out, count);
     return hipGetLastError();
 }
 

@@ -1005,13 +1005,15 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
         const uint32_t rslot = wave_append(nrc.r_count, record);
         if (query && qslot < capacity) {
             for (int k = 0; k < kNrcInputCount; ++k) nrc.q_in[(size_t)k * nrc.q_stride + qslot] = cin[k];
-            nrc.q_thr[qslot] = make_float4(thr.v[0], thr.v[1], thr.v[2], thr.v[3]);
+            nrc.q_thr[qslot] = make_float4(thr.v[0], thr.v[1], thr.v[2], thr.v[3]);  // unchanged: the path ended
             nrc.q_wl[qslot] = wl;
             nrc.q_id[qslot] = id;
         }
         if (record && rslot < nrc.r_cap) {
             for (int k = 0; k < kNrcInputCount; ++k) nrc.r_in[(size_t)k * nrc.r_cap + rslot] = cin[k];
-            nrc.r_thr[rslot] = make_float4(thr.v[0], thr.v[1], thr.v[2], thr.v[3]);
+            // The throughput that reached the vertex (thr now carries this vertex's roulette and BSDF
+            // weight for the continuation): reloaded rather than kept live through the whole body.
+            nrc.r_thr[rslot] = make_float4(in.t0[i], in.t1[i], in.t2[i], in.t3[i]);
             nrc.r_snap[rslot] = nrc.sacc[id];
             nrc.r_id[rslot] = id;
         }
@@ -1037,7 +1039,7 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
         sh.tmax[sslot] = sh_tmax;
         sh.id[sslot] = id;
         sh.r[sslot] = sh_c.x; sh.g[sslot] = sh_c.y; sh.b[sslot] = sh_c.z;
-        sh.s[sslot] = make_float4(sh_s.v[0], sh_s.v[1], sh_s.v[2], sh_s.v[3]);
+        if (f.nrc) sh.s[sslot] = make_float4(sh_s.v[0], sh_s.v[1], sh_s.v[2], sh_s.v[3]);  // read for cache training only
         if (sh.job) sh.job[sslot] = job;
     }
 }
@@ -1049,6 +1051,15 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const uint32_t n = min(*c.in, capacity);
     shade_path<kRestir, kLayered>(sc, f, in, hits, out, sh, c, capacity, radiance, ra, nrc, i, i < n, 0u);
+}
+// The Lambert / delta shading (no ReSTIR) fits 96 VGPRs without spilling: 5 waves per SIMD instead of
+// the 4 its natural allocation allows, for more memory latency hidden.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void shade_lean_kernel(
+    RenderScene sc, FrameArgs f, PathQueue in, HitArgs hits, PathQueue out, ShadowQueue sh, Counters c,
+    uint32_t capacity, float *radiance, NrcArgs nrc) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n = min(*c.in, capacity);
+    shade_path<false, false>(sc, f, in, hits, out, sh, c, capacity, radiance, RestirArgs{}, nrc, i, i < n, 0u);
 }
 
 __global__ __launch_bounds__(256) void resolve_kernel(FrameArgs f, ShadowQueue sh, const uint8_t *occluded,
@@ -1288,8 +1299,7 @@ hipError_t launch_shade(const RenderScene &sc, const FrameArgs &f, const PathQue
         hipLaunchKernelGGL((shade_kernel<false, true>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
                            radiance, RestirArgs{}, nrc);
     else
-        hipLaunchKernelGGL((shade_kernel<false, false>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
-                           radiance, RestirArgs{}, nrc);
+        hipLaunchKernelGGL(shade_lean_kernel, grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity, radiance, nrc);
     return hipGetLastError();
 }
 
