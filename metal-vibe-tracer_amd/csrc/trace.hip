@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "trace_kernels.h"
 #include "traverse8.h"
 
@@ -68,6 +70,73 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(const U4 *__restrict__ no
     }
 }
 
+// Persistent traversal with lane refill.  Each block owns a contiguous range of the rays (the grid is
+// sized to what the chip holds at once); its waves draw batches of 64 ray indices from an LDS counter,
+// and a lane whose ray finishes takes the next index at once, so a wave no longer idles until its
+// slowest ray ends.  Per ray the arithmetic is exactly the one-ray-per-lane kernel's (trav_step).
+template <bool ANY, int kStack>
+__global__ __launch_bounds__(kBlock) void trace_refill_kernel(const U4 *__restrict__ nodes, const F4 *__restrict__ tris,
+                                                              RayArgs rays, HitArgs hits, uint8_t *__restrict__ occluded,
+                                                              uint32_t n, const uint32_t *__restrict__ count,
+                                                              float root_max) {
+    __shared__ uint32_t stack_base[kStack][kBlock];
+    __shared__ uint32_t stack_bits[kStack][kBlock];
+    __shared__ uint32_t next;
+    constexpr uint32_t kNone = 0xffffffffu;
+    if (count) n = min(n, *count);
+    const uint32_t chunk = ((n + gridDim.x - 1) / gridDim.x + 63u) & ~63u;
+    const uint32_t begin = min(n, blockIdx.x * chunk), end = min(n, begin + chunk);
+    if (threadIdx.x == 0) next = begin;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    LdsStack<kStack> st{stack_base, stack_bits, (int)threadIdx.x, 0};
+    TravState ts;
+    uint32_t ray = kNone;
+    // Wave-uniform: the wave's current batch of 64 indices and how many of them are handed out.
+    uint32_t batch = 0, used = 64;
+    bool exhausted = false;
+    for (;;) {
+        const unsigned long long need = __ballot(ray == kNone);
+        if (need && !exhausted) {
+            const uint32_t cnt = (uint32_t)__popcll(need), rank = (uint32_t)__popcll(need & below);
+            const uint32_t avail = 64u - used;
+            uint32_t idx = batch + used + rank;
+            if (cnt > avail) {
+                uint32_t fresh = 0;
+                if (lane == 0) fresh = atomicAdd(&next, 64u);
+                fresh = __shfl(fresh, 0);
+                if (rank >= avail) idx = fresh + (rank - avail);
+                batch = fresh;
+                used = cnt - avail;
+                if (fresh >= end) exhausted = true;
+            } else {
+                used += cnt;
+            }
+            if (ray == kNone && idx < end) {
+                ray = idx;
+                const TraceRay r{rays.ox[ray], rays.oy[ray], rays.oz[ray], rays.dx[ray], rays.dy[ray], rays.dz[ray],
+                                 rays.tmin[ray], rays.tmax[ray]};
+                trav_init(ts, r, root_max);
+                st.sp = 0;
+            }
+        }
+        if (!__ballot(ray != kNone)) break;
+        if (ray != kNone && trav_step<ANY, false>(ts, nodes, tris, st)) {
+            if (ANY) {
+                occluded[ray] = ts.res.occluded ? 1 : 0;
+            } else {
+                hits.t[ray] = ts.res.t;
+                hits.b1[ray] = ts.res.b1;
+                hits.b2[ray] = ts.res.b2;
+                hits.face[ray] = ts.res.face;
+                if (hits.instance) hits.instance[ray] = 0u;
+            }
+            ray = kNone;
+        }
+    }
+}
+
 template <bool ANY>
 __global__ __launch_bounds__(256) void miss_kernel(RayArgs rays, HitArgs hits, uint8_t *occluded, uint32_t n,
                                                    const uint32_t *count) {
@@ -122,6 +191,35 @@ __global__ __launch_bounds__(kBlk) void batch_trace_kernel(const TraceScene *__r
     }
 }
 
+// Blocks of the refill kernel that are resident at once on the current device (at least 1), capped by
+// the blocks n rays need; the occupancy query is made once per kernel.
+uint32_t refill_grid(const void *kernel, uint32_t n) {
+    struct Entry {
+        const void *k;
+        uint32_t blocks;
+    };
+    static Entry cache[8] = {};
+    static std::mutex lock;
+    std::lock_guard<std::mutex> guard(lock);
+    uint32_t resident = 0;
+    for (auto &e : cache)
+        if (e.k == kernel) resident = e.blocks;
+    if (!resident) {
+        int dev = 0, cus = 256, per = 1;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess || per < 1) per = 1;
+        resident = (uint32_t)(cus * per);
+        for (auto &e : cache)
+            if (!e.k) {
+                e = Entry{kernel, resident};
+                break;
+            }
+    }
+    const uint32_t need = (n + kBlock - 1) / kBlock;
+    return need < resident ? (need > 0 ? need : 1) : resident;
+}
+
 }  // namespace
 
 hipError_t launch_trace_batch(const TraceScene *job_trace, const uint32_t *job, const RayArgs &rays, const HitArgs &hits,
@@ -166,6 +264,22 @@ hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitA
     const F4 *tr = reinterpret_cast<const F4 *>(scene.tris);
     const int need = scene.depth > 1 ? (int)scene.depth - 1 : 1;
     if (need > 24) return hipErrorInvalidValue;
+    if (!counters) {
+        // Persistent refill kernel: as many blocks as the chip holds at once (never more than the rays
+        // need), each draining its own range.
+#define VT_REFILL(S)                                                                                                   \
+    do {                                                                                                               \
+        const uint32_t g = refill_grid(any ? (const void *)trace_refill_kernel<true, S> : (const void *)trace_refill_kernel<false, S>, n); \
+        if (any) hipLaunchKernelGGL((trace_refill_kernel<true, S>), dim3(g), block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max); \
+        else hipLaunchKernelGGL((trace_refill_kernel<false, S>), dim3(g), block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max); \
+    } while (0)
+        if (need <= 8) VT_REFILL(8);
+        else if (need <= 12) VT_REFILL(12);
+        else if (need <= 16) VT_REFILL(16);
+        else VT_REFILL(24);
+#undef VT_REFILL
+        return hipGetLastError();
+    }
 #define VT_LAUNCH(S)                                                                                                   \
     do {                                                                                                               \
         if (counters) {                                                                                                \

@@ -123,109 +123,144 @@ struct TraceResult {
     uint32_t tris;   // triangle records tested
 };
 
+// Per-ray traversal state, so a ray can be advanced one node visit at a time (the persistent
+// kernels refill a lane with a new ray as soon as its ray finishes).
+struct TravState {
+    WoopRay w;
+    float ox, oy, oz, tmin, tmax;
+    float lowx, lowy, lowz, highx, highy, highz, invx, invy, invz;
+    bool parx, pary, parz;
+    uint32_t oct;
+    float closest;
+    // Current node group: first child index and (imask << 8 | visit-order bits still to visit).
+    uint32_t cur_base, cur_bits, node;
+    TraceResult res;
+};
+
+VT_HD void trav_init(TravState &st, const TraceRay &r, float root_max) {
+    st.res.t = r.tmax;
+    st.res.b1 = st.res.b2 = 0.0f;
+    st.res.face = -1;
+    st.res.occluded = false;
+    st.res.nodes = st.res.tris = 0;
+    st.w = woop_ray(r.dx, r.dy, r.dz);
+    st.ox = r.ox;
+    st.oy = r.oy;
+    st.oz = r.oz;
+    st.tmin = r.tmin;
+    st.tmax = r.tmax;
+    const float e = 1.1920929e-6f * (fmaxf(fabsf(r.ox), fmaxf(fabsf(r.oy), fabsf(r.oz))) + root_max);
+    st.lowx = r.ox + e;
+    st.lowy = r.oy + e;
+    st.lowz = r.oz + e;
+    st.highx = r.ox - e;
+    st.highy = r.oy - e;
+    st.highz = r.oz - e;
+    st.parx = fabsf(r.dx) < 1e-8f;
+    st.pary = fabsf(r.dy) < 1e-8f;
+    st.parz = fabsf(r.dz) < 1e-8f;
+    st.invx = 1.0f / (st.parx ? 1.0f : r.dx);
+    st.invy = 1.0f / (st.pary ? 1.0f : r.dy);
+    st.invz = 1.0f / (st.parz ? 1.0f : r.dz);
+    st.oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
+    st.closest = r.tmax;
+    st.cur_base = 0;
+    st.cur_bits = 0;
+    st.node = 0;
+}
+
+// Visits st.node: tests its 8 child boxes, tests the triangles of the leaf children hit, and moves to
+// the next node near to far.  Returns true when the ray is finished (result in st.res).
+template <bool ANY, bool STATS, class Stack>
+VT_HD bool trav_step(TravState &st, const U4 *__restrict__ nodes, const F4 *__restrict__ tris, Stack &stack) {
+    const U4 *np = nodes + 5 * (size_t)st.node;
+    const U4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+    if (STATS) ++st.res.nodes;
+    const float px = bits_to_float(w0.x), py = bits_to_float(w0.y), pz = bits_to_float(w0.z);
+    const float sx = bits_to_float((w0.w & 0xffu) << 23);
+    const float sy = bits_to_float(((w0.w >> 8) & 0xffu) << 23);
+    const float sz = bits_to_float(((w0.w >> 16) & 0xffu) << 23);
+    const uint32_t imask = w0.w >> 24;
+    uint32_t hitbits = 0, trimask = 0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const uint32_t meta = byte_of(w1.z, w1.w, s);
+        const float lox = __builtin_fmaf((float)byte_of(w2.x, w2.y, s), sx, px);
+        const float loy = __builtin_fmaf((float)byte_of(w2.z, w2.w, s), sy, py);
+        const float loz = __builtin_fmaf((float)byte_of(w3.x, w3.y, s), sz, pz);
+        const float hix = __builtin_fmaf((float)byte_of(w3.z, w3.w, s), sx, px);
+        const float hiy = __builtin_fmaf((float)byte_of(w4.x, w4.y, s), sy, py);
+        const float hiz = __builtin_fmaf((float)byte_of(w4.z, w4.w, s), sz, pz);
+        const float ax = (lox - st.lowx) * st.invx, bx = (hix - st.highx) * st.invx;
+        const float ay = (loy - st.lowy) * st.invy, by = (hiy - st.highy) * st.invy;
+        const float az = (loz - st.lowz) * st.invz, bz = (hiz - st.highz) * st.invz;
+        const float lx = st.parx ? -1e30f : fminf(ax, bx), hx = st.parx ? 1e30f : fmaxf(ax, bx);
+        const float ly = st.pary ? -1e30f : fminf(ay, by), hy = st.pary ? 1e30f : fmaxf(ay, by);
+        const float lz = st.parz ? -1e30f : fminf(az, bz), hz = st.parz ? 1e30f : fmaxf(az, bz);
+        const float tn = fmaxf(fmaxf(lx, ly), fmaxf(lz, st.tmin));
+        const float tf = fminf(fminf(hx, hy), fminf(hz, st.closest));
+        bool hit = tn <= tf && meta != 0u;
+        if (st.parx) hit = hit && !(st.lowx < lox || st.highx > hix);
+        if (st.pary) hit = hit && !(st.lowy < loy || st.highy > hiy);
+        if (st.parz) hit = hit && !(st.lowz < loz || st.highz > hiz);
+        if (hit) {
+            if (meta & 0x80u) hitbits |= 1u << (s ^ st.oct);
+            else trimask |= ((1u << (meta & 3u)) - 1u) << (meta >> 2);
+        }
+    }
+    if (hitbits) {
+        if (st.cur_bits & 0xffu) {
+            // The builder bounds the depth so a full stack never occurs; if it did, stop this ray
+            // (a wrong result, never an out-of-bounds access).
+            if (stack.full()) return true;
+            stack.push(st.cur_base, st.cur_bits);
+        }
+        st.cur_base = w1.x;
+        st.cur_bits = (imask << 8) | hitbits;
+    }
+    const uint32_t tri_base = w1.y;
+    while (trimask) {
+        const uint32_t b = __builtin_ctz(trimask);
+        trimask &= trimask - 1u;
+        const F4 *tp = tris + 3 * (size_t)(tri_base + b);
+        const F4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+        if (STATS) ++st.res.tris;
+        float t, b1, b2;
+        if (!woop_triangle(st.w, st.ox, st.oy, st.oz, p0, p1, p2, t, b1, b2)) continue;
+        const int32_t face = (int32_t)float_to_bits(p0.w);
+        if (ANY) {
+            if (t >= st.tmin && t < st.tmax) {
+                st.res.occluded = true;
+                return true;
+            }
+        } else if (t >= st.tmin && (t < st.closest || (t == st.closest && st.res.face >= 0 && face < st.res.face))) {
+            st.closest = t;
+            st.res.t = t;
+            st.res.b1 = b1;
+            st.res.b2 = b2;
+            st.res.face = face;
+        }
+    }
+    if ((st.cur_bits & 0xffu) == 0u) {
+        if (stack.empty()) return true;
+        stack.pop(st.cur_base, st.cur_bits);
+    }
+    const uint32_t v = __builtin_ctz(st.cur_bits & 0xffu);
+    st.cur_bits &= ~(1u << v);
+    const uint32_t slot = v ^ st.oct;
+    st.node = st.cur_base + __builtin_popcount((st.cur_bits >> 8) & ((1u << slot) - 1u));
+    return false;
+}
+
 // Traverses one ray.  nodes: Node8 records as 5 U4 each; tris: TriRecord as 3 F4 each.
 template <bool ANY, bool STATS, class Stack>
 VT_HD TraceResult traverse(const U4 *__restrict__ nodes, const F4 *__restrict__ tris, float root_max,
                            const TraceRay &r, Stack &stack) {
-    TraceResult res;
-    res.t = r.tmax;
-    res.b1 = res.b2 = 0.0f;
-    res.face = -1;
-    res.occluded = false;
-    res.nodes = res.tris = 0;
-
-    const WoopRay w = woop_ray(r.dx, r.dy, r.dz);
-    const float e = 1.1920929e-6f * (fmaxf(fabsf(r.ox), fmaxf(fabsf(r.oy), fabsf(r.oz))) + root_max);
-    const float lowx = r.ox + e, lowy = r.oy + e, lowz = r.oz + e;
-    const float highx = r.ox - e, highy = r.oy - e, highz = r.oz - e;
-    const bool parx = fabsf(r.dx) < 1e-8f, pary = fabsf(r.dy) < 1e-8f, parz = fabsf(r.dz) < 1e-8f;
-    const float invx = 1.0f / (parx ? 1.0f : r.dx), invy = 1.0f / (pary ? 1.0f : r.dy),
-                invz = 1.0f / (parz ? 1.0f : r.dz);
-    const uint32_t oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
-
-    float closest = r.tmax;
-    // Current node group: first child index and (imask << 8 | visit-order bits still to visit).
-    uint32_t cur_base = 0, cur_bits = 0;
-    uint32_t node = 0;
-    for (;;) {
-        const U4 *np = nodes + 5 * (size_t)node;
-        const U4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
-        if (STATS) ++res.nodes;
-        const float px = bits_to_float(w0.x), py = bits_to_float(w0.y), pz = bits_to_float(w0.z);
-        const float sx = bits_to_float((w0.w & 0xffu) << 23);
-        const float sy = bits_to_float(((w0.w >> 8) & 0xffu) << 23);
-        const float sz = bits_to_float(((w0.w >> 16) & 0xffu) << 23);
-        const uint32_t imask = w0.w >> 24;
-        uint32_t hitbits = 0, trimask = 0;
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const uint32_t meta = byte_of(w1.z, w1.w, s);
-            const float lox = __builtin_fmaf((float)byte_of(w2.x, w2.y, s), sx, px);
-            const float loy = __builtin_fmaf((float)byte_of(w2.z, w2.w, s), sy, py);
-            const float loz = __builtin_fmaf((float)byte_of(w3.x, w3.y, s), sz, pz);
-            const float hix = __builtin_fmaf((float)byte_of(w3.z, w3.w, s), sx, px);
-            const float hiy = __builtin_fmaf((float)byte_of(w4.x, w4.y, s), sy, py);
-            const float hiz = __builtin_fmaf((float)byte_of(w4.z, w4.w, s), sz, pz);
-            const float ax = (lox - lowx) * invx, bx = (hix - highx) * invx;
-            const float ay = (loy - lowy) * invy, by = (hiy - highy) * invy;
-            const float az = (loz - lowz) * invz, bz = (hiz - highz) * invz;
-            const float lx = parx ? -1e30f : fminf(ax, bx), hx = parx ? 1e30f : fmaxf(ax, bx);
-            const float ly = pary ? -1e30f : fminf(ay, by), hy = pary ? 1e30f : fmaxf(ay, by);
-            const float lz = parz ? -1e30f : fminf(az, bz), hz = parz ? 1e30f : fmaxf(az, bz);
-            const float tn = fmaxf(fmaxf(lx, ly), fmaxf(lz, r.tmin));
-            const float tf = fminf(fminf(hx, hy), fminf(hz, closest));
-            bool hit = tn <= tf && meta != 0u;
-            if (parx) hit = hit && !(lowx < lox || highx > hix);
-            if (pary) hit = hit && !(lowy < loy || highy > hiy);
-            if (parz) hit = hit && !(lowz < loz || highz > hiz);
-            if (hit) {
-                if (meta & 0x80u) hitbits |= 1u << (s ^ oct);
-                else trimask |= ((1u << (meta & 3u)) - 1u) << (meta >> 2);
-            }
-        }
-        if (hitbits) {
-            if (cur_bits & 0xffu) {
-                // The builder bounds the depth so a full stack never occurs; if it did, stop
-                // this ray (a wrong result, never an out-of-bounds access).
-                if (stack.full()) break;
-                stack.push(cur_base, cur_bits);
-            }
-            cur_base = w1.x;
-            cur_bits = (imask << 8) | hitbits;
-        }
-        const uint32_t tri_base = w1.y;
-        while (trimask) {
-            const uint32_t b = __builtin_ctz(trimask);
-            trimask &= trimask - 1u;
-            const F4 *tp = tris + 3 * (size_t)(tri_base + b);
-            const F4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-            if (STATS) ++res.tris;
-            float t, b1, b2;
-            if (!woop_triangle(w, r.ox, r.oy, r.oz, p0, p1, p2, t, b1, b2)) continue;
-            const int32_t face = (int32_t)float_to_bits(p0.w);
-            if (ANY) {
-                if (t >= r.tmin && t < r.tmax) {
-                    res.occluded = true;
-                    return res;
-                }
-            } else if (t >= r.tmin && (t < closest || (t == closest && res.face >= 0 && face < res.face))) {
-                closest = t;
-                res.t = t;
-                res.b1 = b1;
-                res.b2 = b2;
-                res.face = face;
-            }
-        }
-        if ((cur_bits & 0xffu) == 0u) {
-            if (stack.empty()) break;
-            stack.pop(cur_base, cur_bits);
-        }
-        const uint32_t v = __builtin_ctz(cur_bits & 0xffu);
-        cur_bits &= ~(1u << v);
-        const uint32_t slot = v ^ oct;
-        node = cur_base + __builtin_popcount((cur_bits >> 8) & ((1u << slot) - 1u));
+    TravState st;
+    trav_init(st, r, root_max);
+    while (!trav_step<ANY, STATS>(st, nodes, tris, stack)) {
     }
-    return res;
+    return st.res;
 }
 
 }  // namespace vt
