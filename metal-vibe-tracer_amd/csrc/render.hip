@@ -395,18 +395,79 @@ __device__ __forceinline__ float lt_importance(const LtNode &n, V3 p) {
     float cp = cos_sub(sx, cx, sb, cb);
     return cp <= n.cos_e ? 0.0f : n.phi * cp / d2;
 }
+// A light-tree node decoded once per scene (decode_light_tree_kernel, lt_node's arithmetic) into the
+// quantities lt_importance derives from it, 80 bytes: the walk then spends no instructions on
+// dequantisation, the octahedral normal, the centre, the box radius or the cone's sine.
+struct LtDec {
+    V3 pc;
+    float m;  // max(0.5 |hi - lo|, 1e-20)
+    V3 lo;
+    float phi;
+    V3 hi;
+    float dh2;  // |pc - hi|^2
+    V3 w;
+    float cos_o, so, cos_e;
+    uint32_t child, links;  // links: parent | leaf count << 27, as the node's word
+};
+__device__ __forceinline__ LtDec lt_dec(const RenderScene &sc, uint32_t i) {
+    const float4 *q = sc.tree_dec + 5 * (size_t)i;
+    const float4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4];
+    LtDec n;
+    n.pc = v3(a.x, a.y, a.z);
+    n.m = a.w;
+    n.lo = v3(b.x, b.y, b.z);
+    n.phi = b.w;
+    n.hi = v3(c.x, c.y, c.z);
+    n.dh2 = c.w;
+    n.w = v3(d.x, d.y, d.z);
+    n.cos_o = d.w;
+    n.so = e.x;
+    n.cos_e = e.y;
+    n.child = __float_as_uint(e.z);
+    n.links = __float_as_uint(e.w);
+    return n;
+}
+// lt_importance on a decoded node (the same operations in the same order).
+__device__ __forceinline__ float lt_importance(const LtDec &n, V3 p) {
+    if (!(n.phi > 0.0f)) return 0.0f;
+    const V3 dpc = p - n.pc;
+    const float dd = dot(dpc, dpc);
+    const float d2 = fmaxf(dd, n.m);
+    if (p.x >= n.lo.x && p.y >= n.lo.y && p.z >= n.lo.z && p.x <= n.hi.x && p.y <= n.hi.y && p.z <= n.hi.z)
+        return n.phi / d2;
+    const V3 wi = dpc * (1.0f / sqrtf(dd));
+    const float cw = dot(n.w, wi), sw = sqrtf(fmaxf(0.0f, 1.0f - cw * cw));
+    const float s2 = n.dh2 / dd;
+    const float cb = s2 >= 1.0f ? -1.0f : sqrtf(fmaxf(0.0f, 1.0f - s2)), sb = sqrtf(fmaxf(0.0f, 1.0f - cb * cb));
+    const float cx = cos_sub(sw, cw, n.so, n.cos_o), sx = sin_sub(sw, cw, n.so, n.cos_o);
+    const float cp = cos_sub(sx, cx, sb, cb);
+    return cp <= n.cos_e ? 0.0f : n.phi * cp / d2;
+}
+__device__ __forceinline__ float lt_leaf_sum(const RenderScene &sc, uint32_t child, uint32_t count) {
+    float s = 0.0f;
+    for (uint32_t i = 0; i < count; ++i) s += lt_power(sc, child + i);
+    return s;
+}
+
 __device__ __forceinline__ uint32_t lt_sample(const RenderScene &sc, V3 p, float u, float &pmf) {
     pmf = 1.0f;
     uint32_t index = 0;
+    // The chosen child's links (first child, leaf count) are carried down, so a level costs one round
+    // of two independent node loads (the children) instead of two dependent rounds.
+    uint32_t child, count;
+    {
+        const LtDec root = lt_dec(sc, 0);
+        child = root.child;
+        count = root.links >> 27;
+    }
     for (uint32_t depth = 0; depth <= 64u; ++depth) {
-        LtNode n = lt_node(sc, index);
-        if (n.count != 0u) {
-            float sum = lt_leaf_sum(sc, n), target = u * sum, cum = 0.0f, chosen = 0.0f;
+        if (count != 0u) {
+            float sum = lt_leaf_sum(sc, child, count), target = u * sum, cum = 0.0f, chosen = 0.0f;
             uint32_t position = 0xffffffffu;
-            for (uint32_t i = 0; i < n.count && sum > 0.0f; ++i) {
-                float power = lt_power(sc, n.child + i);
+            for (uint32_t i = 0; i < count && sum > 0.0f; ++i) {
+                float power = lt_power(sc, child + i);
                 if (!(power > 0.0f)) continue;
-                position = n.child + i;
+                position = child + i;
                 chosen = power;
                 cum += power;
                 if (target < cum) break;
@@ -415,17 +476,22 @@ __device__ __forceinline__ uint32_t lt_sample(const RenderScene &sc, V3 p, float
             pmf *= chosen / sum;
             return position;
         }
-        float l = lt_importance(lt_node(sc, index + 1u), p), r = lt_importance(lt_node(sc, n.child), p);
+        const LtDec left = lt_dec(sc, index + 1u), right = lt_dec(sc, child);
+        float l = lt_importance(left, p), r = lt_importance(right, p);
         if (!(l + r > 0.0f)) break;
         float pl = l / (l + r);
         if (u < pl) {
             u = fminf(u / pl, 0x1.fffffep-1f);
             index = index + 1u;
             pmf *= pl;
+            child = left.child;
+            count = left.links >> 27;
         } else {
             u = fminf((u - pl) / (1.0f - pl), 0x1.fffffep-1f);
-            index = n.child;
+            index = child;
             pmf *= r / (l + r);
+            child = right.child;
+            count = right.links >> 27;
         }
     }
     pmf = 0.0f;
@@ -433,14 +499,14 @@ __device__ __forceinline__ uint32_t lt_sample(const RenderScene &sc, V3 p, float
 }
 __device__ __forceinline__ float lt_pmf(const RenderScene &sc, V3 p, uint32_t position) {
     uint32_t index = sc.tree_leaves[3 * (size_t)position + 2];
-    LtNode node = lt_node(sc, index);
-    float sum = lt_leaf_sum(sc, node);
+    LtDec node = lt_dec(sc, index);
+    float sum = lt_leaf_sum(sc, node.child, node.links >> 27);
     float pmf = sum > 0.0f ? lt_power(sc, position) / sum : 0.0f;
     for (uint32_t depth = 0; depth <= 64u && pmf > 0.0f; ++depth) {
-        uint32_t parent = node.parent;
+        uint32_t parent = node.links & kNoParent;
         if (parent == kNoParent) return pmf;
-        LtNode up = lt_node(sc, parent);
-        float l = lt_importance(lt_node(sc, parent + 1u), p), r = lt_importance(lt_node(sc, up.child), p);
+        const LtDec up = lt_dec(sc, parent);
+        float l = lt_importance(lt_dec(sc, parent + 1u), p), r = lt_importance(lt_dec(sc, up.child), p);
         float mine = index == parent + 1u ? l : r;
         if (!(l + r > 0.0f) || !(mine > 0.0f)) return 0.0f;
         pmf *= mine / (l + r);
@@ -1052,9 +1118,9 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
     const uint32_t n = min(*c.in, capacity);
     shade_path<kRestir, kLayered>(sc, f, in, hits, out, sh, c, capacity, radiance, ra, nrc, i, i < n, 0u);
 }
-// The Lambert / delta shading (no ReSTIR) fits 96 VGPRs without spilling: 5 waves per SIMD instead of
-// the 4 its natural allocation allows, for more memory latency hidden.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void shade_lean_kernel(
+// The Lambert / delta shading without ReSTIR (a kernel of its own so its register allocation is not
+// shaped by the layered BSDF's).
+__global__ __launch_bounds__(256) void shade_lean_kernel(
     RenderScene sc, FrameArgs f, PathQueue in, HitArgs hits, PathQueue out, ShadowQueue sh, Counters c,
     uint32_t capacity, float *radiance, NrcArgs nrc) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -1110,6 +1176,21 @@ __global__ __launch_bounds__(256) void build_targets_kernel(NrcArgs nrc) {
     out[3] = t.w > 0.0f ? (fin.w - snap.w) / t.w * nrc.unit : 0.0f;
     for (int k = 0; k < 4; ++k)
         if (!isfinite(out[k])) out[k] = 0.0f;
+}
+
+// The decoded light tree (LtDec, 5 float4 per node) from the quantised node words.
+__global__ __launch_bounds__(256) void decode_light_tree_kernel(RenderScene sc, float4 *out) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= sc.tree_node_count) return;
+    const LtNode n = lt_node(sc, i);
+    const V3 pc = (n.lo + n.hi) * 0.5f, dh = pc - n.hi;
+    const uint32_t links = reinterpret_cast<const uint4 *>(sc.tree_nodes)[2 * i + 1].w;
+    float4 *q = out + 5 * (size_t)i;
+    q[0] = make_float4(pc.x, pc.y, pc.z, fmaxf(0.5f * length(n.hi - n.lo), 1e-20f));
+    q[1] = make_float4(n.lo.x, n.lo.y, n.lo.z, n.phi);
+    q[2] = make_float4(n.hi.x, n.hi.y, n.hi.z, dot(dh, dh));
+    q[3] = make_float4(n.w.x, n.w.y, n.w.z, n.cos_o);
+    q[4] = make_float4(sqrtf(fmaxf(0.0f, 1.0f - n.cos_o * n.cos_o)), n.cos_e, __uint_as_float(n.child), __uint_as_float(links));
 }
 
 __global__ __launch_bounds__(256) void zero_f4_kernel(float4 *p, uint32_t n) {
@@ -1337,6 +1418,12 @@ hipError_t launch_cache_resolve(const RenderScene &sc, const NrcArgs &nrc, uint3
 
 hipError_t launch_build_targets(const NrcArgs &nrc, hipStream_t s) {
     hipLaunchKernelGGL(build_targets_kernel, dim3((nrc.r_cap + 255) / 256), dim3(256), 0, s, nrc);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_light_tree(const RenderScene &sc, float4 *out, hipStream_t s) {
+    if (sc.tree_node_count == 0) return hipSuccess;
+    hipLaunchKernelGGL(decode_light_tree_kernel, dim3((sc.tree_node_count + 255) / 256), dim3(256), 0, s, sc, out);
     return hipGetLastError();
 }
 

@@ -35,6 +35,7 @@ struct RenderScene {
     const float *basis;            // 471 x 2 x 4 emission basis (w of the first: D65)
     // Light BVH (tree_node_count 0: power sampling through emitter_cdf).
     const uint32_t *tree_nodes;    // 8 words per node (Sources/LightTree.swift:192-205)
+    const float4 *tree_dec;        // the same nodes decoded, 5 float4 each (launch_decode_light_tree)
     const uint32_t *tree_leaves;   // 3 words per leaf position: face, power bits, leaf node
     const uint32_t *tree_ids;      // emitting faces, ascending (emitter_count entries)
     const uint32_t *tree_pos;      // leaf position of tree_ids[i]
@@ -160,6 +161,8 @@ hipError_t launch_cache_resolve(const RenderScene &sc, const NrcArgs &nrc, uint3
 // Training targets of the wavefront's records: (sacc[id] - snap) / thr per lane (0 where thr is 0).
 hipError_t launch_build_targets(const NrcArgs &nrc, hipStream_t s);
 hipError_t launch_zero_f4(float4 *p, uint32_t n, hipStream_t s);
+// The light tree's nodes decoded for the walk (5 float4 per node) into out.
+hipError_t launch_decode_light_tree(const RenderScene &sc, float4 *out, hipStream_t s);
 // Energy-table bakes: ideal-metal complements (129 x 129, Hammersley 16384 per node) and the
 // dielectric single-scattering albedos E_R, E_T (32 x 32 x 32, Hammersley 1024 per node).
 hipError_t launch_bake_metal(float *out, hipStream_t s);

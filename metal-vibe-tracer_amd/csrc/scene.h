@@ -85,7 +85,7 @@ struct vt_scene {
     vt::DeviceBuffer nodes, tris, counters;
     vt::DeviceBuffer vertices, faces;             // document order
     vt::DeviceBuffer materials, emitter_face, emitter_cdf, icdf, cmf, basis;
-    vt::DeviceBuffer tree_nodes, tree_leaves, tree_ids, tree_pos;
+    vt::DeviceBuffer tree_nodes, tree_leaves, tree_ids, tree_pos, tree_dec;
     uint32_t tree_node_count = 0;
     float tree_lo[3] = {0, 0, 0}, tree_scale[3] = {0, 0, 0};
     bool light_tree = true;
@@ -128,6 +128,7 @@ struct vt_scene {
         r.cmf = cmf.as<const float>();
         r.basis = basis.as<const float>();
         r.tree_nodes = tree_nodes.as<const uint32_t>();
+        r.tree_dec = tree_dec.as<const float4>();
         r.tree_leaves = tree_leaves.as<const uint32_t>();
         r.tree_ids = tree_ids.as<const uint32_t>();
         r.tree_pos = tree_pos.as<const uint32_t>();

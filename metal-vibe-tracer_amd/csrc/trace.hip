@@ -74,24 +74,55 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(const U4 *__restrict__ no
 // sized to what the chip holds at once); its waves draw batches of 64 ray indices from an LDS counter,
 // and a lane whose ray finishes takes the next index at once, so a wave no longer idles until its
 // slowest ray ends.  Per ray the arithmetic is exactly the one-ray-per-lane kernel's (trav_step).
-template <bool ANY, int kStack>
-__global__ __launch_bounds__(kBlock) void trace_refill_kernel(const U4 *__restrict__ nodes, const F4 *__restrict__ tris,
-                                                              RayArgs rays, HitArgs hits, uint8_t *__restrict__ occluded,
-                                                              uint32_t n, const uint32_t *__restrict__ count,
-                                                              float root_max) {
-    __shared__ uint32_t stack_base[kStack][kBlock];
-    __shared__ uint32_t stack_bits[kStack][kBlock];
+// Source: where a ray's scene comes from -- one scene for the launch (OneScene) or the ray's job
+// (JobScenes, batched rendering).
+struct OneScene {
+    const U4 *nodes;
+    const F4 *tris;
+    float root_max;
+    struct Lane {};
+    __device__ __forceinline__ bool begin(uint32_t, Lane &) const { return true; }
+    __device__ __forceinline__ const U4 *node_ptr(const Lane &) const { return nodes; }
+    __device__ __forceinline__ const F4 *tri_ptr(const Lane &) const { return tris; }
+    __device__ __forceinline__ float root(const Lane &) const { return root_max; }
+};
+struct JobScenes {
+    const TraceScene *job_trace;
+    const uint32_t *job;
+    struct Lane {
+        const U4 *nodes;
+        const F4 *tris;
+        float root_max;
+    };
+    // False for a scene without geometry (the ray misses).
+    __device__ __forceinline__ bool begin(uint32_t ray, Lane &l) const {
+        const TraceScene &ts = job_trace[job[ray]];
+        l.nodes = reinterpret_cast<const U4 *>(ts.nodes);
+        l.tris = reinterpret_cast<const F4 *>(ts.tris);
+        l.root_max = ts.root_max;
+        return ts.node_count != 0;
+    }
+    __device__ __forceinline__ const U4 *node_ptr(const Lane &l) const { return l.nodes; }
+    __device__ __forceinline__ const F4 *tri_ptr(const Lane &l) const { return l.tris; }
+    __device__ __forceinline__ float root(const Lane &l) const { return l.root_max; }
+};
+
+template <bool ANY, int kStack, int kBlk, class Source>
+__device__ __forceinline__ void refill_trace(const Source &src, const RayArgs &rays, const HitArgs &hits,
+                                             uint8_t *__restrict__ occluded, uint32_t n) {
+    __shared__ uint32_t stack_base[kStack][kBlk];
+    __shared__ uint32_t stack_bits[kStack][kBlk];
     __shared__ uint32_t next;
     constexpr uint32_t kNone = 0xffffffffu;
-    if (count) n = min(n, *count);
     const uint32_t chunk = ((n + gridDim.x - 1) / gridDim.x + 63u) & ~63u;
     const uint32_t begin = min(n, blockIdx.x * chunk), end = min(n, begin + chunk);
     if (threadIdx.x == 0) next = begin;
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long below = (1ull << lane) - 1ull;
-    LdsStack<kStack> st{stack_base, stack_bits, (int)threadIdx.x, 0};
+    LdsStack<kStack, kBlk> st{stack_base, stack_bits, (int)threadIdx.x, 0};
     TravState ts;
+    typename Source::Lane sl{};
     uint32_t ray = kNone;
     // Wave-uniform: the wave's current batch of 64 indices and how many of them are handed out.
     uint32_t batch = 0, used = 64;
@@ -114,15 +145,26 @@ __global__ __launch_bounds__(kBlock) void trace_refill_kernel(const U4 *__restri
                 used += cnt;
             }
             if (ray == kNone && idx < end) {
-                ray = idx;
-                const TraceRay r{rays.ox[ray], rays.oy[ray], rays.oz[ray], rays.dx[ray], rays.dy[ray], rays.dz[ray],
-                                 rays.tmin[ray], rays.tmax[ray]};
-                trav_init(ts, r, root_max);
-                st.sp = 0;
+                const TraceRay r{rays.ox[idx], rays.oy[idx], rays.oz[idx], rays.dx[idx], rays.dy[idx], rays.dz[idx],
+                                 rays.tmin[idx], rays.tmax[idx]};
+                if (src.begin(idx, sl)) {
+                    ray = idx;
+                    trav_init(ts, r, src.root(sl));
+                    st.sp = 0;
+                } else if (ANY) {
+                    occluded[idx] = 0;
+                } else {
+                    hits.t[idx] = r.tmax;
+                    hits.b1[idx] = 0.0f;
+                    hits.b2[idx] = 0.0f;
+                    hits.face[idx] = -1;
+                    if (hits.instance) hits.instance[idx] = 0u;
+                }
             }
         }
-        if (!__ballot(ray != kNone)) break;
-        if (ray != kNone && trav_step<ANY, false>(ts, nodes, tris, st)) {
+        const bool busy = ray != kNone;
+        if (!__ballot(busy) && exhausted) break;
+        if (busy && trav_step<ANY, false>(ts, src.node_ptr(sl), src.tri_ptr(sl), st)) {
             if (ANY) {
                 occluded[ray] = ts.res.occluded ? 1 : 0;
             } else {
@@ -135,6 +177,15 @@ __global__ __launch_bounds__(kBlock) void trace_refill_kernel(const U4 *__restri
             ray = kNone;
         }
     }
+}
+
+template <bool ANY, int kStack>
+__global__ __launch_bounds__(kBlock) void trace_refill_kernel(const U4 *__restrict__ nodes, const F4 *__restrict__ tris,
+                                                              RayArgs rays, HitArgs hits, uint8_t *__restrict__ occluded,
+                                                              uint32_t n, const uint32_t *__restrict__ count,
+                                                              float root_max) {
+    if (count) n = min(n, *count);
+    refill_trace<ANY, kStack, kBlock>(OneScene{nodes, tris, root_max}, rays, hits, occluded, n);
 }
 
 template <bool ANY>
@@ -154,42 +205,21 @@ __global__ __launch_bounds__(256) void miss_kernel(RayArgs rays, HitArgs hits, u
     }
 }
 
-// Batched rendering: rays of many scenes in one persistent launch.  Ray i belongs to job job[i],
-// whose scene is job_trace[job[i]]; a fixed grid of kBlk-lane blocks strides over the device-side
-// count.  Rays of one job are contiguous in the queues, so a wave rarely spans two scenes.
+// Batched rendering: rays of many scenes in one persistent launch with lane refill.  Ray i belongs to
+// job job[i], whose scene is job_trace[job[i]]; rays of one job are contiguous in the queues, so a wave
+// rarely spans two scenes.
 template <bool ANY, int kStack, int kBlk>
 __global__ __launch_bounds__(kBlk) void batch_trace_kernel(const TraceScene *__restrict__ job_trace,
                                                            const uint32_t *__restrict__ job, RayArgs rays, HitArgs hits,
                                                            uint8_t *__restrict__ occluded, uint32_t n,
                                                            const uint32_t *__restrict__ count) {
-    __shared__ uint32_t stack_base[kStack][kBlk];
-    __shared__ uint32_t stack_bits[kStack][kBlk];
-    const int lane = threadIdx.x;
     n = min(n, *count);
-    for (uint32_t i = blockIdx.x * kBlk + lane; i < n; i += gridDim.x * kBlk) {
-        const TraceScene &ts = job_trace[job[i]];
-        TraceRay r{rays.ox[i], rays.oy[i], rays.oz[i], rays.dx[i], rays.dy[i], rays.dz[i], rays.tmin[i], rays.tmax[i]};
-        TraceResult res;
-        if (ts.node_count == 0) {
-            res.t = r.tmax;
-            res.b1 = res.b2 = 0.0f;
-            res.face = -1;
-            res.occluded = false;
-        } else {
-            LdsStack<kStack, kBlk> st{stack_base, stack_bits, lane, 0};
-            res = traverse<ANY, false>(reinterpret_cast<const U4 *>(ts.nodes), reinterpret_cast<const F4 *>(ts.tris),
-                                       ts.root_max, r, st);
-        }
-        if (ANY) {
-            occluded[i] = res.occluded ? 1 : 0;
-        } else {
-            hits.t[i] = res.t;
-            hits.b1[i] = res.b1;
-            hits.b2[i] = res.b2;
-            hits.face[i] = res.face;
-        }
-    }
+    refill_trace<ANY, kStack, kBlk>(JobScenes{job_trace, job}, rays, hits, occluded, n);
 }
+
+}  // namespace
+
+namespace {
 
 // Blocks of the refill kernel that are resident at once on the current device (at least 1), capped by
 // the blocks n rays need; the occupancy query is made once per kernel.
