@@ -129,8 +129,7 @@ struct TravState {
     WoopRay w;
     float ox, oy, oz, tmin, tmax;
     float lowx, lowy, lowz, highx, highy, highz, invx, invy, invz;
-    bool parx, pary, parz;
-    uint32_t oct;
+    uint32_t oct;  // direction sign bits (x 1, y 2, z 4)
     float closest;
     // Current node group: first child index and (imask << 8 | visit-order bits still to visit).
     uint32_t cur_base, cur_bits, node;
@@ -156,13 +155,15 @@ VT_HD void trav_init(TravState &st, const TraceRay &r, float root_max) {
     st.highx = r.ox - e;
     st.highy = r.oy - e;
     st.highz = r.oz - e;
-    st.parx = fabsf(r.dx) < 1e-8f;
-    st.pary = fabsf(r.dy) < 1e-8f;
-    st.parz = fabsf(r.dz) < 1e-8f;
-    st.invx = 1.0f / (st.parx ? 1.0f : r.dx);
-    st.invy = 1.0f / (st.pary ? 1.0f : r.dy);
-    st.invz = 1.0f / (st.parz ? 1.0f : r.dz);
-    st.oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
+    // A component below 1e-8 in magnitude (the reference's parallel threshold) is taken as +-1e-8 with
+    // its sign: the slab of a parallel axis then spans about -+1e8 x the origin's distance to its
+    // planes -- the whole ray when the origin lies inside the (e-enlarged) slab, nothing reachable
+    // when it lies at least e outside -- so no separate parallel test is needed.
+    const uint32_t sx = float_to_bits(r.dx) >> 31, sy = float_to_bits(r.dy) >> 31, sz = float_to_bits(r.dz) >> 31;
+    st.invx = 1.0f / (fabsf(r.dx) < 1e-8f ? (sx ? -1e-8f : 1e-8f) : r.dx);
+    st.invy = 1.0f / (fabsf(r.dy) < 1e-8f ? (sy ? -1e-8f : 1e-8f) : r.dy);
+    st.invz = 1.0f / (fabsf(r.dz) < 1e-8f ? (sz ? -1e-8f : 1e-8f) : r.dz);
+    st.oct = sx | (sy << 1) | (sz << 2);
     st.closest = r.tmax;
     st.cur_base = 0;
     st.cur_bits = 0;
@@ -181,29 +182,36 @@ VT_HD bool trav_step(TravState &st, const U4 *__restrict__ nodes, const F4 *__re
     const float sy = bits_to_float(((w0.w >> 8) & 0xffu) << 23);
     const float sz = bits_to_float(((w0.w >> 16) & 0xffu) << 23);
     const uint32_t imask = w0.w >> 24;
+    // Slab distances of plane q of an axis: ((q s + p) - o -+ e) / d = q (s / d) + (p - o -+ e) / d, one
+    // fma per plane.  The near plane of an axis is its low plane for a positive direction, its high
+    // plane otherwise, so the byte words are chosen once per node and no per-child min / max is needed.
+    // The boxes are the reference's e-enlarged ones; the rounding differs from the reference's
+    // (plane - o -+ e) / d by a few ulps of terms no larger than the scene, far inside the enlargement,
+    // so no box the reference enters is culled.
+    const bool nx = (st.oct & 1u) != 0u, ny = (st.oct & 2u) != 0u, nz = (st.oct & 4u) != 0u;
+    const float bx = sx * st.invx, by = sy * st.invy, bz = sz * st.invz;
+    const float alx = (px - st.lowx) * st.invx, ahx = (px - st.highx) * st.invx;
+    const float aly = (py - st.lowy) * st.invy, ahy = (py - st.highy) * st.invy;
+    const float alz = (pz - st.lowz) * st.invz, ahz = (pz - st.highz) * st.invz;
+    const float anx = nx ? ahx : alx, afx = nx ? alx : ahx;
+    const float any_ = ny ? ahy : aly, afy = ny ? aly : ahy;
+    const float anz = nz ? ahz : alz, afz = nz ? alz : ahz;
+    const uint32_t nxa = nx ? w3.z : w2.x, nxb = nx ? w3.w : w2.y, fxa = nx ? w2.x : w3.z, fxb = nx ? w2.y : w3.w;
+    const uint32_t nya = ny ? w4.x : w2.z, nyb = ny ? w4.y : w2.w, fya = ny ? w2.z : w4.x, fyb = ny ? w2.w : w4.y;
+    const uint32_t nza = nz ? w4.z : w3.x, nzb = nz ? w4.w : w3.y, fza = nz ? w3.x : w4.z, fzb = nz ? w3.y : w4.w;
     uint32_t hitbits = 0, trimask = 0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         const uint32_t meta = byte_of(w1.z, w1.w, s);
-        const float lox = __builtin_fmaf((float)byte_of(w2.x, w2.y, s), sx, px);
-        const float loy = __builtin_fmaf((float)byte_of(w2.z, w2.w, s), sy, py);
-        const float loz = __builtin_fmaf((float)byte_of(w3.x, w3.y, s), sz, pz);
-        const float hix = __builtin_fmaf((float)byte_of(w3.z, w3.w, s), sx, px);
-        const float hiy = __builtin_fmaf((float)byte_of(w4.x, w4.y, s), sy, py);
-        const float hiz = __builtin_fmaf((float)byte_of(w4.z, w4.w, s), sz, pz);
-        const float ax = (lox - st.lowx) * st.invx, bx = (hix - st.highx) * st.invx;
-        const float ay = (loy - st.lowy) * st.invy, by = (hiy - st.highy) * st.invy;
-        const float az = (loz - st.lowz) * st.invz, bz = (hiz - st.highz) * st.invz;
-        const float lx = st.parx ? -1e30f : fminf(ax, bx), hx = st.parx ? 1e30f : fmaxf(ax, bx);
-        const float ly = st.pary ? -1e30f : fminf(ay, by), hy = st.pary ? 1e30f : fmaxf(ay, by);
-        const float lz = st.parz ? -1e30f : fminf(az, bz), hz = st.parz ? 1e30f : fmaxf(az, bz);
-        const float tn = fmaxf(fmaxf(lx, ly), fmaxf(lz, st.tmin));
-        const float tf = fminf(fminf(hx, hy), fminf(hz, st.closest));
-        bool hit = tn <= tf && meta != 0u;
-        if (st.parx) hit = hit && !(st.lowx < lox || st.highx > hix);
-        if (st.pary) hit = hit && !(st.lowy < loy || st.highy > hiy);
-        if (st.parz) hit = hit && !(st.lowz < loz || st.highz > hiz);
-        if (hit) {
+        const float tnx = __builtin_fmaf((float)byte_of(nxa, nxb, s), bx, anx);
+        const float tny = __builtin_fmaf((float)byte_of(nya, nyb, s), by, any_);
+        const float tnz = __builtin_fmaf((float)byte_of(nza, nzb, s), bz, anz);
+        const float tfx = __builtin_fmaf((float)byte_of(fxa, fxb, s), bx, afx);
+        const float tfy = __builtin_fmaf((float)byte_of(fya, fyb, s), by, afy);
+        const float tfz = __builtin_fmaf((float)byte_of(fza, fzb, s), bz, afz);
+        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, st.tmin));
+        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, st.closest));
+        if (tn <= tf && meta != 0u) {
             if (meta & 0x80u) hitbits |= 1u << (s ^ st.oct);
             else trimask |= ((1u << (meta & 3u)) - 1u) << (meta >> 2);
         }
