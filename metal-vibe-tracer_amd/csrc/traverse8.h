@@ -128,7 +128,7 @@ struct TraceResult {
 struct TravState {
     WoopRay w;
     float ox, oy, oz, tmin, tmax;
-    float lowx, lowy, lowz, highx, highy, highz, invx, invy, invz;
+    float e, invx, invy, invz;  // e: the reference's box enlargement for this ray
     uint32_t oct;  // direction sign bits (x 1, y 2, z 4)
     float closest;
     // Current node group: first child index and (imask << 8 | visit-order bits still to visit).
@@ -149,12 +149,7 @@ VT_HD void trav_init(TravState &st, const TraceRay &r, float root_max) {
     st.tmin = r.tmin;
     st.tmax = r.tmax;
     const float e = 1.1920929e-6f * (fmaxf(fabsf(r.ox), fmaxf(fabsf(r.oy), fabsf(r.oz))) + root_max);
-    st.lowx = r.ox + e;
-    st.lowy = r.oy + e;
-    st.lowz = r.oz + e;
-    st.highx = r.ox - e;
-    st.highy = r.oy - e;
-    st.highz = r.oz - e;
+    st.e = e;
     // A component below 1e-8 in magnitude (the reference's parallel threshold) is taken as +-1e-8 with
     // its sign: the slab of a parallel axis then spans about -+1e8 x the origin's distance to its
     // planes -- the whole ray when the origin lies inside the (e-enlarged) slab, nothing reachable
@@ -190,9 +185,10 @@ VT_HD bool trav_step(TravState &st, const U4 *__restrict__ nodes, const F4 *__re
     // so no box the reference enters is culled.
     const bool nx = (st.oct & 1u) != 0u, ny = (st.oct & 2u) != 0u, nz = (st.oct & 4u) != 0u;
     const float bx = sx * st.invx, by = sy * st.invy, bz = sz * st.invz;
-    const float alx = (px - st.lowx) * st.invx, ahx = (px - st.highx) * st.invx;
-    const float aly = (py - st.lowy) * st.invy, ahy = (py - st.highy) * st.invy;
-    const float alz = (pz - st.lowz) * st.invz, ahz = (pz - st.highz) * st.invz;
+    const float dx = px - st.ox, dy = py - st.oy, dz = pz - st.oz;
+    const float alx = (dx - st.e) * st.invx, ahx = (dx + st.e) * st.invx;
+    const float aly = (dy - st.e) * st.invy, ahy = (dy + st.e) * st.invy;
+    const float alz = (dz - st.e) * st.invz, ahz = (dz + st.e) * st.invz;
     const float anx = nx ? ahx : alx, afx = nx ? alx : ahx;
     const float any_ = ny ? ahy : aly, afy = ny ? aly : ahy;
     const float anz = nz ? ahz : alz, afz = nz ? alz : ahz;
