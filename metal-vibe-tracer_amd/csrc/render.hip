@@ -1118,6 +1118,39 @@ __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f,
     const uint32_t n = min(*c.in, capacity);
     shade_path<kRestir, kLayered>(sc, f, in, hits, out, sh, c, capacity, radiance, ra, nrc, i, i < n, 0u);
 }
+// Material sorting: paths whose hit needs the layered BSDF (GLOSSY above the delta threshold, OPENPBR)
+// are listed from the end of `perm`, all others (Lambert, delta, emitters, misses) from its start, both
+// by wave-aggregated appends; each class is then shaded by its own kernel (shade_sorted_kernel), so a
+// wave runs one BSDF branch and the Lambert class keeps the lean kernel's register budget.
+__device__ __forceinline__ bool needs_layered(const Mat &m) {
+    return m.type == kOpenPbr || (m.type == kGlossy && !is_delta(m));
+}
+__global__ __launch_bounds__(256) void classify_kernel(RenderScene sc, HitArgs hits, const uint32_t *count, uint32_t capacity,
+                                                       uint32_t *perm, uint32_t *classes) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n = min(*count, capacity);
+    const bool active = i < n;
+    bool layered = false;
+    if (active) {
+        const int face = hits.face[i];
+        if (face >= 0) layered = needs_layered(material(sc, sc.faces[4 * (size_t)face + 3]));
+    }
+    const uint32_t a = wave_append(classes, active && !layered);
+    const uint32_t b = wave_append(classes + 1, active && layered);
+    if (active && !layered) perm[a] = i;
+    if (active && layered) perm[capacity - 1u - b] = i;
+}
+template <bool kLayered>
+__global__ __launch_bounds__(256) void shade_sorted_kernel(RenderScene sc, FrameArgs f, PathQueue in, HitArgs hits,
+                                                           PathQueue out, ShadowQueue sh, Counters c, uint32_t capacity,
+                                                           float *radiance, NrcArgs nrc, const uint32_t *perm,
+                                                           const uint32_t *class_count) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    const bool active = k < min(*class_count, capacity);
+    const uint32_t i = active ? perm[kLayered ? capacity - 1u - k : k] : 0u;
+    shade_path<false, kLayered>(sc, f, in, hits, out, sh, c, capacity, radiance, RestirArgs{}, nrc, i, active, 0u);
+}
+
 // The Lambert / delta shading without ReSTIR (a kernel of its own so its register allocation is not
 // shaped by the layered BSDF's).
 __global__ __launch_bounds__(256) void shade_lean_kernel(
@@ -1381,6 +1414,18 @@ hipError_t launch_shade(const RenderScene &sc, const FrameArgs &f, const PathQue
                            radiance, RestirArgs{}, nrc);
     else
         hipLaunchKernelGGL(shade_lean_kernel, grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity, radiance, nrc);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_sorted(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
+                               const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
+                               float *radiance, const NrcArgs &nrc, uint32_t *perm, uint32_t *classes, hipStream_t s) {
+    const dim3 grid((capacity + 255) / 256);
+    hipLaunchKernelGGL(classify_kernel, grid, dim3(256), 0, s, sc, hits, c.in, capacity, perm, classes);
+    hipLaunchKernelGGL((shade_sorted_kernel<false>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity, radiance,
+                       nrc, perm, classes);
+    hipLaunchKernelGGL((shade_sorted_kernel<true>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity, radiance,
+                       nrc, perm, classes + 1);
     return hipGetLastError();
 }
 

@@ -144,6 +144,11 @@ hipError_t launch_generate(const FrameArgs &f, const PathQueue &q, float *radian
 hipError_t launch_shade(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
                         const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
                         float *radiance, const NrcArgs &nrc, hipStream_t s);
+// launch_shade with material sorting: paths are split by whether their hit needs the layered BSDF
+// (perm: capacity slots, classes: 2 zeroed device counters) and each class is shaded by its own kernel.
+hipError_t launch_shade_sorted(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,
+                               const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
+                               float *radiance, const NrcArgs &nrc, uint32_t *perm, uint32_t *classes, hipStream_t s);
 // Pass 1 of ReSTIR DI (restir_temporal_kernel): G-buffer, RIS candidates, temporal merge, encode,
 // for the wavefront's single sample (q holds the camera paths in pixel order, hits their hits).
 hipError_t launch_restir_initial(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,

@@ -46,7 +46,7 @@ struct vt_renderer {
     vt_context *ctx = nullptr;
     uint32_t width = 0, height = 0, frame_index = 0;
     uint32_t capacity = 0;  // paths per wavefront
-    DeviceBuffer accum, radiance, hits, occluded, counters, shadow;
+    DeviceBuffer accum, radiance, hits, occluded, counters, shadow, perm;
     QueueStorage q[2];
     // ReSTIR DI: two sets of G-buffer + reservoir arrays (72 B per pixel each), ping-ponged per
     // sample; restir_history counts consecutive mode-0 samples of this image (reservoirHistory).
@@ -77,6 +77,7 @@ void ensure_capacity(vt_renderer *r, uint32_t cap) {
     r->hits.alloc((size_t)4 * cap * 4);
     r->occluded.alloc(cap);
     r->radiance.alloc((size_t)3 * cap * 4);
+    r->perm.alloc((size_t)cap * 4);
     r->capacity = cap;
 }
 
@@ -190,7 +191,7 @@ vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, 
         // (main.swift:9301-9309).
         std::vector<float> rec((size_t)std::max<uint32_t>(count, 1) * 16, 0.0f);
         std::vector<double> weight(count, 0.0);
-        bool layered = false, delta = false;
+        bool layered = false, delta = false, lean = false;
         for (uint32_t k = 0; k < count; ++k) {
             const vt_material &m = materials[k];
             if (m.type > 4) return fail(VT_ERROR_INVALID_ARGUMENT, "material type must be 0..4");
@@ -215,6 +216,7 @@ vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, 
             r[12] = m.roughness; r[13] = m.ior; r[14] = m.metalness; r[15] = m.transmission;
             if (m.type == 1 || m.type == 4) layered = true;
             if (m.type == 2 || (m.type == 1 && m.roughness < 0.02f)) delta = true;
+            if (m.type == 0 || m.type == 2 || (m.type == 1 && m.roughness < 0.02f)) lean = true;
         }
         if (layered) vt::ensure_energy_tables(scene->ctx);
         // Power-sampling emitter list: every face of an emitting slot, weighted by world area x
@@ -295,6 +297,7 @@ vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, 
         scene->shaded = true;
         scene->has_delta = delta;
         scene->has_layered = layered;
+        scene->mixed = layered && lean;
         scene->generation = vt_scene::next_generation();
         return VT_OK;
     });
@@ -563,6 +566,11 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
                     check(vt::launch_restir_initial(sc, f, qin, hits, ra, s), "restir pass 1");
                     check(vt::launch_shade_restir(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), ra, na, s),
                           "shade");
+                } else if (scene->mixed) {
+                    // Material sorting (classes counted in counters 5 and 6).
+                    check(vt::launch_reset_counters(cnt + 5, 2, s), "reset");
+                    check(vt::launch_shade_sorted(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), na,
+                                                  r->perm.as<uint32_t>(), cnt + 5, s), "shade");
                 } else {
                     check(vt::launch_shade(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), na, s), "shade");
                 }

@@ -96,6 +96,7 @@ struct vt_scene {
     bool shaded = false;  // materials set
     bool has_delta = false;  // some material is a delta (mirror or smooth glass)
     bool has_layered = false;  // some material is GLOSSY or OPENPBR
+    bool mixed = false;        // materials of both shading classes (layered, and Lambert / delta)
     bool box_valid = false;    // box_lo / box_hi computed (the radiance cache's position frame)
     float box_lo[3] = {}, box_hi[3] = {};
     // Identity of the scene's current contents: a process-unique id, bumped by every material or

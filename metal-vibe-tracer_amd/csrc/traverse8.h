@@ -129,8 +129,7 @@ struct TravState {
     WoopRay w;
     float ox, oy, oz, tmin, tmax;
     float e, invx, invy, invz;  // e: the reference's box enlargement for this ray
-    uint32_t oct;  // direction sign bits (x 1, y 2, z 4)
-    float closest;
+    uint32_t oct;  // direction sign bits (x 1, y 2, z 4); the closest hit so far is res.t
     // Current node group: first child index and (imask << 8 | visit-order bits still to visit).
     uint32_t cur_base, cur_bits, node;
     TraceResult res;
@@ -159,7 +158,6 @@ VT_HD void trav_init(TravState &st, const TraceRay &r, float root_max) {
     st.invy = 1.0f / (fabsf(r.dy) < 1e-8f ? (sy ? -1e-8f : 1e-8f) : r.dy);
     st.invz = 1.0f / (fabsf(r.dz) < 1e-8f ? (sz ? -1e-8f : 1e-8f) : r.dz);
     st.oct = sx | (sy << 1) | (sz << 2);
-    st.closest = r.tmax;
     st.cur_base = 0;
     st.cur_bits = 0;
     st.node = 0;
@@ -206,7 +204,7 @@ VT_HD bool trav_step(TravState &st, const U4 *__restrict__ nodes, const F4 *__re
         const float tfy = __builtin_fmaf((float)byte_of(fya, fyb, s), by, afy);
         const float tfz = __builtin_fmaf((float)byte_of(fza, fzb, s), bz, afz);
         const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, st.tmin));
-        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, st.closest));
+        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, st.res.t));
         if (tn <= tf && meta != 0u) {
             if (meta & 0x80u) hitbits |= 1u << (s ^ st.oct);
             else trimask |= ((1u << (meta & 3u)) - 1u) << (meta >> 2);
@@ -237,8 +235,7 @@ VT_HD bool trav_step(TravState &st, const U4 *__restrict__ nodes, const F4 *__re
                 st.res.occluded = true;
                 return true;
             }
-        } else if (t >= st.tmin && (t < st.closest || (t == st.closest && st.res.face >= 0 && face < st.res.face))) {
-            st.closest = t;
+        } else if (t >= st.tmin && (t < st.res.t || (t == st.res.t && st.res.face >= 0 && face < st.res.face))) {
             st.res.t = t;
             st.res.b1 = b1;
             st.res.b2 = b2;
