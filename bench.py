@@ -183,6 +183,22 @@ def bench_learned_cache(args, ctx):
     e_nrc, n_nrc, t_nrc, t_warm = learned(args.cache_warmup)
     e_cold, n_cold, t_cold, _ = learned(0)
     scene.close()
+    # The cache's inference kernel against the matrix cores' dense bf16 peak: 2M queries (a 1080p
+    # frame's worth), timed with HIP events on the launch stream.
+    cache = vt.NRC(ctx, seed=1)
+    stream = torch.cuda.current_stream()
+    x = torch.rand((1 << 21, 11), device="cuda")
+    y = cache.infer(x, stream=stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(10):
+        cache.infer(x, out=y, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    infer_ms = e0.elapsed_time(e1) / 10
+    cache.close()
+    flop = 2.0 * x.shape[0] * (64 * 265 + 3 * 64 * 64 + 4 * 64)
+    infer_tflops = flop / (infer_ms * 1e-3) / 1e12
     log(f"learned cache: plain {args.cache_frames} spp relMSE {e_plain:.4e}; cache {n_nrc} spp relMSE {e_nrc:.4e} "
         f"(cold start {n_cold} spp, {e_cold:.4e})")
     return {"workload": f"emitter-terrain-openpbr-{len(mesh.faces)}tri-{w}x{h}-depth{depth}",
@@ -195,7 +211,11 @@ def bench_learned_cache(args, ctx):
                         "warmup_seconds": round(t_warm, 3)},
             "cold_start": {"spp": n_cold, "seconds": round(t_cold, 4), "relmse": e_cold},
             "relmse_ratio_plain_over_learned": e_plain / e_nrc,
-            "relmse_ratio_cold_start": e_plain / e_cold}
+            "relmse_ratio_cold_start": e_plain / e_cold,
+            "inference_roofline": {"bound": "mfma", "achieved": round(infer_tflops, 1), "peak": 2500.0,
+                                   "unit": "TFLOP/s", "frac": round(infer_tflops / 2500.0, 4),
+                                   "queries": int(x.shape[0]), "ms": round(infer_ms, 4),
+                                   "flops_per_query": 2 * (64 * 265 + 3 * 64 * 64 + 4 * 64)}}
 
 
 def bench_restir(args, ctx):

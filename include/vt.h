@@ -162,6 +162,10 @@ typedef struct vt_frame_params {
                                    2: the same estimator with the cache frozen (no training) */
     uint32_t cache_train_period; /* 0 means 16 */
     float cache_learning_rate;   /* 0 means 1e-3 */
+    uint32_t material_sort;      /* 1: at vertices past the first, split the paths by shading class (the
+                                    layered BSDF or not) and shade each class with its own kernel; the
+                                    image is the same.  0 (default) shades every path in one kernel,
+                                    which measured faster on the configurations here */
 } vt_frame_params;
 
 /* Replaces PathTracerRenderer (main.swift:9514) for this path: owns the accumulation buffer. */

@@ -448,6 +448,7 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
         if (candidates > 256) return fail(VT_ERROR_INVALID_ARGUMENT, "restir_candidates must be at most 256");
         if (!(p->fov_degrees > 0.0f && p->fov_degrees < 180.0f)) return fail(VT_ERROR_INVALID_ARGUMENT, "fov must be in (0, 180)");
         if (p->sample_index == 0) return fail(VT_ERROR_INVALID_ARGUMENT, "sample_index starts at 1");
+        if (p->material_sort > 1) return fail(VT_ERROR_INVALID_ARGUMENT, "material_sort must be 0 or 1");
         if (samples == 0) return VT_OK;
         check(hipSetDevice(r->ctx->device), "hipSetDevice");
         hipStream_t s = (hipStream_t)stream;
@@ -566,7 +567,7 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
                     check(vt::launch_restir_initial(sc, f, qin, hits, ra, s), "restir pass 1");
                     check(vt::launch_shade_restir(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), ra, na, s),
                           "shade");
-                } else if (scene->mixed) {
+                } else if (p->material_sort && scene->mixed) {
                     // Material sorting (classes counted in counters 5 and 6).
                     check(vt::launch_reset_counters(cnt + 5, 2, s), "reset");
                     check(vt::launch_shade_sorted(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), na,

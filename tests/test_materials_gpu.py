@@ -79,3 +79,21 @@ def test_restir_with_layered_primaries_matches_oracle(gpu_ctx):
     got = _render(gpu_ctx, mesh, mats, 96, 8, 3, 0)
     ref = _oracle(mesh, mats, 96, 8, 3, 0)
     assert _compare(got, ref, "restir opaque", 2e-3) > 0.95
+
+
+def test_material_sort_gives_the_same_image(gpu_ctx):
+    # Sorting the paths by shading class changes only which kernel shades which path: the image is
+    # the same, bit for bit (every contribution is added by its own path in vertex order).
+    mesh, mats = scenes.materials_cornell("opaque")
+    scene = gpu_ctx.set_mesh(mesh)
+    scene.set_materials(scenes.vt_materials(mats))
+    imgs = []
+    for sort in (False, True):
+        r = vt.Renderer(gpu_ctx)
+        r.render(scene, 96, 96, vt.orbit_eye(), TARGET, 42.0, max_depth=8, samples=8, sample_index=1,
+                 material_sort=sort)
+        imgs.append(r.readback())
+        r.close()
+    print(f"material sort: max |diff| {np.abs(imgs[0] - imgs[1]).max():.2e}, mean {imgs[0][..., :3].mean():.6f}", flush=True)
+    assert np.array_equal(imgs[0], imgs[1]) and imgs[0][..., :3].mean() > 0
+    scene.close()
