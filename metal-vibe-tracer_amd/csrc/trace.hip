@@ -73,10 +73,13 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(const U4 *__restrict__ no
 // Persistent traversal with lane refill.  Each block owns a contiguous range of the rays (the grid is
 // sized to what the chip holds at once); its waves draw batches of 64 ray indices from an LDS counter,
 // and a lane whose ray finishes takes the next index at once, so a wave no longer idles until its
-// slowest ray ends.  Per ray the arithmetic is exactly the one-ray-per-lane kernel's (trav_step).
+// slowest ray ends.  Node visits are per lane (trav_nodes); the triangles the wave's lanes reach in a
+// step are tested by the whole wave together (wave_triangles).  Per ray the arithmetic, the nodes
+// visited and the result are exactly the one-ray-per-lane kernel's (trav_step).
 // Source: where a ray's scene comes from -- one scene for the launch (OneScene) or the ray's job
 // (JobScenes, batched rendering).
 struct OneScene {
+    static constexpr bool kShared = true;  // every lane reads the same triangle array
     const U4 *nodes;
     const F4 *tris;
     float root_max;
@@ -87,6 +90,7 @@ struct OneScene {
     __device__ __forceinline__ float root(const Lane &) const { return root_max; }
 };
 struct JobScenes {
+    static constexpr bool kShared = false;
     const TraceScene *job_trace;
     const uint32_t *job;
     struct Lane {
@@ -107,12 +111,141 @@ struct JobScenes {
     __device__ __forceinline__ float root(const Lane &l) const { return l.root_max; }
 };
 
+// Order-preserving map of a float's bits to an unsigned integer, and its inverse.
+__device__ __forceinline__ uint32_t ord_bits(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unord_bits(uint32_t o) { return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o); }
+// Position of the k-th set bit (from 0) of m, for k < popcount(m) and m below 2^31.
+__device__ __forceinline__ uint32_t nth_set_bit(uint32_t m, uint32_t k) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t w = 16; w; w >>= 1) {
+        const uint32_t low = __popc(m & ((1u << w) - 1u));
+        const bool up = k >= low;
+        k = up ? k - low : k;
+        m = up ? m >> w : m;
+        pos = up ? pos + w : pos;
+    }
+    return pos;
+}
+// Orders the wave's LDS accesses on either side (no instruction: a wave's LDS operations complete in
+// issue order; this only keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Triangle half of a step for a whole wave.  After its node visit each lane holds up to 24 triangles
+// (trimask over tri_base); tested one lane at a time the wave would loop as often as its fullest lane
+// (5.6 times per step on incoherent bounce rays with 0.44 x 64 tests to do).  Here the wave's tests are
+// numbered by a prefix sum over the lanes' counts and handed out 64 at a time: test j belongs to the
+// lane whose range holds j (binary search over the prefixes) and is its k-th triangle; the worker reads
+// that lane's ray through cross-lane permutes, runs the same Woop test, and folds a hit into the
+// owner's LDS slot with an atomic minimum of (order-preserving t bits << 32 | face) -- the nearest t,
+// equal t going to the smaller face, as the one-lane rule.  The winner of each round stores its
+// barycentrics; the key's identity picks it since a face is tested at most once per ray.  A hit at t
+// must also lie in [tmin, tmax) (the one-lane rule's first accepted hit needs t < tmax).  Any-hit
+// queries only raise the owner's flag.  Must be called by all 64 lanes together.
+template <bool ANY, int kBlk, class Source>
+__device__ __forceinline__ void wave_triangles(const Source &src, const typename Source::Lane &sl, TravState &ts,
+                                               uint32_t trimask, uint32_t tri_base, uint32_t lane,
+                                               unsigned long long below, unsigned long long *key, float (*bary)[kBlk]) {
+    const uint32_t cnt = __popc(trimask);
+    uint32_t pre = 0, total = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+        const unsigned long long m = __ballot((cnt >> b) & 1u);
+        pre += (uint32_t)__popcll(m & below) << b;
+        total += (uint32_t)__popcll(m) << b;
+    }
+    if (total == 0u) return;
+    const uint32_t wave0 = threadIdx.x & ~63u;
+    unsigned long long key0 = 0ull;
+    if (cnt) {
+        if (!ANY) key0 = ((unsigned long long)ord_bits(ts.res.t) << 32) | (uint32_t)ts.res.face;
+        key[threadIdx.x] = key0;
+    }
+    wave_lds_order();
+    const int kpack = ts.w.kx | (ts.w.ky << 2) | (ts.w.kz << 4);
+    const uint64_t tri_bits = reinterpret_cast<uint64_t>(src.tri_ptr(sl));
+    for (uint32_t j0 = 0; j0 < total; j0 += 64u) {
+        const uint32_t j = j0 + lane;
+        uint32_t owner = 0;
+#pragma unroll
+        for (uint32_t step = 32; step; step >>= 1) {
+            const uint32_t cand = owner + step;  // at most 63
+            if ((uint32_t)__shfl((int)pre, (int)cand) <= j) owner = cand;
+        }
+        const int o = (int)owner;
+        const uint32_t k = j - (uint32_t)__shfl((int)pre, o);
+        const uint32_t omask = (uint32_t)__shfl((int)trimask, o), obase = (uint32_t)__shfl((int)tri_base, o);
+        const int kp = __shfl(kpack, o);
+        WoopRay w;
+        w.kx = kp & 3;
+        w.ky = (kp >> 2) & 3;
+        w.kz = kp >> 4;
+        w.sx = __shfl(ts.w.sx, o);
+        w.sy = __shfl(ts.w.sy, o);
+        w.sz = __shfl(ts.w.sz, o);
+        const float ox = __shfl(ts.ox, o), oy = __shfl(ts.oy, o), oz = __shfl(ts.oz, o);
+        const float tmin = __shfl(ts.tmin, o), tmax = __shfl(ts.tmax, o);
+        const F4 *tris;
+        if constexpr (Source::kShared) {
+            tris = src.tri_ptr(sl);
+        } else {
+            const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)tri_bits, o);
+            const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(tri_bits >> 32), o);
+            tris = reinterpret_cast<const F4 *>(((uint64_t)hi << 32) | lo);
+        }
+        bool hit = false;
+        unsigned long long mine = 0ull;
+        float t = 0.0f, b1 = 0.0f, b2 = 0.0f;
+        if (j < total) {
+            const F4 *tp = tris + 3 * (size_t)(obase + nth_set_bit(omask, k));
+            const F4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+            if (woop_triangle(w, ox, oy, oz, p0, p1, p2, t, b1, b2) && t >= tmin && t < tmax) {
+                hit = true;
+                if (ANY) {
+                    key[wave0 + owner] = 1ull;
+                } else {
+                    mine = ((unsigned long long)ord_bits(t) << 32) | float_to_bits(p0.w);
+                    atomicMin(&key[wave0 + owner], mine);
+                }
+            }
+        }
+        if (!ANY) {
+            wave_lds_order();
+            if (hit && key[wave0 + owner] == mine) {
+                bary[0][wave0 + owner] = b1;
+                bary[1][wave0 + owner] = b2;
+            }
+        }
+        wave_lds_order();
+    }
+    if (cnt) {
+        const unsigned long long got = key[threadIdx.x];
+        if (ANY) {
+            if (got) ts.res.occluded = true;
+        } else if (got != key0) {
+            ts.res.t = unord_bits((uint32_t)(got >> 32));
+            ts.res.face = (int32_t)(uint32_t)got;
+            ts.res.b1 = bary[0][threadIdx.x];
+            ts.res.b2 = bary[1][threadIdx.x];
+        }
+    }
+    wave_lds_order();
+}
+
 template <bool ANY, int kStack, int kBlk, class Source>
 __device__ __forceinline__ void refill_trace(const Source &src, const RayArgs &rays, const HitArgs &hits,
                                              uint8_t *__restrict__ occluded, uint32_t n) {
     __shared__ uint32_t stack_base[kStack][kBlk];
     __shared__ uint32_t stack_bits[kStack][kBlk];
     __shared__ uint32_t next;
+    __shared__ unsigned long long tri_key[kBlk];
+    __shared__ float tri_bary[ANY ? 1 : 2][kBlk];
     constexpr uint32_t kNone = 0xffffffffu;
     const uint32_t chunk = ((n + gridDim.x - 1) / gridDim.x + 63u) & ~63u;
     const uint32_t begin = min(n, blockIdx.x * chunk), end = min(n, begin + chunk);
@@ -164,7 +297,11 @@ __device__ __forceinline__ void refill_trace(const Source &src, const RayArgs &r
         }
         const bool busy = ray != kNone;
         if (!__ballot(busy) && exhausted) break;
-        if (busy && trav_step<ANY, false>(ts, src.node_ptr(sl), src.tri_ptr(sl), st)) {
+        uint32_t trimask = 0, tri_base = 0;
+        bool done = false;
+        if (busy) done = trav_nodes<false>(ts, src.node_ptr(sl), st, trimask, tri_base);
+        wave_triangles<ANY, kBlk>(src, sl, ts, trimask, tri_base, lane, below, tri_key, tri_bary);
+        if (busy && (done || (ANY && ts.res.occluded))) {
             if (ANY) {
                 occluded[ray] = ts.res.occluded ? 1 : 0;
             } else {

@@ -163,10 +163,14 @@ VT_HD void trav_init(TravState &st, const TraceRay &r, float root_max) {
     st.node = 0;
 }
 
-// Visits st.node: tests its 8 child boxes, tests the triangles of the leaf children hit, and moves to
-// the next node near to far.  Returns true when the ray is finished (result in st.res).
-template <bool ANY, bool STATS, class Stack>
-VT_HD bool trav_step(TravState &st, const U4 *__restrict__ nodes, const F4 *__restrict__ tris, Stack &stack) {
+// Node half of a step: tests st.node's 8 child boxes, pushes the current group if inner children were
+// hit, and moves st.node to the next node near to far.  Returns the triangles of the leaf children hit
+// (trimask: bits relative to tri_base, at most 24) and true when no node is left to visit (the
+// triangles returned must still be tested).  The next node does not depend on those triangles, so
+// testing them after the move gives the same result as testing them first.
+template <bool STATS, class Stack>
+VT_HD bool trav_nodes(TravState &st, const U4 *__restrict__ nodes, Stack &stack, uint32_t &trimask,
+                      uint32_t &tri_base) {
     const U4 *np = nodes + 5 * (size_t)st.node;
     const U4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
     if (STATS) ++st.res.nodes;
@@ -193,7 +197,9 @@ VT_HD bool trav_step(TravState &st, const U4 *__restrict__ nodes, const F4 *__re
     const uint32_t nxa = nx ? w3.z : w2.x, nxb = nx ? w3.w : w2.y, fxa = nx ? w2.x : w3.z, fxb = nx ? w2.y : w3.w;
     const uint32_t nya = ny ? w4.x : w2.z, nyb = ny ? w4.y : w2.w, fya = ny ? w2.z : w4.x, fyb = ny ? w2.w : w4.y;
     const uint32_t nza = nz ? w4.z : w3.x, nzb = nz ? w4.w : w3.y, fza = nz ? w3.x : w4.z, fzb = nz ? w3.y : w4.w;
-    uint32_t hitbits = 0, trimask = 0;
+    uint32_t hitbits = 0;
+    trimask = 0;
+    tri_base = w1.y;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         const uint32_t meta = byte_of(w1.z, w1.w, s);
@@ -214,33 +220,14 @@ VT_HD bool trav_step(TravState &st, const U4 *__restrict__ nodes, const F4 *__re
         if (st.cur_bits & 0xffu) {
             // The builder bounds the depth so a full stack never occurs; if it did, stop this ray
             // (a wrong result, never an out-of-bounds access).
-            if (stack.full()) return true;
+            if (stack.full()) {
+                trimask = 0;
+                return true;
+            }
             stack.push(st.cur_base, st.cur_bits);
         }
         st.cur_base = w1.x;
         st.cur_bits = (imask << 8) | hitbits;
-    }
-    const uint32_t tri_base = w1.y;
-    while (trimask) {
-        const uint32_t b = __builtin_ctz(trimask);
-        trimask &= trimask - 1u;
-        const F4 *tp = tris + 3 * (size_t)(tri_base + b);
-        const F4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-        if (STATS) ++st.res.tris;
-        float t, b1, b2;
-        if (!woop_triangle(st.w, st.ox, st.oy, st.oz, p0, p1, p2, t, b1, b2)) continue;
-        const int32_t face = (int32_t)float_to_bits(p0.w);
-        if (ANY) {
-            if (t >= st.tmin && t < st.tmax) {
-                st.res.occluded = true;
-                return true;
-            }
-        } else if (t >= st.tmin && (t < st.res.t || (t == st.res.t && st.res.face >= 0 && face < st.res.face))) {
-            st.res.t = t;
-            st.res.b1 = b1;
-            st.res.b2 = b2;
-            st.res.face = face;
-        }
     }
     if ((st.cur_bits & 0xffu) == 0u) {
         if (stack.empty()) return true;
@@ -251,6 +238,50 @@ VT_HD bool trav_step(TravState &st, const U4 *__restrict__ nodes, const F4 *__re
     const uint32_t slot = v ^ st.oct;
     st.node = st.cur_base + __builtin_popcount((st.cur_bits >> 8) & ((1u << slot) - 1u));
     return false;
+}
+
+// A hit of the ray at t on face: the closest-hit rule (nearest t in [tmin, current t), equal t going
+// to the smaller face index).  For the any-hit query any t in [tmin, tmax) ends the ray.
+template <bool ANY>
+VT_HD bool trav_accept(const TravState &st, float t, int32_t face) {
+    if (ANY) return t >= st.tmin && t < st.tmax;
+    return t >= st.tmin && (t < st.res.t || (t == st.res.t && st.res.face >= 0 && face < st.res.face));
+}
+
+// Triangle half of a step, one lane alone: tests the triangles trav_nodes returned.  Returns true
+// when an any-hit query found a hit.
+template <bool ANY, bool STATS>
+VT_HD bool trav_tris(TravState &st, const F4 *__restrict__ tris, uint32_t trimask, uint32_t tri_base) {
+    while (trimask) {
+        const uint32_t b = __builtin_ctz(trimask);
+        trimask &= trimask - 1u;
+        const F4 *tp = tris + 3 * (size_t)(tri_base + b);
+        const F4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+        if (STATS) ++st.res.tris;
+        float t, b1, b2;
+        if (!woop_triangle(st.w, st.ox, st.oy, st.oz, p0, p1, p2, t, b1, b2)) continue;
+        const int32_t face = (int32_t)float_to_bits(p0.w);
+        if (!trav_accept<ANY>(st, t, face)) continue;
+        if (ANY) {
+            st.res.occluded = true;
+            return true;
+        }
+        st.res.t = t;
+        st.res.b1 = b1;
+        st.res.b2 = b2;
+        st.res.face = face;
+    }
+    return false;
+}
+
+// One step of one ray: a node visit and the triangles of the leaves it hits.  Returns true when the
+// ray is finished (result in st.res).
+template <bool ANY, bool STATS, class Stack>
+VT_HD bool trav_step(TravState &st, const U4 *__restrict__ nodes, const F4 *__restrict__ tris, Stack &stack) {
+    uint32_t trimask, tri_base;
+    const bool done = trav_nodes<STATS>(st, nodes, stack, trimask, tri_base);
+    if (trav_tris<ANY, STATS>(st, tris, trimask, tri_base)) return true;
+    return done;
 }
 
 // Traverses one ray.  nodes: Node8 records as 5 U4 each; tris: TriRecord as 3 F4 each.
