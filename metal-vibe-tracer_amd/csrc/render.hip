@@ -449,18 +449,14 @@ __device__ __forceinline__ float lt_leaf_sum(const RenderScene &sc, uint32_t chi
     return s;
 }
 
-__device__ __forceinline__ uint32_t lt_sample(const RenderScene &sc, V3 p, float u, float &pmf) {
-    pmf = 1.0f;
-    uint32_t index = 0;
-    // The chosen child's links (first child, leaf count) are carried down, so a level costs one round
-    // of two independent node loads (the children) instead of two dependent rounds.
-    uint32_t child, count;
-    {
-        const LtDec root = lt_dec(sc, 0);
-        child = root.child;
-        count = root.links >> 27;
-    }
-    for (uint32_t depth = 0; depth <= 64u; ++depth) {
+// The light-tree walk (LightTree sampling, main.swift's imported light tree) from node `index` at
+// `depth` with first child `child` and leaf count `count`, u the remaining uniform and pmf the
+// probability of the path so far.  The chosen child's links (first child, leaf count) are carried
+// down, so a level costs one round of two independent node loads (the children) instead of two
+// dependent rounds.
+__device__ __forceinline__ uint32_t lt_walk(const RenderScene &sc, V3 p, float u, float &pmf, uint32_t index,
+                                            uint32_t child, uint32_t count, uint32_t depth) {
+    for (; depth <= 64u; ++depth) {
         if (count != 0u) {
             float sum = lt_leaf_sum(sc, child, count), target = u * sum, cum = 0.0f, chosen = 0.0f;
             uint32_t position = 0xffffffffu;
@@ -497,6 +493,83 @@ __device__ __forceinline__ uint32_t lt_sample(const RenderScene &sc, V3 p, float
     pmf = 0.0f;
     return 0xffffffffu;
 }
+__device__ __forceinline__ uint32_t lt_sample(const RenderScene &sc, V3 p, float u, float &pmf) {
+    pmf = 1.0f;
+    const float4 root = sc.tree_dec[4];
+    return lt_walk(sc, p, u, pmf, 0u, __float_as_uint(root.z), __float_as_uint(root.w) >> 27, 0u);
+}
+
+// The light tree's top kLtTopLevels levels for one shading point.  Every candidate of a pixel walks the
+// tree from the root at the same point, so the importances of the children of the top nodes are the
+// same for all of them: they are computed once per pixel (2 x 15 evaluations instead of 2 per level per
+// candidate, 256 for 32 candidates) and kept in LDS, [node][thread].  The block shares the top nodes'
+// indices and links (heap order: node h has children 2h + 1 and 2h + 2; kNoNode below a leaf).  The
+// walk through the cached levels performs the same operations on the same values as lt_walk, so the
+// samples are identical.
+constexpr int kLtTopLevels = 4;
+constexpr int kLtTopInner = (1 << kLtTopLevels) - 1;       // heap nodes whose children are cached
+constexpr int kLtTopAll = (1 << (kLtTopLevels + 1)) - 1;   // heap nodes with their links in the table
+constexpr uint32_t kNoNode = 0xffffffffu;
+struct LtTop {
+    uint32_t index[kLtTopAll], child[kLtTopAll], count[kLtTopAll];
+};
+// Heap node h of the tree's top (one thread per h < kLtTopAll): follows h's path from the root.
+__device__ __forceinline__ void lt_top_fill(const RenderScene &sc, LtTop &top, uint32_t h) {
+    uint32_t path = 0, depth = 0;
+    for (uint32_t g = h; g != 0u; g = (g - 1u) >> 1) {
+        path = (path << 1) | ((g & 1u) ? 0u : 1u);  // 0: left child (odd heap index), 1: right
+        ++depth;
+    }
+    uint32_t index = 0;
+    float4 e = sc.tree_dec[4];
+    for (uint32_t d = 0; d < depth; ++d, path >>= 1) {
+        if ((__float_as_uint(e.w) >> 27) != 0u) {  // a leaf has no children
+            index = kNoNode;
+            break;
+        }
+        index = (path & 1u) ? __float_as_uint(e.z) : index + 1u;
+        e = sc.tree_dec[5 * (size_t)index + 4];
+    }
+    top.index[h] = index;
+    top.child[h] = index == kNoNode ? 0u : __float_as_uint(e.z);
+    top.count[h] = index == kNoNode ? 0u : __float_as_uint(e.w) >> 27;
+}
+template <int kBlk>
+__device__ __forceinline__ void lt_top_importances(const RenderScene &sc, const LtTop &top, V3 p,
+                                                   float (*tl)[kBlk], float (*tr)[kBlk]) {
+    for (int h = 0; h < kLtTopInner; ++h) {
+        if (top.index[h] == kNoNode || top.count[h] != 0u) continue;
+        tl[h][threadIdx.x] = lt_importance(lt_dec(sc, top.index[h] + 1u), p);
+        tr[h][threadIdx.x] = lt_importance(lt_dec(sc, top.child[h]), p);
+    }
+}
+template <int kBlk>
+__device__ __forceinline__ uint32_t lt_sample_top(const RenderScene &sc, V3 p, float u, float &pmf, const LtTop &top,
+                                                  const float (*tl)[kBlk], const float (*tr)[kBlk]) {
+    pmf = 1.0f;
+    uint32_t h = 0, child = top.child[0], count = top.count[0], depth = 0;
+    for (; depth < (uint32_t)kLtTopLevels; ++depth) {
+        if (count != 0u) break;
+        const float l = tl[h][threadIdx.x], r = tr[h][threadIdx.x];
+        if (!(l + r > 0.0f)) {
+            pmf = 0.0f;
+            return 0xffffffffu;
+        }
+        const float pl = l / (l + r);
+        if (u < pl) {
+            u = fminf(u / pl, 0x1.fffffep-1f);
+            pmf *= pl;
+            h = 2u * h + 1u;
+        } else {
+            u = fminf((u - pl) / (1.0f - pl), 0x1.fffffep-1f);
+            pmf *= r / (l + r);
+            h = 2u * h + 2u;
+        }
+        child = top.child[h];
+        count = top.count[h];
+    }
+    return lt_walk(sc, p, u, pmf, top.index[h], child, count, depth);
+}
 __device__ __forceinline__ float lt_pmf(const RenderScene &sc, V3 p, uint32_t position) {
     uint32_t index = sc.tree_leaves[3 * (size_t)position + 2];
     LtDec node = lt_dec(sc, index);
@@ -521,7 +594,9 @@ struct LightSample {
     float dist, pdf, geometry;
     uint32_t slot, light;  // light: face + 2 (LightSample.isDirectional), 0 = none
 };
-__device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, V3 p, uint32_t &seed) {
+// walk(u, pmf): the light-tree walk for uniform u (lt_sample, or lt_sample_top with a pixel's cache).
+template <class Walk>
+__device__ __forceinline__ LightSample sample_emitter_via(const RenderScene &sc, V3 p, uint32_t &seed, const Walk &walk) {
     LightSample ls;
     ls.pdf = 0.0f;
     ls.slot = 0;
@@ -538,7 +613,7 @@ __device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, V3 
     uint32_t face;
     float tree_pmf = 0.0f;
     if (sc.tree_node_count) {
-        uint32_t position = lt_sample(sc, p, value, tree_pmf);
+        uint32_t position = walk(value, tree_pmf);
         if (position == 0xffffffffu) return ls;
         face = sc.tree_leaves[3 * (size_t)position];
     } else {
@@ -574,6 +649,9 @@ __device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, V3 
     }
     ls.pdf = geometry > 1e-12f ? area_pdf / geometry : 0.0f;
     return ls;
+}
+__device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, V3 p, uint32_t &seed) {
+    return sample_emitter_via(sc, p, seed, [&](float u, float &pmf) { return lt_sample(sc, p, u, pmf); });
 }
 __device__ __forceinline__ float emitter_pdf(const RenderScene &sc, V3 p, V3 position, uint32_t face, float weight) {
     if (sc.emitter_count == 0) return 0.0f;
@@ -786,6 +864,11 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameArgs f, PathQueue q,
 // (restir_di_merge :4692-4704) and the encode (restir_di_encode :4708-4716).
 __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, FrameArgs f, PathQueue q, HitArgs hits,
                                                              RestirArgs ra) {
+    __shared__ LtTop top;
+    __shared__ float top_l[kLtTopInner][256], top_r[kLtTopInner][256];
+    const bool cached = sc.tree_node_count != 0u;
+    if (cached && threadIdx.x < (unsigned)kLtTopAll) lt_top_fill(sc, top, threadIdx.x);
+    __syncthreads();
     const uint32_t pix = blockIdx.x * 256 + threadIdx.x;
     if (pix >= f.npix) return;
     float4 gpos = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
@@ -807,8 +890,13 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
             const S4 rho_pi = scale(reflectance(sp, lanes, m.lagrange), 1.0f / kPi);
             DiSample sel = di_empty();
             float wsum = 0.0f, M = 0.0f;
+            if (cached && ra.candidates != 0u) lt_top_importances<256>(sc, top, s.p, top_l, top_r);
+            const V3 sp_p = s.p;
             for (uint32_t i = 0; i < ra.candidates; ++i) {
-                const LightSample ls = sample_emitter(sc, s.p, seed);
+                const LightSample ls =
+                    cached ? sample_emitter_via(sc, sp_p, seed,
+                                                [&](float u, float &pmf) { return lt_sample_top<256>(sc, sp_p, u, pmf, top, top_l, top_r); })
+                           : sample_emitter(sc, sp_p, seed);
                 DiSample c;
                 c.position = ls.position;
                 c.emission = ls.emission;
