@@ -256,7 +256,7 @@ typedef struct vt_batch_job {
 typedef struct vt_batch_tuning {
     uint32_t capacity;     /* paths per wavefront chunk (0 or more than a sample's pixels: all of them) */
     uint32_t grid_waves;   /* persistent grids: waves per SIMD, 1..32 */
-    uint32_t trace_block;  /* traversal work-group size: 64, 128 or 256 */
+    uint32_t trace_block;  /* traversal work-group size: 64, 128, 256 or 512 */
     uint32_t leaf_size;    /* BVH leaf size of the batch's scenes, 1..3 (0: keep); changing it rebuilds them */
 } vt_batch_tuning;
 vt_status vt_batch_create(vt_context *ctx, const vt_batch_job *jobs, uint32_t count, uint32_t max_depth,

@@ -207,8 +207,8 @@ void validate_tuning(const vt_batch *b, vt_batch_tuning &t) {
     if (t.capacity == 0 || t.capacity > b->total) t.capacity = b->total;
     t.capacity = (t.capacity + 63u) & ~63u;
     if (t.grid_waves == 0 || t.grid_waves > 32) throw std::invalid_argument("grid_waves must be 1..32");
-    if (t.trace_block != 64 && t.trace_block != 128 && t.trace_block != 256)
-        throw std::invalid_argument("trace_block must be 64, 128 or 256");
+    if (t.trace_block != 64 && t.trace_block != 128 && t.trace_block != 256 && t.trace_block != 512)
+        throw std::invalid_argument("trace_block must be 64, 128, 256 or 512");
 }
 
 // Milliseconds per sample: one warm-up replay, then `reps` timed replays.
@@ -408,7 +408,7 @@ vt_status vt_batch_autotune(vt_batch *b, uint32_t flags, vt_batch_tuning *best, 
             t.grid_waves = w;
             trial(t);
         }
-        for (uint32_t blk : {64u, 128u, 256u}) {
+        for (uint32_t blk : {64u, 128u, 256u, 512u}) {
             vt_batch_tuning t = cur;
             t.trace_block = blk;
             trial(t);

@@ -406,7 +406,7 @@ hipError_t launch_trace_batch(const TraceScene *job_trace, const uint32_t *job, 
                               uint32_t block, uint32_t grid, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     const int need = max_depth > 1 ? (int)max_depth - 1 : 1;
-    if (need > 24 || (block != 64 && block != 128 && block != 256) || grid == 0) return hipErrorInvalidValue;
+    if (need > 24 || (block != 64 && block != 128 && block != 256 && block != 512) || grid == 0) return hipErrorInvalidValue;
 #define VT_BATCH(S, B)                                                                                                 \
     do {                                                                                                               \
         if (any) hipLaunchKernelGGL((batch_trace_kernel<true, S, B>), dim3(grid), dim3(B), 0, stream, job_trace, job,   \
@@ -418,7 +418,8 @@ hipError_t launch_trace_batch(const TraceScene *job_trace, const uint32_t *job, 
     do {                                                                                                               \
         if (block == 64) VT_BATCH(S, 64);                                                                              \
         else if (block == 128) VT_BATCH(S, 128);                                                                       \
-        else VT_BATCH(S, 256);                                                                                         \
+        else if (block == 256) VT_BATCH(S, 256);                                                                       \
+        else VT_BATCH(S, 512);                                                                                         \
     } while (0)
     if (need <= 8) VT_BATCH_STACK(8);
     else if (need <= 12) VT_BATCH_STACK(12);

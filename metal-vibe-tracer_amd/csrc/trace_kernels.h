@@ -29,7 +29,7 @@ hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitA
                         const uint32_t *count = nullptr);
 
 // Batched rendering: ray i traces the scene job_trace[job[i]], for i < min(n, *count), on a
-// persistent grid of `grid` blocks of `block` (64, 128 or 256) lanes; max_depth is the deepest tree.
+// persistent grid of `grid` blocks of `block` (64, 128, 256 or 512) lanes; max_depth is the deepest tree.
 hipError_t launch_trace_batch(const TraceScene *job_trace, const uint32_t *job, const RayArgs &rays, const HitArgs &hits,
                               uint8_t *occluded, uint32_t n, bool any, const uint32_t *count, uint32_t max_depth,
                               uint32_t block, uint32_t grid, hipStream_t stream);

@@ -77,7 +77,8 @@ def test_tuning_does_not_change_the_image(gpu_ctx):
     b.render(2)
     base = [b.readback(k) for k in range(len(jobs))]
     for tuning in (dict(capacity=4096, grid_waves=1, trace_block=64, leaf_size=0),
-                   dict(capacity=0, grid_waves=8, trace_block=256, leaf_size=1)):
+                   dict(capacity=0, grid_waves=8, trace_block=256, leaf_size=1),
+                   dict(capacity=0, grid_waves=4, trace_block=512, leaf_size=0)):
         b.tuning = tuning
         b.reset()
         b.render(2)
@@ -96,7 +97,7 @@ def test_autotune_and_errors(gpu_ctx):
     b = vt.Batch(gpu_ctx, jobs, max_depth=DEPTH)
     best, ms = b.autotune(leaf_size=True)
     print(f"autotune: {best}, {ms:.3f} ms per sample", flush=True)
-    assert ms > 0 and best["trace_block"] in (64, 128, 256) and 1 <= best["leaf_size"] <= 3
+    assert ms > 0 and best["trace_block"] in (64, 128, 256, 512) and 1 <= best["leaf_size"] <= 3
     assert b.tuning == best and b.frames == 0
     b.render(2)
     ref = _single(gpu_ctx, jobs[1], 2)
