@@ -94,6 +94,31 @@ def test_watertight_edge_rays_any(gpu_ctx):
     assert np.all(_closest(scene, rays)["face"] >= 0)
 
 
+@pytest.mark.parametrize("leaf", [1, 3])
+def test_exact_ties_and_negative_tmin(gpu_ctx, leaf):
+    """Every triangle twice (same vertices, two face indices, the copies interleaved at random): each
+    hit is an exact t-tie that the smaller face index must win wherever the copies sit in the tree.
+    The lanes of a wave test a step's triangles together and fold hits by a minimum over (t, face),
+    so this pins that rule. Rays with tmin < 0 accept hits behind the origin (negative t)."""
+    v, f = scenes.scattered_triangles(1500, seed=8)
+    perm = np.random.default_rng(3).permutation(2 * len(f))
+    f2 = np.concatenate([f, f])[perm]
+    osc = oracle.Scene(v, f2, with_bvh=False)
+    scene = gpu_ctx.set_mesh(vt.Mesh(v, f2), leaf_size=leaf)
+    for tmin in (0.0, -0.7, -3.0):
+        rays = scenes.random_rays(6000, 11, tmin=tmin)
+        got, ref = _closest(scene, rays), osc.closest(rays, mode=1)
+        print(f"leaf {leaf} tmin {tmin}: hits {np.mean(ref['face'] >= 0):.3f}, negative t {np.mean(ref['t'] < 0):.3f}",
+              flush=True)
+        _assert_exact(got, ref, f"ties tmin={tmin}")
+        seg = rays.copy()
+        seg[7] = np.random.default_rng(4).uniform(-0.5, 1.0, seg.shape[1]).astype(np.float32)
+        assert np.array_equal(_any(scene, seg), osc.any(seg, mode=1)), f"any tmin={tmin}"
+    if tmin < 0:
+        assert np.mean(ref["t"] < 0) > 0.05
+    scene.close()
+
+
 def test_empty_scene_and_ragged_sizes(gpu_ctx):
     import torch
     empty = gpu_ctx.set_mesh(vt.Mesh(np.zeros((0, 8), np.float32), np.zeros((0, 4), np.uint32)))
