@@ -49,6 +49,8 @@ typedef struct vt_build_options {
     uint32_t leaf_size; /* triangles per leaf, 1..3 (0: 3) */
     uint32_t bins;      /* SAH bins per axis, 4..64 (0: 16, Sources/Acceleration.swift:109) */
     uint32_t max_depth; /* tree depth limit before a median rebuild, <= 24 (0: 24) */
+    uint32_t collapse;  /* 0: SAH-optimal collapse of a binary SAH tree into 8-wide nodes; 1: greedy
+                           widening, splitting the largest child (Sources/Acceleration.swift:247-284) */
 } vt_build_options;
 
 /* Replaces MaterialLibrary.setMesh(_:) for an indexed mesh (Sources/StudioModel.swift:1480-1504,

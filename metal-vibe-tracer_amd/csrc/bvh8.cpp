@@ -1,7 +1,8 @@
 // bvh8.cpp -- host builder of the 8-wide BVH (layout in bvh8.h).
 //
-// Top-down binned SAH (Wald 2007), widened the way the reference widens its 4-wide tree
-// (Sources/Acceleration.swift:247-284): a node starts with one child holding all its items and
+// By default a binary binned-SAH tree (Wald 2007) is collapsed into 8-wide nodes by the SAH-optimal
+// dynamic programme below.  The other mode widens top-down the way the reference widens its 4-wide
+// tree (Sources/Acceleration.swift:247-284): a node starts with one child holding all its items and
 // repeatedly splits the child of largest surface area with a binary binned-SAH split until it
 // has eight children or none is larger than a leaf.  Subtrees below a fraction of the input are
 // built concurrently into private node lists and stitched in a fixed order, so the tree does
@@ -298,6 +299,221 @@ float exp_scale(uint8_t e) {
     return s;
 }
 
+// SAH-optimal collapse of a binary tree into the 8-wide tree (Ylitie, Karras and Laine 2017, "Efficient
+// Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs", section 3.1).  The binary tree is
+// built top-down with the same binned SAH split down to single triangles.  Bottom-up, C(n, k) is the
+// least SAH cost of representing the subtree of n by at most k slots of a wide node: n itself (a
+// leaf of at most leaf_size triangles, cost A(n) c_tri count, or a wide node, cost A(n) c_node plus
+// its children's best use of 8 slots), or its two children sharing the k slots.  Greedy widening
+// leaves many two-child nodes at the bottom of the tree (46% of the nodes of the 1.3M-triangle
+// icosphere field); the optimal collapse fills them.
+struct BinTree {
+    std::vector<Box> box;
+    std::vector<int32_t> start, count, left, right;  // left < 0: a leaf (a single triangle)
+    int32_t add(const Range &r) {
+        box.push_back(r.box);
+        start.push_back(r.start);
+        count.push_back(r.count());
+        left.push_back(-1);
+        right.push_back(-1);
+        return (int32_t)box.size() - 1;
+    }
+};
+
+// A node visit costs the traversal kernel about 300 wave instructions whatever the number of children;
+// a triangle test about 3 per lane when the wave tests them together.  Replayed on the CPU, the
+// icosphere field's bounce rays visit 13.04 nodes per ray with greedy widening, 12.79 with a triangle
+// cost of 0.3, 12.64 with 0.1 and 13.27 with 0.6 (the 10M heightfield: 17.53 -> 16.84 at 0.1).
+constexpr float kCostNode = 1.0f, kCostTri = 0.1f;
+
+// Builds the binary tree of `root` into t (root at the first new index); ranges smaller than `defer`
+// go to `deferred` as (range, node index) and are left as leaves to be replaced.
+void build_binary(Builder &b, const Range &root, BinTree &t, int32_t defer,
+                  std::vector<std::pair<Range, int32_t>> *deferred) {
+    std::vector<std::pair<Range, int32_t>> work{{root, t.add(root)}};
+    while (!work.empty()) {
+        auto [r, i] = work.back();
+        work.pop_back();
+        if (r.count() <= 1) continue;
+        if (deferred && r.count() < defer) {
+            deferred->push_back({r, i});
+            continue;
+        }
+        Range a, c;
+        b.split(r, a, c);
+        const int32_t ia = t.add(a), ic = t.add(c);
+        t.left[i] = ia;
+        t.right[i] = ic;
+        work.push_back({c, ic});
+        work.push_back({a, ia});
+    }
+}
+
+struct Collapse {
+    const BinTree &t;
+    int leaf;
+    std::vector<float> cost;     // 8 per node: C(n, 1..8)
+    std::vector<uint8_t> dec;    // 8 per node: [0] 1 = leaf; [k-1], k >= 2: 0 = as k-1 slots, else slots given to left
+    std::vector<uint8_t> dint;   // slots given to left when n is a wide node
+
+    Collapse(const BinTree &tree, int leaf_size)
+        : t(tree), leaf(leaf_size), cost(8 * tree.box.size()), dec(8 * tree.box.size()), dint(tree.box.size()) {}
+
+    // Children have larger indices than their parent, so a descending sweep over [lo, hi) sees them first.
+    void solve(int32_t lo, int32_t hi) {
+        for (int32_t n = hi - 1; n >= lo; --n) {
+            const float A = t.box[n].half_area();
+            float *C = &cost[8 * (size_t)n];
+            uint8_t *d = &dec[8 * (size_t)n];
+            const float c_leaf = t.count[n] <= leaf ? A * kCostTri * (float)t.count[n] : FLT_MAX;
+            if (t.left[n] < 0) {
+                for (int k = 0; k < 8; ++k) { C[k] = c_leaf; d[k] = 0; }
+                d[0] = 1;
+                continue;
+            }
+            const float *L = &cost[8 * (size_t)t.left[n]], *R = &cost[8 * (size_t)t.right[n]];
+            float D[9];
+            uint8_t J[9];
+            for (int k = 2; k <= 8; ++k) {
+                D[k] = FLT_MAX;
+                J[k] = 1;
+                for (int j = 1; j < k; ++j) {
+                    const float v = L[j - 1] + R[k - j - 1];
+                    if (v < D[k]) { D[k] = v; J[k] = (uint8_t)j; }
+                }
+            }
+            const float c_int = A * kCostNode + D[8];
+            dint[n] = J[8];
+            d[0] = c_leaf <= c_int ? 1 : 0;
+            C[0] = std::min(c_leaf, c_int);
+            for (int k = 2; k <= 8; ++k) {
+                if (D[k] < C[k - 2]) { C[k - 1] = D[k]; d[k - 1] = J[k]; }
+                else { C[k - 1] = C[k - 2]; d[k - 1] = 0; }
+            }
+        }
+    }
+
+    // The slots (binary nodes) representing the subtree of n in at most k slots.
+    void slots(int32_t n, int k, std::vector<int32_t> &out) const {
+        while (k > 1 && t.left[n] >= 0 && dec[8 * (size_t)n + k - 1] == 0) --k;
+        if (k == 1 || t.left[n] < 0) {
+            out.push_back(n);
+            return;
+        }
+        const int j = dec[8 * (size_t)n + k - 1];
+        slots(t.left[n], j, out);
+        slots(t.right[n], k - j, out);
+    }
+
+    // Emits the wide node of binary node n (at index `at` of nodes); returns the depth below it.
+    int emit(int32_t n, int32_t at, std::vector<WideNode> &nodes) const {
+        std::vector<int32_t> sl;
+        if (t.left[n] < 0) {
+            sl.push_back(n);
+        } else {
+            slots(t.left[n], dint[n], sl);
+            slots(t.right[n], 8 - dint[n], sl);
+        }
+        WideNode w;
+        w.nchild = (int)sl.size();
+        std::vector<std::pair<int32_t, int>> inner;
+        for (int k = 0; k < w.nchild; ++k) {
+            const int32_t m = sl[k];
+            w.box[k] = t.box[m];
+            if (dec[8 * (size_t)m] == 1) {
+                w.child[k] = -(t.start[m] + 1);
+                w.count[k] = t.count[m];
+            } else {
+                w.child[k] = 0;
+                w.count[k] = 0;
+                inner.push_back({m, k});
+            }
+        }
+        nodes[at] = w;
+        int depth = 1;
+        for (auto &[m, k] : inner) {
+            const int32_t idx = (int32_t)nodes.size();
+            nodes.emplace_back();
+            nodes[at].child[k] = idx;
+            depth = std::max(depth, 1 + emit(m, idx, nodes));
+        }
+        return depth;
+    }
+};
+
+int build_wide_optimal(const Prims &P, int32_t n, int32_t *order, const BuildOptions &opt, std::vector<WideNode> &nodes) {
+    for (int32_t i = 0; i < n; ++i) order[i] = i;
+    Builder top(P, order, opt, false);
+    const Range root = top.measure(0, n);
+    BinTree t;
+    t.box.reserve(2 * (size_t)n);
+    std::vector<std::pair<Range, int32_t>> deferred;
+    build_binary(top, root, t, n >= 65536 ? n / 64 : 0, n >= 65536 ? &deferred : nullptr);
+    const int32_t top_size = (int32_t)t.box.size();
+    // Deferred subtrees are built concurrently into private trees and appended in a fixed order.
+    std::vector<BinTree> parts(deferred.size());
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+        for (;;) {
+            const size_t k = next.fetch_add(1);
+            if (k >= deferred.size()) return;
+            Builder b(P, order, opt, false);
+            build_binary(b, deferred[k].first, parts[k], 0, nullptr);
+        }
+    };
+    {
+        const int T = std::min<int>(thread_count(), std::max<int>(1, (int)deferred.size()));
+        std::vector<std::thread> pool;
+        for (int i = 1; i < T; ++i) pool.emplace_back(worker);
+        worker();
+        for (auto &th : pool) th.join();
+    }
+    std::vector<int32_t> offset(deferred.size());
+    for (size_t k = 0; k < deferred.size(); ++k) {
+        // The part's root replaces the placeholder; its other nodes are appended.
+        const BinTree &q = parts[k];
+        const int32_t base = (int32_t)t.box.size() - 1;  // part index j > 0 lands at base + j
+        offset[k] = base + 1;
+        const int32_t hole = deferred[k].second;
+        auto map = [&](int32_t j) { return j < 0 ? j : (j == 0 ? hole : base + j); };
+        t.left[hole] = map(q.left[0]);
+        t.right[hole] = map(q.right[0]);
+        for (size_t j = 1; j < q.box.size(); ++j) {
+            t.box.push_back(q.box[j]);
+            t.start.push_back(q.start[j]);
+            t.count.push_back(q.count[j]);
+            t.left.push_back(map(q.left[j]));
+            t.right.push_back(map(q.right[j]));
+        }
+    }
+    parts.clear();
+    Collapse c(t, opt.leaf_size);
+    // Parts occupy [offset[k], offset[k + 1]) after the top; solve them concurrently, then the top
+    // (whose placeholders' children now lie in the parts).
+    {
+        std::vector<std::pair<int32_t, int32_t>> ranges;
+        for (size_t k = 0; k < deferred.size(); ++k)
+            ranges.push_back({offset[k], k + 1 < deferred.size() ? offset[k + 1] : (int32_t)t.box.size()});
+        std::atomic<size_t> nx{0};
+        auto solver = [&]() {
+            for (;;) {
+                const size_t k = nx.fetch_add(1);
+                if (k >= ranges.size()) return;
+                c.solve(ranges[k].first, ranges[k].second);
+            }
+        };
+        const int T = std::min<int>(thread_count(), std::max<int>(1, (int)ranges.size()));
+        std::vector<std::thread> pool;
+        for (int i = 1; i < T; ++i) pool.emplace_back(solver);
+        solver();
+        for (auto &th : pool) th.join();
+        c.solve(0, top_size);
+    }
+    nodes.clear();
+    nodes.emplace_back();
+    return c.emit(0, 0, nodes);
+}
+
 // Octant slot assignment: slot s gets the child whose centre projects lowest on the sign
 // vector of octant s (direction component negative where bit a of s is set).
 void assign_slots(const WideNode &w, int slot_of[8]) {
@@ -440,6 +656,7 @@ Bvh8 build_bvh8(const float *vertices, uint32_t vertex_count, const uint32_t *fa
     if (opt.leaf_size < 1 || opt.leaf_size > 3) throw std::invalid_argument("leaf_size must be 1..3");
     if (opt.bins < 4 || opt.bins > 64) throw std::invalid_argument("bins must be 4..64");
     if (opt.max_depth < 2 || opt.max_depth > 24) throw std::invalid_argument("max_depth must be 2..24");
+    if (opt.collapse < 0 || opt.collapse > 1) throw std::invalid_argument("collapse must be 0 or 1");
     Bvh8 out;
     if (n == 0) return out;
     for (uint32_t f = 0; f < n; ++f)
@@ -479,7 +696,8 @@ Bvh8 build_bvh8(const float *vertices, uint32_t vertex_count, const uint32_t *fa
 
     std::vector<int32_t> order(n);
     std::vector<WideNode> wide;
-    int depth = build_wide(P, (int32_t)n, order.data(), opt, false, wide);
+    int depth = opt.collapse == 0 ? build_wide_optimal(P, (int32_t)n, order.data(), opt, wide) : opt.max_depth + 1;
+    if (depth > opt.max_depth) depth = build_wide(P, (int32_t)n, order.data(), opt, false, wide);
     if (depth > opt.max_depth) {
         wide.clear();
         depth = build_wide(P, (int32_t)n, order.data(), opt, true, wide);

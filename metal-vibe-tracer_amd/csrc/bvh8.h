@@ -44,6 +44,7 @@ struct BuildOptions {
     int leaf_size = 3;
     int bins = 16;
     int max_depth = 24;
+    int collapse = 0;  // 0: SAH-optimal collapse of a binary tree; 1: greedy widening
 };
 
 struct Bvh8 {

@@ -171,6 +171,7 @@ static vt::BuildOptions build_options(const vt_build_options *opts) {
         if (opts->leaf_size) o.leaf_size = (int)opts->leaf_size;
         if (opts->bins) o.bins = (int)opts->bins;
         if (opts->max_depth) o.max_depth = (int)opts->max_depth;
+        o.collapse = (int)opts->collapse;
     }
     return o;
 }

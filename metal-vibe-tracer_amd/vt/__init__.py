@@ -48,7 +48,8 @@ class _ContextDesc(ctypes.Structure):
 
 
 class _BuildOptions(ctypes.Structure):
-    _fields_ = [("leaf_size", ctypes.c_uint32), ("bins", ctypes.c_uint32), ("max_depth", ctypes.c_uint32)]
+    _fields_ = [("leaf_size", ctypes.c_uint32), ("bins", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
+                ("collapse", ctypes.c_uint32)]
 
 
 class _SceneInfo(ctypes.Structure):
@@ -333,16 +334,16 @@ TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("face", "<u4"), ("v1", "<f4", 3), ("pad
 assert NODE_DTYPE.itemsize == 80 and TRI_DTYPE.itemsize == 48
 
 
-def _opts(leaf_size=0, bins=0, max_depth=0):
-    return _BuildOptions(leaf_size, bins, max_depth)
+def _opts(leaf_size=0, bins=0, max_depth=0, collapse=0):
+    return _BuildOptions(leaf_size, bins, max_depth, collapse)
 
 
-def build_bvh8_host(mesh: Mesh, leaf_size=0, bins=0, max_depth=0):
+def build_bvh8_host(mesh: Mesh, leaf_size=0, bins=0, max_depth=0, collapse=0):
     """Returns (nodes structured array, tris structured array, depth, root_max)."""
     lib = load()
     nodes, tris = ctypes.c_void_p(), ctypes.c_void_p()
     count, depth, root_max = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_float()
-    o = _opts(leaf_size, bins, max_depth)
+    o = _opts(leaf_size, bins, max_depth, collapse)
     _check(lib.vt_bvh8_build_host(_ptr(mesh.vertices), len(mesh.vertices), _ptr(mesh.faces), len(mesh.faces),
                                   ctypes.byref(o), ctypes.byref(nodes), ctypes.byref(count), ctypes.byref(tris),
                                   ctypes.byref(depth), ctypes.byref(root_max)))
@@ -389,7 +390,7 @@ class Context:
         _check(load().vt_energy_tables(self._h, _ptr(metal), _ptr(luts)))
         return metal[:129 * 129].reshape(129, 129), metal[129 * 129:], luts
 
-    def set_mesh(self, mesh: Mesh, leaf_size=0, bins=0, max_depth=0) -> "Scene":
+    def set_mesh(self, mesh: Mesh, leaf_size=0, bins=0, max_depth=0, collapse=0) -> "Scene":
         """MaterialLibrary.setMesh for an indexed mesh: builds and uploads the BVH."""
         lib = load()
         v = np.ascontiguousarray(mesh.vertices, np.float32)
@@ -397,7 +398,7 @@ class Context:
         if v.ndim != 2 or v.shape[1] != 8 or f.ndim != 2 or f.shape[1] != 4:
             raise ValueError("vertices must be (V, 8) float32 and faces (F, 4) uint32")
         h = ctypes.c_void_p()
-        o = _opts(leaf_size, bins, max_depth)
+        o = _opts(leaf_size, bins, max_depth, collapse)
         _check(lib.vt_scene_upload_indexed(self._h, _ptr(v), len(v), _ptr(f), len(f), ctypes.byref(o),
                                            ctypes.byref(h)))
         return Scene(self, h)

@@ -99,12 +99,58 @@ def _subtree_tris(nodes, i):
     return np.array(out, np.int64)
 
 
+@pytest.mark.parametrize("collapse", [0, 1])
 @pytest.mark.parametrize("leaf", [1, 2, 3])
-def test_bvh8_layout_invariants(leaf):
+def test_bvh8_layout_invariants(leaf, collapse):
     mesh = vt.icosphere_field(spheres=24, subdivisions=2, seed=3, half_extent=6)
-    nodes, tris, depth, root_max = vt.build_bvh8_host(mesh, leaf_size=leaf)
+    nodes, tris, depth, root_max = vt.build_bvh8_host(mesh, leaf_size=leaf, collapse=collapse)
     _check_bvh8(mesh, nodes, tris, depth, leaf)
     assert root_max == np.abs(mesh.vertices[:, :3]).max()
+
+
+def _sah_cost(nodes, c_node=1.0, c_tri=0.1):
+    """SAH cost of an 8-wide tree from its (dequantised) child boxes, relative to the root box."""
+    def half_area(lo, hi):
+        e = np.maximum(hi.astype(np.float64) - lo.astype(np.float64), 0)
+        return e[0] * e[1] + e[1] * e[2] + e[2] * e[0]
+    total, stack = 0.0, [0]
+    root = nodes[0]
+    valid = [s for s in range(8) if root["meta"][s]]
+    lo = np.min([_dequant(root["qlo"][:, s], root["e"], root["p"]) for s in valid], 0)
+    hi = np.max([_dequant(root["qhi"][:, s], root["e"], root["p"]) for s in valid], 0)
+    a_root = half_area(lo, hi)
+    total += c_node * a_root
+    while stack:
+        nd = nodes[stack.pop()]
+        rank = 0
+        for s in range(8):
+            meta = int(nd["meta"][s])
+            if not meta:
+                continue
+            a = half_area(_dequant(nd["qlo"][:, s], nd["e"], nd["p"]), _dequant(nd["qhi"][:, s], nd["e"], nd["p"]))
+            if meta & 0x80:
+                total += c_node * a
+                stack.append(int(nd["child_base"]) + rank)
+                rank += 1
+            else:
+                total += c_tri * (meta & 3) * a
+    return total / a_root
+
+
+def test_optimal_collapse_fills_nodes_and_lowers_sah_cost():
+    """Greedy widening splits the same ranges with the same binned-SAH splits, so its tree is one of
+    the collapses of the binary tree the dynamic programme chooses among: the programme's tree must
+    cost no more (measured on the quantised boxes) and fill its nodes better."""
+    for mesh in (vt.icosphere_field(spheres=40, subdivisions=3, seed=5, half_extent=20),
+                 vt.heightfield(grid=129, octaves=6, half_extent=30, height=4, seed=2)):
+        opt = vt.build_bvh8_host(mesh, collapse=0)
+        greedy = vt.build_bvh8_host(mesh, collapse=1)
+        _check_bvh8(mesh, opt[0], opt[1], opt[2], 3)
+        fill = lambda n: float((n["meta"] != 0).sum(1).mean())
+        c_opt, c_greedy = _sah_cost(opt[0]), _sah_cost(greedy[0])
+        print(f"{len(mesh.faces)} tris: nodes {len(opt[0])} vs {len(greedy[0])}, children/node {fill(opt[0]):.2f} vs "
+              f"{fill(greedy[0]):.2f}, SAH {c_opt:.2f} vs {c_greedy:.2f}")
+        assert c_opt < c_greedy and fill(opt[0]) > 6.0 > fill(greedy[0]) and len(opt[0]) < len(greedy[0])
 
 
 def test_bvh8_degenerate_inputs():
@@ -113,16 +159,18 @@ def test_bvh8_degenerate_inputs():
     for tris_in in (np.repeat(tri, 50, 0), tri, tri * np.float32(1e-20), tri * np.float32(1e20) + np.float32(3e20)):
         v, f = scenes.mesh_from_triangles(tris_in)
         mesh = vt.Mesh(v, f)
-        nodes, t, depth, _ = vt.build_bvh8_host(mesh)
-        _check_bvh8(mesh, nodes, t, depth, 3)
+        for collapse in (0, 1):
+            nodes, t, depth, _ = vt.build_bvh8_host(mesh, collapse=collapse)
+            _check_bvh8(mesh, nodes, t, depth, 3)
 
 
-def test_bvh8_deterministic():
+@pytest.mark.parametrize("collapse", [0, 1])
+def test_bvh8_deterministic(collapse):
     mesh = vt.icosphere_field(spheres=80, subdivisions=3, seed=4, half_extent=30)
-    a = vt.build_bvh8_host(mesh)
+    a = vt.build_bvh8_host(mesh, collapse=collapse)
     os.environ["VT_BUILD_THREADS"] = "1"
     try:
-        b = vt.build_bvh8_host(mesh)
+        b = vt.build_bvh8_host(mesh, collapse=collapse)
     finally:
         del os.environ["VT_BUILD_THREADS"]
     assert a[0].tobytes() == b[0].tobytes() and a[1].tobytes() == b[1].tobytes()

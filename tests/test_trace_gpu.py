@@ -56,11 +56,11 @@ def _small_cases():
     return out
 
 
-@pytest.mark.parametrize("leaf", [1, 2, 3])
-def test_closest_bit_exact_vs_brute_force(gpu_ctx, leaf):
+@pytest.mark.parametrize("leaf,collapse", [(1, 0), (2, 0), (3, 0), (3, 1)])
+def test_closest_bit_exact_vs_brute_force(gpu_ctx, leaf, collapse):
     for name, v, f, rays in _small_cases():
-        print(f"case {name} leaf {leaf} rays {rays.shape[1]}", flush=True)
-        scene = gpu_ctx.set_mesh(vt.Mesh(v, f), leaf_size=leaf)
+        print(f"case {name} leaf {leaf} collapse {collapse} rays {rays.shape[1]}", flush=True)
+        scene = gpu_ctx.set_mesh(vt.Mesh(v, f), leaf_size=leaf, collapse=collapse)
         ref = oracle.Scene(v, f, with_bvh=False).closest(rays, mode=1)
         _assert_exact(_closest(scene, rays), ref, name)
         scene.close()
