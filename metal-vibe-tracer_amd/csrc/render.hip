@@ -396,8 +396,10 @@ __device__ __forceinline__ float lt_importance(const LtNode &n, V3 p) {
     return cp <= n.cos_e ? 0.0f : n.phi * cp / d2;
 }
 // A light-tree node decoded once per scene (decode_light_tree_kernel, lt_node's arithmetic) into the
-// quantities lt_importance derives from it, 80 bytes: the walk then spends no instructions on
-// dequantisation, the octahedral normal, the centre, the box radius or the cone's sine.
+// quantities lt_importance derives from it: the walk then spends no instructions on dequantisation or
+// the octahedral normal.  Stored as 64 bytes, one cache line each, in four 16-byte words:
+// (lo, phi), (hi, m), (w, cos_o), (cos_e, child, links, dh2); the centre and the cone's sine are
+// re-derived with lt_node's own formulas, which costs fewer cycles than a fifth load per node.
 struct LtDec {
     V3 pc;
     float m;  // max(0.5 |hi - lo|, 1e-20)
@@ -409,23 +411,28 @@ struct LtDec {
     float cos_o, so, cos_e;
     uint32_t child, links;  // links: parent | leaf count << 27, as the node's word
 };
+constexpr int kLtDecWords = 4;
 __device__ __forceinline__ LtDec lt_dec(const RenderScene &sc, uint32_t i) {
-    const float4 *q = sc.tree_dec + 5 * (size_t)i;
-    const float4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4];
+    const float4 *q = sc.tree_dec + kLtDecWords * (size_t)i;
+    const float4 a = q[0], b = q[1], c = q[2], d = q[3];
     LtDec n;
-    n.pc = v3(a.x, a.y, a.z);
-    n.m = a.w;
-    n.lo = v3(b.x, b.y, b.z);
-    n.phi = b.w;
-    n.hi = v3(c.x, c.y, c.z);
-    n.dh2 = c.w;
-    n.w = v3(d.x, d.y, d.z);
-    n.cos_o = d.w;
-    n.so = e.x;
-    n.cos_e = e.y;
-    n.child = __float_as_uint(e.z);
-    n.links = __float_as_uint(e.w);
+    n.lo = v3(a.x, a.y, a.z);
+    n.phi = a.w;
+    n.hi = v3(b.x, b.y, b.z);
+    n.m = b.w;
+    n.w = v3(c.x, c.y, c.z);
+    n.cos_o = c.w;
+    n.cos_e = d.x;
+    n.child = __float_as_uint(d.y);
+    n.links = __float_as_uint(d.z);
+    n.dh2 = d.w;
+    n.pc = (n.lo + n.hi) * 0.5f;
+    n.so = sqrtf(fmaxf(0.0f, 1.0f - n.cos_o * n.cos_o));
     return n;
+}
+// The word holding a node's child and links (x: cos_e, y: child, z: links).
+__device__ __forceinline__ float4 lt_links(const RenderScene &sc, uint32_t i) {
+    return sc.tree_dec[kLtDecWords * (size_t)i + 3];
 }
 // lt_importance on a decoded node (the same operations in the same order).
 __device__ __forceinline__ float lt_importance(const LtDec &n, V3 p) {
@@ -495,8 +502,8 @@ __device__ __forceinline__ uint32_t lt_walk(const RenderScene &sc, V3 p, float u
 }
 __device__ __forceinline__ uint32_t lt_sample(const RenderScene &sc, V3 p, float u, float &pmf) {
     pmf = 1.0f;
-    const float4 root = sc.tree_dec[4];
-    return lt_walk(sc, p, u, pmf, 0u, __float_as_uint(root.z), __float_as_uint(root.w) >> 27, 0u);
+    const float4 root = lt_links(sc, 0);
+    return lt_walk(sc, p, u, pmf, 0u, __float_as_uint(root.y), __float_as_uint(root.z) >> 27, 0u);
 }
 
 // The light tree's top kLtTopLevels levels for one shading point.  Every candidate of a pixel walks the
@@ -521,18 +528,18 @@ __device__ __forceinline__ void lt_top_fill(const RenderScene &sc, LtTop &top, u
         ++depth;
     }
     uint32_t index = 0;
-    float4 e = sc.tree_dec[4];
+    float4 e = lt_links(sc, 0);
     for (uint32_t d = 0; d < depth; ++d, path >>= 1) {
-        if ((__float_as_uint(e.w) >> 27) != 0u) {  // a leaf has no children
+        if ((__float_as_uint(e.z) >> 27) != 0u) {  // a leaf has no children
             index = kNoNode;
             break;
         }
-        index = (path & 1u) ? __float_as_uint(e.z) : index + 1u;
-        e = sc.tree_dec[5 * (size_t)index + 4];
+        index = (path & 1u) ? __float_as_uint(e.y) : index + 1u;
+        e = lt_links(sc, index);
     }
     top.index[h] = index;
-    top.child[h] = index == kNoNode ? 0u : __float_as_uint(e.z);
-    top.count[h] = index == kNoNode ? 0u : __float_as_uint(e.w) >> 27;
+    top.child[h] = index == kNoNode ? 0u : __float_as_uint(e.y);
+    top.count[h] = index == kNoNode ? 0u : __float_as_uint(e.z) >> 27;
 }
 template <int kBlk>
 __device__ __forceinline__ void lt_top_importances(const RenderScene &sc, const LtTop &top, V3 p,
@@ -1306,12 +1313,11 @@ __global__ __launch_bounds__(256) void decode_light_tree_kernel(RenderScene sc, 
     const LtNode n = lt_node(sc, i);
     const V3 pc = (n.lo + n.hi) * 0.5f, dh = pc - n.hi;
     const uint32_t links = reinterpret_cast<const uint4 *>(sc.tree_nodes)[2 * i + 1].w;
-    float4 *q = out + 5 * (size_t)i;
-    q[0] = make_float4(pc.x, pc.y, pc.z, fmaxf(0.5f * length(n.hi - n.lo), 1e-20f));
-    q[1] = make_float4(n.lo.x, n.lo.y, n.lo.z, n.phi);
-    q[2] = make_float4(n.hi.x, n.hi.y, n.hi.z, dot(dh, dh));
-    q[3] = make_float4(n.w.x, n.w.y, n.w.z, n.cos_o);
-    q[4] = make_float4(sqrtf(fmaxf(0.0f, 1.0f - n.cos_o * n.cos_o)), n.cos_e, __uint_as_float(n.child), __uint_as_float(links));
+    float4 *q = out + kLtDecWords * (size_t)i;
+    q[0] = make_float4(n.lo.x, n.lo.y, n.lo.z, n.phi);
+    q[1] = make_float4(n.hi.x, n.hi.y, n.hi.z, fmaxf(0.5f * length(n.hi - n.lo), 1e-20f));
+    q[2] = make_float4(n.w.x, n.w.y, n.w.z, n.cos_o);
+    q[3] = make_float4(n.cos_e, __uint_as_float(n.child), __uint_as_float(links), dot(dh, dh));
 }
 
 __global__ __launch_bounds__(256) void zero_f4_kernel(float4 *p, uint32_t n) {

@@ -35,7 +35,7 @@ struct RenderScene {
     const float *basis;            // 471 x 2 x 4 emission basis (w of the first: D65)
     // Light BVH (tree_node_count 0: power sampling through emitter_cdf).
     const uint32_t *tree_nodes;    // 8 words per node (Sources/LightTree.swift:192-205)
-    const float4 *tree_dec;        // the same nodes decoded, 5 float4 each (launch_decode_light_tree)
+    const float4 *tree_dec;        // the same nodes decoded, 4 float4 each (launch_decode_light_tree)
     const uint32_t *tree_leaves;   // 3 words per leaf position: face, power bits, leaf node
     const uint32_t *tree_ids;      // emitting faces, ascending (emitter_count entries)
     const uint32_t *tree_pos;      // leaf position of tree_ids[i]

@@ -273,7 +273,7 @@ vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, 
                     scene->tree_lo[k] = tree.grid_lo[k];
                     scene->tree_scale[k] = (tree.grid_hi[k] - tree.grid_lo[k]) / 65535.0f;
                 }
-                scene->tree_dec.alloc((size_t)tree.node_count * 5 * 16);
+                scene->tree_dec.alloc((size_t)tree.node_count * 4 * 16);
                 check(vt::launch_decode_light_tree(scene->render_view(), scene->tree_dec.as<float4>(), nullptr),
                       "decode light tree");
                 check(hipDeviceSynchronize(), "hipDeviceSynchronize");
