@@ -697,7 +697,10 @@ Bvh8 build_bvh8(const float *vertices, uint32_t vertex_count, const uint32_t *fa
     std::vector<int32_t> order(n);
     std::vector<WideNode> wide;
     int depth = opt.collapse == 0 ? build_wide_optimal(P, (int32_t)n, order.data(), opt, wide) : opt.max_depth + 1;
-    if (depth > opt.max_depth) depth = build_wide(P, (int32_t)n, order.data(), opt, false, wide);
+    if (depth > opt.max_depth) {
+        wide.clear();
+        depth = build_wide(P, (int32_t)n, order.data(), opt, false, wide);
+    }
     if (depth > opt.max_depth) {
         wide.clear();
         depth = build_wide(P, (int32_t)n, order.data(), opt, true, wide);

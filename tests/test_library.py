@@ -176,6 +176,20 @@ def test_bvh8_deterministic(collapse):
     assert a[0].tobytes() == b[0].tobytes() and a[1].tobytes() == b[1].tobytes()
 
 
+def test_bvh8_depth_limit_falls_back():
+    """A tree deeper than max_depth is rebuilt greedily, then with median splits; the reported depth
+    bounds the real one (the traversal sizes its stack from it), and too small a limit is an error."""
+    mesh = vt.icosphere_field(spheres=24, subdivisions=2, seed=3, half_extent=6)
+    free = vt.build_bvh8_host(mesh)
+    for limit in (free[2] - 1, free[2] - 2):
+        nodes, tris, depth, _ = vt.build_bvh8_host(mesh, max_depth=limit)
+        assert depth <= limit
+        _check_bvh8(mesh, nodes, tris, depth, 3)
+        assert len(nodes) != len(free[0])
+    with pytest.raises(vt.VTError):
+        vt.build_bvh8_host(mesh, max_depth=3)
+
+
 def test_bvh8_empty_scene():
     mesh = vt.Mesh(np.zeros((0, 8), np.float32), np.zeros((0, 4), np.uint32))
     nodes, tris, depth, _ = vt.build_bvh8_host(mesh)
