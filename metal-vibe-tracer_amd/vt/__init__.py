@@ -21,7 +21,7 @@ VT_OK = 0
 STATUS_NAMES = {0: "VT_OK", 1: "VT_ERROR_INVALID_ARGUMENT", 2: "VT_ERROR_OUT_OF_MEMORY", 3: "VT_ERROR_DEVICE",
                 4: "VT_ERROR_UNSUPPORTED"}
 
-# Every symbol include/vt.h and include/vt_workloads.h declare.
+# Every symbol include/vt.h, include/vt_mesh.h and include/vt_workloads.h declare.
 EXPORTED = [
     "vt_context_create", "vt_context_destroy", "vt_last_error", "vt_scene_upload_indexed", "vt_scene_destroy",
     "vt_scene_get_info", "vt_trace_closest", "vt_trace_any", "vt_trace_stats", "vt_bvh8_build_host", "vt_host_free",
@@ -33,6 +33,8 @@ EXPORTED = [
     "vt_nrc_gradients", "vt_renderer_set_cache", "vt_renderer_cache_records", "vt_scene_rebuild",
     "vt_batch_create", "vt_batch_destroy", "vt_batch_render", "vt_batch_reset", "vt_batch_frames",
     "vt_batch_readback", "vt_batch_get_tuning", "vt_batch_set_tuning", "vt_batch_autotune",
+    "vt_mesh_payload_bytes", "vt_mesh_payload_write", "vt_mesh_payload_read", "vt_obj_parse", "vt_obj_destroy",
+    "vt_obj_part_count", "vt_obj_skipped", "vt_obj_library_count", "vt_obj_library", "vt_obj_part", "vt_obj_subset",
 ]
 
 
@@ -170,6 +172,17 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_spectral_reflectance": (i32, [P(f32), P(f32)]),
         "vt_scene_set_light_tree": (i32, [vp, c.c_int]),
         "vt_light_tree_build_host": (i32, [vp, vp, u32, P(vp), P(u32), P(vp), P(vp), P(u32), P(f32), P(f32)]),
+        "vt_mesh_payload_bytes": (c.c_uint64, [u32, u32]),
+        "vt_mesh_payload_write": (i32, [vp, u32, vp, u32, vp, c.c_uint64]),
+        "vt_mesh_payload_read": (i32, [vp, c.c_uint64, u32, P(vp), P(u32), P(vp), P(u32)]),
+        "vt_obj_parse": (i32, [c.c_char_p, c.c_uint64, u32, c.c_uint64, P(vp)]),
+        "vt_obj_destroy": (None, [vp]),
+        "vt_obj_part_count": (u32, [vp]),
+        "vt_obj_skipped": (u32, [vp]),
+        "vt_obj_library_count": (u32, [vp]),
+        "vt_obj_library": (c.c_char_p, [vp, u32]),
+        "vt_obj_part": (i32, [vp, u32, P(c.c_char_p), P(c.c_char_p), P(u32), P(vp), P(u32), P(vp), P(u32)]),
+        "vt_obj_subset": (c.c_char_p, [vp, u32, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -321,6 +334,75 @@ def random_rays(lo, hi, n, seed) -> np.ndarray:
     r = np.empty((8, n), np.float32)
     load().vt_gen_random_rays((ctypes.c_float * 3)(*lo), (ctypes.c_float * 3)(*hi), n, seed, *_ray_args(r))
     return r
+
+
+# --------------------------------------------------------------------------------------------
+# Mesh formats (include/vt_mesh.h): the VTMESH01 payload and the OBJ reader
+# --------------------------------------------------------------------------------------------
+
+def mesh_payload(mesh: Mesh) -> bytes:
+    """The VTMESH01 payload of an indexed mesh."""
+    lib = load()
+    v = np.ascontiguousarray(mesh.vertices, np.float32)
+    f = np.ascontiguousarray(mesh.faces, np.uint32)
+    n = lib.vt_mesh_payload_bytes(len(v), len(f))
+    out = np.empty(n, np.uint8)
+    _check(lib.vt_mesh_payload_write(_ptr(v), len(v), _ptr(f), len(f), _ptr(out), n))
+    return out.tobytes()
+
+
+def read_mesh_payload(data: bytes, maximum_faces=1 << 30) -> Mesh:
+    lib = load()
+    buf = np.frombuffer(data, np.uint8) if len(data) else np.zeros(1, np.uint8)
+    vp, fp = ctypes.c_void_p(), ctypes.c_void_p()
+    nv, nf = ctypes.c_uint32(), ctypes.c_uint32()
+    _check(lib.vt_mesh_payload_read(_ptr(buf), len(data), maximum_faces, ctypes.byref(vp), ctypes.byref(nv),
+                                    ctypes.byref(fp), ctypes.byref(nf)))
+    try:
+        v = np.frombuffer(ctypes.string_at(vp, 32 * nv.value), np.float32).reshape(-1, 8).copy()
+        f = np.frombuffer(ctypes.string_at(fp, 16 * nf.value), np.uint32).reshape(-1, 4).copy()
+    finally:
+        lib.vt_host_free(vp)
+        lib.vt_host_free(fp)
+    return Mesh(v, f)
+
+
+def read_obj(text, node_budget=0, triangle_limit=0):
+    """Parses OBJ text (str or bytes).  Returns (parts, skipped, libraries); each part is a dict with
+    object, group, subsets (material names) and mesh (welded vertices, faces tagged by subset)."""
+    lib = load()
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    h = ctypes.c_void_p()
+    _check(lib.vt_obj_parse(data, len(data), node_budget, triangle_limit, ctypes.byref(h)))
+    try:
+        parts = []
+        for k in range(lib.vt_obj_part_count(h)):
+            obj, grp = ctypes.c_char_p(), ctypes.c_char_p()
+            ns, nv, nf = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+            vp, fp = ctypes.c_void_p(), ctypes.c_void_p()
+            _check(lib.vt_obj_part(h, k, ctypes.byref(obj), ctypes.byref(grp), ctypes.byref(ns), ctypes.byref(vp),
+                                   ctypes.byref(nv), ctypes.byref(fp), ctypes.byref(nf)))
+            v = np.frombuffer(ctypes.string_at(vp, 32 * nv.value), np.float32).reshape(-1, 8).copy()
+            f = np.frombuffer(ctypes.string_at(fp, 16 * nf.value), np.uint32).reshape(-1, 4).copy()
+            parts.append(dict(object=obj.value.decode(), group=grp.value.decode(), mesh=Mesh(v, f),
+                              subsets=[lib.vt_obj_subset(h, k, i).decode() for i in range(ns.value)]))
+        libraries = [lib.vt_obj_library(h, i).decode() for i in range(lib.vt_obj_library_count(h))]
+        return parts, int(lib.vt_obj_skipped(h)), libraries
+    finally:
+        lib.vt_obj_destroy(h)
+
+
+def merge_parts(parts) -> Mesh:
+    """One indexed mesh of OBJ parts (faces keep their part-local subset)."""
+    vs, fs, base = [], [], 0
+    for p in parts:
+        m = p["mesh"]
+        f = m.faces.copy()
+        f[:, :3] += base
+        vs.append(m.vertices)
+        fs.append(f)
+        base += len(m.vertices)
+    return Mesh(np.concatenate(vs), np.concatenate(fs))
 
 
 # --------------------------------------------------------------------------------------------
