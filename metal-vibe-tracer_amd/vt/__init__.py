@@ -21,7 +21,7 @@ VT_OK = 0
 STATUS_NAMES = {0: "VT_OK", 1: "VT_ERROR_INVALID_ARGUMENT", 2: "VT_ERROR_OUT_OF_MEMORY", 3: "VT_ERROR_DEVICE",
                 4: "VT_ERROR_UNSUPPORTED"}
 
-# Every symbol include/vt.h, include/vt_mesh.h and include/vt_workloads.h declare.
+# Every symbol include/vt.h, include/vt_mesh.h, include/vt_sampler.h and include/vt_workloads.h declare.
 EXPORTED = [
     "vt_context_create", "vt_context_destroy", "vt_last_error", "vt_scene_upload_indexed", "vt_scene_destroy",
     "vt_scene_get_info", "vt_trace_closest", "vt_trace_any", "vt_trace_stats", "vt_bvh8_build_host", "vt_host_free",
@@ -35,6 +35,7 @@ EXPORTED = [
     "vt_batch_readback", "vt_batch_get_tuning", "vt_batch_set_tuning", "vt_batch_autotune",
     "vt_mesh_payload_bytes", "vt_mesh_payload_write", "vt_mesh_payload_read", "vt_obj_parse", "vt_obj_destroy",
     "vt_obj_part_count", "vt_obj_skipped", "vt_obj_library_count", "vt_obj_library", "vt_obj_part", "vt_obj_subset",
+    "vt_z_draws",
 ]
 
 
@@ -81,6 +82,24 @@ class BatchJob(ctypes.Structure):
     """vt_batch_job."""
     _fields_ = [("scene", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("eye", ctypes.c_float * 3), ("target", ctypes.c_float * 3), ("fov_degrees", ctypes.c_float)]
+
+
+class ZQuery(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("width", "height", "frame_index", "sample_index", "temporal_model",
+                                               "path_vertex", "stream", "candidate", "candidate_bits", "draws", "kind")]
+
+
+def z_draws(width, height, frame_index, sample_index, temporal_model=0, path_vertex=0, stream=0, candidate=0,
+            candidate_bits=0, draws=1, kind=1, out=None, stream_handle=None):
+    """The Z++ sampler's draws of one event for every pixel on the device (vt_z_draws): a float32 device
+    tensor shaped (height, width, draws, kind)."""
+    import torch
+    if out is None:
+        out = torch.empty((height, width, draws, kind), dtype=torch.float32, device="cuda:0")
+    q = ZQuery(width, height, frame_index, sample_index, temporal_model, path_vertex, stream, candidate,
+               candidate_bits, draws, kind)
+    _check(load().vt_z_draws(ctypes.byref(q), ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream_handle)))
+    return out
 
 
 class BatchTuning(ctypes.Structure):
@@ -183,6 +202,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_obj_library": (c.c_char_p, [vp, u32]),
         "vt_obj_part": (i32, [vp, u32, P(c.c_char_p), P(c.c_char_p), P(u32), P(vp), P(u32), P(vp), P(u32)]),
         "vt_obj_subset": (c.c_char_p, [vp, u32, u32]),
+        "vt_z_draws": (i32, [P(ZQuery), vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -86,6 +86,15 @@ def load():
         "vo_light_tree_importance": (f32, [vp, u32, P(f32)]),
         "vo_light_tree_sample": (u32, [vp, P(f32), f32, P(f32)]),
         "vo_light_tree_pmf": (f32, [vp, P(f32), u32]),
+        "vo_z_mix": (u32, [u32]),
+        "vo_z_shuffle": (u32, [u32, u32]),
+        "vo_z_owen": (u32, [u32, u32]),
+        "vo_z_owen_seed": (u32, [u32, u32]),
+        "vo_z_sobol1": (u32, [u32]),
+        "vo_z_o2m3": (u32, [u32, c.c_int]),
+        "vo_z_morton": (u32, [u32, u32]),
+        "vo_z_pixel_key": (u32, [u32, u32, u32, u32, u32]),
+        "vo_z_draws": (None, [u32] * 11 + [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -446,3 +455,16 @@ def render_nrc(scene: "Scene", materials, frame: dict, sample_index0: int, sampl
                           threads, _p(out))
     del keep
     return out.reshape(h, w, 3)
+
+
+def z_draws(width, height, frame_index, sample_index, temporal_model=0, path_vertex=0, stream=0, candidate=0,
+            candidate_bits=0, draws=1, kind=1):
+    """The Z++ sampler's draws for every pixel (oracle/zsampler.c): (floats, raw 0.32 words), each
+    shaped (height, width, draws, kind)."""
+    n = width * height * draws * kind
+    out = np.empty(n, np.float32)
+    raw = np.empty(n, np.uint32)
+    load().vo_z_draws(width, height, frame_index, sample_index, temporal_model, path_vertex, stream, candidate,
+                      candidate_bits, draws, kind, out.ctypes.data_as(ctypes.c_void_p), raw.ctypes.data_as(ctypes.c_void_p))
+    shape = (height, width, draws, kind)
+    return out.reshape(shape), raw.reshape(shape)
