@@ -12,6 +12,7 @@
 #include <stdint.h>
 
 #include "render_kernels.h"
+#include "zsampler.h"
 #include "traverse8.h"
 
 constexpr int kNrcInputCount = 11;
@@ -33,6 +34,31 @@ __device__ __forceinline__ uint32_t pcg_hash(uint32_t input) {
 __device__ __forceinline__ float rand_f(uint32_t &seed) {
     seed = pcg_hash(seed);
     return (float)(seed >> 8u) * (1.0f / 16777216.0f);
+}
+// A path's random numbers (Sampler, main.swift:213-293): its PCG stream, or in Z mode the Z++ sampler
+// under the pixel's key, each sampling event (a path vertex and a stream) owning 32 dimensions.
+// rand_f3 is one 3D constituent in Z mode and three consecutive draws of the stream otherwise, so the
+// PCG mode draws exactly what it did with single draws.
+struct Rng {
+    uint32_t pcg;
+    ZSampler zs;
+    bool z;
+};
+enum : uint32_t { kZBsdf = 0u, kZNee = 1u, kZRoulette = 2u, kZWavelength = 12u };
+__device__ __forceinline__ float rand_f(Rng &r) { return r.z ? r.zs.rand1() : rand_f(r.pcg); }
+__device__ __forceinline__ void rand_f3(uint32_t &seed, float &a, float &b, float &c) {
+    a = rand_f(seed);
+    b = rand_f(seed);
+    c = rand_f(seed);
+}
+__device__ __forceinline__ void rand_f3(Rng &r, float &a, float &b, float &c) {
+    if (r.z) r.zs.rand3(a, b, c);
+    else rand_f3(r.pcg, a, b, c);
+}
+// sampler_event (main.swift:293-295); the PCG stream ignores events.
+__device__ __forceinline__ void rng_event(uint32_t &, uint32_t, uint32_t) {}
+__device__ __forceinline__ void rng_event(Rng &r, uint32_t vertex, uint32_t stream) {
+    if (r.z) r.zs.event(vertex, stream);
 }
 
 // ---------------------------------------------------------------- small vector helpers
@@ -266,9 +292,9 @@ __device__ __forceinline__ S4 eval_bsdf_pdf(const RenderScene &sc, const Spectra
 
 // sample_bsdf (main.swift:3304-3368): direction, weight = f |cos| / pdf per lane, pdf (0 for the delta
 // branches); false ends the path.
-template <bool kLayered>
+template <bool kLayered, class R>
 __device__ __forceinline__ bool sample_bsdf(const RenderScene &sc, const Spectral &sp, const Lanes &lanes, const Surface &s,
-                                           const Mat &m, V3 in, uint32_t &seed, V3 &dir, S4 &weight, float &pdf) {
+                                           const Mat &m, V3 in, R &seed, V3 &dir, S4 &weight, float &pdf) {
     pdf = 0.0f;
     if (m.type == kDielectric) {
         const float ior = m.ior;
@@ -305,8 +331,9 @@ __device__ __forceinline__ bool sample_bsdf(const RenderScene &sc, const Spectra
         return dot(dir, s.ng) > 0.0f;
     }
     if (m.type == kDiffuse) {
-        (void)rand_f(seed);  // the lobe-choice draw the generic sampler keeps (main.swift:3354)
-        const float r1 = rand_f(seed), r2 = rand_f(seed);
+        float r0, r1, r2;  // r0: the lobe-choice draw the generic sampler keeps (main.swift:3354-3356)
+        rand_f3(seed, r0, r1, r2);
+        (void)r0;
         dir = sample_cosine_hemisphere(s.n, r1, r2);
         pdf = fmaxf(0.0f, dot(s.n, dir)) / kPi;
         weight = reflectance(sp, lanes, m.lagrange);
@@ -317,7 +344,8 @@ __device__ __forceinline__ bool sample_bsdf(const RenderScene &sc, const Spectra
     Pbr p;
     pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, s.n, -in, !s.front,
                 reflectance(sp, lanes, m.lagrange));
-    const float r0 = rand_f(seed), r1 = rand_f(seed), r2 = rand_f(seed);
+    float r0, r1, r2;
+    rand_f3(seed, r0, r1, r2);
     if (!pbr_sample(p, E, r0, r1, r2, dir, weight, pdf)) {
         pdf = 0.0f;
         return false;
@@ -602,8 +630,8 @@ struct LightSample {
     uint32_t slot, light;  // light: face + 2 (LightSample.isDirectional), 0 = none
 };
 // walk(u, pmf): the light-tree walk for uniform u (lt_sample, or lt_sample_top with a pixel's cache).
-template <class Walk>
-__device__ __forceinline__ LightSample sample_emitter_via(const RenderScene &sc, V3 p, uint32_t &seed, const Walk &walk) {
+template <class Walk, class R>
+__device__ __forceinline__ LightSample sample_emitter_via(const RenderScene &sc, V3 p, R &seed, const Walk &walk) {
     LightSample ls;
     ls.pdf = 0.0f;
     ls.slot = 0;
@@ -616,7 +644,9 @@ __device__ __forceinline__ LightSample sample_emitter_via(const RenderScene &sc,
     if (sc.emitter_count == 0) return ls;
     // importedProbability = 1 (no environment): the draw is still consumed.
     if (!(rand_f(seed) < 1.0f)) return ls;
-    float value = rand_f(seed), ry = rand_f(seed), rz = rand_f(seed);
+    // The emitter choice and the point on it: one 3D constituent in Z mode (main.swift:2886-2894).
+    float value, ry, rz;
+    rand_f3(seed, value, ry, rz);
     uint32_t face;
     float tree_pmf = 0.0f;
     if (sc.tree_node_count) {
@@ -657,7 +687,8 @@ __device__ __forceinline__ LightSample sample_emitter_via(const RenderScene &sc,
     ls.pdf = geometry > 1e-12f ? area_pdf / geometry : 0.0f;
     return ls;
 }
-__device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, V3 p, uint32_t &seed) {
+template <class R>
+__device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, V3 p, R &seed) {
     return sample_emitter_via(sc, p, seed, [&](float u, float &pmf) { return lt_sample(sc, p, u, pmf); });
 }
 __device__ __forceinline__ float emitter_pdf(const RenderScene &sc, V3 p, V3 position, uint32_t face, float weight) {
@@ -815,7 +846,13 @@ __device__ __forceinline__ void generate_path(const FrameArgs &f, uint32_t pix, 
     // pixel_sampler (main.swift:508-515) and wavelength_numbers (main.swift:2218-2230).
     uint32_t seed = (gy * f.width + gx) ^ (sample_index * 1999999973u);
     const uint32_t h = pcg_hash(seed ^ 0x5bd1e995u);
-    const float wl = (float)(h >> 8u) * (1.0f / 16777216.0f);
+    float wl = (float)(h >> 8u) * (1.0f / 16777216.0f);
+    if (f.sampler == 1u) {
+        // Z mode: dimension 0 of the vertex-0 Z_WAVELENGTH event under the pixel's key (main.swift:2221-2226).
+        ZSampler zs{z_pixel_key(gx, gy, f.frame0 + (sample_index - f.sample_index0) + 1u, sample_index, f.z_temporal), 0u, 0u, 0u};
+        zs.event(0u, kZWavelength);
+        wl = zs.rand1();
+    }
     // frameJitter: Halton(2, 3) - 0.5 with a 1024-frame period (main.swift:8773-8786).
     uint32_t jn = (max(1u, sample_index) - 1u) % 1024u + 1u;
     float jx = 0.0f, jy = 0.0f;
@@ -984,12 +1021,20 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
     // Cache query (rendering paths) or training record (training paths) of this vertex.
     bool query = false, record = false, training = false;
     float cin[kNrcInputCount];
-    uint32_t id = 0, seed = 0, state = 0;
+    uint32_t id = 0, state = 0;
+    Rng rng{0u, ZSampler{0u, 0u, 0u, 0u}, false};
     float wl = 0.0f;
     S4 thr = s4(0.0f);
     if (active) {
         id = in.id[i];
-        seed = in.seed[i];
+        rng.pcg = in.seed[i];
+        if (f.sampler == 1u) {
+            // Z mode: the pixel's key; path id = sample * npix + pixel, and sample k is frame
+            // frame0 + k + 1 of the accumulation.
+            const uint32_t k = id / f.npix, pixel = id - k * f.npix;
+            rng.zs.key = z_pixel_key(pixel % f.width, pixel / f.width, f.frame0 + k + 1u, f.sample_index0 + k, f.z_temporal);
+            rng.z = true;
+        }
         wl = in.wl[i];
         thr = S4{{in.t0[i], in.t1[i], in.t2[i], in.t3[i]}};
         state = in.state[i];
@@ -1046,7 +1091,7 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
             float wsum = target_pdf_at(sc, sp, m, s, dir, sel) * cw.z * cw.y;
             float M = cw.y;
             for (int k = 0; k < 4; ++k) {
-                const float rx = rand_f(seed), ry = rand_f(seed);
+                const float rx = rand_f(rng.pcg), ry = rand_f(rng.pcg);
                 const int nx = (int)gx + (int)((rx * 2.0f - 1.0f) * 16.0f);
                 const int ny = (int)gy + (int)((ry * 2.0f - 1.0f) * 16.0f);
                 if (nx < 0 || nx >= (int)f.width || ny < 0 || ny >= (int)f.height) continue;
@@ -1062,7 +1107,7 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
                 const float w = target_pdf_at(sc, sp, m, s, dir, ns) * nw.z * nw.y;
                 wsum += w;
                 M += nw.y;
-                if (rand_f(seed) * wsum < w) sel = ns;
+                if (rand_f(rng.pcg) * wsum < w) sel = ns;
             }
             const float fp = target_pdf_at(sc, sp, m, s, dir, sel);
             const float W = (M > 0.0f && fp > 0.0f) ? wsum / (M * fp) : 0.0f;
@@ -1090,7 +1135,8 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
         } else if (alive && !is_delta(m) && f.sampling_mode != 3u) {
             // Next-event estimation at this non-delta vertex (main.swift:7655-7672, 7790-7805).
             const bool use_mis = f.sampling_mode <= 1u && (bounce == 0u ? f.scattering_limit > 0 : sdepth < f.scattering_limit);
-            LightSample ls = sample_emitter(sc, s.p, seed);
+            rng_event(rng, bounce + 1u, kZNee);  // this hit is vertex bounce + 1 (main.swift:7658, 7804)
+            LightSample ls = sample_emitter(sc, s.p, rng);
             if (ls.pdf > 0.0f) {
                 V3 o = ray_origin(s.p, s.ng, ls.wi, s.error);
                 V3 delta = ls.position - o;
@@ -1117,7 +1163,8 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
         if (alive && bounce > 3u) {
             // Russian roulette after the third bounce (main.swift:7819-7827).
             const float survival = fminf(fmaxf(smax(thr), 0.05f), is_delta(m) ? 0.99f : 0.95f);
-            if (rand_f(seed) >= survival) alive = false;
+            rng_event(rng, bounce + 1u, kZRoulette);
+            if (rand_f(rng) >= survival) alive = false;
             else thr = scale(thr, 1.0f / survival);
         }
         const bool prev_delta = alive && is_delta(m);
@@ -1134,7 +1181,8 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
             V3 nd;
             S4 bw;
             float pdf;
-            if (sample_bsdf<kLayered>(sc, sp, lanes, s, m, dir, seed, nd, bw, pdf)) {
+            rng_event(rng, bounce + 1u, kZBsdf);  // vertex bounce + 1 scatters (main.swift:7730)
+            if (sample_bsdf<kLayered>(sc, sp, lanes, s, m, dir, rng, nd, bw, pdf)) {
                 thr = mul(thr, bw);
                 bool finite = isfinite(thr.v[0]) && isfinite(thr.v[1]) && isfinite(thr.v[2]) && isfinite(thr.v[3]);
                 if (finite && smax(thr) > 0.0f) {
@@ -1185,7 +1233,7 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
         out.tmin[slot] = ray_t_min(new_o);
         out.tmax[slot] = 1e20f;
         out.id[slot] = id;
-        out.seed[slot] = seed;
+        out.seed[slot] = rng.pcg;
         out.wl[slot] = wl;
         out.t0[slot] = thr.v[0]; out.t1[slot] = thr.v[1]; out.t2[slot] = thr.v[2]; out.t3[slot] = thr.v[3];
         out.state[slot] = state;

@@ -77,6 +77,9 @@ struct FrameArgs {
     // take its outgoing radiance from the cache, except training paths (1 in train_period, by a hash
     // of pixel and sample), which are traced in full and leave one training record per such vertex.
     uint32_t nrc, train_period;
+    // Sampler: 0 the PCG stream; 1 the Z++ sampler (pixel key from the accumulation's frame
+    // frame0 + sample + 1 and sample index, temporal model z_temporal).
+    uint32_t sampler, z_temporal, frame0;
     float box_lo[3], box_scale[3];  // cache inputs: (p - lo) * scale in [0, 1]
 };
 

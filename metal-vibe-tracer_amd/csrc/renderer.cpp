@@ -449,6 +449,10 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
         if (!(p->fov_degrees > 0.0f && p->fov_degrees < 180.0f)) return fail(VT_ERROR_INVALID_ARGUMENT, "fov must be in (0, 180)");
         if (p->sample_index == 0) return fail(VT_ERROR_INVALID_ARGUMENT, "sample_index starts at 1");
         if (p->material_sort > 1) return fail(VT_ERROR_INVALID_ARGUMENT, "material_sort must be 0 or 1");
+        if (p->sampler > 1 || p->z_temporal > 3)
+            return fail(VT_ERROR_INVALID_ARGUMENT, "sampler must be 0 or 1 and z_temporal 0..3");
+        if (p->sampler == 1 && (p->sampling_mode == 0 || p->radiance_cache))
+            return fail(VT_ERROR_UNSUPPORTED, "the Z sampler is supported with sampling modes 1-3 without the radiance cache");
         if (samples == 0) return VT_OK;
         check(hipSetDevice(r->ctx->device), "hipSetDevice");
         hipStream_t s = (hipStream_t)stream;
@@ -520,6 +524,9 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
         for (uint32_t done = 0; done < samples; done += per) {
             f.samples = std::min(per, samples - done);
             f.sample_index0 = p->sample_index + done;
+            f.sampler = p->sampler;
+            f.z_temporal = p->z_temporal;
+            f.frame0 = r->frame_index;
             const uint32_t n = f.samples * npix;
             check(vt::launch_reset_counters(cnt, 16, s), "reset");
             int cur = 0;

@@ -226,13 +226,44 @@ static int occluded(const vo_render_scene *sc, v3 o, v3 d, real tmin, real tmax)
 
 /* sample_direct_light, imported branch (main.swift:2888-2917): draws 4 numbers; pdf is the
  * solid-angle pdf (0: no sample).  light = face + 2 (LightSample.isDirectional). */
+/* The pixel's random numbers (Sampler, main.swift:213-293): the PCG stream, or in Z mode the Z++
+ * sampler's draws (oracle/zsampler.c) under the pixel's key, each sampling event (a path vertex and a
+ * stream) owning 32 dimensions. */
+typedef struct { uint32_t pcg, key, dim; int z; } rng_t;
+void vo_z_draw(uint32_t key, uint32_t *dimension, uint32_t sub, uint32_t sub_bits, int kind, float *out);
+uint32_t vo_z_pixel_key(uint32_t x, uint32_t y, uint32_t frame_index, uint32_t sample_index, uint32_t model);
+enum { Z_BSDF = 0, Z_NEE = 1, Z_ROULETTE = 2, Z_WAVELENGTH = 12 };
+static real rr(rng_t *r) {
+    if (!r->z) return (real)vo_rand_f(&r->pcg);
+    float f;
+    vo_z_draw(r->key, &r->dim, 0u, 0u, 1, &f);
+    return (real)f;
+}
+static void rr3(rng_t *r, real *a, real *b, real *c) {
+    if (!r->z) {
+        *a = (real)vo_rand_f(&r->pcg);
+        *b = (real)vo_rand_f(&r->pcg);
+        *c = (real)vo_rand_f(&r->pcg);
+        return;
+    }
+    float f[3];
+    vo_z_draw(r->key, &r->dim, 0u, 0u, 3, f);
+    *a = (real)f[0];
+    *b = (real)f[1];
+    *c = (real)f[2];
+}
+/* sampler_event (main.swift:293-295). */
+static void rng_event(rng_t *r, uint32_t vertex, uint32_t stream) { r->dim = (vertex * 16u + stream) * 32u; }
+
 typedef struct { v3 position, wi, emission; real pdf; uint32_t light; } light_sample;
-static light_sample sample_direct_light(const vo_render_scene *sc, v3 p, uint32_t *seed) {
+static light_sample sample_direct_light(const vo_render_scene *sc, v3 p, rng_t *seed) {
     light_sample ls;
     memset(&ls, 0, sizeof ls);
     if (sc->emitter_count == 0) return ls;
-    if (!(vo_rand_f(seed) < 1.0f)) return ls;  /* importedProbability = 1 */
-    real value = (real)vo_rand_f(seed), ry = (real)vo_rand_f(seed), rz = (real)vo_rand_f(seed);
+    if (!(rr(seed) < (real)1)) return ls;  /* importedProbability = 1 */
+    /* Emitter choice and the point on it: one 3D constituent in Z mode (main.swift:2886-2894). */
+    real value, ry, rz;
+    rr3(seed, &value, &ry, &rz);
     uint32_t face;
     float tree_pmf = 0.0f;
     if (sc->tree) {
@@ -303,7 +334,7 @@ static s4 eval_bsdf_pdf(const vo_render_scene *sc, const lanes_t *l, const surf_
 /* sample_bsdf (main.swift:3304-3368): direction, weight = f |cos| / pdf per lane, pdf (0 for the
  * delta branches).  Returns 0 when the path ends here. */
 static int sample_bsdf(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, const mat_t *m, v3 in,
-                       uint32_t *seed, v3 *dir, s4 *weight, real *pdf) {
+                       rng_t *seed, v3 *dir, s4 *weight, real *pdf) {
     *pdf = 0;
     if (m->type == 2u) {
         real ior = m->ior;
@@ -317,7 +348,7 @@ static int sample_bsdf(const vo_render_scene *sc, const lanes_t *l, const surf_t
             real c = eta > 1 && s2t < 1 ? RSQRT((real)1 - s2t) : ci;
             real omc = (real)1 - c;
             real fr = r0 * r0 + ((real)1 - r0 * r0) * (omc * omc * omc * omc * omc);
-            int reflected = s2t >= 1 || (real)vo_rand_f(seed) < fr;
+            int reflected = s2t >= 1 || rr(seed) < fr;
             if (reflected) {
                 *dir = norm(sub(in, mulv(mm, (real)2 * dot(mm, in))));
                 *weight = tint;
@@ -340,8 +371,8 @@ static int sample_bsdf(const vo_render_scene *sc, const lanes_t *l, const surf_t
         return dot(*dir, s->ng) > 0;
     }
     if (m->type == 0u) {
-        vo_rand_f(seed);
-        real r1 = (real)vo_rand_f(seed), r2 = (real)vo_rand_f(seed);
+        real r0, r1, r2;  /* the lobe-choice draw r0 is kept and unused (main.swift:3354-3356) */
+        rr3(seed, &r0, &r1, &r2);
         *dir = cosine_hemisphere(s->n, r1, r2);
         *pdf = RFMAX((real)0, dot(s->n, *dir)) / PI_R;
         *weight = reflectance(sc, l, m->lagrange);
@@ -350,7 +381,8 @@ static int sample_bsdf(const vo_render_scene *sc, const lanes_t *l, const surf_t
     pbr_t p;
     pbr_inputs pin = {(real)m->type, m->roughness, m->ior, m->metalness, m->transmission};
     pbr_prepare(&p, &pin, s->n, neg(in), !s->front, reflectance(sc, l, m->lagrange));
-    real r0 = (real)vo_rand_f(seed), r1 = (real)vo_rand_f(seed), r2 = (real)vo_rand_f(seed);
+    real r0, r1, r2;
+    rr3(seed, &r0, &r1, &r2);
     if (!pbr_sample(&p, r0, r1, r2, dir, weight, pdf)) { *pdf = 0; return 0; }
     if (!(*pdf > 0)) return 0;
     if (!(m->transmission > 0) && dot(*dir, s->ng) <= 0) return 0;
@@ -361,7 +393,7 @@ static int sample_bsdf(const vo_render_scene *sc, const lanes_t *l, const surf_t
 
 /* Next-event estimate at surface s: draws 4 numbers, returns the contribution if visible. */
 static v3 nee(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, const mat_t *m, v3 wo, s4 thr,
-              int use_mis, uint32_t *seed) {
+              int use_mis, rng_t *seed) {
     v3 zero = V(0, 0, 0);
     light_sample ls = sample_direct_light(sc, s->p, seed);
     if (!(ls.pdf > 0)) return zero;
@@ -454,7 +486,7 @@ static v3 camera_dir(const cam_t *c, const vo_frame *fr, uint32_t gx, uint32_t g
  * (samplingMode 0) the departure from a diffuse primary is not MIS-weighted: its DI reservoir
  * estimated the whole direct integral (main.swift:7725-7728). */
 static v3 continue_path(const vo_render_scene *sc, const vo_frame *fr, const lanes_t *l, surf_t s, mat_t m, v3 dir,
-                        uint32_t *seed) {
+                        rng_t *seed) {
     const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
     v3 radiance = V(0, 0, 0);
     s4 thr = S4(1);
@@ -475,6 +507,7 @@ static v3 continue_path(const vo_render_scene *sc, const vo_frame *fr, const lan
         v3 nd;
         s4 bw;
         real bpdf;
+        rng_event(seed, (uint32_t)bounce, Z_BSDF);  /* vertex `bounce` scatters (main.swift:7730) */
         if (!sample_bsdf(sc, l, &s, &m, cur_dir, seed, &nd, &bw, &bpdf)) break;
         thr = smul(thr, bw);
         if (!(isfinite(thr.v[0]) && isfinite(thr.v[1]) && isfinite(thr.v[2]) && isfinite(thr.v[3])) || smax(thr) <= 0) break;
@@ -511,13 +544,16 @@ static v3 continue_path(const vo_render_scene *sc, const vo_frame *fr, const lan
             break;
         }
         if (emission_only) break;
-        if (!is_delta(&rm) && fr->sampling_mode != 3u)
+        if (!is_delta(&rm) && fr->sampling_mode != 3u) {
+            rng_event(seed, (uint32_t)bounce + 1u, Z_NEE);
             radiance = add(radiance, nee(sc, l, &r, &rm, neg(cur_dir), thr, fr->sampling_mode <= 1u && sdepth < limit, seed));
+        }
         s = r;
         m = rm;
         if (bounce > 3) {
             real survival = RFMIN(RFMAX(smax(thr), (real)0.05), is_delta(&m) ? (real)0.99 : (real)0.95);
-            if ((real)vo_rand_f(seed) >= survival) break;
+            rng_event(seed, (uint32_t)bounce + 1u, Z_ROULETTE);
+            if (rr(seed) >= survival) break;
             thr = sscale(thr, (real)1 / survival);
         }
     }
@@ -528,10 +564,18 @@ static v3 continue_path(const vo_render_scene *sc, const vo_frame *fr, const lan
 static v3 sample_pixel(const vo_render_scene *sc, const vo_frame *fr, const cam_t *cam, uint32_t gx, uint32_t gy,
                        uint32_t sample_index) {
     const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
-    uint32_t seed = pixel_seed(fr, gx, gy, sample_index);
-    lanes_t l = lanes_of(sc, pixel_u(seed));
+    uint32_t pcg = pixel_seed(fr, gx, gy, sample_index);
+    rng_t seed = {vo_pcg_hash(pcg ^ 0x27d4eb2fu), 0u, 0u, fr->sampler == 1u};  /* decorrelate_shading_seed (main.swift:106-108) */
+    real u = pixel_u(pcg);
+    if (seed.z) {
+        /* Z mode: the pixel's key (this sample is frame s + 1 of the accumulation), and the wavelength
+         * number from dimension 0 of the vertex-0 Z_WAVELENGTH event (main.swift:2221-2226). */
+        seed.key = vo_z_pixel_key(gx, gy, sample_index - fr->sample_index0 + 1u, sample_index, fr->z_temporal);
+        rng_event(&seed, 0u, Z_WAVELENGTH);
+        u = rr(&seed);
+    }
+    lanes_t l = lanes_of(sc, u);
     v3 dir = camera_dir(cam, fr, gx, gy, sample_index);
-    seed = vo_pcg_hash(seed ^ 0x27d4eb2fu);  /* decorrelate_shading_seed (main.swift:106-108) */
 
     v3 radiance = V(0, 0, 0);
     surf_t s;
@@ -539,15 +583,20 @@ static v3 sample_pixel(const vo_render_scene *sc, const vo_frame *fr, const cam_
     if (!trace(sc, cam->eye, dir, ray_t_min(cam->eye), &s, &face)) return radiance;
     mat_t m = mat_of(sc, s.slot);
     if (m.type == 3) return s.front ? m.emission : V(0, 0, 0);
-    if (!is_delta(&m) && fr->sampling_mode != 3u)
+    if (!is_delta(&m) && fr->sampling_mode != 3u) {
+        rng_event(&seed, 1u, Z_NEE);
         radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), fr->sampling_mode <= 1u && limit > 0, &seed);
+    }
     radiance = add(radiance, continue_path(sc, fr, &l, s, m, dir, &seed));
     if (!(isfinite(radiance.x) && isfinite(radiance.y) && isfinite(radiance.z))) radiance = V(0, 0, 0);
     return radiance;
 }
 
-void SUFFIX(vo_render)(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
+void SUFFIX(vo_render)(const vo_render_scene *sc, const vo_frame *frame, uint32_t sample_index0, uint32_t samples,
                        const uint32_t *pixels, uint32_t npixels, int threads, double *out_rgb) {
+    vo_frame f = *frame;
+    f.sample_index0 = sample_index0;  /* the accumulation starts at sample_index0 */
+    const vo_frame *fr = &f;
     const cam_t cam = camera_of(fr, fr->eye);
     (void)vo_energy_table(0);  /* bake the BSDF tables outside the parallel region */
 #pragma omp parallel for schedule(dynamic, 4) num_threads(threads > 0 ? threads : omp_get_max_threads())
@@ -618,11 +667,11 @@ static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const 
                            uint32_t sample_index, const float *params, const float *box_lo, const float *box_scale,
                            float inv_unit) {
     const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
-    uint32_t seed = pixel_seed(fr, gx, gy, sample_index);
-    const real u = pixel_u(seed);
+    rng_t seed = {pixel_seed(fr, gx, gy, sample_index), 0u, 0u, 0};  /* the PCG stream only */
+    const real u = pixel_u(seed.pcg);
     lanes_t l = lanes_of(sc, u);
     v3 dir = camera_dir(cam, fr, gx, gy, sample_index);
-    seed = vo_pcg_hash(seed ^ 0x27d4eb2fu);
+    seed.pcg = vo_pcg_hash(seed.pcg ^ 0x27d4eb2fu);
     v3 radiance = V(0, 0, 0);
     surf_t s;
     uint32_t face;
@@ -697,7 +746,7 @@ static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const 
         m = rm;
         if (bounce > 3) {
             real survival = RFMIN(RFMAX(smax(thr), (real)0.05), is_delta(&m) ? (real)0.99 : (real)0.95);
-            if ((real)vo_rand_f(&seed) >= survival) break;
+            if ((real)vo_rand_f(&seed.pcg) >= survival) break;
             thr = sscale(thr, (real)1 / survival);
         }
     }
@@ -823,8 +872,8 @@ static void restir_pass1(const vo_render_scene *sc, const vo_frame *fr, const ca
                          int temporal, uint32_t candidates, uint32_t gx, uint32_t gy, uint32_t sample_index,
                          const di_texel *hist, di_texel *cur, primary_t *prim) {
     const uint32_t pix = gy * fr->width + gx;
-    uint32_t seed = pixel_seed(fr, gx, gy, sample_index);
-    const real u = pixel_u(seed);
+    rng_t seed = {pixel_seed(fr, gx, gy, sample_index), 0u, 0u, 0};  /* the PCG stream only */
+    const real u = pixel_u(seed.pcg);
     const v3 dir = camera_dir(cam, fr, gx, gy, sample_index);
     di_texel *o = cur + pix;
     memset(o, 0, sizeof *o);
@@ -857,7 +906,7 @@ static void restir_pass1(const vo_render_scene *sc, const vo_frame *fr, const ca
             real proposal = c.pdf * light_geometry(sc, s->p, &c);
             real w = proposal > 0 ? ph / proposal : 0;
             wsum += w;
-            if ((real)vo_rand_f(&seed) * wsum < w) sel = c;
+            if ((real)vo_rand_f(&seed.pcg) * wsum < w) sel = c;
         }
     }
     int px, py;
@@ -875,7 +924,7 @@ static void restir_pass1(const vo_render_scene *sc, const vo_frame *fr, const ca
             real w = ph * hW * cm;
             M += cm;
             wsum += w;
-            if ((real)vo_rand_f(&seed) * wsum < w) sel = hs;
+            if ((real)vo_rand_f(&seed.pcg) * wsum < w) sel = hs;
         }
     }
     real ph = target_pdf(sc, s, &m, dir, &sel);
@@ -891,9 +940,9 @@ static v3 restir_pass2(const vo_render_scene *sc, const vo_frame *fr, uint32_t g
                        const di_texel *cur, const primary_t *prim) {
     const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
     const uint32_t pix = gy * fr->width + gx;
-    uint32_t seed = pixel_seed(fr, gx, gy, sample_index);
-    lanes_t l = lanes_of(sc, pixel_u(seed));
-    seed = vo_pcg_hash(seed ^ 0x27d4eb2fu);
+    rng_t seed = {pixel_seed(fr, gx, gy, sample_index), 0u, 0u, 0};  /* the PCG stream only */
+    lanes_t l = lanes_of(sc, pixel_u(seed.pcg));
+    seed.pcg = vo_pcg_hash(seed.pcg ^ 0x27d4eb2fu);
     const di_texel *c = cur + pix;
     v3 radiance = V(0, 0, 0);
     if (c->gpos[3] <= 0.0f) return radiance;
@@ -906,7 +955,7 @@ static v3 restir_pass2(const vo_render_scene *sc, const vo_frame *fr, uint32_t g
         real wsum = target_pdf(sc, &s, &m, dir, &sel) * (real)c->weights[2] * (real)c->weights[1];
         real M = (real)c->weights[1];
         for (int i = 0; i < 4; ++i) {
-            real rx = (real)vo_rand_f(&seed), ry = (real)vo_rand_f(&seed);
+            real rx = (real)vo_rand_f(&seed.pcg), ry = (real)vo_rand_f(&seed.pcg);
             int nx = (int)gx + (int)((rx * (real)2 - (real)1) * (real)16);
             int ny = (int)gy + (int)((ry * (real)2 - (real)1) * (real)16);
             if (nx < 0 || nx >= (int)fr->width || ny < 0 || ny >= (int)fr->height) continue;
@@ -919,7 +968,7 @@ static v3 restir_pass2(const vo_render_scene *sc, const vo_frame *fr, uint32_t g
             real w = target_pdf(sc, &s, &m, dir, &ns) * (real)n->weights[2] * (real)n->weights[1];
             wsum += w;
             if (vo_get_fault() != 3) M += (real)n->weights[1];
-            if ((real)vo_rand_f(&seed) * wsum < w) sel = ns;
+            if ((real)vo_rand_f(&seed.pcg) * wsum < w) sel = ns;
         }
         real fp = target_pdf(sc, &s, &m, dir, &sel);
         real W = (M > 0 && fp > 0) ? wsum / (M * fp) : 0;

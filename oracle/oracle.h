@@ -155,6 +155,9 @@ typedef struct vo_frame {
     float eye[3], target[3], fov;
     int32_t max_depth;
     uint32_t sampling_mode;
+    uint32_t sampler;        /* 0: the PCG stream; 1: the Z++ sampler (MIS path loop only) */
+    uint32_t z_temporal;     /* Z++ temporal model 0..3 (main.swift:497-506) */
+    uint32_t sample_index0;  /* first sample of the accumulation (Z mode: frame = sample - this + 1) */
 } vo_frame;
 
 /* Mean linear-sRGB radiance of `samples` samples (sample indices sample_index0...) for each listed

@@ -134,3 +134,13 @@ void vo_z_draws(uint32_t width, uint32_t height, uint32_t frame_index_, uint32_t
             }
         }
 }
+
+/* One draw of `kind` for a sampler with the given key at *dimension (advanced), with the sampler's
+ * candidate bits (used by the oracle integrator's Z mode). */
+void vo_z_draw(uint32_t key, uint32_t *dimension, uint32_t sub, uint32_t sub_bits, int kind, float *out) {
+    zs s = {key, *dimension, sub, sub_bits};
+    uint32_t v[3];
+    draw(&s, kind, v);
+    *dimension = s.dimension;
+    for (int a = 0; a < kind; ++a) out[a] = zf(v[a]);
+}

@@ -70,3 +70,20 @@ def test_depth_limit_and_energy_monotone():
         frame = dict(width=32, height=32, eye=vt.orbit_eye(), target=(0, -0.1, 0), fov=42.0, max_depth=depth)
         means.append(oracle.render(sc, CORNELL_MATS, frame, 1, 16).mean())
     assert all(b > a for a, b in zip(means, means[1:]))
+
+
+def test_z_sampler_mode_is_an_unbiased_estimator_of_the_same_image():
+    """Z mode (Sampler in Z mode, main.swift:213-293) changes only where the numbers come from: the
+    image it converges to is the PCG mode's.  At 64 spp on a 24x24 Cornell box the two means agree to
+    sampling noise, and the Z image is not a copy of the PCG one."""
+    mesh = vt.cornell()
+    sc = oracle.Scene(mesh.vertices, mesh.faces)
+    base = dict(width=24, height=24, eye=vt.orbit_eye(), target=(0.0, -0.1, 0.0), fov=42.0, max_depth=4)
+    pcg = oracle.render(sc, CORNELL_MATS, base, 1, 64)
+    z = oracle.render(sc, CORNELL_MATS, dict(base, sampler=1, z_temporal=1), 1, 64)
+    assert np.isfinite(z).all()
+    assert abs(z.mean() / pcg.mean() - 1) < 0.03, (z.mean(), pcg.mean())
+    assert not np.allclose(z, pcg)
+    z3 = oracle.render(sc, CORNELL_MATS, dict(base, sampler=1, z_temporal=3), 1, 64)
+    assert abs(z3.mean() / pcg.mean() - 1) < 0.03
+    assert not np.allclose(z3, z)
