@@ -27,21 +27,22 @@ def test_device_energy_tables_match_oracle(gpu_ctx):
         assert np.abs(got - ref).max() <= 1.5 / 65535, nm
 
 
-def _render(ctx, mesh, mats, w, spp, depth, mode=1, candidates=4):
+def _render(ctx, mesh, mats, w, spp, depth, mode=1, candidates=4, sampler=0):
     scene = ctx.set_mesh(mesh)
     scene.set_materials(scenes.vt_materials(mats))
     r = vt.Renderer(ctx)
     r.render(scene, w, w, vt.orbit_eye(), TARGET, 42.0, max_depth=depth, samples=spp, sample_index=1,
-             sampling_mode=mode, candidates=candidates)
+             sampling_mode=mode, candidates=candidates, sampler=sampler)
     img = r.readback()[..., :3].astype(np.float64)
     r.close()
     scene.close()
     return img
 
 
-def _oracle(mesh, mats, w, spp, depth, mode=1):
+def _oracle(mesh, mats, w, spp, depth, mode=1, sampler=0):
     sc = oracle.Scene(mesh.vertices, mesh.faces)
-    frame = dict(width=w, height=w, eye=vt.orbit_eye(), target=TARGET, fov=42.0, max_depth=depth, sampling_mode=mode)
+    frame = dict(width=w, height=w, eye=vt.orbit_eye(), target=TARGET, fov=42.0, max_depth=depth, sampling_mode=mode,
+                 sampler=sampler)
     if mode == 0:
         return oracle.render_restir(sc, mats, frame, 1, spp)
     return oracle.render(sc, mats, frame, 1, spp).reshape(w, w, 3)
@@ -64,6 +65,16 @@ def test_materials_match_oracle(gpu_ctx, variant, tol, min_close, depth):
     got = _render(gpu_ctx, mesh, mats, 128, 32, depth)
     ref = _oracle(mesh, mats, 128, 32, depth)
     assert _compare(got, ref, f"{variant} depth {depth}", tol) > min_close
+
+
+@pytest.mark.parametrize("variant,tol,min_close", [("opaque", 1e-3, 0.97), ("layered", 5e-3, 0.9), ("delta", 5e-3, 0.9)])
+def test_materials_in_z_sampler_mode_match_oracle(gpu_ctx, variant, tol, min_close):
+    """Z mode through every BSDF sampler: the layered OpenPBR lobes (one 3D constituent per scatter),
+    Lambert, and the dielectric's 1D draws."""
+    mesh, mats = scenes.materials_cornell(variant)
+    got = _render(gpu_ctx, mesh, mats, 128, 32, 6, sampler=1)
+    ref = _oracle(mesh, mats, 128, 32, 6, sampler=1)
+    assert _compare(got, ref, f"z {variant}", tol) > min_close
 
 
 @pytest.mark.parametrize("mode", [2, 3])

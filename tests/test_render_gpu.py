@@ -94,23 +94,24 @@ def _multi_light_cornell():
     return scenes.multi_light_cornell()
 
 
-@pytest.mark.parametrize("tree", [True, False])
-def test_multi_light_scene_matches_reference_integrator(gpu_ctx, tree):
+@pytest.mark.parametrize("tree,sampler", [(True, 0), (False, 0), (True, 1), (False, 1)])
+def test_multi_light_scene_matches_reference_integrator(gpu_ctx, tree, sampler):
     mesh, mats = _multi_light_cornell()
     w, h, spp = 96, 96, 32
     scene = gpu_ctx.set_mesh(mesh)
     scene.set_light_tree(tree)
     scene.set_materials([vt.material(m["type"], m["albedo"], m.get("emission", (0, 0, 0))) for m in mats])
     r = vt.Renderer(gpu_ctx)
-    r.render(scene, w, h, vt.orbit_eye(), (0.0, -0.1, 0.0), 42.0, max_depth=8, samples=spp, sample_index=1)
+    r.render(scene, w, h, vt.orbit_eye(), (0.0, -0.1, 0.0), 42.0, max_depth=8, samples=spp, sample_index=1,
+             sampler=sampler)
     got = r.readback()[..., :3].astype(np.float64)
     r.close()
     scene.close()
     sc = oracle.Scene(mesh.vertices, mesh.faces)
-    frame = dict(width=w, height=h, eye=vt.orbit_eye(), target=(0.0, -0.1, 0.0), fov=42.0, max_depth=8)
+    frame = dict(width=w, height=h, eye=vt.orbit_eye(), target=(0.0, -0.1, 0.0), fov=42.0, max_depth=8, sampler=sampler)
     ref = oracle.render(sc, mats, frame, 1, spp, light_tree=tree).reshape(h, w, 3)
     rel = abs(got.mean() / ref.mean() - 1)
-    print(f"multi-light tree={tree}: gpu {got.mean():.6f} oracle {ref.mean():.6f} rel {rel:.2e}", flush=True)
+    print(f"multi-light tree={tree} sampler={sampler}: gpu {got.mean():.6f} oracle {ref.mean():.6f} rel {rel:.2e}", flush=True)
     assert rel < 1e-3
     close = np.isclose(got, ref, rtol=1e-3, atol=1e-4).all(axis=2)
     assert close.mean() > 0.97, close.mean()
