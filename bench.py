@@ -221,7 +221,8 @@ def bench_learned_cache(args, ctx):
 def bench_restir(args, ctx):
     """Config 3: the emitter terrain with Lambert and GGX surfaces (~10M triangles, 4096 emitters in
     the light tree), 1920 x 1080, depth 8, ReSTIR DI (32 candidates, 1 temporal + 1 spatial reuse)
-    against MIS, `--restir-frames` frames of 1 spp each, both timed per frame (synchronised).  Error:
+    against MIS, `--restir-frames` frames of 1 spp each, both timed per frame (synchronised), each with
+    the per-pixel PCG streams and with the Z++ sampler.  Error:
     relMSE on the 64 x 36 stratified pixel subset against the committed FP64 oracle image (MIS at
     65536 spp, tests/golden/terrain_restir_ref.npz); ReSTIR DI converges to the same image."""
     import numpy as np
@@ -239,18 +240,19 @@ def bench_restir(args, ctx):
     view = (workloads.TERRAIN_EYE, workloads.TERRAIN_TARGET, workloads.TERRAIN_FOV)
     out = {"workload": f"emitter-terrain-restir-{len(mesh.faces)}tri-{w}x{h}-depth{depth}",
            "reference": {"kind": "FP64 oracle (MIS)", "spp": ref_spp, "pixels": int(len(pix))}}
-    for name, mode in (("mis", 1), ("restir_di", 0)):
+    # *_z: the same with the Z++ sampler (sampler=1) in place of the per-pixel PCG streams.
+    for name, mode, sampler in (("mis", 1, 0), ("restir_di", 0, 0), ("mis_z", 1, 1), ("restir_di_z", 0, 1)):
         r = vt.Renderer(ctx)
         for k in range(2):
             r.render(scene, w, h, *view, max_depth=depth, samples=1, sample_index=900 + k, sampling_mode=mode,
-                     candidates=32)
+                     candidates=32, sampler=sampler)
         r.reset()
         spent = 0.0
         for k in range(args.restir_frames):
             torch.cuda.synchronize()
             t = time.perf_counter()
             r.render(scene, w, h, *view, max_depth=depth, samples=1, sample_index=1 + k, sampling_mode=mode,
-                     candidates=32)
+                     candidates=32, sampler=sampler)
             torch.cuda.synchronize()
             spent += time.perf_counter() - t
         img = r.readback()[..., :3].astype(np.float64).reshape(-1, 3)[pix]
@@ -261,7 +263,10 @@ def bench_restir(args, ctx):
     out["relmse_ratio_mis_over_restir"] = out["mis"]["relmse"] / out["restir_di"]["relmse"]
     out["efficiency_ratio"] = (out["mis"]["relmse"] * out["mis"]["ms_per_frame"]) / \
         (out["restir_di"]["relmse"] * out["restir_di"]["ms_per_frame"])
-    log(f"restir: MIS {out['mis']}, ReSTIR DI {out['restir_di']}")
+    out["relmse_ratio_pcg_over_z"] = {"mis": out["mis"]["relmse"] / out["mis_z"]["relmse"],
+                                      "restir_di": out["restir_di"]["relmse"] / out["restir_di_z"]["relmse"]}
+    log(f"restir: MIS {out['mis']}, ReSTIR DI {out['restir_di']}, Z sampler: MIS {out['mis_z']}, "
+        f"ReSTIR DI {out['restir_di_z']}")
     return out
 
 

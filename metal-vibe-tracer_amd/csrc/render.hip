@@ -44,7 +44,7 @@ struct Rng {
     ZSampler zs;
     bool z;
 };
-enum : uint32_t { kZBsdf = 0u, kZNee = 1u, kZRoulette = 2u, kZWavelength = 12u };
+enum : uint32_t { kZBsdf = 0u, kZNee = 1u, kZRoulette = 2u, kZDi = 4u, kZWavelength = 12u };
 __device__ __forceinline__ float rand_f(Rng &r) { return r.z ? r.zs.rand1() : rand_f(r.pcg); }
 __device__ __forceinline__ void rand_f3(uint32_t &seed, float &a, float &b, float &c) {
     a = rand_f(seed);
@@ -928,7 +928,12 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
         gnorm = pack_half4(s.n.x, s.n.y, s.n.z, (float)m.type);
         if (m.type == kDiffuse) {
             const Spectral sp{sc.icdf, reinterpret_cast<const float4 *>(sc.cmf), reinterpret_cast<const float4 *>(sc.basis)};
-            uint32_t seed = pix ^ (f.sample_index0 * 1999999973u);
+            // The PCG stream for the resampling decisions; in Z mode each light candidate i is
+            // candidate i of the vertex-1 Z_DI event under the pixel's key (restir_di_initial,
+            // main.swift:4635-4652), so the candidates are stratified.
+            Rng rng{pix ^ (f.sample_index0 * 1999999973u), ZSampler{0u, 0u, 0u, 0u}, f.sampler == 1u};
+            if (rng.z) rng.zs.key = z_pixel_key(pix % f.width, pix / f.width, f.frame0 + 1u, f.sample_index0, f.z_temporal);
+            const uint32_t bits = ra.candidates > 1u ? 32u - (uint32_t)__clz(ra.candidates - 1u) : 0u;
             const float u = q.wl[pix];
             const Lanes lanes = lanes_of(sp, u);
             const S4 rho_pi = scale(reflectance(sp, lanes, m.lagrange), 1.0f / kPi);
@@ -937,10 +942,11 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
             if (cached && ra.candidates != 0u) lt_top_importances<256>(sc, top, s.p, top_l, top_r);
             const V3 sp_p = s.p;
             for (uint32_t i = 0; i < ra.candidates; ++i) {
+                if (rng.z) rng.zs.candidate(1u, kZDi, i, bits);
                 const LightSample ls =
-                    cached ? sample_emitter_via(sc, sp_p, seed,
+                    cached ? sample_emitter_via(sc, sp_p, rng,
                                                 [&](float u, float &pmf) { return lt_sample_top<256>(sc, sp_p, u, pmf, top, top_l, top_r); })
-                           : sample_emitter(sc, sp_p, seed);
+                           : sample_emitter(sc, sp_p, rng);
                 DiSample c;
                 c.position = ls.position;
                 c.emission = ls.emission;
@@ -953,7 +959,7 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
                     const float proposal = c.pdf * ls.geometry;
                     const float w = proposal > 0.0f ? ph / proposal : 0.0f;
                     wsum += w;
-                    if (rand_f(seed) * wsum < w) sel = c;
+                    if (rand_f(rng.pcg) * wsum < w) sel = c;
                 }
             }
             if (ra.temporal) {
@@ -982,7 +988,7 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
                                 const float wt = ph * hw.z * cm;
                                 M += cm;
                                 wsum += wt;
-                                if (rand_f(seed) * wsum < wt) sel = hs;
+                                if (rand_f(rng.pcg) * wsum < wt) sel = hs;
                             }
                         }
                     }

@@ -229,14 +229,14 @@ static int occluded(const vo_render_scene *sc, v3 o, v3 d, real tmin, real tmax)
 /* The pixel's random numbers (Sampler, main.swift:213-293): the PCG stream, or in Z mode the Z++
  * sampler's draws (oracle/zsampler.c) under the pixel's key, each sampling event (a path vertex and a
  * stream) owning 32 dimensions. */
-typedef struct { uint32_t pcg, key, dim; int z; } rng_t;
+typedef struct { uint32_t pcg, key, dim; int z; uint32_t sub, sub_bits; } rng_t;
 void vo_z_draw(uint32_t key, uint32_t *dimension, uint32_t sub, uint32_t sub_bits, int kind, float *out);
 uint32_t vo_z_pixel_key(uint32_t x, uint32_t y, uint32_t frame_index, uint32_t sample_index, uint32_t model);
-enum { Z_BSDF = 0, Z_NEE = 1, Z_ROULETTE = 2, Z_WAVELENGTH = 12 };
+enum { Z_BSDF = 0, Z_NEE = 1, Z_ROULETTE = 2, Z_DI = 4, Z_WAVELENGTH = 12 };
 static real rr(rng_t *r) {
     if (!r->z) return (real)vo_rand_f(&r->pcg);
     float f;
-    vo_z_draw(r->key, &r->dim, 0u, 0u, 1, &f);
+    vo_z_draw(r->key, &r->dim, r->sub, r->sub_bits, 1, &f);
     return (real)f;
 }
 static void rr3(rng_t *r, real *a, real *b, real *c) {
@@ -247,13 +247,22 @@ static void rr3(rng_t *r, real *a, real *b, real *c) {
         return;
     }
     float f[3];
-    vo_z_draw(r->key, &r->dim, 0u, 0u, 3, f);
+    vo_z_draw(r->key, &r->dim, r->sub, r->sub_bits, 3, f);
     *a = (real)f[0];
     *b = (real)f[1];
     *c = (real)f[2];
 }
 /* sampler_event (main.swift:293-295). */
-static void rng_event(rng_t *r, uint32_t vertex, uint32_t stream) { r->dim = (vertex * 16u + stream) * 32u; }
+static void rng_event(rng_t *r, uint32_t vertex, uint32_t stream) {
+    r->dim = (vertex * 16u + stream) * 32u;
+    r->sub = r->sub_bits = 0u;
+}
+/* sampler_candidate (main.swift:296-299): candidate `index` of `bits` low frame-index bits. */
+static void rng_candidate(rng_t *r, uint32_t vertex, uint32_t stream, uint32_t index, uint32_t bits) {
+    rng_event(r, vertex, stream);
+    r->sub = index;
+    r->sub_bits = bits;
+}
 
 typedef struct { v3 position, wi, emission; real pdf; uint32_t light; } light_sample;
 static light_sample sample_direct_light(const vo_render_scene *sc, v3 p, rng_t *seed) {
@@ -872,8 +881,18 @@ static void restir_pass1(const vo_render_scene *sc, const vo_frame *fr, const ca
                          int temporal, uint32_t candidates, uint32_t gx, uint32_t gy, uint32_t sample_index,
                          const di_texel *hist, di_texel *cur, primary_t *prim) {
     const uint32_t pix = gy * fr->width + gx;
-    rng_t seed = {pixel_seed(fr, gx, gy, sample_index), 0u, 0u, 0};  /* the PCG stream only */
-    const real u = pixel_u(seed.pcg);
+    rng_t seed = {pixel_seed(fr, gx, gy, sample_index), 0u, 0u, fr->sampler == 1u, 0u, 0u};
+    real u = pixel_u(seed.pcg);
+    if (seed.z) {
+        /* Z mode: the wavelength number as in sample_pixel; the light candidates below are
+         * candidates of the vertex-1 Z_DI event, the resampling decisions stay on the PCG stream
+         * (restir_di_initial, main.swift:4635-4652). */
+        seed.key = vo_z_pixel_key(gx, gy, sample_index - fr->sample_index0 + 1u, sample_index, fr->z_temporal);
+        rng_event(&seed, 0u, Z_WAVELENGTH);
+        u = rr(&seed);
+    }
+    uint32_t bits = 0u;
+    while ((1u << bits) < candidates) ++bits;
     const v3 dir = camera_dir(cam, fr, gx, gy, sample_index);
     di_texel *o = cur + pix;
     memset(o, 0, sizeof *o);
@@ -898,6 +917,7 @@ static void restir_pass1(const vo_render_scene *sc, const vo_frame *fr, const ca
     memset(&sel, 0, sizeof sel);
     real wsum = 0, M = 0;
     for (uint32_t i = 0; i < candidates; ++i) {
+        if (seed.z) rng_candidate(&seed, 1u, Z_DI, i, bits);
         light_sample ls = sample_direct_light(sc, s->p, &seed);
         di_sample c = {ls.position, ls.emission, ls.light, ls.pdf, u};
         if (vo_get_fault() != 4 || c.pdf > 0) M += 1;
@@ -940,8 +960,14 @@ static v3 restir_pass2(const vo_render_scene *sc, const vo_frame *fr, uint32_t g
                        const di_texel *cur, const primary_t *prim) {
     const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
     const uint32_t pix = gy * fr->width + gx;
-    rng_t seed = {pixel_seed(fr, gx, gy, sample_index), 0u, 0u, 0};  /* the PCG stream only */
-    lanes_t l = lanes_of(sc, pixel_u(seed.pcg));
+    rng_t seed = {pixel_seed(fr, gx, gy, sample_index), 0u, 0u, fr->sampler == 1u, 0u, 0u};
+    real u = pixel_u(seed.pcg);
+    if (seed.z) {
+        seed.key = vo_z_pixel_key(gx, gy, sample_index - fr->sample_index0 + 1u, sample_index, fr->z_temporal);
+        rng_event(&seed, 0u, Z_WAVELENGTH);
+        u = rr(&seed);
+    }
+    lanes_t l = lanes_of(sc, u);
     seed.pcg = vo_pcg_hash(seed.pcg ^ 0x27d4eb2fu);
     const di_texel *c = cur + pix;
     v3 radiance = V(0, 0, 0);
@@ -984,6 +1010,7 @@ static v3 restir_pass2(const vo_render_scene *sc, const vo_frame *fr, uint32_t g
             }
         }
     } else if (!is_delta(&m)) {
+        rng_event(&seed, 1u, Z_NEE);
         radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), limit > 0, &seed);
     }
     radiance = add(radiance, continue_path(sc, fr, &l, s, m, dir, &seed));
@@ -1002,6 +1029,9 @@ void SUFFIX(vo_render_restir)(const vo_render_scene *sc, const vo_frame *fr, uin
         return;
     }
     const int nt = threads > 0 ? threads : omp_get_max_threads();
+    vo_frame frame = *fr;
+    frame.sample_index0 = sample_index0;  /* the accumulation starts at sample_index0 */
+    fr = &frame;
     cam_t prev = camera_of(fr, fr->eye);
     (void)vo_energy_table(0);
     for (uint32_t k = 0; k < samples; ++k) {

@@ -159,8 +159,6 @@ def test_z_sampler_mode_rejections(gpu_ctx):
     with pytest.raises(vt.VTError):
         r.render(*args, sampler=1, z_temporal=4)
     with pytest.raises(vt.VTError):
-        r.render(*args, sampler=1, sampling_mode=0)
-    with pytest.raises(vt.VTError):
         r.render(*args, sampler=1, cache=True)
     r.close()
     scene.close()

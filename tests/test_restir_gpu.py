@@ -19,7 +19,7 @@ TARGET = (0.0, -0.1, 0.0)
 
 
 def _gpu(ctx, mesh, mats, w, samples, depth, candidates=4, tree=True, eyes=None, chunks=None, sample_index=1,
-         renderer=None):
+         renderer=None, sampler=0):
     scene = ctx.set_mesh(mesh)
     scene.set_light_tree(tree)
     scene.set_materials([vt.material(m["type"], m["albedo"], m.get("emission", (0, 0, 0))) for m in mats])
@@ -27,12 +27,12 @@ def _gpu(ctx, mesh, mats, w, samples, depth, candidates=4, tree=True, eyes=None,
     if eyes is not None:
         for k in range(samples):
             r.render(scene, w, w, tuple(map(float, eyes[k])), TARGET, 42.0, max_depth=depth, samples=1,
-                     sample_index=sample_index + k, sampling_mode=0, candidates=candidates)
+                     sample_index=sample_index + k, sampling_mode=0, candidates=candidates, sampler=sampler)
     else:
         s = sample_index
         for c in chunks or [samples]:
             r.render(scene, w, w, vt.orbit_eye(), TARGET, 42.0, max_depth=depth, samples=c, sample_index=s,
-                     sampling_mode=0, candidates=candidates)
+                     sampling_mode=0, candidates=candidates, sampler=sampler)
             s += c
     img = r.readback()[..., :3].astype(np.float64)
     if renderer is None:
@@ -41,9 +41,9 @@ def _gpu(ctx, mesh, mats, w, samples, depth, candidates=4, tree=True, eyes=None,
     return img
 
 
-def _oracle(mesh, mats, w, samples, depth, candidates=4, tree=True, eyes=None, sample_index=1):
+def _oracle(mesh, mats, w, samples, depth, candidates=4, tree=True, eyes=None, sample_index=1, sampler=0):
     sc = oracle.Scene(mesh.vertices, mesh.faces)
-    frame = dict(width=w, height=w, eye=vt.orbit_eye(), target=TARGET, fov=42.0, max_depth=depth)
+    frame = dict(width=w, height=w, eye=vt.orbit_eye(), target=TARGET, fov=42.0, max_depth=depth, sampler=sampler)
     return oracle.render_restir(sc, mats, frame, sample_index, samples, candidates=candidates, eyes=eyes,
                                 light_tree=tree)
 
@@ -74,12 +74,16 @@ def test_frames_with_history_match_oracle(gpu_ctx, depth):
     assert close > 0.99
 
 
-@pytest.mark.parametrize("tree,candidates", [(True, 4), (False, 4), (True, 32)])
-def test_many_lights_match_oracle(gpu_ctx, tree, candidates):
+@pytest.mark.parametrize("tree,candidates,sampler", [(True, 4, 0), (False, 4, 0), (True, 32, 0), (True, 4, 1),
+                                                     (False, 4, 1), (True, 32, 1), (True, 3, 1)])
+def test_many_lights_match_oracle(gpu_ctx, tree, candidates, sampler):
+    """sampler=1: the light candidates are candidates of the vertex-1 Z_DI event (ceil(log2 M) index
+    bits), the decisions the PCG stream's (restir_di_initial, main.swift:4635-4652); 16 frames over two
+    calls, so the pixel key's frame numbering crosses a call."""
     mesh, mats = scenes.multi_light_cornell()
-    got = _gpu(gpu_ctx, mesh, mats, 96, 16, 3, candidates=candidates, tree=tree)
-    ref = _oracle(mesh, mats, 96, 16, 3, candidates=candidates, tree=tree)
-    assert _compare(got, ref, f"restir many lights tree={tree} candidates={candidates}") > 0.99
+    got = _gpu(gpu_ctx, mesh, mats, 96, 16, 3, candidates=candidates, tree=tree, sampler=sampler, chunks=[5, 11])
+    ref = _oracle(mesh, mats, 96, 16, 3, candidates=candidates, tree=tree, sampler=sampler)
+    assert _compare(got, ref, f"restir many lights tree={tree} candidates={candidates} sampler={sampler}") > 0.99
 
 
 def test_moving_camera_matches_oracle(gpu_ctx):

@@ -127,3 +127,16 @@ def test_temporal_history_is_used_and_moving_camera_stays_unbiased():
     a = np.mean([oracle.render(sc, CORNELL, dict(fr, eye=tuple(map(float, eyes[k]))), 1 + k, 1).mean()
                  for k in range(n)])
     assert abs(b / a - 1) < 3e-3, (a, b)
+
+
+def test_restir_in_z_sampler_mode_converges_to_the_same_image():
+    """Z mode changes where ReSTIR's candidates come from (stratified candidates of one event), not
+    what the estimator converges to: 32 frames agree with the PCG mode's mean to sampling noise."""
+    mesh, mats = scenes.multi_light_cornell()
+    sc = oracle.Scene(mesh.vertices, mesh.faces)
+    frame = dict(width=32, height=32, eye=vt.orbit_eye(), target=(0.0, -0.1, 0.0), fov=42.0, max_depth=3)
+    pcg = oracle.render_restir(sc, mats, frame, 1, 32, candidates=4)
+    z = oracle.render_restir(sc, mats, dict(frame, sampler=1), 1, 32, candidates=4)
+    assert np.isfinite(z).all()
+    assert abs(z.mean() / pcg.mean() - 1) < 0.03, (z.mean(), pcg.mean())
+    assert not np.allclose(z, pcg)
