@@ -172,8 +172,7 @@ typedef struct vt_frame_params {
                                     1: the Z++ sampler (main.swift:122-305): the pixel's key and each
                                     draw's sampling event as in the reference's Z mode (ReSTIR's light
                                     candidates are candidates of one event; its resampling
-                                    decisions stay on the PCG stream).  Z mode is supported
-                                    without the radiance cache */
+                                    decisions stay on the PCG stream) */
     uint32_t z_temporal;         /* Z++ temporal model 0..3 of the pixel key (main.swift:497-506; the
                                     reference's default is 1) */
 } vt_frame_params;

@@ -451,8 +451,6 @@ vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params
         if (p->material_sort > 1) return fail(VT_ERROR_INVALID_ARGUMENT, "material_sort must be 0 or 1");
         if (p->sampler > 1 || p->z_temporal > 3)
             return fail(VT_ERROR_INVALID_ARGUMENT, "sampler must be 0 or 1 and z_temporal 0..3");
-        if (p->sampler == 1 && p->radiance_cache)
-            return fail(VT_ERROR_UNSUPPORTED, "the Z sampler is supported without the radiance cache");
         if (samples == 0) return VT_OK;
         check(hipSetDevice(r->ctx->device), "hipSetDevice");
         hipStream_t s = (hipStream_t)stream;

@@ -676,8 +676,13 @@ static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const 
                            uint32_t sample_index, const float *params, const float *box_lo, const float *box_scale,
                            float inv_unit) {
     const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
-    rng_t seed = {pixel_seed(fr, gx, gy, sample_index), 0u, 0u, 0};  /* the PCG stream only */
-    const real u = pixel_u(seed.pcg);
+    rng_t seed = {pixel_seed(fr, gx, gy, sample_index), 0u, 0u, fr->sampler == 1u, 0u, 0u};
+    real u = pixel_u(seed.pcg);
+    if (seed.z) {  /* Z mode: the events of sample_pixel / continue_path */
+        seed.key = vo_z_pixel_key(gx, gy, sample_index - fr->sample_index0 + 1u, sample_index, fr->z_temporal);
+        rng_event(&seed, 0u, Z_WAVELENGTH);
+        u = rr(&seed);
+    }
     lanes_t l = lanes_of(sc, u);
     v3 dir = camera_dir(cam, fr, gx, gy, sample_index);
     seed.pcg = vo_pcg_hash(seed.pcg ^ 0x27d4eb2fu);
@@ -687,7 +692,10 @@ static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const 
     if (!trace(sc, cam->eye, dir, ray_t_min(cam->eye), &s, &face)) return radiance;
     mat_t m = mat_of(sc, s.slot);
     if (m.type == 3) return s.front ? m.emission : V(0, 0, 0);
-    if (!is_delta(&m)) radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), limit > 0, &seed);
+    if (!is_delta(&m)) {
+        rng_event(&seed, 1u, Z_NEE);
+        radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), limit > 0, &seed);
+    }
     s4 thr = S4(1);
     int sdepth = 0;
     for (int bounce = 1;; ++bounce) {
@@ -697,6 +705,7 @@ static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const 
         v3 nd;
         s4 bw;
         real bpdf;
+        rng_event(&seed, (uint32_t)bounce, Z_BSDF);
         if (!sample_bsdf(sc, &l, &s, &m, dir, &seed, &nd, &bw, &bpdf)) break;
         thr = smul(thr, bw);
         if (!(isfinite(thr.v[0]) && isfinite(thr.v[1]) && isfinite(thr.v[2]) && isfinite(thr.v[3])) || smax(thr) <= 0) break;
@@ -749,13 +758,16 @@ static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const 
             radiance = add(radiance, spectrum_rgb(sc, &l, v));
             break;
         }
-        if (!is_delta(&rm))
+        if (!is_delta(&rm)) {
+            rng_event(&seed, (uint32_t)bounce + 1u, Z_NEE);
             radiance = add(radiance, nee(sc, &l, &r, &rm, neg(dir), thr, sdepth < limit, &seed));
+        }
         s = r;
         m = rm;
         if (bounce > 3) {
             real survival = RFMIN(RFMAX(smax(thr), (real)0.05), is_delta(&m) ? (real)0.99 : (real)0.95);
-            if ((real)vo_rand_f(&seed.pcg) >= survival) break;
+            rng_event(&seed, (uint32_t)bounce + 1u, Z_ROULETTE);
+            if (rr(&seed) >= survival) break;
             thr = sscale(thr, (real)1 / survival);
         }
     }
@@ -766,6 +778,9 @@ static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const 
 void SUFFIX(vo_render_nrc)(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
                            uint32_t period, const float *params, const float *box_lo, const float *box_scale, int threads,
                            double *out_rgb) {
+    vo_frame frame = *fr;
+    frame.sample_index0 = sample_index0;  /* the accumulation starts at sample_index0 */
+    fr = &frame;
     const cam_t cam = camera_of(fr, fr->eye);
     (void)vo_energy_table(0);
     const int64_t npix = (int64_t)fr->width * fr->height;

@@ -16,7 +16,10 @@ pytestmark = pytest.mark.gpu
 TARGET = (0.0, -0.1, 0.0)
 
 
-def test_frozen_cache_matches_oracle(gpu_ctx):
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_frozen_cache_matches_oracle(gpu_ctx, sampler):
+    """sampler=1: the Z++ sampler's events as on the plain path; the cache adds no draws of its own
+    (its training subset is a hash of pixel and sample)."""
     mesh = vt.cornell()
     rng = np.random.default_rng(3)
     p = rng.uniform(-0.12, 0.12, onrc.N_PARAMS)
@@ -31,15 +34,16 @@ def test_frozen_cache_matches_oracle(gpu_ctx):
     r = vt.Renderer(gpu_ctx)
     r.set_cache(cache)
     w, spp = 96, 16
-    r.render(scene, w, w, vt.orbit_eye(), TARGET, 42.0, max_depth=8, samples=spp, sample_index=1, cache=2)
+    r.render(scene, w, w, vt.orbit_eye(), TARGET, 42.0, max_depth=8, samples=spp, sample_index=1, cache=2,
+             sampler=sampler)
     got = r.readback()[..., :3].astype(np.float64)
     assert np.array_equal(cache.params, params)             # frozen
     sc = oracle.Scene(mesh.vertices, mesh.faces)
-    frame = dict(width=w, height=w, eye=vt.orbit_eye(), target=TARGET, fov=42.0, max_depth=8)
+    frame = dict(width=w, height=w, eye=vt.orbit_eye(), target=TARGET, fov=42.0, max_depth=8, sampler=sampler)
     ref = oracle.render_nrc(sc, scenes.CORNELL_MATS, frame, 1, spp, params)
     rel = abs(got.mean() / ref.mean() - 1)
     close = np.isclose(got, ref, rtol=1e-2, atol=1e-4).all(axis=2).mean()
-    print(f"frozen cache: gpu {got.mean():.6f} oracle {ref.mean():.6f} rel {rel:.2e} close {close:.4f}", flush=True)
+    print(f"frozen cache sampler {sampler}: gpu {got.mean():.6f} oracle {ref.mean():.6f} rel {rel:.2e} close {close:.4f}", flush=True)
     assert rel < 3e-3 and close > 0.95
     r.close()
     scene.close()

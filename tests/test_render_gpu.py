@@ -158,7 +158,5 @@ def test_z_sampler_mode_rejections(gpu_ctx):
         r.render(*args, sampler=2)
     with pytest.raises(vt.VTError):
         r.render(*args, sampler=1, z_temporal=4)
-    with pytest.raises(vt.VTError):
-        r.render(*args, sampler=1, cache=True)
     r.close()
     scene.close()
