@@ -1,16 +1,22 @@
 #!/usr/bin/env python3
-"""Closest-hit throughput benchmark on one MI355X.
+"""Closest-hit throughput and path-tracing benchmarks on one MI355X.
 
-Workload (one step): the traversal-stress scene -- 1024 icospheres of 1280 triangles each
-(1,310,720 triangles), radius lognormal(0, 0.5), centres uniform in [-50, 50]^3, seed 7, 8-wide
-SAH BVH -- traced with 2,097,152 camera rays (2048 x 1024 jittered pinhole) and 2,097,152 cosine
-bounce rays spawned from those rays' hits: two vt_trace_closest launches over device-resident
-SoA ray buffers.  `value` = rays traced per second over the K timed steps (Mrays/s).
+Headline workload (one step): the large scene -- an fBm heightfield on a 2237 x 2237 vertex grid
+(8 octaves, gain 0.5, seed 3; 9,999,392 triangles) under an 8-wide SAH BVH -- traced with 2,097,152
+camera rays (2048 x 1024 jittered pinhole framing the surface: 99% of them hit) and 2,097,152 cosine
+bounce rays spawned from those rays' hits: two vt_trace_closest launches over device-resident SoA
+ray buffers.  `value` = rays traced per second over the K timed steps (Mrays/s).
 
-Also reported: the roofline of the traversal kernel (algorithmic node + triangle bytes per
-launch / average launch time measured with HIP events on the launch stream, against 8 TB/s HBM),
-and the CPU baseline: the oracle's restatement of the reference's 4-wide traversal timed on this
-host's cores over a strided sample of the same rays.
+Also reported for the headline scene: the roofline of the traversal kernel (algorithmic node +
+triangle bytes per launch / average launch time measured with HIP events on the launch stream,
+against 8 TB/s HBM, with the measured HBM traffic of the committed rocprofv3 profile), the BVH build
+time, the closest-hit rate on the reference's own incoherent ray recipe (4,194,304 rays, origins
+uniform in the scene's box, directions uniform on the sphere, best of 3 passes after a warm-up,
+scripts/measure_acceleration.swift:176-187, 237-269), and the CPU baseline: the oracle's restatement
+of the reference's 4-wide traversal, built -O3 -march=native on this host, over every ray of the
+step, median of 5 runs after a warm-up.  The traversal-stress scene (1024 icospheres, 1,310,720
+triangles) follows as a secondary block with the same figures; then the integrator, learned-cache,
+ReSTIR DI and batched-rendering workloads.
 """
 import argparse
 import json
@@ -31,7 +37,8 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-TRAFFIC_PROFILES = {"icosphere-field-1024x1280": "profiles/trace_closest_icosphere_field_1M3_traffic.json"}
+TRAFFIC_PROFILES = {"fbm-heightfield-2237^2": "profiles/trace_closest_heightfield_10M_traffic.json",
+                    "icosphere-field-1024x1280": "profiles/trace_closest_icosphere_field_1M3_traffic.json"}
 
 
 def measured_traffic(name):
@@ -55,8 +62,137 @@ def workload_stress():
 def workload_heightfield():
     import vt
     mesh = vt.heightfield(grid=2237, octaves=8, gain=0.5, half_extent=100.0, height=12.0, seed=3)
-    prim = vt.camera_rays((0.0, 45.0, -130.0), (0.0, 0.0, 10.0), 50.0, 2048, 1024, 3)
+    prim = vt.camera_rays((0.0, 70.0, -70.0), (0.0, 0.0, 0.0), 40.0, 2048, 1024, 3)
     return "fbm-heightfield-2237^2", mesh, prim, 3
+
+
+def measure_traversal(args, ctx, name, mesh, prim, seed):
+    """One traversal workload: K timed steps of camera + bounce closest-hit launches, per-launch
+    kernel times, roofline, incoherent-ray rate, BVH build time and the CPU baseline."""
+    import numpy as np
+    import torch
+
+    import vt
+    dev = torch.device("cuda:0")
+    log(f"workload {name}: {len(mesh.faces)} triangles, {prim.shape[1]} primary rays")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    scene = ctx.set_mesh(mesh)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    info = scene.info
+    log(f"bvh: {info}, built and uploaded in {build_s:.2f} s")
+    stream = torch.cuda.current_stream()
+    prim_d = torch.from_numpy(prim).to(dev)
+    h = scene.trace_closest(prim_d, stream=stream)
+    torch.cuda.synchronize()
+    hp = {k: v.cpu().numpy() for k, v in h.items()}
+    bounce = vt.bounce_rays(mesh, prim, hp["face"], hp["b1"], hp["b2"], seed)
+    bounce_d = torch.from_numpy(bounce).to(dev)
+    n_p, n_b = prim.shape[1], bounce.shape[1]
+    out_p = {k: torch.empty(n_p, dtype=torch.int32 if k == "face" else torch.float32, device=dev)
+             for k in ("t", "b1", "b2", "face")}
+    out_b = {k: torch.empty(n_b, dtype=torch.int32 if k == "face" else torch.float32, device=dev)
+             for k in ("t", "b1", "b2", "face")}
+
+    def step(ev=None):
+        if ev:
+            ev[0].record(stream)
+        scene.trace_closest(prim_d, out=out_p, stream=stream)
+        if ev:
+            ev[1].record(stream)
+        scene.trace_closest(bounce_d, out=out_b, stream=stream)
+        if ev:
+            ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    # Timed region: exactly K steps, bracketed by device synchronisation.
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = (n_p + n_b) * args.steps / elapsed / 1e6
+
+    # Per-launch kernel durations (HIP events on the launch stream), a separate pass.
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    t_prim = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    t_bnc = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+
+    # Algorithmic bytes: node records fetched x node size + triangle records tested x their size.
+    np_, tp_ = scene.trace_stats(prim_d, stream=stream)
+    nb_, tb_ = scene.trace_stats(bounce_d, stream=stream)
+    sn, st = info["node_bytes"], info["tri_bytes"]
+    bytes_p = np_ * sn + tp_ * st
+    bytes_b = nb_ * sn + tb_ * st
+    achieved = (bytes_p + bytes_b) / ((t_prim + t_bnc) * 1e-3) / 1e9
+    hits_p = int((out_p["face"] >= 0).sum().item())
+    hits_b = int((out_b["face"] >= 0).sum().item())
+    traffic, traffic_src = measured_traffic(name)
+
+    # The reference's incoherent probe: origins uniform in the scene's box, uniform directions,
+    # 2^22 rays, best of 3 passes after a warm-up pass.
+    lo, hi = mesh.vertices[:, :3].min(0), mesh.vertices[:, :3].max(0)
+    inc = vt.random_rays(tuple(float(v) for v in lo), tuple(float(v) for v in hi), 1 << 22, seed)
+    inc_d = torch.from_numpy(inc).to(dev)
+    out_i = {k: torch.empty(inc.shape[1], dtype=torch.int32 if k == "face" else torch.float32, device=dev)
+             for k in ("t", "b1", "b2", "face")}
+    best = float("inf")
+    for k in range(4):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        scene.trace_closest(inc_d, out=out_i, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if k:
+            best = min(best, e0.elapsed_time(e1))
+    hits_i = int((out_i["face"] >= 0).sum().item())
+    incoherent = {"rays": int(inc.shape[1]), "hits": hits_i, "ms": round(best, 4),
+                  "mrays_per_s": round(inc.shape[1] / best / 1e3, 1),
+                  "recipe": "origins uniform in the scene box, directions uniform on the sphere, best of 3 after a warm-up"}
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        sample = np.concatenate([prim, bounce], axis=1)
+        c0 = time.perf_counter()
+        osc = oracle.Scene(mesh.vertices, mesh.faces)
+        cpu_build = time.perf_counter() - c0
+        osc.closest(sample, mode=0)                      # warm-up
+        times = []
+        for _ in range(5):
+            c0 = time.perf_counter()
+            osc.closest(sample, mode=0)
+            times.append(time.perf_counter() - c0)
+        cpu = {"value": round(sample.shape[1] / float(np.median(times)) / 1e6, 3), "unit": "Mrays/s",
+               "cores": oracle.max_threads(), "kind": "port",
+               "sample": f"every primary and bounce ray of the step ({sample.shape[1]} rays), reference 4-wide "
+                         f"binned-SAH BVH (leaf 2) and traversal restated in C, gcc -O3 -march=native, OpenMP "
+                         f"over rays, median of 5 after a warm-up",
+               "bvh_build_seconds": round(cpu_build, 2)}
+        del osc
+    scene.close()
+    return {
+        "workload": name, "value": round(value, 2), "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "config": {"workload": name, "triangles": int(len(mesh.faces)), "primary_rays": n_p, "bounce_rays": n_b,
+                   "primary_hits": hits_p, "bounce_hits": hits_b,
+                   "bvh": f"8-wide binned SAH, leaf<={info['leaf_size']}, {sn} B nodes, {st} B triangles, "
+                          f"depth {info['depth']}, {info['node_count']} nodes"},
+        "bvh_build_seconds": round(build_s, 3),
+        "kernel_ms": {"primary": round(t_prim, 4), "bounce": round(t_bnc, 4)},
+        "mrays_per_s": {"primary": round(n_p / t_prim / 1e3, 1), "bounce": round(n_b / t_bnc / 1e3, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": (bytes_p + bytes_b) // 2,
+                     "work": {"node_fetches": np_ + nb_, "triangle_tests": tp_ + tb_, "bytes_per_step": bytes_p + bytes_b}},
+        "incoherent": incoherent,
+        "cpu_baseline": cpu,
+    }
 
 
 CORNELL_MATERIALS = [dict(type=0, albedo=(0.73, 0.73, 0.73)), dict(type=0, albedo=(0.65, 0.05, 0.05)),
@@ -320,9 +456,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["stress", "heightfield"], default="stress")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="rays in the CPU-baseline sample")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--workload", choices=["heightfield", "stress"], default="heightfield",
+                    help="the headline traversal workload (the other one follows as a secondary block)")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the secondary traversal workload")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-integrator", action="store_true", help="skip the path-tracer (Cornell) measurement")
     ap.add_argument("--reference-spp", type=int, default=65536, help="FP64 oracle samples per pixel for relMSE")
     ap.add_argument("--no-cache", action="store_true", help="skip the learned-cache equal-time measurement")
@@ -343,90 +480,20 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     import vt
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda:0")
-    name, mesh, prim, seed = workload_heightfield() if args.workload == "heightfield" else workload_stress()
-    log(f"workload {name}: {len(mesh.faces)} triangles, {prim.shape[1]} primary rays")
-    ctx = vt.Context(0)
-    scene = ctx.set_mesh(mesh)
-    info = scene.info
-    log(f"bvh: {info}")
-    stream = torch.cuda.current_stream()
-    prim_d = torch.from_numpy(prim).to(dev)
-    h = scene.trace_closest(prim_d, stream=stream)
-    torch.cuda.synchronize()
-    hp = {k: v.cpu().numpy() for k, v in h.items()}
-    bounce = vt.bounce_rays(mesh, prim, hp["face"], hp["b1"], hp["b2"], seed)
-    bounce_d = torch.from_numpy(bounce).to(dev)
-    n_p, n_b = prim.shape[1], bounce.shape[1]
-    out_p = {k: torch.empty(n_p, dtype=torch.int32 if k == "face" else torch.float32, device=dev)
-             for k in ("t", "b1", "b2", "face")}
-    out_b = {k: torch.empty(n_b, dtype=torch.int32 if k == "face" else torch.float32, device=dev)
-             for k in ("t", "b1", "b2", "face")}
-
-    def step(ev=None):
-        if ev:
-            ev[0].record(stream)
-        scene.trace_closest(prim_d, out=out_p, stream=stream)
-        if ev:
-            ev[1].record(stream)
-        scene.trace_closest(bounce_d, out=out_b, stream=stream)
-        if ev:
-            ev[2].record(stream)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    # Timed region: exactly K steps, bracketed by device synchronisation.
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    rays_per_step = n_p + n_b
-    value = rays_per_step * args.steps / elapsed / 1e6
-
-    # Per-launch kernel durations (HIP events on the launch stream), a separate pass.
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    for k in range(args.steps):
-        step(evs[k])
-    torch.cuda.synchronize()
-    t_prim = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
-    t_bnc = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
-
-    # Algorithmic bytes: node records fetched x 80 B + triangle records tested x 48 B.
-    np_, tp_ = scene.trace_stats(prim_d, stream=stream)
-    nb_, tb_ = scene.trace_stats(bounce_d, stream=stream)
-    sn, st = info["node_bytes"], info["tri_bytes"]
-    bytes_p = np_ * sn + tp_ * st
-    bytes_b = nb_ * sn + tb_ * st
-    achieved = (bytes_p + bytes_b) / ((t_prim + t_bnc) * 1e-3) / 1e9
-    hits_p = int((out_p["face"] >= 0).sum().item())
-    hits_b = int((out_b["face"] >= 0).sum().item())
-
-    traffic, traffic_src = measured_traffic(name)
-
-    cpu = None
     if not args.no_cpu:
         import oracle
-        stride = max(1, (n_p + n_b) // args.cpu_sample)
-        sample = np.concatenate([prim[:, ::stride], bounce[:, ::stride]], axis=1)
-        osc = oracle.Scene(mesh.vertices, mesh.faces)
-        osc.closest(sample[:, :4096], mode=0)
-        times = []
-        for _ in range(3):
-            c0 = time.perf_counter()
-            osc.closest(sample, mode=0)
-            times.append(time.perf_counter() - c0)
-        cpu = {"value": round(sample.shape[1] / float(np.median(times)) / 1e6, 3), "unit": "Mrays/s",
-               "cores": oracle.max_threads(), "kind": "port",
-               "sample": f"every {stride}th primary and bounce ray ({sample.shape[1]} rays), reference 4-wide "
-                         f"binned-SAH BVH (leaf 2) and traversal restated in C, median of 3"}
+        oracle.load(oracle.native_library())     # the CPU baseline, built for this host
+    ctx = vt.Context(0)
+    first, second = (workload_heightfield, workload_stress) if args.workload == "heightfield" else \
+        (workload_stress, workload_heightfield)
+    head = measure_traversal(args, ctx, *first())
+    secondary = None if args.no_secondary else measure_traversal(args, ctx, *second())
 
     integrator = None if args.no_integrator else bench_integrator(args, ctx)
     learned_cache = None if args.no_cache else bench_learned_cache(args, ctx)
@@ -435,27 +502,25 @@ def main():
 
     line = {
         "metric": METRIC,
-        "value": round(value, 2),
+        "value": head["value"],
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": name, "triangles": int(len(mesh.faces)), "primary_rays": n_p, "bounce_rays": n_b,
-                   "primary_hits": hits_p, "bounce_hits": hits_b, "bvh": f"8-wide binned SAH, leaf<={info['leaf_size']}, "
-                   f"{sn} B nodes, {st} B triangles, depth {info['depth']}, {info['node_count']} nodes"},
-        "kernel_ms": {"primary": round(t_prim, 4), "bounce": round(t_bnc, 4)},
-        "mrays_per_s": {"primary": round(n_p / t_prim / 1e3, 1), "bounce": round(n_b / t_bnc / 1e3, 1)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                     "algorithmic_bytes_per_launch": (bytes_p + bytes_b) // 2,
-                     "work": {"node_fetches": np_ + nb_, "triangle_tests": tp_ + tb_, "bytes_per_step": bytes_p + bytes_b}},
-        "cpu_baseline": cpu,
+        "config": head["config"],
+        "bvh_build_seconds": head["bvh_build_seconds"],
+        "kernel_ms": head["kernel_ms"],
+        "mrays_per_s": head["mrays_per_s"],
+        "roofline": head["roofline"],
+        "incoherent": head["incoherent"],
+        "cpu_baseline": head["cpu_baseline"],
+        "secondary": secondary,
         "integrator": integrator,
         "learned_cache": learned_cache,
         "restir": restir,
@@ -469,7 +534,6 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"copy calibration: 3 x {args.copy_calibration} MiB device-to-device")
-    scene.close()
     ctx.close()
 
 

@@ -44,6 +44,12 @@ void vt_context_destroy(vt_context *ctx);
 /* Message of the last failed call on this thread ("" if none); replaces the NSError built by
  * MaterialLibrary.error (main.swift:9128). */
 const char *vt_last_error(void);
+/* Waits for the context's device and reports faults its kernels raised since the last check (then
+ * clears them): VT_ERROR_DEVICE when a traversal's node-group stack filled, so that some rays ended
+ * before their traversal did.  The stack is sized to the tree's depth, which a traversal never
+ * exceeds; only vt_scene_set_stack_cap can make it happen.  No counterpart in the reference, whose
+ * fixed 64-entry stack fails silently (main.swift:1788-1812). */
+vt_status vt_context_check(vt_context *ctx);
 
 typedef struct vt_build_options {
     uint32_t leaf_size; /* triangles per leaf, 1..3 (0: 3) */
@@ -62,6 +68,13 @@ typedef struct vt_build_options {
 vt_status vt_scene_upload_indexed(vt_context *ctx, const float *vertices, uint32_t vertex_count,
                                   const uint32_t *faces, uint32_t face_count,
                                   const vt_build_options *opts, vt_scene **out);
+/* Replaces MaterialLibrary.setMesh(_ triangles: [MeshTriangle]) (Sources/StudioModel.swift:1480-1494):
+ * `count` 128-byte MeshTriangle records (32 floats: a, b, c, na, nb, nc as float4, uvab, uvc with
+ * uvc.z the material slot) welded into indexed geometry as MeshGeometry(triangles:) does
+ * (vt_mesh_from_triangles), then published as vt_scene_upload_indexed.  Face IDs are the triangles'
+ * positions in the array. */
+vt_status vt_scene_upload_triangles(vt_context *ctx, const float *triangles, uint32_t count,
+                                    const vt_build_options *opts, vt_scene **out);
 void vt_scene_destroy(vt_scene *scene);
 
 typedef struct vt_scene_info {
@@ -132,6 +145,31 @@ typedef struct vt_material {
  * builds the spectral tables (D65 emitters) and the material reflectance spectra. */
 vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, uint32_t count);
 
+/* The reference's own material records, for callers that hold them: SurfaceSettings (64 bytes,
+ * main.swift:2253-2263; Swift 8911-8919) per slot and the emissions map (main.swift:9002). */
+typedef struct vt_surface_settings {
+    float color[4];          /* base colour, linear sRGB (w unused); default (1, 1, 1, 1) */
+    float surface[4];        /* roughness, metalness, coat, anisotropy; default (0.3, 0, 0, 0) */
+    float detail[4];         /* fuzz, IOR, transmission, UV repeat; default (0, 1.5, 0, 1) */
+    uint32_t enabled;        /* 0: the slot keeps the imported-mesh default */
+    uint32_t map_mask;       /* texture maps (bit per channel) */
+    float normal_strength;
+    uint32_t padding;
+} vt_surface_settings;       /* 64 bytes */
+typedef struct vt_emission {
+    uint32_t slot;           /* material slot (face tag) */
+    float rgb[3];            /* linear sRGB radiance under D65; any > 0 makes the slot's faces emitters */
+} vt_emission;
+/* Materials of an imported mesh from SurfaceSettings and emissions, as resolve_material and
+ * trace_scene assign them (main.swift:2008-2011, 2390-2400): a slot with a positive emission is
+ * EMISSIVE; else an enabled slot is OPENPBR with color.rgb, roughness surface.x, metalness surface.y,
+ * IOR detail.y and transmission detail.z; else the imported-mesh default, DIFFUSE with albedo 0.7.
+ * Slots beyond slot_count (emitters named only in `emissions`) take the default.  Refused
+ * (VT_ERROR_UNSUPPORTED): coat, anisotropy or fuzz other than 0, and texture maps (map_mask != 0), which
+ * are outside this path.  Then as vt_scene_set_materials. */
+vt_status vt_scene_set_surface_settings(vt_scene *scene, const vt_surface_settings *settings, uint32_t slot_count,
+                                        const vt_emission *emissions, uint32_t emission_count);
+
 /* Emitter selection for next-event estimation (MaterialLibrary.lightTree, main.swift:9006): 1 (default)
  * builds a light BVH over the emitting faces (Sources/LightTree.swift) when there are at least two and
  * the tree has more than one node; 0 keeps power sampling.  Takes effect at vt_scene_set_materials. */
@@ -188,6 +226,15 @@ void vt_renderer_destroy(vt_renderer *r);
  * until a reset, a size change or a sample in another mode.  Asynchronous on `stream`. */
 vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params *params, uint32_t samples,
                           void *stream);
+/* Completion callbacks, standing in for the renderer's onFrameUpdate(UInt32) and onError(String)
+ * (main.swift:10292-10293, 11198-11202).  After each vt_render_frame call has queued its work,
+ * on_frame(user, frames) is queued on the same stream (hipLaunchHostFunc): it runs on a runtime thread
+ * once the call's samples are in the accumulation, with the accumulated sample count, and must not
+ * call HIP or this library.  A call that fails calls on_error(user, message) on the calling thread
+ * before it returns its status.  Either may be NULL; NULL, NULL detaches. */
+typedef void (*vt_frame_fn)(void *user, uint32_t frames);
+typedef void (*vt_error_fn)(void *user, const char *message);
+vt_status vt_renderer_set_callbacks(vt_renderer *r, vt_frame_fn on_frame, vt_error_fn on_error, void *user);
 /* Attaches a learned radiance cache (owned by the caller, which keeps it alive while attached); NULL
  * detaches.  No counterpart in the reference. */
 vt_status vt_renderer_set_cache(vt_renderer *r, vt_nrc *cache);
@@ -247,6 +294,9 @@ vt_status vt_nrc_gradients(vt_nrc *nrc, const float *inputs, const float *target
  * options (materials and light tree kept).  The reference rebuilds only when the mesh changes
  * (Sources/Acceleration.swift:652-782); this serves the batch autotuner's leaf-size sweep. */
 vt_status vt_scene_rebuild(vt_scene *scene, const vt_build_options *opts);
+/* Test hook: caps the scene's traversal stack at `groups` node groups (0: the tree's own need), so
+ * the overflow report of vt_context_check can be exercised; rays needing more end early. */
+vt_status vt_scene_set_stack_cap(vt_scene *scene, uint32_t groups);
 
 /* ---- Batched rendering.  No counterpart in the reference, which renders one view at a time: many
  * images (jobs) of many scenes are advanced together by one wavefront of persistent kernels, and one
@@ -272,7 +322,10 @@ vt_status vt_batch_create(vt_context *ctx, const vt_batch_job *jobs, uint32_t co
                           uint32_t sampling_mode, vt_batch **out);
 void vt_batch_destroy(vt_batch *batch);
 /* Replays the captured sample `samples` times after the work already queued on `stream`; `stream`
- * waits for them. */
+ * waits for them.  A job's scene may be rebuilt (vt_scene_rebuild) or given new materials between
+ * calls: the next call sees the scene's new generation, waits for the batch's earlier samples, and
+ * uploads the tables and captures the sample again (the accumulation is kept).  Scenes must outlive
+ * the batch, and must not be rebuilt while samples of the batch are still running. */
 vt_status vt_batch_render(vt_batch *batch, uint32_t samples, void *stream);
 /* Sample indices back to 1 and the accumulation cleared; synchronises. */
 vt_status vt_batch_reset(vt_batch *batch);

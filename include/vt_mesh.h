@@ -29,6 +29,15 @@ vt_status vt_mesh_payload_write(const float *vertices, uint32_t vertex_count, co
 vt_status vt_mesh_payload_read(const uint8_t *data, uint64_t length, uint32_t maximum_faces, float **vertices,
                                uint32_t *vertex_count, uint32_t **faces, uint32_t *face_count);
 
+/* MeshGeometry(triangles:) (Sources/MeshGeometry.swift:21-26, 49-57): the indexed arrays of `count`
+ * MeshTriangle records (32 floats each, Sources/StudioModel.swift:13-15: corners a, b, c and normals
+ * na, nb, nc as float4 with w ignored, uvab = (uv of a, uv of b), uvc = (uv of c, subset, unused)),
+ * welding corners whose position, normal and uv bits are identical (0 and -0 stay distinct).  The
+ * subset is uvc.z truncated when finite and in [0, 4e9), else 0.  *vertices and *faces are malloc'ed:
+ * free both with vt_host_free. */
+vt_status vt_mesh_from_triangles(const float *triangles, uint32_t count, float **vertices, uint32_t *vertex_count,
+                                 uint32_t **faces, uint32_t *face_count);
+
 /* ---------------------------------------------------------------- OBJ reader
  * Replaces OBJMesh.parts (Sources/StudioModel.swift:444-684): statements o, g, usemtl, mtllib, v, vn,
  * vt (v defaults to 0 and is stored as 1 - v), f (v, v/vt, v//vn, v/vt/vn; 3-4096 corners, negative

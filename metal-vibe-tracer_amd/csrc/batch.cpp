@@ -52,6 +52,9 @@ struct vt_batch {
     hipGraphExec_t exec = nullptr;
     bool layered = false;
     uint32_t tree_depth = 1;
+    // vt_scene::generation of each scene when its views went into the tables: a rebuilt hierarchy or
+    // new materials replace the buffers the captured graph reads, so a change re-uploads and recaptures.
+    std::vector<uint64_t> scene_generation;
 
     ~vt_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -77,7 +80,9 @@ void upload_tables(vt_batch *b) {
     std::vector<vt::RenderScene> rs;
     b->layered = false;
     b->tree_depth = 1;
+    b->scene_generation.clear();
     for (vt_scene *s : b->scenes) {
+        b->scene_generation.push_back(s->generation);
         if (s->has_layered) vt::ensure_energy_tables(b->ctx);
         rs.push_back(s->render_view());
         b->layered = b->layered || s->has_layered;
@@ -330,6 +335,16 @@ vt_status vt_batch_render(vt_batch *b, uint32_t samples, void *stream) {
     return vt::guarded_call([&]() -> vt_status {
         if (!b) return fail(VT_ERROR_INVALID_ARGUMENT, "batch must not be NULL");
         check(hipSetDevice(b->ctx->device), "hipSetDevice");
+        // A scene rebuilt or given new materials since the capture: its old buffers are gone, and a
+        // deeper tree needs a deeper traversal stack, so the tables and the graph are made again.
+        bool stale = false;
+        for (size_t k = 0; k < b->scenes.size(); ++k) stale = stale || b->scenes[k]->generation != b->scene_generation[k];
+        if (stale) {
+            check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+            b->tuning.leaf_size = b->scenes[0]->info.leaf_size;
+            upload_tables(b);
+            capture(b);
+        }
         hipStream_t user = (hipStream_t)stream;
         hipEvent_t ev;
         check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");

@@ -17,6 +17,7 @@
 #include "scene.h"
 #include "trace_kernels.h"
 #include "vt.h"
+#include "vt_mesh.h"
 
 namespace {
 
@@ -93,6 +94,7 @@ std::unique_ptr<vt_scene> set_mesh(vt_context *ctx, const float *vertices, uint3
     s->view.node_count = (uint32_t)bvh.nodes.size();
     s->view.depth = (uint32_t)bvh.depth;
     s->view.root_max = bvh.root_max;
+    s->view.fault = ctx->fault.as<uint32_t>();
     vt_scene_info &i = s->info;
     i.face_count = face_count;
     i.vertex_count = vertex_count;
@@ -158,12 +160,40 @@ vt_status vt_context_create(const vt_context_desc *desc, vt_context **out) {
         c->device = device;
         c->budget = desc && desc->memory_budget ? desc->memory_budget
                                                 : std::max<uint64_t>(256ull << 20, (uint64_t)total / 4);
+        const uint32_t zero = 0;
+        c->fault.upload(&zero, 4);
         *out = c.release();
         return VT_OK;
     });
 }
 
 void vt_context_destroy(vt_context *ctx) { delete ctx; }
+
+vt_status vt_context_check(vt_context *ctx) {
+    return guarded([&]() -> vt_status {
+        if (!ctx) return fail(VT_ERROR_INVALID_ARGUMENT, "context must not be NULL");
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        uint32_t word = 0;
+        check(hipMemcpy(&word, ctx->fault.ptr, 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (word) {
+            const uint32_t zero = 0;
+            check(hipMemcpy(ctx->fault.ptr, &zero, 4, hipMemcpyHostToDevice), "hipMemcpy");
+            return fail(VT_ERROR_DEVICE, "BVH traversal stack overflow: some rays ended before their traversal did");
+        }
+        return VT_OK;
+    });
+}
+
+vt_status vt_scene_set_stack_cap(vt_scene *scene, uint32_t groups) {
+    return guarded([&]() -> vt_status {
+        if (!scene) return fail(VT_ERROR_INVALID_ARGUMENT, "scene must not be NULL");
+        if (groups > 24) return fail(VT_ERROR_INVALID_ARGUMENT, "groups must be 0..24");
+        scene->view.stack_cap = groups;
+        scene->generation = vt_scene::next_generation();
+        return VT_OK;
+    });
+}
 
 static vt::BuildOptions build_options(const vt_build_options *opts) {
     vt::BuildOptions o;
@@ -216,6 +246,22 @@ vt_status vt_scene_upload_indexed(vt_context *ctx, const float *vertices, uint32
         if (!ctx || !out) return fail(VT_ERROR_INVALID_ARGUMENT, "context and out must not be NULL");
         *out = nullptr;
         *out = vt::set_mesh(ctx, vertices, vertex_count, faces, face_count, build_options(opts)).release();
+        return VT_OK;
+    });
+}
+
+vt_status vt_scene_upload_triangles(vt_context *ctx, const float *triangles, uint32_t count,
+                                    const vt_build_options *opts, vt_scene **out) {
+    return guarded([&]() -> vt_status {
+        if (!ctx || !out) return fail(VT_ERROR_INVALID_ARGUMENT, "context and out must not be NULL");
+        *out = nullptr;
+        float *v = nullptr;
+        uint32_t *f = nullptr, nv = 0, nf = 0;
+        const vt_status st = vt_mesh_from_triangles(triangles, count, &v, &nv, &f, &nf);
+        if (st != VT_OK) return st;
+        std::unique_ptr<float, void (*)(void *)> vg(v, std::free);
+        std::unique_ptr<uint32_t, void (*)(void *)> fg(f, std::free);
+        *out = vt::set_mesh(ctx, v, nv, f, nf, build_options(opts)).release();
         return VT_OK;
     });
 }

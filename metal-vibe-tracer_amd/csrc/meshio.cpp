@@ -377,6 +377,43 @@ vt_status vt_mesh_payload_write(const float *vertices, uint32_t vertex_count, co
     });
 }
 
+vt_status vt_mesh_from_triangles(const float *triangles, uint32_t count, float **vertices, uint32_t *vertex_count,
+                                 uint32_t **faces, uint32_t *face_count) {
+    return guarded([&]() -> vt_status {
+        if (!vertices || !vertex_count || !faces || !face_count || (count && !triangles))
+            return vt::fail(VT_ERROR_INVALID_ARGUMENT, "arguments must not be NULL");
+        Builder b;
+        b.faces.reserve((size_t)count * 4);
+        for (uint32_t i = 0; i < count; ++i) {
+            // MeshTriangle: a, b, c, na, nb, nc (float4 each), uvab (uv of a, uv of b), uvc (uv of c, subset).
+            const float *t = triangles + 32 * (size_t)i;
+            const float *uvab = t + 24, *uvc = t + 28;
+            const float corner[3][8] = {{t[0], t[1], t[2], t[12], t[13], t[14], uvab[0], uvab[1]},
+                                        {t[4], t[5], t[6], t[16], t[17], t[18], uvab[2], uvab[3]},
+                                        {t[8], t[9], t[10], t[20], t[21], t[22], uvc[0], uvc[1]}};
+            const float z = uvc[2];
+            const uint32_t subset = std::isfinite(z) && z >= 0.0f && z < 4e9f ? (uint32_t)z : 0u;
+            for (int k = 0; k < 3; ++k) b.faces.push_back(b.vertex(corner[k]));
+            b.faces.push_back(subset);
+        }
+        const size_t nv = b.vertices.size() / 8, nf = b.faces.size() / 4;
+        float *v = (float *)std::malloc(nv ? 32 * nv : 1);
+        uint32_t *f = (uint32_t *)std::malloc(nf ? 16 * nf : 1);
+        if (!v || !f) {
+            std::free(v);
+            std::free(f);
+            throw std::bad_alloc();
+        }
+        if (nv) std::memcpy(v, b.vertices.data(), 32 * nv);
+        if (nf) std::memcpy(f, b.faces.data(), 16 * nf);
+        *vertices = v;
+        *faces = f;
+        *vertex_count = (uint32_t)nv;
+        *face_count = (uint32_t)nf;
+        return VT_OK;
+    });
+}
+
 vt_status vt_mesh_payload_read(const uint8_t *data, uint64_t length, uint32_t maximum_faces, float **vertices,
                                uint32_t *vertex_count, uint32_t **faces, uint32_t *face_count) {
     return guarded([&]() -> vt_status {

@@ -1406,36 +1406,6 @@ __global__ __launch_bounds__(64) void bake_metal_kernel(float *out) {
     }
     out[node] = sum / 16384.0f;
 }
-// Dielectric single scattering per node (relative IOR, roughness j/31, cosine k/31): E_R = <F G1(l_r)>,
-// E_T = <(1 - F) G1(l_t)> over 1024 Hammersley VNDF samples.
-__global__ __launch_bounds__(64) void bake_dielectric_kernel(float *er, float *et) {
-    const uint32_t node = blockIdx.x * 64 + threadIdx.x;
-    if (node >= 32768u) return;
-    const uint32_t i = node >> 10, j = (node >> 5) & 31u, k = node & 31u;
-    const float eta = i >= 16u ? 1.0f + (float)(i - 16u) / 15.0f * 1.5f : 1.0f / (1.0f + (float)(15u - i) / 15.0f * 1.5f);
-    const float roughness = (float)j / 31.0f, alpha = roughness * roughness;
-    const float c = fmaxf(1e-5f, (float)k / 31.0f);
-    const V3 v = v3(sqrtf(fmaxf(0.0f, 1.0f - c * c)), 0.0f, c);
-    float R = 0.0f, T = 0.0f;
-    for (uint32_t q = 0; q < 1024u; ++q) {
-        const float r0 = ((float)q + 0.5f) / 1024.0f, r1 = (float)reverse_bits32(q) * 2.3283064365386963e-10f;
-        const V3 h = sample_vndf(alpha, v, r0, r1);
-        const float d = dot(v, h);
-        if (!(d > 0.0f)) continue;
-        const float F = fresnel(eta, d);
-        const V3 l = -v + h * (2.0f * d);
-        if (l.z > 0.0f) R += F * smith_g1(l, alpha);
-        const float inv = 1.0f / eta, s2t = inv * inv * fmaxf(0.0f, 1.0f - d * d);
-        if (s2t < 1.0f) {
-            const float ct = sqrtf(1.0f - s2t);
-            const V3 t = normalize(v * -inv + h * (inv * d - ct));
-            if (t.z < 0.0f) T += (1.0f - F) * smith_g1(t, alpha);
-        }
-    }
-    er[node] = R / 1024.0f;
-    et[node] = T / 1024.0f;
-}
-
 __global__ void reset_kernel(uint32_t *counters, uint32_t n) {
     if (threadIdx.x < n) counters[threadIdx.x] = 0u;
 }
@@ -1628,11 +1598,6 @@ hipError_t launch_zero_f4(float4 *p, uint32_t n, hipStream_t s) {
 
 hipError_t launch_bake_metal(float *out, hipStream_t s) {
     hipLaunchKernelGGL(bake_metal_kernel, dim3((129 * 129 + 63) / 64), dim3(64), 0, s, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_bake_dielectric(float *er, float *et, hipStream_t s) {
-    hipLaunchKernelGGL(bake_dielectric_kernel, dim3(32768 / 64), dim3(64), 0, s, er, et);
     return hipGetLastError();
 }
 

@@ -171,10 +171,8 @@ hipError_t launch_build_targets(const NrcArgs &nrc, hipStream_t s);
 hipError_t launch_zero_f4(float4 *p, uint32_t n, hipStream_t s);
 // The light tree's nodes decoded for the walk (5 float4 per node) into out.
 hipError_t launch_decode_light_tree(const RenderScene &sc, float4 *out, hipStream_t s);
-// Energy-table bakes: ideal-metal complements (129 x 129, Hammersley 16384 per node) and the
-// dielectric single-scattering albedos E_R, E_T (32 x 32 x 32, Hammersley 1024 per node).
+// Energy-table bake: ideal-metal complements (129 x 129, Hammersley 16384 per node).
 hipError_t launch_bake_metal(float *out, hipStream_t s);
-hipError_t launch_bake_dielectric(float *er, float *et, hipStream_t s);
 hipError_t launch_reset_counters(uint32_t *counters, uint32_t n, hipStream_t s);
 hipError_t launch_accumulate(const FrameArgs &f, const float *radiance, float *accum, uint32_t frame_index,
                              hipStream_t s);

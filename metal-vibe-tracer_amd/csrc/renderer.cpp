@@ -65,6 +65,9 @@ struct vt_renderer {
         if (host_count) (void)hipHostFree(host_count);
     }
     float prev_camera[14] = {};  // eye, forward, right, up, fov_scale, aspect of the last sample
+    vt_frame_fn on_frame = nullptr;  // completion callbacks (vt_renderer_set_callbacks)
+    vt_error_fn on_error = nullptr;
+    void *callback_user = nullptr;
 };
 
 namespace {
@@ -303,6 +306,49 @@ vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, 
     });
 }
 
+vt_status vt_scene_set_surface_settings(vt_scene *scene, const vt_surface_settings *settings, uint32_t slot_count,
+                                        const vt_emission *emissions, uint32_t emission_count) {
+    return vt::guarded_call([&]() -> vt_status {
+        if (!scene || (slot_count && !settings) || (emission_count && !emissions))
+            return fail(VT_ERROR_INVALID_ARGUMENT, "scene, settings and emissions must not be NULL");
+        uint32_t slots = slot_count;
+        for (uint32_t k = 0; k < emission_count; ++k) {
+            if (emissions[k].slot >= (1u << 24)) return fail(VT_ERROR_INVALID_ARGUMENT, "emission slot out of range");
+            slots = std::max(slots, emissions[k].slot + 1);
+        }
+        std::vector<vt_material> mats(slots);
+        for (uint32_t k = 0; k < slots; ++k) {
+            vt_material &m = mats[k];
+            m = vt_material{};
+            m.type = 0;  // the imported-mesh default (main.swift:2009)
+            m.albedo[0] = m.albedo[1] = m.albedo[2] = 0.7f;
+            m.roughness = 0.5f;
+            m.ior = 1.5f;
+            m.specular_weight = 1.0f;
+            if (k >= slot_count) continue;
+            const vt_surface_settings &s = settings[k];
+            if (s.map_mask != 0) return fail(VT_ERROR_UNSUPPORTED, "texture maps are outside this path (map_mask must be 0)");
+            if (!s.enabled) continue;
+            if (s.surface[2] != 0.0f || s.surface[3] != 0.0f || s.detail[0] != 0.0f)
+                return fail(VT_ERROR_UNSUPPORTED, "coat, anisotropy and fuzz are outside the layered BSDF subset");
+            m.type = 4;
+            for (int c = 0; c < 3; ++c) m.albedo[c] = s.color[c];
+            m.roughness = s.surface[0];
+            m.metalness = s.surface[1];
+            m.ior = s.detail[1];
+            m.transmission = s.detail[2];
+        }
+        for (uint32_t k = 0; k < emission_count; ++k) {
+            const vt_emission &e = emissions[k];
+            if (!(e.rgb[0] > 0.0f || e.rgb[1] > 0.0f || e.rgb[2] > 0.0f)) continue;
+            vt_material &m = mats[e.slot];
+            m.type = 3;
+            for (int c = 0; c < 3; ++c) m.emission[c] = e.rgb[c];
+        }
+        return vt_scene_set_materials(scene, mats.data(), slots);
+    });
+}
+
 static const vt::SpectralTables &host_tables() {
     static vt::SpectralTables *t = [] {
         auto *p = new vt::SpectralTables();
@@ -436,7 +482,49 @@ vt_status vt_renderer_cache_records(vt_renderer *r, float *inputs, float *target
     });
 }
 
+static vt_status render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params *p, uint32_t samples,
+                              void *stream);
+
+namespace {
+struct FrameDone {
+    vt_frame_fn fn;
+    void *user;
+    uint32_t frames;
+};
+void frame_done(void *payload) {
+    FrameDone *d = static_cast<FrameDone *>(payload);
+    d->fn(d->user, d->frames);
+    delete d;
+}
+}  // namespace
+
+vt_status vt_renderer_set_callbacks(vt_renderer *r, vt_frame_fn on_frame, vt_error_fn on_error, void *user) {
+    if (!r) return fail(VT_ERROR_INVALID_ARGUMENT, "renderer must not be NULL");
+    r->on_frame = on_frame;
+    r->on_error = on_error;
+    r->callback_user = user;
+    return VT_OK;
+}
+
 vt_status vt_render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params *p, uint32_t samples, void *stream) {
+    vt_status st = render_frame(r, scene, p, samples, stream);
+    if (st == VT_OK && r->on_frame && samples) {
+        st = vt::guarded_call([&]() -> vt_status {
+            auto *d = new FrameDone{r->on_frame, r->callback_user, r->frame_index};
+            const hipError_t e = hipLaunchHostFunc((hipStream_t)stream, frame_done, d);
+            if (e != hipSuccess) {
+                delete d;
+                return fail(VT_ERROR_DEVICE, std::string("hipLaunchHostFunc: ") + hipGetErrorString(e));
+            }
+            return VT_OK;
+        });
+    }
+    if (st != VT_OK && r && r->on_error) r->on_error(r->callback_user, vt_last_error());
+    return st;
+}
+
+static vt_status render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_params *p, uint32_t samples,
+                              void *stream) {
     return vt::guarded_call([&]() -> vt_status {
         if (!r || !scene || !p) return fail(VT_ERROR_INVALID_ARGUMENT, "renderer, scene and params must not be NULL");
         if (!scene->shaded) return fail(VT_ERROR_INVALID_ARGUMENT, "scene has no materials (vt_scene_set_materials)");

@@ -67,6 +67,8 @@ struct vt_context {
     // Energy-compensation tables of the layered BSDF, baked on the device on first use.
     vt::DeviceBuffer energy_metal, energy_luts;
     bool energy_ready = false;
+    // Traversal fault word (TraceScene::fault) of every scene of the context, read by vt_context_check.
+    vt::DeviceBuffer fault;
 };
 
 namespace vt {

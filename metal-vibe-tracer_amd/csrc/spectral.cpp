@@ -2,8 +2,11 @@
 #include "spectral.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 namespace vt {
@@ -78,9 +81,9 @@ bool solve3(const double m[9], const double f[3], double x[3]) {
 
 }  // namespace
 
-double solve_lagrange(const SpectralTables &t, const double target_in[3], double lam[3]) {
-    SolveData s;
-    solve_data(t, s);
+namespace {
+
+double solve_with(const SolveData &s, const double target_in[3], double lam[3]) {
     double target[3];
     for (int k = 0; k < 3; ++k) target[k] = std::min(std::max(target_in[k], kEpsilon), 1.0 - kEpsilon);
     lam[0] = lam[1] = lam[2] = 0.0;
@@ -128,6 +131,219 @@ double solve_lagrange(const SpectralTables &t, const double target_in[3], double
     return err;
 }
 
+double srgb_eotf(double v) { return v <= 0.04045 ? v / 12.92 : std::pow((v + 0.055) / 1.055, 2.4); }
+double srgb_oetf(double v) {
+    v = std::min(std::max(v, 0.0), 1.0);
+    return v <= 0.0031308 ? 12.92 * v : 1.055 * std::pow(v, 1.0 / 2.4) - 0.055;
+}
+
+// Bounded trigonometric moments c_j = (1/pi) int_{-pi}^{0} g(phi) cos(j phi) dphi of the reflectance
+// of Lagrange multipliers l, by the 1024-node midpoint rule (generate_spectral_tables.py:227-239).
+void moments_of_lagrange(const double l[3], double m[3]) {
+    const double h = kPi / 1024.0;
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+    for (int i = 0; i < 1024; ++i) {
+        double cp = std::cos(-kPi + (i + 0.5) * h), c2p = 2.0 * cp * cp - 1.0;
+        double v = std::atan(l[0] + 2.0 * l[1] * cp + 2.0 * l[2] * c2p) / kPi + 0.5;
+        c0 += v;
+        c1 += v * cp;
+        c2 += v * c2p;
+    }
+    m[0] = c0 * h / kPi;
+    m[1] = c1 * h / kPi;
+    m[2] = c2 * h / kPi;
+}
+
+// Lagrange multipliers of the bounded MESE of three real moments (Peters et al. 2019, Eqs. 6, 7, 10;
+// Levinson's recursion with the biasing of FOURIERSRGB2019 Alg. 2), as the reference's host copy
+// SpectralColour.fourierLagrange (main.swift:8640-8680) and MSL vibe_fourier_lagrange: moments
+// outside the valid set are pulled back, so every input yields a reflectance.
+struct Cx {
+    double x, y;
+};
+Cx cmul(Cx a, Cx b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+Cx conj(Cx a) { return {a.x, -a.y}; }
+Cx operator+(Cx a, Cx b) { return {a.x + b.x, a.y + b.y}; }
+Cx operator-(Cx a, Cx b) { return {a.x - b.x, a.y - b.y}; }
+Cx operator*(double s, Cx a) { return {s * a.x, s * a.y}; }
+double cdot(Cx a, Cx b) { return a.x * b.x + a.y * b.y; }
+
+void fourier_lagrange(const double c[3], double out[3]) {
+    const double c0 = std::min(std::max(c[0], kEpsilon), 1.0 - kEpsilon);
+    const Cx g0p = (1.0 / (4.0 * kPi)) * Cx{std::sin(kPi * c0), -std::cos(kPi * c0)};
+    const double g0 = 2.0 * g0p.x;
+    Cx g1 = (2.0 * kPi * c[1]) * Cx{-g0p.y, g0p.x};
+    Cx g2 = kPi * Cx{-(2.0 * c[2] * g0p.y + c[1] * g1.y), 2.0 * c[2] * g0p.x + c[1] * g1.x};
+    const double q0 = 1.0 / g0;
+    Cx u1 = q0 * g1;
+    double n1 = cdot(u1, u1), keep = 0.9999;
+    if (n1 >= 1.0) {
+        u1 = (keep / std::sqrt(n1)) * u1;
+        g1 = (1.0 / q0) * u1;
+        n1 = keep * keep;
+        keep = 0.0;
+    }
+    const double d1 = 1.0 / (1.0 - n1), a0 = q0 * d1;
+    const Cx a1 = (-(q0 * d1)) * u1;
+    Cx u2 = a0 * g2 + cmul(a1, g1);
+    double n2 = cdot(u2, u2);
+    if (n2 >= 1.0) {
+        u2 = (keep / std::sqrt(n2)) * u2;
+        g2 = (1.0 / a0) * (u2 - cmul(a1, g1));
+        n2 = keep * keep;
+    }
+    const double d2 = 1.0 / (1.0 - n2), b0 = a0 * d2;
+    const Cx b1 = d2 * (a1 - cmul(u2, conj(a1))), b2 = (-(a0 * d2)) * u2;
+    const Cx r0 = {b0 * b0 + cdot(b1, b1) + cdot(b2, b2), 0.0};
+    const Cx r1 = cmul(conj(b1), Cx{b0, 0.0}) + cmul(conj(b2), b1), r2 = cmul(conj(b2), Cx{b0, 0.0});
+    const Cx s0 = cmul(g0p, r0) + cmul(g1, r1) + cmul(g2, r2), s1 = cmul(g0p, r1) + cmul(g1, r2), s2 = cmul(g0p, r2);
+    const double scale = 2.0 / b0;
+    out[0] = s0.y * scale;
+    out[1] = s1.y * scale;
+    out[2] = s2.y * scale;
+}
+
+// Linear sRGB under D65 of the reflectance with multipliers l (the generator's rgb_of_lagrange).
+void rgb_of_lagrange(const SolveData &s, const double l[3], double rgb[3]) {
+    rgb[0] = rgb[1] = rgb[2] = 0.0;
+    for (int i = 0; i < kSpectralSamples; ++i) {
+        double v = std::atan(l[0] + l[1] * s.c1[i] + l[2] * s.c2[i]) / kPi + 0.5;
+        for (int k = 0; k < 3; ++k) rgb[k] += s.w[k][i] * v;
+    }
+}
+
+// The reference's uplift grid, evaluated lazily: exact moments of the coarse nodes (8-bit codes
+// 0, 3, ..., 255; FourierSRGB86.bin) and of the codes of refined cells, and the refinement flag of
+// each cell, cached for the process (every scene's reflectances are taken under the same D65 data).
+struct UpliftGrid {
+    SolveData s;
+    std::mutex mu;
+    std::unordered_map<uint32_t, std::array<float, 3>> exact;
+    std::unordered_map<uint32_t, bool> refined;
+    float nodes[86];
+
+    explicit UpliftGrid(const SpectralTables &t) {
+        solve_data(t, s);
+        for (int i = 0; i < 86; ++i) nodes[i] = (float)srgb_eotf(3.0 * i / 255.0);
+    }
+
+    // Exact moments of code (r, g, b): Levenberg-Marquardt to the code's linear colour, then the
+    // 1024-node moments (the generator's solve_code, :381-386), stored as float like the grid file.
+    std::array<float, 3> code_moments(int r, int g, int b) {
+        const uint32_t key = (uint32_t)r << 16 | (uint32_t)g << 8 | (uint32_t)b;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            auto it = exact.find(key);
+            if (it != exact.end()) return it->second;
+        }
+        const double target[3] = {srgb_eotf(r / 255.0), srgb_eotf(g / 255.0), srgb_eotf(b / 255.0)};
+        double l[3], m[3];
+        solve_with(s, target, l);
+        moments_of_lagrange(l, m);
+        const std::array<float, 3> v = {(float)m[0], (float)m[1], (float)m[2]};
+        std::lock_guard<std::mutex> lock(mu);
+        exact.emplace(key, v);
+        return v;
+    }
+
+    // Trilinear interpolation in linear light of the coarse cell (i) at colour c (MSL
+    // spectral_moments_coarse, main.swift:855-869).
+    void coarse(const double c[3], const int i[3], double m[3]) {
+        double t[3];
+        for (int k = 0; k < 3; ++k) {
+            const double lo = nodes[i[k]], hi = nodes[i[k] + 1];
+            t[k] = std::min(std::max((c[k] - lo) / (hi - lo), 0.0), 1.0);
+        }
+        m[0] = m[1] = m[2] = 0.0;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int d = 0; d < 2; ++d) {
+                    const double w = (a ? t[0] : 1.0 - t[0]) * (b ? t[1] : 1.0 - t[1]) * (d ? t[2] : 1.0 - t[2]);
+                    if (w == 0.0) continue;
+                    const auto e = code_moments(3 * (i[0] + a), 3 * (i[1] + b), 3 * (i[2] + d));
+                    for (int k = 0; k < 3; ++k) m[k] += w * (double)e[k];
+                }
+    }
+
+    // Whether the reference refines cell i: some 8-bit code of the cell comes back from the coarse
+    // conversion more than 0.35 steps off (spectral_code_error / spectral_refine_flags_kernel,
+    // main.swift:8212-8233).  The codes of cell i are 3i, 3i + 1, 3i + 2 per axis (and 255 in the
+    // last cell); greys are skipped.
+    bool is_refined(const int i[3]) {
+        const uint32_t key = (uint32_t)i[0] << 16 | (uint32_t)i[1] << 8 | (uint32_t)i[2];
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            auto it = refined.find(key);
+            if (it != refined.end()) return it->second;
+        }
+        bool flag = false;
+        int n[3];
+        for (int k = 0; k < 3; ++k) n[k] = i[k] == 84 ? 4 : 3;
+        for (int a = 0; a < n[0] && !flag; ++a)
+            for (int b = 0; b < n[1] && !flag; ++b)
+                for (int d = 0; d < n[2] && !flag; ++d) {
+                    const int code[3] = {3 * i[0] + a, 3 * i[1] + b, 3 * i[2] + d};
+                    if (code[0] == code[1] && code[1] == code[2]) continue;
+                    double c[3], m[3], l[3], rgb[3];
+                    for (int k = 0; k < 3; ++k) c[k] = srgb_eotf(code[k] / 255.0);
+                    coarse(c, i, m);
+                    fourier_lagrange(m, l);
+                    rgb_of_lagrange(s, l, rgb);
+                    for (int k = 0; k < 3; ++k)
+                        if (std::fabs(srgb_oetf(rgb[k]) * 255.0 - code[k]) > 0.35) flag = true;
+                }
+        std::lock_guard<std::mutex> lock(mu);
+        refined.emplace(key, flag);
+        return flag;
+    }
+
+    // Moments of a bounded linear colour (MSL spectral_moments, main.swift:870-884; host copy
+    // SpectralColour.lagrange, :8596-8635): coarse cell, or in a refined cell the exact moments of
+    // its codes interpolated at one-code spacing.
+    void moments(const double c[3], double m[3]) {
+        int i[3];
+        double code85[3];
+        for (int k = 0; k < 3; ++k) {
+            code85[k] = srgb_oetf(c[k]) * 85.0;
+            i[k] = std::min(std::max((int)code85[k], 0), 84);
+        }
+        if (!is_refined(i)) {
+            coarse(c, i, m);
+            return;
+        }
+        int n0[3];
+        double t[3];
+        for (int k = 0; k < 3; ++k) {
+            const int sk = std::min(std::max((int)(code85[k] * 3.0 - 3.0 * i[k]), 0), 2);
+            n0[k] = 3 * i[k] + sk;
+            const double lo = srgb_eotf(n0[k] / 255.0), hi = srgb_eotf((n0[k] + 1) / 255.0);
+            t[k] = std::min(std::max((c[k] - lo) / (hi - lo), 0.0), 1.0);
+        }
+        m[0] = m[1] = m[2] = 0.0;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int d = 0; d < 2; ++d) {
+                    const double w = (a ? t[0] : 1.0 - t[0]) * (b ? t[1] : 1.0 - t[1]) * (d ? t[2] : 1.0 - t[2]);
+                    if (w == 0.0) continue;
+                    const auto e = code_moments(std::min(n0[0] + a, 255), std::min(n0[1] + b, 255), std::min(n0[2] + d, 255));
+                    for (int k = 0; k < 3; ++k) m[k] += w * (double)e[k];
+                }
+    }
+};
+
+UpliftGrid &uplift_grid(const SpectralTables &t) {
+    static UpliftGrid *g = new UpliftGrid(t);
+    return *g;
+}
+
+}  // namespace
+
+double solve_lagrange(const SpectralTables &t, const double target[3], double lam[3]) {
+    SolveData s;
+    solve_data(t, s);
+    return solve_with(s, target, lam);
+}
+
 void reflectance_lagrange(const SpectralTables &t, const float rgb[3], float out[4]) {
     float c[3];
     for (int k = 0; k < 3; ++k) c[k] = std::min(std::max(rgb[k], 0.0f), 1.0f);
@@ -138,8 +354,10 @@ void reflectance_lagrange(const SpectralTables &t, const float rgb[3], float out
         out[3] = 2.0f;
         return;
     }
-    double target[3] = {c[0], c[1], c[2]}, l[3];
-    solve_lagrange(t, target, l);
+    const double cd[3] = {c[0], c[1], c[2]};
+    double m[3], l[3];
+    uplift_grid(t).moments(cd, m);
+    fourier_lagrange(m, l);
     out[0] = (float)l[0];
     out[1] = (float)l[1];
     out[2] = (float)l[2];
@@ -212,9 +430,13 @@ void build_spectral_tables(SpectralTables &t) {
 
     // Emission basis: the bounded-MESE spectra of the six saturated corners (spectral_icdf_kernel,
     // main.swift:8117-8131), and the normalised D65.
-    const double corners[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
+    const float corners[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
     double lag[6][3];
-    for (int c = 0; c < 6; ++c) solve_lagrange(t, corners[c], lag[c]);
+    for (int c = 0; c < 6; ++c) {
+        float l[4];
+        reflectance_lagrange(t, corners[c], l);
+        for (int k = 0; k < 3; ++k) lag[c][k] = l[k];
+    }
     for (int i = 0; i < kSpectralSamples; ++i) {
         double ph = phase_of(kLambdaMin + i), c1 = std::cos(ph);
         float v[6];

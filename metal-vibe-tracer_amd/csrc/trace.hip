@@ -27,12 +27,13 @@ struct LdsStack {
     uint32_t (*bits)[kBlk];
     int lane;
     int sp;
+    int cap;  // groups the stack may hold: kStack, or less under TraceScene::stack_cap
     __device__ __forceinline__ void push(uint32_t b, uint32_t m) {
         base[sp][lane] = b;
         bits[sp][lane] = m;
         ++sp;
     }
-    __device__ __forceinline__ bool full() const { return sp >= kStack; }
+    __device__ __forceinline__ bool full() const { return sp >= cap; }
     __device__ __forceinline__ void pop(uint32_t &b, uint32_t &m) {
         --sp;
         b = base[sp][lane];
@@ -45,7 +46,8 @@ template <bool ANY, bool STATS, int kStack>
 __global__ __launch_bounds__(kBlock) void trace_kernel(const U4 *__restrict__ nodes, const F4 *__restrict__ tris,
                                                        RayArgs rays, HitArgs hits, uint8_t *__restrict__ occluded,
                                                        uint32_t n, const uint32_t *__restrict__ count,
-                                                       float root_max, unsigned long long *__restrict__ counters) {
+                                                       float root_max, unsigned long long *__restrict__ counters,
+                                                       uint32_t *__restrict__ fault, int cap) {
     __shared__ uint32_t stack_base[kStack][kBlock];
     __shared__ uint32_t stack_bits[kStack][kBlock];
     const int lane = threadIdx.x;
@@ -53,8 +55,9 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(const U4 *__restrict__ no
     if (count) n = min(n, *count);
     if (i >= n) return;
     TraceRay r{rays.ox[i], rays.oy[i], rays.oz[i], rays.dx[i], rays.dy[i], rays.dz[i], rays.tmin[i], rays.tmax[i]};
-    LdsStack<kStack> st{stack_base, stack_bits, lane, 0};
+    LdsStack<kStack> st{stack_base, stack_bits, lane, 0, cap};
     TraceResult res = traverse<ANY, STATS>(nodes, tris, root_max, r, st);
+    if (res.overflow && fault) atomicOr(fault, 1u);
     if (ANY) {
         occluded[i] = res.occluded ? 1 : 0;
     } else {
@@ -84,11 +87,15 @@ struct OneScene {
     const U4 *nodes;
     const F4 *tris;
     float root_max;
+    uint32_t *fault;
+    int cap;
     struct Lane {};
     __device__ __forceinline__ bool begin(uint32_t, Lane &) const { return true; }
     __device__ __forceinline__ const U4 *node_ptr(const Lane &) const { return nodes; }
     __device__ __forceinline__ const F4 *tri_ptr(const Lane &) const { return tris; }
     __device__ __forceinline__ float root(const Lane &) const { return root_max; }
+    __device__ __forceinline__ uint32_t *fault_word(const Lane &) const { return fault; }
+    __device__ __forceinline__ int stack_cap(const Lane &, int k) const { return cap < k ? cap : k; }
 };
 struct JobScenes {
     static constexpr bool kShared = false;
@@ -98,6 +105,8 @@ struct JobScenes {
         const U4 *nodes;
         const F4 *tris;
         float root_max;
+        uint32_t *fault;
+        uint32_t cap;
     };
     // False for a scene without geometry (the ray misses).
     __device__ __forceinline__ bool begin(uint32_t ray, Lane &l) const {
@@ -105,19 +114,25 @@ struct JobScenes {
         l.nodes = reinterpret_cast<const U4 *>(ts.nodes);
         l.tris = reinterpret_cast<const F4 *>(ts.tris);
         l.root_max = ts.root_max;
+        l.fault = ts.fault;
+        l.cap = ts.stack_cap;
         return ts.node_count != 0;
     }
     __device__ __forceinline__ const U4 *node_ptr(const Lane &l) const { return l.nodes; }
     __device__ __forceinline__ const F4 *tri_ptr(const Lane &l) const { return l.tris; }
     __device__ __forceinline__ float root(const Lane &l) const { return l.root_max; }
+    __device__ __forceinline__ uint32_t *fault_word(const Lane &l) const { return l.fault; }
+    __device__ __forceinline__ int stack_cap(const Lane &l, int k) const {
+        return l.cap && (int)l.cap < k ? (int)l.cap : k;
+    }
 };
 
-// Order-preserving map of a float's bits to an unsigned integer, and its inverse.
+// Order-preserving map of a float's bits to an unsigned integer.  -0 maps with +0, since the closest-hit
+// rule compares t numerically (a hit at t = -0 ties with one at +0 and the smaller face wins).
 __device__ __forceinline__ uint32_t ord_bits(float f) {
-    const uint32_t u = __float_as_uint(f);
+    const uint32_t u = __float_as_uint(f == 0.0f ? 0.0f : f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
-__device__ __forceinline__ float unord_bits(uint32_t o) { return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o); }
 // Position of the k-th set bit (from 0) of m, for k < popcount(m) and m below 2^31.
 __device__ __forceinline__ uint32_t nth_set_bit(uint32_t m, uint32_t k) {
     uint32_t pos = 0;
@@ -145,7 +160,7 @@ __device__ __forceinline__ void wave_lds_order() {
 // lane whose range holds j (binary search over the prefixes) and is its k-th triangle; the worker reads
 // that lane's ray through cross-lane permutes, runs the same Woop test, and folds a hit into the
 // owner's LDS slot with an atomic minimum of (order-preserving t bits << 32 | face) -- the nearest t,
-// equal t going to the smaller face, as the one-lane rule.  The winner of each round stores its
+// equal t going to the smaller face, as the one-lane rule.  The winner of each round stores its t and
 // barycentrics; the key's identity picks it since a face is tested at most once per ray.  A hit at t
 // must also lie in [tmin, tmax) (the one-lane rule's first accepted hit needs t < tmax).  Any-hit
 // queries only raise the owner's flag.  Must be called by all 64 lanes together.
@@ -165,7 +180,7 @@ __device__ __forceinline__ void wave_prefix(uint32_t cnt, unsigned long long bel
 template <bool ANY, int kBlk, class Source>
 __device__ __forceinline__ void wave_triangles(const Source &src, const typename Source::Lane &sl, TravState &ts,
                                                uint32_t trimask, uint32_t tri_base, uint32_t lane, uint32_t pre,
-                                               uint32_t total, unsigned long long *key, float (*bary)[kBlk]) {
+                                               uint32_t total, unsigned long long *key, float (*hit_tb)[kBlk]) {
     const uint32_t cnt = __popc(trimask);
     if (total == 0u) return;
     const uint32_t wave0 = threadIdx.x & ~63u;
@@ -225,8 +240,9 @@ __device__ __forceinline__ void wave_triangles(const Source &src, const typename
         if (!ANY) {
             wave_lds_order();
             if (hit && key[wave0 + owner] == mine) {
-                bary[0][wave0 + owner] = b1;
-                bary[1][wave0 + owner] = b2;
+                hit_tb[0][wave0 + owner] = t;
+                hit_tb[1][wave0 + owner] = b1;
+                hit_tb[2][wave0 + owner] = b2;
             }
         }
         wave_lds_order();
@@ -236,10 +252,10 @@ __device__ __forceinline__ void wave_triangles(const Source &src, const typename
         if (ANY) {
             if (got) ts.res.occluded = true;
         } else if (got != key0) {
-            ts.res.t = unord_bits((uint32_t)(got >> 32));
+            ts.res.t = hit_tb[0][threadIdx.x];
             ts.res.face = (int32_t)(uint32_t)got;
-            ts.res.b1 = bary[0][threadIdx.x];
-            ts.res.b2 = bary[1][threadIdx.x];
+            ts.res.b1 = hit_tb[1][threadIdx.x];
+            ts.res.b2 = hit_tb[2][threadIdx.x];
         }
     }
     wave_lds_order();
@@ -252,7 +268,7 @@ __device__ __forceinline__ void refill_trace(const Source &src, const RayArgs &r
     __shared__ uint32_t stack_bits[kStack][kBlk];
     __shared__ uint32_t next;
     __shared__ unsigned long long tri_key[kBlk];
-    __shared__ float tri_bary[ANY ? 1 : 2][kBlk];
+    __shared__ float tri_hit[ANY ? 1 : 3][kBlk];
     constexpr uint32_t kNone = 0xffffffffu;
     const uint32_t chunk = ((n + gridDim.x - 1) / gridDim.x + 63u) & ~63u;
     const uint32_t begin = min(n, blockIdx.x * chunk), end = min(n, begin + chunk);
@@ -260,7 +276,7 @@ __device__ __forceinline__ void refill_trace(const Source &src, const RayArgs &r
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long below = (1ull << lane) - 1ull;
-    LdsStack<kStack, kBlk> st{stack_base, stack_bits, (int)threadIdx.x, 0};
+    LdsStack<kStack, kBlk> st{stack_base, stack_bits, (int)threadIdx.x, 0, kStack};
     TravState ts;
     typename Source::Lane sl{};
     uint32_t ray = kNone;
@@ -298,6 +314,7 @@ __device__ __forceinline__ void refill_trace(const Source &src, const RayArgs &r
                     ray = idx;
                     trav_init(ts, r, src.root(sl));
                     st.sp = 0;
+                    st.cap = src.stack_cap(sl, kStack);
                     done = false;
                 } else if (ANY) {
                     occluded[idx] = 0;
@@ -323,10 +340,14 @@ __device__ __forceinline__ void refill_trace(const Source &src, const RayArgs &r
         uint32_t pre, total;
         wave_prefix(__popc(trimask), below, pre, total);
         if (total >= 64u || 4u * (uint32_t)__popcll(waiting) >= (uint32_t)__popcll(live)) {
-            wave_triangles<ANY, kBlk>(src, sl, ts, trimask, tri_base, lane, pre, total, tri_key, tri_bary);
+            wave_triangles<ANY, kBlk>(src, sl, ts, trimask, tri_base, lane, pre, total, tri_key, tri_hit);
             trimask = 0u;
         }
         if (busy && ((done && trimask == 0u) || (ANY && ts.res.occluded))) {
+            if (ts.res.overflow) {
+                uint32_t *f = src.fault_word(sl);
+                if (f) atomicOr(f, 1u);
+            }
             if (ANY) {
                 occluded[ray] = ts.res.occluded ? 1 : 0;
             } else {
@@ -353,9 +374,10 @@ template <bool ANY, int kStack>
 __global__ __launch_bounds__(refill_block<kStack>()) void trace_refill_kernel(const U4 *__restrict__ nodes, const F4 *__restrict__ tris,
                                                               RayArgs rays, HitArgs hits, uint8_t *__restrict__ occluded,
                                                               uint32_t n, const uint32_t *__restrict__ count,
-                                                              float root_max) {
+                                                              float root_max, uint32_t *__restrict__ fault, int cap) {
     if (count) n = min(n, *count);
-    refill_trace<ANY, kStack, refill_block<kStack>()>(OneScene{nodes, tris, root_max}, rays, hits, occluded, n);
+    refill_trace<ANY, kStack, refill_block<kStack>()>(OneScene{nodes, tris, root_max, fault, cap}, rays, hits, occluded,
+                                                      n);
 }
 
 template <bool ANY>
@@ -465,6 +487,7 @@ hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitA
     const F4 *tr = reinterpret_cast<const F4 *>(scene.tris);
     const int need = scene.depth > 1 ? (int)scene.depth - 1 : 1;
     if (need > 24) return hipErrorInvalidValue;
+    const int cap = scene.stack_cap ? (int)scene.stack_cap : 24;
     if (!counters) {
         // Persistent refill kernel: as many blocks as the chip holds at once (never more than the rays
         // need), each draining its own range.
@@ -472,8 +495,8 @@ hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitA
     do {                                                                                                               \
         constexpr int b = refill_block<S>();                                                                          \
         const uint32_t g = refill_grid(any ? (const void *)trace_refill_kernel<true, S> : (const void *)trace_refill_kernel<false, S>, n, b); \
-        if (any) hipLaunchKernelGGL((trace_refill_kernel<true, S>), dim3(g), dim3(b), 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max); \
-        else hipLaunchKernelGGL((trace_refill_kernel<false, S>), dim3(g), dim3(b), 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max); \
+        if (any) hipLaunchKernelGGL((trace_refill_kernel<true, S>), dim3(g), dim3(b), 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, scene.fault, cap < S ? cap : S); \
+        else hipLaunchKernelGGL((trace_refill_kernel<false, S>), dim3(g), dim3(b), 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, scene.fault, cap < S ? cap : S); \
     } while (0)
         if (need <= 8) VT_REFILL(8);
         else if (need <= 12) VT_REFILL(12);
@@ -485,11 +508,11 @@ hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitA
 #define VT_LAUNCH(S)                                                                                                   \
     do {                                                                                                               \
         if (counters) {                                                                                                \
-            if (any) hipLaunchKernelGGL((trace_kernel<true, true, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, counters); \
-            else hipLaunchKernelGGL((trace_kernel<false, true, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, counters); \
+            if (any) hipLaunchKernelGGL((trace_kernel<true, true, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, counters, scene.fault, cap < S ? cap : S); \
+            else hipLaunchKernelGGL((trace_kernel<false, true, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, counters, scene.fault, cap < S ? cap : S); \
         } else {                                                                                                       \
-            if (any) hipLaunchKernelGGL((trace_kernel<true, false, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, nullptr); \
-            else hipLaunchKernelGGL((trace_kernel<false, false, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, nullptr); \
+            if (any) hipLaunchKernelGGL((trace_kernel<true, false, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, nullptr, scene.fault, cap < S ? cap : S); \
+            else hipLaunchKernelGGL((trace_kernel<false, false, S>), grid, block, 0, stream, nd, tr, rays, hits, occluded, n, count, scene.root_max, nullptr, scene.fault, cap < S ? cap : S); \
         }                                                                                                              \
     } while (0)
     if (need <= 8) VT_LAUNCH(8);

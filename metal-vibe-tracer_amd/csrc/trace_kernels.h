@@ -19,6 +19,12 @@ struct TraceScene {
     uint32_t node_count;
     uint32_t depth;  // tree depth (root = 1); the traversal stack holds depth - 1 groups
     float root_max;
+    // Device word raised (bit 0) by a ray whose node-group stack filled: the ray ends early with the
+    // hit found so far.  The launcher sizes the stack to depth - 1 groups, which a traversal never
+    // exceeds (one pending group per level), so only a stack capped below that (stack_cap, a test
+    // hook; 0 = no cap) can raise it.
+    uint32_t *fault;
+    uint32_t stack_cap;
 };
 
 // Closest hit (any = false) into `hits`, or any hit into `occluded`.  `counters` (device, 128

@@ -119,6 +119,7 @@ struct TraceResult {
     float t, b1, b2;
     int32_t face;    // -1: miss
     bool occluded;   // any-hit result
+    bool overflow;   // the node-group stack filled: the ray ended before its traversal did
     uint32_t nodes;  // node records fetched
     uint32_t tris;   // triangle records tested
 };
@@ -140,6 +141,7 @@ VT_HD void trav_init(TravState &st, const TraceRay &r, float root_max) {
     st.res.b1 = st.res.b2 = 0.0f;
     st.res.face = -1;
     st.res.occluded = false;
+    st.res.overflow = false;
     st.res.nodes = st.res.tris = 0;
     st.w = woop_ray(r.dx, r.dy, r.dz);
     st.ox = r.ox;
@@ -218,9 +220,10 @@ VT_HD bool trav_nodes(TravState &st, const U4 *__restrict__ nodes, Stack &stack,
     }
     if (hitbits) {
         if (st.cur_bits & 0xffu) {
-            // The builder bounds the depth so a full stack never occurs; if it did, stop this ray
-            // (a wrong result, never an out-of-bounds access).
+            // The launcher sizes the stack to the tree's depth, so a full stack never occurs; if it did,
+            // the ray stops (never an out-of-bounds access) and the kernel reports it.
             if (stack.full()) {
+                st.res.overflow = true;
                 trimask = 0;
                 return true;
             }

@@ -35,7 +35,8 @@ EXPORTED = [
     "vt_batch_readback", "vt_batch_get_tuning", "vt_batch_set_tuning", "vt_batch_autotune",
     "vt_mesh_payload_bytes", "vt_mesh_payload_write", "vt_mesh_payload_read", "vt_obj_parse", "vt_obj_destroy",
     "vt_obj_part_count", "vt_obj_skipped", "vt_obj_library_count", "vt_obj_library", "vt_obj_part", "vt_obj_subset",
-    "vt_z_draws",
+    "vt_z_draws", "vt_context_check", "vt_scene_set_stack_cap", "vt_mesh_from_triangles",
+    "vt_scene_upload_triangles", "vt_scene_set_surface_settings", "vt_renderer_set_callbacks",
 ]
 
 
@@ -96,6 +97,10 @@ def z_draws(width, height, frame_index, sample_index, temporal_model=0, path_ver
     import torch
     if out is None:
         out = torch.empty((height, width, draws, kind), dtype=torch.float32, device="cuda:0")
+    elif (out.dtype != torch.float32 or not out.is_cuda or not out.is_contiguous()
+          or out.numel() < height * width * draws * kind):
+        raise ValueError("out must be a contiguous float32 device tensor of at least height * width * draws * kind "
+                         "elements")
     q = ZQuery(width, height, frame_index, sample_index, temporal_model, path_vertex, stream, candidate,
                candidate_bits, draws, kind)
     _check(load().vt_z_draws(ctypes.byref(q), ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream_handle)))
@@ -203,6 +208,12 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_obj_part": (i32, [vp, u32, P(c.c_char_p), P(c.c_char_p), P(u32), P(vp), P(u32), P(vp), P(u32)]),
         "vt_obj_subset": (c.c_char_p, [vp, u32, u32]),
         "vt_z_draws": (i32, [P(ZQuery), vp, vp]),
+        "vt_context_check": (i32, [vp]),
+        "vt_mesh_from_triangles": (i32, [vp, u32, P(vp), P(u32), P(vp), P(u32)]),
+        "vt_scene_upload_triangles": (i32, [vp, vp, u32, P(_BuildOptions), P(vp)]),
+        "vt_scene_set_surface_settings": (i32, [vp, vp, u32, vp, u32]),
+        "vt_renderer_set_callbacks": (i32, [vp, FRAME_FN, ERROR_FN, vp]),
+        "vt_scene_set_stack_cap": (i32, [vp, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -387,6 +398,59 @@ def read_mesh_payload(data: bytes, maximum_faces=1 << 30) -> Mesh:
     return Mesh(v, f)
 
 
+def mesh_from_triangles(triangles) -> Mesh:
+    """MeshGeometry(triangles:): (n, 32) float32 MeshTriangle records -> welded indexed Mesh."""
+    lib = load()
+    t = np.ascontiguousarray(triangles, np.float32).reshape(-1, 32)
+    vp, fp = ctypes.c_void_p(), ctypes.c_void_p()
+    nv, nf = ctypes.c_uint32(), ctypes.c_uint32()
+    _check(lib.vt_mesh_from_triangles(_ptr(t) if len(t) else None, len(t), ctypes.byref(vp), ctypes.byref(nv),
+                                      ctypes.byref(fp), ctypes.byref(nf)))
+    try:
+        v = np.frombuffer(ctypes.string_at(vp, 32 * nv.value), np.float32).reshape(-1, 8).copy()
+        f = np.frombuffer(ctypes.string_at(fp, 16 * nf.value), np.uint32).reshape(-1, 4).copy()
+    finally:
+        lib.vt_host_free(vp)
+        lib.vt_host_free(fp)
+    return Mesh(v, f)
+
+
+def mesh_triangles(mesh: Mesh) -> np.ndarray:
+    """The MeshTriangle records (n, 32) of an indexed mesh (MeshTriangle(face:vertices:)): corners w = 1,
+    normals w = 0, uvc = (uv of c, subset, 0)."""
+    v, f = mesh.vertices, mesh.faces
+    t = np.zeros((len(f), 32), np.float32)
+    for k in range(3):
+        c = v[f[:, k]]
+        t[:, 4 * k:4 * k + 3] = c[:, :3]
+        t[:, 4 * k + 3] = 1.0
+        t[:, 12 + 4 * k:15 + 4 * k] = c[:, 3:6]
+    t[:, 24:26] = v[f[:, 0], 6:8]
+    t[:, 26:28] = v[f[:, 1], 6:8]
+    t[:, 28:30] = v[f[:, 2], 6:8]
+    t[:, 30] = f[:, 3].astype(np.float32)
+    return t
+
+
+class SurfaceSettings(ctypes.Structure):
+    """SurfaceSettings (main.swift:2253-2263), 64 bytes, with the reference's host defaults."""
+    _fields_ = [("color", ctypes.c_float * 4), ("surface", ctypes.c_float * 4), ("detail", ctypes.c_float * 4),
+                ("enabled", ctypes.c_uint32), ("map_mask", ctypes.c_uint32), ("normal_strength", ctypes.c_float),
+                ("padding", ctypes.c_uint32)]
+
+    def __init__(self, color=(1.0, 1.0, 1.0), roughness=0.3, metalness=0.0, coat=0.0, anisotropy=0.0, fuzz=0.0,
+                 ior=1.5, transmission=0.0, uv_repeat=1.0, enabled=1, map_mask=0, normal_strength=1.0):
+        super().__init__((ctypes.c_float * 4)(*color[:3], 1.0), (ctypes.c_float * 4)(roughness, metalness, coat, anisotropy),
+                         (ctypes.c_float * 4)(fuzz, ior, transmission, uv_repeat), enabled, map_mask, normal_strength, 0)
+
+
+assert ctypes.sizeof(SurfaceSettings) == 64
+
+
+class Emission(ctypes.Structure):
+    _fields_ = [("slot", ctypes.c_uint32), ("rgb", ctypes.c_float * 3)]
+
+
 def read_obj(text, node_budget=0, triangle_limit=0):
     """Parses OBJ text (str or bytes).  Returns (parts, skipped, libraries); each part is a dict with
     object, group, subsets (material names) and mesh (welded vertices, faces tagged by subset)."""
@@ -485,6 +549,10 @@ class Context:
         except Exception:
             pass
 
+    def check(self):
+        """vt_context_check: waits for the device; raises VTError for a fault its kernels reported."""
+        _check(load().vt_context_check(self._h))
+
     def energy_tables(self):
         """(metal (129, 129), metal averages (129,), luts (5, 32, 32, 32)) as baked on the device."""
         metal = np.empty(129 * 129 + 129, np.float32)
@@ -503,6 +571,15 @@ class Context:
         o = _opts(leaf_size, bins, max_depth, collapse)
         _check(lib.vt_scene_upload_indexed(self._h, _ptr(v), len(v), _ptr(f), len(f), ctypes.byref(o),
                                            ctypes.byref(h)))
+        return Scene(self, h)
+
+    def set_triangles(self, triangles, leaf_size=0, bins=0, max_depth=0, collapse=0) -> "Scene":
+        """MaterialLibrary.setMesh(_ triangles: [MeshTriangle]): (n, 32) float32 records."""
+        t = np.ascontiguousarray(triangles, np.float32).reshape(-1, 32)
+        h = ctypes.c_void_p()
+        o = _opts(leaf_size, bins, max_depth, collapse)
+        _check(load().vt_scene_upload_triangles(self._h, _ptr(t) if len(t) else None, len(t), ctypes.byref(o),
+                                                ctypes.byref(h)))
         return Scene(self, h)
 
 
@@ -569,15 +646,34 @@ class Scene:
     def set_light_tree(self, enabled: bool):
         _check(load().vt_scene_set_light_tree(self._h, 1 if enabled else 0))
 
+    def set_stack_cap(self, groups: int):
+        """Test hook (vt_scene_set_stack_cap): caps the traversal stack at `groups` node groups (0: none)."""
+        _check(load().vt_scene_set_stack_cap(self._h, groups))
+
+    def rebuild(self, leaf_size=0, bins=0, max_depth=0, collapse=0):
+        """vt_scene_rebuild: the hierarchy again from the scene's own streams with other options."""
+        _check(load().vt_scene_rebuild(self._h, ctypes.byref(_opts(leaf_size, bins, max_depth, collapse))))
+
     def set_materials(self, materials):
         arr = (Material * len(materials))(*materials)
         _check(load().vt_scene_set_materials(self._h, arr, len(materials)))
+
+    def set_surface_settings(self, settings, emissions=None):
+        """settings: list of SurfaceSettings per slot; emissions: {slot: (r, g, b)}."""
+        s = (SurfaceSettings * max(1, len(settings)))(*settings)
+        em = [Emission(k, (ctypes.c_float * 3)(*rgb)) for k, rgb in (emissions or {}).items()]
+        e = (Emission * max(1, len(em)))(*em)
+        _check(load().vt_scene_set_surface_settings(self._h, s, len(settings), e, len(em)))
 
     def trace_stats(self, rays, any_hit=False, stream=None):
         rs, n = _rays_struct(rays)
         c = (ctypes.c_uint64 * 2)()
         _check(load().vt_trace_stats(self._h, ctypes.byref(rs), n, 1 if any_hit else 0, c, _stream_ptr(stream)))
         return int(c[0]), int(c[1])
+
+
+FRAME_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32)
+ERROR_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_char_p)
 
 
 class Renderer:
@@ -612,6 +708,14 @@ class Renderer:
 
     def reset(self):
         _check(load().vt_reset_accumulation(self._h))
+
+    def set_callbacks(self, on_frame=None, on_error=None):
+        """vt_renderer_set_callbacks: on_frame(frames) after each render call's samples (on a runtime
+        thread), on_error(message) when a call fails."""
+        f = FRAME_FN((lambda user, frames: on_frame(frames)) if on_frame else 0)
+        e = ERROR_FN((lambda user, msg: on_error(msg.decode())) if on_error else 0)
+        self._callbacks = (f, e)          # kept alive while attached
+        _check(load().vt_renderer_set_callbacks(self._h, f, e, None))
 
     def cache_records(self, max_records=1 << 22):
         """(inputs (n, 11), targets (n, 4), made) of the last wavefront's cache training records."""

@@ -22,37 +22,17 @@ TERRAIN_TARGET = (0.0, 0.0, 30.0)
 TERRAIN_FOV = 40.0
 
 
-def emitter_terrain(kind="restir", grid=2237, emitters=4096, slots=64, seed=3, power=(1.0, 100.0)):
-    """Returns (mesh, materials): face tags 0..slots-1 pick a surface slot, slots + k emitter k."""
-    import vt
+def _terrain_draws(kind, nf, emitters, slots, seed, power):
+    """The recipe's random draws, in their fixed order: face tags, emitter centres, heights above the
+    terrain, yaws, powers, then the surface palette (slot materials)."""
     if kind not in ("restir", "openpbr"):
         raise ValueError("kind must be 'restir' or 'openpbr'")
-    mesh = vt.heightfield(grid=grid, octaves=8, gain=0.5, half_extent=100.0, height=12.0, seed=seed)
-    nf, nv = len(mesh.faces), len(mesh.vertices)
     rng = np.random.default_rng(seed)
     tags = rng.integers(0, slots, nf, dtype=np.uint32)
-    # Emitters: unwelded triangles appended after the terrain.
     cx, cz = rng.uniform(-90.0, 90.0, emitters), rng.uniform(-90.0, 90.0, emitters)
-    step = 200.0 / (grid - 1)
-    gi = np.clip(np.rint((cx + 100.0) / step), 0, grid - 1).astype(np.int64)
-    gj = np.clip(np.rint((cz + 100.0) / step), 0, grid - 1).astype(np.int64)
-    cy = mesh.vertices[gj * grid + gi, 1].astype(np.float64) + rng.uniform(2.0, 6.0, emitters)
+    lift = rng.uniform(2.0, 6.0, emitters)
     yaw = rng.uniform(0.0, 2.0 * np.pi, emitters)
-    # Corners at yaw, yaw + 120, yaw + 240 degrees: (b - a) x (c - a) points down (-y).
-    ang = yaw[:, None] + np.array([0.0, 2.0, 4.0]) * np.pi / 3.0
-    corners = np.stack([cx[:, None] + 0.35 * np.cos(ang), np.repeat(cy[:, None], 3, 1), cz[:, None] + 0.35 * np.sin(ang)], -1)
-    ev = np.zeros((3 * emitters, 8), np.float32)
-    ev[:, :3] = corners.reshape(-1, 3)
-    ev[:, 4] = -1.0
-    ef = np.zeros((emitters, 4), np.uint32)
-    ef[:, :3] = (nv + np.arange(3 * emitters)).reshape(emitters, 3)
-    ef[:, 3] = slots + np.arange(emitters, dtype=np.uint32)
-    mesh.faces[:, 3] = tags
-    mesh.vertices = np.concatenate([mesh.vertices, ev])
-    mesh.faces = np.concatenate([mesh.faces, ef])
-    area = 0.75 * np.sqrt(3.0) * 0.35 ** 2
     watts = rng.uniform(power[0], power[1], emitters)
-    radiance = watts / (np.pi * area)
     mats = []
     for k in range(slots):
         r = rng.random(4)
@@ -70,6 +50,42 @@ def emitter_terrain(kind="restir", grid=2237, emitters=4096, slots=64, seed=3, p
                 mats.append(dict(type=4, albedo=tuple(0.5 + 0.45 * r[:3]), roughness=rough, ior=1.5, metalness=1.0))
             else:
                 mats.append(dict(type=4, albedo=tuple(0.8 + 0.2 * r[:3]), roughness=rough, ior=ior, transmission=1.0))
+    return tags, cx, cz, lift, yaw, watts, mats
+
+
+def surface_palette(kind="restir", grid=2237, emitters=4096, slots=64, seed=3, power=(1.0, 100.0)):
+    """The surface slots' materials of emitter_terrain(kind) without building the terrain."""
+    nf = 2 * (grid - 1) * (grid - 1)
+    return _terrain_draws(kind, nf, emitters, slots, seed, power)[-1]
+
+
+def emitter_terrain(kind="restir", grid=2237, emitters=4096, slots=64, seed=3, power=(1.0, 100.0)):
+    """Returns (mesh, materials): face tags 0..slots-1 pick a surface slot, slots + k emitter k."""
+    import vt
+    if kind not in ("restir", "openpbr"):
+        raise ValueError("kind must be 'restir' or 'openpbr'")
+    mesh = vt.heightfield(grid=grid, octaves=8, gain=0.5, half_extent=100.0, height=12.0, seed=seed)
+    nf, nv = len(mesh.faces), len(mesh.vertices)
+    tags, cx, cz, lift, yaw, watts, mats = _terrain_draws(kind, nf, emitters, slots, seed, power)
+    # Emitters: unwelded triangles appended after the terrain.
+    step = 200.0 / (grid - 1)
+    gi = np.clip(np.rint((cx + 100.0) / step), 0, grid - 1).astype(np.int64)
+    gj = np.clip(np.rint((cz + 100.0) / step), 0, grid - 1).astype(np.int64)
+    cy = mesh.vertices[gj * grid + gi, 1].astype(np.float64) + lift
+    # Corners at yaw, yaw + 120, yaw + 240 degrees: (b - a) x (c - a) points down (-y).
+    ang = yaw[:, None] + np.array([0.0, 2.0, 4.0]) * np.pi / 3.0
+    corners = np.stack([cx[:, None] + 0.35 * np.cos(ang), np.repeat(cy[:, None], 3, 1), cz[:, None] + 0.35 * np.sin(ang)], -1)
+    ev = np.zeros((3 * emitters, 8), np.float32)
+    ev[:, :3] = corners.reshape(-1, 3)
+    ev[:, 4] = -1.0
+    ef = np.zeros((emitters, 4), np.uint32)
+    ef[:, :3] = (nv + np.arange(3 * emitters)).reshape(emitters, 3)
+    ef[:, 3] = slots + np.arange(emitters, dtype=np.uint32)
+    mesh.faces[:, 3] = tags
+    mesh.vertices = np.concatenate([mesh.vertices, ev])
+    mesh.faces = np.concatenate([mesh.faces, ef])
+    area = 0.75 * np.sqrt(3.0) * 0.35 ** 2
+    radiance = watts / (np.pi * area)
     for L in radiance:
         mats.append(dict(type=3, albedo=(0.0, 0.0, 0.0), emission=(float(L),) * 3))
     return mesh, mats

@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 _lib = None
+_lib_path = None
 
 
 class WideNode(ctypes.Structure):
@@ -42,13 +43,23 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def load():
-    global _lib
-    if _lib is not None:
+def native_library() -> str:
+    """Builds the oracle with -O3 -march=native for this host (oracle/_native/liboracle.so, a few
+    seconds) and returns its path: the build of the CPU baseline."""
+    subprocess.run(["make", "-s", "-C", HERE, "native"], check=True)
+    return os.path.join(HERE, "_native", "liboracle.so")
+
+
+def load(path=None):
+    """The oracle library: oracle/liboracle.so, or the library at `path` from then on (every later
+    call uses it)."""
+    global _lib, _lib_path
+    path = path or _lib_path or LIB_PATH
+    if _lib is not None and path == _lib_path:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} missing: run make -C oracle")
-    lib = ctypes.CDLL(LIB_PATH)
+    if not os.path.exists(path):
+        raise ImportError(f"{path} missing: run make -C oracle")
+    lib = ctypes.CDLL(path)
     c = ctypes
     vp, f32, u32, i32 = c.c_void_p, c.c_float, c.c_uint32, c.c_int32
     P = c.POINTER
@@ -73,6 +84,7 @@ def load():
         "vo_bsdf_sample_f32": (c.c_int, [vp, vp, vp, vp, c.c_int, vp, vp, vp, vp]),
         "vo_bsdf_eval_f64": (None, [vp, vp, vp, vp, vp, c.c_int, vp, vp]),
         "vo_bsdf_sample_f64": (c.c_int, [vp, vp, vp, vp, c.c_int, vp, vp, vp, vp]),
+        "vo_energy_set_dielectric": (None, [vp]),
         "vo_energy_table": (c.POINTER(c.c_uint16), [c.c_int]),
         "vo_metal_energy_table": (c.POINTER(c.c_float), []),
         "vo_energy_ior_node": (c.c_float, [c.c_int]),
@@ -100,8 +112,24 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
+    tables = reference_energy_tables()
+    flat = np.concatenate([tables[k].ravel() for k in DIELECTRIC_KEYS]).astype(np.uint16)
+    lib.vo_energy_set_dielectric(flat.ctypes.data_as(ctypes.c_void_p))
+    _lib, _lib_path = lib, path
     return lib
+
+
+# OpenPBR's dielectric tables in the oracle's LUT order (VO_LUT_* in oracle.h).
+DIELECTRIC_KEYS = ("ideal_complement", "ideal_avg_complement", "reflection_ratio", "opaque_complement",
+                   "opaque_avg_complement")
+ENERGY_FIXTURE = os.path.join(os.path.dirname(HERE), "tests", "golden", "openpbr_energy_tables.npz")
+
+
+def reference_energy_tables():
+    """The reference's energy tables as committed data (tests/golden/make_energy_golden.py): OpenPBR's
+    vendored unorm16 dielectric arrays and the reference's ideal-metal table (MetalEnergy.metal)."""
+    with np.load(ENERGY_FIXTURE) as z:
+        return {k: z[k] for k in z.files}
 
 
 def _p(a):

@@ -177,12 +177,15 @@ void vo_render_restir_f32(const vo_render_scene *sc, const vo_frame *fr, uint32_
 void vo_render_restir_f64(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
                           uint32_t candidates, const float *eyes, int threads, double *out_rgb);
 
-/* Energy-compensation tables of the layered BSDF (energy.c), baked on first use.
- * vo_energy_table(id): unorm16, 32 x 32 x 32 (IOR, roughness, cosine) or 32 x 32 (IOR, roughness).
+/* Energy-compensation tables of the layered BSDF (energy.c).
+ * vo_energy_set_dielectric(tables): OpenPBR's five dielectric unorm16 tables back to back, in
+ * VO_LUT order (32768, 1024, 1024, 32768, 1024 values); must precede any table read.
+ * vo_energy_table(id): unorm16, 32 x 32 x 32 (IOR, sqrt(alpha), cosine) or 32 x 32 (IOR, sqrt(alpha)).
  * vo_metal_energy_table(): 129 x 129 ideal-metal complements (roughness, sqrt cosine), then the
  * 129 average complements. */
 enum { VO_LUT_DIELECTRIC = 0, VO_LUT_DIELECTRIC_AVG, VO_LUT_DIELECTRIC_RATIO, VO_LUT_OPAQUE, VO_LUT_OPAQUE_AVG,
        VO_LUT_COUNT };
+void vo_energy_set_dielectric(const uint16_t *tables);
 const uint16_t *vo_energy_table(int id);
 const float *vo_metal_energy_table(void);
 float vo_energy_ior_node(int i);

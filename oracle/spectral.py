@@ -8,9 +8,18 @@ Follows scripts/generate_spectral_tables.py (the reference's offline generator):
   inverse_cdf              :507-530  exact inversion of the piecewise-linear density, 1025 nodes
   rgb_and_jacobian,
   solve_lagrange           :276-340  Levenberg-Marquardt in Lagrange-multiplier space
-and main.swift:895-903 (spectral_lagrange / spectral_from_lagrange: greys flat, bounded MESE
+  moments_of_lagrange      :227-239  bounded moments by the 1024-node midpoint rule
+  solve_code               :381-386  exact moments of an 8-bit code (the coarse grid's nodes)
+  code_error               :357-366  8-bit round trip of a code through the interpolated moments
+and main.swift's conversion of a colour to a reflectance: spectral_moments_coarse /
+spectral_moments :855-884 (trilinear interpolation in linear light on the 86^3 grid of exact solves,
+refined cells at one-code spacing), the refinement rule of spectral_refine_flags_kernel :8223-8233
+(a cell is refined when one of its codes comes back more than 0.35 steps off; node codes counted in
+the cell above them), SpectralColour.fourierLagrange :8640-8680 (bounded-MESE multipliers with
+Levinson's recursion and biasing), spectral_lagrange / spectral_from_lagrange :886-903 (greys flat,
 reflectance atan(L0 + 2 L1 cos p + 2 L2 cos 2p) / pi + 1/2).  Input data: the CIE tables and
 Peters' warp as vendored by the reference (numbers in metal-vibe-tracer_amd/csrc/spectral_data.inc).
+Pinned by tests/golden/spectral_uplift.npz, made with the reference's own generator.
 """
 from __future__ import annotations
 
@@ -34,6 +43,60 @@ def _arrays():
         body = re.sub(r"//[^\n]*", "", m.group(1))
         out[name] = np.array([float(v) for v in re.findall(r"[-+0-9.eE]+", body)])
     return out["kCieXyz"].reshape(SAMPLES, 3), out["kD65"], out["kPhaseWarp"]
+
+
+def srgb_eotf(v):
+    return v / 12.92 if v <= 0.04045 else ((v + 0.055) / 1.055) ** 2.4
+
+
+def srgb_oetf(v):
+    v = min(max(float(v), 0.0), 1.0)
+    return 12.92 * v if v <= 0.0031308 else 1.055 * v ** (1.0 / 2.4) - 0.055
+
+
+def moments_of_lagrange(lam, nodes=1024):
+    """c_j = (1/pi) int_{-pi}^{0} g(phi) cos(j phi) dphi, midpoint rule with `nodes` nodes."""
+    h = math.pi / nodes
+    cp = np.cos(-math.pi + (np.arange(nodes) + 0.5) * h)
+    c2p = 2.0 * cp * cp - 1.0
+    v = np.arctan(lam[0] + 2.0 * lam[1] * cp + 2.0 * lam[2] * c2p) / math.pi + 0.5
+    return np.array([v.sum(), (v * cp).sum(), (v * c2p).sum()]) * h / math.pi
+
+
+def fourier_lagrange(c):
+    """Bounded-MESE Lagrange multipliers of moments c (Peters et al. 2019, Eqs. 6, 7, 10), Levinson's
+    recursion with the biasing that pulls invalid moments back (keep 0.9999, then 0)."""
+    pi = math.pi
+    c0 = min(max(float(c[0]), EPSILON), 1.0 - EPSILON)
+    g0p = complex(math.sin(pi * c0), -math.cos(pi * c0)) / (4.0 * pi)
+    g0 = 2.0 * g0p.real
+    g1 = 2j * pi * c[1] * g0p
+    g2 = 1j * pi * (2.0 * c[2] * g0p + c[1] * g1)
+    q0 = 1.0 / g0
+    u1 = q0 * g1
+    n1, keep = abs(u1) ** 2, 0.9999
+    if n1 >= 1.0:
+        u1 *= keep / math.sqrt(n1)
+        g1 = u1 / q0
+        n1, keep = keep * keep, 0.0
+    d1 = 1.0 / (1.0 - n1)
+    a0, a1 = q0 * d1, -u1 * q0 * d1
+    u2 = a0 * g2 + a1 * g1
+    n2 = abs(u2) ** 2
+    if n2 >= 1.0:
+        u2 *= keep / math.sqrt(n2)
+        g2 = (u2 - a1 * g1) / a0
+        n2 = keep * keep
+    d2 = 1.0 / (1.0 - n2)
+    b = (a0 * d2, (a1 - u2 * a1.conjugate()) * d2, -u2 * a0 * d2)
+    gamma = (g0p, g1, g2)
+    out = []
+    for l in range(3):
+        acc = 0j
+        for k in range(3 - l):
+            acc += gamma[k] * sum(b[j + k + l].conjugate() * b[j] for j in range(3 - k - l))
+        out.append(2.0 * acc.imag / b[0].real)
+    return np.array(out)
 
 
 class Data:
@@ -124,11 +187,70 @@ class Data:
             lam, jac, f, err = trial, jac_t, f_t, err_t
         return lam, err
 
+    # -------------------------------------------------------------------- uplift grid
+    def code_moments(self, code):
+        """Exact moments of an 8-bit code, float32 as the grid file stores them."""
+        cache = self.__dict__.setdefault("_exact", {})
+        code = tuple(int(v) for v in code)
+        if code not in cache:
+            lam, _ = self.solve_lagrange([srgb_eotf(v / 255.0) for v in code])
+            cache[code] = moments_of_lagrange(lam).astype(np.float32)
+        return cache[code]
+
+    def _interpolate(self, c, base, step, t):
+        m = np.zeros(3)
+        for a in (0, 1):
+            for b in (0, 1):
+                for d in (0, 1):
+                    w = (t[0] if a else 1 - t[0]) * (t[1] if b else 1 - t[1]) * (t[2] if d else 1 - t[2])
+                    if w == 0.0:
+                        continue
+                    code = [min(base[k] + (a, b, d)[k] * step, 255) for k in range(3)]
+                    m += w * self.code_moments(code).astype(np.float64)
+        return m
+
+    def coarse_moments(self, c, cell):
+        nodes = np.float32([srgb_eotf(3.0 * i / 255.0) for i in range(86)]).astype(np.float64)
+        lo, hi = nodes[list(cell)], nodes[[i + 1 for i in cell]]
+        t = np.clip((np.asarray(c, np.float64) - lo) / (hi - lo), 0.0, 1.0)
+        return self._interpolate(c, [3 * i for i in cell], 3, t)
+
+    def refined(self, cell):
+        """A cell whose codes come back from the coarse conversion more than 0.35 8-bit steps off."""
+        flags = self.__dict__.setdefault("_refined", {})
+        cell = tuple(int(i) for i in cell)
+        if cell not in flags:
+            ranges = [range(3 * i, 3 * i + (4 if i == 84 else 3)) for i in cell]
+            worst = 0.0
+            for r in ranges[0]:
+                for g in ranges[1]:
+                    for b in ranges[2]:
+                        if r == g == b:
+                            continue
+                        c = [srgb_eotf(v / 255.0) for v in (r, g, b)]
+                        lam = fourier_lagrange(self.coarse_moments(c, cell))
+                        rgb = self.weights @ (np.arctan(lam[0] + lam[1] * self.cos1 + lam[2] * self.cos2) / math.pi + 0.5)
+                        worst = max(worst, max(abs(srgb_oetf(rgb[k]) * 255.0 - (r, g, b)[k]) for k in range(3)))
+            flags[cell] = worst > 0.35
+        return flags[cell]
+
+    def uplift_moments(self, c):
+        c = np.asarray(c, np.float64)
+        code85 = np.array([srgb_oetf(v) * 85.0 for v in c])
+        cell = [min(max(int(v), 0), 84) for v in code85]
+        if not self.refined(cell):
+            return self.coarse_moments(c, cell)
+        s = [min(max(int(code85[k] * 3.0 - 3.0 * cell[k]), 0), 2) for k in range(3)]
+        base = [3 * cell[k] + s[k] for k in range(3)]
+        lo = np.array([srgb_eotf(b / 255.0) for b in base])
+        hi = np.array([srgb_eotf((b + 1) / 255.0) for b in base])
+        return self._interpolate(c, base, 1, np.clip((c - lo) / (hi - lo), 0.0, 1.0))
+
     def reflectance_lagrange(self, rgb):
         c = np.clip(np.asarray(rgb, np.float32), 0.0, 1.0)
         if c.max() - c.min() <= np.float32(1e-4):
             return np.array([(c[0] + c[1] + c[2]) / 3.0, 0.0, 0.0, 2.0])
-        lam, _ = self.solve_lagrange(c.astype(np.float64))
+        lam = fourier_lagrange(self.uplift_moments(c.astype(np.float64)))
         return np.array([lam[0], lam[1], lam[2], 1.0])
 
     def reflectance_spectrum(self, lagrange):
@@ -149,7 +271,7 @@ class Data:
         cmf[:, :3] = self.rgb_cmf
         cmf[:, 3] = np.cos(self.phases)
         corners = [(1, 0, 0), (0, 1, 0), (0, 0, 1), (0, 1, 1), (1, 0, 1), (1, 1, 0)]
-        spectra = [self.reflectance_spectrum(np.append(self.solve_lagrange(c)[0], 1.0)) for c in corners]
+        spectra = [self.reflectance_spectrum(self.reflectance_lagrange(c)) for c in corners]
         basis = np.zeros((SAMPLES, 2, 4), np.float32)
         for k in range(3):
             basis[:, 0, k] = spectra[k]

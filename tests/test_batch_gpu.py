@@ -1,11 +1,14 @@
 """GPU tests of batched rendering (vt_batch_*, metal-vibe-tracer_amd/csrc/batch.cpp): every job of a
-batch of uneven scenes and image sizes is the single-image renderer's estimator sample for sample
-(same pixel streams, same traversal), whatever the tuning -- chunk capacity, persistent grid,
-traversal work-group size, BVH leaf size -- and the autotuner ends on a valid tuning with the
-accumulation reset.  The single-image renderer is itself checked against the oracle elsewhere."""
+batch of uneven scenes and image sizes is the oracle's restatement of the reference integrator
+(oracle/integrator.c) pixel for pixel up to float order, and the single-image renderer's estimator
+sample for sample (same pixel streams, same traversal), whatever the tuning -- chunk capacity,
+persistent grid, traversal work-group size, BVH leaf size; the autotuner ends on a valid tuning with
+the accumulation reset; and the full 64-scene workload renders finite images whose jobs equal their
+single-image renders."""
 import numpy as np
 import pytest
 
+import oracle
 import scenes
 import vt
 from vt import workloads
@@ -50,6 +53,61 @@ def _single(ctx, job, spp):
     return img
 
 
+def test_batch_jobs_match_oracle(gpu_ctx):
+    # Each job against the oracle's MIS integrator with the same camera and sample indices 1..spp:
+    # the same sample streams, so pixels agree up to float-order differences.
+    sc = _scenes(gpu_ctx)
+    mats_of = {id(s): (m, s) for s, m, _ in sc}
+    jobs = _jobs(sc)
+    b = vt.Batch(gpu_ctx, jobs, max_depth=DEPTH)
+    spp = 4
+    b.render(spp)
+    meshes = [workloads.icosphere_scene(12_000, 5)[0], workloads.icosphere_scene(60_000, 6)[0],
+              scenes.materials_cornell("opaque")[0]]
+    mesh_of = {id(s): m for (s, _, _), m in zip(sc, meshes)}
+    for k, (scene, w, h, eye, target, fov) in enumerate(jobs):
+        mats, _ = mats_of[id(scene)]
+        mesh = mesh_of[id(scene)]
+        frame = dict(width=w, height=h, eye=eye, target=target, fov=fov, max_depth=DEPTH)
+        ref = oracle.render(oracle.Scene(mesh.vertices, mesh.faces), mats, frame, 1, spp).reshape(h, w, 3)
+        got = b.readback(k)[..., :3].astype(np.float64)
+        rel = abs(got.mean() / ref.mean() - 1)
+        close = np.isclose(got, ref, rtol=1e-3, atol=1e-4).all(axis=2).mean()
+        print(f"job {k} {w}x{h}: batch {got.mean():.6f} oracle {ref.mean():.6f} rel {rel:.2e} close {close:.4f}",
+              flush=True)
+        assert rel < 5e-3 and close > 0.95
+    b.close()
+    for s, _, _ in sc:
+        s.close()
+
+
+def test_full_batch_workload_properties(gpu_ctx):
+    # The benchmark's 64 icosphere scenes (10k-2M triangles, four image sizes) in one batch, 1 spp:
+    # every image finite and lit, and the four smallest jobs equal to their single-image renders.
+    work = workloads.batch_workload(count=64)
+    scs, jobs = [], []
+    for mesh, mats, w, h, eye, target, fov in work:
+        s = gpu_ctx.set_mesh(mesh)
+        s.set_materials(workloads.as_materials(mats))
+        scs.append(s)
+        jobs.append((s, w, h, eye, target, fov))
+    b = vt.Batch(gpu_ctx, jobs, max_depth=DEPTH)
+    b.render(1)
+    means = []
+    for k in range(len(jobs)):
+        img = b.readback(k)
+        assert np.isfinite(img).all(), k
+        means.append(float(img[..., :3].mean()))
+    print(f"64 jobs: mean radiance {np.mean(means):.5f}, min {min(means):.5f}", flush=True)
+    assert min(means) > 0
+    for k in sorted(range(len(jobs)), key=lambda j: jobs[j][1] * jobs[j][2])[:4]:
+        ref = _single(gpu_ctx, jobs[k], 1)
+        assert np.isclose(b.readback(k), ref, rtol=1e-4, atol=1e-6).all(axis=2).mean() > 0.999, k
+    b.close()
+    for s in scs:
+        s.close()
+
+
 def test_batch_matches_single_renders(gpu_ctx):
     sc = _scenes(gpu_ctx)
     jobs = _jobs(sc)
@@ -87,6 +145,29 @@ def test_tuning_does_not_change_the_image(gpu_ctx):
         print(f"tuning {b.tuning}: identical {same}", flush=True)
         assert same
     b.close()
+    for s, _, _ in sc:
+        s.close()
+
+
+def test_scene_rebuilt_under_a_batch(gpu_ctx):
+    # A batched scene rebuilt with another leaf size (new node and triangle buffers, another depth)
+    # between renders: the batch recaptures and renders what a freshly created batch renders.
+    sc = _scenes(gpu_ctx)
+    jobs = _jobs(sc)
+    b = vt.Batch(gpu_ctx, jobs, max_depth=DEPTH)
+    b.render(1)
+    leaf = sc[1][0].info["leaf_size"]
+    sc[1][0].rebuild(leaf_size=1 if leaf != 1 else 3)
+    assert sc[1][0].info["leaf_size"] != leaf
+    b.reset()
+    b.render(2)
+    got = [b.readback(k) for k in range(len(jobs))]
+    b.close()
+    fresh = vt.Batch(gpu_ctx, jobs, max_depth=DEPTH)
+    fresh.render(2)
+    for k in range(len(jobs)):
+        assert np.array_equal(got[k], fresh.readback(k)), k
+    fresh.close()
     for s, _, _ in sc:
         s.close()
 

@@ -1,9 +1,12 @@
 """CPU tests that pin the oracle's layered-BSDF subset (oracle/openpbr_impl.h) and its energy tables
-(oracle/energy.c) by what the mathematics fixes: the GGX single-scattering albedo by independent
-quadrature, Fresnel at zero roughness, sampler-evaluator consistency, pdf normalisation,
-reciprocity, white-furnace energy conservation of metal, glass and plastic with multiple-scattering
-compensation, and a planted mistake those checks must catch."""
+(oracle/energy.c): the tables against the reference's own data (OpenPBR's vendored dielectric arrays,
+the reference's MetalEnergy.metal), then by what the mathematics fixes: the GGX single-scattering
+albedo by independent quadrature, Fresnel at zero roughness, sampler-evaluator consistency, pdf
+normalisation, reciprocity, white-furnace energy conservation of metal, glass and plastic with
+multiple-scattering compensation, and a planted mistake those checks must catch."""
 import math
+import os
+import sys
 
 import numpy as np
 import pytest
@@ -48,6 +51,41 @@ def _metal_albedo_quadrature(alpha, mu, nt=4000, nphi=1000):
     return float(f.mean() * 2 * np.pi)
 
 
+REFERENCE = "/root/reference"
+
+
+def test_energy_tables_are_the_reference_data():
+    # The oracle's dielectric tables are OpenPBR's unorm16 arrays value for value, and its metal bake
+    # (the recipe of scripts/generate_metal_energy.py, float sums) lands within float rounding of the
+    # table the reference committed (Shaders/MetalEnergy.metal).
+    table, avg, luts, _ = oracle.energy_tables()
+    ref = oracle.reference_energy_tables()
+    pairs = zip(("dielectric", "dielectric_avg", "dielectric_ratio", "opaque", "opaque_avg"), oracle.DIELECTRIC_KEYS)
+    for nm, key in pairs:
+        assert np.array_equal(np.rint(luts[nm] * 65535).astype(np.uint16), ref[key]), nm
+    assert np.abs(table - ref["metal_complement"]).max() < 5e-6
+    assert np.abs(avg - ref["metal_avg_complement"]).max() < 5e-6
+    # Spot values read off the vendored files (reflection ratio row 0, opaque average node (31, 31)).
+    assert ref["reflection_ratio"][0, 0] == 10108 and ref["reflection_ratio"][0, 1] == 39576
+    assert ref["metal_complement"].shape == (129, 129) and np.all(ref["metal_complement"][0] == 0)
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE, "Vendor", "OpenPBR")), reason="reference checkout absent")
+def test_energy_fixture_equals_vendored_arrays():
+    # The committed fixture is the reference's data: re-read the vendored headers and compare.
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_energy_golden as g
+    ref = oracle.reference_energy_tables()
+    data = os.path.join(REFERENCE, "Vendor", "OpenPBR", "impl", "data")
+    import re
+    for key, fname, shape in g.DIELECTRIC:
+        vals = [int(v) for v in re.findall(r"\d+", g.strip_comments(open(os.path.join(data, fname)).read()))]
+        assert np.array_equal(np.array(vals, np.uint16).reshape(shape), ref[key]), key
+    m = g.metal_arrays(os.path.join(REFERENCE, "Shaders", "MetalEnergy.metal"))
+    assert np.array_equal(m["vibeMetalEnergy"].reshape(129, 129), ref["metal_complement"])
+    assert np.array_equal(m["vibeMetalAverageEnergy"], ref["metal_avg_complement"])
+
+
 def test_metal_table_matches_independent_quadrature():
     table, avg, _, _ = oracle.energy_tables()
     for i in (40, 64, 96, 128):
@@ -75,7 +113,7 @@ def _fresnel(eta, c):
 
 
 def test_dielectric_tables_reduce_to_fresnel_when_smooth():
-    # At roughness 0 the single-scattering reflection is Fresnel's and nothing is lost.
+    # At roughness 0 the reflection is Fresnel's and nothing is lost (a property of OpenPBR's data).
     _, _, luts, iors = oracle.energy_tables()
     for i in (0, 8, 20, 31):
         for k in (3, 16, 31):

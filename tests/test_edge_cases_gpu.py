@@ -90,3 +90,34 @@ def test_batch_of_one_job_and_an_empty_scene(gpu_ctx):
     r.close()
     e.close()
     s.close()
+
+
+def test_completion_callbacks(gpu_ctx):
+    # onFrameUpdate / onError stand-ins: each render call queues on_frame with the accumulated sample
+    # count after its work; a refused call reports its message through on_error before returning.
+    import threading
+    import torch
+    scene = gpu_ctx.set_mesh(vt.cornell())
+    scene.set_materials(vt.cornell_materials(17.0))
+    r = vt.Renderer(gpu_ctx)
+    frames, errors, threads = [], [], set()
+
+    def on_frame(n):
+        frames.append(n)
+        threads.add(threading.get_ident())
+
+    r.set_callbacks(on_frame, errors.append)
+    r.render(scene, 48, 48, vt.orbit_eye(), (0.0, -0.1, 0.0), 42.0, max_depth=4, samples=3, sample_index=1)
+    r.render(scene, 48, 48, vt.orbit_eye(), (0.0, -0.1, 0.0), 42.0, max_depth=4, samples=2, sample_index=4)
+    torch.cuda.synchronize()
+    for _ in range(100):
+        if len(frames) == 2:
+            break
+        threading.Event().wait(0.01)
+    assert frames == [3, 5] and r.frames == 5
+    with pytest.raises(vt.VTError):
+        r.render(scene, 48, 48, vt.orbit_eye(), (0.0, -0.1, 0.0), 42.0, max_depth=0, samples=1)
+    assert len(errors) == 1 and "max_depth" in errors[0]
+    r.set_callbacks(None, None)
+    r.close()
+    scene.close()

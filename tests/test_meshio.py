@@ -191,3 +191,36 @@ def test_obj_feeds_the_builder():
     assert np.array_equal(got.vertices[got.faces[:, :3]][..., :3], mesh.vertices[mesh.faces[:, :3]][..., :3])
     nodes, tris, depth, _ = vt.build_bvh8_host(got)
     assert len(tris) == len(got.faces) and depth >= 1
+
+
+# ------------------------------------------------------------------ MeshGeometry(triangles:)
+def test_triangles_weld_back_to_the_indexed_mesh():
+    # MeshTriangle records of an indexed mesh weld back to it: corners with identical position,
+    # normal and uv bits share a vertex (Sources/MeshGeometry.swift:35-57), in order of first use.
+    mesh = vt.icosphere_field(spheres=5, subdivisions=1, seed=4)
+    tris = vt.mesh_triangles(mesh)
+    assert tris.shape == (len(mesh.faces), 32)
+    back = vt.mesh_from_triangles(tris)
+    assert len(back.faces) == len(mesh.faces) and len(back.vertices) <= len(mesh.vertices)
+    assert np.array_equal(back.vertices[back.faces[:, :3]].view(np.uint32),
+                          mesh.vertices[mesh.faces[:, :3]].view(np.uint32))
+    assert np.array_equal(back.faces[:, 3], mesh.faces[:, 3])
+    # First use order: the first face's corners are vertices 0, 1, 2.
+    assert back.faces[0, :3].tolist() == [0, 1, 2]
+
+
+def test_triangle_welding_rules():
+    t = np.zeros((4, 32), np.float32)
+    corners = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    for i in range(4):
+        t[i, 0:3], t[i, 4:7], t[i, 8:11] = corners
+    t[1, 0] = -0.0                          # 0 and -0 stay distinct
+    t[2, 12] = 1.0                          # another normal at corner a: another vertex
+    t[:, 30] = [3.0, 7.9, np.nan, -1.0]     # subset: truncated, or 0 when not finite / negative
+    m = vt.mesh_from_triangles(t)
+    assert m.faces[:, 3].tolist() == [3, 7, 0, 0]
+    assert m.faces[0, :3].tolist() == [0, 1, 2] and m.faces[3, :3].tolist() == [0, 1, 2]
+    assert m.faces[1, :3].tolist() == [3, 1, 2] and m.faces[2, :3].tolist() == [4, 1, 2]
+    assert len(m.vertices) == 5
+    empty = vt.mesh_from_triangles(np.zeros((0, 32), np.float32))
+    assert len(empty.faces) == 0 and len(empty.vertices) == 0

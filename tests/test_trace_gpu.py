@@ -119,6 +119,58 @@ def test_exact_ties_and_negative_tmin(gpu_ctx, leaf):
     scene.close()
 
 
+def test_signed_zero_ties_on_coplanar_duplicates(gpu_ctx):
+    """Pairs of coincident triangles with opposite winding, rays starting inside them with tmin 0 and
+    -1: the pair's two hits sit at t = +0 and t = -0 (the sign of T / det), an exact tie that the
+    smaller face index must win, also when the wave's triangle tests fold them together."""
+    rng = np.random.default_rng(5)
+    n = 200
+    tris, rays = [], np.empty((8, 2 * n), np.float32)
+    for k in range(n):
+        a = rng.uniform(-1, 1, 3).astype(np.float32)
+        b, c = a + np.float32([0.25, 0, 0]), a + np.float32([0, 0, 0.25])
+        pair = [(a, b, c), (a, c, b)]
+        tris += pair if k % 2 else pair[::-1]
+        o = (a + np.float32(0.25) * (b - a) + np.float32(0.25) * (c - a)).astype(np.float32)
+        for j in range(2):
+            d = rng.normal(size=3)
+            rays[:3, 2 * k + j] = o
+            rays[3:6, 2 * k + j] = d / np.linalg.norm(d)
+            rays[6, 2 * k + j] = -float(j)
+            rays[7, 2 * k + j] = 1e20
+    v, f = scenes.mesh_from_triangles(np.array(tris, np.float32))
+    ref = oracle.Scene(v, f, with_bvh=False).closest(rays, mode=1)
+    assert np.mean(np.signbit(ref["t"]) & (ref["t"] == 0)) > 0.2     # the case is exercised
+    for leaf in (1, 2, 3):
+        scene = gpu_ctx.set_mesh(vt.Mesh(v, f), leaf_size=leaf)
+        got = _closest(scene, rays)
+        _assert_exact(got, ref, f"signed-zero ties leaf {leaf}")
+        assert np.array_equal(np.signbit(got["t"]), np.signbit(ref["t"])), "the winner's own t, sign included"
+        scene.close()
+
+
+def test_stack_overflow_is_reported(gpu_ctx):
+    """A traversal whose node-group stack fills ends the ray and raises the context's fault word, which
+    vt_context_check reports (and clears).  The stack is sized to the tree, so the test caps it with the
+    scene's test hook; every kernel still exits normally.  Without the cap the same rays are exact and
+    nothing is reported."""
+    v, f = scenes.scattered_triangles(3000, seed=6)
+    scene = gpu_ctx.set_mesh(vt.Mesh(v, f), leaf_size=1)
+    assert scene.info["depth"] >= 4
+    rays = scenes.random_rays(4000, 12)
+    gpu_ctx.check()
+    scene.set_stack_cap(1)
+    _closest(scene, rays)
+    _any(scene, rays)
+    with pytest.raises(vt.VTError, match="stack overflow"):
+        gpu_ctx.check()
+    gpu_ctx.check()                                   # reported once, then cleared
+    scene.set_stack_cap(0)
+    _assert_exact(_closest(scene, rays), oracle.Scene(v, f, with_bvh=False).closest(rays, mode=1), "uncapped")
+    gpu_ctx.check()
+    scene.close()
+
+
 def test_empty_scene_and_ragged_sizes(gpu_ctx):
     import torch
     empty = gpu_ctx.set_mesh(vt.Mesh(np.zeros((0, 8), np.float32), np.zeros((0, 4), np.uint32)))
@@ -156,13 +208,21 @@ def stress():
     return mesh, prim
 
 
-def test_stress_parity_vs_reference_traversal(gpu_ctx, stress):
-    mesh, prim = stress
+@pytest.fixture(scope="module")
+def heightfield():
+    """The large scene: the 2237^2 fBm heightfield (9,999,392 triangles), 2M camera rays framing the
+    surface (2048 x 1024), as the benchmark's headline workload."""
+    mesh = vt.heightfield(grid=2237, octaves=8, gain=0.5, half_extent=100.0, height=12.0, seed=3)
+    prim = vt.camera_rays((0.0, 70.0, -70.0), (0.0, 0.0, 0.0), 40.0, 2048, 1024, 3)
+    return mesh, prim
+
+
+def _parity_vs_reference(gpu_ctx, mesh, prim, seed, what):
     scene = gpu_ctx.set_mesh(mesh)
-    print("stress: scene", scene.info, flush=True)
+    print(f"{what}: scene", scene.info, flush=True)
     osc = oracle.Scene(mesh.vertices, mesh.faces)
     gp = _closest(scene, prim)
-    bounce = vt.bounce_rays(mesh, prim, gp["face"], gp["b1"], gp["b2"], 7)
+    bounce = vt.bounce_rays(mesh, prim, gp["face"], gp["b1"], gp["b2"], seed)
     gb = _closest(scene, bounce)
     for name, rays, got in (("primary", prim, gp), ("secondary", bounce, gb)):
         ref = osc.closest(rays, mode=0)
@@ -171,7 +231,7 @@ def test_stress_parity_vs_reference_traversal(gpu_ctx, stress):
         # Faces may differ only where two triangles are hit at exactly the same t.
         ties = int(diff.sum())
         assert np.all(got["face"][diff] >= 0) and np.all(ref["face"][diff] >= 0)
-        print(f"stress {name}: {rays.shape[1]} rays, {(got['face'] >= 0).sum()} hits, {ties} exact-t ties "
+        print(f"{what} {name}: {rays.shape[1]} rays, {(got['face'] >= 0).sum()} hits, {ties} exact-t ties "
               f"resolved differently from the reference visit order", flush=True)
         assert ties <= rays.shape[1] // 1000
         # Where the faces agree, barycentrics are bit-exact too.
@@ -181,3 +241,17 @@ def test_stress_parity_vs_reference_traversal(gpu_ctx, stress):
     occ = _any(scene, bounce)
     assert np.array_equal(occ.astype(bool), gb["face"] >= 0)
     scene.close()
+    return gp, gb
+
+
+def test_stress_parity_vs_reference_traversal(gpu_ctx, stress):
+    mesh, prim = stress
+    _parity_vs_reference(gpu_ctx, mesh, prim, 7, "stress")
+
+
+def test_heightfield_10m_parity_vs_reference_traversal(gpu_ctx, heightfield):
+    # The 10M-triangle scene at full size: t bit-exact for 2M camera + 2M bounce rays, faces equal
+    # except exact-t ties, against the reference's 4-wide traversal restated in the oracle.
+    mesh, prim = heightfield
+    gp, gb = _parity_vs_reference(gpu_ctx, mesh, prim, 3, "heightfield")
+    assert (gp["face"] >= 0).mean() > 0.95      # the camera frames the surface
