@@ -118,6 +118,15 @@ vt_status vt_trace_any(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, uint
  * traversal.  counters is a HOST array; the call synchronises the stream. */
 vt_status vt_trace_stats(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, int any, uint64_t counters[2],
                          void *stream);
+/* vt_trace_stats for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc); no counterpart in the reference.  It allocates nothing: the
+ * caller passes the results (hits for the closest-hit count, occluded for any != 0) and `counters`,
+ * 2 uint64 in DEVICE memory (node records fetched, triangle records tested), and the 64 partial
+ * counter pairs live in a 1 KiB static device array the library holds on each GPU for the process,
+ * cleared by a kernel at every call; its size does not depend on n, so no ray count is too large.
+ * Asynchronous on `stream` (no synchronisation).  Calls that overlap on one GPU race on that array. */
+vt_status vt_trace_stats_noalloc(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, int any,
+                                 const vt_hit_soa *hits, uint8_t *occluded, uint64_t *counters, void *stream);
 
 /* ---------------------------------------------------------------- materials and rendering */
 

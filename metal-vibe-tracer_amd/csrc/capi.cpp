@@ -162,6 +162,7 @@ vt_status vt_context_create(const vt_context_desc *desc, vt_context **out) {
                                                 : std::max<uint64_t>(256ull << 20, (uint64_t)total / 4);
         const uint32_t zero = 0;
         c->fault.upload(&zero, 4);
+        check(vt::scratch_init(device), "hipGetSymbolAddress");
         *out = c.release();
         return VT_OK;
     });
@@ -306,6 +307,27 @@ vt_status vt_trace_any(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, uint
         vt::HitArgs h{};
         check(vt::launch_trace(scene->view, ray_args(rays), h, occluded, n, true, nullptr, (hipStream_t)stream),
               "trace_any launch");
+        return VT_OK;
+    });
+}
+
+vt_status vt_trace_stats_noalloc(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, int any,
+                                 const vt_hit_soa *hits, uint8_t *occluded, uint64_t *counters, void *stream) {
+    return guarded([&]() -> vt_status {
+        if (!scene || !counters) return fail(VT_ERROR_INVALID_ARGUMENT, "scene and counters must not be NULL");
+        if (n && !rays_ok(rays)) return fail(VT_ERROR_INVALID_ARGUMENT, "ray arrays must not be NULL");
+        vt::HitArgs h{};
+        if (n && any && !occluded) return fail(VT_ERROR_INVALID_ARGUMENT, "occluded must not be NULL");
+        if (n && !any) {
+            if (!hits || !hits->t || !hits->b1 || !hits->b2 || !hits->face)
+                return fail(VT_ERROR_INVALID_ARGUMENT, "hit arrays must not be NULL");
+            h = vt::HitArgs{hits->t, hits->b1, hits->b2, hits->face, hits->instance};
+        }
+        check(hipSetDevice(scene->ctx->device), "hipSetDevice");
+        const vt::RayArgs r = n ? ray_args(rays) : vt::RayArgs{};
+        check(vt::launch_trace_stats_scratch(scene->ctx->device, scene->view, r, h, any ? occluded : nullptr, n, any != 0,
+                                             (unsigned long long *)counters, (hipStream_t)stream),
+              "trace_stats_noalloc launch");
         return VT_OK;
     });
 }

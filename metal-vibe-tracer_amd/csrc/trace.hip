@@ -444,6 +444,59 @@ uint32_t refill_grid(const void *kernel, uint32_t n, int block) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- allocation-free work counters
+// The counting traversal's 64 partial (node fetches, triangle tests) pairs live in this file-scope
+// array: 1 KiB per device, whatever the number of rays.  Host code never passes the symbol itself
+// (in host code it only names a stand-in); it passes the per-device address scratch_init found.
+static __device__ __align__(256) unsigned long long scratch[kStatsScratchWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_lock;
+
+__global__ void zero_scratch_kernel(unsigned long long *p) {
+    if (threadIdx.x < kStatsScratchWords) p[threadIdx.x] = 0ull;
+}
+__global__ void sum_pairs_kernel(const unsigned long long *p, unsigned long long *out) {
+    if (threadIdx.x != 0) return;
+    unsigned long long nodes = 0, tris = 0;
+    for (int k = 0; k < kStatsScratchWords / 2; ++k) {
+        nodes += p[2 * k];
+        tris += p[2 * k + 1];
+    }
+    out[0] = nodes;
+    out[1] = tris;
+}
+}  // namespace
+
+hipError_t scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(g_scratch_lock);
+    if (g_scratch[device]) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = p;
+    return hipSuccess;
+}
+
+hipError_t launch_trace_stats_scratch(int device, const TraceScene &scene, const RayArgs &rays, const HitArgs &hits,
+                                      uint8_t *occluded, uint32_t n, bool any, unsigned long long *out,
+                                      hipStream_t stream) {
+    if (device < 0 || device >= kMaxDevices || !g_scratch[device]) return hipErrorNotInitialized;
+    unsigned long long *p = static_cast<unsigned long long *>(g_scratch[device]);
+    hipLaunchKernelGGL(zero_scratch_kernel, dim3(1), dim3(kStatsScratchWords), 0, stream, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (n) {
+        e = launch_trace(scene, rays, hits, occluded, n, any, p, stream);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(sum_pairs_kernel, dim3(1), dim3(64), 0, stream, p, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_trace_batch(const TraceScene *job_trace, const uint32_t *job, const RayArgs &rays, const HitArgs &hits,
                               uint8_t *occluded, uint32_t n, bool any, const uint32_t *count, uint32_t max_depth,
                               uint32_t block, uint32_t grid, hipStream_t stream) {

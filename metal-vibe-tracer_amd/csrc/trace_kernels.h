@@ -34,6 +34,18 @@ hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitA
                         uint32_t n, bool any, unsigned long long *counters, hipStream_t stream,
                         const uint32_t *count = nullptr);
 
+// The counting traversal of launch_trace (counters variant) with its partial counters in the
+// library's static device scratch (kStatsScratchWords u64, 64 pairs) instead of an allocation:
+// zeroes the scratch with a kernel, traces into the caller's hits / occluded, and sums the pairs into
+// out[0] (node fetches) and out[1] (triangle tests), device memory.  Asynchronous; allocates nothing.
+// scratch_init(device) must have run once with `device` current.  Calls that overlap on one device
+// race on the scratch.
+constexpr int kStatsScratchWords = 128;
+hipError_t scratch_init(int device);
+hipError_t launch_trace_stats_scratch(int device, const TraceScene &scene, const RayArgs &rays, const HitArgs &hits,
+                                      uint8_t *occluded, uint32_t n, bool any, unsigned long long *out,
+                                      hipStream_t stream);
+
 // Batched rendering: ray i traces the scene job_trace[job[i]], for i < min(n, *count), on a
 // persistent grid of `grid` blocks of `block` (64, 128, 256 or 512) lanes; max_depth is the deepest tree.
 hipError_t launch_trace_batch(const TraceScene *job_trace, const uint32_t *job, const RayArgs &rays, const HitArgs &hits,

@@ -37,6 +37,7 @@ EXPORTED = [
     "vt_obj_part_count", "vt_obj_skipped", "vt_obj_library_count", "vt_obj_library", "vt_obj_part", "vt_obj_subset",
     "vt_z_draws", "vt_context_check", "vt_scene_set_stack_cap", "vt_mesh_from_triangles",
     "vt_scene_upload_triangles", "vt_scene_set_surface_settings", "vt_renderer_set_callbacks",
+    "vt_trace_stats_noalloc",
 ]
 
 
@@ -213,6 +214,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_scene_upload_triangles": (i32, [vp, vp, u32, P(_BuildOptions), P(vp)]),
         "vt_scene_set_surface_settings": (i32, [vp, vp, u32, vp, u32]),
         "vt_renderer_set_callbacks": (i32, [vp, FRAME_FN, ERROR_FN, vp]),
+        "vt_trace_stats_noalloc": (i32, [vp, P(_RaySoA), u32, c.c_int, P(_HitSoA), vp, vp, vp]),
         "vt_scene_set_stack_cap": (i32, [vp, u32]),
     }
     for name, (res, args) in sig.items():
@@ -664,6 +666,29 @@ class Scene:
         em = [Emission(k, (ctypes.c_float * 3)(*rgb)) for k, rgb in (emissions or {}).items()]
         e = (Emission * max(1, len(em)))(*em)
         _check(load().vt_scene_set_surface_settings(self._h, s, len(settings), e, len(em)))
+
+    def trace_stats_noalloc(self, rays, counters, out=None, any_hit=False, stream=None):
+        """vt_trace_stats_noalloc: counters (2 int64 device tensor) <- (node fetches, triangle tests);
+        results into `out` (closest: dict t, b1, b2, face; any: uint8 tensor), allocated here when None."""
+        import torch
+        rs, n = _rays_struct(rays)
+        if counters.dtype != torch.int64 or not counters.is_cuda or not counters.is_contiguous() or counters.numel() < 2:
+            raise ValueError("counters must be a contiguous int64 device tensor of 2 elements")
+        if any_hit:
+            if out is None:
+                out = torch.empty(n, dtype=torch.uint8, device=rays.device)
+            _check(load().vt_trace_stats_noalloc(self._h, ctypes.byref(rs), n, 1, None, ctypes.c_void_p(out.data_ptr()),
+                                                 ctypes.c_void_p(counters.data_ptr()), _stream_ptr(stream)))
+            return out
+        if out is None:
+            out = {"t": torch.empty(n, dtype=torch.float32, device=rays.device),
+                   "b1": torch.empty(n, dtype=torch.float32, device=rays.device),
+                   "b2": torch.empty(n, dtype=torch.float32, device=rays.device),
+                   "face": torch.empty(n, dtype=torch.int32, device=rays.device)}
+        hs = _HitSoA(out["t"].data_ptr(), out["b1"].data_ptr(), out["b2"].data_ptr(), out["face"].data_ptr(), None)
+        _check(load().vt_trace_stats_noalloc(self._h, ctypes.byref(rs), n, 0, ctypes.byref(hs), None,
+                                             ctypes.c_void_p(counters.data_ptr()), _stream_ptr(stream)))
+        return out
 
     def trace_stats(self, rays, any_hit=False, stream=None):
         rs, n = _rays_struct(rays)
