@@ -260,13 +260,14 @@ def bench_learned_cache(args, ctx):
     """Config 5: the emitter terrain with the OpenPBR glossy / metal / transmissive mix (~10M
     triangles, 4096 emitters in the light tree), 1920 x 1080, depth 8, MIS, 1 sample per frame.
 
-    Plain: `--cache-frames` frames of the plain tracer, each timed (synchronised), accumulated.
-    Learned cache, equal time: the cache is first trained online over `--cache-warmup` frames (the
-    state of an interactive session that has been running; their time is reported, not charged),
-    then frames are accumulated from a reset until the plain run's time is spent.  Cold start: a
-    fresh cache whose every frame, training included, is accumulated within the same time.  Error:
-    relMSE on the 64 x 36 stratified pixel subset against the committed FP64 oracle image at 65536
-    spp (tests/golden/terrain_openpbr_ref.npz, made by tests/golden/make_terrain_reference.py)."""
+    For each budget of B plain frames (`--cache-budgets`, default 16, 64 and 256): B frames of the
+    plain tracer, each timed (synchronised) and accumulated; then, from a cold start, a fresh cache's
+    frames -- online training included -- accumulated until the plain run's time is spent.  The
+    headline ratio is the cold start's at the first budget.  A warm variant (the cache trained over
+    `--cache-warmup` frames first, their time reported but not charged: an interactive session that
+    has been running) is reported at the first budget.  Error: relMSE on the 64 x 36 stratified pixel
+    subset against the committed FP64 oracle image at 65536 spp (tests/golden/terrain_openpbr_ref.npz,
+    made by tests/golden/make_terrain_reference.py)."""
     import numpy as np
     import torch
 
@@ -292,15 +293,17 @@ def bench_learned_cache(args, ctx):
     def error(r):
         return relmse(r.readback()[..., :3].astype(np.float64).reshape(-1, 3)[pix], ref)
 
-    r = vt.Renderer(ctx)
-    for k in range(3):
-        frame(r, 1000 + k)
-    r.reset()
-    t_plain = sum(frame(r, 1 + k) for k in range(args.cache_frames))
-    e_plain = error(r)
-    r.close()
+    def plain(frames):
+        r = vt.Renderer(ctx)
+        for k in range(3):
+            frame(r, 1000 + k)
+        r.reset()
+        t = sum(frame(r, 1 + k) for k in range(frames))
+        e = error(r)
+        r.close()
+        return e, t
 
-    def learned(warm):
+    def learned(budget, warm):
         cache = vt.NRC(ctx, seed=1)
         r = vt.Renderer(ctx)
         r.set_cache(cache)
@@ -308,7 +311,7 @@ def bench_learned_cache(args, ctx):
         if warm:
             r.reset()
         spent, n = 0.0, 0
-        while spent < t_plain:
+        while spent < budget:
             spent += frame(r, 1 + n, cache=1)
             n += 1
         e = error(r)
@@ -316,8 +319,19 @@ def bench_learned_cache(args, ctx):
         cache.close()
         return e, n, spent, t_warm
 
-    e_nrc, n_nrc, t_nrc, t_warm = learned(args.cache_warmup)
-    e_cold, n_cold, t_cold, _ = learned(0)
+    budgets = []
+    for frames in args.cache_budgets:
+        e_plain, t_plain = plain(frames)
+        e_cold, n_cold, t_cold, _ = learned(t_plain, 0)
+        budgets.append({"plain": {"spp": frames, "seconds": round(t_plain, 4), "relmse": e_plain,
+                                  "ms_per_frame": round(t_plain / frames * 1e3, 3)},
+                        "cold_start": {"spp": n_cold, "seconds": round(t_cold, 4), "relmse": e_cold,
+                                       "ms_per_frame": round(t_cold / n_cold * 1e3, 3)},
+                        "relmse_ratio": e_plain / e_cold})
+        log(f"learned cache, {frames} plain frames ({t_plain * 1e3:.1f} ms): plain relMSE {e_plain:.4e}; "
+            f"cold-start cache {n_cold} frames relMSE {e_cold:.4e}; ratio {e_plain / e_cold:.2f}")
+    first = budgets[0]
+    e_warm, n_warm, t_nrc, t_warm = learned(first["plain"]["seconds"], args.cache_warmup)
     scene.close()
     # The cache's inference kernel against the matrix cores' dense bf16 peak: 2M queries (a 1080p
     # frame's worth), timed with HIP events on the launch stream.
@@ -335,19 +349,14 @@ def bench_learned_cache(args, ctx):
     cache.close()
     flop = 2.0 * x.shape[0] * (64 * 265 + 3 * 64 * 64 + 4 * 64)
     infer_tflops = flop / (infer_ms * 1e-3) / 1e12
-    log(f"learned cache: plain {args.cache_frames} spp relMSE {e_plain:.4e}; cache {n_nrc} spp relMSE {e_nrc:.4e} "
-        f"(cold start {n_cold} spp, {e_cold:.4e})")
     return {"workload": f"emitter-terrain-openpbr-{len(mesh.faces)}tri-{w}x{h}-depth{depth}",
             "reference": {"kind": "FP64 oracle", "spp": int(np.load(path)["spp"]), "pixels": int(len(pix))},
             "learning_rate": lr, "train_period": 16,
-            "plain": {"spp": args.cache_frames, "seconds": round(t_plain, 4), "relmse": e_plain,
-                      "ms_per_frame": round(t_plain / args.cache_frames * 1e3, 3)},
-            "learned": {"spp": n_nrc, "seconds": round(t_nrc, 4), "relmse": e_nrc,
-                        "ms_per_frame": round(t_nrc / n_nrc * 1e3, 3), "warmup_frames": args.cache_warmup,
-                        "warmup_seconds": round(t_warm, 3)},
-            "cold_start": {"spp": n_cold, "seconds": round(t_cold, 4), "relmse": e_cold},
-            "relmse_ratio_plain_over_learned": e_plain / e_nrc,
-            "relmse_ratio_cold_start": e_plain / e_cold,
+            "relmse_ratio_cold_start": first["relmse_ratio"],
+            "budgets": budgets,
+            "warm": {"plain_spp": first["plain"]["spp"], "spp": n_warm, "seconds": round(t_nrc, 4), "relmse": e_warm,
+                     "warmup_frames": args.cache_warmup, "warmup_seconds": round(t_warm, 3),
+                     "relmse_ratio": first["plain"]["relmse"] / e_warm},
             "inference_roofline": {"bound": "mfma", "achieved": round(infer_tflops, 1), "peak": 2500.0,
                                    "unit": "TFLOP/s", "frac": round(infer_tflops / 2500.0, 4),
                                    "queries": int(x.shape[0]), "ms": round(infer_ms, 4),
@@ -463,7 +472,8 @@ def main():
     ap.add_argument("--no-integrator", action="store_true", help="skip the path-tracer (Cornell) measurement")
     ap.add_argument("--reference-spp", type=int, default=65536, help="FP64 oracle samples per pixel for relMSE")
     ap.add_argument("--no-cache", action="store_true", help="skip the learned-cache equal-time measurement")
-    ap.add_argument("--cache-frames", type=int, default=16, help="plain-tracer frames (1 spp each) setting the time budget")
+    ap.add_argument("--cache-budgets", type=int, nargs="+", default=[16, 64, 256],
+                    help="plain-tracer frames (1 spp each) setting each equal-time budget")
     ap.add_argument("--cache-warmup", type=int, default=64, help="online training frames before the equal-time run")
     ap.add_argument("--cache-lr", type=float, default=1e-3, help="Adam learning rate of the online training")
     ap.add_argument("--no-restir", action="store_true", help="skip the ReSTIR DI large-scene measurement")
