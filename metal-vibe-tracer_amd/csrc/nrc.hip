@@ -114,74 +114,106 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // ---------------------------------------------------------------- inference
-// The weights are the A operand and the activations the B operand (D^T = W . X^T), so a layer's
-// output tile -- lane l holds neurons 16 m + 4 (l / 16) + j of query l % 16 -- is exactly the B
-// operand of k-step m of the next layer: activations stay in registers from the encoding to the
-// output.  A wave handles 64 queries (4 tiles of 16) per pass, so every weight fragment read from
-// LDS feeds 4 products.  LDS rows are padded (276 / 68 bf16) so the 16 rows a quarter-wave reads
-// fall in distinct banks.
-constexpr int kW1Pitch = 276, kWhPitch = 68;
+// v_mfma_f32_16x16x32_bf16 (gfx950: twice the K of the 16x16x16 form for the same issue cost).  The
+// weights are the A operand and the activations the B operand (D^T = W . X^T).  Lane l = r + 16 g
+// holds A row r / B column r and the eight k of group g; the sum over k does not depend on which
+// hardware k a (group, element) pair stands for, as long as A and B agree, so each layer's inputs are
+// numbered for the convenience of the data:
+//   layer 1: k-step ks, group g, element e is feature 32 ks + 8 g + e in the network's own order
+//            (24 per input: sin and cos of bands 0..11), i.e. four consecutive bands of one input,
+//            computed by the lane itself; features 264 (the constant 1) .. 287 complete the 9 steps;
+//   layers 2-5: k-step ks, group g, element e is input neuron 32 ks + 16 (e / 4) + 4 g + e % 4, the
+//            neurons the lane already holds in its output tiles 2 ks and 2 ks + 1 (D layout: lane l
+//            holds rows 4 g + j of a tile), so activations never leave registers.
+// The LDS copy of each weight row is stored in that k order.  A wave handles 16 T queries per pass.
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+constexpr int kK1 = 288;                           // layer-1 k: 272 features padded to 9 steps of 32
+constexpr int kW1Pitch = kK1 + 8, kWhPitch = 64 + 8;  // bf16; rows 592 / 144 bytes apart: the 16 rows
+                                                      // a quarter-wave reads fall in distinct banks
 constexpr int kLdsW1 = 0, kLdsW2 = kLdsW1 + kNrcHidden * kW1Pitch, kLdsW3 = kLdsW2 + kNrcHidden * kWhPitch,
               kLdsW4 = kLdsW3 + kNrcHidden * kWhPitch, kLdsW5 = kLdsW4 + kNrcHidden * kWhPitch;
-constexpr int kInferLds = (kLdsW5 + kNrcOutPad * kWhPitch) * 2;  // 63,616 bytes
+constexpr int kInferLds = (kLdsW5 + kNrcOutPad * kWhPitch) * 2;  // 67,840 bytes
 #ifndef VT_NRC_INFER_TILES
 #define VT_NRC_INFER_TILES 2
 #endif
-#ifndef VT_NRC_RELOAD
-#define VT_NRC_RELOAD 1
-#endif
 constexpr int kInferTiles = VT_NRC_INFER_TILES;
+#ifndef VT_NRC_RELOAD
+#define VT_NRC_RELOAD 0
+#endif
 
-__device__ __forceinline__ void load_weights_padded(uint16_t *w, const uint16_t *src) {
-    for (int c = threadIdx.x; c < kNrcParams / 4; c += blockDim.x) {
-        const int e = 4 * c;
-        int dst;
-        if (e < kNrcW1) {
-            const int row = e / kNrcFeatures;
-            dst = kLdsW1 + row * kW1Pitch + (e - row * kNrcFeatures);
-        } else {
-            const int f = e - kNrcW1, row = f / kNrcHidden;  // W2..W4 then W5: rows of 64 back to back
-            dst = kLdsW2 + row * kWhPitch + (f - row * kNrcHidden);
-        }
-        *reinterpret_cast<uint2 *>(w + dst) = *reinterpret_cast<const uint2 *>(src + e);
+// Input neuron of position p (0..63) of a permuted hidden-layer row.
+__host__ __device__ constexpr int hidden_input(int p) {
+    return 32 * (p >> 5) + 16 * ((p & 7) >> 2) + 4 * ((p >> 3) & 3) + (p & 3);
+}
+
+__device__ __forceinline__ void load_weights_permuted(uint16_t *w, const uint16_t *src) {
+    // Layer 1: rows of 272 features in their own order, zero up to 288.
+    for (int c = threadIdx.x; c < kNrcHidden * (kK1 / 8); c += blockDim.x) {
+        const int row = c / (kK1 / 8), f = 8 * (c - row * (kK1 / 8));
+        uint4 v = {0u, 0u, 0u, 0u};
+        if (f < kNrcFeatures) v = *reinterpret_cast<const uint4 *>(src + row * kNrcFeatures + f);
+        *reinterpret_cast<uint4 *>(w + kLdsW1 + row * kW1Pitch + f) = v;
+    }
+    // Layers 2-4 (64 rows each) and 5 (16 rows): each row's 64 inputs in the permuted order.
+    constexpr int kRows = 3 * kNrcHidden + kNrcOutPad;
+    for (int c = threadIdx.x; c < kRows * 64; c += blockDim.x) {
+        const int row = c >> 6, p = c & 63;
+        w[kLdsW2 + row * kWhPitch + p] = src[kNrcW1 + row * 64 + hidden_input(p)];
     }
 }
 
-// relu and bf16 of a layer's output tiles: the next layer's B operands, h[k-step][query tile].
+__device__ __forceinline__ f4v mfma32(bf8v a, bf8v b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ bf8v lds8(const uint16_t *p) { return *reinterpret_cast<const bf8v *>(p); }
+
+// Features 8 q .. 8 q + 7 of input value x (q = 0, 1, 2: bands 4q .. 4q + 3): sin and cos of
+// pi 2^b x per band, as bf16.  2^(b-1) x is exact in fp32 and so is its fractional part, so the
+// hardware sine and cosine (argument in turns) work on the exactly reduced angle.
+__device__ __forceinline__ bf8v band_quad(float x, int b0) {
+    bf8v r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float t = __builtin_amdgcn_fractf(ldexpf(x, b0 + k - 1));
+        r[2 * k] = (__bf16)__builtin_amdgcn_sinf(t);
+        r[2 * k + 1] = (__bf16)__builtin_amdgcn_cosf(t);
+    }
+    return r;
+}
+
+// relu and bf16 of output tiles 2 ks and 2 ks + 1: the B operand of the next layer's k-step ks.
+__device__ __forceinline__ bf8v relu_pair(f4v lo, f4v hi) {
+    bf8v r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        r[j] = (__bf16)fmaxf(lo[j], 0.0f);
+        r[4 + j] = (__bf16)fmaxf(hi[j], 0.0f);
+    }
+    return r;
+}
+
+// One hidden layer: 64 neurons x 16 T queries in (acc, D layout) -> relu(W .) in acc.
 template <int T>
-__device__ __forceinline__ void relu_pack(const f4v (&acc)[4][T], s4v (&h)[4][T]) {
+__device__ __forceinline__ void hidden_layer(const uint16_t *w, f4v (&acc)[4][T], int r, int g) {
+    bf8v h[2][T];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+        for (int t = 0; t < T; ++t) h[ks][t] = relu_pair(acc[2 * ks][t], acc[2 * ks + 1][t]);
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
-            s4v v;
-            v.x = (short)to_bf16(fmaxf(acc[m][t][0], 0.0f));
-            v.y = (short)to_bf16(fmaxf(acc[m][t][1], 0.0f));
-            v.z = (short)to_bf16(fmaxf(acc[m][t][2], 0.0f));
-            v.w = (short)to_bf16(fmaxf(acc[m][t][3], 0.0f));
-            h[m][t] = v;
+        for (int t = 0; t < T; ++t) acc[m][t] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const bf8v a = lds8(w + (16 * m + r) * kWhPitch + 32 * ks + 8 * g);
+#pragma unroll
+            for (int t = 0; t < T; ++t) acc[m][t] = mfma32(a, h[ks][t], acc[m][t]);
         }
 }
 
-// One hidden layer: h (64 neurons x 16 T queries, B layout) -> relu(W h) in place.
-template <int T>
-__device__ __forceinline__ void hidden_layer(const uint16_t *w, s4v (&h)[4][T], int r, int g) {
-    f4v acc[4][T] = {};
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-        s4v a[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) a[m] = load4(w + (16 * m + r) * kWhPitch + 16 * ks + 4 * g);
-#pragma unroll
-        for (int t = 0; t < T; ++t)
-#pragma unroll
-            for (int m = 0; m < 4; ++m) acc[m][t] = mfma(a[m], h[ks][t], acc[m][t]);
-    }
-    relu_pack<T>(acc, h);
-}
-
-// A wave handles 16 T queries per pass (T tiles of 16); blocks of kInferBlock lanes share one LDS copy
-// of the network.
+// Blocks of kInferBlock lanes share one LDS copy of the network.
 #ifndef VT_NRC_INFER_BLOCK
 #define VT_NRC_INFER_BLOCK 512
 #endif
@@ -192,60 +224,56 @@ __global__ __launch_bounds__(kInferBlock) void nrc_infer_kernel(const uint16_t *
     extern __shared__ __align__(16) uint16_t lds[];
     if (count) n = min(n, *count);
     if (n == 0) return;
-    load_weights_padded(lds, wsrc);
+    load_weights_permuted(lds, wsrc);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
     const uint32_t passes = (n + 16 * T - 1) / (16 * T);
     for (uint32_t p = blockIdx.x * kInferWaves + wave; p < passes; p += gridDim.x * kInferWaves) {
         const uint32_t base = p * 16 * T;
+        bool live[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) live[t] = base + 16 * t + r < n;
+        // Layer 1 over 9 k-steps: lane group g of step ks holds features f = 32 ks + 8 g .. + 7, four
+        // bands of input f / 24 (f < 264), or the constant 1 and zeros.
+        f4v acc[4][T] = {};
+#pragma unroll 1
+        for (int ks = 0; ks < kK1 / 32; ++ks) {
+            bf8v a[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) a[m] = lds8(lds + kLdsW1 + (16 * m + r) * kW1Pitch + 32 * ks + 8 * g);
+            const int f = 32 * ks + 8 * g;
+            constexpr int kLast = kNrcInputs * 2 * kNrcBands;  // 264
+            const int i = f < kLast ? f / 24 : 0, b0 = (f - 24 * i) / 2;
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                bf8v bv = {};
+                if (f < kLast) {
+                    const float xv = live[t] ? query_input(inputs, stride, base + 16 * t + r, i) : 0.0f;
+                    bv = band_quad(xv, b0);
+                } else if (f == kLast) {
+                    bv[0] = (__bf16)1.0f;
+                }
+#pragma unroll
+                for (int m = 0; m < 4; ++m) acc[m][t] = mfma32(a[m], bv, acc[m][t]);
+            }
+        }
 #if VT_NRC_RELOAD
-        // An offset the compiler cannot see through: the hidden layers' weight fragments are read from
-        // LDS each pass instead of being hoisted into (and occupying) 104 registers per lane.
         int zero = 0;
         asm volatile("" : "+v"(zero));
         const uint16_t *wl = lds + zero;
 #else
         const uint16_t *wl = lds;
 #endif
-        bool live[T];
-#pragma unroll
-        for (int t = 0; t < T; ++t) live[t] = base + 16 * t + r < n;
-        // Layer 1 over the 17 k-steps of the encoding: lane l's features f = 16 ks + 4 g .. + 3 are
-        // bands b, b + 1 of input i = f / 24 (f < 264), or the constant 1 and zero padding.
-        f4v acc[4][T] = {};
-#pragma unroll 1
-        for (int ks = 0; ks < kNrcFeatures / 16; ++ks) {
-            s4v a[4];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) a[m] = load4(lds + kLdsW1 + (16 * m + r) * kW1Pitch + 16 * ks + 4 * g);
-            const int f = 16 * ks + 4 * g;
-            constexpr int kLast = kNrcInputs * 2 * kNrcBands;  // 264
-            const int i = f < kLast ? f / 24 : 0, b = (f - 24 * i) / 2;
-#pragma unroll
-            for (int t = 0; t < T; ++t) {
-                s4v bv = {0, 0, 0, 0};
-                if (f < kLast) {
-                    const float xv = live[t] ? query_input(inputs, stride, base + 16 * t + r, i) : 0.0f;
-                    bv = band_pair(xv, b);
-                } else if (f == kLast) {
-                    bv.x = (short)to_bf16(1.0f);
-                }
-#pragma unroll
-                for (int m = 0; m < 4; ++m) acc[m][t] = mfma(a[m], bv, acc[m][t]);
-            }
-        }
-        s4v h[4][T];
-        relu_pack<T>(acc, h);
-        hidden_layer<T>(wl + kLdsW2, h, r, g);
-        hidden_layer<T>(wl + kLdsW3, h, r, g);
-        hidden_layer<T>(wl + kLdsW4, h, r, g);
+        hidden_layer<T>(wl + kLdsW2, acc, r, g);
+        hidden_layer<T>(wl + kLdsW3, acc, r, g);
+        hidden_layer<T>(wl + kLdsW4, acc, r, g);
         f4v y[T] = {};
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            const s4v a = load4(wl + kLdsW5 + r * kWhPitch + 16 * ks + 4 * g);
+        for (int ks = 0; ks < 2; ++ks) {
+            const bf8v a = lds8(wl + kLdsW5 + r * kWhPitch + 32 * ks + 8 * g);
 #pragma unroll
-            for (int t = 0; t < T; ++t) y[t] = mfma(a, h[ks][t], y[t]);
+            for (int t = 0; t < T; ++t) y[t] = mfma32(a, relu_pair(acc[2 * ks][t], acc[2 * ks + 1][t]), y[t]);
         }
         // Lanes 0..15 hold outputs 0..3 of query 16 t + l.
         if (g == 0) {
