@@ -119,61 +119,93 @@ __device__ __forceinline__ void wave_sync() {
 // holds A row r / B column r and the eight k of group g; the sum over k does not depend on which
 // hardware k a (group, element) pair stands for, as long as A and B agree, so each layer's inputs are
 // numbered for the convenience of the data:
-//   layer 1: k-step ks, group g, element e is feature 32 ks + 8 g + e in the network's own order
-//            (24 per input: sin and cos of bands 0..11), i.e. four consecutive bands of one input,
-//            computed by the lane itself; features 264 (the constant 1) .. 287 complete the 9 steps;
+//   layer 1: k-step ks, group g carries feature quad Q = 9 g + ks, element e feature 8 Q + e of the
+//            network's order (24 per input: sin and cos of bands 0..11), i.e. bands 4 (ks % 3) ..
+//            4 (ks % 3) + 3 of input 3 g + ks / 3.  A lane thus needs three inputs of its query, and
+//            the band group of every step is a compile-time constant.  Quads 33 .. 35 (lanes of
+//            group 3 in steps 6 .. 8) have zero weights: the constant feature 264 is the bias,
+//            which starts the accumulation instead (fp32 copy of its bf16 weights);
 //   layers 2-5: k-step ks, group g, element e is input neuron 32 ks + 16 (e / 4) + 4 g + e % 4, the
 //            neurons the lane already holds in its output tiles 2 ks and 2 ks + 1 (D layout: lane l
 //            holds rows 4 g + j of a tile), so activations never leave registers.
-// The LDS copy of each weight row is stored in that k order.  A wave handles 16 T queries per pass.
+// The LDS copy of each weight row is stored in that k order.  A wave handles 16 T queries per pass
+// and loads the next pass's inputs while it computes the current one.
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef short s8v __attribute__((ext_vector_type(8)));
 
 constexpr int kK1 = 288;                           // layer-1 k: 272 features padded to 9 steps of 32
 constexpr int kW1Pitch = kK1 + 8, kWhPitch = 64 + 8;  // bf16; rows 592 / 144 bytes apart: the 16 rows
                                                       // a quarter-wave reads fall in distinct banks
 constexpr int kLdsW1 = 0, kLdsW2 = kLdsW1 + kNrcHidden * kW1Pitch, kLdsW3 = kLdsW2 + kNrcHidden * kWhPitch,
               kLdsW4 = kLdsW3 + kNrcHidden * kWhPitch, kLdsW5 = kLdsW4 + kNrcHidden * kWhPitch;
-constexpr int kInferLds = (kLdsW5 + kNrcOutPad * kWhPitch) * 2;  // 67,840 bytes
-#ifndef VT_NRC_INFER_TILES
-#define VT_NRC_INFER_TILES 2
-#endif
-constexpr int kInferTiles = VT_NRC_INFER_TILES;
-#ifndef VT_NRC_RELOAD
-#define VT_NRC_RELOAD 0
-#endif
+constexpr int kLdsBias = kLdsW5 + kNrcOutPad * kWhPitch;          // 64 floats
+constexpr int kInferLds = kLdsBias * 2 + kNrcHidden * 4;          // 68,096 bytes
+constexpr int kInferTiles = 2;
 
 // Input neuron of position p (0..63) of a permuted hidden-layer row.
 __host__ __device__ constexpr int hidden_input(int p) {
     return 32 * (p >> 5) + 16 * ((p & 7) >> 2) + 4 * ((p >> 3) & 3) + (p & 3);
 }
+// The weight copy moves positions 8 k .. 8 k + 7 as two runs of four inputs.
+constexpr bool hidden_runs_agree() {
+    for (int p = 0; p < 64; ++p)
+        if (hidden_input(p) != 32 * (p >> 5) + 4 * ((p >> 3) & 3) + 16 * ((p & 7) >> 2) + (p & 3)) return false;
+    return true;
+}
+static_assert(hidden_runs_agree(), "hidden-layer weight permutation");
 
+// Copies the network into LDS in the permuted orders, all loads of a thread issued before its stores.
+template <int kThreads>
 __device__ __forceinline__ void load_weights_permuted(uint16_t *w, const uint16_t *src) {
-    // Layer 1: rows of 272 features in their own order, zero up to 288.
-    for (int c = threadIdx.x; c < kNrcHidden * (kK1 / 8); c += blockDim.x) {
-        const int row = c / (kK1 / 8), f = 8 * (c - row * (kK1 / 8));
-        uint4 v = {0u, 0u, 0u, 0u};
-        if (f < kNrcFeatures) v = *reinterpret_cast<const uint4 *>(src + row * kNrcFeatures + f);
-        *reinterpret_cast<uint4 *>(w + kLdsW1 + row * kW1Pitch + f) = v;
+    // Layer 1: position 32 ks + 8 g + e of a row holds feature 8 (9 g + ks) + e (below 264), else 0;
+    // the weights of feature 264 (the constant 1) are the bias, as floats.  Hidden layers: positions
+    // 8 k .. 8 k + 7 of a row hold inputs b .. b + 3 and b + 16 .. b + 19, b = 32 (k / 4) + 4 (k % 4).
+    constexpr int kChunks1 = kNrcHidden * (kK1 / 8), kRowsH = 3 * kNrcHidden + kNrcOutPad, kChunksH = kRowsH * 8;
+    constexpr int kI1 = (kChunks1 + kThreads - 1) / kThreads, kIH = (kChunksH + kThreads - 1) / kThreads;
+    const int tid = threadIdx.x;
+    uint4 v1[kI1], vh[kIH];
+#pragma unroll
+    for (int i = 0; i < kI1; ++i) {
+        const int c = tid + i * kThreads, row = c / (kK1 / 8), chunk = c - row * (kK1 / 8);
+        const int q = 9 * (chunk & 3) + (chunk >> 2);
+        v1[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (c < kChunks1 && q < 33) v1[i] = *reinterpret_cast<const uint4 *>(src + row * kNrcFeatures + 8 * q);
     }
-    // Layers 2-4 (64 rows each) and 5 (16 rows): each row's 64 inputs in the permuted order.
-    constexpr int kRows = 3 * kNrcHidden + kNrcOutPad;
-    for (int c = threadIdx.x; c < kRows * 64; c += blockDim.x) {
-        const int row = c >> 6, p = c & 63;
-        w[kLdsW2 + row * kWhPitch + p] = src[kNrcW1 + row * 64 + hidden_input(p)];
+#pragma unroll
+    for (int i = 0; i < kIH; ++i) {
+        const int c = tid + i * kThreads, row = c >> 3, k = c & 7;
+        if (c < kChunksH) {
+            const uint16_t *r = src + kNrcW1 + row * 64 + 32 * (k >> 2) + 4 * (k & 3);
+            const uint2 lo = *reinterpret_cast<const uint2 *>(r), hi = *reinterpret_cast<const uint2 *>(r + 16);
+            vh[i] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+        }
     }
+    const float bias = tid < kNrcHidden ? from_bf16(src[tid * kNrcFeatures + 264]) : 0.0f;
+#pragma unroll
+    for (int i = 0; i < kI1; ++i) {
+        const int c = tid + i * kThreads, row = c / (kK1 / 8), chunk = c - row * (kK1 / 8);
+        if (c < kChunks1) *reinterpret_cast<uint4 *>(w + kLdsW1 + row * kW1Pitch + 8 * chunk) = v1[i];
+    }
+#pragma unroll
+    for (int i = 0; i < kIH; ++i) {
+        const int c = tid + i * kThreads;
+        if (c < kChunksH) *reinterpret_cast<uint4 *>(w + kLdsW2 + (c >> 3) * kWhPitch + 8 * (c & 7)) = vh[i];
+    }
+    if (tid < kNrcHidden) reinterpret_cast<float *>(w + kLdsBias)[tid] = bias;
 }
 
 __device__ __forceinline__ f4v mfma32(bf8v a, bf8v b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ bf8v lds8(const uint16_t *p) { return *reinterpret_cast<const bf8v *>(p); }
 
-// Features 8 q .. 8 q + 7 of input value x (q = 0, 1, 2: bands 4q .. 4q + 3): sin and cos of
-// pi 2^b x per band, as bf16.  2^(b-1) x is exact in fp32 and so is its fractional part, so the
-// hardware sine and cosine (argument in turns) work on the exactly reduced angle.
-__device__ __forceinline__ bf8v band_quad(float x, int b0) {
+// Features of bands b0 .. b0 + 3 of input value x: sin and cos of pi 2^b x per band, as bf16.
+// 2^(b-1) x is exact in fp32 and so is its fractional part, so the hardware sine and cosine
+// (argument in turns) work on the exactly reduced angle.
+template <int B0>
+__device__ __forceinline__ bf8v band_quad(float x) {
     bf8v r;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const float t = __builtin_amdgcn_fractf(ldexpf(x, b0 + k - 1));
+        const float t = __builtin_amdgcn_fractf(x * __builtin_ldexpf(1.0f, B0 + k - 1));
         r[2 * k] = (__bf16)__builtin_amdgcn_sinf(t);
         r[2 * k + 1] = (__bf16)__builtin_amdgcn_cosf(t);
     }
@@ -181,17 +213,21 @@ __device__ __forceinline__ bf8v band_quad(float x, int b0) {
 }
 
 // relu and bf16 of output tiles 2 ks and 2 ks + 1: the B operand of the next layer's k-step ks.
+// Rounded first, then the relu as a signed 16-bit max with 0 (a bf16 with its sign bit set is a
+// negative short): the same bits as rounding the relu.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef short s2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_bf16x2(float a, float b) {
+    const s2v v = __builtin_bit_cast(s2v, __builtin_convertvector(f2v{a, b}, bf2v));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, s2v{0, 0}));
+}
 __device__ __forceinline__ bf8v relu_pair(f4v lo, f4v hi) {
-    bf8v r;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        r[j] = (__bf16)fmaxf(lo[j], 0.0f);
-        r[4 + j] = (__bf16)fmaxf(hi[j], 0.0f);
-    }
-    return r;
+    const uint4 u = {relu_bf16x2(lo[0], lo[1]), relu_bf16x2(lo[2], lo[3]), relu_bf16x2(hi[0], hi[1]), relu_bf16x2(hi[2], hi[3])};
+    return __builtin_bit_cast(bf8v, u);
 }
 
-// One hidden layer: 64 neurons x 16 T queries in (acc, D layout) -> relu(W .) in acc.
+// One hidden layer: 64 neurons x 16 T queries in (acc, D layout) -> W . relu(acc) in acc.
 template <int T>
 __device__ __forceinline__ void hidden_layer(const uint16_t *w, f4v (&acc)[4][T], int r, int g) {
     bf8v h[2][T];
@@ -200,86 +236,109 @@ __device__ __forceinline__ void hidden_layer(const uint16_t *w, f4v (&acc)[4][T]
 #pragma unroll
         for (int t = 0; t < T; ++t) h[ks][t] = relu_pair(acc[2 * ks][t], acc[2 * ks + 1][t]);
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int t = 0; t < T; ++t) acc[m][t] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
     for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             const bf8v a = lds8(w + (16 * m + r) * kWhPitch + 32 * ks + 8 * g);
 #pragma unroll
-            for (int t = 0; t < T; ++t) acc[m][t] = mfma32(a, h[ks][t], acc[m][t]);
+            for (int t = 0; t < T; ++t) acc[m][t] = mfma32(a, h[ks][t], ks ? acc[m][t] : f4v{0.0f, 0.0f, 0.0f, 0.0f});
         }
 }
 
+// Inputs 3 g .. 3 g + 2 (capped at the last) of query 16 T p + 16 t + r, zeros past n.
+template <int T>
+__device__ __forceinline__ void load_inputs(float (&x)[T][3], const float *inputs, uint32_t stride, uint32_t n, uint32_t p,
+                                            int r, int g) {
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const uint64_t q = (uint64_t)p * (16 * T) + 16 * t + r;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int i = min(3 * g + c, kNrcInputs - 1);
+            const float v = query_input(inputs, stride, (uint32_t)min(q, (uint64_t)n - 1), i);
+            x[t][c] = q < n ? v : 0.0f;
+        }
+    }
+}
+
+// Layer-1 steps 3 c .. 3 c + 2 (input slot c of each query, band groups 0, 1, 2); the first
+// accumulates onto the bias.
+template <int T, bool kFirst>
+__device__ __forceinline__ void layer1_steps(const uint16_t *lds, f4v (&acc)[4][T], const f4v (&bias)[4],
+                                             const float (&x)[T][3], int c, int r, int g) {
+    float xc[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) xc[t] = kFirst ? x[t][0] : c == 1 ? x[t][1] : x[t][2];
+#pragma unroll
+    for (int bg = 0; bg < 3; ++bg) {
+        const int ks = 3 * c + bg;
+        bf8v a[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) a[m] = lds8(lds + kLdsW1 + (16 * m + r) * kW1Pitch + 32 * ks + 8 * g);
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const bf8v bv = bg == 0 ? band_quad<0>(xc[t]) : bg == 1 ? band_quad<4>(xc[t]) : band_quad<8>(xc[t]);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) acc[m][t] = mfma32(a[m], bv, kFirst && bg == 0 ? bias[m] : acc[m][t]);
+        }
+    }
+}
+
 // Blocks of kInferBlock lanes share one LDS copy of the network.
-#ifndef VT_NRC_INFER_BLOCK
-#define VT_NRC_INFER_BLOCK 512
-#endif
-constexpr int kInferBlock = VT_NRC_INFER_BLOCK, kInferWaves = kInferBlock / 64;
+constexpr int kInferBlock = 512, kInferWaves = kInferBlock / 64;
 template <int T>
 __global__ __launch_bounds__(kInferBlock) void nrc_infer_kernel(const uint16_t *wsrc, const float *inputs, uint32_t stride,
                                                                 uint32_t n, float *out, const uint32_t *count) {
     extern __shared__ __align__(16) uint16_t lds[];
     if (count) n = min(n, *count);
     if (n == 0) return;
-    load_weights_permuted(lds, wsrc);
+    load_weights_permuted<kInferBlock>(lds, wsrc);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
-    const uint32_t passes = (n + 16 * T - 1) / (16 * T);
-    for (uint32_t p = blockIdx.x * kInferWaves + wave; p < passes; p += gridDim.x * kInferWaves) {
-        const uint32_t base = p * 16 * T;
-        bool live[T];
+    const uint32_t passes = (n + 16 * T - 1) / (16 * T), step = gridDim.x * kInferWaves;
+    float xn[T][3];
+    uint32_t p = blockIdx.x * kInferWaves + wave;
+    load_inputs<T>(xn, inputs, stride, n, p, r, g);
+    for (; p < passes; p += step) {
+        float x[T][3];
 #pragma unroll
-        for (int t = 0; t < T; ++t) live[t] = base + 16 * t + r < n;
-        // Layer 1 over 9 k-steps: lane group g of step ks holds features f = 32 ks + 8 g .. + 7, four
-        // bands of input f / 24 (f < 264), or the constant 1 and zeros.
-        f4v acc[4][T] = {};
-#pragma unroll 1
-        for (int ks = 0; ks < kK1 / 32; ++ks) {
-            bf8v a[4];
+        for (int t = 0; t < T; ++t)
 #pragma unroll
-            for (int m = 0; m < 4; ++m) a[m] = lds8(lds + kLdsW1 + (16 * m + r) * kW1Pitch + 32 * ks + 8 * g);
-            const int f = 32 * ks + 8 * g;
-            constexpr int kLast = kNrcInputs * 2 * kNrcBands;  // 264
-            const int i = f < kLast ? f / 24 : 0, b0 = (f - 24 * i) / 2;
+            for (int c = 0; c < 3; ++c) x[t][c] = xn[t][c];
+        load_inputs<T>(xn, inputs, stride, n, p + step, r, g);
+        // Layer 1: steps ks = 3 c + bg take input slot c, bands 4 bg .. 4 bg + 3, from the bias.
+        f4v acc[4][T], bias[4];
+        {
+            // Read each pass (an opaque zero offset keeps the compiler from holding the bias in
+            // registers across the loop, which it needs for the hidden layers' weights).
+            int zero = 0;
+            asm volatile("" : "+v"(zero));
+            const float *bl = reinterpret_cast<const float *>(lds + kLdsBias) + zero;
 #pragma unroll
-            for (int t = 0; t < T; ++t) {
-                bf8v bv = {};
-                if (f < kLast) {
-                    const float xv = live[t] ? query_input(inputs, stride, base + 16 * t + r, i) : 0.0f;
-                    bv = band_quad(xv, b0);
-                } else if (f == kLast) {
-                    bv[0] = (__bf16)1.0f;
-                }
-#pragma unroll
-                for (int m = 0; m < 4; ++m) acc[m][t] = mfma32(a[m], bv, acc[m][t]);
-            }
+            for (int m = 0; m < 4; ++m) bias[m] = *reinterpret_cast<const f4v *>(bl + 16 * m + 4 * g);
         }
-#if VT_NRC_RELOAD
-        int zero = 0;
-        asm volatile("" : "+v"(zero));
-        const uint16_t *wl = lds + zero;
-#else
-        const uint16_t *wl = lds;
-#endif
-        hidden_layer<T>(wl + kLdsW2, acc, r, g);
-        hidden_layer<T>(wl + kLdsW3, acc, r, g);
-        hidden_layer<T>(wl + kLdsW4, acc, r, g);
-        f4v y[T] = {};
+        layer1_steps<T, true>(lds, acc, bias, x, 0, r, g);
+#pragma unroll 1
+        for (int c = 1; c < 3; ++c) layer1_steps<T, false>(lds, acc, bias, x, c, r, g);
+        hidden_layer<T>(lds + kLdsW2, acc, r, g);
+        hidden_layer<T>(lds + kLdsW3, acc, r, g);
+        hidden_layer<T>(lds + kLdsW4, acc, r, g);
+        f4v y[T];
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
-            const bf8v a = lds8(wl + kLdsW5 + r * kWhPitch + 32 * ks + 8 * g);
+            const bf8v a = lds8(lds + kLdsW5 + r * kWhPitch + 32 * ks + 8 * g);
 #pragma unroll
-            for (int t = 0; t < T; ++t) y[t] = mfma32(a, relu_pair(acc[2 * ks][t], acc[2 * ks + 1][t]), y[t]);
+            for (int t = 0; t < T; ++t)
+                y[t] = mfma32(a, relu_pair(acc[2 * ks][t], acc[2 * ks + 1][t]), ks ? y[t] : f4v{0.0f, 0.0f, 0.0f, 0.0f});
         }
         // Lanes 0..15 hold outputs 0..3 of query 16 t + l.
         if (g == 0) {
 #pragma unroll
-            for (int t = 0; t < T; ++t)
-                if (live[t]) *reinterpret_cast<float4 *>(out + 4 * (size_t)(base + 16 * t + r)) = make_float4(y[t][0], y[t][1], y[t][2], y[t][3]);
+            for (int t = 0; t < T; ++t) {
+                const uint64_t q = (uint64_t)p * (16 * T) + 16 * t + r;
+                if (q < n) *reinterpret_cast<float4 *>(out + 4 * q) = make_float4(y[t][0], y[t][1], y[t][2], y[t][3]);
+            }
         }
     }
 }

@@ -25,8 +25,14 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
 
 
-@pytest.mark.parametrize("n", [1, 17, 1000, 4099])
+@pytest.mark.parametrize("n", [1, 17, 1000, 4099, 65536])
 def test_forward_matches_oracle(gpu_ctx, n):
+    # Against the oracle that rounds weights and every layer input to bf16 as the kernel does, the
+    # outputs agree to fp32 accumulation order except where a hidden activation's fp32 sum lies within
+    # that order's rounding of a bf16 rounding boundary: the activation then rounds the other way (one
+    # bf16 step, 2^-8 relative) and moves its query's outputs by up to about 1e-3 of the output scale.
+    # Bars: 99.9% of outputs within 1e-3, the mean within 1e-5, every output within 5e-3 (a few such
+    # steps in one query).
     cache = vt.NRC(gpu_ctx, seed=3)
     p = _params(1)
     cache.params = p
@@ -35,10 +41,12 @@ def test_forward_matches_oracle(gpu_ctx, n):
     ref, _ = onrc.forward(p.astype(np.float64), x, bf16=True)
     exact, _ = onrc.forward(p.astype(np.float64), x)
     scale = np.abs(exact).max() + 1e-6
-    err_bf16 = np.abs(y - ref).max() / scale
+    err = np.abs(y - ref) / scale
     err_exact = np.abs(y - exact).max() / scale
-    print(f"n {n}: max |gpu - oracle(bf16)| / scale {err_bf16:.2e}, vs fp64 {err_exact:.2e}", flush=True)
-    assert err_bf16 < 2e-2 and err_exact < 5e-2
+    print(f"n {n}: |gpu - oracle(bf16)| / scale max {err.max():.2e} p99.9 {np.quantile(err, 0.999):.2e} "
+          f"mean {err.mean():.2e}, vs fp64 {err_exact:.2e}", flush=True)
+    assert np.quantile(err, 0.999) < 1e-3 and err.mean() < 1e-5 and err.max() < 5e-3
+    assert err_exact < 2e-2
     cache.close()
 
 
