@@ -128,6 +128,51 @@ vt_status vt_trace_stats(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, in
 vt_status vt_trace_stats_noalloc(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, int any,
                                  const vt_hit_soa *hits, uint8_t *occluded, uint64_t *counters, void *stream);
 
+/* ---------------------------------------------------------------- two-level scenes (instances) */
+
+/* One instance of a published scene (the reference's MeshSceneLayout.Instance,
+ * Sources/Acceleration.swift:367-376): `mesh` is its asset, whose hierarchy serves as the BLAS, and
+ * world[] the rows of its object-to-world matrix (x = world[0..2] . p + world[3], y from world[4..7],
+ * z from world[8..11]). */
+typedef struct vt_instance {
+    vt_scene *mesh;
+    float world[12];
+} vt_instance;
+typedef struct vt_tlas vt_tlas;
+
+/* Replaces MeshSceneLayout.addInstance + sceneBlob (Sources/Acceleration.swift:470-573): checks each
+ * transform as addInstance does (VT_ERROR_INVALID_ARGUMENT with its messages: "Combined hierarchy
+ * scale exceeds the supported range.", "Singular USD transform.", "Hierarchy transform exceeds the
+ * supported scene extent."), builds the instances' world-to-object rows and world bounds, and an
+ * 8-wide hierarchy over those bounds, one instance per leaf.  The meshes must outlive the vt_tlas;
+ * a mesh rebuilt with vt_scene_rebuild is picked up by the next trace call (which then waits for
+ * the device).  An instance of a mesh without faces is kept and never hit. */
+vt_status vt_tlas_create(vt_context *ctx, const vt_instance *instances, uint32_t count, vt_tlas **out);
+void vt_tlas_destroy(vt_tlas *tlas);
+
+typedef struct vt_tlas_info {
+    uint32_t instance_count, node_count, depth; /* the instance hierarchy */
+    uint32_t blas_depth;                        /* deepest mesh hierarchy */
+    uint64_t rendered_triangles;                /* sum of the instances' face counts */
+    uint64_t device_bytes;                      /* instance nodes + instance records */
+    float enlarge, reach; /* MeshSceneHeader lo.w and hi.w: per-ray box growth enlarge (|origin|max + reach) */
+    float build_seconds;
+} vt_tlas_info;
+vt_status vt_tlas_get_info(const vt_tlas *tlas, vt_tlas_info *info);
+
+/* Replaces the software branch of mesh_closest (main.swift:1843-1900): closest hit over the
+ * instances in [tmin, tmax).  hits->face is the face's index in its mesh (document order),
+ * hits->instance the instance's position in the list given to vt_tlas_create (0xffffffff on a
+ * miss), b1 / b2 the barycentrics of the object-space hit.  Each instance traces its mesh with the
+ * object-space ray origin' = W^-1 (origin), direction' = W^-1 (direction), not renormalised (main.swift:
+ * 1634-1637), so t is the world ray's parameter.  Among hits at exactly equal t the smaller instance
+ * index wins, then the smaller face index.  Asynchronous on `stream`. */
+vt_status vt_tlas_trace_closest(vt_tlas *tlas, const vt_ray_soa *rays, uint32_t n, const vt_hit_soa *hits,
+                                void *stream);
+/* Replaces the software branch of mesh_occluded (main.swift:1902-1983, without the endpoint slack):
+ * occluded[i] = 1 when any instance's mesh is hit in [tmin, tmax). */
+vt_status vt_tlas_trace_any(vt_tlas *tlas, const vt_ray_soa *rays, uint32_t n, uint8_t *occluded, void *stream);
+
 /* ---------------------------------------------------------------- materials and rendering */
 
 typedef struct vt_renderer vt_renderer;

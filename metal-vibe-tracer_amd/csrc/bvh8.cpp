@@ -848,6 +848,7 @@ struct Layout {
     const uint32_t *faces;
     std::vector<Node8> &out;
     std::vector<TriRecord> &tris;
+    std::vector<uint32_t> *items = nullptr;  // box hierarchies: leaf slots name items, not triangles
 
     void tri_record(int32_t item, TriRecord &t) const {
         uint32_t f = (uint32_t)item;
@@ -901,7 +902,8 @@ struct Layout {
         }
         if (next_node != N || next_tri != face_count) throw std::runtime_error("BVH layout size mismatch");
         out.resize(N);
-        tris.resize(face_count);
+        if (items) items->resize(face_count);
+        else tris.resize(face_count);
         // Contiguous runs of wide nodes, whose records and triangles lie close together in the output
         // (interleaved nodes would have threads writing into the same cache lines).
         parallel_run(4 * T, [&](int th) {
@@ -944,7 +946,10 @@ struct Layout {
                 const int c = w.count[k];
                 if (c < 1 || c > 3 || tri_off + c > 24) throw std::runtime_error("leaf does not fit the node encoding");
                 node.meta[s] = (uint8_t)((tri_off << 2) | c);
-                for (int q = 0; q < c; ++q) tri_record(order[first + q], tris[tri_base + (uint32_t)(tri_off + q)]);
+                for (int q = 0; q < c; ++q) {
+                    if (items) (*items)[tri_base + (uint32_t)(tri_off + q)] = (uint32_t)order[first + q];
+                    else tri_record(order[first + q], tris[tri_base + (uint32_t)(tri_off + q)]);
+                }
                 tri_off += c;
             }
         }
@@ -968,8 +973,11 @@ struct Layout {
 
 }  // namespace
 
-Bvh8 build_bvh8(const float *vertices, uint32_t vertex_count, const uint32_t *faces, uint32_t n,
-                const BuildOptions &opt_in) {
+// The wide tree over n items whose bounds `item_bounds(refs, scene)` writes in document order, laid
+// out over the triangles of `faces` (tris) or, with `items`, naming the items of each leaf slot.
+template <class ItemBounds>
+static Bvh8 build_over(uint32_t n, const BuildOptions &opt_in, ItemBounds &&item_bounds, const float *vertices,
+                       const uint32_t *faces, std::vector<uint32_t> *items) {
     auto t0 = std::chrono::steady_clock::now();
     BuildOptions opt = opt_in;
     if (opt.leaf_size < 1 || opt.leaf_size > 3) throw std::invalid_argument("leaf_size must be 1..3");
@@ -978,7 +986,60 @@ Bvh8 build_bvh8(const float *vertices, uint32_t vertex_count, const uint32_t *fa
     if (opt.collapse < 0 || opt.collapse > 1) throw std::invalid_argument("collapse must be 0 or 1");
     Bvh8 out;
     if (n == 0) return out;
+    std::unique_ptr<PrimRef[]> refs = raw_array<PrimRef>(n);
+    Box scene;
+    item_bounds(refs.get(), scene);
 
+    // Each attempt starts from the items in document order; the order the winning attempt leaves
+    // them in is the leaf order.
+    std::vector<WideNode> wide;
+    int depth = opt.max_depth + 1;
+    if (opt.collapse == 0) depth = build_wide_optimal(refs.get(), (int32_t)n, opt, wide);
+    if (depth > opt.max_depth) {
+        wide.clear();
+        if (opt.collapse == 0) item_bounds(refs.get(), scene);
+        depth = build_wide(refs.get(), (int32_t)n, opt, false, wide);
+    }
+    if (depth > opt.max_depth) {
+        wide.clear();
+        item_bounds(refs.get(), scene);
+        depth = build_wide(refs.get(), (int32_t)n, opt, true, wide);
+        if (depth > opt.max_depth) throw std::runtime_error("BVH deeper than the traversal stack");
+    }
+    std::vector<int32_t> order(n);
+    for (uint32_t k = 0; k < n; ++k) order[k] = refs[k].idx;
+    refs.reset();
+    out.depth = depth;
+    Layout L{wide, order.data(), vertices, faces, out.nodes, out.tris, items};
+    L.run(n);
+    float m = 0;
+    for (int a = 0; a < 3; ++a) m = std::max(m, std::max(std::fabs(scene.lo[a]), std::fabs(scene.hi[a])));
+    out.root_max = m;
+    out.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return out;
+}
+
+Bvh8 build_bvh8_boxes(const float *boxes, uint32_t n, const BuildOptions &opt, std::vector<uint32_t> &items) {
+    auto item_bounds = [&](PrimRef *refs, Box &scene) {
+        scene = Box();
+        for (uint32_t i = 0; i < n; ++i) {
+            PrimRef &q = refs[i];
+            for (int k = 0; k < 3; ++k) {
+                q.lo[k] = boxes[6 * (size_t)i + k];
+                q.hi[k] = boxes[6 * (size_t)i + 3 + k];
+                if (!std::isfinite(q.lo[k]) || !std::isfinite(q.hi[k]) || q.lo[k] > q.hi[k])
+                    throw std::invalid_argument("invalid item box");
+            }
+            q.idx = (int32_t)i;
+            q.pad = 0.0f;
+            scene.grow(q.lo, q.hi);
+        }
+    };
+    return build_over(n, opt, item_bounds, nullptr, nullptr, &items);
+}
+
+Bvh8 build_bvh8(const float *vertices, uint32_t vertex_count, const uint32_t *faces, uint32_t n,
+                const BuildOptions &opt_in) {
     // Item bounds (the reference's MeshSceneLayout.bounds, Sources/Acceleration.swift:445-458) in
     // document order, by all threads; the scene box.
     auto item_bounds = [&](PrimRef *refs, Box &scene) {
@@ -1006,37 +1067,7 @@ Bvh8 build_bvh8(const float *vertices, uint32_t vertex_count, const uint32_t *fa
         scene = Box();
         for (auto &b : part) scene.grow(b);
     };
-    std::unique_ptr<PrimRef[]> refs = raw_array<PrimRef>(n);
-    Box scene;
-    item_bounds(refs.get(), scene);
-
-    // Each attempt starts from the items in document order; the order the winning attempt leaves
-    // them in is the leaf order.
-    std::vector<WideNode> wide;
-    int depth = opt.max_depth + 1;
-    if (opt.collapse == 0) depth = build_wide_optimal(refs.get(), (int32_t)n, opt, wide);
-    if (depth > opt.max_depth) {
-        wide.clear();
-        if (opt.collapse == 0) item_bounds(refs.get(), scene);
-        depth = build_wide(refs.get(), (int32_t)n, opt, false, wide);
-    }
-    if (depth > opt.max_depth) {
-        wide.clear();
-        item_bounds(refs.get(), scene);
-        depth = build_wide(refs.get(), (int32_t)n, opt, true, wide);
-        if (depth > opt.max_depth) throw std::runtime_error("BVH deeper than the traversal stack");
-    }
-    std::vector<int32_t> order(n);
-    for (uint32_t k = 0; k < n; ++k) order[k] = refs[k].idx;
-    refs.reset();
-    out.depth = depth;
-    Layout L{wide, order.data(), vertices, faces, out.nodes, out.tris};
-    L.run(n);
-    float m = 0;
-    for (int a = 0; a < 3; ++a) m = std::max(m, std::max(std::fabs(scene.lo[a]), std::fabs(scene.hi[a])));
-    out.root_max = m;
-    out.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    return out;
+    return build_over(n, opt_in, item_bounds, vertices, faces, nullptr);
 }
 
 }  // namespace vt

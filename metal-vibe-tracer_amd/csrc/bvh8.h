@@ -60,4 +60,9 @@ struct Bvh8 {
 Bvh8 build_bvh8(const float *vertices, uint32_t vertex_count, const uint32_t *faces, uint32_t face_count,
                 const BuildOptions &opt);
 
+// The same hierarchy over n boxes (6 floats each: lo xyz, hi xyz), e.g. instance bounds.  Leaf slots
+// name items instead of holding triangle records: tri_base + offset indexes `items`, which receives
+// the item numbers in leaf order (Bvh8::tris stays empty).
+Bvh8 build_bvh8_boxes(const float *boxes, uint32_t n, const BuildOptions &opt, std::vector<uint32_t> &items);
+
 }  // namespace vt

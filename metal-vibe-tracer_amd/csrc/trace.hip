@@ -380,9 +380,78 @@ __global__ __launch_bounds__(refill_block<kStack>()) void trace_refill_kernel(co
                                                       n);
 }
 
+// Two-level traversal, one ray per lane (main.swift:1843-1900 mesh_closest, 1902-1983 mesh_occluded,
+// software branch).  The instance hierarchy is walked as a one-level one whose leaves name instances,
+// with the boxes grown by enlarge (|origin|max + reach); a leaf's instance traces its asset with the
+// object-space ray, up to the closest hit so far.  Each level keeps its node-group stack in LDS.
+// Ties: the closest-hit rule compares (t, instance, face), so the result does not depend on the order
+// instances are visited in.
+template <bool ANY, int kTop, int kBot>
+__global__ __launch_bounds__(kBlock) void trace_tlas_kernel(TlasScene sc, RayArgs rays, HitArgs hits,
+                                                            uint8_t *__restrict__ occluded, uint32_t n) {
+    __shared__ uint32_t top_base[kTop][kBlock], top_bits[kTop][kBlock];
+    __shared__ uint32_t bot_base[kBot][kBlock], bot_bits[kBot][kBlock];
+    const int lane = threadIdx.x;
+    const uint32_t i = blockIdx.x * kBlock + lane;
+    if (i >= n) return;
+    const TraceRay r{rays.ox[i], rays.oy[i], rays.oz[i], rays.dx[i], rays.dy[i], rays.dz[i], rays.tmin[i], rays.tmax[i]};
+    LdsStack<kTop> top{top_base, top_bits, lane, 0, kTop};
+    LdsStack<kBot> bot{bot_base, bot_bits, lane, 0, kBot};
+    TravState ts;
+    trav_init_e(ts, r, sc.enlarge * (max_abs3(r.ox, r.oy, r.oz) + sc.reach));
+    float bt = r.tmax, bb1 = 0.0f, bb2 = 0.0f;
+    int32_t bface = -1;
+    uint32_t binst = 0xffffffffu;
+    bool occ = false, overflow = false;
+    const U4 *tn = reinterpret_cast<const U4 *>(sc.nodes);
+    for (;;) {
+        uint32_t items, base;
+        const bool done = trav_nodes<false>(ts, tn, top, items, base);
+        overflow |= ts.res.overflow;
+        while (items && !occ) {
+            const uint32_t b = __builtin_ctz(items);
+            items &= items - 1u;
+            const InstanceRecord &R = sc.instances[base + b];
+            if (R.faces == 0) continue;
+            TravState bs;
+            trav_init(bs, instance_ray(R.local, r, bt), R.root_max);
+            // Equal t: a hit of this instance wins over the current one only from a smaller index.
+            const int32_t sentinel = (bface >= 0 && R.index < binst) ? 0x7fffffff : -1;
+            bs.res.face = sentinel;
+            bot.sp = 0;
+            const U4 *bn = reinterpret_cast<const U4 *>(R.nodes);
+            const F4 *btr = reinterpret_cast<const F4 *>(R.tris);
+            while (!trav_step<ANY, false>(bs, bn, btr, bot)) {
+            }
+            overflow |= bs.res.overflow;
+            if (ANY) {
+                occ = bs.res.occluded;
+            } else if (bs.res.face != sentinel) {
+                bt = bs.res.t;
+                bb1 = bs.res.b1;
+                bb2 = bs.res.b2;
+                bface = bs.res.face;
+                binst = R.index;
+                ts.res.t = bt;
+            }
+        }
+        if (occ || done) break;
+    }
+    if (overflow && sc.fault) atomicOr(sc.fault, 1u);
+    if (ANY) {
+        occluded[i] = occ ? 1 : 0;
+    } else {
+        hits.t[i] = bt;
+        hits.b1[i] = bb1;
+        hits.b2[i] = bb2;
+        hits.face[i] = bface;
+        if (hits.instance) hits.instance[i] = binst;
+    }
+}
+
 template <bool ANY>
 __global__ __launch_bounds__(256) void miss_kernel(RayArgs rays, HitArgs hits, uint8_t *occluded, uint32_t n,
-                                                   const uint32_t *count) {
+                                                   const uint32_t *count, uint32_t miss_instance) {
     uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (count) n = min(n, *count);
     if (i >= n) return;
@@ -393,7 +462,7 @@ __global__ __launch_bounds__(256) void miss_kernel(RayArgs rays, HitArgs hits, u
         hits.b1[i] = 0.0f;
         hits.b2[i] = 0.0f;
         hits.face[i] = -1;
-        if (hits.instance) hits.instance[i] = 0u;
+        if (hits.instance) hits.instance[i] = miss_instance;
     }
 }
 
@@ -526,13 +595,41 @@ hipError_t launch_trace_batch(const TraceScene *job_trace, const uint32_t *job, 
     return hipGetLastError();
 }
 
+hipError_t launch_trace_tlas(const TlasScene &scene, const RayArgs &rays, const HitArgs &hits, uint8_t *occluded,
+                             uint32_t n, bool any, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    if (scene.node_count == 0) {
+        dim3 grid((n + 255) / 256);
+        if (any) hipLaunchKernelGGL(miss_kernel<true>, grid, dim3(256), 0, stream, rays, hits, occluded, n, nullptr, 0u);
+        else hipLaunchKernelGGL(miss_kernel<false>, grid, dim3(256), 0, stream, rays, hits, occluded, n, nullptr, 0xffffffffu);
+        return hipGetLastError();
+    }
+    const int top = scene.depth > 1 ? (int)scene.depth - 1 : 1, bot = scene.blas_depth > 1 ? (int)scene.blas_depth - 1 : 1;
+    if (top > 16 || bot > 24) return hipErrorInvalidValue;
+    dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+#define VT_TLAS(T, B)                                                                                                  \
+    do {                                                                                                               \
+        if (any) hipLaunchKernelGGL((trace_tlas_kernel<true, T, B>), grid, block, 0, stream, scene, rays, hits, occluded, n); \
+        else hipLaunchKernelGGL((trace_tlas_kernel<false, T, B>), grid, block, 0, stream, scene, rays, hits, occluded, n); \
+    } while (0)
+    if (top <= 8) {
+        if (bot <= 12) VT_TLAS(8, 12);
+        else VT_TLAS(8, 24);
+    } else {
+        if (bot <= 12) VT_TLAS(16, 12);
+        else VT_TLAS(16, 24);
+    }
+#undef VT_TLAS
+    return hipGetLastError();
+}
+
 hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitArgs &hits, uint8_t *occluded,
                         uint32_t n, bool any, unsigned long long *counters, hipStream_t stream, const uint32_t *count) {
     if (n == 0) return hipSuccess;
     if (scene.node_count == 0) {
         dim3 grid((n + 255) / 256);
-        if (any) hipLaunchKernelGGL(miss_kernel<true>, grid, dim3(256), 0, stream, rays, hits, occluded, n, count);
-        else hipLaunchKernelGGL(miss_kernel<false>, grid, dim3(256), 0, stream, rays, hits, occluded, n, count);
+        if (any) hipLaunchKernelGGL(miss_kernel<true>, grid, dim3(256), 0, stream, rays, hits, occluded, n, count, 0u);
+        else hipLaunchKernelGGL(miss_kernel<false>, grid, dim3(256), 0, stream, rays, hits, occluded, n, count, 0u);
         return hipGetLastError();
     }
     dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);

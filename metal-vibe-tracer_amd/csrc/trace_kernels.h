@@ -27,6 +27,33 @@ struct TraceScene {
     uint32_t stack_cap;
 };
 
+// Two-level scenes (the reference's MeshSceneLayout, Sources/Acceleration.swift:356-573): an 8-wide
+// hierarchy over the instances' world bounds whose leaf slots name InstanceRecords (in leaf order),
+// each tracing its asset's hierarchy with the object-space ray.
+struct alignas(16) InstanceRecord {
+    float local[3][4];  // rows of the world-to-object matrix: x = row.xyz . p + row.w
+    const void *nodes;  // the asset's Node8 records
+    const void *tris;   // the asset's TriRecord records
+    float root_max;     // the asset's max |local coordinate| (its box enlargement term)
+    uint32_t index;     // the instance's position in the caller's list
+    uint32_t faces;     // the asset's face count (0: nothing to trace)
+    uint32_t pad;
+};
+static_assert(sizeof(InstanceRecord) == 80, "InstanceRecord is five 16-byte words");
+struct TlasScene {
+    const void *nodes;                  // Node8 records over instance bounds
+    const InstanceRecord *instances;    // leaf order
+    uint32_t node_count, depth;         // of the instance hierarchy
+    uint32_t blas_depth;                // deepest asset hierarchy
+    float enlarge, reach;               // per-ray growth of the instance boxes: enlarge (|origin|max + reach)
+    uint32_t *fault;                    // raised (bit 0) by a ray whose node-group stack filled
+};
+// Closest hit (hits.instance = the instance's index, 0xffffffff on a miss) or any hit over a
+// two-level scene.  Among hits at exactly equal t the smaller instance index wins, then the smaller
+// face index.
+hipError_t launch_trace_tlas(const TlasScene &scene, const RayArgs &rays, const HitArgs &hits, uint8_t *occluded,
+                             uint32_t n, bool any, hipStream_t stream);
+
 // Closest hit (any = false) into `hits`, or any hit into `occluded`.  `counters` (device, 128
 // u64, zeroed by the caller) selects the counting variant: 64 pairs (node fetches, triangle tests).
 // `count` (device, optional) caps n at launch time: the wavefront integrator's queue length.

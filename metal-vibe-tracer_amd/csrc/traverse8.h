@@ -136,7 +136,8 @@ struct TravState {
     TraceResult res;
 };
 
-VT_HD void trav_init(TravState &st, const TraceRay &r, float root_max) {
+// Starts a ray with box enlargement e (trav_init: the reference's e for a one-level hierarchy).
+VT_HD void trav_init_e(TravState &st, const TraceRay &r, float e) {
     st.res.t = r.tmax;
     st.res.b1 = st.res.b2 = 0.0f;
     st.res.face = -1;
@@ -149,7 +150,6 @@ VT_HD void trav_init(TravState &st, const TraceRay &r, float root_max) {
     st.oz = r.oz;
     st.tmin = r.tmin;
     st.tmax = r.tmax;
-    const float e = 1.1920929e-6f * (fmaxf(fabsf(r.ox), fmaxf(fabsf(r.oy), fabsf(r.oz))) + root_max);
     st.e = e;
     // A component below 1e-8 in magnitude (the reference's parallel threshold) is taken as +-1e-8 with
     // its sign: the slab of a parallel axis then spans about -+1e8 x the origin's distance to its
@@ -163,6 +163,30 @@ VT_HD void trav_init(TravState &st, const TraceRay &r, float root_max) {
     st.cur_base = 0;
     st.cur_bits = 0;
     st.node = 0;
+}
+
+VT_HD float max_abs3(float x, float y, float z) { return fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z))); }
+
+// main.swift:1790-1792: e = 1.1920929e-6 (|origin|max + root max).
+VT_HD void trav_init(TravState &st, const TraceRay &r, float root_max) {
+    trav_init_e(st, r, 1.1920929e-6f * (max_abs3(r.ox, r.oy, r.oz) + root_max));
+}
+
+// The object-space ray of an instance (main.swift:1634-1637 instance_ray): origin and direction by
+// the rows of the world-to-object matrix, the direction not renormalised, so t keeps its meaning.
+// Each dot product is evaluated as ((r0 x + r1 y) + r2 z) (+ r3), every operation rounded.
+VT_HD TraceRay instance_ray(const float (&local)[3][4], const TraceRay &r, float tmax) {
+#pragma clang fp contract(off)
+    TraceRay l;
+    l.ox = ((local[0][0] * r.ox + local[0][1] * r.oy) + local[0][2] * r.oz) + local[0][3];
+    l.oy = ((local[1][0] * r.ox + local[1][1] * r.oy) + local[1][2] * r.oz) + local[1][3];
+    l.oz = ((local[2][0] * r.ox + local[2][1] * r.oy) + local[2][2] * r.oz) + local[2][3];
+    l.dx = (local[0][0] * r.dx + local[0][1] * r.dy) + local[0][2] * r.dz;
+    l.dy = (local[1][0] * r.dx + local[1][1] * r.dy) + local[1][2] * r.dz;
+    l.dz = (local[2][0] * r.dx + local[2][1] * r.dy) + local[2][2] * r.dz;
+    l.tmin = r.tmin;
+    l.tmax = tmax;
+    return l;
 }
 
 // Node half of a step: tests st.node's 8 child boxes, pushes the current group if inner children were

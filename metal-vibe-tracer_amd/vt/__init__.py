@@ -37,7 +37,8 @@ EXPORTED = [
     "vt_obj_part_count", "vt_obj_skipped", "vt_obj_library_count", "vt_obj_library", "vt_obj_part", "vt_obj_subset",
     "vt_z_draws", "vt_context_check", "vt_scene_set_stack_cap", "vt_mesh_from_triangles",
     "vt_scene_upload_triangles", "vt_scene_set_surface_settings", "vt_renderer_set_callbacks",
-    "vt_trace_stats_noalloc",
+    "vt_trace_stats_noalloc", "vt_tlas_create", "vt_tlas_destroy", "vt_tlas_get_info", "vt_tlas_trace_closest",
+    "vt_tlas_trace_any",
 ]
 
 
@@ -62,6 +63,19 @@ class _SceneInfo(ctypes.Structure):
                 ("depth", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32), ("width", ctypes.c_uint32),
                 ("node_bytes", ctypes.c_uint32), ("tri_bytes", ctypes.c_uint32), ("device_bytes", ctypes.c_uint64),
                 ("root_max", ctypes.c_float), ("build_seconds", ctypes.c_float)]
+
+
+class Instance(ctypes.Structure):
+    """vt_instance: a published scene (its `mesh` handle) and the 12 rows-major floats of its
+    object-to-world matrix (3 rows of 4)."""
+    _fields_ = [("mesh", ctypes.c_void_p), ("world", ctypes.c_float * 12)]
+
+
+class _TlasInfo(ctypes.Structure):
+    _fields_ = [("instance_count", ctypes.c_uint32), ("node_count", ctypes.c_uint32), ("depth", ctypes.c_uint32),
+                ("blas_depth", ctypes.c_uint32), ("rendered_triangles", ctypes.c_uint64),
+                ("device_bytes", ctypes.c_uint64), ("enlarge", ctypes.c_float), ("reach", ctypes.c_float),
+                ("build_seconds", ctypes.c_float)]
 
 
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -157,6 +171,11 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_scene_get_info": (i32, [vp, P(_SceneInfo)]),
         "vt_trace_closest": (i32, [vp, P(_RaySoA), u32, P(_HitSoA), vp]),
         "vt_trace_any": (i32, [vp, P(_RaySoA), u32, vp, vp]),
+        "vt_tlas_create": (i32, [vp, vp, u32, P(vp)]),
+        "vt_tlas_destroy": (None, [vp]),
+        "vt_tlas_get_info": (i32, [vp, P(_TlasInfo)]),
+        "vt_tlas_trace_closest": (i32, [vp, P(_RaySoA), u32, P(_HitSoA), vp]),
+        "vt_tlas_trace_any": (i32, [vp, P(_RaySoA), u32, vp, vp]),
         "vt_trace_stats": (i32, [vp, P(_RaySoA), u32, c.c_int, P(c.c_uint64), vp]),
         "vt_bvh8_build_host": (i32, [vp, u32, vp, u32, P(_BuildOptions), P(vp), P(u32), P(vp), P(u32), P(f32)]),
         "vt_host_free": (None, [vp]),
@@ -830,6 +849,64 @@ class Batch:
         ms = ctypes.c_double()
         _check(load().vt_batch_autotune(self._h, 1 if leaf_size else 0, ctypes.byref(t), ctypes.byref(ms)))
         return t.as_dict(), ms.value
+
+
+class Tlas:
+    """A two-level scene (vt_tlas): instances = [(scene, world)], world a (3, 4) object-to-world
+    matrix.  The scenes must stay open while the Tlas is used."""
+
+    def __init__(self, ctx: "Context", instances):
+        arr = (Instance * max(1, len(instances)))()
+        self._scenes = []
+        for k, (scene, world) in enumerate(instances):
+            w = np.ascontiguousarray(world, np.float32).reshape(12)
+            arr[k].mesh = scene._h.value
+            arr[k].world[:] = w.tolist()
+            self._scenes.append(scene)
+        h = ctypes.c_void_p()
+        _check(load().vt_tlas_create(ctx._h, arr, len(instances), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            load().vt_tlas_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def info(self) -> dict:
+        i = _TlasInfo()
+        _check(load().vt_tlas_get_info(self._h, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in _TlasInfo._fields_}
+
+    def trace_closest(self, rays, out=None, stream=None):
+        """rays: (8, n) float32 device tensor.  Returns device tensors t, b1, b2, face, instance."""
+        import torch
+        rs, n = _rays_struct(rays)
+        if out is None:
+            out = {"t": torch.empty(n, dtype=torch.float32, device=rays.device),
+                   "b1": torch.empty(n, dtype=torch.float32, device=rays.device),
+                   "b2": torch.empty(n, dtype=torch.float32, device=rays.device),
+                   "face": torch.empty(n, dtype=torch.int32, device=rays.device),
+                   "instance": torch.empty(n, dtype=torch.int32, device=rays.device)}
+        hs = _HitSoA(out["t"].data_ptr(), out["b1"].data_ptr(), out["b2"].data_ptr(), out["face"].data_ptr(),
+                     out["instance"].data_ptr())
+        _check(load().vt_tlas_trace_closest(self._h, ctypes.byref(rs), n, ctypes.byref(hs), _stream_ptr(stream)))
+        return out
+
+    def trace_any(self, rays, out=None, stream=None):
+        import torch
+        rs, n = _rays_struct(rays)
+        if out is None:
+            out = torch.empty(n, dtype=torch.uint8, device=rays.device)
+        _check(load().vt_tlas_trace_any(self._h, ctypes.byref(rs), n, ctypes.c_void_p(out.data_ptr()),
+                                        _stream_ptr(stream)))
+        return out
 
 
 class NRC:
