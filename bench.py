@@ -66,6 +66,62 @@ def workload_heightfield():
     return "fbm-heightfield-2237^2", mesh, prim, 3
 
 
+def shadow_rays(mesh, bounce, seed):
+    """(8, n) float32 rays from the bounce rays' origins to uniform points of a square light of half
+    the scene box's largest extent, centred half an extent above the box (+y); tmax stops 1e-4 of
+    the distance short of the light."""
+    import numpy as np
+    lo, hi = mesh.vertices[:, :3].min(0).astype(np.float64), mesh.vertices[:, :3].max(0).astype(np.float64)
+    ext = float((hi - lo).max())
+    n = bounce.shape[1]
+    rng = np.random.default_rng(seed + 101)
+    c = 0.5 * (lo + hi)
+    light = np.stack([c[0] + (rng.random(n) - 0.5) * 0.5 * ext, np.full(n, hi[1] + 0.5 * ext),
+                      c[2] + (rng.random(n) - 0.5) * 0.5 * ext])
+    o = bounce[0:3].astype(np.float64)
+    d = light - o
+    dist = np.sqrt((d * d).sum(0))
+    out = np.empty((8, n), np.float32)
+    out[0:3] = bounce[0:3]
+    out[3:6] = d / dist
+    out[6] = bounce[6]
+    out[7] = dist * (1.0 - 1e-4)
+    return out
+
+
+def timed_render(fn, *args, **kw):
+    """(seconds, result) of one oracle render call.  Its callers time a second call over a single
+    pixel sample to measure the call's set-up (material records, emitter list, light tree) and leave
+    it out of the path-tracing rate."""
+    c0 = time.perf_counter()
+    out = fn(*args, **kw)
+    return time.perf_counter() - c0, out
+
+
+def measure_bandwidth(dev):
+    """Device-to-device copy rates (read + write bytes / time): a 2 x 1 GiB copy (HBM) and a
+    2 x 32 MiB copy repeated 20 times (a working set inside the 256 MB Infinity Cache), best of 3."""
+    import torch
+    out = {}
+    for name, mib, reps in (("hbm_copy_gbs", 1024, 1), ("mall_copy_gbs", 32, 20)):
+        src = torch.ones(mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        best = float("inf")
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                dst.copy_(src)
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1) / reps)
+        out[name] = round(2 * (mib << 20) / (best * 1e-3) / 1e9, 1)
+        del src, dst
+    torch.cuda.empty_cache()
+    return out
+
+
 def measure_traversal(args, ctx, name, mesh, prim, seed):
     """One traversal workload: K timed steps of camera + bounce closest-hit launches, per-launch
     kernel times, roofline, incoherent-ray rate, BVH build time and the CPU baseline."""
@@ -156,7 +212,28 @@ def measure_traversal(args, ctx, name, mesh, prim, seed):
                   "mrays_per_s": round(inc.shape[1] / best / 1e3, 1),
                   "recipe": "origins uniform in the scene box, directions uniform on the sphere, best of 3 after a warm-up"}
 
+    # Shadow set: from each bounce origin (a camera hit point) to a uniform point of a square area
+    # light above the scene, ending just short of it; any-hit traversal.
+    shadow = shadow_rays(mesh, bounce, seed)
+    shadow_d = torch.from_numpy(shadow).to(dev)
+    occ = scene.trace_any(shadow_d, stream=stream)
+    ms_any = []
+    for k in range(6):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        scene.trace_any(shadow_d, out=occ, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if k:
+            ms_any.append(e0.elapsed_time(e1))
+    t_any = float(np.mean(ms_any))
+    any_hit = {"rays": int(shadow.shape[1]), "occluded": int(occ.sum().item()), "ms": round(t_any, 4),
+               "mrays_per_s": round(shadow.shape[1] / t_any / 1e3, 1),
+               "recipe": "bounce origins to uniform points of a square light above the scene box (half the "
+                         "box's largest extent wide, half an extent above it), mean of 5 after a warm-up"}
+
     cpu = None
+    cross_check = None
     if not args.no_cpu:
         import oracle
         sample = np.concatenate([prim, bounce], axis=1)
@@ -175,6 +252,23 @@ def measure_traversal(args, ctx, name, mesh, prim, seed):
                          f"binned-SAH BVH (leaf 2) and traversal restated in C, gcc -O3 -march=native, OpenMP "
                          f"over rays, median of 5 after a warm-up",
                "bvh_build_seconds": round(cpu_build, 2)}
+        osc.any(shadow, mode=0)
+        times = []
+        for _ in range(5):
+            c0 = time.perf_counter()
+            osc.any(shadow, mode=0)
+            times.append(time.perf_counter() - c0)
+        cpu["any_hit"] = {"value": round(shadow.shape[1] / float(np.median(times)) / 1e6, 3), "unit": "Mrays/s",
+                          "sample": f"the shadow set ({shadow.shape[1]} rays), median of 5 after a warm-up"}
+        # The same step priced with the reference 4-wide hierarchy's own work (its traversal's node
+        # visits x 128 B + triangle tests x 112 B) over this build's kernel time.
+        c4 = [0, 0]
+        osc.closest(sample, mode=0, counters=c4)
+        bytes4 = c4[0] * 128 + c4[1] * 112
+        ach4 = bytes4 / ((t_prim + t_bnc) * 1e-3) / 1e9
+        cross_check = {"achieved": round(ach4, 1), "frac": round(ach4 / HBM_PEAK_GBS, 4), "unit": "GB/s",
+                       "node_visits": int(c4[0]), "triangle_tests": int(c4[1]), "bytes_per_step": int(bytes4),
+                       "sizes": "128 B per 4-wide node, 112 B per triangle test"}
         del osc
     scene.close()
     return {
@@ -191,6 +285,8 @@ def measure_traversal(args, ctx, name, mesh, prim, seed):
                      "algorithmic_bytes_per_launch": (bytes_p + bytes_b) // 2,
                      "work": {"node_fetches": np_ + nb_, "triangle_tests": tp_ + tb_, "bytes_per_step": bytes_p + bytes_b}},
         "incoherent": incoherent,
+        "any_hit": any_hit,
+        "roofline_reference_4wide": cross_check,
         "cpu_baseline": cpu,
     }
 
@@ -315,9 +411,12 @@ def bench_learned_cache(args, ctx):
             spent += frame(r, 1 + n, cache=1)
             n += 1
         e = error(r)
+        trained.append(cache.params)
         r.close()
         cache.close()
         return e, n, spent, t_warm
+
+    trained = []
 
     budgets = []
     for frames in args.cache_budgets:
@@ -333,6 +432,31 @@ def bench_learned_cache(args, ctx):
     first = budgets[0]
     e_warm, n_warm, t_nrc, t_warm = learned(first["plain"]["seconds"], args.cache_warmup)
     scene.close()
+    cpu = None
+    if not args.no_cpu:
+        # Reference integrator (MIS) on the relMSE pixel subset, and the cache-terminated estimator
+        # with the warm cache's trained weights at a quarter of the resolution, both in C (oracle).
+        import oracle
+        osc = oracle.Scene(mesh.vertices, mesh.faces)
+        full = dict(width=w, height=h, eye=view[0], target=view[1], fov=view[2], max_depth=depth)
+        spp_cpu = 64
+        setup, _ = timed_render(oracle.render, osc, mats, full, 1, 1, pixels=pix[:1])
+        dt, img = timed_render(oracle.render, osc, mats, full, 1, spp_cpu, pixels=pix)
+        dt -= setup
+        small = dict(full, width=w // 4, height=h // 4)
+        setup_c, _ = timed_render(oracle.render_nrc, osc, mats, dict(full, width=1, height=1), 1, 1, trained[-1])
+        dt_c, img_c = timed_render(oracle.render_nrc, osc, mats, small, 1, 2, trained[-1])
+        dt_c -= setup_c
+        cpu = {"value": round(len(pix) * spp_cpu / dt / 1e6, 4), "unit": "Msamples/s", "cores": oracle.max_threads(),
+               "kind": "port", "mean_radiance": float(img.mean()),
+               "sample": f"the {len(pix)}-pixel relMSE subset x {spp_cpu} spp, reference MIS integrator in C "
+                         f"(per-call set-up, {setup:.2f} s, measured on one pixel sample and left out)",
+               "cache": {"value": round(small["width"] * small["height"] * 2 / dt_c / 1e6, 4), "unit": "Msamples/s",
+                         "mean_radiance": float(img_c.mean()),
+                         "sample": f"2 frames at {small['width']}x{small['height']}, the cache-terminated estimator "
+                                   f"with the warm cache's weights (fixed), in C"}}
+        del osc
+        log(f"learned cache CPU baseline: {cpu}")
     # The cache's inference kernel against the matrix cores' dense bf16 peak: 2M queries (a 1080p
     # frame's worth), timed with HIP events on the launch stream.
     cache = vt.NRC(ctx, seed=1)
@@ -353,6 +477,9 @@ def bench_learned_cache(args, ctx):
             "reference": {"kind": "FP64 oracle", "spp": int(np.load(path)["spp"]), "pixels": int(len(pix))},
             "learning_rate": lr, "train_period": 16,
             "relmse_ratio_cold_start": first["relmse_ratio"],
+            "gpu_msamples_per_s": {"plain": round(w * h * first["plain"]["spp"] / first["plain"]["seconds"] / 1e6, 2),
+                                   "cold_start": round(w * h * first["cold_start"]["spp"] / first["cold_start"]["seconds"] / 1e6, 2)},
+            "cpu_baseline": cpu,
             "budgets": budgets,
             "warm": {"plain_spp": first["plain"]["spp"], "spp": n_warm, "seconds": round(t_nrc, 4), "relmse": e_warm,
                      "warmup_frames": args.cache_warmup, "warmup_seconds": round(t_warm, 3),
@@ -400,11 +527,39 @@ def bench_restir(args, ctx):
                      candidates=32, sampler=sampler)
             torch.cuda.synchronize()
             spent += time.perf_counter() - t
-        img = r.readback()[..., :3].astype(np.float64).reshape(-1, 3)[pix]
+        full_img = r.readback()[..., :3].astype(np.float64)
+        img = full_img.reshape(-1, 3)[pix]
         out[name] = {"frames": args.restir_frames, "ms_per_frame": round(spent / args.restir_frames * 1e3, 3),
-                     "relmse": relmse(img, ref)}
+                     "msamples_per_s": round(w * h * args.restir_frames / spent / 1e6, 2),
+                     "relmse": relmse(img, ref), "mean_radiance": float(full_img.mean())}
         r.close()
     scene.close()
+    if not args.no_cpu:
+        # The reference's two ReSTIR DI passes and its MIS integrator restated in C (oracle): ReSTIR
+        # at a quarter of the resolution (reuse needs whole frames), MIS on the relMSE pixel subset.
+        import oracle
+        osc = oracle.Scene(mesh.vertices, mesh.faces)
+        full = dict(width=w, height=h, eye=view[0], target=view[1], fov=view[2], max_depth=depth)
+        small = dict(full, width=w // 4, height=h // 4)
+        frames = 2
+        setup_r, _ = timed_render(oracle.render_restir, osc, mats, dict(full, width=1, height=1), 1, 1, candidates=32)
+        dt_r, img_r = timed_render(oracle.render_restir, osc, mats, small, 1, frames, candidates=32)
+        dt_r -= setup_r
+        spp_cpu = 64
+        setup_m, _ = timed_render(oracle.render, osc, mats, full, 1, 1, pixels=pix[:1])
+        dt_m, img_m = timed_render(oracle.render, osc, mats, full, 1, spp_cpu, pixels=pix)
+        dt_m -= setup_m
+        out["cpu_baseline"] = {
+            "value": round(small["width"] * small["height"] * frames / dt_r / 1e6, 4), "unit": "Msamples/s",
+            "cores": oracle.max_threads(), "kind": "port", "mean_radiance": float(img_r.mean()),
+            "sample": f"{frames} ReSTIR DI frames (32 candidates, temporal + spatial reuse) at "
+                      f"{small['width']}x{small['height']}, same scene and camera, in C (per-call set-up, "
+                      f"{setup_r:.2f} s, measured on a 1x1 frame and left out)",
+            "mis": {"value": round(len(pix) * spp_cpu / dt_m / 1e6, 4), "unit": "Msamples/s",
+                    "mean_radiance": float(img_m.mean()),
+                    "sample": f"the {len(pix)}-pixel relMSE subset x {spp_cpu} spp, MIS integrator in C"}}
+        del osc
+        log(f"restir CPU baseline: {out['cpu_baseline']}")
     out["relmse_ratio_mis_over_restir"] = out["mis"]["relmse"] / out["restir_di"]["relmse"]
     out["efficiency_ratio"] = (out["mis"]["relmse"] * out["mis"]["ms_per_frame"]) / \
         (out["restir_di"]["relmse"] * out["restir_di"]["ms_per_frame"])
@@ -447,17 +602,45 @@ def bench_batch(args, ctx):
     b.render(spp)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    mean = float(np.mean([b.readback(k)[..., :3].mean() for k in range(len(jobs))]))
+    images = [b.readback(k)[..., :3] for k in range(len(jobs))]
+    mean = float(np.mean([im.mean() for im in images]))
     b.close()
     for s in scenes:
         s.close()
     log(f"batch: {len(jobs)} scenes, {tris} triangles, {pixels} pixels a sample: {spp} spp in {dt * 1e3:.1f} ms "
         f"({pixels * spp / dt / 1e6:.1f} Mpaths/s), tuning {best}")
+    cpu = None
+    if not args.no_cpu:
+        # Per-scene sequential path tracing with the reference integrator in C (oracle): every 509th
+        # pixel of each image at 2 spp, scene after scene (hierarchy builds untimed); the total for
+        # the whole batch (every pixel, 16 spp) is extrapolated from the sampled paths' rate.
+        import oracle
+        spent, paths, acc, acc_gpu = 0.0, 0, 0.0, 0.0
+        for k, (mesh, mats, w, h, eye, target, fov) in enumerate(work):
+            osc = oracle.Scene(mesh.vertices, mesh.faces)
+            cpix = np.arange(0, w * h, 509, dtype=np.uint32)
+            acc_gpu += float(images[k].reshape(-1, 3)[cpix].astype(np.float64).sum())
+            frame = dict(width=w, height=h, eye=eye, target=target, fov=fov, max_depth=6)
+            setup, _ = timed_render(oracle.render, osc, mats, frame, 1, 1, pixels=cpix[:1])
+            dt, img = timed_render(oracle.render, osc, mats, frame, 1, 2, pixels=cpix)
+            spent += max(dt - setup, 1e-9)
+            paths += len(cpix) * 2
+            acc += float(img.sum()) * 2
+            del osc
+        rate = paths / spent
+        cpu = {"value": round(rate / 1e6, 4), "unit": "Mpaths/s", "cores": oracle.max_threads(), "kind": "port",
+               "sequential_seconds_extrapolated": round(pixels * spp / rate, 1),
+               "mean_radiance": acc / (3 * paths), "gpu_mean_radiance_same_pixels": acc_gpu / (3 * paths / 2),
+               "sample": f"every 509th pixel of each of the {len(work)} images at 2 spp ({paths} paths), scenes one "
+                         f"after another, reference MIS integrator in C (each call's set-up, measured on one pixel "
+                         f"sample, left out); the full batch's time extrapolated"}
+        log(f"batch CPU baseline: {cpu}")
     return {"workload": f"batch-{len(jobs)}-icosphere-scenes-{tris}tri-{pixels}px-16spp-depth6",
             "scenes": len(jobs), "triangles": int(tris), "pixels_per_sample": int(pixels), "spp": spp,
             "seconds": round(dt, 4), "mpaths_per_s": round(pixels * spp / dt / 1e6, 1),
             "ms_per_sample": round(dt / spp * 1e3, 3), "tuning": best, "tuned_ms_per_sample": round(best_ms, 3),
-            "autotune_seconds": round(tune_s, 2), "scene_build_seconds": round(build_s, 2), "mean_radiance": mean}
+            "autotune_seconds": round(tune_s, 2), "scene_build_seconds": round(build_s, 2), "mean_radiance": mean,
+            "cpu_baseline": cpu}
 
 
 def main():
@@ -500,10 +683,19 @@ def main():
         import oracle
         oracle.load(oracle.native_library())     # the CPU baseline, built for this host
     ctx = vt.Context(0)
+    bandwidth = measure_bandwidth(dev)
+    log(f"measured copy bandwidth: {bandwidth}")
     first, second = (workload_heightfield, workload_stress) if args.workload == "heightfield" else \
         (workload_stress, workload_heightfield)
     head = measure_traversal(args, ctx, *first())
     secondary = None if args.no_secondary else measure_traversal(args, ctx, *second())
+    # Fractions of the measured copy rates: the HBM one for every workload, and the Infinity Cache
+    # one for the workload whose hierarchy fits in it (the 1.3M-triangle field, 113 MB).
+    for block in (head, secondary):
+        if block:
+            ach = block["roofline"]["achieved"]
+            block["roofline"]["frac_of_measured"] = {"hbm_copy": round(ach / bandwidth["hbm_copy_gbs"], 4),
+                                                     "mall_copy": round(ach / bandwidth["mall_copy_gbs"], 4)}
 
     integrator = None if args.no_integrator else bench_integrator(args, ctx)
     learned_cache = None if args.no_cache else bench_learned_cache(args, ctx)
@@ -530,6 +722,9 @@ def main():
         "roofline": head["roofline"],
         "incoherent": head["incoherent"],
         "cpu_baseline": head["cpu_baseline"],
+        "any_hit": head["any_hit"],
+        "roofline_reference_4wide": head["roofline_reference_4wide"],
+        "bandwidth_measured": bandwidth,
         "secondary": secondary,
         "integrator": integrator,
         "learned_cache": learned_cache,
