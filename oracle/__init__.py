@@ -311,7 +311,7 @@ class Frame(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("eye", ctypes.c_float * 3),
                 ("target", ctypes.c_float * 3), ("fov", ctypes.c_float), ("max_depth", ctypes.c_int32),
                 ("sampling_mode", ctypes.c_uint32), ("sampler", ctypes.c_uint32), ("z_temporal", ctypes.c_uint32),
-                ("sample_index0", ctypes.c_uint32)]
+                ("sample_index0", ctypes.c_uint32), ("indirect_reuse", ctypes.c_uint32)]
 
 
 _SPECTRAL = None
@@ -435,6 +435,7 @@ def _render_inputs(scene, materials, frame, light_tree):
                (ctypes.c_float * 3)(*frame["target"]), frame["fov"], frame["max_depth"], frame.get("sampling_mode", 1))
     fr.sampler = frame.get("sampler", 0)
     fr.z_temporal = frame.get("z_temporal", 1)
+    fr.indirect_reuse = frame.get("indirect_reuse", 1)
     return rs, fr, keep
 
 

@@ -232,7 +232,7 @@ static int occluded(const vo_render_scene *sc, v3 o, v3 d, real tmin, real tmax)
 typedef struct { uint32_t pcg, key, dim; int z; uint32_t sub, sub_bits; } rng_t;
 void vo_z_draw(uint32_t key, uint32_t *dimension, uint32_t sub, uint32_t sub_bits, int kind, float *out);
 uint32_t vo_z_pixel_key(uint32_t x, uint32_t y, uint32_t frame_index, uint32_t sample_index, uint32_t model);
-enum { Z_BSDF = 0, Z_NEE = 1, Z_ROULETTE = 2, Z_DI = 4, Z_WAVELENGTH = 12 };
+enum { Z_BSDF = 0, Z_NEE = 1, Z_ROULETTE = 2, Z_DI = 4, Z_GI_BSDF = 5, Z_GI_NEE = 6, Z_WAVELENGTH = 12 };
 static real rr(rng_t *r) {
     if (!r->z) return (real)vo_rand_f(&r->pcg);
     float f;
@@ -494,8 +494,16 @@ static v3 camera_dir(const cam_t *c, const vo_frame *fr, uint32_t gx, uint32_t g
  * emission_weight, NEE with MIS at every later vertex, roulette after bounce 3.  In ReSTIR mode
  * (samplingMode 0) the departure from a diffuse primary is not MIS-weighted: its DI reservoir
  * estimated the whole direct integral (main.swift:7725-7728). */
+static v3 continue_path_gi(const vo_render_scene *sc, const vo_frame *fr, const lanes_t *l, surf_t s, mat_t m, v3 dir,
+                           rng_t *seed, int gi_secondary);
 static v3 continue_path(const vo_render_scene *sc, const vo_frame *fr, const lanes_t *l, surf_t s, mat_t m, v3 dir,
                         rng_t *seed) {
+    return continue_path_gi(sc, fr, l, s, m, dir, seed, 0);
+}
+/* gi_secondary: ReSTIR GI holds this path's x1 -> x2 -> light suffix, so a diffuse secondary vertex
+ * takes no NEE (restirGISecondary, main.swift:7801-7803). */
+static v3 continue_path_gi(const vo_render_scene *sc, const vo_frame *fr, const lanes_t *l, surf_t s, mat_t m, v3 dir,
+                           rng_t *seed, int gi_secondary) {
     const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
     v3 radiance = V(0, 0, 0);
     s4 thr = S4(1);
@@ -553,7 +561,8 @@ static v3 continue_path(const vo_render_scene *sc, const vo_frame *fr, const lan
             break;
         }
         if (emission_only) break;
-        if (!is_delta(&rm) && fr->sampling_mode != 3u) {
+        const int gi_skip = gi_secondary && bounce == 1 && rm.type == 0u;
+        if (!is_delta(&rm) && fr->sampling_mode != 3u && !gi_skip) {
             rng_event(seed, (uint32_t)bounce + 1u, Z_NEE);
             radiance = add(radiance, nee(sc, l, &r, &rm, neg(cur_dir), thr, fr->sampling_mode <= 1u && sdepth < limit, seed));
         }
@@ -892,9 +901,108 @@ static int backproject(const cam_t *prev, const vo_frame *fr, v3 p, int *px, int
     return *px >= 0 && *px < (int)fr->width && *py >= 0 && *py < (int)fr->height;
 }
 
+/* ReSTIR GI (IndirectReuse.restirGI, main.swift:3200-3236, 4625, 4718-4812, 7586-7657): the reservoir
+ * of a diffuse primary hit x1 holds a secondary point x2 and its one-sample outgoing direct radiance
+ * toward x1.  Stored as the reference's textures: position + source pdf and radiance and weights in
+ * rgba32f, the normal (+ a valid flag) in rgba16f. */
+typedef struct {
+    float pos[4];      /* x2, source pdf (area measure) */
+    float nrm[4];      /* n2 rounded to half precision, 1 if a sample is held */
+    float rad[4];      /* secondary radiance at the sample's four wavelengths */
+    float weights[4];  /* weight sum, M, W, wavelength number */
+} gi_texel;
+typedef struct { v3 position, normal; s4 radiance; real source_pdf, wsum, M, u; } gi_res;
+
+/* gi_geometry (main.swift:3201-3207). */
+static real gi_geometry(v3 x1, v3 n1, v3 x2, v3 n2) {
+    v3 delta = sub(x2, x1);
+    real d2 = dot(delta, delta);
+    if (d2 < (real)1e-10) return 0;
+    v3 dir = mulv(delta, (real)1 / RSQRT(d2));
+    return RFMAX((real)0, dot(n1, dir)) * RFMAX((real)0, dot(n2, neg(dir))) / d2;
+}
+/* restir_gi_jacobian / restir_gi_accepts_shift (main.swift:3213-3226). */
+static int gi_accepts_shift(v3 x1r, v3 x1q, v3 x2, v3 n2) {
+    v3 cur = sub(x1r, x2), src = sub(x1q, x2);
+    real cd2 = dot(cur, cur), sd2 = dot(src, src);
+    if (cd2 < (real)1e-10 || sd2 < (real)1e-10) return 0;
+    real cc = RFMAX((real)0, dot(n2, mulv(cur, (real)1 / RSQRT(cd2))));
+    real sco = RFMAX((real)0, dot(n2, mulv(src, (real)1 / RSQRT(sd2))));
+    real j = cc * sd2 / (sco * cd2);
+    if (!isfinite(j)) j = 0;
+    return j >= (real)0.1 && j <= (real)10;
+}
+/* eval_restir_gi_target (main.swift:3230-3236) at a diffuse x1, the sample's wavelengths lanes(u). */
+static real gi_target(const vo_render_scene *sc, const surf_t *s, const mat_t *m, v3 raydir, v3 x2, v3 n2, s4 rad, real u) {
+    real g = gi_geometry(s->p, s->n, x2, n2);
+    if (g <= 0 || !(isfinite(rad.v[0]) && isfinite(rad.v[1]) && isfinite(rad.v[2]) && isfinite(rad.v[3]))) return 0;
+    v3 dir = norm(sub(x2, s->p));
+    lanes_t l = lanes_of(sc, u);
+    s4 f = diffuse_f(sc, &l, m, s, neg(raydir), dir);
+    return len(spectrum_rgb(sc, &l, sscale(smul(f, rad), g)));
+}
+/* gi_connection_visible (main.swift:3238-3251): x1 sees x2. */
+static int gi_visible(const vo_render_scene *sc, const surf_t *s, v3 x2) {
+    v3 delta = sub(x2, s->p);
+    real dist = len(delta);
+    if (dist <= (real)2 * ray_epsilon(s->p)) return 0;
+    v3 dir = mulv(delta, (real)1 / dist);
+    v3 o = ray_origin(s->p, s->ng, dir, s->error);
+    v3 ed = sub(x2, o);
+    real d = len(ed);
+    return !occluded(sc, o, mulv(ed, (real)1 / d), ray_t_min(o), d - endpoint_tolerance(o, x2, d));
+}
+static s4 gi_rad(const gi_texel *t) { s4 r = {{t->rad[0], t->rad[1], t->rad[2], t->rad[3]}}; return r; }
+
+/* restir_gi_initial (main.swift:4722-4776): x1 -> x2 (BSDF sample, traced) -> a light sample at x2. */
+static gi_res gi_initial(const vo_render_scene *sc, const vo_frame *fr, const surf_t *s, const mat_t *m, v3 view,
+                         real u, rng_t *seed) {
+    const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
+    gi_res r;
+    memset(&r, 0, sizeof r);
+    r.M = 1;
+    r.u = u;
+    lanes_t l = lanes_of(sc, u);
+    v3 gdir;
+    s4 bw;
+    real bpdf;
+    rng_event(seed, 1u, Z_GI_BSDF);
+    if (!sample_bsdf(sc, &l, s, m, view, seed, &gdir, &bw, &bpdf) || !(bpdf > 0)) return r;
+    v3 o = ray_origin(s->p, s->ng, gdir, s->error);
+    surf_t s2;
+    uint32_t face;
+    if (!trace(sc, o, gdir, ray_t_min(o), &s2, &face)) return r;
+    mat_t m2 = mat_of(sc, s2.slot);
+    if (m2.type != 0u) return r;
+    s4 rad = S4(0);
+    rng_event(seed, 2u, Z_GI_NEE);
+    light_sample ls = sample_direct_light(sc, s2.p, seed);
+    if (ls.pdf > 0 && light_visible(sc, &s2, ls.wi, ls.position)) {
+        real c2 = RFMAX((real)0, dot(s2.n, ls.wi));
+        real pdf2;
+        s4 f2 = eval_bsdf_pdf(sc, &l, &s2, &m2, neg(gdir), ls.wi, &pdf2);
+        real mis = limit >= 2 ? power_heuristic(ls.pdf, pdf2) : 1;
+        rad = sscale(smul(f2, emission_spectrum(sc, &l, ls.emission)), c2 * (mis / ls.pdf));
+    }
+    v3 gd = sub(s2.p, s->p);
+    real d2 = dot(gd, gd);
+    real cs = RFMAX((real)0, dot(s2.n, neg(gdir)));
+    real src = d2 > (real)1e-10 ? bpdf * cs / d2 : 0;
+    if (vo_get_fault() == 10) src = bpdf;  /* planted: the source pdf left in solid-angle measure */
+    real ph = gi_target(sc, s, m, view, s2.p, s2.n, rad, u);
+    if (src > 0 && ph > 0) {
+        r.position = s2.p;
+        r.normal = s2.n;
+        r.radiance = rad;
+        r.source_pdf = src;
+        r.wsum = ph / src;
+    }
+    return r;
+}
+
 static void restir_pass1(const vo_render_scene *sc, const vo_frame *fr, const cam_t *cam, const cam_t *prev,
                          int temporal, uint32_t candidates, uint32_t gx, uint32_t gy, uint32_t sample_index,
-                         const di_texel *hist, di_texel *cur, primary_t *prim) {
+                         const di_texel *hist, di_texel *cur, primary_t *prim, const gi_texel *ghist, gi_texel *gcur) {
     const uint32_t pix = gy * fr->width + gx;
     rng_t seed = {pixel_seed(fr, gx, gy, sample_index), 0u, 0u, fr->sampler == 1u, 0u, 0u};
     real u = pixel_u(seed.pcg);
@@ -911,6 +1019,8 @@ static void restir_pass1(const vo_render_scene *sc, const vo_frame *fr, const ca
     const v3 dir = camera_dir(cam, fr, gx, gy, sample_index);
     di_texel *o = cur + pix;
     memset(o, 0, sizeof *o);
+    gi_texel *g = gcur + pix;
+    memset(g, 0, sizeof *g);
     primary_t *pr = prim + pix;
     uint32_t face;
     pr->dir = dir;
@@ -945,7 +1055,8 @@ static void restir_pass1(const vo_render_scene *sc, const vo_frame *fr, const ca
         }
     }
     int px, py;
-    if (temporal && backproject(prev, fr, s->p, &px, &py)) {
+    const int back = temporal && backproject(prev, fr, s->p, &px, &py);
+    if (back) {
         const di_texel *h = hist + (size_t)py * fr->width + px;
         real hM = (real)h->weights[1], hW = (real)h->weights[2];
         v3 hp = V(h->gpos[0], h->gpos[1], h->gpos[2]);
@@ -969,10 +1080,51 @@ static void restir_pass1(const vo_render_scene *sc, const vo_frame *fr, const ca
     o->emit_pdf[0] = (float)sel.emission.x; o->emit_pdf[1] = (float)sel.emission.y;
     o->emit_pdf[2] = (float)sel.emission.z; o->emit_pdf[3] = (float)sel.pdf;
     o->weights[0] = (float)wsum; o->weights[1] = (float)M; o->weights[2] = (float)W; o->weights[3] = (float)sel.u;
+    /* ReSTIR GI (main.swift:4975-5005): depth 1 has no indirect bounce. */
+    const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
+    if (fr->indirect_reuse != 0u || limit < 1) return;
+    gi_res gr = gi_initial(sc, fr, s, &m, dir, u, &seed);
+    if (back) {
+        const di_texel *h = hist + (size_t)py * fr->width + px;
+        const gi_texel *gh = ghist + (size_t)py * fr->width + px;
+        v3 hp = V(h->gpos[0], h->gpos[1], h->gpos[2]);
+        v3 hn = V(h->gnorm[0], h->gnorm[1], h->gnorm[2]);
+        int same = h->gpos[3] > 0.0f && h->gnorm[3] == (float)m.type && dot(hn, s->n) > (real)0.95 &&
+                   len(sub(hp, s->p)) < RFMAX((real)0.01, s->t * (real)0.01);
+        if (same && gh->weights[1] > 0.0f && gh->weights[2] > 0.0f && gh->pos[3] > 0.0f && gh->nrm[3] > 0.0f) {
+            /* restir_gi_merge (main.swift:4784-4801) */
+            v3 op = ld(gh->pos), on = ld(gh->nrm);
+            if (gi_accepts_shift(s->p, hp, op, on)) {
+                real ph = gi_target(sc, s, &m, dir, op, on, gi_rad(gh), (real)gh->weights[3]);
+                real cm = RFMIN((real)gh->weights[1], (real)20);
+                real w = ph * (real)gh->weights[2] * cm;
+                gr.M += cm;
+                gr.wsum += w;
+                if ((real)vo_rand_f(&seed.pcg) * gr.wsum < w) {
+                    gr.position = op;
+                    gr.normal = on;
+                    gr.radiance = gi_rad(gh);
+                    gr.source_pdf = (real)gh->pos[3];
+                    gr.u = (real)gh->weights[3];
+                }
+            }
+        }
+    }
+    /* restir_gi_encode (main.swift:4803-4811) */
+    real gt = gi_target(sc, s, &m, dir, gr.position, gr.normal, gr.radiance, gr.u);
+    real gW = gr.M > 0 && gt > 0 ? gr.wsum / (gr.M * gt) : 0;
+    g->pos[0] = (float)gr.position.x; g->pos[1] = (float)gr.position.y; g->pos[2] = (float)gr.position.z;
+    g->pos[3] = (float)gr.source_pdf;
+    g->nrm[0] = half_round((float)gr.normal.x);
+    g->nrm[1] = half_round((float)gr.normal.y);
+    g->nrm[2] = half_round((float)gr.normal.z);
+    g->nrm[3] = gr.source_pdf > 0 ? 1.0f : 0.0f;
+    for (int k = 0; k < 4; ++k) g->rad[k] = (float)gr.radiance.v[k];
+    g->weights[0] = (float)gr.wsum; g->weights[1] = (float)gr.M; g->weights[2] = (float)gW; g->weights[3] = (float)gr.u;
 }
 
 static v3 restir_pass2(const vo_render_scene *sc, const vo_frame *fr, uint32_t gx, uint32_t gy, uint32_t sample_index,
-                       const di_texel *cur, const primary_t *prim) {
+                       const di_texel *cur, const primary_t *prim, const gi_texel *gcur) {
     const int limit = fr->max_depth > 1 ? fr->max_depth - 1 : 0;
     const uint32_t pix = gy * fr->width + gx;
     rng_t seed = {pixel_seed(fr, gx, gy, sample_index), 0u, 0u, fr->sampler == 1u, 0u, 0u};
@@ -1024,11 +1176,60 @@ static v3 restir_pass2(const vo_render_scene *sc, const vo_frame *fr, uint32_t g
                 radiance = add(radiance, spectrum_rgb(sc, &sl, v));
             }
         }
+        /* Spatial ReSTIR GI reuse, uniform taps (main.swift:7586-7657). */
+        if (fr->indirect_reuse == 0u && limit >= 1) {
+            const gi_texel *gc = gcur + pix;
+            v3 spos = ld(gc->pos), snrm = ld(gc->nrm);
+            s4 srad = gi_rad(gc);
+            real su = (real)gc->weights[3];
+            int sflag = gc->nrm[3] > 0.0f;
+            real gt = sflag ? gi_target(sc, &s, &m, dir, spos, snrm, srad, su) : 0;
+            real gws = gt * (real)gc->weights[2] * (real)gc->weights[1];
+            real gM = (real)gc->weights[1];
+            /* Test hook 9 turns the GI taps off (the estimator of initial + temporal reuse alone). */
+            const int taps = vo_get_fault() == 9 ? 0 : 4;
+            for (int i = 0; i < taps; ++i) {
+                real rx = (real)vo_rand_f(&seed.pcg), ry = (real)vo_rand_f(&seed.pcg);
+                int nx = (int)gx + (int)((rx * (real)2 - (real)1) * (real)16);
+                int ny = (int)gy + (int)((ry * (real)2 - (real)1) * (real)16);
+                if (nx < 0 || nx >= (int)fr->width || ny < 0 || ny >= (int)fr->height) continue;
+                const di_texel *n = cur + (size_t)ny * fr->width + nx;
+                const gi_texel *gn = gcur + (size_t)ny * fr->width + nx;
+                v3 nn = V(n->gnorm[0], n->gnorm[1], n->gnorm[2]);
+                int compatible = n->gpos[3] > 0.0f && n->gnorm[3] == (float)m.type && dot(s.n, nn) > (real)0.95 &&
+                                 RFABS((real)c->gpos[3] - (real)n->gpos[3]) < (real)0.05 * (real)c->gpos[3];
+                if (!compatible) continue;
+                if (gn->weights[1] <= 0.0f || gn->weights[2] <= 0.0f || gn->pos[3] <= 0.0f || gn->nrm[3] <= 0.0f) continue;
+                v3 np = ld(gn->pos), nnrm = ld(gn->nrm);
+                if (!gi_accepts_shift(s.p, V(n->gpos[0], n->gpos[1], n->gpos[2]), np, nnrm)) continue;
+                real t = gi_target(sc, &s, &m, dir, np, nnrm, gi_rad(gn), (real)gn->weights[3]);
+                real w = t * (real)gn->weights[2] * (real)gn->weights[1];
+                gws += w;
+                gM += (real)gn->weights[1];
+                if ((real)vo_rand_f(&seed.pcg) * gws < w) {
+                    spos = np;
+                    snrm = nnrm;
+                    srad = gi_rad(gn);
+                    su = (real)gn->weights[3];
+                    sflag = 1;
+                }
+            }
+            real ft = sflag ? gi_target(sc, &s, &m, dir, spos, snrm, srad, su) : 0;
+            real gW = gM > 0 && ft > 0 ? gws / (gM * ft) : 0;
+            if (gW > 0 && gi_visible(sc, &s, spos)) {
+                v3 wi = norm(sub(spos, s.p));
+                lanes_t sl = lanes_of(sc, su);
+                s4 f = diffuse_f(sc, &sl, &m, &s, neg(dir), wi);
+                real geo = gi_geometry(s.p, s.n, spos, snrm);
+                radiance = add(radiance, spectrum_rgb(sc, &sl, sscale(smul(f, srad), geo * gW)));
+            }
+        }
     } else if (!is_delta(&m)) {
         rng_event(&seed, 1u, Z_NEE);
         radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), limit > 0, &seed);
     }
-    radiance = add(radiance, continue_path(sc, fr, &l, s, m, dir, &seed));
+    radiance = add(radiance, continue_path_gi(sc, fr, &l, s, m, dir, &seed,
+                                              m.type == 0u && fr->indirect_reuse == 0u && limit >= 1));
     if (!(isfinite(radiance.x) && isfinite(radiance.y) && isfinite(radiance.z))) radiance = V(0, 0, 0);
     return radiance;
 }
@@ -1037,10 +1238,11 @@ void SUFFIX(vo_render_restir)(const vo_render_scene *sc, const vo_frame *fr, uin
                               uint32_t candidates, const float *eyes, int threads, double *out_rgb) {
     const size_t npix = (size_t)fr->width * fr->height;
     di_texel *buf[2] = {calloc(npix, sizeof(di_texel)), calloc(npix, sizeof(di_texel))};
+    gi_texel *gbuf[2] = {calloc(npix, sizeof(gi_texel)), calloc(npix, sizeof(gi_texel))};
     primary_t *prim = calloc(npix, sizeof(primary_t));
     double *acc = calloc(3 * npix, sizeof(double));
-    if (!buf[0] || !buf[1] || !prim || !acc) {
-        free(buf[0]); free(buf[1]); free(prim); free(acc);
+    if (!buf[0] || !buf[1] || !gbuf[0] || !gbuf[1] || !prim || !acc) {
+        free(buf[0]); free(buf[1]); free(gbuf[0]); free(gbuf[1]); free(prim); free(acc);
         return;
     }
     const int nt = threads > 0 ? threads : omp_get_max_threads();
@@ -1054,13 +1256,15 @@ void SUFFIX(vo_render_restir)(const vo_render_scene *sc, const vo_frame *fr, uin
         const cam_t cam = camera_of(fr, eyes ? eyes + 3 * (size_t)k : fr->eye);
         di_texel *cur = buf[k & 1u];
         const di_texel *hist = buf[(k + 1u) & 1u];
+        gi_texel *gcur = gbuf[k & 1u];
+        const gi_texel *ghist = gbuf[(k + 1u) & 1u];
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
         for (int64_t p = 0; p < (int64_t)npix; ++p)
             restir_pass1(sc, fr, &cam, &prev, k > 0, candidates, (uint32_t)(p % fr->width), (uint32_t)(p / fr->width),
-                         si, hist, cur, prim);
+                         si, hist, cur, prim, ghist, gcur);
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
         for (int64_t p = 0; p < (int64_t)npix; ++p) {
-            v3 c = restir_pass2(sc, fr, (uint32_t)(p % fr->width), (uint32_t)(p / fr->width), si, cur, prim);
+            v3 c = restir_pass2(sc, fr, (uint32_t)(p % fr->width), (uint32_t)(p / fr->width), si, cur, prim, gcur);
             acc[3 * p] += (double)c.x;
             acc[3 * p + 1] += (double)c.y;
             acc[3 * p + 2] += (double)c.z;
@@ -1068,5 +1272,5 @@ void SUFFIX(vo_render_restir)(const vo_render_scene *sc, const vo_frame *fr, uin
         prev = cam;
     }
     for (size_t q = 0; q < 3 * npix; ++q) out_rgb[q] = samples ? acc[q] / (double)samples : 0.0;
-    free(buf[0]); free(buf[1]); free(prim); free(acc);
+    free(buf[0]); free(buf[1]); free(gbuf[0]); free(gbuf[1]); free(prim); free(acc);
 }

@@ -158,6 +158,8 @@ typedef struct vo_frame {
     uint32_t sampler;        /* 0: the PCG stream; 1: the Z++ sampler (MIS path loop only) */
     uint32_t z_temporal;     /* Z++ temporal model 0..3 (main.swift:497-506) */
     uint32_t sample_index0;  /* first sample of the accumulation (Z mode: frame = sample - this + 1) */
+    uint32_t indirect_reuse; /* sampling mode 0: 0 = ReSTIR GI at the first indirect diffuse vertex (the
+                                reference's IndirectReuse.restirGI), 1 = none (plain MIS suffix) */
 } vo_frame;
 
 /* Mean linear-sRGB radiance of `samples` samples (sample indices sample_index0...) for each listed
@@ -215,7 +217,8 @@ void vo_render_nrc_f64(const vo_render_scene *sc, const vo_frame *fr, uint32_t s
 /* For tests that plant a mistake in the oracle (never the product): 0 = faithful,
  * 1 = drop the FMA tie-break of watertight_edge, 2 = swap b1/b2, 3 = ReSTIR spatial reuse leaves
  * the neighbours' confidence out of M, 4 = ReSTIR RIS counts only candidates with pdf > 0 in M,
- * 5 = the GGX reflection lobe drops the shadowing term G1(wi). */
+ * 5 = the GGX reflection lobe drops the shadowing term G1(wi), 10 = ReSTIR GI keeps its source pdf in
+ * solid-angle measure.  9 is a test switch, not a mistake: ReSTIR GI without its spatial taps. */
 void vo_set_fault(int fault);
 int vo_get_fault(void);
 /* Nearest half-precision value of x (ties to even): what an rgba16f texel store keeps. */
