@@ -267,6 +267,14 @@ typedef struct vt_frame_params {
                                     decisions stay on the PCG stream) */
     uint32_t z_temporal;         /* Z++ temporal model 0..3 of the pixel key (main.swift:497-506; the
                                     reference's default is 1) */
+    uint32_t indirect_reuse;     /* mode 0, Uniforms.indirectReuse: 0 = ReSTIR GI at the first indirect
+                                    diffuse vertex (IndirectReuse.restirGI, main.swift:8419-8434:
+                                    initial x1 -> x2 -> light sample, temporal and spatial reuse by
+                                    reconnection; the diffuse secondary vertex then takes no NEE);
+                                    1 = no indirect reuse, the MIS path suffix of mode 1 (the
+                                    reference's values 1 and 2, ReSTIR PT, are not built here).  GI
+                                    is off with depth 1 and with the radiance cache, which answers
+                                    that vertex's light itself */
 } vt_frame_params;
 
 /* Replaces PathTracerRenderer (main.swift:9514) for this path: owns the accumulation buffer. */

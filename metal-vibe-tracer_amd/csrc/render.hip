@@ -44,7 +44,7 @@ struct Rng {
     ZSampler zs;
     bool z;
 };
-enum : uint32_t { kZBsdf = 0u, kZNee = 1u, kZRoulette = 2u, kZDi = 4u, kZWavelength = 12u };
+enum : uint32_t { kZBsdf = 0u, kZNee = 1u, kZRoulette = 2u, kZDi = 4u, kZGiBsdf = 5u, kZGiNee = 6u, kZWavelength = 12u };
 __device__ __forceinline__ float rand_f(Rng &r) { return r.z ? r.zs.rand1() : rand_f(r.pcg); }
 __device__ __forceinline__ void rand_f3(uint32_t &seed, float &a, float &b, float &c) {
     a = rand_f(seed);
@@ -791,6 +791,38 @@ __device__ __forceinline__ float target_pdf_at(const RenderScene &sc, const Spec
     return target_pdf(sp, l, scale(reflectance(sp, l, m.lagrange), 1.0f / kPi), s, raydir, d, light_geometry(sc, s.p, d));
 }
 
+// ---------------------------------------------------------------- ReSTIR GI (main.swift:3200-3251)
+// gi_geometry: cosines at both ends over the squared distance.
+__device__ __forceinline__ float gi_geometry(V3 x1, V3 n1, V3 x2, V3 n2) {
+    const V3 delta = x2 - x1;
+    const float d2 = dot(delta, delta);
+    if (d2 < 1e-10f) return 0.0f;
+    const V3 dir = delta * (1.0f / sqrtf(d2));
+    return fmaxf(0.0f, dot(n1, dir)) * fmaxf(0.0f, dot(n2, -dir)) / d2;
+}
+// restir_gi_jacobian / restir_gi_accepts_shift: the reconnection's solid-angle Jacobian in [0.1, 10].
+__device__ __forceinline__ bool gi_accepts_shift(V3 x1r, V3 x1q, V3 x2, V3 n2) {
+    const V3 cur = x1r - x2, src = x1q - x2;
+    const float cd2 = dot(cur, cur), sd2 = dot(src, src);
+    if (cd2 < 1e-10f || sd2 < 1e-10f) return false;
+    const float cc = fmaxf(0.0f, dot(n2, cur * (1.0f / sqrtf(cd2))));
+    const float sco = fmaxf(0.0f, dot(n2, src * (1.0f / sqrtf(sd2))));
+    float j = cc * sd2 / (sco * cd2);
+    if (!isfinite(j)) j = 0.0f;
+    return j >= 0.1f && j <= 10.0f;
+}
+// eval_restir_gi_target at a diffuse x1: |rgb(f rad G)| at the sample's wavelengths (number u).
+__device__ __forceinline__ float gi_target(const Spectral &sp, const Mat &m, const Surface &s, V3 raydir, V3 x2, V3 n2,
+                                           S4 rad, float u) {
+    const float g = gi_geometry(s.p, s.n, x2, n2);
+    if (g <= 0.0f || !(isfinite(rad.v[0]) && isfinite(rad.v[1]) && isfinite(rad.v[2]) && isfinite(rad.v[3]))) return 0.0f;
+    const V3 dir = normalize(x2 - s.p);
+    const Lanes l = lanes_of(sp, u);
+    const S4 f = diffuse_f(s, scale(reflectance(sp, l, m.lagrange), 1.0f / kPi), -raydir, dir);
+    return length(spectrum_rgb(sp, l, scale(mul(f, rad), g)));
+}
+__device__ __forceinline__ S4 s4_of(float4 v) { return S4{{v.x, v.y, v.z, v.w}}; }
+
 // ---------------------------------------------------------------- wave-aggregated queue append
 __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
     const unsigned long long mask = __ballot(pred);
@@ -811,6 +843,8 @@ constexpr uint32_t kPrevDelta = 1u << 16, kPrevNee = 1u << 17, kPrevMis = 1u << 
 constexpr uint32_t kEmissionOnly = 1u << 19;
 // Training paths of the radiance cache: the path has recorded its cache vertex.
 constexpr uint32_t kRecorded = 1u << 20;
+// ReSTIR GI: the path left a diffuse primary whose GI reservoir holds the light of its secondary vertex.
+constexpr uint32_t kGiSecondary = 1u << 21;
 
 // ---------------------------------------------------------------- learned radiance cache
 // Training paths: 1 in f.train_period, by a hash of pixel and sample index.
@@ -918,6 +952,7 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
     float4 gpos = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     uint2 gnorm = pack_half4(0.0f, 0.0f, 0.0f, -1.0f);
     float4 posdir = make_float4(0.0f, 0.0f, 0.0f, 0.0f), emitpdf = posdir, weights = posdir;
+    bool gi_ray = false;
     const int face = hits.face[pix];
     if (face >= 0) {
         const V3 dir = v3(q.dx[pix], q.dy[pix], q.dz[pix]);
@@ -999,6 +1034,35 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
             posdir = make_float4(sel.position.x, sel.position.y, sel.position.z, pack_light(sel.light));
             emitpdf = make_float4(sel.emission.x, sel.emission.y, sel.emission.z, sel.pdf);
             weights = make_float4(wsum, M, W, sel.u);
+            if (ra.gi && f.scattering_limit >= 1) {
+                // ReSTIR GI's x1 -> x2 direction (restir_gi_initial, main.swift:4733-4743): a BSDF sample
+                // at x1 after the DI draws, traced by the next stage.
+                rng_event(rng, 1u, kZGiBsdf);
+                V3 gd;
+                S4 gw;
+                float gpdf;
+                if (sample_bsdf<false>(sc, sp, lanes, s, m, dir, rng, gd, gw, gpdf) && gpdf > 0.0f) {
+                    const V3 o = ray_origin(s.p, s.ng, gd, s.error);
+                    gi_ray = true;
+                    ra.gw.ox[pix] = o.x; ra.gw.oy[pix] = o.y; ra.gw.oz[pix] = o.z;
+                    ra.gw.dx[pix] = gd.x; ra.gw.dy[pix] = gd.y; ra.gw.dz[pix] = gd.z;
+                    ra.gw.tmin[pix] = ray_t_min(o);
+                    ra.gw.tmax[pix] = 1e20f;
+                    ra.gw.bpdf[pix] = gpdf;
+                }
+                ra.gw.seed[pix] = rng.pcg;
+            }
+        }
+    }
+    if (ra.gi) {
+        ra.gw.stage[pix] = gi_ray ? 1u : 0u;
+        if (!gi_ray) {
+            // No x2 to find: a ray that ends before it starts.
+            ra.gw.ox[pix] = ra.gw.oy[pix] = ra.gw.oz[pix] = 0.0f;
+            ra.gw.dx[pix] = ra.gw.dy[pix] = 0.0f;
+            ra.gw.dz[pix] = 1.0f;
+            ra.gw.tmin[pix] = 1.0f;
+            ra.gw.tmax[pix] = 0.0f;
         }
     }
     ra.cur.gpos[pix] = gpos;
@@ -1006,6 +1070,175 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
     ra.cur.posdir[pix] = posdir;
     ra.cur.emitpdf[pix] = emitpdf;
     ra.cur.weights[pix] = weights;
+}
+
+// ReSTIR GI stage 2 (restir_gi_initial at x2, main.swift:4744-4756): at a diffuse secondary hit, the
+// light sample of the vertex-2 Z_GI_NEE event and its shadow ray (light_visible).  Stage 2 marks a
+// sample whose visibility the shadow ray decides, 3 one with no pdf or no room between the offset
+// origin and the light (not visible).
+__global__ __launch_bounds__(256) void restir_gi_secondary_kernel(RenderScene sc, FrameArgs f, RestirArgs ra) {
+    const uint32_t pix = blockIdx.x * 256 + threadIdx.x;
+    if (pix >= f.npix) return;
+    const GiWork &g = ra.gw;
+    bool ray = false;
+    V3 so = v3(0.0f, 0.0f, 0.0f), sd = v3(0.0f, 0.0f, 1.0f);
+    float stmin = 1.0f, stmax = 0.0f;
+    if (g.stage[pix] == 1u && g.face[pix] >= 0) {
+        const V3 gd = v3(g.dx[pix], g.dy[pix], g.dz[pix]);
+        const Surface x2 = surface_at(sc, (uint32_t)g.face[pix], g.b1[pix], g.b2[pix], gd);
+        const Mat m2 = material(sc, x2.slot);
+        if (m2.type == kDiffuse) {
+            Rng rng{g.seed[pix], ZSampler{0u, 0u, 0u, 0u}, f.sampler == 1u};
+            if (rng.z) rng.zs.key = z_pixel_key(pix % f.width, pix / f.width, f.frame0 + 1u, f.sample_index0, f.z_temporal);
+            rng_event(rng, 2u, kZGiNee);
+            const LightSample ls = sample_emitter(sc, x2.p, rng);
+            g.seed[pix] = rng.pcg;
+            g.light_wi[pix] = make_float4(ls.wi.x, ls.wi.y, ls.wi.z, ls.pdf);
+            g.light_le[pix] = make_float4(ls.emission.x, ls.emission.y, ls.emission.z, 0.0f);
+            uint32_t stage = 3u;
+            if (ls.pdf > 0.0f) {
+                const V3 o = ray_origin(x2.p, x2.ng, ls.wi, x2.error);
+                const V3 delta = ls.position - o;
+                const float d = length(delta);
+                const float tol = endpoint_tolerance(o, ls.position, d);
+                if (d > tol) {
+                    so = o;
+                    sd = normalize(delta);
+                    stmin = ray_t_min(o);
+                    stmax = d - tol;
+                    ray = true;
+                    stage = 2u;
+                }
+            }
+            g.stage[pix] = stage;
+        }
+    }
+    g.sox[pix] = so.x; g.soy[pix] = so.y; g.soz[pix] = so.z;
+    g.sdx[pix] = sd.x; g.sdy[pix] = sd.y; g.sdz[pix] = sd.z;
+    g.stmin[pix] = ray ? stmin : 1.0f;
+    g.stmax[pix] = ray ? stmax : 0.0f;
+}
+
+// ReSTIR GI stage 3: the initial reservoir (restir_gi_initial :4757-4774), the temporal merge of the
+// backprojected history (restir_gi_merge :4784-4801) on the PCG stream continued from the stages
+// before, and the encode (restir_gi_encode :4803-4811) into ra.gcur.
+__global__ __launch_bounds__(256) void restir_gi_finish_kernel(RenderScene sc, FrameArgs f, PathQueue q, HitArgs hits,
+                                                               RestirArgs ra) {
+    const uint32_t pix = blockIdx.x * 256 + threadIdx.x;
+    if (pix >= f.npix) return;
+    const GiWork &g = ra.gw;
+    float4 opos = make_float4(0.0f, 0.0f, 0.0f, 0.0f), orad = opos, ow = opos;
+    uint2 onrm = pack_half4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int face = hits.face[pix];
+    if (face >= 0) {
+        const V3 dir = v3(q.dx[pix], q.dy[pix], q.dz[pix]);
+        const Surface s = surface_at(sc, (uint32_t)face, hits.b1[pix], hits.b2[pix], dir);
+        const Mat m = material(sc, s.slot);
+        if (m.type == kDiffuse) {
+            const Spectral sp{sc.icdf, reinterpret_cast<const float4 *>(sc.cmf), reinterpret_cast<const float4 *>(sc.basis)};
+            const float u = q.wl[pix];
+            V3 rpos = v3(0.0f, 0.0f, 0.0f), rnrm = rpos;
+            S4 rrad = s4(0.0f);
+            float rsrc = 0.0f, wsum = 0.0f, M = 1.0f, ru = u;
+            if (g.stage[pix] == 2u && !g.occluded[pix]) {
+                const V3 gd = v3(g.dx[pix], g.dy[pix], g.dz[pix]);
+                const Surface x2 = surface_at(sc, (uint32_t)g.face[pix], g.b1[pix], g.b2[pix], gd);
+                const Mat m2 = material(sc, x2.slot);
+                const Lanes lanes = lanes_of(sp, u);
+                const float4 lw = g.light_wi[pix], le = g.light_le[pix];
+                const V3 wi = v3(lw.x, lw.y, lw.z);
+                const float c2 = fmaxf(0.0f, dot(x2.n, wi));
+                float pdf2;
+                const S4 f2 = eval_bsdf_pdf<false>(sc, sp, lanes, x2, m2, -gd, wi, pdf2);
+                const float mis = f.scattering_limit >= 2 ? power_heuristic(lw.w, pdf2) : 1.0f;
+                const S4 rad = scale(mul(f2, emission_spectrum(sp, lanes, v3(le.x, le.y, le.z))), c2 * (mis / lw.w));
+                const V3 dd = x2.p - s.p;
+                const float d2 = dot(dd, dd);
+                const float cs = fmaxf(0.0f, dot(x2.n, -gd));
+                const float src = d2 > 1e-10f ? g.bpdf[pix] * cs / d2 : 0.0f;
+                const float ph = gi_target(sp, m, s, dir, x2.p, x2.n, rad, u);
+                if (src > 0.0f && ph > 0.0f) {
+                    rpos = x2.p;
+                    rnrm = x2.n;
+                    rrad = rad;
+                    rsrc = src;
+                    wsum = ph / src;
+                }
+            }
+            if (ra.gi_temporal) {
+                // The history pixel and the same-surface test of the DI merge (restir_backproject,
+                // restir_same_surface), then the reconnection's Jacobian test.
+                uint32_t seed = g.seed[pix];
+                const V3 d = s.p - load_v3(ra.prev_eye);
+                const float w = dot(d, load_v3(ra.prev_forward));
+                if (w > 0.0f) {
+                    const float x = dot(d, load_v3(ra.prev_right)) / (w * ra.prev_aspect * ra.prev_fov_scale);
+                    const float y = dot(d, load_v3(ra.prev_up)) / (w * ra.prev_fov_scale);
+                    const float ux = x * 0.5f + 0.5f, uy = y * -0.5f + 0.5f;
+                    if (ux >= 0.0f && uy >= 0.0f && ux < 1.0f && uy < 1.0f) {
+                        const int px = (int)(ux * (float)f.width), py = (int)(uy * (float)f.height);
+                        if (px >= 0 && px < (int)f.width && py >= 0 && py < (int)f.height) {
+                            const uint32_t hp = (uint32_t)py * f.width + (uint32_t)px;
+                            const float4 og = ra.hist.gpos[hp];
+                            const float4 on = unpack_half4(ra.hist.gnorm[hp]);
+                            const bool same = og.w > 0.0f && on.w == (float)m.type &&
+                                              dot(v3(on.x, on.y, on.z), s.n) > 0.95f &&
+                                              length(v3(og.x, og.y, og.z) - s.p) < fmaxf(0.01f, hits.t[pix] * 0.01f);
+                            const float4 hw = ra.ghist.weights[hp], hpos = ra.ghist.pos[hp];
+                            const float4 hn = unpack_half4(ra.ghist.nrm[hp]);
+                            if (same && hw.y > 0.0f && hw.z > 0.0f && hpos.w > 0.0f && hn.w > 0.0f) {
+                                const V3 op = v3(hpos.x, hpos.y, hpos.z), onr = v3(hn.x, hn.y, hn.z);
+                                if (gi_accepts_shift(s.p, v3(og.x, og.y, og.z), op, onr)) {
+                                    const S4 hrad = s4_of(ra.ghist.rad[hp]);
+                                    const float ph = gi_target(sp, m, s, dir, op, onr, hrad, hw.w);
+                                    const float cm = fminf(hw.y, 20.0f);
+                                    const float tw = ph * hw.z * cm;
+                                    M += cm;
+                                    wsum += tw;
+                                    if (rand_f(seed) * wsum < tw) {
+                                        rpos = op;
+                                        rnrm = onr;
+                                        rrad = hrad;
+                                        rsrc = hpos.w;
+                                        ru = hw.w;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            const float gt = gi_target(sp, m, s, dir, rpos, rnrm, rrad, ru);
+            const float W = M > 0.0f && gt > 0.0f ? wsum / (M * gt) : 0.0f;
+            opos = make_float4(rpos.x, rpos.y, rpos.z, rsrc);
+            onrm = pack_half4(rnrm.x, rnrm.y, rnrm.z, rsrc > 0.0f ? 1.0f : 0.0f);
+            orad = make_float4(rrad.v[0], rrad.v[1], rrad.v[2], rrad.v[3]);
+            ow = make_float4(wsum, M, W, ru);
+        }
+    }
+    ra.gcur.pos[pix] = opos;
+    ra.gcur.nrm[pix] = onrm;
+    ra.gcur.rad[pix] = orad;
+    ra.gcur.weights[pix] = ow;
+    // The visibility arrays now hold pass 2's connection rays: none until the shading writes one.
+    g.sox[pix] = g.soy[pix] = g.soz[pix] = 0.0f;
+    g.sdx[pix] = g.sdy[pix] = 0.0f;
+    g.sdz[pix] = 1.0f;
+    g.stmin[pix] = 1.0f;
+    g.stmax[pix] = 0.0f;
+    g.rgb[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// The GI connections of pass 2 that x1 sees add their contribution (gi_connection_visible,
+// main.swift:7640-7657).  Runs after the DI shadow resolve, so each pixel has one writer at a time.
+__global__ __launch_bounds__(256) void restir_gi_resolve_kernel(FrameArgs f, RestirArgs ra, float *radiance) {
+    const uint32_t pix = blockIdx.x * 256 + threadIdx.x;
+    if (pix >= f.npix) return;
+    if (ra.gw.occluded[pix]) return;  // pixels without a connection hold a zero contribution
+    const float4 v = ra.gw.rgb[pix];
+    radiance[3 * (size_t)pix] += v.x;
+    radiance[3 * (size_t)pix + 1] += v.y;
+    radiance[3 * (size_t)pix + 2] += v.z;
 }
 
 // Shading of one wavefront vertex per path.  kRestir: sampling mode 0, where a diffuse primary vertex
@@ -1138,7 +1371,73 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
                     shadow = true;
                 }
             }
-        } else if (alive && !is_delta(m) && f.sampling_mode != 3u) {
+            if (ra.gi) {
+                // ReSTIR GI spatial reuse (main.swift:7586-7657) on the same PCG stream: the pixel's GI
+                // reservoir and four uniform taps, each tap's x2 reconnected to this x1 when the
+                // neighbour's surface is compatible and the shift's Jacobian is in range.
+                const float4 g0 = ra.gcur.pos[pix], w0 = ra.gcur.weights[pix];
+                const float4 n0 = unpack_half4(ra.gcur.nrm[pix]);
+                V3 spos = v3(g0.x, g0.y, g0.z), snrm = v3(n0.x, n0.y, n0.z);
+                S4 srad = s4_of(ra.gcur.rad[pix]);
+                float su = w0.w;
+                bool sflag = n0.w > 0.0f;
+                float gws = (sflag ? gi_target(sp, m, s, dir, spos, snrm, srad, su) : 0.0f) * w0.z * w0.y;
+                float gM = w0.y;
+                for (int k = 0; k < 4; ++k) {
+                    const float rx = rand_f(rng.pcg), ry = rand_f(rng.pcg);
+                    const int nx = (int)gx + (int)((rx * 2.0f - 1.0f) * 16.0f);
+                    const int ny = (int)gy + (int)((ry * 2.0f - 1.0f) * 16.0f);
+                    if (nx < 0 || nx >= (int)f.width || ny < 0 || ny >= (int)f.height) continue;
+                    const uint32_t np = (uint32_t)ny * f.width + (uint32_t)nx;
+                    const float4 ng = ra.cur.gpos[np];
+                    const float4 nn = unpack_half4(ra.cur.gnorm[np]);
+                    const bool compatible = ng.w > 0.0f && nn.w == (float)m.type && dot(s.n, v3(nn.x, nn.y, nn.z)) > 0.95f &&
+                                            fabsf(cg.w - ng.w) < 0.05f * cg.w;
+                    if (!compatible) continue;
+                    const float4 qw = ra.gcur.weights[np], qp = ra.gcur.pos[np];
+                    const float4 qn = unpack_half4(ra.gcur.nrm[np]);
+                    if (qw.y <= 0.0f || qw.z <= 0.0f || qp.w <= 0.0f || qn.w <= 0.0f) continue;
+                    const V3 q2 = v3(qp.x, qp.y, qp.z), qn2 = v3(qn.x, qn.y, qn.z);
+                    if (!gi_accepts_shift(s.p, v3(ng.x, ng.y, ng.z), q2, qn2)) continue;
+                    const S4 qrad = s4_of(ra.gcur.rad[np]);
+                    const float w = gi_target(sp, m, s, dir, q2, qn2, qrad, qw.w) * qw.z * qw.y;
+                    gws += w;
+                    gM += qw.y;
+                    if (rand_f(rng.pcg) * gws < w) {
+                        spos = q2;
+                        snrm = qn2;
+                        srad = qrad;
+                        su = qw.w;
+                        sflag = true;
+                    }
+                }
+                const float ft = sflag ? gi_target(sp, m, s, dir, spos, snrm, srad, su) : 0.0f;
+                const float gW = gM > 0.0f && ft > 0.0f ? gws / (gM * ft) : 0.0f;
+                const V3 delta = spos - s.p;
+                const float dist = length(delta);
+                if (gW > 0.0f && dist > 2.0f * ray_epsilon(s.p)) {
+                    // The connection ray of gi_connection_visible, traced after this pass; its
+                    // contribution f rad G W at the sample's wavelengths.
+                    const V3 o = ray_origin(s.p, s.ng, delta * (1.0f / dist), s.error);
+                    const V3 ed = spos - o;
+                    const float d = length(ed);
+                    const V3 cd = ed * (1.0f / d);
+                    const V3 wi = normalize(delta);
+                    const Lanes sl = lanes_of(sp, su);
+                    const S4 fb = diffuse_f(s, scale(reflectance(sp, sl, m.lagrange), 1.0f / kPi), -dir, wi);
+                    const V3 rgb = spectrum_rgb(sp, sl, scale(mul(fb, srad), gi_geometry(s.p, s.n, spos, snrm) * gW));
+                    const GiWork &g = ra.gw;
+                    g.sox[pix] = o.x; g.soy[pix] = o.y; g.soz[pix] = o.z;
+                    g.sdx[pix] = cd.x; g.sdy[pix] = cd.y; g.sdz[pix] = cd.z;
+                    g.stmin[pix] = ray_t_min(o);
+                    g.stmax[pix] = d - endpoint_tolerance(o, spos, d);
+                    g.rgb[pix] = make_float4(rgb.x, rgb.y, rgb.z, 0.0f);
+                }
+            }
+        } else if (alive && !is_delta(m) && f.sampling_mode != 3u &&
+                   !((state & kGiSecondary) != 0u && bounce == 1u && m.type == kDiffuse)) {
+            // (A diffuse secondary vertex of a ReSTIR GI path takes no NEE: the GI reservoir holds
+            // that light, restirGISecondary, main.swift:7801-7803.)
             // Next-event estimation at this non-delta vertex (main.swift:7655-7672, 7790-7805).
             const bool use_mis = f.sampling_mode <= 1u && (bounce == 0u ? f.scattering_limit > 0 : sdepth < f.scattering_limit);
             rng_event(rng, bounce + 1u, kZNee);  // this hit is vertex bounce + 1 (main.swift:7658, 7804)
@@ -1200,8 +1499,10 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
                     // whole direct integral, so in ReSTIR mode its continuation is not MIS-weighted.
                     const bool mis = f.sampling_mode == 1u || (f.sampling_mode == 0u && (bounce > 0u || m.type != kDiffuse));
                     const bool nee = !prev_delta && f.sampling_mode != 3u;
+                    const bool gi_path = kRestir && ra.gi && bounce == 0u && m.type == kDiffuse;
                     uint32_t flags = (prev_delta ? kPrevDelta : 0u) | (nee ? kPrevNee : 0u) | (mis ? kPrevMis : 0u) |
-                                     (emission_only ? kEmissionOnly : 0u) | ((record || (state & kRecorded)) ? kRecorded : 0u);
+                                     (emission_only ? kEmissionOnly : 0u) | ((record || (state & kRecorded)) ? kRecorded : 0u) |
+                                     (gi_path ? kGiSecondary : 0u);
                     state = min(bounce + 1u, 255u) | ((uint32_t)min(sdepth, 255) << 8) | flags;
                     cont = true;
                 }
@@ -1563,6 +1864,22 @@ hipError_t launch_shade_restir(const RenderScene &sc, const FrameArgs &f, const 
 hipError_t launch_restir_initial(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
                                  const RestirArgs &ra, hipStream_t s) {
     hipLaunchKernelGGL(restir_initial_kernel, dim3((f.npix + 255) / 256), dim3(256), 0, s, sc, f, q, hits, ra);
+    return hipGetLastError();
+}
+
+hipError_t launch_restir_gi_secondary(const RenderScene &sc, const FrameArgs &f, const RestirArgs &ra, hipStream_t s) {
+    hipLaunchKernelGGL(restir_gi_secondary_kernel, dim3((f.npix + 255) / 256), dim3(256), 0, s, sc, f, ra);
+    return hipGetLastError();
+}
+
+hipError_t launch_restir_gi_finish(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
+                                   const RestirArgs &ra, hipStream_t s) {
+    hipLaunchKernelGGL(restir_gi_finish_kernel, dim3((f.npix + 255) / 256), dim3(256), 0, s, sc, f, q, hits, ra);
+    return hipGetLastError();
+}
+
+hipError_t launch_restir_gi_resolve(const FrameArgs &f, const RestirArgs &ra, float *radiance, hipStream_t s) {
+    hipLaunchKernelGGL(restir_gi_resolve_kernel, dim3((f.npix + 255) / 256), dim3(256), 0, s, f, ra, radiance);
     return hipGetLastError();
 }
 

@@ -56,8 +56,39 @@ struct RestirBuffers {
     uint2 *gnorm;
     float4 *posdir, *emitpdf, *weights;
 };
+// ReSTIR GI reservoir per pixel (main.swift:4625, 4803-4811): pos = secondary point x2 + source pdf
+// (area measure), nrm = its normal and a sample flag as four halves (rgba16f), rad = its secondary
+// radiance at the sample's four wavelengths, weights = weight sum, M, W, wavelength number.
+struct GiBuffers {
+    float4 *pos;
+    uint2 *nrm;
+    float4 *rad, *weights;
+};
+// Work arrays of ReSTIR GI, per pixel: the x1 -> x2 rays (RayArgs layout) and their hits, the light
+// sample drawn at x2 (direction, emission, solid-angle pdf), the stage reached (0 no GI ray, 1 ray
+// traced, 2 light sample with a shadow ray, 3 light sample that sees no light), the BSDF pdf of the GI
+// direction and the PCG state after the stage's draws.  The visibility rays (s*) and their results
+// serve twice: the shadow rays of the x2 light samples in pass 1, then the x1 -> x2 connection rays of
+// the spatial reuse in pass 2, whose unoccluded contribution (rgb) is added by the GI resolve.
+struct GiWork {
+    float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+    float *t, *b1, *b2;
+    int32_t *face;
+    float *sox, *soy, *soz, *sdx, *sdy, *sdz, *stmin, *stmax;
+    uint8_t *occluded;
+    float4 *light_wi;    // wi, pdf
+    float4 *light_le;    // emission rgb
+    float4 *rgb;         // pass 2: the connection's contribution if x1 sees x2
+    uint32_t *stage, *seed;
+    float *bpdf;
+};
+
 struct RestirArgs {
     RestirBuffers cur, hist;  // written by this sample's pass 1 / the previous sample's
+    GiBuffers gcur, ghist;    // the same for ReSTIR GI
+    GiWork gw;
+    uint32_t gi;              // ReSTIR GI at the first indirect diffuse vertex (IndirectReuse.restirGI)
+    uint32_t gi_temporal;     // ghist holds the previous sample's GI reservoirs
     uint32_t candidates;      // RIS candidates per pixel (the reference uses 4)
     uint32_t temporal;        // history holds the previous sample of the same image
     // Previous sample's camera for backprojection (prevViewProj, main.swift:4828-4835).
@@ -156,6 +187,15 @@ hipError_t launch_shade_sorted(const RenderScene &sc, const FrameArgs &f, const 
 // for the wavefront's single sample (q holds the camera paths in pixel order, hits their hits).
 hipError_t launch_restir_initial(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
                                  const RestirArgs &ra, hipStream_t s);
+// ReSTIR GI, after pass 1 traced its x1 -> x2 rays (ra.gw): the light sample at each diffuse x2 and
+// its shadow ray (stage 2); then, with the shadow rays traced, the initial reservoir, the temporal
+// merge and the encode into ra.gcur.
+hipError_t launch_restir_gi_secondary(const RenderScene &sc, const FrameArgs &f, const RestirArgs &ra, hipStream_t s);
+hipError_t launch_restir_gi_finish(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
+                                   const RestirArgs &ra, hipStream_t s);
+// After pass 2's connection rays (ra.gw s*) are traced into ra.gw.occluded: radiance[pixel] += the
+// contribution of each connection that is not occluded.
+hipError_t launch_restir_gi_resolve(const FrameArgs &f, const RestirArgs &ra, float *radiance, hipStream_t s);
 // launch_shade with ReSTIR DI at the primary vertex (sampling mode 0): spatial reuse of ra.cur,
 // shading with one visibility ray, then the path suffix.
 hipError_t launch_shade_restir(const RenderScene &sc, const FrameArgs &f, const PathQueue &in, const HitArgs &hits,

@@ -53,6 +53,10 @@ struct vt_renderer {
     DeviceBuffer restir[2];
     int restir_cur = 0;
     uint32_t restir_history = 0;
+    // ReSTIR GI: reservoirs (56 B per pixel each) ping-ponged with the DI ones, and the per-pixel
+    // work arrays (141 B); gi_history: the history reservoirs were written by a GI sample.
+    DeviceBuffer gi_res[2], gi_work;
+    bool gi_history = false;
     uint64_t restir_generation = 0;  // vt_scene::generation the history was rendered with
     uint32_t *host_count = nullptr;  // pinned: live paths, read between vertices of delta scenes
     // Learned radiance cache: the attached network and its queues (sized with the path capacity).
@@ -93,6 +97,39 @@ vt::RestirBuffers restir_view(const DeviceBuffer &b, uint32_t npix) {
     v.emitpdf = reinterpret_cast<float4 *>(p + (size_t)40 * npix);
     v.weights = reinterpret_cast<float4 *>(p + (size_t)56 * npix);
     return v;
+}
+
+vt::GiBuffers gi_view(const DeviceBuffer &b, uint32_t npix) {
+    char *p = b.as<char>();
+    vt::GiBuffers v;
+    v.pos = reinterpret_cast<float4 *>(p);
+    v.rad = reinterpret_cast<float4 *>(p + (size_t)16 * npix);
+    v.weights = reinterpret_cast<float4 *>(p + (size_t)32 * npix);
+    v.nrm = reinterpret_cast<uint2 *>(p + (size_t)48 * npix);
+    return v;
+}
+
+// Work arrays: three float4 arrays, 23 four-byte arrays, the visibility bytes.
+constexpr size_t kGiWorkBytes = 3 * 16 + 23 * 4 + 1;
+
+vt::GiWork gi_work_view(const DeviceBuffer &b, uint32_t npix) {
+    char *p = b.as<char>();
+    vt::GiWork w;
+    w.light_wi = reinterpret_cast<float4 *>(p);
+    w.light_le = reinterpret_cast<float4 *>(p + (size_t)16 * npix);
+    w.rgb = reinterpret_cast<float4 *>(p + (size_t)32 * npix);
+    char *a = p + (size_t)48 * npix;
+    auto f = [&](int k) { return reinterpret_cast<float *>(a + (size_t)k * npix * 4); };
+    w.ox = f(0); w.oy = f(1); w.oz = f(2); w.dx = f(3); w.dy = f(4); w.dz = f(5); w.tmin = f(6); w.tmax = f(7);
+    w.t = f(8); w.b1 = f(9); w.b2 = f(10);
+    w.face = reinterpret_cast<int32_t *>(f(11));
+    w.sox = f(12); w.soy = f(13); w.soz = f(14); w.sdx = f(15); w.sdy = f(16); w.sdz = f(17);
+    w.stmin = f(18); w.stmax = f(19);
+    w.stage = reinterpret_cast<uint32_t *>(f(20));
+    w.seed = reinterpret_cast<uint32_t *>(f(21));
+    w.bpdf = f(22);
+    w.occluded = reinterpret_cast<uint8_t *>(a + (size_t)23 * npix * 4);
+    return w;
 }
 
 vt::ShadowQueue shadow_view(const vt_renderer *r) {
@@ -552,7 +589,12 @@ static vt_status render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_pa
             r->restir_history = 0;
             r->restir[0].reset();
             r->restir[1].reset();
+            r->gi_res[0].reset();
+            r->gi_res[1].reset();
+            r->gi_work.reset();
+            r->gi_history = false;
         }
+        if (p->indirect_reuse > 2) return fail(VT_ERROR_INVALID_ARGUMENT, "indirect_reuse must be 0, 1 or 2");
         const bool restir = p->sampling_mode == 0;
         if (restir && !r->restir[0].ptr) {
             r->restir[0].alloc((size_t)72 * npix);
@@ -578,6 +620,13 @@ static vt_status render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_pa
         if (use_cache && (period < 2 || period > 4096)) return fail(VT_ERROR_INVALID_ARGUMENT, "cache_train_period must be 2..4096");
         f.nrc = use_cache ? 1u : 0u;
         f.train_period = period;
+        const bool gi = restir && p->indirect_reuse == 0 && f.scattering_limit >= 1 && !use_cache;
+        if (gi && !r->gi_res[0].ptr) {
+            r->gi_res[0].alloc((size_t)56 * npix);
+            r->gi_res[1].alloc((size_t)56 * npix);
+            r->gi_work.alloc(kGiWorkBytes * npix);
+            r->gi_history = false;
+        }
         if (use_cache) {
             if (!scene->box_valid) {
                 float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
@@ -638,6 +687,13 @@ static vt_status render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_pa
                 std::memcpy(ra.prev_up, r->prev_camera + 9, 3 * sizeof(float));
                 ra.prev_fov_scale = r->prev_camera[12];
                 ra.prev_aspect = r->prev_camera[13];
+                if (gi) {
+                    ra.gi = 1u;
+                    ra.gcur = gi_view(r->gi_res[r->restir_cur], npix);
+                    ra.ghist = gi_view(r->gi_res[1 - r->restir_cur], npix);
+                    ra.gw = gi_work_view(r->gi_work, npix);
+                    ra.gi_temporal = ra.temporal && r->gi_history ? 1u : 0u;
+                }
             }
             // Vertices: the scattering budget bounds non-delta events only; with delta materials in the
             // scene, keep extending (checking the live count every 8 vertices) until no path is left.
@@ -658,6 +714,19 @@ static vt_status render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_pa
                 vt::Counters c{cin, cout, csh};
                 if (restir && b == 0) {
                     check(vt::launch_restir_initial(sc, f, qin, hits, ra, s), "restir pass 1");
+                    if (gi) {
+                        // ReSTIR GI's x1 -> x2 rays, the light samples at x2 and their shadow rays, then
+                        // the reservoirs.
+                        const vt::GiWork &w = ra.gw;
+                        const vt::RayArgs grays{w.ox, w.oy, w.oz, w.dx, w.dy, w.dz, w.tmin, w.tmax};
+                        check(vt::launch_trace(scene->view, grays, vt::HitArgs{w.t, w.b1, w.b2, w.face, nullptr}, nullptr,
+                                               npix, false, nullptr, s), "restir gi rays");
+                        check(vt::launch_restir_gi_secondary(sc, f, ra, s), "restir gi secondary");
+                        const vt::RayArgs vis{w.sox, w.soy, w.soz, w.sdx, w.sdy, w.sdz, w.stmin, w.stmax};
+                        check(vt::launch_trace(scene->view, vis, vt::HitArgs{}, w.occluded, npix, true, nullptr, s),
+                              "restir gi shadow");
+                        check(vt::launch_restir_gi_finish(sc, f, qin, hits, ra, s), "restir gi reservoirs");
+                    }
                     check(vt::launch_shade_restir(sc, f, qin, hits, qout, sh, c, n, r->radiance.as<float>(), ra, na, s),
                           "shade");
                 } else if (p->material_sort && scene->mixed) {
@@ -672,6 +741,14 @@ static vt_status render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_pa
                 check(vt::launch_trace(scene->view, srays, vt::HitArgs{}, r->occluded.as<uint8_t>(), n, true, nullptr, s, csh),
                       "shadow");
                 check(vt::launch_resolve(f, sh, r->occluded.as<uint8_t>(), csh, n, r->radiance.as<float>(), na, s), "resolve");
+                if (restir && b == 0 && gi) {
+                    // The spatial reuse's x1 -> x2 connections.
+                    const vt::GiWork &w = ra.gw;
+                    const vt::RayArgs vis{w.sox, w.soy, w.soz, w.sdx, w.sdy, w.sdz, w.stmin, w.stmax};
+                    check(vt::launch_trace(scene->view, vis, vt::HitArgs{}, w.occluded, npix, true, nullptr, s),
+                          "restir gi connections");
+                    check(vt::launch_restir_gi_resolve(f, ra, r->radiance.as<float>(), s), "restir gi resolve");
+                }
                 cur = 1 - cur;
             }
             if (use_cache) {
@@ -696,6 +773,7 @@ static vt_status render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_pa
             if (restir) {
                 r->restir_cur = 1 - r->restir_cur;
                 r->restir_history = std::min<uint32_t>(r->restir_history, 1u << 24) + 1u;
+                r->gi_history = gi;
                 std::memcpy(r->prev_camera, f.eye, 3 * sizeof(float));
                 std::memcpy(r->prev_camera + 3, f.forward, 3 * sizeof(float));
                 std::memcpy(r->prev_camera + 6, f.right, 3 * sizeof(float));

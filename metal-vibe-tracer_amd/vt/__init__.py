@@ -137,7 +137,8 @@ class _FrameParams(ctypes.Structure):
                 ("sampling_mode", ctypes.c_uint32), ("sample_index", ctypes.c_uint32),
                 ("restir_candidates", ctypes.c_uint32), ("radiance_cache", ctypes.c_uint32),
                 ("cache_train_period", ctypes.c_uint32), ("cache_learning_rate", ctypes.c_float),
-                ("material_sort", ctypes.c_uint32), ("sampler", ctypes.c_uint32), ("z_temporal", ctypes.c_uint32)]
+                ("material_sort", ctypes.c_uint32), ("sampler", ctypes.c_uint32), ("z_temporal", ctypes.c_uint32),
+                ("indirect_reuse", ctypes.c_uint32)]
 
 
 class _RaySoA(ctypes.Structure):
@@ -743,10 +744,10 @@ class Renderer:
 
     def render(self, scene: Scene, width, height, eye, target, fov, max_depth=8, samples=1, sample_index=1,
                sampling_mode=1, candidates=4, cache=False, train_period=16, learning_rate=1e-3, material_sort=False,
-               sampler=0, z_temporal=1, stream=None):
+               sampler=0, z_temporal=1, indirect_reuse=0, stream=None):
         p = _FrameParams(width, height, (ctypes.c_float * 3)(*eye), (ctypes.c_float * 3)(*target), fov, max_depth,
                          sampling_mode, sample_index, candidates, int(cache), train_period, learning_rate,
-                         int(material_sort), sampler, z_temporal)
+                         int(material_sort), sampler, z_temporal, indirect_reuse)
         _check(load().vt_render_frame(self._h, scene._h, ctypes.byref(p), samples, _stream_ptr(stream)))
         self.size = (width, height)
 

@@ -435,7 +435,7 @@ def _render_inputs(scene, materials, frame, light_tree):
                (ctypes.c_float * 3)(*frame["target"]), frame["fov"], frame["max_depth"], frame.get("sampling_mode", 1))
     fr.sampler = frame.get("sampler", 0)
     fr.z_temporal = frame.get("z_temporal", 1)
-    fr.indirect_reuse = frame.get("indirect_reuse", 1)
+    fr.indirect_reuse = frame.get("indirect_reuse", 0)  # 0: ReSTIR GI, as the library's default
     return rs, fr, keep
 
 
@@ -455,8 +455,9 @@ def render(scene: "Scene", materials, frame: dict, sample_index0: int, samples: 
 
 def render_restir(scene: "Scene", materials, frame: dict, sample_index0: int, samples: int, candidates=4, eyes=None,
                   precision="f32", threads=0, light_tree=True):
-    """ReSTIR DI frames (sampling mode 0): per-pixel mean over `samples` consecutive samples, the
-    first without history -> (height, width, 3) float64.  eyes: None or (samples, 3) cameras."""
+    """ReSTIR frames (sampling mode 0: ReSTIR DI, and ReSTIR GI unless frame["indirect_reuse"] is 1):
+    per-pixel mean over `samples` consecutive samples, the first without history -> (height, width, 3)
+    float64.  eyes: None or (samples, 3) cameras."""
     lib = load()
     rs, fr, keep = _render_inputs(scene, materials, dict(frame, sampling_mode=0), light_tree)
     w, h = frame["width"], frame["height"]
