@@ -448,7 +448,7 @@ def bench_learned_cache(args, ctx):
         setup = render_setup(oracle.render, osc, mats, full, 1, 1, pixels=pix[:1])
         dt, img = timed_render(oracle.render, osc, mats, full, 1, spp_cpu, pixels=pix)
         dt -= setup
-        small = dict(full, width=w // 4, height=h // 4)
+        small = dict(full, width=w // 4, height=h // 4, indirect_reuse=1)
         setup_c = render_setup(oracle.render_nrc, osc, mats, dict(full, width=1, height=1), 1, 1, trained[-1])
         dt_c, img_c = timed_render(oracle.render_nrc, osc, mats, small, 1, 2, trained[-1])
         dt_c -= setup_c
@@ -499,9 +499,11 @@ def bench_restir(args, ctx):
     """Config 3: the emitter terrain with Lambert and GGX surfaces (~10M triangles, 4096 emitters in
     the light tree), 1920 x 1080, depth 8, ReSTIR DI (32 candidates, 1 temporal + 1 spatial reuse)
     against MIS, `--restir-frames` frames of 1 spp each, both timed per frame (synchronised), each with
-    the per-pixel PCG streams and with the Z++ sampler.  Error:
-    relMSE on the 64 x 36 stratified pixel subset against the committed FP64 oracle image (MIS at
-    65536 spp, tests/golden/terrain_restir_ref.npz); ReSTIR DI converges to the same image."""
+    the per-pixel PCG streams and with the Z++ sampler; and ReSTIR DI + GI (the renderer's default
+    mode-0 estimator: GI at the first indirect diffuse vertex).  Error: relMSE on the 64 x 36
+    stratified pixel subset against the committed FP64 oracle image (MIS at 65536 spp,
+    tests/golden/terrain_restir_ref.npz); ReSTIR DI converges to the same image, ReSTIR GI to one
+    slightly brighter (the bias of its uniform spatial taps)."""
     import numpy as np
     import torch
 
@@ -518,18 +520,19 @@ def bench_restir(args, ctx):
     out = {"workload": f"emitter-terrain-restir-{len(mesh.faces)}tri-{w}x{h}-depth{depth}",
            "reference": {"kind": "FP64 oracle (MIS)", "spp": ref_spp, "pixels": int(len(pix))}}
     # *_z: the same with the Z++ sampler (sampler=1) in place of the per-pixel PCG streams.
-    for name, mode, sampler in (("mis", 1, 0), ("restir_di", 0, 0), ("mis_z", 1, 1), ("restir_di_z", 0, 1)):
+    for name, mode, sampler, reuse in (("mis", 1, 0, 1), ("restir_di", 0, 0, 1), ("mis_z", 1, 1, 1),
+                                       ("restir_di_z", 0, 1, 1), ("restir_gi", 0, 0, 0)):
         r = vt.Renderer(ctx)
         for k in range(2):
             r.render(scene, w, h, *view, max_depth=depth, samples=1, sample_index=900 + k, sampling_mode=mode,
-                     candidates=32, sampler=sampler)
+                     candidates=32, sampler=sampler, indirect_reuse=reuse)
         r.reset()
         spent = 0.0
         for k in range(args.restir_frames):
             torch.cuda.synchronize()
             t = time.perf_counter()
             r.render(scene, w, h, *view, max_depth=depth, samples=1, sample_index=1 + k, sampling_mode=mode,
-                     candidates=32, sampler=sampler)
+                     candidates=32, sampler=sampler, indirect_reuse=reuse)
             torch.cuda.synchronize()
             spent += time.perf_counter() - t
         full_img = r.readback()[..., :3].astype(np.float64)
@@ -547,7 +550,7 @@ def bench_restir(args, ctx):
         full = dict(width=w, height=h, eye=view[0], target=view[1], fov=view[2], max_depth=depth)
         small = dict(full, width=w // 4, height=h // 4)
         frames = 2
-        setup_r = render_setup(oracle.render_restir, osc, mats, dict(full, width=1, height=1), 1, 1, candidates=32)
+        setup_r = render_setup(oracle.render_restir, osc, mats, dict(small, width=1, height=1), 1, 1, candidates=32)
         dt_r, img_r = timed_render(oracle.render_restir, osc, mats, small, 1, frames, candidates=32)
         dt_r -= setup_r
         spp_cpu = 64
@@ -570,8 +573,10 @@ def bench_restir(args, ctx):
         (out["restir_di"]["relmse"] * out["restir_di"]["ms_per_frame"])
     out["relmse_ratio_pcg_over_z"] = {"mis": out["mis"]["relmse"] / out["mis_z"]["relmse"],
                                       "restir_di": out["restir_di"]["relmse"] / out["restir_di_z"]["relmse"]}
+    out["efficiency_ratio_gi"] = (out["mis"]["relmse"] * out["mis"]["ms_per_frame"]) / \
+        (out["restir_gi"]["relmse"] * out["restir_gi"]["ms_per_frame"])
     log(f"restir: MIS {out['mis']}, ReSTIR DI {out['restir_di']}, Z sampler: MIS {out['mis_z']}, "
-        f"ReSTIR DI {out['restir_di_z']}")
+        f"ReSTIR DI {out['restir_di_z']}, ReSTIR DI + GI {out['restir_gi']}")
     return out
 
 

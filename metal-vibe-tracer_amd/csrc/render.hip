@@ -952,7 +952,6 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
     float4 gpos = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     uint2 gnorm = pack_half4(0.0f, 0.0f, 0.0f, -1.0f);
     float4 posdir = make_float4(0.0f, 0.0f, 0.0f, 0.0f), emitpdf = posdir, weights = posdir;
-    bool gi_ray = false;
     const int face = hits.face[pix];
     if (face >= 0) {
         const V3 dir = v3(q.dx[pix], q.dy[pix], q.dz[pix]);
@@ -1034,35 +1033,7 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
             posdir = make_float4(sel.position.x, sel.position.y, sel.position.z, pack_light(sel.light));
             emitpdf = make_float4(sel.emission.x, sel.emission.y, sel.emission.z, sel.pdf);
             weights = make_float4(wsum, M, W, sel.u);
-            if (ra.gi && f.scattering_limit >= 1) {
-                // ReSTIR GI's x1 -> x2 direction (restir_gi_initial, main.swift:4733-4743): a BSDF sample
-                // at x1 after the DI draws, traced by the next stage.
-                rng_event(rng, 1u, kZGiBsdf);
-                V3 gd;
-                S4 gw;
-                float gpdf;
-                if (sample_bsdf<false>(sc, sp, lanes, s, m, dir, rng, gd, gw, gpdf) && gpdf > 0.0f) {
-                    const V3 o = ray_origin(s.p, s.ng, gd, s.error);
-                    gi_ray = true;
-                    ra.gw.ox[pix] = o.x; ra.gw.oy[pix] = o.y; ra.gw.oz[pix] = o.z;
-                    ra.gw.dx[pix] = gd.x; ra.gw.dy[pix] = gd.y; ra.gw.dz[pix] = gd.z;
-                    ra.gw.tmin[pix] = ray_t_min(o);
-                    ra.gw.tmax[pix] = 1e20f;
-                    ra.gw.bpdf[pix] = gpdf;
-                }
-                ra.gw.seed[pix] = rng.pcg;
-            }
-        }
-    }
-    if (ra.gi) {
-        ra.gw.stage[pix] = gi_ray ? 1u : 0u;
-        if (!gi_ray) {
-            // No x2 to find: a ray that ends before it starts.
-            ra.gw.ox[pix] = ra.gw.oy[pix] = ra.gw.oz[pix] = 0.0f;
-            ra.gw.dx[pix] = ra.gw.dy[pix] = 0.0f;
-            ra.gw.dz[pix] = 1.0f;
-            ra.gw.tmin[pix] = 1.0f;
-            ra.gw.tmax[pix] = 0.0f;
+            if (ra.gi) ra.gw.seed[pix] = rng.pcg;  // ReSTIR GI's draws continue this stream
         }
     }
     ra.cur.gpos[pix] = gpos;
@@ -1070,6 +1041,45 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
     ra.cur.posdir[pix] = posdir;
     ra.cur.emitpdf[pix] = emitpdf;
     ra.cur.weights[pix] = weights;
+}
+
+// ReSTIR GI stage 1 (restir_gi_initial, main.swift:4733-4743): at a diffuse primary hit, the x1 -> x2
+// direction, a BSDF sample drawn after ReSTIR DI's draws (the PCG state pass 1 left in gw.seed), and
+// its ray, traced by the next stage.  Pixels without one get a ray that ends before it starts.
+__global__ __launch_bounds__(256) void restir_gi_rays_kernel(RenderScene sc, FrameArgs f, PathQueue q, HitArgs hits,
+                                                             RestirArgs ra) {
+    const uint32_t pix = blockIdx.x * 256 + threadIdx.x;
+    if (pix >= f.npix) return;
+    const GiWork &g = ra.gw;
+    bool ray = false;
+    V3 o = v3(0.0f, 0.0f, 0.0f), gd = v3(0.0f, 0.0f, 1.0f);
+    const int face = hits.face[pix];
+    if (face >= 0) {
+        const V3 dir = v3(q.dx[pix], q.dy[pix], q.dz[pix]);
+        const Surface s = surface_at(sc, (uint32_t)face, hits.b1[pix], hits.b2[pix], dir);
+        const Mat m = material(sc, s.slot);
+        if (m.type == kDiffuse) {
+            const Spectral sp{sc.icdf, reinterpret_cast<const float4 *>(sc.cmf), reinterpret_cast<const float4 *>(sc.basis)};
+            Rng rng{g.seed[pix], ZSampler{0u, 0u, 0u, 0u}, f.sampler == 1u};
+            if (rng.z) rng.zs.key = z_pixel_key(pix % f.width, pix / f.width, f.frame0 + 1u, f.sample_index0, f.z_temporal);
+            rng_event(rng, 1u, kZGiBsdf);
+            S4 bw;
+            float gpdf;
+            if (sample_bsdf<false>(sc, sp, lanes_of(sp, q.wl[pix]), s, m, dir, rng, gd, bw, gpdf) && gpdf > 0.0f) {
+                o = ray_origin(s.p, s.ng, gd, s.error);
+                g.bpdf[pix] = gpdf;
+                ray = true;
+            } else {
+                gd = v3(0.0f, 0.0f, 1.0f);
+            }
+            g.seed[pix] = rng.pcg;
+        }
+    }
+    g.stage[pix] = ray ? 1u : 0u;
+    g.ox[pix] = o.x; g.oy[pix] = o.y; g.oz[pix] = o.z;
+    g.dx[pix] = gd.x; g.dy[pix] = gd.y; g.dz[pix] = gd.z;
+    g.tmin[pix] = ray ? ray_t_min(o) : 1.0f;
+    g.tmax[pix] = ray ? 1e20f : 0.0f;
 }
 
 // ReSTIR GI stage 2 (restir_gi_initial at x2, main.swift:4744-4756): at a diffuse secondary hit, the
@@ -1864,6 +1874,12 @@ hipError_t launch_shade_restir(const RenderScene &sc, const FrameArgs &f, const 
 hipError_t launch_restir_initial(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
                                  const RestirArgs &ra, hipStream_t s) {
     hipLaunchKernelGGL(restir_initial_kernel, dim3((f.npix + 255) / 256), dim3(256), 0, s, sc, f, q, hits, ra);
+    return hipGetLastError();
+}
+
+hipError_t launch_restir_gi_rays(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
+                                 const RestirArgs &ra, hipStream_t s) {
+    hipLaunchKernelGGL(restir_gi_rays_kernel, dim3((f.npix + 255) / 256), dim3(256), 0, s, sc, f, q, hits, ra);
     return hipGetLastError();
 }
 

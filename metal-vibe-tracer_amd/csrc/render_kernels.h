@@ -187,9 +187,11 @@ hipError_t launch_shade_sorted(const RenderScene &sc, const FrameArgs &f, const 
 // for the wavefront's single sample (q holds the camera paths in pixel order, hits their hits).
 hipError_t launch_restir_initial(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
                                  const RestirArgs &ra, hipStream_t s);
-// ReSTIR GI, after pass 1 traced its x1 -> x2 rays (ra.gw): the light sample at each diffuse x2 and
-// its shadow ray (stage 2); then, with the shadow rays traced, the initial reservoir, the temporal
-// merge and the encode into ra.gcur.
+// ReSTIR GI after pass 1: the x1 -> x2 rays (ra.gw); once they are traced, the light sample at each
+// diffuse x2 and its shadow ray (stage 2); then, with the shadow rays traced, the initial reservoir,
+// the temporal merge and the encode into ra.gcur.
+hipError_t launch_restir_gi_rays(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
+                                 const RestirArgs &ra, hipStream_t s);
 hipError_t launch_restir_gi_secondary(const RenderScene &sc, const FrameArgs &f, const RestirArgs &ra, hipStream_t s);
 hipError_t launch_restir_gi_finish(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
                                    const RestirArgs &ra, hipStream_t s);

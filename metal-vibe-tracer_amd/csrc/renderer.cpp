@@ -718,6 +718,7 @@ static vt_status render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_pa
                         // ReSTIR GI's x1 -> x2 rays, the light samples at x2 and their shadow rays, then
                         // the reservoirs.
                         const vt::GiWork &w = ra.gw;
+                        check(vt::launch_restir_gi_rays(sc, f, qin, hits, ra, s), "restir gi directions");
                         const vt::RayArgs grays{w.ox, w.oy, w.oz, w.dx, w.dy, w.dz, w.tmin, w.tmax};
                         check(vt::launch_trace(scene->view, grays, vt::HitArgs{w.t, w.b1, w.b2, w.face, nullptr}, nullptr,
                                                npix, false, nullptr, s), "restir gi rays");
