@@ -107,6 +107,11 @@ def load(path=None):
         "vo_z_morton": (u32, [u32, u32]),
         "vo_z_pixel_key": (u32, [u32, u32, u32, u32, u32]),
         "vo_z_draws": (None, [u32] * 11 + [vp, vp]),
+        "vo_env_tables": (i32, [vp, u32, u32, vp, vp]),
+        "vo_env_query_f32": (None, [vp, vp, u32, u32, vp, vp]),
+        "vo_env_query_f64": (None, [vp, vp, u32, u32, vp, vp]),
+        "vo_light_samples_f32": (None, [vp, vp, u32, u32, u32, vp, vp, vp, vp]),
+        "vo_light_samples_f64": (None, [vp, vp, u32, u32, u32, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -304,7 +309,23 @@ class RenderScene(ctypes.Structure):
     _fields_ = [("geo", ctypes.c_void_p), ("materials", ctypes.c_void_p), ("material_count", ctypes.c_uint32),
                 ("emitter_face", ctypes.c_void_p), ("emitter_cdf", ctypes.c_void_p),
                 ("emitter_count", ctypes.c_uint32), ("emitter_total", ctypes.c_float), ("icdf", ctypes.c_void_p),
-                ("cmf", ctypes.c_void_p), ("basis", ctypes.c_void_p), ("tree", ctypes.c_void_p)]
+                ("cmf", ctypes.c_void_p), ("basis", ctypes.c_void_p), ("tree", ctypes.c_void_p),
+                ("env", ctypes.c_void_p), ("env_rows", ctypes.c_void_p), ("env_cols", ctypes.c_void_p),
+                ("env_w", ctypes.c_uint32), ("env_h", ctypes.c_uint32), ("env_strength", ctypes.c_float),
+                ("env_rotation", ctypes.c_float)]
+
+
+def env_tables(rgba):
+    """MaterialLibrary.environmentImportance restated (oracle/environment.c): rgba (H, W, 4) float32
+    -> (rows (H, 2), cols (2, H, W)) float32: per table (0 compensated, 1 original) the marginal row
+    CDF and the conditional column CDFs."""
+    img = np.ascontiguousarray(rgba, np.float32)
+    h, w = img.shape[:2]
+    rows = np.zeros((h, 2), np.float32)
+    cols = np.zeros((2, h, w), np.float32)
+    if load().vo_env_tables(_p(img), w, h, _p(rows), _p(cols)) != 0:
+        raise ValueError("environment map must be non-empty")
+    return rows, cols
 
 
 class Frame(ctypes.Structure):
@@ -436,6 +457,16 @@ def _render_inputs(scene, materials, frame, light_tree):
     fr.sampler = frame.get("sampler", 0)
     fr.z_temporal = frame.get("z_temporal", 1)
     fr.indirect_reuse = frame.get("indirect_reuse", 0)  # 0: ReSTIR GI, as the library's default
+    env = frame.get("environment")
+    if env is not None and frame.get("env_strength", 1.0) > 0:
+        # frame["environment"]: (H, W, 4) rgba; env_strength (default 1) and env_rotation (radians).
+        img = np.ascontiguousarray(env, np.float32)
+        rows, cols = env_tables(img)
+        keep = keep + (img, rows, cols)
+        rs.env, rs.env_rows, rs.env_cols = _p(img), _p(rows), _p(cols)
+        rs.env_h, rs.env_w = img.shape[0], img.shape[1]
+        rs.env_strength = float(frame.get("env_strength", 1.0))
+        rs.env_rotation = float(frame.get("env_rotation", 0.0))
     return rs, fr, keep
 
 
@@ -449,6 +480,31 @@ def render(scene: "Scene", materials, frame: dict, sample_index0: int, samples: 
     out = np.zeros((len(pix), 3), np.float64)
     fn = lib.vo_render_f64 if precision == "f64" else lib.vo_render_f32
     fn(ctypes.byref(rs), ctypes.byref(fr), sample_index0, samples, _p(pix), len(pix), threads, _p(out))
+    del keep
+    return out
+
+
+def env_query(scene: "Scene", materials, frame: dict, dirs, table=0, precision="f32", light_tree=True):
+    """The environment of frame["environment"] (with env_strength / env_rotation) seen by the oracle:
+    (radiance (n, 3), light-sample density (n,) of `table`, importedProbability applied)."""
+    rs, _, keep = _render_inputs(scene, materials, frame, light_tree)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    rgb = np.zeros((len(d), 3), np.float32)
+    pdf = np.zeros(len(d), np.float32)
+    fn = load().vo_env_query_f64 if precision == "f64" else load().vo_env_query_f32
+    fn(ctypes.byref(rs), _p(d), len(d), table, _p(rgb), _p(pdf))
+    del keep
+    return rgb, pdf
+
+
+def light_samples(scene: "Scene", materials, frame: dict, p, n, seed=1, table=0, precision="f32", light_tree=True):
+    """n light samples at point p (sample_direct_light on the PCG stream `seed`) -> dict wi (n, 3),
+    pdf, rgb (n, 3), light (0 none, 1 environment, face + 2)."""
+    rs, _, keep = _render_inputs(scene, materials, frame, light_tree)
+    out = {"wi": np.zeros((n, 3), np.float32), "pdf": np.zeros(n, np.float32), "rgb": np.zeros((n, 3), np.float32),
+           "light": np.zeros(n, np.uint32)}
+    fn = load().vo_light_samples_f64 if precision == "f64" else load().vo_light_samples_f32
+    fn(ctypes.byref(rs), _f3(p), n, seed, table, _p(out["wi"]), _p(out["pdf"]), _p(out["rgb"]), _p(out["light"]))
     del keep
     return out
 

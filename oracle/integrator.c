@@ -38,6 +38,9 @@
 #define RFMAX fmaxf
 #define RFMIN fminf
 #define RFLOOR floorf
+#define RACOS acosf
+#define RATAN2 atan2f
+#define RNEXTAFTER nextafterf
 #endif
 
 typedef REAL real;
@@ -239,6 +242,17 @@ static real rr(rng_t *r) {
     vo_z_draw(r->key, &r->dim, r->sub, r->sub_bits, 1, &f);
     return (real)f;
 }
+static void rr2(rng_t *r, real *a, real *b) {
+    if (!r->z) {
+        *a = (real)vo_rand_f(&r->pcg);
+        *b = (real)vo_rand_f(&r->pcg);
+        return;
+    }
+    float f[2];
+    vo_z_draw(r->key, &r->dim, r->sub, r->sub_bits, 2, f);
+    *a = (real)f[0];
+    *b = (real)f[1];
+}
 static void rr3(rng_t *r, real *a, real *b, real *c) {
     if (!r->z) {
         *a = (real)vo_rand_f(&r->pcg);
@@ -264,12 +278,126 @@ static void rng_candidate(rng_t *r, uint32_t vertex, uint32_t stream, uint32_t i
     r->sub_bits = bits;
 }
 
+/* ------------------------------------------------------------------ environment map
+ * An image environment (Uniforms.environment.z = 1) without a sun (sunParams.w = 0), in an imported
+ * scene (sceneIndex 6): eval_environment :2123-2135 (bilinear, wrapping in longitude), the sampling
+ * tables' cell density environment_cell_pdf :2102-2112, environment_image_pdf :2114-2121, the binary
+ * search environment_cdf_index :2088-2096, the image branch of sample_direct_light :2934-2949,
+ * environment_mixture_pdf :2707-2713 (sun probability 0), eval_environment_pdf :2880-2883 and
+ * imported_light_probability :2716-2719. */
+enum { ENV_COMPENSATED = 0, ENV_ORIGINAL = 1 };
+static const float TWO_PI_F = 6.28318530717958647692f;
+static int env_on(const vo_render_scene *sc) { return sc->env != NULL && sc->env_strength > 0.0f; }
+/* The probability that a light sample takes the triangle emitters rather than the environment. */
+static real imported_probability(const vo_render_scene *sc) {
+    if (sc->emitter_count == 0) return 0;
+    return env_on(sc) ? (real)0.5 : (real)1;
+}
+static const float *env_texel(const vo_render_scene *sc, int x, int y) {
+    return sc->env + 4 * ((size_t)y * sc->env_w + (size_t)x);
+}
+/* eval_environment: the map at direction d, bilinear between texel centres (the sampler's linear
+ * filter with repeat addressing; latitude clamped half a texel inside), times the strength. */
+static v3 eval_environment(const vo_render_scene *sc, v3 d) {
+    if (!env_on(sc)) return V(0, 0, 0);
+    const real w = (real)sc->env_w, h = (real)sc->env_h;
+    real dy = d.y < (real)-1 ? (real)-1 : (d.y > (real)1 ? (real)1 : d.y);
+    real u = RATAN2(d.z, d.x) / (real)TWO_PI_F + (real)0.5 + (real)sc->env_rotation / (real)TWO_PI_F;
+    real v = RACOS(dy) / PI_R;
+    const real vlo = (real)0.5 / h, vhi = (real)1 - (real)0.5 / h;
+    v = v < vlo ? vlo : (v > vhi ? vhi : v);
+    real fx = u * w - (real)0.5, fy = v * h - (real)0.5;
+    real x0f = RFLOOR(fx), y0f = RFLOOR(fy);
+    real ax = fx - x0f, ay = fy - y0f;
+    long long x0 = (long long)x0f, y0 = (long long)y0f;
+    int xa = (int)(((x0 % (long long)sc->env_w) + sc->env_w) % sc->env_w), xb = (xa + 1) % (int)sc->env_w;
+    int ya = (int)(y0 < 0 ? 0 : (y0 >= (long long)sc->env_h ? sc->env_h - 1 : y0));
+    int yb = ya + 1 < (int)sc->env_h ? ya + 1 : (int)sc->env_h - 1;
+    const float *t00 = env_texel(sc, xa, ya), *t10 = env_texel(sc, xb, ya), *t01 = env_texel(sc, xa, yb),
+                *t11 = env_texel(sc, xb, yb);
+    real c[3];
+    for (int k = 0; k < 3; ++k) {
+        real top = (real)t00[k] * ((real)1 - ax) + (real)t10[k] * ax;
+        real bot = (real)t01[k] * ((real)1 - ax) + (real)t11[k] * ax;
+        c[k] = top * ((real)1 - ay) + bot * ay;
+    }
+    const real st = (real)sc->env_strength;
+    return V(c[0] * st, c[1] * st, c[2] * st);
+}
+static real env_cell_pdf(const vo_render_scene *sc, uint32_t x, uint32_t y, real sin_theta, uint32_t table) {
+    const uint32_t w = sc->env_w, h = sc->env_h, cond = y + table * h;
+    real row = (real)sc->env_rows[2 * y + table];
+    real prev_row = y == 0 ? 0 : (real)sc->env_rows[2 * (y - 1) + table];
+    real col = (real)sc->env_cols[(size_t)cond * w + x];
+    real prev_col = x == 0 ? 0 : (real)sc->env_cols[(size_t)cond * w + x - 1];
+    real p = RFMAX((real)0, row - prev_row) * RFMAX((real)0, col - prev_col);
+    if (vo_get_fault() == 11) return p * (real)((float)(w * h)) / ((real)2 * PI_R * PI_R);  /* planted */
+    return p * (real)((float)(w * h)) / ((real)2 * PI_R * PI_R * RFMAX((real)1e-5, sin_theta));
+}
+static real env_image_pdf(const vo_render_scene *sc, v3 wi, uint32_t table) {
+    const uint32_t w = sc->env_w, h = sc->env_h;
+    real cy = wi.y < (real)-1 ? (real)-1 : (wi.y > (real)1 ? (real)1 : wi.y);
+    real theta = RACOS(cy);
+    real u = RATAN2(wi.z, wi.x) / (real)TWO_PI_F + (real)0.5 + (real)sc->env_rotation / (real)TWO_PI_F;
+    real v = theta / PI_R;
+    u = u - RFLOOR(u);
+    if (u > (real)0x1.fffffep-1f) u = (real)0x1.fffffep-1f;
+    v = v < 0 ? 0 : (v > (real)1 - (real)1e-7 ? (real)1 - (real)1e-7 : v);
+    uint32_t x = (uint32_t)(u * (real)w), y = (uint32_t)(v * (real)h);
+    if (x > w - 1) x = w - 1;
+    if (y > h - 1) y = h - 1;
+    return env_cell_pdf(sc, x, y, RSIN(theta), table);
+}
+/* eval_environment_pdf: the density of a light sample's environment proposal in `table`. */
+static real env_pdf(const vo_render_scene *sc, v3 wi, uint32_t table) {
+    if (!env_on(sc)) return 0;
+    return ((real)1 - imported_probability(sc)) * env_image_pdf(sc, wi, table);
+}
+/* environment_cdf_index: the first entry >= value among count entries (at most 16 halvings). */
+static uint32_t env_cdf_index(const float *cdf, uint32_t count, size_t base, size_t step, real value) {
+    uint32_t low = 0, high = (count > 1 ? count : 1) - 1;
+    for (uint32_t it = 0; it < 16 && low < high; ++it) {
+        uint32_t mid = (low + high) / 2;
+        if ((real)cdf[base + (size_t)mid * step] >= value) high = mid;
+        else low = mid + 1;
+    }
+    return low;
+}
+
 typedef struct { v3 position, wi, emission; real pdf; uint32_t light; } light_sample;
-static light_sample sample_direct_light(const vo_render_scene *sc, v3 p, rng_t *seed) {
+/* The environment branch of sample_direct_light: 4 draws (two 2D constituents in Z mode). */
+static light_sample sample_environment(const vo_render_scene *sc, v3 p, rng_t *seed, uint32_t table) {
     light_sample ls;
     memset(&ls, 0, sizeof ls);
-    if (sc->emitter_count == 0) return ls;
-    if (!(rr(seed) < (real)1)) return ls;  /* importedProbability = 1 */
+    const uint32_t w = sc->env_w, h = sc->env_h;
+    real cx, cy;
+    rr2(seed, &cx, &cy);
+    uint32_t y = env_cdf_index(sc->env_rows, h, table, 2, cx);
+    uint32_t x = env_cdf_index(sc->env_cols, w, (size_t)(y + table * h) * w, 1, cy);
+    real ox, oy;
+    rr2(seed, &ox, &oy);
+    /* The in-texel offset stays inside the chosen cell (uv below the cell's far edge). */
+    real ux = (real)x / (real)w + ox / (real)w, uy = (real)y / (real)h + oy / (real)h;
+    real ex = RNEXTAFTER(((real)x + 1) / (real)w, (real)0), ey = RNEXTAFTER(((real)y + 1) / (real)h, (real)0);
+    ux = ux < ex ? ux : ex;
+    uy = uy < ey ? uy : ey;
+    real theta = uy * PI_R, phi = (ux - (real)0.5 - (real)sc->env_rotation / (real)TWO_PI_F) * (real)TWO_PI_F;
+    real st = RSIN(theta);
+    ls.wi = norm(V(st * RCOS(phi), RCOS(theta), st * RSIN(phi)));
+    real image = env_cell_pdf(sc, x, y, st, table);
+    ls.position = add(p, mulv(ls.wi, (real)1e6));
+    ls.light = 1u;
+    ls.pdf = ((real)1 - imported_probability(sc)) * image;
+    ls.emission = eval_environment(sc, ls.wi);
+    return ls;
+}
+
+static light_sample sample_direct_light(const vo_render_scene *sc, v3 p, rng_t *seed, uint32_t table) {
+    light_sample ls;
+    memset(&ls, 0, sizeof ls);
+    const real pimp = imported_probability(sc);
+    if (vo_get_fault() == 12) table = ENV_COMPENSATED;  /* planted */
+    if (!(pimp > 0 && rr(seed) < pimp)) return env_on(sc) ? sample_environment(sc, p, seed, table) : ls;
     /* Emitter choice and the point on it: one 3D constituent in Z mode (main.swift:2886-2894). */
     real value, ry, rz;
     rr3(seed, &value, &ry, &rz);
@@ -306,18 +434,27 @@ static light_sample sample_direct_light(const vo_render_scene *sc, v3 p, rng_t *
     } else {
         area_pdf = sc->emitter_total > 0 ? em.weight / (real)sc->emitter_total : 0;
     }
-    ls.pdf = geometry > (real)1e-12 ? area_pdf / geometry : 0;
+    ls.pdf = geometry > (real)1e-12 ? pimp * area_pdf / geometry : 0;
     return ls;
 }
 
 /* light_visible (main.swift:3254-3264): shadow ray from the offset origin to the point, stopped
- * short by the endpoint tolerance. */
-static int light_visible(const vo_render_scene *sc, const surf_t *s, v3 wi, v3 pos) {
+ * short by the endpoint tolerance; toward the environment (light 1) along wi for 1e6 less twice the
+ * ray epsilon. */
+static int light_visible_l(const vo_render_scene *sc, const surf_t *s, v3 wi, v3 pos, uint32_t light) {
     v3 o = ray_origin(s->p, s->ng, wi, s->error);
+    if (light == 1u) {
+        real tol = (real)2 * ray_epsilon(s->p);
+        if ((real)1e6 <= tol) return 0;
+        return !occluded(sc, o, norm(wi), ray_t_min(o), (real)1e6 - tol);
+    }
     v3 dd = sub(pos, o);
     real d = len(dd), tol = endpoint_tolerance(o, pos, d);
     if (d <= tol) return 0;
     return !occluded(sc, o, norm(dd), ray_t_min(o), d - tol);
+}
+static int light_visible(const vo_render_scene *sc, const surf_t *s, v3 wi, v3 pos) {
+    return light_visible_l(sc, s, wi, pos, 0u);
 }
 
 /* eval_bsdf_with_pdf (main.swift:2659-2691): f (not cosine-weighted) and the BSDF pdf of wi. */
@@ -402,11 +539,11 @@ static int sample_bsdf(const vo_render_scene *sc, const lanes_t *l, const surf_t
 
 /* Next-event estimate at surface s: draws 4 numbers, returns the contribution if visible. */
 static v3 nee(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, const mat_t *m, v3 wo, s4 thr,
-              int use_mis, rng_t *seed) {
+              int use_mis, uint32_t table, rng_t *seed) {
     v3 zero = V(0, 0, 0);
-    light_sample ls = sample_direct_light(sc, s->p, seed);
+    light_sample ls = sample_direct_light(sc, s->p, seed, table);
     if (!(ls.pdf > 0)) return zero;
-    if (!light_visible(sc, s, ls.wi, ls.position)) return zero;
+    if (!light_visible_l(sc, s, ls.wi, ls.position, ls.light)) return zero;
     real bpdf;
     s4 f = eval_bsdf_pdf(sc, l, s, m, wo, ls.wi, &bpdf);
     real cosine = RFABS(dot(s->n, ls.wi));
@@ -494,6 +631,39 @@ static v3 camera_dir(const cam_t *c, const vo_frame *fr, uint32_t gx, uint32_t g
  * emission_weight, NEE with MIS at every later vertex, roulette after bounce 3.  In ReSTIR mode
  * (samplingMode 0) the departure from a diffuse primary is not MIS-weighted: its DI reservoir
  * estimated the whole direct integral (main.swift:7725-7728). */
+/* eval_light_pdf for an imported emitter face reached from prev (main.swift:3070-3075, with
+ * imported_emitter_area_pdf :2853-2864): importedProbability x area density / geometry. */
+static real hit_light_pdf(const vo_render_scene *sc, v3 prev, const surf_t *r, uint32_t face, const mat_t *rm) {
+    tri_t t = tri_of(sc, face);
+    real g = imported_geometry(prev, r->p, &t);
+    if (sc->tree) {
+        uint32_t lo = 0, hi = sc->tree->count;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) / 2;
+            if (sc->tree->ids[mid] < face) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < sc->tree->count && sc->tree->ids[lo] == face && g > (real)1e-12) {
+            float fp[3] = {f_of(prev.x), f_of(prev.y), f_of(prev.z)};
+            real area = (real)0.5 * len(cross(sub(t.b, t.a), sub(t.c, t.a)));
+            real pmf = (real)vo_light_tree_pmf(sc->tree, fp, sc->tree->positions[lo]);
+            return area > 0 ? imported_probability(sc) * (pmf / area) / g : 0;
+        }
+        return 0;
+    }
+    if (sc->emitter_count && sc->emitter_total > 0 && rm->weight > 0 && g > (real)1e-12)
+        return imported_probability(sc) * (rm->weight / (real)sc->emitter_total) / g;
+    return 0;
+}
+/* A continuation that leaves the scene (main.swift:7751-7758): the environment with the MIS weight
+ * of its light-sample density in the compensated table. */
+static v3 env_miss(const vo_render_scene *sc, const lanes_t *l, s4 thr, v3 d, int prev_delta, int prev_nee,
+                   int prev_mis, real bpdf) {
+    if (!env_on(sc)) return V(0, 0, 0);
+    real w = emission_weight(prev_delta, prev_nee, prev_mis, bpdf, env_pdf(sc, d, ENV_COMPENSATED));
+    return spectrum_rgb(sc, l, sscale(smul(thr, emission_spectrum(sc, l, eval_environment(sc, d))), w));
+}
+
 static v3 continue_path_gi(const vo_render_scene *sc, const vo_frame *fr, const lanes_t *l, surf_t s, mat_t m, v3 dir,
                            rng_t *seed, int gi_secondary);
 static v3 continue_path(const vo_render_scene *sc, const vo_frame *fr, const lanes_t *l, surf_t s, mat_t m, v3 dir,
@@ -532,30 +702,13 @@ static v3 continue_path_gi(const vo_render_scene *sc, const vo_frame *fr, const 
         v3 o = ray_origin(s.p, s.ng, nd, s.error);
         cur_dir = nd;
         surf_t r;
-        if (!trace(sc, o, nd, ray_t_min(o), &r, &face)) break;
+        if (!trace(sc, o, nd, ray_t_min(o), &r, &face)) {
+            radiance = add(radiance, env_miss(sc, l, thr, nd, prev_delta, prev_nee, prev_mis, bpdf));
+            break;
+        }
         mat_t rm = mat_of(sc, r.slot);
         if (rm.type == 3) {
-            tri_t t = tri_of(sc, face);
-            real g = imported_geometry(prev, r.p, &t);
-            real light_pdf = 0;
-            if (sc->tree) {
-                /* imported_emitter_area_pdf with a tree (main.swift:2853-2860) */
-                uint32_t lo = 0, hi = sc->tree->count;
-                while (lo < hi) {
-                    uint32_t mid = (lo + hi) / 2;
-                    if (sc->tree->ids[mid] < face) lo = mid + 1;
-                    else hi = mid;
-                }
-                if (lo < sc->tree->count && sc->tree->ids[lo] == face && g > (real)1e-12) {
-                    float fp[3] = {f_of(prev.x), f_of(prev.y), f_of(prev.z)};
-                    real area = (real)0.5 * len(cross(sub(t.b, t.a), sub(t.c, t.a)));
-                    real pmf = (real)vo_light_tree_pmf(sc->tree, fp, sc->tree->positions[lo]);
-                    light_pdf = area > 0 ? pmf / area / g : 0;
-                }
-            } else if (sc->emitter_count && sc->emitter_total > 0 && rm.weight > 0 && g > (real)1e-12) {
-                light_pdf = (rm.weight / (real)sc->emitter_total) / g;
-            }
-            real w = emission_weight(prev_delta, prev_nee, prev_mis, bpdf, light_pdf);
+            real w = emission_weight(prev_delta, prev_nee, prev_mis, bpdf, hit_light_pdf(sc, prev, &r, face, &rm));
             v3 e = r.front ? rm.emission : V(0, 0, 0);
             radiance = add(radiance, spectrum_rgb(sc, l, sscale(smul(thr, emission_spectrum(sc, l, e)), w)));
             break;
@@ -564,7 +717,8 @@ static v3 continue_path_gi(const vo_render_scene *sc, const vo_frame *fr, const 
         const int gi_skip = gi_secondary && bounce == 1 && rm.type == 0u;
         if (!is_delta(&rm) && fr->sampling_mode != 3u && !gi_skip) {
             rng_event(seed, (uint32_t)bounce + 1u, Z_NEE);
-            radiance = add(radiance, nee(sc, l, &r, &rm, neg(cur_dir), thr, fr->sampling_mode <= 1u && sdepth < limit, seed));
+            radiance = add(radiance, nee(sc, l, &r, &rm, neg(cur_dir), thr, fr->sampling_mode <= 1u && sdepth < limit,
+                                         sdepth < limit ? ENV_COMPENSATED : ENV_ORIGINAL, seed));
         }
         s = r;
         m = rm;
@@ -598,16 +752,50 @@ static v3 sample_pixel(const vo_render_scene *sc, const vo_frame *fr, const cam_
     v3 radiance = V(0, 0, 0);
     surf_t s;
     uint32_t face;
-    if (!trace(sc, cam->eye, dir, ray_t_min(cam->eye), &s, &face)) return radiance;
+    /* A camera ray that leaves the scene sees the environment's exact colour (environment_rgb). */
+    if (!trace(sc, cam->eye, dir, ray_t_min(cam->eye), &s, &face)) return eval_environment(sc, dir);
     mat_t m = mat_of(sc, s.slot);
     if (m.type == 3) return s.front ? m.emission : V(0, 0, 0);
     if (!is_delta(&m) && fr->sampling_mode != 3u) {
         rng_event(&seed, 1u, Z_NEE);
-        radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), fr->sampling_mode <= 1u && limit > 0, &seed);
+        radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), fr->sampling_mode <= 1u && limit > 0,
+                       limit > 0 ? ENV_COMPENSATED : ENV_ORIGINAL, &seed);
     }
     radiance = add(radiance, continue_path(sc, fr, &l, s, m, dir, &seed));
     if (!(isfinite(radiance.x) && isfinite(radiance.y) && isfinite(radiance.z))) radiance = V(0, 0, 0);
     return radiance;
+}
+
+/* The environment on its own, for tests: for n directions (3 floats each), the radiance (rgb, strength
+ * applied) and the light-sample density of `table` (importedProbability applied). */
+void SUFFIX(vo_env_query)(const vo_render_scene *sc, const float *dirs, uint32_t n, uint32_t table, float *rgb,
+                          float *pdf) {
+    for (uint32_t i = 0; i < n; ++i) {
+        v3 d = ld(dirs + 3 * (size_t)i);
+        v3 c = eval_environment(sc, d);
+        rgb[3 * (size_t)i] = (float)c.x;
+        rgb[3 * (size_t)i + 1] = (float)c.y;
+        rgb[3 * (size_t)i + 2] = (float)c.z;
+        pdf[i] = (float)env_pdf(sc, d, table);
+    }
+}
+/* n light samples at point p from the PCG stream `seed` (sample_direct_light with `table`): direction,
+ * solid-angle pdf, emission rgb and light id (0 none, 1 environment, face + 2). */
+void SUFFIX(vo_light_samples)(const vo_render_scene *sc, const float *p, uint32_t n, uint32_t seed, uint32_t table,
+                              float *wi, float *pdf, float *rgb, uint32_t *light) {
+    rng_t r = {seed, 0u, 0u, 0, 0u, 0u};
+    const v3 q = ld(p);
+    for (uint32_t i = 0; i < n; ++i) {
+        light_sample ls = sample_direct_light(sc, q, &r, table);
+        wi[3 * (size_t)i] = (float)ls.wi.x;
+        wi[3 * (size_t)i + 1] = (float)ls.wi.y;
+        wi[3 * (size_t)i + 2] = (float)ls.wi.z;
+        pdf[i] = (float)ls.pdf;
+        rgb[3 * (size_t)i] = (float)ls.emission.x;
+        rgb[3 * (size_t)i + 1] = (float)ls.emission.y;
+        rgb[3 * (size_t)i + 2] = (float)ls.emission.z;
+        light[i] = ls.light;
+    }
 }
 
 void SUFFIX(vo_render)(const vo_render_scene *sc, const vo_frame *frame, uint32_t sample_index0, uint32_t samples,
@@ -698,12 +886,12 @@ static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const 
     v3 radiance = V(0, 0, 0);
     surf_t s;
     uint32_t face;
-    if (!trace(sc, cam->eye, dir, ray_t_min(cam->eye), &s, &face)) return radiance;
+    if (!trace(sc, cam->eye, dir, ray_t_min(cam->eye), &s, &face)) return eval_environment(sc, dir);
     mat_t m = mat_of(sc, s.slot);
     if (m.type == 3) return s.front ? m.emission : V(0, 0, 0);
     if (!is_delta(&m)) {
         rng_event(&seed, 1u, Z_NEE);
-        radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), limit > 0, &seed);
+        radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), limit > 0, limit > 0 ? ENV_COMPENSATED : ENV_ORIGINAL, &seed);
     }
     s4 thr = S4(1);
     int sdepth = 0;
@@ -722,28 +910,13 @@ static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const 
         v3 o = ray_origin(s.p, s.ng, nd, s.error);
         dir = nd;
         surf_t r;
-        if (!trace(sc, o, nd, ray_t_min(o), &r, &face)) break;
+        if (!trace(sc, o, nd, ray_t_min(o), &r, &face)) {
+            radiance = add(radiance, env_miss(sc, &l, thr, nd, prev_delta, !prev_delta, 1, bpdf));
+            break;
+        }
         mat_t rm = mat_of(sc, r.slot);
         if (rm.type == 3) {
-            tri_t t = tri_of(sc, face);
-            real g = imported_geometry(prev, r.p, &t);
-            real light_pdf = 0;
-            if (sc->tree) {
-                uint32_t lo = 0, hi = sc->tree->count;
-                while (lo < hi) {
-                    uint32_t mid = (lo + hi) / 2;
-                    if (sc->tree->ids[mid] < face) lo = mid + 1;
-                    else hi = mid;
-                }
-                if (lo < sc->tree->count && sc->tree->ids[lo] == face && g > (real)1e-12) {
-                    float fp[3] = {f_of(prev.x), f_of(prev.y), f_of(prev.z)};
-                    real area = (real)0.5 * len(cross(sub(t.b, t.a), sub(t.c, t.a)));
-                    light_pdf = area > 0 ? (real)vo_light_tree_pmf(sc->tree, fp, sc->tree->positions[lo]) / area / g : 0;
-                }
-            } else if (sc->emitter_count && sc->emitter_total > 0 && rm.weight > 0 && g > (real)1e-12) {
-                light_pdf = (rm.weight / (real)sc->emitter_total) / g;
-            }
-            real w = emission_weight(prev_delta, !prev_delta, 1, bpdf, light_pdf);
+            real w = emission_weight(prev_delta, !prev_delta, 1, bpdf, hit_light_pdf(sc, prev, &r, face, &rm));
             v3 e = r.front ? rm.emission : V(0, 0, 0);
             radiance = add(radiance, spectrum_rgb(sc, &l, sscale(smul(thr, emission_spectrum(sc, &l, e)), w)));
             break;
@@ -769,7 +942,8 @@ static v3 sample_pixel_nrc(const vo_render_scene *sc, const vo_frame *fr, const 
         }
         if (!is_delta(&rm)) {
             rng_event(&seed, (uint32_t)bounce + 1u, Z_NEE);
-            radiance = add(radiance, nee(sc, &l, &r, &rm, neg(dir), thr, sdepth < limit, &seed));
+            radiance = add(radiance, nee(sc, &l, &r, &rm, neg(dir), thr, sdepth < limit,
+                                         sdepth < limit ? ENV_COMPENSATED : ENV_ORIGINAL, &seed));
         }
         s = r;
         m = rm;
@@ -855,7 +1029,11 @@ float vo_half_round(float x) {
 #endif
 
 /* light_geometry for an imported emitter (main.swift:3165-3166). */
+/* light_geometry (main.swift:3164-3167): 1 for the environment (a direction), the imported
+ * emitter's cosine over distance squared otherwise.  A stored environment sample holds its direction
+ * in `position` (restir_di_encode :4714). */
 static real light_geometry(const vo_render_scene *sc, v3 p, const di_sample *c) {
+    if (c->light == 1u) return 1;
     if (c->light < 2u) return 0;
     tri_t t = tri_of(sc, c->light - 2u);
     return imported_geometry(p, c->position, &t);
@@ -868,14 +1046,18 @@ static s4 diffuse_f(const vo_render_scene *sc, const lanes_t *l, const mat_t *m,
     return dot(s->n, wi) > 0 ? sscale(reflectance(sc, l, m->lagrange), (real)1 / PI_R) : S4(0);
 }
 /* eval_restir_target_pdf (main.swift:3187-3196): |rgb(f L cos)| G at the sample's own wavelengths. */
+/* light_spectrum's RGB input (main.swift:2181-2186): an environment sample re-reads the map. */
+static v3 di_emission(const vo_render_scene *sc, const di_sample *c) {
+    return c->light == 1u ? eval_environment(sc, c->position) : c->emission;
+}
 static real target_pdf(const vo_render_scene *sc, const surf_t *s, const mat_t *m, v3 raydir, const di_sample *c) {
     if (c->pdf <= 0) return 0;
-    v3 dir = norm(sub(c->position, s->p));
+    v3 dir = c->light == 1u ? c->position : norm(sub(c->position, s->p));
     real ct = RFMAX((real)0, dot(s->n, dir));
     if (ct <= 0) return 0;
     lanes_t l = lanes_of(sc, c->u);
     s4 f = diffuse_f(sc, &l, m, s, neg(raydir), dir);
-    v3 rgb = spectrum_rgb(sc, &l, sscale(smul(f, emission_spectrum(sc, &l, c->emission)), ct));
+    v3 rgb = spectrum_rgb(sc, &l, sscale(smul(f, emission_spectrum(sc, &l, di_emission(sc, c))), ct));
     return len(rgb) * light_geometry(sc, s->p, c);
 }
 static di_sample stored_sample(const di_texel *t) {
@@ -976,8 +1158,8 @@ static gi_res gi_initial(const vo_render_scene *sc, const vo_frame *fr, const su
     if (m2.type != 0u) return r;
     s4 rad = S4(0);
     rng_event(seed, 2u, Z_GI_NEE);
-    light_sample ls = sample_direct_light(sc, s2.p, seed);
-    if (ls.pdf > 0 && light_visible(sc, &s2, ls.wi, ls.position)) {
+    light_sample ls = sample_direct_light(sc, s2.p, seed, limit >= 2 ? ENV_COMPENSATED : ENV_ORIGINAL);
+    if (ls.pdf > 0 && light_visible_l(sc, &s2, ls.wi, ls.position, ls.light)) {
         real c2 = RFMAX((real)0, dot(s2.n, ls.wi));
         real pdf2;
         s4 f2 = eval_bsdf_pdf(sc, &l, &s2, &m2, neg(gdir), ls.wi, &pdf2);
@@ -1043,8 +1225,8 @@ static void restir_pass1(const vo_render_scene *sc, const vo_frame *fr, const ca
     real wsum = 0, M = 0;
     for (uint32_t i = 0; i < candidates; ++i) {
         if (seed.z) rng_candidate(&seed, 1u, Z_DI, i, bits);
-        light_sample ls = sample_direct_light(sc, s->p, &seed);
-        di_sample c = {ls.position, ls.emission, ls.light, ls.pdf, u};
+        light_sample ls = sample_direct_light(sc, s->p, &seed, fr->max_depth > 1 ? ENV_COMPENSATED : ENV_ORIGINAL);
+        di_sample c = {ls.light == 1u ? ls.wi : ls.position, ls.emission, ls.light, ls.pdf, u};
         if (vo_get_fault() != 4 || c.pdf > 0) M += 1;
         if (c.pdf > 0) {
             real ph = target_pdf(sc, s, &m, dir, &c);
@@ -1138,7 +1320,7 @@ static v3 restir_pass2(const vo_render_scene *sc, const vo_frame *fr, uint32_t g
     seed.pcg = vo_pcg_hash(seed.pcg ^ 0x27d4eb2fu);
     const di_texel *c = cur + pix;
     v3 radiance = V(0, 0, 0);
-    if (c->gpos[3] <= 0.0f) return radiance;
+    if (c->gpos[3] <= 0.0f) return eval_environment(sc, prim[pix].dir);
     const surf_t s = prim[pix].s;
     const v3 dir = prim[pix].dir;
     mat_t m = mat_of(sc, s.slot);
@@ -1166,12 +1348,12 @@ static v3 restir_pass2(const vo_render_scene *sc, const vo_frame *fr, uint32_t g
         real fp = target_pdf(sc, &s, &m, dir, &sel);
         real W = (M > 0 && fp > 0) ? wsum / (M * fp) : 0;
         if (W > 0) {
-            v3 wi = norm(sub(sel.position, s.p));
-            if (light_visible(sc, &s, wi, sel.position)) {
+            v3 wi = sel.light == 1u ? sel.position : norm(sub(sel.position, s.p));
+            if (light_visible_l(sc, &s, wi, sel.position, sel.light)) {
                 real ct = RFMAX((real)0, dot(s.n, wi));
                 lanes_t sl = lanes_of(sc, sel.u);
                 s4 f = diffuse_f(sc, &sl, &m, &s, neg(dir), wi);
-                s4 v = sscale(sscale(smul(sscale(f, ct), emission_spectrum(sc, &sl, sel.emission)), W),
+                s4 v = sscale(sscale(smul(sscale(f, ct), emission_spectrum(sc, &sl, di_emission(sc, &sel))), W),
                               light_geometry(sc, s.p, &sel));
                 radiance = add(radiance, spectrum_rgb(sc, &sl, v));
             }
@@ -1226,7 +1408,7 @@ static v3 restir_pass2(const vo_render_scene *sc, const vo_frame *fr, uint32_t g
         }
     } else if (!is_delta(&m)) {
         rng_event(&seed, 1u, Z_NEE);
-        radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), limit > 0, &seed);
+        radiance = nee(sc, &l, &s, &m, neg(dir), S4(1), limit > 0, limit > 0 ? ENV_COMPENSATED : ENV_ORIGINAL, &seed);
     }
     radiance = add(radiance, continue_path_gi(sc, fr, &l, s, m, dir, &seed,
                                               m.type == 0u && fr->indirect_reuse == 0u && limit >= 1));

@@ -10,4 +10,7 @@
 #define RFMAX fmax
 #define RFMIN fmin
 #define RFLOOR floor
+#define RACOS acos
+#define RATAN2 atan2
+#define RNEXTAFTER nextafter
 #include "integrator.c"

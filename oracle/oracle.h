@@ -148,7 +148,19 @@ typedef struct vo_render_scene {
     const float *cmf;               /* 471 x 4: rgb CMF, cos phase */
     const float *basis;             /* 471 x 2 x 4: R G B D65 | C M Y 0 */
     const vo_light_view *tree;      /* light tree, or NULL for power sampling */
+    /* Environment map (lat-long rgba32f, env_w x env_h, row 0 at the zenith) with its sampling tables
+     * (vo_env_tables), and the frame's Uniforms.environment: strength (x) and rotation in radians (y).
+     * env NULL or strength <= 0: no environment light. */
+    const float *env, *env_rows, *env_cols;
+    uint32_t env_w, env_h;
+    float env_strength, env_rotation;
 } vo_render_scene;
+
+/* MaterialLibrary.environmentImportance (Sources/StudioModel.swift:1300-1380): the sampling tables of
+ * a width x height rgba32f lat-long map.  rows: 2 * height floats, rows[2 y + table]; cols:
+ * 2 * height * width floats, cols[(table * height + y) * width + x]; table 0 is the MIS-compensated
+ * table, 1 the original.  Returns 0, or -1 for an empty map or no memory. */
+int vo_env_tables(const float *rgba, uint32_t width, uint32_t height, float *rows, float *cols);
 
 typedef struct vo_frame {
     uint32_t width, height;
@@ -174,6 +186,14 @@ void vo_render_f64(const vo_render_scene *sc, const vo_frame *fr, uint32_t sampl
  * pass 1 (RIS over `candidates` light samples + temporal reuse of the previous sample's reservoirs)
  * then pass 2 (spatial reuse, shading, path suffix); the first sample has no history.  eyes: NULL
  * (fixed camera) or 3 floats per sample.  out_rgb: width * height * 3 per-pixel means. */
+/* Environment queries for tests (integrator.c): radiance and light-sample density per direction, and
+ * light samples (direction, pdf, emission, light id) from a PCG stream. */
+void vo_env_query_f32(const vo_render_scene *sc, const float *dirs, uint32_t n, uint32_t table, float *rgb, float *pdf);
+void vo_env_query_f64(const vo_render_scene *sc, const float *dirs, uint32_t n, uint32_t table, float *rgb, float *pdf);
+void vo_light_samples_f32(const vo_render_scene *sc, const float *p, uint32_t n, uint32_t seed, uint32_t table,
+                          float *wi, float *pdf, float *rgb, uint32_t *light);
+void vo_light_samples_f64(const vo_render_scene *sc, const float *p, uint32_t n, uint32_t seed, uint32_t table,
+                          float *wi, float *pdf, float *rgb, uint32_t *light);
 void vo_render_restir_f32(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
                           uint32_t candidates, const float *eyes, int threads, double *out_rgb);
 void vo_render_restir_f64(const vo_render_scene *sc, const vo_frame *fr, uint32_t sample_index0, uint32_t samples,
@@ -218,7 +238,9 @@ void vo_render_nrc_f64(const vo_render_scene *sc, const vo_frame *fr, uint32_t s
  * 1 = drop the FMA tie-break of watertight_edge, 2 = swap b1/b2, 3 = ReSTIR spatial reuse leaves
  * the neighbours' confidence out of M, 4 = ReSTIR RIS counts only candidates with pdf > 0 in M,
  * 5 = the GGX reflection lobe drops the shadowing term G1(wi), 10 = ReSTIR GI keeps its source pdf in
- * solid-angle measure.  9 is a test switch, not a mistake: ReSTIR GI without its spatial taps. */
+ * solid-angle measure, 11 = the environment's cell density drops its 1 / sin(theta), 12 = every
+ * environment light sample uses the compensated table (also where no BSDF continuation covers its
+ * zeros).  9 is a test switch, not a mistake: ReSTIR GI without its spatial taps. */
 void vo_set_fault(int fault);
 int vo_get_fault(void);
 /* Nearest half-precision value of x (ties to even): what an rgba16f texel store keeps. */

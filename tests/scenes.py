@@ -199,3 +199,54 @@ def vt_materials(mats):
     import vt
     return [vt.material(m["type"], m["albedo"], m.get("emission", (0, 0, 0)), m.get("roughness", 0.5),
                         m.get("ior", 1.5), m.get("metalness", 0.0), 1.0, m.get("transmission", 0.0)) for m in mats]
+
+
+def sky_map(width=64, height=32, sun=(0.3, 0.28), sun_radiance=300.0, seed=2):
+    """A lat-long rgba32f sky (row 0 at the zenith): blue toward the zenith and whiter toward the
+    horizon, a dark brown ground below it, a bright 2 x 2-texel sun at (u, v) = `sun`, and mild
+    per-texel variation."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(height, dtype=np.float64) + 0.5) / height
+    up = np.cos(v * np.pi)[:, None] * np.ones((1, width))
+    sky = np.stack([0.35 + 0.5 * (1 - up), 0.55 + 0.4 * (1 - up), np.full_like(up, 1.0)], -1)
+    ground = np.broadcast_to(np.array([0.18, 0.12, 0.08]), sky.shape)
+    img = np.where((up > 0)[..., None], sky, ground) * rng.uniform(0.85, 1.15, (height, width, 1))
+    sx, sy = int(sun[0] * width), int(sun[1] * height)
+    img[sy:sy + 2, sx:sx + 2] = (sun_radiance, 0.9 * sun_radiance, 0.75 * sun_radiance)
+    out = np.zeros((height, width, 4), np.float32)
+    out[..., :3] = img
+    out[..., 3] = 1.0
+    return out
+
+
+def environment_scene(emitter=True, seed=4):
+    """An open scene for environment lighting: a grey ground quad at y = -1 (tag 0), six diffuse
+    icospheres (tag 1) and, with `emitter`, a small downward ceiling light at y = 1.6 (tag 2)."""
+    import vt
+    ground = np.array([[[-6, -1, -6], [6, -1, 6], [6, -1, -6]], [[-6, -1, -6], [-6, -1, 6], [6, -1, 6]]], np.float32)
+    gv, gf = mesh_from_triangles(ground)
+    spheres = vt.icosphere_field(spheres=6, subdivisions=2, mu=-0.7, sigma=0.2, half_extent=1.4, seed=seed)
+    sf = spheres.faces.copy()
+    sf[:, 3] = 1
+    parts = [(spheres.vertices, sf)]
+    mats = [dict(type=0, albedo=(0.6, 0.6, 0.6)), dict(type=0, albedo=(0.7, 0.45, 0.3))]
+    if emitter:
+        parts.append(_ceiling_quad(-0.3, -0.3, 0.6, 1.6, 2))
+        mats.append(dict(type=3, albedo=(0, 0, 0), emission=(6.0, 6.0, 6.0)))
+    return _append(vt.Mesh(gv, gf), parts, mats, [])
+
+
+def furnace_sphere():
+    """One unit icosphere at the origin with flat shading (vertex normals zero, so the shading normal is
+    the geometric one and the convex body never sees itself), tag 0."""
+    import vt
+    s = vt.icosphere_field(spheres=1, subdivisions=3, mu=0.0, sigma=0.0, half_extent=1e-3, seed=1)
+    v = s.vertices.copy()
+    v[:, 3:6] = 0.0
+    f = s.faces.copy()
+    f[:, 3] = 0
+    return vt.Mesh(v, f)
+
+
+ENV_EYE = (0.0, 1.2, -6.5)
+ENV_TARGET = (0.0, -0.3, 0.0)
