@@ -236,6 +236,24 @@ vt_status vt_light_tree_build_host(const float *tris, const double *power, uint3
                                    uint32_t *node_count, uint32_t **order, uint32_t **leaf_node, uint32_t *depth,
                                    float grid_lo[3], float grid_hi[3]);
 
+/* Replaces MaterialLibrary.makeEnvironment + publishEnvironment / setEnvironment
+ * (Sources/StudioModel.swift:1399-1470): the scene's environment map, `width` x `height` linear rgba32f
+ * pixels in latitude-longitude layout (row 0 at the zenith, +y up; column u at azimuth
+ * atan2(z, x) = 2 pi (u - 1/2)), copied to the device with its sampling tables (the reference's
+ * environmentImportance, see vt_environment_tables).  The reference's checks and messages: at most
+ * 16384 x 8192 and 33,554,432 pixels ("Could not decode the environment image (maximum 16384×8192)."),
+ * finite pixels ("Environment pixels must be finite."), negative values clamped to 0, and the memory
+ * budget (24 bytes per pixel + 8 per row: "Environment image exceeds the safe GPU memory budget.").
+ * rgba NULL removes the map.  The map lights a frame whose environment_strength is positive. */
+vt_status vt_scene_set_environment(vt_scene *scene, const float *rgba, uint32_t width, uint32_t height);
+
+/* The environment's sampling tables on the host (MaterialLibrary.environmentImportance,
+ * Sources/StudioModel.swift:1300-1380): rows (2 x height floats, rows[2 y + table]) holds each table's
+ * marginal row CDF, cols (2 x height x width floats, cols[(table height + y) width + x]) the conditional
+ * column CDFs; table 0 is the MIS-compensated table (max(0, L - mean L)), 1 the luminance itself, both
+ * over the 3 x 3-dilated luminance times sin(theta). */
+vt_status vt_environment_tables(const float *rgba, uint32_t width, uint32_t height, float *rows, float *cols);
+
 /* Per-call frame parameters: the subset of Uniforms (main.swift:393-420) this path reads. */
 typedef struct vt_frame_params {
     uint32_t width, height;
@@ -275,6 +293,12 @@ typedef struct vt_frame_params {
                                     reference's values 1 and 2, ReSTIR PT, are not built here).  GI
                                     is off with depth 1 and with the radiance cache, which answers
                                     that vertex's light itself */
+    float environment_strength;  /* Uniforms.environment.x: the scene's environment map (set with
+                                    vt_scene_set_environment) lights the frame when positive: camera
+                                    misses see it, continuations that leave the scene gather it with
+                                    MIS, and light samples choose it with probability 1/2 when the scene
+                                    also has emitters (imported_light_probability); 0 = black */
+    float environment_rotation;  /* Uniforms.environment.y: azimuthal rotation of the map, radians */
 } vt_frame_params;
 
 /* Replaces PathTracerRenderer (main.swift:9514) for this path: owns the accumulation buffer. */

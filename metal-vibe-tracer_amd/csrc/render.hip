@@ -51,6 +51,14 @@ __device__ __forceinline__ void rand_f3(uint32_t &seed, float &a, float &b, floa
     b = rand_f(seed);
     c = rand_f(seed);
 }
+__device__ __forceinline__ void rand_f2(uint32_t &seed, float &a, float &b) {
+    a = rand_f(seed);
+    b = rand_f(seed);
+}
+__device__ __forceinline__ void rand_f2(Rng &r, float &a, float &b) {
+    if (r.z) r.zs.rand2(a, b);
+    else rand_f2(r.pcg, a, b);
+}
 __device__ __forceinline__ void rand_f3(Rng &r, float &a, float &b, float &c) {
     if (r.z) r.zs.rand3(a, b, c);
     else rand_f3(r.pcg, a, b, c);
@@ -624,14 +632,159 @@ __device__ __forceinline__ float lt_pmf(const RenderScene &sc, V3 p, uint32_t po
     return 0.0f;
 }
 
+// light_visible's shadow ray (main.swift:3254-3264) from surface s toward a light sample: to the point,
+// stopped short by the endpoint tolerance, or toward the environment (light 1) along wi for 1e6 less
+// twice the ray epsilon.  False when the sample is too close to need one (it is then not visible).
+__device__ __forceinline__ bool shadow_ray(const Surface &s, uint32_t light, V3 wi, V3 position, V3 &o, V3 &d,
+                                           float &tmin, float &tmax) {
+    o = ray_origin(s.p, s.ng, wi, s.error);
+    V3 delta;
+    float dist, tol;
+    if (light == 1u) {
+        delta = wi;
+        dist = 1e6f;
+        tol = 2.0f * ray_epsilon(s.p);
+    } else {
+        delta = position - o;
+        dist = length(delta);
+        tol = endpoint_tolerance(o, position, dist);
+    }
+    if (!(dist > tol)) return false;
+    d = normalize(delta);
+    tmin = ray_t_min(o);
+    tmax = dist - tol;
+    return true;
+}
+
 struct LightSample {
     V3 position, wi, emission;
     float dist, pdf, geometry;
-    uint32_t slot, light;  // light: face + 2 (LightSample.isDirectional), 0 = none
+    uint32_t slot, light;  // light: face + 2 (LightSample.isDirectional), 1 = the environment, 0 = none
 };
+
+// ---------------------------------------------------------------- environment map (main.swift:2077-2135, 2700-2719, 2880-2949)
+// An image environment without a sun, in an imported scene: lat-long lookup with bilinear filtering,
+// image sampling from the two tables (kEnvCompensated where a BSDF continuation covers the texels the
+// MIS-compensated table leaves out, kEnvOriginal elsewhere), and the split of light samples between
+// the triangle emitters and the environment.
+constexpr uint32_t kEnvCompensated = 0u, kEnvOriginal = 1u;
+// a / b correctly rounded.  This file is compiled with the hardware's approximate fp32 division; the
+// environment's lookups divide in double (exact once rounded back: 53 >= 2 x 24 + 2 bits) so that a
+// coordinate lands in the texel the host's computation of the same lookup picks.
+__device__ __forceinline__ float div_rn(float a, float b) { return (float)((double)a / (double)b); }
+__device__ __forceinline__ bool env_on(const RenderScene &sc, const FrameArgs &f) {
+    return sc.env != nullptr && f.env_strength > 0.0f;
+}
+// The frame a kernel shades with: with kEnv false the environment is off at compile time, so every
+// environment branch folds away and kernels for frames without a map keep their register budget.
+template <bool kEnv>
+__device__ __forceinline__ FrameArgs frame_env(const FrameArgs &f) {
+    FrameArgs r = f;
+    if (!kEnv) r.env_strength = 0.0f;
+    return r;
+}
+__host__ __forceinline__ bool frame_has_env(const RenderScene &sc, const FrameArgs &f) {
+    return sc.env != nullptr && f.env_strength > 0.0f;
+}
+// imported_light_probability: the chance that a light sample takes the emitters.
+__device__ __forceinline__ float imported_probability(const RenderScene &sc, const FrameArgs &f) {
+    if (sc.emitter_count == 0) return 0.0f;
+    return env_on(sc, f) ? 0.5f : 1.0f;
+}
+// eval_environment: bilinear between texel centres (wrapping in longitude, latitude clamped half a texel
+// inside the map), times the strength.
+__device__ __forceinline__ V3 eval_environment(const RenderScene &sc, const FrameArgs &f, V3 d) {
+    if (!env_on(sc, f)) return v3(0.0f, 0.0f, 0.0f);
+    const float w = (float)sc.env_w, h = (float)sc.env_h;
+    float u = div_rn(atan2f(d.z, d.x), kTwoPi) + 0.5f + div_rn(f.env_rotation, kTwoPi);
+    // Repeat addressing: only the fraction matters (kept in [0, 1], so texel indices stay in range).
+    u = fminf(fmaxf(u - floorf(u), 0.0f), 1.0f);
+    float v = div_rn(acosf(fminf(fmaxf(d.y, -1.0f), 1.0f)), kPi);
+    v = fminf(fmaxf(v, div_rn(0.5f, h)), 1.0f - div_rn(0.5f, h));
+    // Products and sums rounded one by one (no fused multiply-add): next to the sun, texels differ by
+    // orders of magnitude, so a fused rounding of a filter weight shows in the image.  The row
+    // coordinate is kept inside the map (the clamp above in texel units, whatever its rounding), so a
+    // lookup at a pole reads the pole row alone.
+    const float fx = __fmul_rn(u, w) - 0.5f, fy = fminf(fmaxf(__fmul_rn(v, h) - 0.5f, 0.0f), h - 1.0f);
+    const float x0f = floorf(fx), y0f = floorf(fy);
+    const float ax = fx - x0f, ay = fy - y0f;
+    int xa = (int)x0f % (int)sc.env_w;
+    if (xa < 0) xa += (int)sc.env_w;
+    const int xb = xa + 1 == (int)sc.env_w ? 0 : xa + 1;
+    const int ya = min(max((int)y0f, 0), (int)sc.env_h - 1), yb = min(ya + 1, (int)sc.env_h - 1);
+    const float4 t00 = sc.env[(size_t)ya * sc.env_w + xa], t10 = sc.env[(size_t)ya * sc.env_w + xb];
+    const float4 t01 = sc.env[(size_t)yb * sc.env_w + xa], t11 = sc.env[(size_t)yb * sc.env_w + xb];
+    const float bx = 1.0f - ax, by = 1.0f - ay;
+    auto lerp = [](float a, float wa, float b, float wb) { return __fadd_rn(__fmul_rn(a, wa), __fmul_rn(b, wb)); };
+    const V3 top = v3(lerp(t00.x, bx, t10.x, ax), lerp(t00.y, bx, t10.y, ax), lerp(t00.z, bx, t10.z, ax));
+    const V3 bot = v3(lerp(t01.x, bx, t11.x, ax), lerp(t01.y, bx, t11.y, ax), lerp(t01.z, bx, t11.z, ax));
+    return v3(lerp(top.x, by, bot.x, ay), lerp(top.y, by, bot.y, ay), lerp(top.z, by, bot.z, ay)) * f.env_strength;
+}
+// environment_cell_pdf: the solid-angle density of texel (x, y) in `table`.
+__device__ __forceinline__ float env_cell_pdf(const RenderScene &sc, uint32_t x, uint32_t y, float sin_theta,
+                                              uint32_t table) {
+    const uint32_t cond = y + table * sc.env_h;
+    const float row = sc.env_rows[2 * y + table], prev_row = y == 0 ? 0.0f : sc.env_rows[2 * (y - 1) + table];
+    const float *c = sc.env_cols + (size_t)cond * sc.env_w;
+    const float col = c[x], prev_col = x == 0 ? 0.0f : c[x - 1];
+    const float p = fmaxf(0.0f, row - prev_row) * fmaxf(0.0f, col - prev_col);
+    return div_rn(p * (float)(sc.env_w * sc.env_h), 2.0f * kPi * kPi * fmaxf(1e-5f, sin_theta));
+}
+// eval_environment_pdf: the density of a light sample's environment proposal (importedProbability applied).
+__device__ __forceinline__ float env_pdf(const RenderScene &sc, const FrameArgs &f, V3 wi, uint32_t table) {
+    if (!env_on(sc, f)) return 0.0f;
+    const float theta = acosf(fminf(fmaxf(wi.y, -1.0f), 1.0f));
+    float u = div_rn(atan2f(wi.z, wi.x), kTwoPi) + 0.5f + div_rn(f.env_rotation, kTwoPi);
+    u = fminf(u - floorf(u), 0x1.fffffep-1f);
+    const float v = fminf(fmaxf(div_rn(theta, kPi), 0.0f), 1.0f - 1e-7f);
+    const uint32_t x = min((uint32_t)(u * (float)sc.env_w), sc.env_w - 1u);
+    const uint32_t y = min((uint32_t)(v * (float)sc.env_h), sc.env_h - 1u);
+    return (1.0f - imported_probability(sc, f)) * env_cell_pdf(sc, x, y, sinf(theta), table);
+}
+// environment_cdf_index: the first of `count` CDF entries (entry i at cdf[i * step]) >= value.
+__device__ __forceinline__ uint32_t env_cdf_index(const float *cdf, uint32_t count, uint32_t step, float value) {
+    uint32_t low = 0, high = max(count, 1u) - 1u;
+    for (uint32_t it = 0; it < 16u && low < high; ++it) {
+        const uint32_t mid = (low + high) / 2;
+        if (cdf[(size_t)mid * step] >= value) high = mid;
+        else low = mid + 1;
+    }
+    return low;
+}
+// The largest float below x (> 0): nextafter(x, 0).
+__device__ __forceinline__ float float_below(float x) { return __uint_as_float(__float_as_uint(x) - 1u); }
+// The environment branch of sample_direct_light (4 draws; two 2D constituents in Z mode).
+template <class R>
+__device__ __forceinline__ LightSample sample_environment(const RenderScene &sc, const FrameArgs &f, V3 p, R &seed,
+                                                          uint32_t table) {
+    const uint32_t w = sc.env_w, h = sc.env_h;
+    float cx, cy, ox, oy;
+    rand_f2(seed, cx, cy);
+    const uint32_t y = env_cdf_index(sc.env_rows + table, h, 2u, cx);
+    const uint32_t x = env_cdf_index(sc.env_cols + (size_t)(y + table * h) * w, w, 1u, cy);
+    rand_f2(seed, ox, oy);
+    // The in-texel offset stays inside the chosen cell, whose density is the sample's.
+    const float fw = (float)w, fh = (float)h;
+    const float ux = fminf(div_rn((float)x, fw) + div_rn(ox, fw), float_below(div_rn((float)x + 1.0f, fw)));
+    const float uy = fminf(div_rn((float)y, fh) + div_rn(oy, fh), float_below(div_rn((float)y + 1.0f, fh)));
+    const float theta = uy * kPi, phi = (ux - 0.5f - div_rn(f.env_rotation, kTwoPi)) * kTwoPi;
+    const float st = sinf(theta);
+    LightSample ls;
+    ls.wi = normalize(v3(st * cosf(phi), cosf(theta), st * sinf(phi)));
+    ls.position = p + ls.wi * 1e6f;
+    ls.dist = 1e6f;
+    ls.light = 1u;
+    ls.slot = 0u;
+    ls.geometry = 1.0f;
+    ls.pdf = (1.0f - imported_probability(sc, f)) * env_cell_pdf(sc, x, y, st, table);
+    ls.emission = eval_environment(sc, f, ls.wi);
+    return ls;
+}
+
 // walk(u, pmf): the light-tree walk for uniform u (lt_sample, or lt_sample_top with a pixel's cache).
 template <class Walk, class R>
-__device__ __forceinline__ LightSample sample_emitter_via(const RenderScene &sc, V3 p, R &seed, const Walk &walk) {
+__device__ __forceinline__ LightSample sample_emitter_via(const RenderScene &sc, const FrameArgs &f, V3 p, R &seed,
+                                                          uint32_t table, const Walk &walk) {
     LightSample ls;
     ls.pdf = 0.0f;
     ls.slot = 0;
@@ -641,9 +794,10 @@ __device__ __forceinline__ LightSample sample_emitter_via(const RenderScene &sc,
     ls.position = p;
     ls.wi = v3(0.0f, 0.0f, 1.0f);
     ls.dist = 0.0f;
-    if (sc.emitter_count == 0) return ls;
-    // importedProbability = 1 (no environment): the draw is still consumed.
-    if (!(rand_f(seed) < 1.0f)) return ls;
+    // The emitters with probability importedProbability (its draw is taken whenever it is positive,
+    // even at 1), the environment otherwise.
+    const float pimp = imported_probability(sc, f);
+    if (!(pimp > 0.0f && rand_f(seed) < pimp)) return env_on(sc, f) ? sample_environment(sc, f, p, seed, table) : ls;
     // The emitter choice and the point on it: one 3D constituent in Z mode (main.swift:2886-2894).
     float value, ry, rz;
     rand_f3(seed, value, ry, rz);
@@ -684,15 +838,20 @@ __device__ __forceinline__ LightSample sample_emitter_via(const RenderScene &sc,
     } else {
         area_pdf = sc.emitter_total > 0.0f ? m.weight / sc.emitter_total : 0.0f;
     }
-    ls.pdf = geometry > 1e-12f ? area_pdf / geometry : 0.0f;
+    ls.pdf = geometry > 1e-12f ? pimp * area_pdf / geometry : 0.0f;
     return ls;
 }
 template <class R>
-__device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, V3 p, R &seed) {
-    return sample_emitter_via(sc, p, seed, [&](float u, float &pmf) { return lt_sample(sc, p, u, pmf); });
+__device__ __forceinline__ LightSample sample_emitter(const RenderScene &sc, const FrameArgs &f, V3 p, R &seed,
+                                                      uint32_t table) {
+    return sample_emitter_via(sc, f, p, seed, table, [&](float u, float &pmf) { return lt_sample(sc, p, u, pmf); });
 }
-__device__ __forceinline__ float emitter_pdf(const RenderScene &sc, V3 p, V3 position, uint32_t face, float weight) {
+// eval_light_pdf of an imported emitter face reached from p (main.swift:3070-3075): importedProbability
+// x the area density over the geometry term.
+__device__ __forceinline__ float emitter_pdf(const RenderScene &sc, const FrameArgs &f, V3 p, V3 position, uint32_t face,
+                                             float weight) {
     if (sc.emitter_count == 0) return 0.0f;
+    const float pimp = imported_probability(sc, f);
     V3 a, b, c;
     uint32_t tag;
     const float *va, *vb, *vc;
@@ -709,10 +868,10 @@ __device__ __forceinline__ float emitter_pdf(const RenderScene &sc, V3 p, V3 pos
         }
         if (lo >= sc.emitter_count || sc.tree_ids[lo] != face) return 0.0f;
         float area = 0.5f * length(cross(b - a, c - a));
-        return area > 0.0f ? lt_pmf(sc, p, sc.tree_pos[lo]) / area / g : 0.0f;
+        return area > 0.0f ? pimp * (lt_pmf(sc, p, sc.tree_pos[lo]) / area) / g : 0.0f;
     }
     if (!(sc.emitter_total > 0.0f) || !(weight > 0.0f)) return 0.0f;
-    return (weight / sc.emitter_total) / g;
+    return pimp * (weight / sc.emitter_total) / g;
 }
 
 // ---------------------------------------------------------------- ReSTIR DI (main.swift:3165-3196, 4618-4716, 4812-4835, 5015-5033)
@@ -759,6 +918,7 @@ __device__ __forceinline__ DiSample di_stored(float4 posdir, float4 emitpdf, flo
 }
 // light_geometry of an imported emitter (main.swift:3165-3166).
 __device__ __forceinline__ float light_geometry(const RenderScene &sc, V3 p, const DiSample &d) {
+    if (d.light == 1u) return 1.0f;  // the environment: a direction
     if (d.light < 2u || d.light - 2u >= sc.face_count) return 0.0f;
     V3 a, b, c;
     uint32_t tag;
@@ -774,19 +934,26 @@ __device__ __forceinline__ S4 diffuse_f(const Surface &s, S4 rho_pi, V3 wo, V3 w
     return dot(s.n, wi) > 0.0f ? rho_pi : s4(0.0f);
 }
 // eval_restir_target_pdf (main.swift:3187-3196): |rgb(f L cos)| G at the sample's wavelengths
-// (lanes and rho_pi belong to d.u; g is light_geometry(p, d)).
+// (lanes and rho_pi belong to d.u; g is light_geometry(p, d)).  An environment sample (light 1) holds
+// its direction in `position` (restir_di_encode, main.swift:4714).
 __device__ __forceinline__ float target_pdf(const Spectral &sp, const Lanes &l, S4 rho_pi, const Surface &s, V3 raydir,
                                             const DiSample &d, float g) {
     if (d.pdf <= 0.0f) return 0.0f;
-    V3 dir = normalize(d.position - s.p);
+    V3 dir = d.light == 1u ? d.position : normalize(d.position - s.p);
     float ct = fmaxf(0.0f, dot(s.n, dir));
     if (ct <= 0.0f) return 0.0f;
     S4 f = diffuse_f(s, rho_pi, -raydir, dir);
     return length(spectrum_rgb(sp, l, scale(mul(f, emission_spectrum(sp, l, d.emission)), ct))) * g;
 }
-__device__ __forceinline__ float target_pdf_at(const RenderScene &sc, const Spectral &sp, const Mat &m, const Surface &s,
-                                               V3 raydir, const DiSample &d) {
+// The DI sample's emission as light_spectrum reads it (main.swift:2181-2186): the environment re-reads
+// the map in the sample's direction.
+__device__ __forceinline__ V3 di_emission(const RenderScene &sc, const FrameArgs &f, const DiSample &d) {
+    return d.light == 1u ? eval_environment(sc, f, d.position) : d.emission;
+}
+__device__ __forceinline__ float target_pdf_at(const RenderScene &sc, const FrameArgs &f, const Spectral &sp, const Mat &m,
+                                               const Surface &s, V3 raydir, DiSample d) {
     if (d.pdf <= 0.0f) return 0.0f;
+    d.emission = di_emission(sc, f, d);
     const Lanes l = lanes_of(sp, d.u);
     return target_pdf(sp, l, scale(reflectance(sp, l, m.lagrange), 1.0f / kPi), s, raydir, d, light_geometry(sc, s.p, d));
 }
@@ -940,8 +1107,12 @@ __global__ __launch_bounds__(256) void generate_kernel(FrameArgs f, PathQueue q,
 // reservoir: RIS over ra.candidates light samples on the pixel's un-decorrelated PCG stream
 // (restir_di_initial :4637-4668), then the temporal merge of the backprojected history reservoir
 // (restir_di_merge :4692-4704) and the encode (restir_di_encode :4708-4716).
-__global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, FrameArgs f, PathQueue q, HitArgs hits,
-                                                             RestirArgs ra) {
+// Without the environment the kernel fits 128 registers, 4 waves per SIMD (asked for explicitly: the
+// light-tree walk is latency-bound and a fourth wave is worth more than the few registers it saves).
+template <bool kEnv>
+__global__ __launch_bounds__(256, kEnv ? 1 : 4) void restir_initial_kernel(RenderScene sc, FrameArgs fa, PathQueue q,
+                                                                          HitArgs hits, RestirArgs ra) {
+    const FrameArgs f = frame_env<kEnv>(fa);
     __shared__ LtTop top;
     __shared__ float top_l[kLtTopInner][256], top_r[kLtTopInner][256];
     const bool cached = sc.tree_node_count != 0u;
@@ -968,6 +1139,8 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
             Rng rng{pix ^ (f.sample_index0 * 1999999973u), ZSampler{0u, 0u, 0u, 0u}, f.sampler == 1u};
             if (rng.z) rng.zs.key = z_pixel_key(pix % f.width, pix / f.width, f.frame0 + 1u, f.sample_index0, f.z_temporal);
             const uint32_t bits = ra.candidates > 1u ? 32u - (uint32_t)__clz(ra.candidates - 1u) : 0u;
+            // The primary's BSDF continuation covers the compensated table's zeros unless the depth is 1.
+            const uint32_t table = f.scattering_limit > 0 ? kEnvCompensated : kEnvOriginal;
             const float u = q.wl[pix];
             const Lanes lanes = lanes_of(sp, u);
             const S4 rho_pi = scale(reflectance(sp, lanes, m.lagrange), 1.0f / kPi);
@@ -978,11 +1151,11 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
             for (uint32_t i = 0; i < ra.candidates; ++i) {
                 if (rng.z) rng.zs.candidate(1u, kZDi, i, bits);
                 const LightSample ls =
-                    cached ? sample_emitter_via(sc, sp_p, rng,
+                    cached ? sample_emitter_via(sc, f, sp_p, rng, table,
                                                 [&](float u, float &pmf) { return lt_sample_top<256>(sc, sp_p, u, pmf, top, top_l, top_r); })
-                           : sample_emitter(sc, sp_p, rng);
+                           : sample_emitter(sc, f, sp_p, rng, table);
                 DiSample c;
-                c.position = ls.position;
+                c.position = ls.light == 1u ? ls.wi : ls.position;
                 c.emission = ls.emission;
                 c.light = ls.light;
                 c.pdf = ls.pdf;
@@ -1017,7 +1190,7 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
                                               length(v3(og.x, og.y, og.z) - s.p) < fmaxf(0.01f, t * 0.01f);
                             if (same && hw.y > 0.0f && hw.z > 0.0f) {
                                 const DiSample hs = di_stored(ra.hist.posdir[hp], ra.hist.emitpdf[hp], hw);
-                                const float ph = target_pdf_at(sc, sp, m, s, dir, hs);
+                                const float ph = target_pdf_at(sc, f, sp, m, s, dir, hs);
                                 const float cm = fminf(hw.y, 20.0f);
                                 const float wt = ph * hw.z * cm;
                                 M += cm;
@@ -1028,7 +1201,7 @@ __global__ __launch_bounds__(256) void restir_initial_kernel(RenderScene sc, Fra
                     }
                 }
             }
-            const float ph = target_pdf_at(sc, sp, m, s, dir, sel);
+            const float ph = target_pdf_at(sc, f, sp, m, s, dir, sel);
             const float W = (M > 0.0f && ph > 0.0f) ? wsum / (M * ph) : 0.0f;
             posdir = make_float4(sel.position.x, sel.position.y, sel.position.z, pack_light(sel.light));
             emitpdf = make_float4(sel.emission.x, sel.emission.y, sel.emission.z, sel.pdf);
@@ -1101,25 +1274,14 @@ __global__ __launch_bounds__(256) void restir_gi_secondary_kernel(RenderScene sc
             Rng rng{g.seed[pix], ZSampler{0u, 0u, 0u, 0u}, f.sampler == 1u};
             if (rng.z) rng.zs.key = z_pixel_key(pix % f.width, pix / f.width, f.frame0 + 1u, f.sample_index0, f.z_temporal);
             rng_event(rng, 2u, kZGiNee);
-            const LightSample ls = sample_emitter(sc, x2.p, rng);
+            const LightSample ls =
+                sample_emitter(sc, f, x2.p, rng, f.scattering_limit >= 2 ? kEnvCompensated : kEnvOriginal);
             g.seed[pix] = rng.pcg;
             g.light_wi[pix] = make_float4(ls.wi.x, ls.wi.y, ls.wi.z, ls.pdf);
             g.light_le[pix] = make_float4(ls.emission.x, ls.emission.y, ls.emission.z, 0.0f);
             uint32_t stage = 3u;
-            if (ls.pdf > 0.0f) {
-                const V3 o = ray_origin(x2.p, x2.ng, ls.wi, x2.error);
-                const V3 delta = ls.position - o;
-                const float d = length(delta);
-                const float tol = endpoint_tolerance(o, ls.position, d);
-                if (d > tol) {
-                    so = o;
-                    sd = normalize(delta);
-                    stmin = ray_t_min(o);
-                    stmax = d - tol;
-                    ray = true;
-                    stage = 2u;
-                }
-            }
+            if (ls.pdf > 0.0f) ray = shadow_ray(x2, ls.light, ls.wi, ls.position, so, sd, stmin, stmax);
+            if (ray) stage = 2u;
             g.stage[pix] = stage;
         }
     }
@@ -1297,6 +1459,20 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
         bool alive = face >= 0;
         Surface s;
         Mat m;
+        if (!alive && env_on(sc, f)) {
+            // A ray that leaves the scene: the camera sees the environment's exact colour
+            // (environment_rgb); a continuation gathers it with the MIS weight of its light-sample
+            // density in the compensated table (main.swift:7454, 7751-7758).
+            if (bounce == 0u) {
+                add = eval_environment(sc, f, dir);
+            } else {
+                const float w = emission_weight((state & kPrevDelta) != 0u, (state & kPrevNee) != 0u,
+                                                (state & kPrevMis) != 0u, in.pdf[i], env_pdf(sc, f, dir, kEnvCompensated));
+                const S4 se = scale(mul(thr, emission_spectrum(sp, lanes, eval_environment(sc, f, dir))), w);
+                add = spectrum_rgb(sp, lanes, se);
+                if (f.nrc && nrc_training(f, id)) add_f4(nrc.sacc + id, se);
+            }
+        }
         if (alive) {
             s = surface_at(sc, (uint32_t)face, hits.b1[i], hits.b2[i], dir);
             m = material(sc, s.slot);
@@ -1306,7 +1482,7 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
                     add = e;  // camera rays see the exact RGB (emitter_rgb, D65)
                 } else {
                     const V3 prev = v3(in.px[i], in.py[i], in.pz[i]);
-                    float light_pdf = emitter_pdf(sc, prev, s.p, (uint32_t)face, m.weight);
+                    float light_pdf = emitter_pdf(sc, f, prev, s.p, (uint32_t)face, m.weight);
                     float w = emission_weight((state & kPrevDelta) != 0u, (state & kPrevNee) != 0u,
                                               (state & kPrevMis) != 0u, in.pdf[i], light_pdf);
                     const S4 se = scale(mul(thr, emission_spectrum(sp, lanes, e)), w);
@@ -1337,7 +1513,7 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
             const uint32_t gx = pix % f.width, gy = pix / f.width;
             const float4 cg = ra.cur.gpos[pix], cw = ra.cur.weights[pix];
             DiSample sel = di_stored(ra.cur.posdir[pix], ra.cur.emitpdf[pix], cw);
-            float wsum = target_pdf_at(sc, sp, m, s, dir, sel) * cw.z * cw.y;
+            float wsum = target_pdf_at(sc, f, sp, m, s, dir, sel) * cw.z * cw.y;
             float M = cw.y;
             for (int k = 0; k < 4; ++k) {
                 const float rx = rand_f(rng.pcg), ry = rand_f(rng.pcg);
@@ -1353,31 +1529,23 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
                 const float4 nw = ra.cur.weights[np];
                 if (!(nw.y > 0.0f && nw.z > 0.0f)) continue;
                 const DiSample ns = di_stored(ra.cur.posdir[np], ra.cur.emitpdf[np], nw);
-                const float w = target_pdf_at(sc, sp, m, s, dir, ns) * nw.z * nw.y;
+                const float w = target_pdf_at(sc, f, sp, m, s, dir, ns) * nw.z * nw.y;
                 wsum += w;
                 M += nw.y;
                 if (rand_f(rng.pcg) * wsum < w) sel = ns;
             }
-            const float fp = target_pdf_at(sc, sp, m, s, dir, sel);
+            const float fp = target_pdf_at(sc, f, sp, m, s, dir, sel);
             const float W = (M > 0.0f && fp > 0.0f) ? wsum / (M * fp) : 0.0f;
             if (W > 0.0f) {
                 // Shading with visibility (light_visible): bsdf cos L W G at the sample's wavelengths.
-                const V3 wi = normalize(sel.position - s.p);
-                const V3 o = ray_origin(s.p, s.ng, wi, s.error);
-                const V3 delta = sel.position - o;
-                const float d = length(delta);
-                const float tol = endpoint_tolerance(o, sel.position, d);
-                if (d > tol) {
+                const V3 wi = sel.light == 1u ? sel.position : normalize(sel.position - s.p);
+                if (shadow_ray(s, sel.light, wi, sel.position, sh_o, sh_d, sh_tmin, sh_tmax)) {
                     const float ct = fmaxf(0.0f, dot(s.n, wi));
                     const Lanes sl = lanes_of(sp, sel.u);
                     const S4 fb = diffuse_f(s, scale(reflectance(sp, sl, m.lagrange), 1.0f / kPi), -dir, wi);
-                    const S4 v = scale(scale(mul(scale(fb, ct), emission_spectrum(sp, sl, sel.emission)), W),
+                    const S4 v = scale(scale(mul(scale(fb, ct), emission_spectrum(sp, sl, di_emission(sc, f, sel))), W),
                                        light_geometry(sc, s.p, sel));
                     sh_c = spectrum_rgb(sp, sl, v);  // at the sample's own lanes; not a cache target (primary)
-                    sh_o = o;
-                    sh_d = normalize(delta);
-                    sh_tmin = ray_t_min(o);
-                    sh_tmax = d - tol;
                     shadow = true;
                 }
             }
@@ -1451,27 +1619,25 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
             // Next-event estimation at this non-delta vertex (main.swift:7655-7672, 7790-7805).
             const bool use_mis = f.sampling_mode <= 1u && (bounce == 0u ? f.scattering_limit > 0 : sdepth < f.scattering_limit);
             rng_event(rng, bounce + 1u, kZNee);  // this hit is vertex bounce + 1 (main.swift:7658, 7804)
-            LightSample ls = sample_emitter(sc, s.p, rng);
-            if (ls.pdf > 0.0f) {
-                V3 o = ray_origin(s.p, s.ng, ls.wi, s.error);
-                V3 delta = ls.position - o;
-                float d = length(delta);
-                float tol = endpoint_tolerance(o, ls.position, d);
-                if (d > tol) {
-                    float pdf;
-                    const S4 fbsdf = eval_bsdf_pdf<kLayered>(sc, sp, lanes, s, m, -dir, ls.wi, pdf);
-                    if (smax(fbsdf) > 0.0f) {
-                        const float cosine = fabsf(dot(s.n, ls.wi));
-                        const float w = use_mis ? power_heuristic(ls.pdf, pdf) : 1.0f;
-                        const S4 le = emission_spectrum(sp, lanes, ls.emission);
-                        sh_s = scale(mul(mul(thr, fbsdf), le), cosine * w / ls.pdf);
-                        sh_c = spectrum_rgb(sp, lanes, sh_s);
-                        sh_o = o;
-                        sh_d = normalize(delta);
-                        sh_tmin = ray_t_min(o);
-                        sh_tmax = d - tol;
-                        shadow = true;
-                    }
+            // The environment's compensated table where this vertex's BSDF continuation follows.
+            const bool continues = bounce == 0u ? f.scattering_limit > 0 : sdepth < f.scattering_limit;
+            LightSample ls = sample_emitter(sc, f, s.p, rng, continues ? kEnvCompensated : kEnvOriginal);
+            V3 o, d;
+            float tmin, tmax;
+            if (ls.pdf > 0.0f && shadow_ray(s, ls.light, ls.wi, ls.position, o, d, tmin, tmax)) {
+                float pdf;
+                const S4 fbsdf = eval_bsdf_pdf<kLayered>(sc, sp, lanes, s, m, -dir, ls.wi, pdf);
+                if (smax(fbsdf) > 0.0f) {
+                    const float cosine = fabsf(dot(s.n, ls.wi));
+                    const float w = use_mis ? power_heuristic(ls.pdf, pdf) : 1.0f;
+                    const S4 le = emission_spectrum(sp, lanes, ls.emission);
+                    sh_s = scale(mul(mul(thr, fbsdf), le), cosine * w / ls.pdf);
+                    sh_c = spectrum_rgb(sp, lanes, sh_s);
+                    sh_o = o;
+                    sh_d = d;
+                    sh_tmin = tmin;
+                    sh_tmax = tmax;
+                    shadow = true;
                 }
             }
         }
@@ -1570,13 +1736,13 @@ __device__ __forceinline__ void shade_path(const RenderScene &sc, const FrameArg
     }
 }
 
-template <bool kRestir, bool kLayered>
+template <bool kRestir, bool kLayered, bool kEnv>
 __global__ __launch_bounds__(256) void shade_kernel(RenderScene sc, FrameArgs f, PathQueue in, HitArgs hits,
                                                     PathQueue out, ShadowQueue sh, Counters c, uint32_t capacity,
                                                     float *radiance, RestirArgs ra, NrcArgs nrc) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const uint32_t n = min(*c.in, capacity);
-    shade_path<kRestir, kLayered>(sc, f, in, hits, out, sh, c, capacity, radiance, ra, nrc, i, i < n, 0u);
+    shade_path<kRestir, kLayered>(sc, frame_env<kEnv>(f), in, hits, out, sh, c, capacity, radiance, ra, nrc, i, i < n, 0u);
 }
 // Material sorting: paths whose hit needs the layered BSDF (GLOSSY above the delta threshold, OPENPBR)
 // are listed from the end of `perm`, all others (Lambert, delta, emitters, misses) from its start, both
@@ -1613,12 +1779,14 @@ __global__ __launch_bounds__(256) void shade_sorted_kernel(RenderScene sc, Frame
 
 // The Lambert / delta shading without ReSTIR (a kernel of its own so its register allocation is not
 // shaped by the layered BSDF's).
+template <bool kEnv>
 __global__ __launch_bounds__(256) void shade_lean_kernel(
     RenderScene sc, FrameArgs f, PathQueue in, HitArgs hits, PathQueue out, ShadowQueue sh, Counters c,
     uint32_t capacity, float *radiance, NrcArgs nrc) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const uint32_t n = min(*c.in, capacity);
-    shade_path<false, false>(sc, f, in, hits, out, sh, c, capacity, radiance, RestirArgs{}, nrc, i, i < n, 0u);
+    shade_path<false, false>(sc, frame_env<kEnv>(f), in, hits, out, sh, c, capacity, radiance, RestirArgs{}, nrc, i,
+                             i < n, 0u);
 }
 
 __global__ __launch_bounds__(256) void resolve_kernel(FrameArgs f, ShadowQueue sh, const uint8_t *occluded,
@@ -1782,8 +1950,9 @@ __global__ __launch_bounds__(256) void batch_shade_kernel(BatchTables bt, PathQu
         const uint32_t i = base + lane;
         const bool active = i < n;
         const uint32_t j = active ? in.job[i] : 0u;
-        shade_path<false, kLayered>(bt.scenes[bt.job_scene[j]], bt.frames[j], in, hits, out, sh, c, capacity, radiance,
-                                    RestirArgs{}, NrcArgs{}, i, active, j);
+        // Batched jobs render without environment lighting.
+        shade_path<false, kLayered>(bt.scenes[bt.job_scene[j]], frame_env<false>(bt.frames[j]), in, hits, out, sh, c,
+                                    capacity, radiance, RestirArgs{}, NrcArgs{}, i, active, j);
     }
 }
 
@@ -1838,11 +2007,19 @@ hipError_t launch_shade(const RenderScene &sc, const FrameArgs &f, const PathQue
                         const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
                         float *radiance, const NrcArgs &nrc, hipStream_t s) {
     const dim3 grid((capacity + 255) / 256);
-    if (sc.layered)
-        hipLaunchKernelGGL((shade_kernel<false, true>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
-                           radiance, RestirArgs{}, nrc);
+    const bool env = frame_has_env(sc, f);
+    if (sc.layered && env)
+        hipLaunchKernelGGL((shade_kernel<false, true, true>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c,
+                           capacity, radiance, RestirArgs{}, nrc);
+    else if (sc.layered)
+        hipLaunchKernelGGL((shade_kernel<false, true, false>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c,
+                           capacity, radiance, RestirArgs{}, nrc);
+    else if (env)
+        hipLaunchKernelGGL(shade_lean_kernel<true>, grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
+                           radiance, nrc);
     else
-        hipLaunchKernelGGL(shade_lean_kernel, grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity, radiance, nrc);
+        hipLaunchKernelGGL(shade_lean_kernel<false>, grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
+                           radiance, nrc);
     return hipGetLastError();
 }
 
@@ -1862,18 +2039,28 @@ hipError_t launch_shade_restir(const RenderScene &sc, const FrameArgs &f, const 
                                const PathQueue &out, const ShadowQueue &sh, const Counters &c, uint32_t capacity,
                                float *radiance, const RestirArgs &ra, const NrcArgs &nrc, hipStream_t s) {
     const dim3 grid((capacity + 255) / 256);
-    if (sc.layered)
-        hipLaunchKernelGGL((shade_kernel<true, true>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
-                           radiance, ra, nrc);
+    const bool env = frame_has_env(sc, f);
+    if (sc.layered && env)
+        hipLaunchKernelGGL((shade_kernel<true, true, true>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c,
+                           capacity, radiance, ra, nrc);
+    else if (sc.layered)
+        hipLaunchKernelGGL((shade_kernel<true, true, false>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c,
+                           capacity, radiance, ra, nrc);
+    else if (env)
+        hipLaunchKernelGGL((shade_kernel<true, false, true>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c,
+                           capacity, radiance, ra, nrc);
     else
-        hipLaunchKernelGGL((shade_kernel<true, false>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c, capacity,
-                           radiance, ra, nrc);
+        hipLaunchKernelGGL((shade_kernel<true, false, false>), grid, dim3(256), 0, s, sc, f, in, hits, out, sh, c,
+                           capacity, radiance, ra, nrc);
     return hipGetLastError();
 }
 
 hipError_t launch_restir_initial(const RenderScene &sc, const FrameArgs &f, const PathQueue &q, const HitArgs &hits,
                                  const RestirArgs &ra, hipStream_t s) {
-    hipLaunchKernelGGL(restir_initial_kernel, dim3((f.npix + 255) / 256), dim3(256), 0, s, sc, f, q, hits, ra);
+    if (frame_has_env(sc, f))
+        hipLaunchKernelGGL(restir_initial_kernel<true>, dim3((f.npix + 255) / 256), dim3(256), 0, s, sc, f, q, hits, ra);
+    else
+        hipLaunchKernelGGL(restir_initial_kernel<false>, dim3((f.npix + 255) / 256), dim3(256), 0, s, sc, f, q, hits, ra);
     return hipGetLastError();
 }
 

@@ -45,6 +45,12 @@ struct RenderScene {
     const float *energy_metal;
     const float *energy_luts;
     uint32_t layered;  // some slot is GLOSSY or OPENPBR (the shading kernels then carry the layered BSDF)
+    // Environment map (null: none): env_w x env_h rgba32f texels, row 0 at the zenith; its sampling
+    // tables, rows[2 y + table] (marginal CDFs) and cols[(table env_h + y) env_w + x] (conditional
+    // CDFs), table 0 MIS-compensated, 1 original.
+    const float4 *env;
+    const float *env_rows, *env_cols;
+    uint32_t env_w, env_h;
 };
 
 // ReSTIR DI state per pixel, the reference's G-buffer and reservoir textures (main.swift:4842-4940):
@@ -112,6 +118,8 @@ struct FrameArgs {
     // frame0 + sample + 1 and sample index, temporal model z_temporal).
     uint32_t sampler, z_temporal, frame0;
     float box_lo[3], box_scale[3];  // cache inputs: (p - lo) * scale in [0, 1]
+    // Uniforms.environment: the map's strength (> 0 with a map: environment lighting on) and rotation.
+    float env_strength, env_rotation;
 };
 
 // Queues of the learned radiance cache (indexed by queue slot unless stated).

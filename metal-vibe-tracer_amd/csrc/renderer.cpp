@@ -595,6 +595,12 @@ static vt_status render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_pa
             r->gi_history = false;
         }
         if (p->indirect_reuse > 2) return fail(VT_ERROR_INVALID_ARGUMENT, "indirect_reuse must be 0, 1 or 2");
+        if (!(std::isfinite(p->environment_strength) && p->environment_strength >= 0.0f) ||
+            !std::isfinite(p->environment_rotation))
+            return fail(VT_ERROR_INVALID_ARGUMENT, "environment strength must be finite and >= 0, rotation finite");
+        if (p->environment_strength > 0.0f && !scene->env_image.ptr)
+            return fail(VT_ERROR_UNSUPPORTED,
+                        "environment_strength needs an environment map (vt_scene_set_environment); the procedural sky is not built");
         const bool restir = p->sampling_mode == 0;
         if (restir && !r->restir[0].ptr) {
             r->restir[0].alloc((size_t)72 * npix);
@@ -612,6 +618,8 @@ static vt_status render_frame(vt_renderer *r, vt_scene *scene, const vt_frame_pa
         vt::camera_frame(p->eye, p->target, p->fov_degrees, p->width, p->height, f);
         f.scattering_limit = std::max(0, (int)p->max_depth - 1);
         f.sampling_mode = p->sampling_mode;
+        f.env_strength = p->environment_strength;
+        f.env_rotation = p->environment_rotation;
         if (p->radiance_cache > 2) return fail(VT_ERROR_INVALID_ARGUMENT, "radiance_cache must be 0, 1 or 2");
         const bool use_cache = p->radiance_cache != 0, train_cache = p->radiance_cache == 1;
         if (use_cache && !r->cache) return fail(VT_ERROR_INVALID_ARGUMENT, "radiance_cache needs vt_renderer_set_cache");

@@ -88,6 +88,10 @@ struct vt_scene {
     vt::DeviceBuffer vertices, faces;             // document order
     vt::DeviceBuffer materials, emitter_face, emitter_cdf, icdf, cmf, basis;
     vt::DeviceBuffer tree_nodes, tree_leaves, tree_ids, tree_pos, tree_dec;
+    // Environment map (rgba32f, lat-long) and its sampling tables (vt_scene_set_environment).
+    vt::DeviceBuffer env_image, env_rows, env_cols;
+    uint32_t env_w = 0, env_h = 0;
+    uint64_t env_accounted = 0;
     uint32_t tree_node_count = 0;
     float tree_lo[3] = {0, 0, 0}, tree_scale[3] = {0, 0, 0};
     bool light_tree = true;
@@ -113,7 +117,7 @@ struct vt_scene {
     std::unique_ptr<vt::SpectralTables> spectral;
     uint64_t accounted = 0;
     ~vt_scene() {
-        if (ctx) ctx->resident -= accounted;
+        if (ctx) ctx->resident -= accounted + env_accounted;
     }
     vt::RenderScene render_view() const {
         vt::RenderScene r{};
@@ -139,6 +143,11 @@ struct vt_scene {
         r.energy_metal = ctx->energy_metal.as<const float>();
         r.energy_luts = ctx->energy_luts.as<const float>();
         r.layered = has_layered ? 1u : 0u;
+        r.env = env_image.as<const float4>();
+        r.env_rows = env_rows.as<const float>();
+        r.env_cols = env_cols.as<const float>();
+        r.env_w = env_w;
+        r.env_h = env_h;
         for (int k = 0; k < 3; ++k) {
             r.tree_lo[k] = tree_lo[k];
             r.tree_scale[k] = tree_scale[k];

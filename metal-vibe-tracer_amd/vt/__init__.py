@@ -38,7 +38,7 @@ EXPORTED = [
     "vt_z_draws", "vt_context_check", "vt_scene_set_stack_cap", "vt_mesh_from_triangles",
     "vt_scene_upload_triangles", "vt_scene_set_surface_settings", "vt_renderer_set_callbacks",
     "vt_trace_stats_noalloc", "vt_tlas_create", "vt_tlas_destroy", "vt_tlas_get_info", "vt_tlas_trace_closest",
-    "vt_tlas_trace_any",
+    "vt_tlas_trace_any", "vt_scene_set_environment", "vt_environment_tables",
 ]
 
 
@@ -138,7 +138,8 @@ class _FrameParams(ctypes.Structure):
                 ("restir_candidates", ctypes.c_uint32), ("radiance_cache", ctypes.c_uint32),
                 ("cache_train_period", ctypes.c_uint32), ("cache_learning_rate", ctypes.c_float),
                 ("material_sort", ctypes.c_uint32), ("sampler", ctypes.c_uint32), ("z_temporal", ctypes.c_uint32),
-                ("indirect_reuse", ctypes.c_uint32)]
+                ("indirect_reuse", ctypes.c_uint32), ("environment_strength", ctypes.c_float),
+                ("environment_rotation", ctypes.c_float)]
 
 
 class _RaySoA(ctypes.Structure):
@@ -216,6 +217,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_spectral_tables": (i32, [vp, vp, vp]),
         "vt_spectral_reflectance": (i32, [P(f32), P(f32)]),
         "vt_scene_set_light_tree": (i32, [vp, c.c_int]),
+        "vt_scene_set_environment": (i32, [vp, vp, u32, u32]),
+        "vt_environment_tables": (i32, [vp, u32, u32, vp, vp]),
         "vt_light_tree_build_host": (i32, [vp, vp, u32, P(vp), P(u32), P(vp), P(vp), P(u32), P(f32), P(f32)]),
         "vt_mesh_payload_bytes": (c.c_uint64, [u32, u32]),
         "vt_mesh_payload_write": (i32, [vp, u32, vp, u32, vp, c.c_uint64]),
@@ -262,6 +265,17 @@ def _ptr(a: np.ndarray):
 class Mesh:
     vertices: np.ndarray  # (V, 8) float32: position, normal, uv
     faces: np.ndarray     # (F, 4) uint32: a, b, c, tag
+
+
+def environment_tables(rgba):
+    """vt_environment_tables: the library's sampling tables of an (H, W, 4) float32 map -> (rows (H, 2),
+    cols (2, H, W)): per table (0 MIS-compensated, 1 original) the marginal and conditional CDFs."""
+    img = np.ascontiguousarray(rgba, np.float32)
+    h, w = img.shape[:2]
+    rows = np.zeros((h, 2), np.float32)
+    cols = np.zeros((2, h, w), np.float32)
+    _check(load().vt_environment_tables(_ptr(img), w, h, _ptr(rows), _ptr(cols)))
+    return rows, cols
 
 
 def icosphere_field(spheres=1024, subdivisions=3, mu=0.0, sigma=0.5, half_extent=50.0, seed=7) -> Mesh:
@@ -668,6 +682,16 @@ class Scene:
     def set_light_tree(self, enabled: bool):
         _check(load().vt_scene_set_light_tree(self._h, 1 if enabled else 0))
 
+    def set_environment(self, rgba):
+        """vt_scene_set_environment: (H, W, 4) float32 linear rgba, row 0 at the zenith; None removes it."""
+        if rgba is None:
+            _check(load().vt_scene_set_environment(self._h, None, 0, 0))
+            return
+        img = np.ascontiguousarray(rgba, np.float32)
+        if img.ndim != 3 or img.shape[2] != 4:
+            raise ValueError("environment map must be (height, width, 4)")
+        _check(load().vt_scene_set_environment(self._h, _ptr(img), img.shape[1], img.shape[0]))
+
     def set_stack_cap(self, groups: int):
         """Test hook (vt_scene_set_stack_cap): caps the traversal stack at `groups` node groups (0: none)."""
         _check(load().vt_scene_set_stack_cap(self._h, groups))
@@ -744,10 +768,10 @@ class Renderer:
 
     def render(self, scene: Scene, width, height, eye, target, fov, max_depth=8, samples=1, sample_index=1,
                sampling_mode=1, candidates=4, cache=False, train_period=16, learning_rate=1e-3, material_sort=False,
-               sampler=0, z_temporal=1, indirect_reuse=0, stream=None):
+               sampler=0, z_temporal=1, indirect_reuse=0, env_strength=0.0, env_rotation=0.0, stream=None):
         p = _FrameParams(width, height, (ctypes.c_float * 3)(*eye), (ctypes.c_float * 3)(*target), fov, max_depth,
                          sampling_mode, sample_index, candidates, int(cache), train_period, learning_rate,
-                         int(material_sort), sampler, z_temporal, indirect_reuse)
+                         int(material_sort), sampler, z_temporal, indirect_reuse, env_strength, env_rotation)
         _check(load().vt_render_frame(self._h, scene._h, ctypes.byref(p), samples, _stream_ptr(stream)))
         self.size = (width, height)
 

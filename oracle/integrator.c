@@ -303,10 +303,12 @@ static v3 eval_environment(const vo_render_scene *sc, v3 d) {
     const real w = (real)sc->env_w, h = (real)sc->env_h;
     real dy = d.y < (real)-1 ? (real)-1 : (d.y > (real)1 ? (real)1 : d.y);
     real u = RATAN2(d.z, d.x) / (real)TWO_PI_F + (real)0.5 + (real)sc->env_rotation / (real)TWO_PI_F;
+    u = u - RFLOOR(u);  /* repeat addressing */
     real v = RACOS(dy) / PI_R;
     const real vlo = (real)0.5 / h, vhi = (real)1 - (real)0.5 / h;
     v = v < vlo ? vlo : (v > vhi ? vhi : v);
     real fx = u * w - (real)0.5, fy = v * h - (real)0.5;
+    fy = fy < 0 ? 0 : (fy > h - 1 ? h - 1 : fy);  /* inside the map whatever the rounding of 0.5 / h */
     real x0f = RFLOOR(fx), y0f = RFLOOR(fy);
     real ax = fx - x0f, ay = fy - y0f;
     long long x0 = (long long)x0f, y0 = (long long)y0f;

@@ -204,13 +204,17 @@ def vt_materials(mats):
 def sky_map(width=64, height=32, sun=(0.3, 0.28), sun_radiance=300.0, seed=2):
     """A lat-long rgba32f sky (row 0 at the zenith): blue toward the zenith and whiter toward the
     horizon, a dark brown ground below it, a bright 2 x 2-texel sun at (u, v) = `sun`, and mild
-    per-texel variation."""
+    per-texel variation except in the two pole rows, which are constant as in any map of a real sky
+    (a pole is a single direction; a varying pole row would make lookups there depend on an azimuth
+    that rounding decides)."""
     rng = np.random.default_rng(seed)
     v = (np.arange(height, dtype=np.float64) + 0.5) / height
     up = np.cos(v * np.pi)[:, None] * np.ones((1, width))
     sky = np.stack([0.35 + 0.5 * (1 - up), 0.55 + 0.4 * (1 - up), np.full_like(up, 1.0)], -1)
     ground = np.broadcast_to(np.array([0.18, 0.12, 0.08]), sky.shape)
-    img = np.where((up > 0)[..., None], sky, ground) * rng.uniform(0.85, 1.15, (height, width, 1))
+    noise = rng.uniform(0.85, 1.15, (height, width, 1))
+    noise[[0, -1]] = 1.0
+    img = np.where((up > 0)[..., None], sky, ground) * noise
     sx, sy = int(sun[0] * width), int(sun[1] * height)
     img[sy:sy + 2, sx:sx + 2] = (sun_radiance, 0.9 * sun_radiance, 0.75 * sun_radiance)
     out = np.zeros((height, width, 4), np.float32)

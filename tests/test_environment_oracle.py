@@ -259,3 +259,20 @@ def test_environment_off_leaves_the_image_unchanged():
     a = oracle.render(sc, mats, _frame(24, 16, 3, env=None), 1, 8)
     b = oracle.render(sc, mats, _frame(24, 16, 3, env=scenes.sky_map(), strength=0.0), 1, 8)
     assert np.array_equal(a, b)
+
+
+def test_library_tables_equal_the_oracle_bit_for_bit():
+    # The library's host construction (csrc/environment.cpp) and the oracle's restatement follow the
+    # same float operations, so the tables agree exactly; sampling on the GPU then picks the oracle's cells.
+    rng = np.random.default_rng(11)
+    for h, w in ((16, 32), (7, 5), (1, 9), (33, 64)):
+        img = np.zeros((h, w, 4), np.float32)
+        img[..., :3] = rng.lognormal(0.0, 1.5, (h, w, 3)).astype(np.float32)
+        a = vt.environment_tables(img)
+        b = oracle.env_tables(img)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (h, w)
+    flat = np.full((4, 8, 4), 2.0, np.float32)
+    a, b = vt.environment_tables(flat), oracle.env_tables(flat)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    with pytest.raises(vt.VTError):
+        vt.environment_tables(np.zeros((8193, 1, 4), np.float32))
