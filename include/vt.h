@@ -56,7 +56,13 @@ typedef struct vt_build_options {
     uint32_t bins;      /* SAH bins per axis, 4..64 (0: 16, Sources/Acceleration.swift:109) */
     uint32_t max_depth; /* tree depth limit before a median rebuild, <= 24 (0: 24) */
     uint32_t collapse;  /* 0: SAH-optimal collapse of a binary SAH tree into 8-wide nodes; 1: greedy
-                           widening, splitting the largest child (Sources/Acceleration.swift:247-284) */
+                           widening, splitting the largest child (Sources/Acceleration.swift:247-284);
+                           2: no 8-wide tree -- the reference's flat binary median hierarchy
+                           (OBJMesh.hierarchy, Sources/StudioModel.swift:701-760: 48-byte MeshNode
+                           records, at most 4 triangles per leaf, leaf_size / bins / max_depth
+                           ignored), walked as the reference's software trace_scene walks it
+                           (main.swift:2058-2080).  Hits are the same; the traversal counters,
+                           batched rendering and instancing need 0 or 1 (VT_ERROR_UNSUPPORTED) */
 } vt_build_options;
 
 /* Replaces MaterialLibrary.setMesh(_:) for an indexed mesh (Sources/StudioModel.swift:1480-1504,
@@ -435,6 +441,14 @@ vt_status vt_bvh8_build_host(const float *vertices, uint32_t vertex_count, const
                              const vt_build_options *opts, void **nodes, uint32_t *node_count, void **tris,
                              uint32_t *depth, float *root_max);
 void vt_host_free(void *p);
+
+/* Host-only build of the flat binary hierarchy of build option collapse = 2, replacing OBJMesh.build /
+ * hierarchy (Sources/StudioModel.swift:679-760): *order (face_count u32: hierarchy position -> face) and
+ * *nodes (node_count 48-byte MeshNode records: lo, hi as 4 floats, links as 4 int32) are malloc'ed
+ * (free with vt_host_free); *depth is the number of levels. */
+vt_status vt_hierarchy_build_host(const float *vertices, uint32_t vertex_count, const uint32_t *faces,
+                                  uint32_t face_count, uint32_t **order, void **nodes, uint32_t *node_count,
+                                  uint32_t *depth);
 
 #ifdef __cplusplus
 }

@@ -256,6 +256,8 @@ vt_status vt_batch_create(vt_context *ctx, const vt_batch_job *jobs, uint32_t co
         for (uint32_t j = 0; j < count; ++j) {
             const vt_batch_job &jb = jobs[j];
             if (!jb.scene || !jb.scene->shaded) return fail(VT_ERROR_INVALID_ARGUMENT, "every job needs a scene with materials");
+            if (jb.scene->view.kind != 0u)
+                return fail(VT_ERROR_UNSUPPORTED, "batched rendering traces 8-wide hierarchies (build option collapse 0 or 1)");
             if (jb.scene->has_delta)
                 return fail(VT_ERROR_INVALID_ARGUMENT, "batched rendering bounds paths at max_depth vertices: scenes with "
                                                        "mirrors or smooth glass need vt_render_frame");
@@ -339,6 +341,10 @@ vt_status vt_batch_render(vt_batch *b, uint32_t samples, void *stream) {
         // deeper tree needs a deeper traversal stack, so the tables and the graph are made again.
         bool stale = false;
         for (size_t k = 0; k < b->scenes.size(); ++k) stale = stale || b->scenes[k]->generation != b->scene_generation[k];
+        for (vt_scene *s : b->scenes)
+            if (s->view.kind != 0u)
+                return fail(VT_ERROR_UNSUPPORTED, "a batched scene was rebuilt as the flat binary hierarchy (collapse 2); "
+                                                  "batched rendering traces 8-wide hierarchies");
         if (stale) {
             check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
             b->tuning.leaf_size = b->scenes[0]->info.leaf_size;

@@ -44,7 +44,8 @@ struct BuildOptions {
     int leaf_size = 3;
     int bins = 16;
     int max_depth = 24;
-    int collapse = 0;  // 0: SAH-optimal collapse of a binary tree; 1: greedy widening
+    int collapse = 0;  // 0: SAH-optimal collapse of a binary tree; 1: greedy widening; 2: the flat
+                       // binary median hierarchy (MedianBvh) instead of an 8-wide tree
 };
 
 struct Bvh8 {
@@ -59,6 +60,28 @@ struct Bvh8 {
 // Deterministic: the result does not depend on the number of threads.
 Bvh8 build_bvh8(const float *vertices, uint32_t vertex_count, const uint32_t *faces, uint32_t face_count,
                 const BuildOptions &opt);
+
+// The reference's flat binary hierarchy (OBJMesh.hierarchy, Sources/StudioModel.swift:701-760), which
+// its software trace_scene walks for a published single-level mesh (main.swift:2056-2080): median
+// splits on the longest axis of a node's triangle bounds, at most four triangles per leaf, nodes in
+// creation (pre-)order.  MeshNode is the reference's 48-byte record: lo, hi (w unused), links = (left,
+// right, split axis, 0) for an interior node, (0, 0, first triangle, count) for a leaf.
+struct alignas(16) MeshNode {
+    float lo[4], hi[4];
+    int32_t links[4];
+};
+static_assert(sizeof(MeshNode) == 48, "MeshNode is three 16-byte words");
+
+struct MedianBvh {
+    std::vector<MeshNode> nodes;
+    std::vector<TriRecord> tris;   // in hierarchy order (tris[i] is face order[i])
+    std::vector<uint32_t> order;
+    int depth = 0;                 // levels (root = 1)
+    float root_max = 0;            // max |coordinate| of the root bounds
+    double build_seconds = 0;
+};
+
+MedianBvh build_median_bvh(const float *vertices, uint32_t vertex_count, const uint32_t *faces, uint32_t face_count);
 
 // The same hierarchy over n boxes (6 floats each: lo xyz, hi xyz), e.g. instance bounds.  Leaf slots
 // name items instead of holding triangle records: tri_base + offset indexes `items`, which receives

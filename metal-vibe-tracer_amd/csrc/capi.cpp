@@ -67,20 +67,82 @@ namespace vt {
 // Mirrors MaterialLibrary.setMesh for an indexed mesh (Sources/StudioModel.swift:1480-1504 with
 // Sources/Acceleration.swift:652-782): validate, build the hierarchy on the host, check the GPU
 // budget with the reference's message, then copy the streams to device memory.
+// The scene's hierarchy as the traversal reads it: the 8-wide tree, or with opt.collapse == 2 the
+// reference's flat binary median hierarchy.  Built on the host; `tree` receives the node and triangle
+// records, the view and info fields the chosen kind.
+struct HostTree {
+    std::vector<uint8_t> nodes, tris;
+    uint32_t node_count = 0, depth = 0, kind = 0, width = 8, node_bytes = 0;
+    float root_max = 0;
+    double seconds = 0;
+};
+static HostTree build_tree(const float *vertices, uint32_t vertex_count, const uint32_t *faces, uint32_t face_count,
+                           const BuildOptions &opt) {
+    HostTree t;
+    auto copy = [](std::vector<uint8_t> &dst, const void *src, size_t bytes) {
+        dst.resize(bytes);
+        if (bytes) std::memcpy(dst.data(), src, bytes);
+    };
+    if (opt.collapse == 2) {
+        MedianBvh m = build_median_bvh(vertices, vertex_count, faces, face_count);
+        if (m.depth > (int)kMedianStack - 1) throw std::invalid_argument("mesh hierarchy deeper than the traversal stack");
+        copy(t.nodes, m.nodes.data(), m.nodes.size() * sizeof(MeshNode));
+        copy(t.tris, m.tris.data(), m.tris.size() * sizeof(TriRecord));
+        t.node_count = (uint32_t)m.nodes.size();
+        t.depth = (uint32_t)m.depth;
+        t.kind = 1;
+        t.width = 2;
+        t.node_bytes = sizeof(MeshNode);
+        t.root_max = m.root_max;
+        t.seconds = m.build_seconds;
+        return t;
+    }
+    Bvh8 bvh = build_bvh8(vertices, vertex_count, faces, face_count, opt);
+    if (bvh.nodes.size() >= (1u << 26)) throw std::invalid_argument("mesh exceeds the node index range");
+    copy(t.nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(Node8));
+    copy(t.tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord));
+    t.node_count = (uint32_t)bvh.nodes.size();
+    t.depth = (uint32_t)bvh.depth;
+    t.node_bytes = sizeof(Node8);
+    t.root_max = bvh.root_max;
+    t.seconds = bvh.build_seconds;
+    return t;
+}
+static void publish_tree(vt_scene *s, const HostTree &t, const BuildOptions &opt) {
+    s->nodes.upload(t.nodes.data(), t.nodes.size());
+    s->tris.upload(t.tris.data(), t.tris.size());
+    s->view.nodes = s->nodes.ptr;
+    s->view.tris = s->tris.ptr;
+    s->view.node_count = t.node_count;
+    s->view.depth = t.depth;
+    s->view.root_max = t.root_max;
+    s->view.kind = t.kind;
+    vt_scene_info &i = s->info;
+    i.node_count = t.node_count;
+    i.depth = t.depth;
+    i.leaf_size = t.kind == 1 ? 4u : (uint32_t)opt.leaf_size;
+    i.width = t.width;
+    i.node_bytes = t.node_bytes;
+    i.tri_bytes = sizeof(TriRecord);
+    i.root_max = t.root_max;
+    i.build_seconds = (float)t.seconds;
+}
+
+// Mirrors MaterialLibrary.setMesh for an indexed mesh (Sources/StudioModel.swift:1480-1504 with
+// Sources/Acceleration.swift:652-782): validate, build the hierarchy on the host, check the GPU
+// budget with the reference's message, then copy the streams to device memory.
 std::unique_ptr<vt_scene> set_mesh(vt_context *ctx, const float *vertices, uint32_t vertex_count,
                                    const uint32_t *faces, uint32_t face_count, const BuildOptions &opt) {
     if (face_count > 0 && (!vertices || !faces)) throw std::invalid_argument("vertices and faces must not be NULL");
     if (face_count >= (1u << 30)) throw std::invalid_argument("Stored mesh assets exceed the 1,073,741,824 triangle limit.");
-    Bvh8 bvh = build_bvh8(vertices, vertex_count, faces, face_count, opt);
-    const uint64_t bytes = bvh.nodes.size() * sizeof(Node8) + bvh.tris.size() * sizeof(TriRecord) +
-                           (uint64_t)vertex_count * 32 + (uint64_t)face_count * 16;
-    if (bvh.nodes.size() >= (1u << 26)) throw std::invalid_argument("mesh exceeds the node index range");
+    if (opt.collapse < 0 || opt.collapse > 2) throw std::invalid_argument("collapse must be 0, 1 or 2");
+    HostTree tree = build_tree(vertices, vertex_count, faces, face_count, opt);
+    const uint64_t bytes = tree.nodes.size() + tree.tris.size() + (uint64_t)vertex_count * 32 + (uint64_t)face_count * 16;
     if (ctx->resident + bytes > ctx->budget)
         throw BudgetError("The imported meshes exceed the safe GPU memory budget.");
     check(hipSetDevice(ctx->device), "hipSetDevice");
     auto s = std::make_unique<vt_scene>();
-    s->nodes.upload(bvh.nodes.data(), bvh.nodes.size() * sizeof(Node8));
-    s->tris.upload(bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord));
+    publish_tree(s.get(), tree, opt);
     // Document-order geometry for hit attributes and emitter sampling (the reference's MeshStreams).
     s->vertices.upload(vertices, (size_t)vertex_count * 8 * sizeof(float));
     s->faces.upload(faces, (size_t)face_count * 4 * sizeof(uint32_t));
@@ -89,52 +151,26 @@ std::unique_ptr<vt_scene> set_mesh(vt_context *ctx, const float *vertices, uint3
     s->ctx = ctx;
     s->accounted = s->nodes.bytes + s->tris.bytes + s->vertices.bytes + s->faces.bytes;
     ctx->resident += s->accounted;
-    s->view.nodes = s->nodes.ptr;
-    s->view.tris = s->tris.ptr;
-    s->view.node_count = (uint32_t)bvh.nodes.size();
-    s->view.depth = (uint32_t)bvh.depth;
-    s->view.root_max = bvh.root_max;
     s->view.fault = ctx->fault.as<uint32_t>();
-    vt_scene_info &i = s->info;
-    i.face_count = face_count;
-    i.vertex_count = vertex_count;
-    i.node_count = (uint32_t)bvh.nodes.size();
-    i.depth = (uint32_t)bvh.depth;
-    i.leaf_size = (uint32_t)opt.leaf_size;
-    i.width = 8;
-    i.node_bytes = sizeof(Node8);
-    i.tri_bytes = sizeof(TriRecord);
-    i.device_bytes = bytes;
-    i.root_max = bvh.root_max;
-    i.build_seconds = (float)bvh.build_seconds;
+    s->info.face_count = face_count;
+    s->info.vertex_count = vertex_count;
+    s->info.device_bytes = bytes;
     return s;
 }
 
 // Rebuilds a published scene's hierarchy from its document-order streams with other options (the
 // autotuner's leaf-size sweep); materials and the light tree are kept.
 void rebuild_bvh(vt_scene *s, const BuildOptions &opt) {
+    if (opt.collapse < 0 || opt.collapse > 2) throw std::invalid_argument("collapse must be 0, 1 or 2");
     const uint32_t vertex_count = (uint32_t)(s->host_vertices.size() / 8), face_count = (uint32_t)(s->host_faces.size() / 4);
-    Bvh8 bvh = build_bvh8(s->host_vertices.data(), vertex_count, s->host_faces.data(), face_count, opt);
-    if (bvh.nodes.size() >= (1u << 26)) throw std::invalid_argument("mesh exceeds the node index range");
+    HostTree tree = build_tree(s->host_vertices.data(), vertex_count, s->host_faces.data(), face_count, opt);
     check(hipSetDevice(s->ctx->device), "hipSetDevice");
     s->ctx->resident -= s->nodes.bytes + s->tris.bytes;
     s->accounted -= s->nodes.bytes + s->tris.bytes;
-    s->nodes.upload(bvh.nodes.data(), bvh.nodes.size() * sizeof(Node8));
-    s->tris.upload(bvh.tris.data(), bvh.tris.size() * sizeof(TriRecord));
+    publish_tree(s, tree, opt);
     s->ctx->resident += s->nodes.bytes + s->tris.bytes;
     s->accounted += s->nodes.bytes + s->tris.bytes;
-    s->view.nodes = s->nodes.ptr;
-    s->view.tris = s->tris.ptr;
-    s->view.node_count = (uint32_t)bvh.nodes.size();
-    s->view.depth = (uint32_t)bvh.depth;
-    s->view.root_max = bvh.root_max;
-    s->info.node_count = (uint32_t)bvh.nodes.size();
-    s->info.depth = (uint32_t)bvh.depth;
-    s->info.leaf_size = (uint32_t)opt.leaf_size;
-    s->info.device_bytes = bvh.nodes.size() * sizeof(Node8) + bvh.tris.size() * sizeof(TriRecord) +
-                           (uint64_t)vertex_count * 32 + (uint64_t)face_count * 16;
-    s->info.root_max = bvh.root_max;
-    s->info.build_seconds = (float)bvh.build_seconds;
+    s->info.device_bytes = tree.nodes.size() + tree.tris.size() + (uint64_t)vertex_count * 32 + (uint64_t)face_count * 16;
     s->generation = vt_scene::next_generation();
 }
 
@@ -211,6 +247,28 @@ vt_status vt_scene_rebuild(vt_scene *scene, const vt_build_options *opts) {
     return guarded([&]() -> vt_status {
         if (!scene) return fail(VT_ERROR_INVALID_ARGUMENT, "scene must not be NULL");
         vt::rebuild_bvh(scene, build_options(opts));
+        return VT_OK;
+    });
+}
+
+vt_status vt_hierarchy_build_host(const float *vertices, uint32_t vertex_count, const uint32_t *faces,
+                                  uint32_t face_count, uint32_t **order, void **nodes, uint32_t *node_count,
+                                  uint32_t *depth) {
+    return guarded([&]() -> vt_status {
+        if (!order || !nodes || !node_count || !depth) return fail(VT_ERROR_INVALID_ARGUMENT, "outputs must not be NULL");
+        if (face_count > 0 && (!vertices || !faces)) return fail(VT_ERROR_INVALID_ARGUMENT, "vertices and faces must not be NULL");
+        *order = nullptr;
+        *nodes = nullptr;
+        vt::MedianBvh m = vt::build_median_bvh(vertices, vertex_count, faces, face_count);
+        void *o = std::malloc(std::max<size_t>(1, m.order.size() * sizeof(uint32_t)));
+        void *n = std::malloc(std::max<size_t>(1, m.nodes.size() * sizeof(vt::MeshNode)));
+        if (!o || !n) { std::free(o); std::free(n); throw std::bad_alloc(); }
+        if (!m.order.empty()) std::memcpy(o, m.order.data(), m.order.size() * sizeof(uint32_t));
+        if (!m.nodes.empty()) std::memcpy(n, m.nodes.data(), m.nodes.size() * sizeof(vt::MeshNode));
+        *order = static_cast<uint32_t *>(o);
+        *nodes = n;
+        *node_count = (uint32_t)m.nodes.size();
+        *depth = (uint32_t)m.depth;
         return VT_OK;
     });
 }
@@ -315,6 +373,8 @@ vt_status vt_trace_stats_noalloc(vt_scene *scene, const vt_ray_soa *rays, uint32
                                  const vt_hit_soa *hits, uint8_t *occluded, uint64_t *counters, void *stream) {
     return guarded([&]() -> vt_status {
         if (!scene || !counters) return fail(VT_ERROR_INVALID_ARGUMENT, "scene and counters must not be NULL");
+        if (scene->view.kind != 0u)
+            return fail(VT_ERROR_UNSUPPORTED, "traversal counters are kept for the 8-wide hierarchy only");
         if (n && !rays_ok(rays)) return fail(VT_ERROR_INVALID_ARGUMENT, "ray arrays must not be NULL");
         vt::HitArgs h{};
         if (n && any && !occluded) return fail(VT_ERROR_INVALID_ARGUMENT, "occluded must not be NULL");
@@ -337,6 +397,8 @@ vt_status vt_trace_stats(vt_scene *scene, const vt_ray_soa *rays, uint32_t n, in
     return guarded([&]() -> vt_status {
         if (!scene || !counters) return fail(VT_ERROR_INVALID_ARGUMENT, "scene and counters must not be NULL");
         counters[0] = counters[1] = 0;
+        if (scene->view.kind != 0u)
+            return fail(VT_ERROR_UNSUPPORTED, "traversal counters are kept for the 8-wide hierarchy only");
         if (n == 0) return VT_OK;
         if (!rays_ok(rays)) return fail(VT_ERROR_INVALID_ARGUMENT, "ray arrays must not be NULL");
         check(hipSetDevice(scene->ctx->device), "hipSetDevice");

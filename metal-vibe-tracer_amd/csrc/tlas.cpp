@@ -150,6 +150,9 @@ std::unique_ptr<vt_tlas> build_tlas(vt_context *ctx, const vt_instance *list, ui
     for (uint32_t k = 0; k < count; ++k) {
         const vt_instance &in = list[k];
         if (!in.mesh) throw std::invalid_argument("instance " + std::to_string(k) + ": mesh must not be NULL");
+        if (in.mesh->view.kind != 0u)
+            throw std::invalid_argument("instance " + std::to_string(k) + ": instanced meshes need an 8-wide hierarchy "
+                                        "(build option collapse 0 or 1)");
         if (in.mesh->ctx != ctx) throw std::invalid_argument("instance " + std::to_string(k) + ": mesh of another context");
         for (int q = 0; q < 12; ++q)
             if (!std::isfinite(in.world[q])) throw std::invalid_argument("Singular USD transform.");
@@ -246,6 +249,9 @@ std::unique_ptr<vt_tlas> build_tlas(vt_context *ctx, const vt_instance *list, ui
 void refresh(vt_tlas *t) {
     for (size_t a = 0; a < t->assets.size(); ++a)
         if (t->assets[a]->generation != t->asset_generation[a]) {
+            if (t->assets[a]->view.kind != 0u)
+                throw std::invalid_argument("an instanced mesh was rebuilt as the flat binary hierarchy (collapse 2); "
+                                            "instances need an 8-wide hierarchy");
             check(hipSetDevice(t->ctx->device), "hipSetDevice");
             check(hipDeviceSynchronize(), "hipDeviceSynchronize");
             write_records(t);

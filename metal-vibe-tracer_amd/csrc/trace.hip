@@ -466,6 +466,97 @@ __global__ __launch_bounds__(256) void miss_kernel(RayArgs rays, HitArgs hits, u
     }
 }
 
+// The reference's software traversal of its flat binary hierarchy (trace_scene's mesh branch,
+// main.swift:2058-2080, with mesh_box_ray / mesh_node_hit / push_mesh_children :1511-1528): one ray per
+// lane, a stack of node indices in LDS, the child on the ray's side of the node's split axis first.
+// Boxes are the reference's: the slabs of [lo, hi] against the origin moved by -+e, e = 2^-23 (1 + ...)
+// x (|origin|max + |root|max), a direction component below 1e-8 testing the origin against the
+// enlarged slab.  Triangles are the same watertight test as the 8-wide traversal, and the closest-hit
+// rule the same (t, then the smaller document face index at equal t), so hits do not depend on the
+// hierarchy.  A ray whose stack would overflow (never, for the depths the host admits) raises *fault.
+template <bool ANY>
+__global__ __launch_bounds__(256) void trace_median_kernel(const F4 *__restrict__ nodes, const F4 *__restrict__ tris,
+                                                           RayArgs rays, HitArgs hits, uint8_t *__restrict__ occluded,
+                                                           uint32_t n, const uint32_t *__restrict__ count, float root_max,
+                                                           uint32_t *__restrict__ fault) {
+    __shared__ uint32_t stack[kMedianStack][256];
+    const uint32_t lane = threadIdx.x, i = blockIdx.x * 256 + lane;
+    if (count) n = min(n, *count);
+    if (i >= n) return;
+    const float o[3] = {rays.ox[i], rays.oy[i], rays.oz[i]}, d[3] = {rays.dx[i], rays.dy[i], rays.dz[i]};
+    const float tmin = rays.tmin[i], tmax = rays.tmax[i];
+    const WoopRay w = woop_ray(d[0], d[1], d[2]);
+    const float e = 1.1920929e-6f * (fmaxf(fabsf(o[0]), fmaxf(fabsf(o[1]), fabsf(o[2]))) + root_max);
+    const float low[3] = {o[0] + e, o[1] + e, o[2] + e}, high[3] = {o[0] - e, o[1] - e, o[2] - e};
+    float best = tmax, bb1 = 0.0f, bb2 = 0.0f;
+    int32_t bface = -1;
+    bool hit = false, overflow = false;
+    uint32_t top = 0;
+    stack[top++][lane] = 0u;
+    while (top > 0u) {
+        const uint32_t ni = stack[--top][lane];
+        const F4 lo = nodes[3 * (size_t)ni], hi = nodes[3 * (size_t)ni + 1], lk = nodes[3 * (size_t)ni + 2];
+        const int32_t links[4] = {(int32_t)__float_as_uint(lk.x), (int32_t)__float_as_uint(lk.y),
+                                  (int32_t)__float_as_uint(lk.z), (int32_t)__float_as_uint(lk.w)};
+        // mesh_node_hit against [tmin, closest hit so far].
+        float near_t = tmin, far_t = ANY ? tmax : best;
+        const float nlo[3] = {lo.x, lo.y, lo.z}, nhi[3] = {hi.x, hi.y, hi.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (fabsf(d[a]) < 1e-8f) {
+                if (low[a] < nlo[a] || high[a] > nhi[a]) far_t = -1.0f;
+            } else {
+                const float ta = (nlo[a] - low[a]) / d[a], tb = (nhi[a] - high[a]) / d[a];
+                near_t = fmaxf(near_t, fminf(ta, tb));
+                far_t = fminf(far_t, fmaxf(ta, tb));
+            }
+        }
+        if (!(near_t <= far_t)) continue;
+        if (links[3] == 0) {
+            if (top + 2u > kMedianStack) {
+                overflow = true;
+                continue;
+            }
+            const int axis = min(max(links[2], 0), 2);
+            const bool left_first = d[axis] >= 0.0f;
+            stack[top++][lane] = (uint32_t)(left_first ? links[1] : links[0]);
+            stack[top++][lane] = (uint32_t)(left_first ? links[0] : links[1]);
+            continue;
+        }
+        for (int32_t k = 0; k < links[3]; ++k) {
+            const size_t r = 3 * (size_t)(links[2] + k);
+            const F4 p0 = tris[r], p1 = tris[r + 1], p2 = tris[r + 2];
+            const int32_t face = (int32_t)__float_as_uint(p0.w);
+            float t, b1, b2;
+            if (!woop_triangle(w, o[0], o[1], o[2], p0, p1, p2, t, b1, b2)) continue;
+            if (ANY) {
+                if (t >= tmin && t < tmax) {
+                    hit = true;
+                    break;
+                }
+                continue;
+            }
+            if (t >= tmin && (t < best || (t == best && bface >= 0 && face < bface))) {
+                best = t;
+                bb1 = b1;
+                bb2 = b2;
+                bface = face;
+            }
+        }
+        if (ANY && hit) break;
+    }
+    if (overflow && fault) atomicOr(fault, 1u);
+    if (ANY) {
+        occluded[i] = hit ? 1 : 0;
+    } else {
+        hits.t[i] = best;
+        hits.b1[i] = bb1;
+        hits.b2[i] = bb2;
+        hits.face[i] = bface;
+        if (hits.instance) hits.instance[i] = 0u;
+    }
+}
+
 // Batched rendering: rays of many scenes in one persistent launch with lane refill.  Ray i belongs to
 // job job[i], whose scene is job_trace[job[i]]; rays of one job are contiguous in the queues, so a wave
 // rarely spans two scenes.
@@ -630,6 +721,19 @@ hipError_t launch_trace(const TraceScene &scene, const RayArgs &rays, const HitA
         dim3 grid((n + 255) / 256);
         if (any) hipLaunchKernelGGL(miss_kernel<true>, grid, dim3(256), 0, stream, rays, hits, occluded, n, count, 0u);
         else hipLaunchKernelGGL(miss_kernel<false>, grid, dim3(256), 0, stream, rays, hits, occluded, n, count, 0u);
+        return hipGetLastError();
+    }
+    if (scene.kind == 1u) {
+        // The reference's flat binary hierarchy: no counting variant.
+        if (counters) return hipErrorInvalidValue;
+        const dim3 grid((n + 255) / 256);
+        const F4 *nd = reinterpret_cast<const F4 *>(scene.nodes), *tr = reinterpret_cast<const F4 *>(scene.tris);
+        if (any)
+            hipLaunchKernelGGL(trace_median_kernel<true>, grid, dim3(256), 0, stream, nd, tr, rays, hits, occluded, n,
+                               count, scene.root_max, scene.fault);
+        else
+            hipLaunchKernelGGL(trace_median_kernel<false>, grid, dim3(256), 0, stream, nd, tr, rays, hits, occluded, n,
+                               count, scene.root_max, scene.fault);
         return hipGetLastError();
     }
     dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);

@@ -25,7 +25,11 @@ struct TraceScene {
     // hook; 0 = no cap) can raise it.
     uint32_t *fault;
     uint32_t stack_cap;
+    // 0: Node8 records (the 8-wide tree); 1: MeshNode records (the reference's flat binary median
+    // hierarchy, bvh8.h), walked by trace_median_kernel with a stack of kMedianStack entries.
+    uint32_t kind = 0;
 };
+constexpr uint32_t kMedianStack = 32;
 
 // Two-level scenes (the reference's MeshSceneLayout, Sources/Acceleration.swift:356-573): an 8-wide
 // hierarchy over the instances' world bounds whose leaf slots name InstanceRecords (in leaf order),

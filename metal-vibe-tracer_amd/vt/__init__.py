@@ -38,7 +38,7 @@ EXPORTED = [
     "vt_z_draws", "vt_context_check", "vt_scene_set_stack_cap", "vt_mesh_from_triangles",
     "vt_scene_upload_triangles", "vt_scene_set_surface_settings", "vt_renderer_set_callbacks",
     "vt_trace_stats_noalloc", "vt_tlas_create", "vt_tlas_destroy", "vt_tlas_get_info", "vt_tlas_trace_closest",
-    "vt_tlas_trace_any", "vt_scene_set_environment", "vt_environment_tables",
+    "vt_tlas_trace_any", "vt_scene_set_environment", "vt_environment_tables", "vt_hierarchy_build_host",
 ]
 
 
@@ -219,6 +219,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_scene_set_light_tree": (i32, [vp, c.c_int]),
         "vt_scene_set_environment": (i32, [vp, vp, u32, u32]),
         "vt_environment_tables": (i32, [vp, u32, u32, vp, vp]),
+        "vt_hierarchy_build_host": (i32, [vp, u32, vp, u32, P(c.c_void_p), P(c.c_void_p), P(u32), P(u32)]),
         "vt_light_tree_build_host": (i32, [vp, vp, u32, P(vp), P(u32), P(vp), P(vp), P(u32), P(f32), P(f32)]),
         "vt_mesh_payload_bytes": (c.c_uint64, [u32, u32]),
         "vt_mesh_payload_write": (i32, [vp, u32, vp, u32, vp, c.c_uint64]),
@@ -265,6 +266,28 @@ def _ptr(a: np.ndarray):
 class Mesh:
     vertices: np.ndarray  # (V, 8) float32: position, normal, uv
     faces: np.ndarray     # (F, 4) uint32: a, b, c, tag
+
+
+def hierarchy_build_host(vertices, faces):
+    """vt_hierarchy_build_host: the flat binary median hierarchy (build option collapse 2) ->
+    (order (F,) uint32, nodes (N, 12) with lo[4], hi[4] as float32 and links[4] as int32, depth)."""
+    v = np.ascontiguousarray(vertices, np.float32)
+    f = np.ascontiguousarray(faces, np.uint32)
+    order, nodes = ctypes.c_void_p(), ctypes.c_void_p()
+    count, depth = ctypes.c_uint32(), ctypes.c_uint32()
+    _check(load().vt_hierarchy_build_host(_ptr(v), len(v), _ptr(f), len(f), ctypes.byref(order), ctypes.byref(nodes),
+                                          ctypes.byref(count), ctypes.byref(depth)))
+    try:
+        o = np.ctypeslib.as_array(ctypes.cast(order, ctypes.POINTER(ctypes.c_uint32)), (len(f),)).copy() if len(f) else \
+            np.zeros(0, np.uint32)
+        raw = np.ctypeslib.as_array(ctypes.cast(nodes, ctypes.POINTER(ctypes.c_uint32)), (count.value * 12,)).copy() \
+            if count.value else np.zeros(0, np.uint32)
+    finally:
+        load().vt_host_free(order)
+        load().vt_host_free(nodes)
+    raw = raw.reshape(-1, 12)
+    return o, {"lo": raw[:, 0:4].view(np.float32), "hi": raw[:, 4:8].view(np.float32),
+               "links": raw[:, 8:12].view(np.int32)}, depth.value
 
 
 def environment_tables(rgba):
