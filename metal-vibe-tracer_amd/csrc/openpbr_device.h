@@ -3,7 +3,11 @@
 //
 // The reference evaluates the vendored Adobe OpenPBR (pinned c91aad1) through prepare_openpbr_lanes /
 // prepared_eval / prepared_pdf / prepared_sample (main.swift:2485-2645).  This is that BSDF for
-// coat = fuzz = subsurface = thin film = dispersion = anisotropy = 0 and base_diffuse_roughness = 0:
+// fuzz = subsurface = thin film = dispersion = anisotropy = 0, base_diffuse_roughness = 0 and the
+// materials' coat (weight from the material, roughness 0.12, IOR 1.6, white, darkening 1:
+// main.swift:2494-2495, Vendor/OpenPBR/openpbr_resolved_inputs.h:123-127): the coating lobe
+// (impl/openpbr_coating_lobe.h; the fuzz lobe at weight 0 passes through) with its GGX reflection lobe
+// (impl/openpbr_generic_minimal_microfacet_lobe.h) over
 // the comprehensive GGX microfacet reflection/transmission lobe with dielectric Fresnel and F82-tint
 // metal (Vendor/OpenPBR/impl/openpbr_comprehensive_microfacet_lobe.h,
 // openpbr_reflection_transmission_coefficient.h), the dielectric and metal multiple-scattering lobes
@@ -168,6 +172,14 @@ __device__ __forceinline__ float metal_avg_complement(const EnergyTables &E, flo
     return t[ix] + (t[ix + 1] - t[ix]) * (x - (float)ix);
 }
 __device__ __forceinline__ float clamp_avg(float v) { return fmaxf(v, 1e-12f); }
+// mix(x, y, a) as the shading language defines it.
+__device__ __forceinline__ float mixf(float x, float y, float a) { return x + (y - x) * a; }
+__device__ __forceinline__ float average_fresnel(float eta) {
+    if (eta > 1.0f) return (eta - 1.0f) / (4.08567f + 1.00071f * eta);
+    return 0.997118f + 0.1014f * eta - 0.965241f * (eta * eta) - 0.130607f * (eta * eta) * eta;
+}
+constexpr float kCoatIor = 1.6f, kCoatRoughness = 0.12f;
+constexpr float kCoatAlpha = kCoatRoughness * kCoatRoughness;  // above the 1e-6 floor of the microfacet alpha
 
 // ---------------------------------------------------------------- prepared BSDF
 struct Pbr {
@@ -183,7 +195,9 @@ struct Pbr {
     S4 diffuse_albedo;
     float diffuse_comp;
     float w[4], total;       // lobe weights: specular, dielectric MMS, metal MMS, diffuse
-    bool d_on, m_on;
+    float coat, c_prob;      // coat presence and coat reflection probability (entering a coated surface)
+    S4 c_scale;              // in_base_layer_scale: (1 - in_reflected) x darkening, per lane
+    bool d_on, m_on, c_on;
 };
 
 __device__ __forceinline__ void pbr_coefficients(const Pbr &p, float c, S4 &refl, S4 &trans) {
@@ -204,7 +218,8 @@ __device__ __forceinline__ void pbr_probabilities(const Pbr &p, float c, S4 &ref
 // prepare_openpbr_lanes for the subset.  type: 1 GLOSSY (metalness 1) or 4 OPENPBR; n: shading normal
 // on the arriving side; wo: toward the viewer; inside: hit from the back; rho: base colour per lane.
 __device__ __forceinline__ void pbr_prepare(Pbr &p, const EnergyTables &E, uint32_t type, float roughness, float ior_in,
-                                            float metal_in, float transw, V3 n, V3 wo, bool inside_flag, S4 rho) {
+                                            float metal_in, float transw, float coat, V3 n, V3 wo, bool inside_flag,
+                                            S4 rho) {
     const float rough = fmaxf(0.03f, roughness);
     const float metalness = type == kGlossy ? 1.0f : metal_in;
     const float ior = fmaxf(1.01f, ior_in);
@@ -219,19 +234,28 @@ __device__ __forceinline__ void pbr_prepare(Pbr &p, const EnergyTables &E, uint3
         p.sb.n = -gb.n;
     }
     p.n = p.sb.n;
-    const float wsi = 1.0f * apply_specular_weight_to_ior(ior / 1.0f, 1.0f);
-    float r_refr, r_trans, r_opaque;
+    // IORs (openpbr_bsdf.h:339-418): entering a coated surface, the base sits under the coat, whose IOR
+    // is blended by the coat weight; hit from inside, the coat is ignored.
+    const bool enter_coat = coat > 0.0f && !back;
+    const float wci = enter_coat ? mixf(1.0f, kCoatIor, coat) : 1.0f;
+    const float wsi = wci * apply_specular_weight_to_ior(ior / wci, 1.0f);
+    float r_refr, r_trans, r_opaque, rel_coat = 1.0f;
     if (back) {
         r_refr = 1.0f / wsi;
         r_trans = r_refr;
         r_opaque = wsi / 1.0f;
     } else {
         r_refr = wsi / 1.0f;
-        r_trans = r_refr;
+        r_trans = !enter_coat ? r_refr : wci > wsi && wsi >= 1.0f ? wci / wsi : wsi / wci;
+        if (enter_coat) rel_coat = kCoatIor / 1.0f;
         r_opaque = r_trans;
     }
     r_refr = r_refr >= 1.0f ? fmaxf(r_refr, 1.0001f) : fminf(r_refr, 1.0f / 1.0001f);
-    const float salpha = rough * rough;
+    // The base roughened by the coat (openpbr_bsdf.h:437-460).
+    const float adjust = mixf(1.0f, 0.0f, rel_coat >= 1.0f ? 1.0f / rel_coat : rel_coat);
+    const float adj_rough =
+        mixf(rough, sqrtf(sqrtf(fminf(1.0f, sqf(sqf(rough)) + adjust * sqf(sqf(kCoatRoughness))))), coat);
+    const float salpha = adj_rough * adj_rough;
     p.iso_alpha = salpha;
     p.alpha = fmaxf(salpha, 1e-6f);
     const float dielectric = 1.0f - metalness;
@@ -265,6 +289,26 @@ __device__ __forceinline__ void pbr_prepare(Pbr &p, const EnergyTables &E, uint3
     p.w[2] = p.m_on ? max3(p.m_scale) * p.m_ec : 0.0f;
     p.w[3] = max3(scale(p.diffuse_albedo, p.diffuse_comp));
     p.total = p.w[0] + p.w[1] + p.w[2] + p.w[3];
+    // The coating lobe (openpbr_bsdf.h:498-552, 831-840; openpbr_coating_lobe.h:137-185): the darkening
+    // Delta = (1 - K) / (1 - E_b K) lerped in by the coat weight, the view side's coat reflectance from
+    // the opaque-dielectric energy table, and the coat-or-base choice probability.
+    p.c_on = enter_coat;
+    p.coat = coat;
+    p.c_prob = 0.0f;
+    p.c_scale = s4(1.0f);
+    if (enter_coat) {
+        const float ks = average_fresnel(rel_coat);
+        const float kr = 1.0f - (1.0f - average_fresnel(rel_coat)) / sqf(rel_coat);
+        const float so = average_fresnel(r_opaque), st = average_fresnel(r_trans);
+        const float spec_base = (opaque * so + (0.0f + trans) * st) + metalness;
+        const float kk = mixf(ks, kr, mixf(1.0f, adj_rough, spec_base));
+        const float refl = idotn > 0.0f ? (1.0f - opaque_complement(E, rel_coat, kCoatAlpha, idotn)) * coat : 0.0f;
+        for (int k = 0; k < 4; ++k) {
+            const float eb = ((dmetal * avg_f.v[k] + opaque * mixf(rho.v[k], 1.0f, so)) + 0.0f) + trans;
+            p.c_scale.v[k] = (1.0f - refl) * mixf(1.0f, (1.0f - kk) / (1.0f - eb * kk), coat * 1.0f);
+        }
+        p.c_prob = safe_div(refl, refl + p.total * max3(p.c_scale), 0.5f);
+    }
 }
 
 __device__ __forceinline__ S4 spec_value(const Pbr &p, const EnergyTables &, V3 wi) {
@@ -376,17 +420,18 @@ __device__ __forceinline__ S4 diffuse_value(const Pbr &p, const EnergyTables &E,
 }
 __device__ __forceinline__ float diffuse_pdf(const Pbr &p, V3 wi) { return fmaxf(dot(wi, p.n), 0.0f) * kRcpPi; }
 
-// openpbr_eval: f |cos| per lane.
-__device__ __forceinline__ S4 pbr_eval(const Pbr &p, const EnergyTables &E, V3 wi) {
+// The base layer (the aggregate lobe): f |cos| per lane.
+__device__ __forceinline__ S4 base_eval(const Pbr &p, const EnergyTables &E, V3 wi) {
     return sadd(sadd(sadd(spec_value(p, E, wi), dmms_value(p, E, wi)), mmms_value(p, E, wi)), diffuse_value(p, E, wi));
 }
-__device__ __forceinline__ float pbr_pdf(const Pbr &p, V3 wi) {
+__device__ __forceinline__ float base_pdf(const Pbr &p, V3 wi) {
     float s = p.w[0] * spec_pdf(p, wi) + p.w[1] * dmms_pdf(p, wi) + p.w[2] * mmms_pdf(p, wi) + p.w[3] * diffuse_pdf(p, wi);
     return p.total > 0.0f ? s / p.total : 0.0f;
 }
-// openpbr_sample: lobe by weight, the lobe's own weight and pdf, then every other lobe folded in.
-__device__ __forceinline__ bool pbr_sample(const Pbr &p, const EnergyTables &E, float r0, float r1, float r2, V3 &wi,
-                                           S4 &weight, float &pdf) {
+// The aggregate lobe's sample: lobe by weight, the lobe's own weight and pdf, then every other lobe
+// folded in.
+__device__ __forceinline__ bool base_sample(const Pbr &p, const EnergyTables &E, float r0, float r1, float r2, V3 &wi,
+                                            S4 &weight, float &pdf) {
     if (p.total <= 1.175494351e-38f) return false;
     float x = r0 * p.total, acc = 0.0f;
     int lobe = -1;
@@ -493,5 +538,81 @@ __device__ __forceinline__ bool pbr_sample(const Pbr &p, const EnergyTables &E, 
     wi = d;
     pdf = ptot;
     weight = scale(bc, 1.0f / ptot);
+    return true;
+}
+
+// ---------------------------------------------------------------- the coat
+// The coat's reflection lobe: GGX with the coat's (grey) dielectric Fresnel, f |cos|.
+__device__ __forceinline__ float coat_value(const Pbr &p, V3 wi) {
+    const V3 wo = p.view;
+    const float idotn = dot(p.n, wo), odotn = dot(p.n, wi);
+    if (!(idotn * odotn > 0.0f)) return 0.0f;
+    const V3 h = normalize(wo + wi);
+    const float D = ggx_d(to_local(p.sb, h), kCoatAlpha);
+    const float F = fresnel(kCoatIor, fabsf(dot(wo, h)));
+    const float G = smith_g1(to_local(p.sb, wo), kCoatAlpha) * smith_g1(to_local(p.sb, wi), kCoatAlpha);
+    return G * (D * (1.0f / (4.0f * idotn)) * F);
+}
+__device__ __forceinline__ float coat_pdf(const Pbr &p, V3 wi) {
+    const V3 wo = p.view;
+    const float idotn = dot(p.n, wo), odotn = dot(p.n, wi);
+    if (!(idotn * odotn > 0.0f)) return 0.0f;
+    const V3 h = normalize(wo + wi);
+    const float idoth = dot(wo, h), odoth = dot(wi, h);
+    if (idoth * odoth < 0.0f) return 0.0f;
+    const float D = ggx_d(to_local(p.sb, h), kCoatAlpha);
+    return smith_g1(to_local(p.sb, wo), kCoatAlpha) * (D * (idoth / (4.0f * odoth * idotn)));
+}
+// openpbr_eval: the coat (times its presence) over the base (times in_base_layer_scale; the light
+// side's scale is 1 under OpenPBR's default view-side attenuation).
+__device__ __forceinline__ S4 pbr_eval(const Pbr &p, const EnergyTables &E, V3 wi) {
+    const S4 b = base_eval(p, E, wi);
+    if (!p.c_on) return b;
+    const float c = coat_value(p, wi) * p.coat;
+    S4 r;
+    for (int k = 0; k < 4; ++k) r.v[k] = c + b.v[k] * p.c_scale.v[k];
+    return r;
+}
+// openpbr_pdf: mix(base pdf, coat pdf, coat reflection probability).
+__device__ __forceinline__ float pbr_pdf(const Pbr &p, V3 wi) {
+    const float b = base_pdf(p, wi);
+    return p.c_on ? mixf(b, coat_pdf(p, wi), p.c_prob) : b;
+}
+// openpbr_sample through the coating lobe (openpbr_coating_lobe.h:258-331): the first random number
+// picks the coat or the base and is remapped; the other layer's value and pdf are folded in.
+__device__ __forceinline__ bool pbr_sample(const Pbr &p, const EnergyTables &E, float r0, float r1, float r2, V3 &wi,
+                                           S4 &weight, float &pdf) {
+    if (!p.c_on) return base_sample(p, E, r0, r1, r2, wi, weight, pdf);
+    const float cp = p.c_prob;
+    V3 d;
+    S4 comb;
+    float ptot;
+    if (r0 < cp) {
+        const V3 wo = p.view;
+        const float idotn = dot(wo, p.n);
+        const V3 h = to_world(p.sb, sample_vndf(kCoatAlpha, to_local(p.sb, wo), fminf(r0 * (1.0f / cp), kLargestBelowOne), r1));
+        const float idoth = dot(wo, h);
+        if (idoth < 0.0f) return false;
+        d = normalize(-wo + h * (2.0f * idoth));
+        const float odotn = dot(p.n, d);
+        if (idotn * odotn <= 0.0f) return false;
+        const float D = ggx_d(to_local(p.sb, h), kCoatAlpha);
+        if (D <= 0.0f) return false;
+        const float pd = D * smith_g1(to_local(p.sb, wo), kCoatAlpha) * idoth / (4.0f * dot(d, h) * idotn);
+        const float bc = fresnel(kCoatIor, idoth) * smith_g1(to_local(p.sb, d), kCoatAlpha) * pd * p.coat;
+        const S4 be = base_eval(p, E, d);
+        for (int k = 0; k < 4; ++k) comb.v[k] = bc + be.v[k] * p.c_scale.v[k];
+        ptot = mixf(base_pdf(p, d), pd, cp);
+    } else {
+        S4 w;
+        float pd;
+        if (!base_sample(p, E, fminf((r0 - cp) * (1.0f / (1.0f - cp)), kLargestBelowOne), r1, r2, d, w, pd)) return false;
+        const float ce = coat_value(p, d) * p.coat;
+        for (int k = 0; k < 4; ++k) comb.v[k] = ce + (w.v[k] * pd) * p.c_scale.v[k];
+        ptot = mixf(pd, coat_pdf(p, d), cp);
+    }
+    wi = d;
+    pdf = ptot;
+    weight = scale(comb, 1.0f / ptot);
     return true;
 }

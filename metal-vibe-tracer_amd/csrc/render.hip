@@ -216,7 +216,7 @@ struct Mat {
     float4 lagrange;
     V3 emission;
     float weight;
-    float roughness, ior, metalness, transmission;
+    float roughness, ior, metalness, transmission, coat;
 };
 __device__ __forceinline__ Mat material(const RenderScene &sc, uint32_t slot) {
     Mat m;
@@ -230,14 +230,16 @@ __device__ __forceinline__ Mat material(const RenderScene &sc, uint32_t slot) {
         m.ior = 1.5f;
         m.metalness = 0.0f;
         m.transmission = 0.0f;
+        m.coat = 0.0f;
         return m;
     }
-    const float4 *r = reinterpret_cast<const float4 *>(sc.materials) + 4 * (size_t)slot;
+    const float4 *r = reinterpret_cast<const float4 *>(sc.materials) + (kMaterialFloats / 4) * (size_t)slot;
     float4 r0 = r[0], r2 = r[2], r3 = r[3];
     m.roughness = r3.x;
     m.ior = r3.y;
     m.metalness = r3.z;
     m.transmission = r3.w;
+    m.coat = r[4].x;
     m.type = __float_as_uint(r0.x);
     m.albedo = v3(r0.y, r0.z, r0.w);
     m.lagrange = r[1];
@@ -291,7 +293,7 @@ __device__ __forceinline__ S4 eval_bsdf_pdf(const RenderScene &sc, const Spectra
     if (cosine < 1e-7f || dot(s.n, wo) <= 0.0f) return s4(0.0f);
     const EnergyTables E{sc.energy_metal, sc.energy_luts};
     Pbr p;
-    pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, s.n, wo, !s.front,
+    pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, m.coat, s.n, wo, !s.front,
                 reflectance(sp, lanes, m.lagrange));
     pdf = pbr_pdf(p, wi);
     if (!(m.transmission > 0.0f) && dot(wi, s.ng) <= 0.0f) return s4(0.0f);
@@ -350,7 +352,7 @@ __device__ __forceinline__ bool sample_bsdf(const RenderScene &sc, const Spectra
     if constexpr (!kLayered) return false;
     const EnergyTables E{sc.energy_metal, sc.energy_luts};
     Pbr p;
-    pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, s.n, -in, !s.front,
+    pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, m.coat, s.n, -in, !s.front,
                 reflectance(sp, lanes, m.lagrange));
     float r0, r1, r2;
     rand_f3(seed, r0, r1, r2);

@@ -12,11 +12,13 @@
 
 namespace vt {
 
-// Material slot record: 4 x float4 per slot (indexed by the face tag).
+// Material slot record: 5 x float4 per slot (indexed by the face tag).
 //   [0] type (as float bits), albedo rgb
 //   [1] reflectance Lagrange multipliers (L0, L1, L2, w): w = 1 MESE, w = 2 flat grey in x
 //   [2] emission rgb, light-selection weight (luminance; 0 = not an emitter)
-//   [3] roughness, ior, metalness, specular weight (OpenPBR inputs)
+//   [3] roughness, ior, metalness, transmission (OpenPBR inputs)
+//   [4] coat weight, 3 unused
+constexpr uint32_t kMaterialFloats = 20;
 enum MaterialType : uint32_t { kDiffuse = 0, kGlossy = 1, kDielectric = 2, kEmissive = 3, kOpenPbr = 4 };
 
 struct RenderScene {
@@ -24,7 +26,7 @@ struct RenderScene {
     const float *vertices;     // 8 floats per vertex (document order)
     const uint32_t *faces;     // 4 u32 per face (document order)
     uint32_t face_count;
-    const float *materials;    // 16 floats per slot
+    const float *materials;    // kMaterialFloats per slot
     uint32_t material_count;
     const uint32_t *emitter_face;  // emitting faces (power sampling), ascending face index
     const float *emitter_cdf;      // cumulative area x weight / total; last = 1

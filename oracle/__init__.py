@@ -348,9 +348,10 @@ def spectral_data():
 
 
 def material_records(materials):
-    """materials: list of dicts with type, albedo, emission -> (records (n, 16) float32, weights)."""
+    """materials: list of dicts with type, albedo, emission, and for the layered types roughness, ior,
+    metalness, transmission, coat -> (records (n, 20) float32, weights)."""
     d, _ = spectral_data()
-    rec = np.zeros((len(materials), 16), np.float32)
+    rec = np.zeros((len(materials), 20), np.float32)
     weights = np.zeros(len(materials))
     for k, m in enumerate(materials):
         rec[k, 0] = np.frombuffer(np.uint32(m["type"]).tobytes(), np.float32)[0]
@@ -363,6 +364,7 @@ def material_records(materials):
             weights[k] = max(1e-8, 0.2126 * float(e[0]) + 0.7152 * float(e[1]) + 0.0722 * float(e[2]))
         rec[k, 11] = np.float32(weights[k])
         rec[k, 12:16] = (m.get("roughness", 0.5), m.get("ior", 1.5), m.get("metalness", 0.0), m.get("transmission", 0.0))
+        rec[k, 16] = m.get("coat", 0.0)
     return rec, weights
 
 
@@ -403,11 +405,12 @@ def build_scene_tree(vertices, faces, ids, weights):
 
 
 class BSDF:
-    """The oracle's layered BSDF subset for one material: type 1 (GLOSSY: metal) or 4 (OPENPBR)."""
+    """The oracle's layered BSDF subset for one material: type 1 (GLOSSY: metal) or 4 (OPENPBR), with a
+    coat of weight `coat` (roughness 0.12, IOR 1.6)."""
 
     def __init__(self, type, roughness, ior=1.5, metalness=0.0, transmission=0.0, rho=(0.8, 0.8, 0.8, 0.8),
-                 precision="f32"):
-        self.params = np.array([type, roughness, ior, metalness, transmission], np.float32)
+                 precision="f32", coat=0.0):
+        self.params = np.array([type, roughness, ior, metalness, transmission, coat], np.float32)
         self.rho = np.asarray(rho, np.float32)
         self.sfx = precision
 

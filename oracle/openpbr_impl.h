@@ -5,9 +5,16 @@
  *
  * The reference evaluates the vendored Adobe OpenPBR (Vendor/OpenPBR, pinned c91aad1) through
  * prepare_openpbr_lanes / prepare_bsdf / prepared_eval / prepared_pdf / prepared_sample
- * (main.swift:2485-2645).  Restated here, for coat = fuzz = subsurface = thin film = dispersion =
- * anisotropy = 0 and base_diffuse_roughness = 0 (the inputs those materials set):
- *   openpbr_prepare_lobes          Vendor/OpenPBR/impl/openpbr_bsdf.h:289-560, 600-860
+ * (main.swift:2485-2645).  Restated here for the inputs those materials set: fuzz = subsurface = thin
+ * film = dispersion = anisotropy = 0, base_diffuse_roughness = 0, and a coat of weight `coat` with
+ * coat_roughness 0.12 (main.swift:2494-2495) and OpenPBR's other coat defaults (coat_ior 1.6, white
+ * coat_color, coat_darkening 1: Vendor/OpenPBR/openpbr_resolved_inputs.h:123-127):
+ *   openpbr_prepare_lobes          Vendor/OpenPBR/impl/openpbr_bsdf.h:289-560, 600-860 (the coat's
+ *                                  IORs, roughening and darkening at :339-470, 498-552, 831-840)
+ *   coating lobe                   impl/openpbr_coating_lobe.h (with the fuzz lobe at weight 0 passing
+ *                                  through, impl/openpbr_fuzz_lobe.h:437-545) and its reflection lobe,
+ *                                  impl/openpbr_generic_minimal_microfacet_lobe.h with
+ *                                  impl/openpbr_ior_reflection_coefficient.h
  *   comprehensive microfacet lobe  impl/openpbr_comprehensive_microfacet_lobe.h (eval, pdf, sample)
  *   reflection/transmission coeff  impl/openpbr_reflection_transmission_coefficient.h
  *   dielectric / metal MMS lobes   impl/openpbr_microfacet_multiple_scattering_lobes.h
@@ -192,6 +199,10 @@ static real metal_avg_complement(real alpha) {
     return (real)t[ix] + ((real)t[ix + 1] - (real)t[ix]) * (x - (real)ix);
 }
 static real clamp_avg(real v) { return RFMAX(v, (real)1e-12); }
+/* mix(x, y, a) = x + (y - x) a, as the shading language defines it. */
+static real mixr(real x, real y, real a) { return x + (y - x) * a; }
+#define COAT_IOR ((real)1.6)
+#define COAT_ROUGHNESS ((real)0.12)
 
 /* ------------------------------------------------------------------ prepared BSDF */
 typedef struct {
@@ -215,9 +226,13 @@ typedef struct {
     real diffuse_comp;       /* cached_specular_energy_compensation */
     real weights[4];         /* specular, dielectric MMS, metal MMS, diffuse */
     real total;
+    /* coating lobe, used when a coated surface is entered from outside */
+    int c_on;
+    real coat, c_alpha, c_prob;  /* presence, coat alpha (clamped), coat reflection probability */
+    s4 c_scale;              /* in_base_layer_scale: (1 - in_reflected) x darkening, per lane */
 } pbr_t;
 
-typedef struct { real type, roughness, ior, metalness, transmission; } pbr_inputs;
+typedef struct { real type, roughness, ior, metalness, transmission, coat; } pbr_inputs;
 
 /* all_coefficients at cosine c: reflection and transmission per lane. */
 static void coefficients(const pbr_t *p, real c, s4 *refl, s4 *trans) {
@@ -253,21 +268,36 @@ static void pbr_prepare(pbr_t *p, const pbr_inputs *in, v3 n, v3 wo, int inside_
     int inside = back;  /* not thin-walled */
     p->spec_basis = back ? invert_basis(gb) : gb;
     p->n = p->spec_basis.n;
-    /* IORs (no coat: the surrounding medium is the exterior, IOR 1). */
-    real wsi = (real)1 * apply_specular_weight_to_ior(ior / (real)1, spec_w);
-    real r_refr, r_trans, r_opaque;
+    /* IORs (openpbr_bsdf.h:339-418): the exterior is IOR 1; entering a coated surface, the base sits
+     * under the coat, whose IOR is blended by the coat weight. */
+    const real coat = in->coat;
+    const int enter_coat = coat > 0 && !inside;
+    const real wci = enter_coat ? mixr(1, COAT_IOR, coat) : (real)1;
+    real wsi = wci * apply_specular_weight_to_ior(ior / wci, spec_w);
+    real r_refr, r_trans, r_opaque, rel_coat = 1;
     if (inside) {
         r_refr = (real)1 / wsi;
         r_trans = r_refr;
         r_opaque = wsi / (real)1;
     } else {
         r_refr = wsi / (real)1;
-        r_trans = r_refr;
+        if (enter_coat) {
+            /* no flip from external to internal reflection under the coat */
+            r_trans = wci > wsi && wsi >= 1 ? wci / wsi : wsi / wci;
+            rel_coat = COAT_IOR / (real)1;
+        } else {
+            r_trans = r_refr;
+        }
         r_opaque = r_trans;
     }
     if (r_refr >= 1) r_refr = RFMAX(r_refr, (real)1.0001);
     else r_refr = RFMIN(r_refr, (real)1 / (real)1.0001);
-    real salpha = rough * rough;
+    /* Roughness under the coat (openpbr_bsdf.h:424-460; fuzz 0 leaves the coat roughness as given). */
+    const real c_rough = COAT_ROUGHNESS;
+    const real adjust = mixr(1, 0, rel_coat >= 1 ? 1 / rel_coat : rel_coat);
+    const real adj_rough =
+        mixr(rough, RSQRT(RSQRT(RFMIN((real)1, sq(sq(rough)) + adjust * sq(sq(c_rough))))), coat);
+    real salpha = adj_rough * adj_rough;
     p->iso_alpha = salpha;
     p->alpha = RFMAX(salpha, (real)1e-6);
     real non_trans = 1 - transw, dielectric = 1 - metalness;
@@ -312,6 +342,34 @@ static void pbr_prepare(pbr_t *p, const pbr_inputs *in, v3 n, v3 wo, int inside_
     p->weights[2] = p->m_on ? max3(p->m_scale) * p->m_ec_idotn : 0;
     p->weights[3] = max3(sscale(p->diffuse_albedo, p->diffuse_comp));
     p->total = p->weights[0] + p->weights[1] + p->weights[2] + p->weights[3];
+    /* The coating lobe (openpbr_bsdf.h:498-552, 831-840; openpbr_coating_lobe.h:167-185, 137-155).
+     * Hit from inside, the coat only tints (white here: no change); at weight 0 it passes through. */
+    p->c_on = enter_coat;
+    p->coat = coat;
+    p->c_alpha = RFMAX(sq(c_rough), (real)1e-6);
+    p->c_prob = 0;
+    p->c_scale = S4(1);
+    if (enter_coat) {
+        /* darkening by internal reflections under the coat, OpenPBR v1.1.1's interfaced-Lambertian
+         * Delta = (1 - K) / (1 - E_b K), lerped in by the coat weight */
+        const real ks = average_fresnel(rel_coat);
+        const real kr = 1 - (1 - average_fresnel(rel_coat)) / sq(rel_coat);
+        const real so = average_fresnel(r_opaque), st = average_fresnel(r_trans);
+        const real spec_base = (opaque * so + ((real)0 + trans) * st) + metalness;
+        const real kk = mixr(ks, kr, mixr(1, adj_rough, spec_base));
+        real refl = 0;
+        if (idotn > 0) {
+            const real c_iso = sq(c_rough);
+            refl = (c_iso == 0 ? fresnel(rel_coat, idotn) : 1 - opaque_complement(rel_coat, c_iso, idotn)) * coat;
+        }
+        for (int k = 0; k < 4; ++k) {
+            const real eb = ((dmetal * avg_f.v[k] + opaque * mixr(rho.v[k], 1, so)) + (real)0) + trans;
+            const real delta = (1 - kk) / (1 - eb * kk);
+            const real dark = vo_get_fault() == 13 ? (real)1 : mixr(1, delta, coat * (real)1);
+            p->c_scale.v[k] = (1 - refl) * dark;
+        }
+        p->c_prob = safe_div(refl, refl + p->total * max3(p->c_scale), (real)0.5);
+    }
 }
 
 /* ---- lobe values (f |cos|) and pdfs */
@@ -425,12 +483,12 @@ static s4 diffuse_value(const pbr_t *p, v3 wi) {
 }
 static real diffuse_pdf(const pbr_t *p, v3 wi) { return RFMAX(dot(wi, p->n), (real)0) * (real)0.318309886; }
 
-/* openpbr_eval: f |cos| per lane. */
-static s4 pbr_eval(const pbr_t *p, v3 wi) {
+/* The base layer (the aggregate lobe): f |cos| per lane. */
+static s4 base_eval(const pbr_t *p, v3 wi) {
     return sadd(sadd(sadd(spec_value(p, wi), dmms_value(p, wi)), mmms_value(p, wi)), diffuse_value(p, wi));
 }
-/* openpbr_pdf (aggregate): weight-averaged lobe pdfs. */
-static real pbr_pdf(const pbr_t *p, v3 wi) {
+/* The aggregate lobe's pdf: weight-averaged lobe pdfs. */
+static real base_pdf(const pbr_t *p, v3 wi) {
     real s = p->weights[0] * spec_pdf(p, wi) + p->weights[1] * dmms_pdf(p, wi) + p->weights[2] * mmms_pdf(p, wi) +
              p->weights[3] * diffuse_pdf(p, wi);
     return p->total > 0 ? s / p->total : 0;
@@ -438,7 +496,7 @@ static real pbr_pdf(const pbr_t *p, v3 wi) {
 /* openpbr_sample (aggregate lobe, openpbr_aggregate_lobe.h): choose a lobe by weight, sample it
  * with its own weight and pdf, then fold in every other lobe's value and pdf (no lobe of the subset
  * is a delta).  Returns 0 when the sample is invalid. */
-static int pbr_sample(const pbr_t *p, real r0, real r1, real r2, v3 *wi, s4 *weight, real *pdf) {
+static int base_sample(const pbr_t *p, real r0, real r1, real r2, v3 *wi, s4 *weight, real *pdf) {
     if (p->total <= (real)1.175494351e-38) return 0;
     real x = r0 * p->total, acc = 0;
     int lobe = -1;
@@ -539,6 +597,93 @@ static int pbr_sample(const pbr_t *p, real r0, real r1, real r2, v3 *wi, s4 *wei
     *wi = d;
     *pdf = ptot;
     *weight = sscale(bc, (real)1 / ptot);
+    return 1;
+}
+
+/* ---- the coat's reflection lobe: GGX with the coat's dielectric Fresnel (grey) */
+static real coat_value(const pbr_t *p, v3 wi) {
+    v3 wo = p->view;
+    real idotn = dot(p->n, wo), odotn = dot(p->n, wi);
+    if (!(idotn * odotn > 0)) return 0;
+    v3 h = norm(add(wo, wi));
+    real idoth = dot(wo, h);
+    real D = ggx_d(to_local(&p->spec_basis, h), p->c_alpha);
+    real other = (real)1 / ((real)4 * idotn);
+    real F = fresnel(COAT_IOR, RFABS(idoth));
+    real G = smith_g1(to_local(&p->spec_basis, wo), p->c_alpha) * smith_g1(to_local(&p->spec_basis, wi), p->c_alpha);
+    return G * (D * other * F);
+}
+static real coat_pdf(const pbr_t *p, v3 wi) {
+    v3 wo = p->view;
+    real idotn = dot(p->n, wo), odotn = dot(p->n, wi);
+    if (!(idotn * odotn > 0)) return 0;
+    v3 h = norm(add(wo, wi));
+    real idoth = dot(wo, h), odoth = dot(wi, h);
+    if (idoth * odoth < 0) return 0;
+    real D = ggx_d(to_local(&p->spec_basis, h), p->c_alpha);
+    real other = idoth / ((real)4 * odoth * idotn);
+    return smith_g1(to_local(&p->spec_basis, wo), p->c_alpha) * (D * other);
+}
+static int coat_sample(const pbr_t *p, real r0, real r1, v3 *wi, real *weight, real *pdf) {
+    v3 wo = p->view;
+    real idotn = dot(wo, p->n);
+    v3 h = to_world(&p->spec_basis, sample_vndf(p->c_alpha, to_local(&p->spec_basis, wo), r0, r1));
+    real idoth = dot(wo, h);
+    if (idoth < 0) return 0;
+    v3 d = norm(add(neg(wo), mulv(h, (real)2 * idoth)));
+    real odotn = dot(p->n, d);
+    if (idotn * odotn <= 0) return 0;
+    real odoth = dot(d, h);
+    real w = fresnel(COAT_IOR, idoth) * smith_g1(to_local(&p->spec_basis, d), p->c_alpha);
+    real D = ggx_d(to_local(&p->spec_basis, h), p->c_alpha);
+    if (D <= 0) return 0;
+    *pdf = D * smith_g1(to_local(&p->spec_basis, wo), p->c_alpha) * idoth / ((real)4 * odoth * idotn);
+    *weight = w;
+    *wi = d;
+    return 1;
+}
+
+/* openpbr_eval: the coat (scaled by its presence) over the base (scaled by in_base_layer_scale; the
+ * light side's scale is 1 with OpenPBR's default view-side attenuation), f |cos| per lane. */
+static s4 pbr_eval(const pbr_t *p, v3 wi) {
+    s4 b = base_eval(p, wi);
+    if (!p->c_on) return b;
+    real c = coat_value(p, wi);
+    s4 r;
+    for (int k = 0; k < 4; ++k) r.v[k] = c * p->coat + b.v[k] * p->c_scale.v[k];
+    return r;
+}
+/* openpbr_pdf: mix(base pdf, coat pdf, coat reflection probability). */
+static real pbr_pdf(const pbr_t *p, v3 wi) {
+    if (!p->c_on) return base_pdf(p, wi);
+    return mixr(base_pdf(p, wi), coat_pdf(p, wi), p->c_prob);
+}
+/* openpbr_sample through the coating lobe (openpbr_coating_lobe.h:258-331): the first random number
+ * chooses the coat or the base and is remapped; the other layer's value and pdf are folded in. */
+static int pbr_sample(const pbr_t *p, real r0, real r1, real r2, v3 *wi, s4 *weight, real *pdf) {
+    if (!p->c_on) return base_sample(p, r0, r1, r2, wi, weight, pdf);
+    const real cp = p->c_prob;
+    v3 d;
+    real ptot;
+    s4 comb;
+    if (r0 < cp) {
+        real x = RFMIN(r0 * ((real)1 / cp), LARGEST_BELOW_ONE), w, pd;
+        if (!coat_sample(p, x, r1, &d, &w, &pd)) return 0;
+        real bc = w * pd;
+        s4 be = base_eval(p, d);
+        for (int k = 0; k < 4; ++k) comb.v[k] = bc * p->coat + be.v[k] * p->c_scale.v[k];
+        ptot = mixr(base_pdf(p, d), pd, cp);
+    } else {
+        real x = RFMIN((r0 - cp) * ((real)1 / (1 - cp)), LARGEST_BELOW_ONE), pd;
+        s4 w;
+        if (!base_sample(p, x, r1, r2, &d, &w, &pd)) return 0;
+        real ce = coat_value(p, d);
+        for (int k = 0; k < 4; ++k) comb.v[k] = ce * p->coat + (w.v[k] * pd) * p->c_scale.v[k];
+        ptot = mixr(pd, coat_pdf(p, d), cp);
+    }
+    *wi = d;
+    *pdf = ptot;
+    *weight = sscale(comb, (real)1 / ptot);
     return 1;
 }
 

@@ -170,7 +170,9 @@ def materials_cornell(variant="layered", light_size=0.25):
     materials.  'layered': rough gold-like metal (GLOSSY, roughness 0.3), rough glass (OPENPBR,
     transmission 1, roughness 0.2, IOR 1.5) and a glossy plastic floor (OPENPBR, roughness 0.4);
     'delta': a mirror (GLOSSY, roughness 0) and smooth glass (DIELECTRIC, IOR 1.5), Lambert floor;
-    'opaque': the metal, a half-metallic blue OPENPBR box (roughness 0.25, IOR 1.6) and the plastic floor."""
+    'opaque': the metal, a half-metallic blue OPENPBR box (roughness 0.25, IOR 1.6) and the plastic floor;
+    'coated': coats (roughness 0.12, IOR 1.6) over them -- a fully coated metal, a half-coated rough glass,
+    and a red plastic floor under a 0.7 coat."""
     import vt
     base = vt.cornell(light_size)
     faces = base.faces.copy()
@@ -184,6 +186,10 @@ def materials_cornell(variant="layered", light_size=0.25):
         mats += [dict(type=1, albedo=(0.95, 0.64, 0.54), roughness=0.3),
                  dict(type=4, albedo=(0.2, 0.4, 0.8), roughness=0.25, ior=1.6, metalness=0.5),
                  dict(type=4, albedo=(0.73, 0.73, 0.73), roughness=0.4, ior=1.5)]
+    elif variant == "coated":
+        mats += [dict(type=1, albedo=(0.95, 0.64, 0.54), roughness=0.3, coat=1.0),
+                 dict(type=4, albedo=(1.0, 1.0, 1.0), roughness=0.3, ior=1.5, transmission=1.0, coat=0.5),
+                 dict(type=4, albedo=(0.7, 0.15, 0.1), roughness=0.5, ior=1.5, coat=0.7)]
     elif variant == "layered":
         mats += [dict(type=1, albedo=(0.95, 0.64, 0.54), roughness=0.3),
                  dict(type=4, albedo=(1.0, 1.0, 1.0), roughness=0.2, ior=1.5, transmission=1.0),
@@ -198,7 +204,8 @@ def materials_cornell(variant="layered", light_size=0.25):
 def vt_materials(mats):
     import vt
     return [vt.material(m["type"], m["albedo"], m.get("emission", (0, 0, 0)), m.get("roughness", 0.5),
-                        m.get("ior", 1.5), m.get("metalness", 0.0), 1.0, m.get("transmission", 0.0)) for m in mats]
+                        m.get("ior", 1.5), m.get("metalness", 0.0), 1.0, m.get("transmission", 0.0), m.get("coat", 0.0))
+            for m in mats]
 
 
 def sky_map(width=64, height=32, sun=(0.3, 0.28), sun_radiance=300.0, seed=2):

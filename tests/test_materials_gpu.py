@@ -56,7 +56,8 @@ def _compare(got, ref, label, tol):
     return close
 
 
-@pytest.mark.parametrize("variant,tol,min_close", [("opaque", 1e-3, 0.97), ("layered", 5e-3, 0.9), ("delta", 5e-3, 0.9)])
+@pytest.mark.parametrize("variant,tol,min_close", [("opaque", 1e-3, 0.97), ("layered", 5e-3, 0.9), ("delta", 5e-3, 0.9),
+                                                  ("coated", 5e-3, 0.9)])
 @pytest.mark.parametrize("depth", [3, 8])
 def test_materials_match_oracle(gpu_ctx, variant, tol, min_close, depth):
     # Measured: opaque 8e-6, glass variants 1e-3 to 2.7e-3 (a refraction whose last bit differs sends
@@ -67,7 +68,8 @@ def test_materials_match_oracle(gpu_ctx, variant, tol, min_close, depth):
     assert _compare(got, ref, f"{variant} depth {depth}", tol) > min_close
 
 
-@pytest.mark.parametrize("variant,tol,min_close", [("opaque", 1e-3, 0.97), ("layered", 5e-3, 0.9), ("delta", 5e-3, 0.9)])
+@pytest.mark.parametrize("variant,tol,min_close", [("opaque", 1e-3, 0.97), ("layered", 5e-3, 0.9), ("delta", 5e-3, 0.9),
+                                                  ("coated", 5e-3, 0.9)])
 def test_materials_in_z_sampler_mode_match_oracle(gpu_ctx, variant, tol, min_close):
     """Z mode through every BSDF sampler: the layered OpenPBR lobes (one 3D constituent per scatter),
     Lambert, and the dielectric's 1D draws."""
@@ -77,12 +79,13 @@ def test_materials_in_z_sampler_mode_match_oracle(gpu_ctx, variant, tol, min_clo
     assert _compare(got, ref, f"z {variant}", tol) > min_close
 
 
+@pytest.mark.parametrize("variant", ["opaque", "coated"])
 @pytest.mark.parametrize("mode", [2, 3])
-def test_nee_only_and_bsdf_only_match_oracle(gpu_ctx, mode):
-    mesh, mats = scenes.materials_cornell("opaque", 0.5)
+def test_nee_only_and_bsdf_only_match_oracle(gpu_ctx, mode, variant):
+    mesh, mats = scenes.materials_cornell(variant, 0.5)
     got = _render(gpu_ctx, mesh, mats, 96, 32, 3, mode)
     ref = _oracle(mesh, mats, 96, 32, 3, mode)
-    assert _compare(got, ref, f"mode {mode}", 2e-3) > 0.95
+    assert _compare(got, ref, f"{variant} mode {mode}", 2e-3) > 0.95
 
 
 def test_restir_with_layered_primaries_matches_oracle(gpu_ctx):
