@@ -51,17 +51,27 @@ __device__ __forceinline__ Basis make_basis(V3 n) {
 __device__ __forceinline__ V3 to_local(const Basis &B, V3 d) { return v3(dot(d, B.t), dot(d, B.b), dot(d, B.n)); }
 __device__ __forceinline__ V3 to_world(const Basis &B, V3 d) { return B.t * d.x + B.b * d.y + B.n * d.z; }
 
-__device__ __forceinline__ float ggx_d(V3 h, float alpha) {
-    return 1.0f / (kPi * alpha * alpha * sqf(sqf(h.x / alpha) + sqf(h.y / alpha) + sqf(h.z)));
+// openpbr_make_basis(normal, tangent, handedness 1): the tangent made perpendicular to the normal.
+__device__ __forceinline__ Basis make_basis_t(V3 n, V3 t) {
+    Basis B;
+    B.n = n;
+    B.t = normalize(t - n * dot(t, n));
+    B.b = cross(n, B.t);
+    return B;
 }
-__device__ __forceinline__ float smith_g1(V3 v, float alpha) {
+
+// Anisotropic GGX and Smith G1 with alphas along the basis' tangent (ax) and bitangent (ay).
+__device__ __forceinline__ float ggx_d(V3 h, float ax, float ay) {
+    return 1.0f / (kPi * ax * ay * sqf(sqf(h.x / ax) + sqf(h.y / ay) + sqf(h.z)));
+}
+__device__ __forceinline__ float smith_g1(V3 v, float ax, float ay) {
     float z2 = sqf(v.z);
     if (z2 == 0.0f) return 0.0f;
-    return 2.0f / (1.0f + sqrtf(1.0f + (sqf(alpha * v.x) + sqf(alpha * v.y)) / z2));
+    return 2.0f / (1.0f + sqrtf(1.0f + (sqf(ax * v.x) + sqf(ay * v.y)) / z2));
 }
-// openpbr_sample_aniso_ggx_smith_vndf, isotropic: spherical-cap sampling of the visible normals.
-__device__ __forceinline__ V3 sample_vndf(float alpha, V3 wi, float r0, float r1) {
-    V3 ih = normalize(v3(alpha * wi.x, alpha * wi.y, wi.z));
+// openpbr_sample_aniso_ggx_smith_vndf: spherical-cap sampling of the visible normals.
+__device__ __forceinline__ V3 sample_vndf(float ax, float ay, V3 wi, float r0, float r1) {
+    V3 ih = normalize(v3(ax * wi.x, ay * wi.y, wi.z));
     float c = fminf(ih.z, 1.0f);
     float ang = kTwoPi * r0;
     float opc = 1.0f + c, hz = (1.0f - r1) * opc, omz = r1 * opc, omc = 1.0f - c, opz = hz + omc;
@@ -69,7 +79,7 @@ __device__ __forceinline__ V3 sample_vndf(float alpha, V3 wi, float r0, float r1
     float sn, cs;
     sincosf(ang, &sn, &cs);
     V3 m = v3(tc * cs + ih.x, tc * sn + ih.y, hz);
-    return normalize(v3(alpha * m.x, alpha * m.y, m.z));
+    return normalize(v3(ax * m.x, ay * m.y, m.z));
 }
 __device__ __forceinline__ float fresnel(float eta, float ci) {
     if (eta == 1.0f) return 0.0f;
@@ -185,7 +195,7 @@ constexpr float kCoatAlpha = kCoatRoughness * kCoatRoughness;  // above the 1e-6
 struct Pbr {
     V3 view, n;
     Basis sb;                // specular_anisotropy_basis_ff
-    float alpha, iso_alpha;
+    float ax, ay, iso_alpha; // the specular lobe's alphas (floored at 1e-6) and specular_alpha
     float eta_trans, eta_opaque, eta_refr;
     float scale_trans_refl, scale_opaque_refl, transmission, metal_amount;
     S4 f0;
@@ -218,13 +228,16 @@ __device__ __forceinline__ void pbr_probabilities(const Pbr &p, float c, S4 &ref
 // prepare_openpbr_lanes for the subset.  type: 1 GLOSSY (metalness 1) or 4 OPENPBR; n: shading normal
 // on the arriving side; wo: toward the viewer; inside: hit from the back; rho: base colour per lane.
 __device__ __forceinline__ void pbr_prepare(Pbr &p, const EnergyTables &E, uint32_t type, float roughness, float ior_in,
-                                            float metal_in, float transw, float coat, V3 n, V3 wo, bool inside_flag,
-                                            S4 rho) {
+                                            float metal_in, float transw, float coat, float aniso, V3 tangent, V3 n,
+                                            V3 wo, bool inside_flag, S4 rho) {
     const float rough = fmaxf(0.03f, roughness);
     const float metalness = type == kGlossy ? 1.0f : metal_in;
     const float ior = fmaxf(1.01f, ior_in);
     V3 outward = transw > 0.0f && inside_flag ? -n : n;
-    Basis gb = make_basis(outward);
+    // geometry_basis (main.swift:2508-2509): the tangent frame where the tangent is usable.  Only an
+    // anisotropic lobe depends on the frame's azimuth; isotropic ones keep the normal's own frame.
+    Basis gb = aniso > 0.0f && dot(tangent, tangent) > 0.5f && fabsf(dot(tangent, outward)) < 0.99f
+                   ? make_basis_t(outward, tangent) : make_basis(outward);
     p.view = wo;
     const bool back = dot(wo, gb.n) < 0.0f;
     p.sb = gb;
@@ -257,7 +270,14 @@ __device__ __forceinline__ void pbr_prepare(Pbr &p, const EnergyTables &E, uint3
         mixf(rough, sqrtf(sqrtf(fminf(1.0f, sqf(sqf(rough)) + adjust * sqf(sqf(kCoatRoughness))))), coat);
     const float salpha = adj_rough * adj_rough;
     p.iso_alpha = salpha;
-    p.alpha = fmaxf(salpha, 1e-6f);
+    // openpbr_compute_anisotropic_alpha (openpbr_lobe_utils.h:41-72), floored at 1e-6.
+    p.ax = p.ay = salpha;
+    if (aniso > 0.0f) {
+        p.ax = salpha * sqrtf(2.0f / (1.0f + sqf(1.0f - aniso)));
+        p.ay = (1.0f - aniso) * p.ax;
+    }
+    p.ax = fmaxf(p.ax, 1e-6f);
+    p.ay = fmaxf(p.ay, 1e-6f);
     const float dielectric = 1.0f - metalness;
     const float opaque = (1.0f - transw) * dielectric, trans = dielectric * transw, dmetal = metalness * 1.0f;
     const S4 avg_f = metal_average_f82(rho);
@@ -317,11 +337,11 @@ __device__ __forceinline__ S4 spec_value(const Pbr &p, const EnergyTables &, V3 
     if (idotn * odotn > 0.0f) {
         V3 h = normalize(wo + wi);
         float idoth = dot(wo, h);
-        float D = ggx_d(to_local(p.sb, h), p.alpha);
+        float D = ggx_d(to_local(p.sb, h), p.ax, p.ay);
         float other = 1.0f / (4.0f * idotn);
         S4 F, T;
         pbr_coefficients(p, fabsf(idoth), F, T);
-        float G = smith_g1(to_local(p.sb, wo), p.alpha) * smith_g1(to_local(p.sb, wi), p.alpha);
+        float G = smith_g1(to_local(p.sb, wo), p.ax, p.ay) * smith_g1(to_local(p.sb, wi), p.ax, p.ay);
         return scale(F, G * D * other);
     }
     if (idotn * odotn < 0.0f) {
@@ -330,11 +350,11 @@ __device__ __forceinline__ S4 spec_value(const Pbr &p, const EnergyTables &, V3 
         if (!valid) return s4(0.0f);
         float idoth = dot(wo, h), odoth = dot(wi, h);
         if (idoth * odoth >= 0.0f) return s4(0.0f);
-        float D = ggx_d(to_local(p.sb, h), p.alpha);
+        float D = ggx_d(to_local(p.sb, h), p.ax, p.ay);
         float other = sqf(p.eta_refr) * fabsf(idoth) * fabsf(odoth) / (sqf(idoth + p.eta_refr * odoth) * idotn);
         S4 F, T;
         pbr_coefficients(p, fabsf(idoth), F, T);
-        float G = smith_g1(to_local(p.sb, wo), p.alpha) * smith_g1(to_local(p.sb, wi), p.alpha);
+        float G = smith_g1(to_local(p.sb, wo), p.ax, p.ay) * smith_g1(to_local(p.sb, wi), p.ax, p.ay);
         return scale(T, G * D * other);
     }
     return s4(0.0f);
@@ -348,10 +368,10 @@ __device__ __forceinline__ float spec_pdf(const Pbr &p, V3 wi) {
         V3 h = normalize(wo + wi);
         float idoth = dot(wo, h), odoth = dot(wi, h);
         if (idoth * odoth < 0.0f) return 0.0f;
-        float D = ggx_d(to_local(p.sb, h), p.alpha);
+        float D = ggx_d(to_local(p.sb, h), p.ax, p.ay);
         float other = idoth / (4.0f * odoth * idotn);
         pbr_probabilities(p, fabsf(idoth), F, T, pr, pt);
-        return smith_g1(to_local(p.sb, wo), p.alpha) * (pr * D * other);
+        return smith_g1(to_local(p.sb, wo), p.ax, p.ay) * (pr * D * other);
     }
     if (idotn * odotn < 0.0f) {
         V3 h = -normalize(wo + wi * p.eta_refr);
@@ -359,10 +379,10 @@ __device__ __forceinline__ float spec_pdf(const Pbr &p, V3 wi) {
         if (!valid) return 0.0f;
         float idoth = dot(wo, h), odoth = dot(wi, h);
         if (idoth * odoth >= 0.0f) return 0.0f;
-        float D = ggx_d(to_local(p.sb, h), p.alpha);
+        float D = ggx_d(to_local(p.sb, h), p.ax, p.ay);
         float other = sqf(p.eta_refr) * fabsf(idoth) * fabsf(odoth) / (sqf(idoth + p.eta_refr * odoth) * idotn);
         pbr_probabilities(p, fabsf(idoth), F, T, pr, pt);
-        return smith_g1(to_local(p.sb, wo), p.alpha) * (pt * D * other);
+        return smith_g1(to_local(p.sb, wo), p.ax, p.ay) * (pt * D * other);
     }
     return 0.0f;
 }
@@ -459,7 +479,7 @@ __device__ __forceinline__ bool base_sample(const Pbr &p, const EnergyTables &E,
     float pd;
     if (lobe == 0) {
         float idotn = dot(wo, p.n);
-        V3 h = to_world(p.sb, sample_vndf(p.alpha, to_local(p.sb, wo), x, r1));
+        V3 h = to_world(p.sb, sample_vndf(p.ax, p.ay, to_local(p.sb, wo), x, r1));
         float idoth = dot(wo, h);
         if (idoth < 0.0f) return false;
         S4 F, T;
@@ -471,19 +491,19 @@ __device__ __forceinline__ bool base_sample(const Pbr &p, const EnergyTables &E,
             float odotn = dot(p.n, d);
             if (idotn * odotn <= 0.0f) return false;
             float odoth = dot(d, h);
-            w = scale(scale(F, 1.0f / pr), smith_g1(to_local(p.sb, d), p.alpha));
-            float D = ggx_d(to_local(p.sb, h), p.alpha);
+            w = scale(scale(F, 1.0f / pr), smith_g1(to_local(p.sb, d), p.ax, p.ay));
+            float D = ggx_d(to_local(p.sb, h), p.ax, p.ay);
             if (D <= 0.0f) return false;
-            pd = pr * D * smith_g1(to_local(p.sb, wo), p.alpha) * idoth / (4.0f * odoth * idotn);
+            pd = pr * D * smith_g1(to_local(p.sb, wo), p.ax, p.ay) * idoth / (4.0f * odoth * idotn);
         } else {
             if (!refract_dir(wo, h, idoth, p.eta_refr, d)) return false;
             float odotn = dot(d, p.n);
             if (idotn * odotn >= 0.0f) return false;
             float odoth = dot(d, h);
-            w = scale(scale(T, 1.0f / pt), smith_g1(to_local(p.sb, d), p.alpha));
-            float D = ggx_d(to_local(p.sb, h), p.alpha);
+            w = scale(scale(T, 1.0f / pt), smith_g1(to_local(p.sb, d), p.ax, p.ay));
+            float D = ggx_d(to_local(p.sb, h), p.ax, p.ay);
             if (D <= 0.0f) return false;
-            pd = pt * D * smith_g1(to_local(p.sb, wo), p.alpha) * sqf(p.eta_refr) * fabsf(idoth) * fabsf(odoth) /
+            pd = pt * D * smith_g1(to_local(p.sb, wo), p.ax, p.ay) * sqf(p.eta_refr) * fabsf(idoth) * fabsf(odoth) /
                  (sqf(idoth + p.eta_refr * odoth) * idotn);
         }
     } else if (lobe == 1) {
@@ -548,9 +568,10 @@ __device__ __forceinline__ float coat_value(const Pbr &p, V3 wi) {
     const float idotn = dot(p.n, wo), odotn = dot(p.n, wi);
     if (!(idotn * odotn > 0.0f)) return 0.0f;
     const V3 h = normalize(wo + wi);
-    const float D = ggx_d(to_local(p.sb, h), kCoatAlpha);
+    const float D = ggx_d(to_local(p.sb, h), kCoatAlpha, kCoatAlpha);
     const float F = fresnel(kCoatIor, fabsf(dot(wo, h)));
-    const float G = smith_g1(to_local(p.sb, wo), kCoatAlpha) * smith_g1(to_local(p.sb, wi), kCoatAlpha);
+    const float G =
+        smith_g1(to_local(p.sb, wo), kCoatAlpha, kCoatAlpha) * smith_g1(to_local(p.sb, wi), kCoatAlpha, kCoatAlpha);
     return G * (D * (1.0f / (4.0f * idotn)) * F);
 }
 __device__ __forceinline__ float coat_pdf(const Pbr &p, V3 wi) {
@@ -560,8 +581,8 @@ __device__ __forceinline__ float coat_pdf(const Pbr &p, V3 wi) {
     const V3 h = normalize(wo + wi);
     const float idoth = dot(wo, h), odoth = dot(wi, h);
     if (idoth * odoth < 0.0f) return 0.0f;
-    const float D = ggx_d(to_local(p.sb, h), kCoatAlpha);
-    return smith_g1(to_local(p.sb, wo), kCoatAlpha) * (D * (idoth / (4.0f * odoth * idotn)));
+    const float D = ggx_d(to_local(p.sb, h), kCoatAlpha, kCoatAlpha);
+    return smith_g1(to_local(p.sb, wo), kCoatAlpha, kCoatAlpha) * (D * (idoth / (4.0f * odoth * idotn)));
 }
 // openpbr_eval: the coat (times its presence) over the base (times in_base_layer_scale; the light
 // side's scale is 1 under OpenPBR's default view-side attenuation).
@@ -590,16 +611,17 @@ __device__ __forceinline__ bool pbr_sample(const Pbr &p, const EnergyTables &E, 
     if (r0 < cp) {
         const V3 wo = p.view;
         const float idotn = dot(wo, p.n);
-        const V3 h = to_world(p.sb, sample_vndf(kCoatAlpha, to_local(p.sb, wo), fminf(r0 * (1.0f / cp), kLargestBelowOne), r1));
+        const float x = fminf(r0 * (1.0f / cp), kLargestBelowOne);
+        const V3 h = to_world(p.sb, sample_vndf(kCoatAlpha, kCoatAlpha, to_local(p.sb, wo), x, r1));
         const float idoth = dot(wo, h);
         if (idoth < 0.0f) return false;
         d = normalize(-wo + h * (2.0f * idoth));
         const float odotn = dot(p.n, d);
         if (idotn * odotn <= 0.0f) return false;
-        const float D = ggx_d(to_local(p.sb, h), kCoatAlpha);
+        const float D = ggx_d(to_local(p.sb, h), kCoatAlpha, kCoatAlpha);
         if (D <= 0.0f) return false;
-        const float pd = D * smith_g1(to_local(p.sb, wo), kCoatAlpha) * idoth / (4.0f * dot(d, h) * idotn);
-        const float bc = fresnel(kCoatIor, idoth) * smith_g1(to_local(p.sb, d), kCoatAlpha) * pd * p.coat;
+        const float pd = D * smith_g1(to_local(p.sb, wo), kCoatAlpha, kCoatAlpha) * idoth / (4.0f * dot(d, h) * idotn);
+        const float bc = fresnel(kCoatIor, idoth) * smith_g1(to_local(p.sb, d), kCoatAlpha, kCoatAlpha) * pd * p.coat;
         const S4 be = base_eval(p, E, d);
         for (int k = 0; k < 4; ++k) comb.v[k] = bc + be.v[k] * p.c_scale.v[k];
         ptot = mixf(base_pdf(p, d), pd, cp);

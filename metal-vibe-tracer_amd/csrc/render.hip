@@ -179,7 +179,7 @@ struct Surface {
     V3 p, n, ng;
     float error;
     bool front;
-    uint32_t slot;
+    uint32_t slot, face;
 };
 __device__ __forceinline__ void face_corners(const RenderScene &sc, uint32_t face, V3 &a, V3 &b, V3 &c, uint32_t &tag,
                                              const float **va, const float **vb, const float **vc) {
@@ -208,7 +208,23 @@ __device__ __forceinline__ Surface surface_at(const RenderScene &sc, uint32_t fa
     if (dot(s.n, s.ng) < 0.0f) s.n = -s.n;
     s.error = fmaxf(1e-7f, 1.907349e-6f * fmaxf(max_abs(a), fmaxf(max_abs(b), max_abs(c))));
     s.slot = tag;
+    s.face = face;
     return s;
+}
+// The hit's tangent dP/du made perpendicular to the shading normal (mesh_hit_attributes,
+// main.swift:1594-1599); read only by anisotropic lobes.
+__device__ __forceinline__ V3 surface_tangent(const RenderScene &sc, const Surface &s) {
+    V3 a, b, c;
+    uint32_t tag;
+    const float *va, *vb, *vc;
+    face_corners(sc, s.face, a, b, c, tag, &va, &vb, &vc);
+    const V3 e1 = b - a, e2 = c - a;
+    const float d1x = vb[6] - va[6], d1y = vb[7] - va[7], d2x = vc[6] - va[6], d2y = vc[7] - va[7];
+    const float det = d1x * d2y - d1y * d2x;
+    V3 t = fabsf(det) > 1e-8f ? (e1 * d2y - e2 * d1y) * (1.0f / det) : e1;
+    t = t - s.n * dot(t, s.n);
+    if (dot(t, t) < 1e-12f) t = cross(s.n, fabsf(s.n.y) < 0.9f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f));
+    return normalize(t);
 }
 struct Mat {
     uint32_t type;
@@ -216,7 +232,7 @@ struct Mat {
     float4 lagrange;
     V3 emission;
     float weight;
-    float roughness, ior, metalness, transmission, coat;
+    float roughness, ior, metalness, transmission, coat, anisotropy;
 };
 __device__ __forceinline__ Mat material(const RenderScene &sc, uint32_t slot) {
     Mat m;
@@ -231,6 +247,7 @@ __device__ __forceinline__ Mat material(const RenderScene &sc, uint32_t slot) {
         m.metalness = 0.0f;
         m.transmission = 0.0f;
         m.coat = 0.0f;
+        m.anisotropy = 0.0f;
         return m;
     }
     const float4 *r = reinterpret_cast<const float4 *>(sc.materials) + (kMaterialFloats / 4) * (size_t)slot;
@@ -239,7 +256,9 @@ __device__ __forceinline__ Mat material(const RenderScene &sc, uint32_t slot) {
     m.ior = r3.y;
     m.metalness = r3.z;
     m.transmission = r3.w;
-    m.coat = r[4].x;
+    const float4 r4 = r[4];
+    m.coat = r4.x;
+    m.anisotropy = r4.y;
     m.type = __float_as_uint(r0.x);
     m.albedo = v3(r0.y, r0.z, r0.w);
     m.lagrange = r[1];
@@ -293,7 +312,8 @@ __device__ __forceinline__ S4 eval_bsdf_pdf(const RenderScene &sc, const Spectra
     if (cosine < 1e-7f || dot(s.n, wo) <= 0.0f) return s4(0.0f);
     const EnergyTables E{sc.energy_metal, sc.energy_luts};
     Pbr p;
-    pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, m.coat, s.n, wo, !s.front,
+    pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, m.coat, m.anisotropy,
+                m.anisotropy > 0.0f ? surface_tangent(sc, s) : v3(0.0f, 0.0f, 0.0f), s.n, wo, !s.front,
                 reflectance(sp, lanes, m.lagrange));
     pdf = pbr_pdf(p, wi);
     if (!(m.transmission > 0.0f) && dot(wi, s.ng) <= 0.0f) return s4(0.0f);
@@ -352,7 +372,8 @@ __device__ __forceinline__ bool sample_bsdf(const RenderScene &sc, const Spectra
     if constexpr (!kLayered) return false;
     const EnergyTables E{sc.energy_metal, sc.energy_luts};
     Pbr p;
-    pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, m.coat, s.n, -in, !s.front,
+    pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, m.coat, m.anisotropy,
+                m.anisotropy > 0.0f ? surface_tangent(sc, s) : v3(0.0f, 0.0f, 0.0f), s.n, -in, !s.front,
                 reflectance(sp, lanes, m.lagrange));
     float r0, r1, r2;
     rand_f3(seed, r0, r1, r2);
@@ -1879,10 +1900,10 @@ __global__ __launch_bounds__(64) void bake_metal_kernel(float *out) {
     float sum = 0.0f;
     for (uint32_t k = 0; k < 16384u; ++k) {
         const float r0 = ((float)k + 0.5f) / 16384.0f, r1 = (float)reverse_bits32(k) * 2.3283064365386963e-10f;
-        const V3 h = sample_vndf(alpha, v, r0, r1);
+        const V3 h = sample_vndf(alpha, alpha, v, r0, r1);
         const float d = dot(v, h);
         const V3 l = -v + h * (2.0f * d);
-        const float e = l.z > 0.0f ? smith_g1(l, alpha) : 0.0f;
+        const float e = l.z > 0.0f ? smith_g1(l, alpha, alpha) : 0.0f;
         sum += 1.0f - e;
     }
     out[node] = sum / 16384.0f;

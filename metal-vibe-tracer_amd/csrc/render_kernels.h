@@ -17,7 +17,7 @@ namespace vt {
 //   [1] reflectance Lagrange multipliers (L0, L1, L2, w): w = 1 MESE, w = 2 flat grey in x
 //   [2] emission rgb, light-selection weight (luminance; 0 = not an emitter)
 //   [3] roughness, ior, metalness, transmission (OpenPBR inputs)
-//   [4] coat weight, 3 unused
+//   [4] coat weight, specular anisotropy, 2 unused
 constexpr uint32_t kMaterialFloats = 20;
 enum MaterialType : uint32_t { kDiffuse = 0, kGlossy = 1, kDielectric = 2, kEmissive = 3, kOpenPbr = 4 };
 

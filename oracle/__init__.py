@@ -349,7 +349,7 @@ def spectral_data():
 
 def material_records(materials):
     """materials: list of dicts with type, albedo, emission, and for the layered types roughness, ior,
-    metalness, transmission, coat -> (records (n, 20) float32, weights)."""
+    metalness, transmission, coat, anisotropy -> (records (n, 20) float32, weights)."""
     d, _ = spectral_data()
     rec = np.zeros((len(materials), 20), np.float32)
     weights = np.zeros(len(materials))
@@ -365,6 +365,7 @@ def material_records(materials):
         rec[k, 11] = np.float32(weights[k])
         rec[k, 12:16] = (m.get("roughness", 0.5), m.get("ior", 1.5), m.get("metalness", 0.0), m.get("transmission", 0.0))
         rec[k, 16] = m.get("coat", 0.0)
+        rec[k, 17] = m.get("anisotropy", 0.0)
     return rec, weights
 
 
@@ -406,11 +407,12 @@ def build_scene_tree(vertices, faces, ids, weights):
 
 class BSDF:
     """The oracle's layered BSDF subset for one material: type 1 (GLOSSY: metal) or 4 (OPENPBR), with a
-    coat of weight `coat` (roughness 0.12, IOR 1.6)."""
+    coat of weight `coat` (roughness 0.12, IOR 1.6) and a specular lobe of anisotropy `anisotropy` along
+    `tangent`."""
 
     def __init__(self, type, roughness, ior=1.5, metalness=0.0, transmission=0.0, rho=(0.8, 0.8, 0.8, 0.8),
-                 precision="f32", coat=0.0):
-        self.params = np.array([type, roughness, ior, metalness, transmission, coat], np.float32)
+                 precision="f32", coat=0.0, anisotropy=0.0, tangent=(1.0, 0.0, 0.0)):
+        self.params = np.array([type, roughness, ior, metalness, transmission, coat, anisotropy, *tangent], np.float32)
         self.rho = np.asarray(rho, np.float32)
         self.sfx = precision
 

@@ -149,7 +149,7 @@ static v3 cosine_hemisphere(v3 n, real r0, real r1) {
 }
 
 /* ------------------------------------------------------------------ scene access */
-typedef struct { v3 a, b, c, na, nb, nc; uint32_t tag; } tri_t;
+typedef struct { v3 a, b, c, na, nb, nc; real uv[3][2]; uint32_t tag; } tri_t;
 static tri_t tri_of(const vo_render_scene *sc, uint32_t f) {
     const uint32_t *fc = sc->geo->faces + 4 * (size_t)f;
     const float *va = sc->geo->vertices + 8 * (size_t)fc[0], *vb = sc->geo->vertices + 8 * (size_t)fc[1],
@@ -157,6 +157,8 @@ static tri_t tri_of(const vo_render_scene *sc, uint32_t f) {
     tri_t t;
     t.a = ld(va); t.b = ld(vb); t.c = ld(vc);
     t.na = ld(va + 3); t.nb = ld(vb + 3); t.nc = ld(vc + 3);
+    const float *vs[3] = {va, vb, vc};
+    for (int k = 0; k < 3; ++k) { t.uv[k][0] = (real)vs[k][6]; t.uv[k][1] = (real)vs[k][7]; }
     t.tag = fc[3];
     return t;
 }
@@ -164,14 +166,14 @@ typedef struct {
     uint32_t type;
     const float *lagrange;
     v3 emission;
-    real weight, roughness, ior, metalness, transmission, coat;
+    real weight, roughness, ior, metalness, transmission, coat, anisotropy;
 } mat_t;
 static mat_t mat_of(const vo_render_scene *sc, uint32_t slot) {
     static const float grey[4] = {0.7f, 0, 0, 2.0f};
     mat_t m;
     if (slot >= sc->material_count) {
         m.type = 0; m.lagrange = grey; m.emission = V(0, 0, 0); m.weight = 0;
-        m.roughness = (real)0.5; m.ior = (real)1.5; m.metalness = 0; m.transmission = 0; m.coat = 0;
+        m.roughness = (real)0.5; m.ior = (real)1.5; m.metalness = 0; m.transmission = 0; m.coat = 0; m.anisotropy = 0;
         return m;
     }
     const float *r = sc->materials + VO_MATERIAL_FLOATS * (size_t)slot;
@@ -184,6 +186,7 @@ static mat_t mat_of(const vo_render_scene *sc, uint32_t slot) {
     m.metalness = (real)r[14];
     m.transmission = (real)r[15];
     m.coat = (real)r[16];
+    m.anisotropy = (real)r[17];
     if (r[8] > 0 || r[9] > 0 || r[10] > 0) m.type = 3;
     return m;
 }
@@ -197,7 +200,7 @@ static real imported_geometry(v3 p, v3 pos, const tri_t *t) {
 }
 static float f_of(real x) { return (float)x; }
 
-typedef struct { v3 p, n, ng; real error, t; int front; uint32_t slot; } surf_t;
+typedef struct { v3 p, n, ng, tan; real error, t; int front; uint32_t slot; } surf_t;
 
 #include "openpbr_impl.h"
 
@@ -216,6 +219,15 @@ static int trace(const vo_render_scene *sc, v3 o, v3 d, real tmin, surf_t *s, ui
     v3 smooth = add(add(mulv(t.na, b0), mulv(t.nb, b1)), mulv(t.nc, b2));
     s->n = dot(smooth, smooth) > (real)1e-12 ? norm(smooth) : s->ng;
     if (dot(s->n, s->ng) < 0) s->n = neg(s->n);
+    /* the tangent dP/du, made perpendicular to the shading normal (mesh_hit_attributes, main.swift:1596-1599) */
+    v3 e1 = sub(t.b, t.a), e2 = sub(t.c, t.a);
+    real d1x = t.uv[1][0] - t.uv[0][0], d1y = t.uv[1][1] - t.uv[0][1];
+    real d2x = t.uv[2][0] - t.uv[0][0], d2y = t.uv[2][1] - t.uv[0][1];
+    real det = d1x * d2y - d1y * d2x;
+    v3 tg = RFABS(det) > (real)1e-8 ? mulv(sub(mulv(e1, d2y), mulv(e2, d1y)), (real)1 / det) : e1;
+    tg = sub(tg, mulv(s->n, dot(tg, s->n)));
+    if (dot(tg, tg) < (real)1e-12) tg = cross(s->n, RFABS(s->n.y) < (real)0.9 ? V(0, 1, 0) : V(1, 0, 0));
+    s->tan = norm(tg);
     s->error = RFMAX((real)1e-7, (real)1.907349e-6 * RFMAX(maxabs(t.a), RFMAX(maxabs(t.b), maxabs(t.c))));
     s->slot = t.tag;
     s->t = (real)h.t;
@@ -473,7 +485,7 @@ static s4 eval_bsdf_pdf(const vo_render_scene *sc, const lanes_t *l, const surf_
     real cosine = RFABS(dot(s->n, wi));
     if (cosine < (real)1e-7 || dot(s->n, wo) <= 0) return S4(0);
     pbr_t p;
-    pbr_inputs in = {(real)m->type, m->roughness, m->ior, m->metalness, m->transmission, m->coat};
+    pbr_inputs in = {(real)m->type, m->roughness, m->ior, m->metalness, m->transmission, m->coat, m->anisotropy, s->tan};
     pbr_prepare(&p, &in, s->n, wo, !s->front, reflectance(sc, l, m->lagrange));
     *pdf = pbr_pdf(&p, wi);
     if (!(m->transmission > 0) && dot(wi, s->ng) <= 0) return S4(0);
@@ -528,7 +540,8 @@ static int sample_bsdf(const vo_render_scene *sc, const lanes_t *l, const surf_t
         return *pdf > 0 && dot(*dir, s->ng) > 0;
     }
     pbr_t p;
-    pbr_inputs pin = {(real)m->type, m->roughness, m->ior, m->metalness, m->transmission, m->coat};
+    pbr_inputs pin = {(real)m->type, m->roughness, m->ior, m->metalness, m->transmission, m->coat, m->anisotropy,
+                      s->tan};
     pbr_prepare(&p, &pin, s->n, neg(in), !s->front, reflectance(sc, l, m->lagrange));
     real r0, r1, r2;
     rr3(seed, &r0, &r1, &r2);
@@ -555,13 +568,14 @@ static v3 nee(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, cons
     return spectrum_rgb(sc, l, sscale(smul(smul(thr, f), le), cosine * w / ls.pdf));
 }
 
-/* The layered BSDF on its own, for tests: params = type, roughness, ior, metalness, transmission, coat;
+/* The layered BSDF on its own, for tests: params = type, roughness, ior, metalness, transmission, coat,
+ * anisotropy, tangent xyz;
  * n is the shading normal on the arriving side, wo the direction toward the viewer, rho the base
  * colour per lane.  eval: f |cos| per lane and the aggregate pdf of wi. */
 void SUFFIX(vo_bsdf_eval)(const float *params, const float *n, const float *wo, const float *wi, const float *rho,
                           int inside, float *fcos, float *pdf) {
     pbr_t p;
-    pbr_inputs in = {params[0], params[1], params[2], params[3], params[4], params[5]};
+    pbr_inputs in = {params[0], params[1], params[2], params[3], params[4], params[5], params[6], ld(params + 7)};
     s4 r = {{rho[0], rho[1], rho[2], rho[3]}};
     pbr_prepare(&p, &in, ld(n), ld(wo), inside, r);
     s4 f = pbr_eval(&p, ld(wi));
@@ -572,7 +586,7 @@ void SUFFIX(vo_bsdf_eval)(const float *params, const float *n, const float *wo, 
 int SUFFIX(vo_bsdf_sample)(const float *params, const float *n, const float *wo, const float *rho, int inside,
                            const float *u3, float *dir, float *weight, float *pdf) {
     pbr_t p;
-    pbr_inputs in = {params[0], params[1], params[2], params[3], params[4], params[5]};
+    pbr_inputs in = {params[0], params[1], params[2], params[3], params[4], params[5], params[6], ld(params + 7)};
     s4 r = {{rho[0], rho[1], rho[2], rho[3]}};
     pbr_prepare(&p, &in, ld(n), ld(wo), inside, r);
     v3 d;

@@ -56,21 +56,31 @@ static basis_t make_basis(v3 n) {
     B.b = V((real)1 + s * n.x * n.x * a, s * b, -s * n.x);
     return B;
 }
+/* openpbr_make_basis(normal, tangent, handedness 1): the tangent made perpendicular to the normal. */
+static basis_t make_basis_t(v3 n, v3 t) {
+    basis_t B;
+    B.n = n;
+    B.t = norm(sub(t, mulv(n, dot(t, n))));
+    B.b = cross(n, B.t);
+    return B;
+}
 static basis_t invert_basis(basis_t B) { B.t = neg(B.t); B.b = neg(B.b); B.n = neg(B.n); return B; }
 static v3 to_local(const basis_t *B, v3 d) { return V(dot(d, B->t), dot(d, B->b), dot(d, B->n)); }
 static v3 to_world(const basis_t *B, v3 d) { return add(add(mulv(B->t, d.x), mulv(B->b, d.y)), mulv(B->n, d.z)); }
 
-static real ggx_d(v3 h, real alpha) {
-    return (real)1 / (PI_R * alpha * alpha * sq(sq(h.x / alpha) + sq(h.y / alpha) + sq(h.z)));
+/* openpbr_eval_aniso_ggx, openpbr_eval_aniso_smith_g1 (openpbr_lobe_utils.h:76-137): alphas along the
+ * basis' tangent (ax) and bitangent (ay). */
+static real ggx_d(v3 h, real ax, real ay) {
+    return (real)1 / (PI_R * ax * ay * sq(sq(h.x / ax) + sq(h.y / ay) + sq(h.z)));
 }
-static real smith_g1(v3 v, real alpha) {
+static real smith_g1(v3 v, real ax, real ay) {
     real z2 = sq(v.z);
     if (z2 == 0) return 0;
-    return (real)2 / ((real)1 + RSQRT((real)1 + (sq(alpha * v.x) + sq(alpha * v.y)) / z2));
+    return (real)2 / ((real)1 + RSQRT((real)1 + (sq(ax * v.x) + sq(ay * v.y)) / z2));
 }
-/* openpbr_sample_aniso_ggx_smith_vndf (isotropic alpha): spherical-cap VNDF sampling. */
-static v3 sample_vndf(real alpha, v3 wi, real r0, real r1) {
-    v3 ih = norm(V(alpha * wi.x, alpha * wi.y, wi.z));
+/* openpbr_sample_aniso_ggx_smith_vndf: spherical-cap VNDF sampling. */
+static v3 sample_vndf(real ax, real ay, v3 wi, real r0, real r1) {
+    v3 ih = norm(V(ax * wi.x, ay * wi.y, wi.z));
     real c = RFMIN(ih.z, (real)1);
     real ang = (real)6.28318530717958647692 * r0;
     real opc = (real)1 + c;
@@ -80,7 +90,7 @@ static v3 sample_vndf(real alpha, v3 wi, real r0, real r1) {
     real opz = hz + omc;
     real tc = RSQRT(opz * omz);
     v3 m = V(tc * RCOS(ang) + ih.x, tc * RSIN(ang) + ih.y, hz);
-    return norm(V(alpha * m.x, alpha * m.y, m.z));
+    return norm(V(ax * m.x, ay * m.y, m.z));
 }
 static real fresnel(real eta, real ci) {
     if (eta == 1) return 0;
@@ -209,7 +219,7 @@ typedef struct {
     v3 view;                 /* wo */
     basis_t spec_basis;      /* specular_anisotropy_basis_ff */
     v3 n;                    /* normal_ff */
-    real alpha, iso_alpha;   /* microfacet alpha (clamped) and specular_alpha */
+    real ax, ay, iso_alpha;  /* the specular lobe's alphas (clamped) and specular_alpha */
     /* reflection / transmission coefficient */
     real eta_trans, eta_opaque, eta_refr;
     real scale_trans_refl, scale_opaque_refl, transmission, metal_amount;
@@ -232,7 +242,10 @@ typedef struct {
     s4 c_scale;              /* in_base_layer_scale: (1 - in_reflected) x darkening, per lane */
 } pbr_t;
 
-typedef struct { real type, roughness, ior, metalness, transmission, coat; } pbr_inputs;
+typedef struct {
+    real type, roughness, ior, metalness, transmission, coat, anisotropy;
+    v3 tangent;              /* the hit's tangent (dP/du); used by an anisotropic specular lobe */
+} pbr_inputs;
 
 /* all_coefficients at cosine c: reflection and transmission per lane. */
 static void coefficients(const pbr_t *p, real c, s4 *refl, s4 *trans) {
@@ -261,7 +274,11 @@ static void pbr_prepare(pbr_t *p, const pbr_inputs *in, v3 n, v3 wo, int inside_
     real ior = RFMAX((real)1.01, in->ior);
     real transw = in->transmission;
     v3 outward = transw > 0 && inside_flag ? neg(n) : n;
-    basis_t gb = make_basis(outward);
+    /* geometry_basis (main.swift:2508-2509): the tangent frame where the tangent is usable.  Only an
+     * anisotropic lobe depends on the frame's azimuth; isotropic ones keep the normal's own frame. */
+    const real aniso = in->anisotropy;
+    basis_t gb = aniso > 0 && dot(in->tangent, in->tangent) > (real)0.5 && RFABS(dot(in->tangent, outward)) < (real)0.99
+                     ? make_basis_t(outward, in->tangent) : make_basis(outward);
     p->view = wo;
     real cos_theta = dot(wo, gb.n);
     int back = cos_theta < 0;
@@ -299,7 +316,15 @@ static void pbr_prepare(pbr_t *p, const pbr_inputs *in, v3 n, v3 wo, int inside_
         mixr(rough, RSQRT(RSQRT(RFMIN((real)1, sq(sq(rough)) + adjust * sq(sq(c_rough))))), coat);
     real salpha = adj_rough * adj_rough;
     p->iso_alpha = salpha;
-    p->alpha = RFMAX(salpha, (real)1e-6);
+    /* openpbr_compute_anisotropic_alpha (openpbr_lobe_utils.h:41-72), floored at 1e-6 */
+    p->ax = p->ay = salpha;
+    if (aniso > 0) {
+        p->ax = salpha * RSQRT((real)2 / ((real)1 + sq((real)1 - aniso)));
+        p->ay = ((real)1 - aniso) * p->ax;
+    }
+    p->ax = RFMAX(p->ax, (real)1e-6);
+    p->ay = RFMAX(p->ay, (real)1e-6);
+    if (vo_get_fault() == 14) { real t = p->ax; p->ax = p->ay; p->ay = t; }  /* planted */
     real non_trans = 1 - transw, dielectric = 1 - metalness;
     real opaque = non_trans * dielectric;
     real trans = dielectric * transw;
@@ -379,12 +404,12 @@ static s4 spec_value(const pbr_t *p, v3 wi) {
     if (idotn * odotn > 0) {
         v3 h = norm(add(wo, wi));
         real idoth = dot(wo, h);
-        real D = ggx_d(to_local(&p->spec_basis, h), p->alpha);
+        real D = ggx_d(to_local(&p->spec_basis, h), p->ax, p->ay);
         real other = (real)1 / ((real)4 * idotn);
         s4 F, T;
         coefficients(p, RFABS(idoth), &F, &T);
-        real G = smith_g1(to_local(&p->spec_basis, wo), p->alpha) *
-                 (vo_get_fault() == 5 ? (real)1 : smith_g1(to_local(&p->spec_basis, wi), p->alpha));
+        real G = smith_g1(to_local(&p->spec_basis, wo), p->ax, p->ay) *
+                 (vo_get_fault() == 5 ? (real)1 : smith_g1(to_local(&p->spec_basis, wi), p->ax, p->ay));
         return sscale(F, G * D * other);
     }
     if (idotn * odotn < 0) {
@@ -393,11 +418,11 @@ static s4 spec_value(const pbr_t *p, v3 wi) {
         if (!valid) return S4(0);
         real idoth = dot(wo, h), odoth = dot(wi, h);
         if (idoth * odoth >= 0) return S4(0);
-        real D = ggx_d(to_local(&p->spec_basis, h), p->alpha);
+        real D = ggx_d(to_local(&p->spec_basis, h), p->ax, p->ay);
         real other = sq(p->eta_refr) * RFABS(idoth) * RFABS(odoth) / (sq(idoth + p->eta_refr * odoth) * idotn);
         s4 F, T;
         coefficients(p, RFABS(idoth), &F, &T);
-        real G = smith_g1(to_local(&p->spec_basis, wo), p->alpha) * smith_g1(to_local(&p->spec_basis, wi), p->alpha);
+        real G = smith_g1(to_local(&p->spec_basis, wo), p->ax, p->ay) * smith_g1(to_local(&p->spec_basis, wi), p->ax, p->ay);
         return sscale(T, G * D * other);
     }
     return S4(0);
@@ -411,10 +436,10 @@ static real spec_pdf(const pbr_t *p, v3 wi) {
         v3 h = norm(add(wo, wi));
         real idoth = dot(wo, h), odoth = dot(wi, h);
         if (idoth * odoth < 0) return 0;
-        real D = ggx_d(to_local(&p->spec_basis, h), p->alpha);
+        real D = ggx_d(to_local(&p->spec_basis, h), p->ax, p->ay);
         real other = idoth / ((real)4 * odoth * idotn);
         probabilities(p, RFABS(idoth), &F, &T, &pr, &pt);
-        return smith_g1(to_local(&p->spec_basis, wo), p->alpha) * (pr * D * other);
+        return smith_g1(to_local(&p->spec_basis, wo), p->ax, p->ay) * (pr * D * other);
     }
     if (idotn * odotn < 0) {
         v3 h = neg(norm(add(wo, mulv(wi, p->eta_refr))));
@@ -422,10 +447,10 @@ static real spec_pdf(const pbr_t *p, v3 wi) {
         if (!valid) return 0;
         real idoth = dot(wo, h), odoth = dot(wi, h);
         if (idoth * odoth >= 0) return 0;
-        real D = ggx_d(to_local(&p->spec_basis, h), p->alpha);
+        real D = ggx_d(to_local(&p->spec_basis, h), p->ax, p->ay);
         real other = sq(p->eta_refr) * RFABS(idoth) * RFABS(odoth) / (sq(idoth + p->eta_refr * odoth) * idotn);
         probabilities(p, RFABS(idoth), &F, &T, &pr, &pt);
-        return smith_g1(to_local(&p->spec_basis, wo), p->alpha) * (pt * D * other);
+        return smith_g1(to_local(&p->spec_basis, wo), p->ax, p->ay) * (pt * D * other);
     }
     return 0;
 }
@@ -516,7 +541,7 @@ static int base_sample(const pbr_t *p, real r0, real r1, real r2, v3 *wi, s4 *we
     if (lobe == 0) {
         /* openpbr_sample_lobe, comprehensive microfacet lobe */
         real idotn = dot(wo, p->n);
-        v3 h = to_world(&p->spec_basis, sample_vndf(p->alpha, to_local(&p->spec_basis, wo), x, r1));
+        v3 h = to_world(&p->spec_basis, sample_vndf(p->ax, p->ay, to_local(&p->spec_basis, wo), x, r1));
         real idoth = dot(wo, h);
         if (idoth < 0) return 0;
         s4 F, T;
@@ -529,22 +554,22 @@ static int base_sample(const pbr_t *p, real r0, real r1, real r2, v3 *wi, s4 *we
             real odotn = dot(p->n, d);
             if (idotn * odotn <= 0) return 0;
             real odoth = dot(d, h);
-            real gs = smith_g1(to_local(&p->spec_basis, d), p->alpha);
+            real gs = smith_g1(to_local(&p->spec_basis, d), p->ax, p->ay);
             w = sscale(sscale(F, (real)1 / pr), gs);
-            D = ggx_d(to_local(&p->spec_basis, h), p->alpha);
+            D = ggx_d(to_local(&p->spec_basis, h), p->ax, p->ay);
             if (D <= 0) return 0;
-            gm = smith_g1(to_local(&p->spec_basis, wo), p->alpha);
+            gm = smith_g1(to_local(&p->spec_basis, wo), p->ax, p->ay);
             pd = pr * D * gm * idoth / ((real)4 * odoth * idotn);
         } else {
             if (!refract_dir(wo, h, idoth, p->eta_refr, &d)) return 0;
             real odotn = dot(d, p->n);
             if (idotn * odotn >= 0) return 0;
             real odoth = dot(d, h);
-            real gs = smith_g1(to_local(&p->spec_basis, d), p->alpha);
+            real gs = smith_g1(to_local(&p->spec_basis, d), p->ax, p->ay);
             w = sscale(sscale(T, (real)1 / pt), gs);
-            D = ggx_d(to_local(&p->spec_basis, h), p->alpha);
+            D = ggx_d(to_local(&p->spec_basis, h), p->ax, p->ay);
             if (D <= 0) return 0;
-            gm = smith_g1(to_local(&p->spec_basis, wo), p->alpha);
+            gm = smith_g1(to_local(&p->spec_basis, wo), p->ax, p->ay);
             pd = pt * D * gm * sq(p->eta_refr) * RFABS(idoth) * RFABS(odoth) / (sq(idoth + p->eta_refr * odoth) * idotn);
         }
     } else if (lobe == 1) {
@@ -607,10 +632,11 @@ static real coat_value(const pbr_t *p, v3 wi) {
     if (!(idotn * odotn > 0)) return 0;
     v3 h = norm(add(wo, wi));
     real idoth = dot(wo, h);
-    real D = ggx_d(to_local(&p->spec_basis, h), p->c_alpha);
+    real D = ggx_d(to_local(&p->spec_basis, h), p->c_alpha, p->c_alpha);
     real other = (real)1 / ((real)4 * idotn);
     real F = fresnel(COAT_IOR, RFABS(idoth));
-    real G = smith_g1(to_local(&p->spec_basis, wo), p->c_alpha) * smith_g1(to_local(&p->spec_basis, wi), p->c_alpha);
+    real G = smith_g1(to_local(&p->spec_basis, wo), p->c_alpha, p->c_alpha) *
+             smith_g1(to_local(&p->spec_basis, wi), p->c_alpha, p->c_alpha);
     return G * (D * other * F);
 }
 static real coat_pdf(const pbr_t *p, v3 wi) {
@@ -620,24 +646,24 @@ static real coat_pdf(const pbr_t *p, v3 wi) {
     v3 h = norm(add(wo, wi));
     real idoth = dot(wo, h), odoth = dot(wi, h);
     if (idoth * odoth < 0) return 0;
-    real D = ggx_d(to_local(&p->spec_basis, h), p->c_alpha);
+    real D = ggx_d(to_local(&p->spec_basis, h), p->c_alpha, p->c_alpha);
     real other = idoth / ((real)4 * odoth * idotn);
-    return smith_g1(to_local(&p->spec_basis, wo), p->c_alpha) * (D * other);
+    return smith_g1(to_local(&p->spec_basis, wo), p->c_alpha, p->c_alpha) * (D * other);
 }
 static int coat_sample(const pbr_t *p, real r0, real r1, v3 *wi, real *weight, real *pdf) {
     v3 wo = p->view;
     real idotn = dot(wo, p->n);
-    v3 h = to_world(&p->spec_basis, sample_vndf(p->c_alpha, to_local(&p->spec_basis, wo), r0, r1));
+    v3 h = to_world(&p->spec_basis, sample_vndf(p->c_alpha, p->c_alpha, to_local(&p->spec_basis, wo), r0, r1));
     real idoth = dot(wo, h);
     if (idoth < 0) return 0;
     v3 d = norm(add(neg(wo), mulv(h, (real)2 * idoth)));
     real odotn = dot(p->n, d);
     if (idotn * odotn <= 0) return 0;
     real odoth = dot(d, h);
-    real w = fresnel(COAT_IOR, idoth) * smith_g1(to_local(&p->spec_basis, d), p->c_alpha);
-    real D = ggx_d(to_local(&p->spec_basis, h), p->c_alpha);
+    real w = fresnel(COAT_IOR, idoth) * smith_g1(to_local(&p->spec_basis, d), p->c_alpha, p->c_alpha);
+    real D = ggx_d(to_local(&p->spec_basis, h), p->c_alpha, p->c_alpha);
     if (D <= 0) return 0;
-    *pdf = D * smith_g1(to_local(&p->spec_basis, wo), p->c_alpha) * idoth / ((real)4 * odoth * idotn);
+    *pdf = D * smith_g1(to_local(&p->spec_basis, wo), p->c_alpha, p->c_alpha) * idoth / ((real)4 * odoth * idotn);
     *weight = w;
     *wi = d;
     return 1;

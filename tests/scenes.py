@@ -172,7 +172,8 @@ def materials_cornell(variant="layered", light_size=0.25):
     'delta': a mirror (GLOSSY, roughness 0) and smooth glass (DIELECTRIC, IOR 1.5), Lambert floor;
     'opaque': the metal, a half-metallic blue OPENPBR box (roughness 0.25, IOR 1.6) and the plastic floor;
     'coated': coats (roughness 0.12, IOR 1.6) over them -- a fully coated metal, a half-coated rough glass,
-    and a red plastic floor under a 0.7 coat."""
+    and a red plastic floor under a 0.7 coat; 'anisotropic': a brushed metal, an anisotropic rough glass
+    and a half-metallic floor with a half coat, their specular lobes stretched along each face's tangent."""
     import vt
     base = vt.cornell(light_size)
     faces = base.faces.copy()
@@ -190,6 +191,10 @@ def materials_cornell(variant="layered", light_size=0.25):
         mats += [dict(type=1, albedo=(0.95, 0.64, 0.54), roughness=0.3, coat=1.0),
                  dict(type=4, albedo=(1.0, 1.0, 1.0), roughness=0.3, ior=1.5, transmission=1.0, coat=0.5),
                  dict(type=4, albedo=(0.7, 0.15, 0.1), roughness=0.5, ior=1.5, coat=0.7)]
+    elif variant == "anisotropic":
+        mats += [dict(type=1, albedo=(0.95, 0.64, 0.54), roughness=0.35, anisotropy=0.8),
+                 dict(type=4, albedo=(1.0, 1.0, 1.0), roughness=0.3, ior=1.5, transmission=1.0, anisotropy=0.5),
+                 dict(type=4, albedo=(0.2, 0.4, 0.8), roughness=0.4, ior=1.5, metalness=0.6, anisotropy=0.7, coat=0.5)]
     elif variant == "layered":
         mats += [dict(type=1, albedo=(0.95, 0.64, 0.54), roughness=0.3),
                  dict(type=4, albedo=(1.0, 1.0, 1.0), roughness=0.2, ior=1.5, transmission=1.0),
@@ -198,14 +203,20 @@ def materials_cornell(variant="layered", light_size=0.25):
         mats += [dict(type=1, albedo=(0.9, 0.9, 0.9), roughness=0.0),
                  dict(type=2, albedo=(1.0, 1.0, 1.0), ior=1.5),
                  dict(type=0, albedo=(0.73, 0.73, 0.73))]
-    return vt.Mesh(base.vertices, faces), mats
+    verts = base.vertices
+    if variant == "anisotropic":
+        # uv = (x + 0.3 y, z - 0.2 y): every face gets a tangent dP/du from its uv (none degenerate)
+        verts = verts.copy()
+        verts[:, 6] = verts[:, 0] + 0.3 * verts[:, 1]
+        verts[:, 7] = verts[:, 2] - 0.2 * verts[:, 1]
+    return vt.Mesh(verts, faces), mats
 
 
 def vt_materials(mats):
     import vt
     return [vt.material(m["type"], m["albedo"], m.get("emission", (0, 0, 0)), m.get("roughness", 0.5),
-                        m.get("ior", 1.5), m.get("metalness", 0.0), 1.0, m.get("transmission", 0.0), m.get("coat", 0.0))
-            for m in mats]
+                        m.get("ior", 1.5), m.get("metalness", 0.0), 1.0, m.get("transmission", 0.0), m.get("coat", 0.0),
+                        m.get("anisotropy", 0.0)) for m in mats]
 
 
 def sky_map(width=64, height=32, sun=(0.3, 0.28), sun_radiance=300.0, seed=2):
