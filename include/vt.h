@@ -201,7 +201,9 @@ typedef struct vt_material {
                                dielectric coat of roughness 0.12 and IOR 1.6 over the layered BSDF */
     float anisotropy;       /* GLOSSY / OPENPBR specular_roughness_anisotropy in [0, 1] (main.swift:2496): the
                                specular lobe stretched along the hit's tangent dP/du (from the vertices' uv) */
-    float reserved[2];
+    float fuzz;             /* GLOSSY / OPENPBR fuzz_weight in [0, 1] (main.swift:2497): a white sheen layer of
+                               roughness 0.5 (OpenPBR's fuzz lobe) over the coat and base */
+    float reserved;
 } vt_material;              /* 64 bytes */
 
 /* Replaces MaterialLibrary's settings/emissions and its emitter list (main.swift:9296-9346):
@@ -227,10 +229,10 @@ typedef struct vt_emission {
 /* Materials of an imported mesh from SurfaceSettings and emissions, as resolve_material and
  * trace_scene assign them (main.swift:2008-2011, 2390-2400): a slot with a positive emission is
  * EMISSIVE; else an enabled slot is OPENPBR with color.rgb, roughness surface.x, metalness surface.y,
- * coat surface.z, anisotropy surface.w (both clamped to [0, 1]), IOR detail.y and transmission detail.z;
- * else the imported-mesh default, DIFFUSE with albedo 0.7.  Slots beyond slot_count (emitters named
- * only in `emissions`) take the default.  Refused (VT_ERROR_UNSUPPORTED): fuzz other than 0 and texture
- * maps (map_mask != 0), which are outside this path.  Then as vt_scene_set_materials. */
+ * coat surface.z, anisotropy surface.w, fuzz detail.x (the three clamped to [0, 1]), IOR detail.y and
+ * transmission detail.z; else the imported-mesh default, DIFFUSE with albedo 0.7.  Slots beyond
+ * slot_count (emitters named only in `emissions`) take the default.  Refused (VT_ERROR_UNSUPPORTED):
+ * texture maps (map_mask != 0), which are outside this path.  Then as vt_scene_set_materials. */
 vt_status vt_scene_set_surface_settings(vt_scene *scene, const vt_surface_settings *settings, uint32_t slot_count,
                                         const vt_emission *emissions, uint32_t emission_count);
 

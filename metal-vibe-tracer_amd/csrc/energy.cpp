@@ -7,6 +7,8 @@
 // are OpenPBR's constant data (Vendor/OpenPBR/impl/data/openpbr_*dielectric*_data.h, read through
 // openpbr_microfacet_multiple_scattering_data.h:140-165): unorm16 values on 32-node axes (relative
 // IOR, sqrt(alpha), cosine), stored here as floats v / 65535 in the order the lookups index them.
+// The fuzz lobe's LTC table (Vendor/OpenPBR/impl/data/openpbr_ltc_data.h: 32 x 32 triples a_inv,
+// b_inv, R over sheen roughness and cosine) follows them in the same buffer (kLtcOffset).
 #include <hip/hip_runtime_api.h>
 
 #include <cmath>
@@ -46,13 +48,15 @@ void ensure_energy_tables(vt_context *ctx) {
         metal[129 * 129 + row] = (float)integral;
     }
     // Five slots of 32768 floats: ideal complement, ideal average, reflection ratio, opaque
-    // complement, opaque average (the 2D tables use the first 1024 floats of their slot).
-    std::vector<float> luts((size_t)5 * 32768, 0.0f);
+    // complement, opaque average (the 2D tables use the first 1024 floats of their slot); then the LTC
+    // table's 3072 floats.
+    std::vector<float> luts((size_t)kLtcOffset + 3072, 0.0f);
     put_unorm16(luts.data() + 0 * 32768, kOpenPbrIdealDielectricComplement, 32768);
     put_unorm16(luts.data() + 1 * 32768, kOpenPbrIdealDielectricAvgComplement, 1024);
     put_unorm16(luts.data() + 2 * 32768, kOpenPbrIdealDielectricReflectionRatio, 1024);
     put_unorm16(luts.data() + 3 * 32768, kOpenPbrOpaqueDielectricComplement, 32768);
     put_unorm16(luts.data() + 4 * 32768, kOpenPbrOpaqueDielectricAvgComplement, 1024);
+    for (int i = 0; i < 3072; ++i) luts[kLtcOffset + i] = kOpenPbrLtcSheen[i];
     ctx->energy_metal.upload(metal.data(), metal.size() * 4);
     ctx->energy_luts.upload(luts.data(), luts.size() * 4);
     ctx->energy_ready = true;

@@ -189,7 +189,40 @@ __device__ __forceinline__ float average_fresnel(float eta) {
     return 0.997118f + 0.1014f * eta - 0.965241f * (eta * eta) - 0.130607f * (eta * eta) * eta;
 }
 constexpr float kCoatIor = 1.6f, kCoatRoughness = 0.12f;
-constexpr float kCoatAlpha = kCoatRoughness * kCoatRoughness;  // above the 1e-6 floor of the microfacet alpha
+// fuzz_color white and fuzz_roughness 0.5: OpenPBR's defaults (openpbr_resolved_inputs.h:131-133).
+constexpr float kFuzzRoughness = 0.5f;
+
+// ---------------------------------------------------------------- fuzz (Disney sheen LTC)
+// openpbr_disney_sheen_fetch_coeffs, array mode (openpbr_fuzz_lobe.h:147-176): bilinear in the 32 x 32
+// table over (roughness, cosine), which follows the dielectric tables in the energy buffer.
+__device__ __forceinline__ void ltc_fetch(const EnergyTables &E, float cos_t, float out[3]) {
+    const float *T = E.luts + kLtcOffset;
+    const float row = fmaxf(0.0f, fminf(kFuzzRoughness, kLargestBelowOne)) * 31.0f;
+    const float col = fmaxf(0.0f, fminf(cos_t, kLargestBelowOne)) * 31.0f;
+    const float r = floorf(row), c = floorf(col), rf = row - r, cf = col - c;
+    const int ri = (int)r, ci = (int)c;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float a = mixf(T[(ri * 32 + ci) * 3 + k], T[(ri * 32 + ci + 1) * 3 + k], cf);
+        const float b = mixf(T[((ri + 1) * 32 + ci) * 3 + k], T[((ri + 1) * 32 + ci + 1) * 3 + k], cf);
+        out[k] = mixf(a, b, rf);
+    }
+}
+// openpbr_disney_sheen_rotate_vector: about z by the azimuth of (x, y).
+__device__ __forceinline__ V3 ltc_rotate(V3 v, float x, float y) {
+    const float r2 = x * x + y * y;
+    if (r2 == 0.0f) return v;
+    const float inv = rsqrtf(r2), sn = y * inv, cs = x * inv;
+    return v3(cs * v.x + sn * -v.y, cs * v.y + sn * v.x, v.z);
+}
+// openpbr_disney_sheen_eval_ltc: the LTC's density in the aligned frame.
+__device__ __forceinline__ float ltc_eval(V3 wi, float ai, float bi) {
+    const V3 o = v3(ai * wi.x + bi * wi.z, ai * wi.y, wi.z);
+    const float z = fmaxf(0.0f, wi.z), l2 = dot(o, o);
+    if (l2 == 0.0f || ai == 0.0f || z == 0.0f) return 0.0f;
+    const float k = ai / l2;
+    return kRcpPi * (z * k) * k;
+}
 
 // ---------------------------------------------------------------- prepared BSDF
 struct Pbr {
@@ -206,7 +239,9 @@ struct Pbr {
     float diffuse_comp;
     float w[4], total;       // lobe weights: specular, dielectric MMS, metal MMS, diffuse
     float coat, c_prob;      // coat presence and coat reflection probability (entering a coated surface)
+    float c_alpha;           // the coat's alpha (its roughness 0.12 adjusted for the fuzz over it)
     S4 c_scale;              // in_base_layer_scale: (1 - in_reflected) x darkening, per lane
+    float fuzz, f_refl, f_prob;  // fuzz presence (0 off or inside), view_reflected, sheen probability
     bool d_on, m_on, c_on;
 };
 
@@ -228,8 +263,8 @@ __device__ __forceinline__ void pbr_probabilities(const Pbr &p, float c, S4 &ref
 // prepare_openpbr_lanes for the subset.  type: 1 GLOSSY (metalness 1) or 4 OPENPBR; n: shading normal
 // on the arriving side; wo: toward the viewer; inside: hit from the back; rho: base colour per lane.
 __device__ __forceinline__ void pbr_prepare(Pbr &p, const EnergyTables &E, uint32_t type, float roughness, float ior_in,
-                                            float metal_in, float transw, float coat, float aniso, V3 tangent, V3 n,
-                                            V3 wo, bool inside_flag, S4 rho) {
+                                            float metal_in, float transw, float coat, float aniso, V3 tangent,
+                                            float fuzz, V3 n, V3 wo, bool inside_flag, S4 rho) {
     const float rough = fmaxf(0.03f, roughness);
     const float metalness = type == kGlossy ? 1.0f : metal_in;
     const float ior = fmaxf(1.01f, ior_in);
@@ -264,10 +299,13 @@ __device__ __forceinline__ void pbr_prepare(Pbr &p, const EnergyTables &E, uint3
         r_opaque = r_trans;
     }
     r_refr = r_refr >= 1.0f ? fmaxf(r_refr, 1.0001f) : fminf(r_refr, 1.0f / 1.0001f);
-    // The base roughened by the coat (openpbr_bsdf.h:437-460).
+    // Roughnesses (openpbr_bsdf.h:424-460): the coat's adjusted for the fuzz over it, the base's for the coat.
+    const float fuzz_adjust = (3.0f * (1.0f / 3.0f)) * kFuzzRoughness * 0.005f;
+    const float c_rough = mixf(
+        kCoatRoughness, sqrtf(sqrtf(fminf(1.0f, sqf(sqf(kCoatRoughness)) + fuzz_adjust * sqf(sqf(kFuzzRoughness))))), fuzz);
     const float adjust = mixf(1.0f, 0.0f, rel_coat >= 1.0f ? 1.0f / rel_coat : rel_coat);
     const float adj_rough =
-        mixf(rough, sqrtf(sqrtf(fminf(1.0f, sqf(sqf(rough)) + adjust * sqf(sqf(kCoatRoughness))))), coat);
+        mixf(rough, sqrtf(sqrtf(fminf(1.0f, sqf(sqf(rough)) + adjust * sqf(sqf(c_rough))))), coat);
     const float salpha = adj_rough * adj_rough;
     p.iso_alpha = salpha;
     // openpbr_compute_anisotropic_alpha (openpbr_lobe_utils.h:41-72), floored at 1e-6.
@@ -316,18 +354,33 @@ __device__ __forceinline__ void pbr_prepare(Pbr &p, const EnergyTables &E, uint3
     p.coat = coat;
     p.c_prob = 0.0f;
     p.c_scale = s4(1.0f);
+    p.c_alpha = fmaxf(sqf(c_rough), 1e-6f);
+    float below = p.total;  // the coating lobe's estimated contribution, for the sheen probability
     if (enter_coat) {
         const float ks = average_fresnel(rel_coat);
         const float kr = 1.0f - (1.0f - average_fresnel(rel_coat)) / sqf(rel_coat);
         const float so = average_fresnel(r_opaque), st = average_fresnel(r_trans);
         const float spec_base = (opaque * so + (0.0f + trans) * st) + metalness;
         const float kk = mixf(ks, kr, mixf(1.0f, adj_rough, spec_base));
-        const float refl = idotn > 0.0f ? (1.0f - opaque_complement(E, rel_coat, kCoatAlpha, idotn)) * coat : 0.0f;
+        const float refl = idotn > 0.0f ? (1.0f - opaque_complement(E, rel_coat, sqf(c_rough), idotn)) * coat : 0.0f;
         for (int k = 0; k < 4; ++k) {
             const float eb = ((dmetal * avg_f.v[k] + opaque * mixf(rho.v[k], 1.0f, so)) + 0.0f) + trans;
             p.c_scale.v[k] = (1.0f - refl) * mixf(1.0f, (1.0f - kk) / (1.0f - eb * kk), coat * 1.0f);
         }
         p.c_prob = safe_div(refl, refl + p.total * max3(p.c_scale), 0.5f);
+        below = refl + p.total * max3(p.c_scale);
+    }
+    // The fuzz lobe (openpbr_bsdf.h:842-858, openpbr_fuzz_lobe.h:395-430): presence `fuzz` outside, 0
+    // inside; its normal blends the base and coat normals, which coincide here.
+    p.fuzz = fuzz > 0.0f && !back ? fuzz : 0.0f;
+    p.f_refl = 0.0f;
+    p.f_prob = 0.0f;
+    if (p.fuzz > 0.0f) {
+        const float cos_o = idotn;  // the view's cosine in the fuzz frame (the normal's own)
+        float coef[3];
+        ltc_fetch(E, cos_o, coef);
+        p.f_refl = cos_o <= 0.0f ? 0.0f : p.fuzz * coef[2];
+        p.f_prob = safe_div(p.f_refl, p.f_refl + below * (1.0f - p.f_refl), 0.5f);
     }
 }
 
@@ -568,10 +621,10 @@ __device__ __forceinline__ float coat_value(const Pbr &p, V3 wi) {
     const float idotn = dot(p.n, wo), odotn = dot(p.n, wi);
     if (!(idotn * odotn > 0.0f)) return 0.0f;
     const V3 h = normalize(wo + wi);
-    const float D = ggx_d(to_local(p.sb, h), kCoatAlpha, kCoatAlpha);
+    const float D = ggx_d(to_local(p.sb, h), p.c_alpha, p.c_alpha);
     const float F = fresnel(kCoatIor, fabsf(dot(wo, h)));
     const float G =
-        smith_g1(to_local(p.sb, wo), kCoatAlpha, kCoatAlpha) * smith_g1(to_local(p.sb, wi), kCoatAlpha, kCoatAlpha);
+        smith_g1(to_local(p.sb, wo), p.c_alpha, p.c_alpha) * smith_g1(to_local(p.sb, wi), p.c_alpha, p.c_alpha);
     return G * (D * (1.0f / (4.0f * idotn)) * F);
 }
 __device__ __forceinline__ float coat_pdf(const Pbr &p, V3 wi) {
@@ -581,12 +634,12 @@ __device__ __forceinline__ float coat_pdf(const Pbr &p, V3 wi) {
     const V3 h = normalize(wo + wi);
     const float idoth = dot(wo, h), odoth = dot(wi, h);
     if (idoth * odoth < 0.0f) return 0.0f;
-    const float D = ggx_d(to_local(p.sb, h), kCoatAlpha, kCoatAlpha);
-    return smith_g1(to_local(p.sb, wo), kCoatAlpha, kCoatAlpha) * (D * (idoth / (4.0f * odoth * idotn)));
+    const float D = ggx_d(to_local(p.sb, h), p.c_alpha, p.c_alpha);
+    return smith_g1(to_local(p.sb, wo), p.c_alpha, p.c_alpha) * (D * (idoth / (4.0f * odoth * idotn)));
 }
-// openpbr_eval: the coat (times its presence) over the base (times in_base_layer_scale; the light
+// The coating lobe: the coat (times its presence) over the base (times in_base_layer_scale; the light
 // side's scale is 1 under OpenPBR's default view-side attenuation).
-__device__ __forceinline__ S4 pbr_eval(const Pbr &p, const EnergyTables &E, V3 wi) {
+__device__ __forceinline__ S4 coated_eval(const Pbr &p, const EnergyTables &E, V3 wi) {
     const S4 b = base_eval(p, E, wi);
     if (!p.c_on) return b;
     const float c = coat_value(p, wi) * p.coat;
@@ -594,15 +647,15 @@ __device__ __forceinline__ S4 pbr_eval(const Pbr &p, const EnergyTables &E, V3 w
     for (int k = 0; k < 4; ++k) r.v[k] = c + b.v[k] * p.c_scale.v[k];
     return r;
 }
-// openpbr_pdf: mix(base pdf, coat pdf, coat reflection probability).
-__device__ __forceinline__ float pbr_pdf(const Pbr &p, V3 wi) {
+// The coating lobe's pdf: mix(base pdf, coat pdf, coat reflection probability).
+__device__ __forceinline__ float coated_pdf(const Pbr &p, V3 wi) {
     const float b = base_pdf(p, wi);
     return p.c_on ? mixf(b, coat_pdf(p, wi), p.c_prob) : b;
 }
-// openpbr_sample through the coating lobe (openpbr_coating_lobe.h:258-331): the first random number
-// picks the coat or the base and is remapped; the other layer's value and pdf are folded in.
-__device__ __forceinline__ bool pbr_sample(const Pbr &p, const EnergyTables &E, float r0, float r1, float r2, V3 &wi,
-                                           S4 &weight, float &pdf) {
+// The coating lobe's sample (openpbr_coating_lobe.h:258-331): the first random number picks the coat
+// or the base and is remapped; the other layer's value and pdf are folded in.
+__device__ __forceinline__ bool coated_sample(const Pbr &p, const EnergyTables &E, float r0, float r1, float r2, V3 &wi,
+                                              S4 &weight, float &pdf) {
     if (!p.c_on) return base_sample(p, E, r0, r1, r2, wi, weight, pdf);
     const float cp = p.c_prob;
     V3 d;
@@ -612,16 +665,16 @@ __device__ __forceinline__ bool pbr_sample(const Pbr &p, const EnergyTables &E, 
         const V3 wo = p.view;
         const float idotn = dot(wo, p.n);
         const float x = fminf(r0 * (1.0f / cp), kLargestBelowOne);
-        const V3 h = to_world(p.sb, sample_vndf(kCoatAlpha, kCoatAlpha, to_local(p.sb, wo), x, r1));
+        const V3 h = to_world(p.sb, sample_vndf(p.c_alpha, p.c_alpha, to_local(p.sb, wo), x, r1));
         const float idoth = dot(wo, h);
         if (idoth < 0.0f) return false;
         d = normalize(-wo + h * (2.0f * idoth));
         const float odotn = dot(p.n, d);
         if (idotn * odotn <= 0.0f) return false;
-        const float D = ggx_d(to_local(p.sb, h), kCoatAlpha, kCoatAlpha);
+        const float D = ggx_d(to_local(p.sb, h), p.c_alpha, p.c_alpha);
         if (D <= 0.0f) return false;
-        const float pd = D * smith_g1(to_local(p.sb, wo), kCoatAlpha, kCoatAlpha) * idoth / (4.0f * dot(d, h) * idotn);
-        const float bc = fresnel(kCoatIor, idoth) * smith_g1(to_local(p.sb, d), kCoatAlpha, kCoatAlpha) * pd * p.coat;
+        const float pd = D * smith_g1(to_local(p.sb, wo), p.c_alpha, p.c_alpha) * idoth / (4.0f * dot(d, h) * idotn);
+        const float bc = fresnel(kCoatIor, idoth) * smith_g1(to_local(p.sb, d), p.c_alpha, p.c_alpha) * pd * p.coat;
         const S4 be = base_eval(p, E, d);
         for (int k = 0; k < 4; ++k) comb.v[k] = bc + be.v[k] * p.c_scale.v[k];
         ptot = mixf(base_pdf(p, d), pd, cp);
@@ -636,5 +689,77 @@ __device__ __forceinline__ bool pbr_sample(const Pbr &p, const EnergyTables &E, 
     wi = d;
     pdf = ptot;
     weight = scale(comb, 1.0f / ptot);
+    return true;
+}
+
+// ---------------------------------------------------------------- the fuzz
+// The sheen's pdf (the LTC density in the view-aligned frame) and its coefficients' reflectance R, in
+// the normal's own frame; the coefficients are re-read per call (they depend only on the view).
+__device__ __forceinline__ float sheen_pdf(const Pbr &p, const EnergyTables &E, V3 wi, float &refl) {
+    const Basis B = make_basis(p.n);
+    const V3 wo = to_local(B, p.view), wl = to_local(B, wi);
+    float coef[3];
+    ltc_fetch(E, wo.z, coef);
+    refl = coef[2];
+    if (wo.z <= 0.0f || wl.z <= 0.0f) return 0.0f;
+    return ltc_eval(ltc_rotate(wl, wo.x, -wo.y), coef[0], coef[1]);
+}
+// openpbr_eval: the sheen over the coating lobe, which is scaled by 1 - view_reflected; f |cos|.
+__device__ __forceinline__ S4 pbr_eval(const Pbr &p, const EnergyTables &E, V3 wi) {
+    const S4 b = coated_eval(p, E, wi);
+    if (!(p.fuzz > 0.0f)) return b;
+    float r;
+    const float sh = sheen_pdf(p, E, wi, r) * (p.fuzz * r * 1.0f);
+    S4 out;
+    for (int k = 0; k < 4; ++k) out.v[k] = b.v[k] * ((1.0f - p.f_refl) * 1.0f) + sh;
+    return out;
+}
+// openpbr_pdf: mix(coating pdf, sheen pdf, sheen probability).
+__device__ __forceinline__ float pbr_pdf(const Pbr &p, const EnergyTables &E, V3 wi) {
+    const float b = coated_pdf(p, wi);
+    if (!(p.fuzz > 0.0f)) return b;
+    float r;
+    return mixf(b, sheen_pdf(p, E, wi, r), p.f_prob);
+}
+// openpbr_sample through the fuzz lobe (openpbr_fuzz_lobe.h:468-545): the first random number picks the
+// sheen or the coating lobe; the sheen samples its LTC with the other two.
+__device__ __forceinline__ bool pbr_sample(const Pbr &p, const EnergyTables &E, float r0, float r1, float r2, V3 &wi,
+                                           S4 &weight, float &pdf) {
+    if (!(p.fuzz > 0.0f)) return coated_sample(p, E, r0, r1, r2, wi, weight, pdf);
+    const float sp = p.f_prob, bp = 1.0f - sp;
+    V3 d;
+    if (r0 < sp) {
+        const Basis B = make_basis(p.n);
+        const V3 wo = to_local(B, p.view);
+        if (wo.z <= 0.0f) return false;
+        float coef[3];
+        ltc_fetch(E, wo.z, coef);
+        float sn, cs;
+        sincosf(kTwoPi * r1, &sn, &cs);
+        const float z = sqrtf(r2), rr = sqrtf(1.0f - r2), a = 1.0f / coef[0];
+        const V3 o = v3(cs * rr, sn * rr, z);
+        const V3 wl = ltc_rotate(normalize(v3(o.x * a - o.z * coef[1] * a, o.y * a, o.z)), wo.x, wo.y);
+        if (!(wo.z * wl.z > 0.0f)) return false;
+        d = normalize(to_world(B, wl));
+        float r;
+        float pd = sheen_pdf(p, E, d, r) * sp;
+        pd += coated_pdf(p, d) * bp;
+        weight = scale(pbr_eval(p, E, d), 1.0f / pd);
+        pdf = pd;
+    } else {
+        S4 w;
+        float pd;
+        if (!coated_sample(p, E, fminf((r0 - sp) * (1.0f / bp), kLargestBelowOne), r1, r2, d, w, pd)) return false;
+        float r;
+        const float shp = sheen_pdf(p, E, d, r);
+        const float sh = shp * (p.fuzz * r * 1.0f);
+        S4 bc;
+        for (int k = 0; k < 4; ++k) bc.v[k] = (w.v[k] * pd) * ((1.0f - p.f_refl) * 1.0f) + sh;
+        pd *= bp;
+        pd += shp * sp;
+        weight = scale(bc, 1.0f / pd);
+        pdf = pd;
+    }
+    wi = d;
     return true;
 }

@@ -232,7 +232,7 @@ struct Mat {
     float4 lagrange;
     V3 emission;
     float weight;
-    float roughness, ior, metalness, transmission, coat, anisotropy;
+    float roughness, ior, metalness, transmission, coat, anisotropy, fuzz;
 };
 __device__ __forceinline__ Mat material(const RenderScene &sc, uint32_t slot) {
     Mat m;
@@ -248,6 +248,7 @@ __device__ __forceinline__ Mat material(const RenderScene &sc, uint32_t slot) {
         m.transmission = 0.0f;
         m.coat = 0.0f;
         m.anisotropy = 0.0f;
+        m.fuzz = 0.0f;
         return m;
     }
     const float4 *r = reinterpret_cast<const float4 *>(sc.materials) + (kMaterialFloats / 4) * (size_t)slot;
@@ -259,6 +260,7 @@ __device__ __forceinline__ Mat material(const RenderScene &sc, uint32_t slot) {
     const float4 r4 = r[4];
     m.coat = r4.x;
     m.anisotropy = r4.y;
+    m.fuzz = r4.z;
     m.type = __float_as_uint(r0.x);
     m.albedo = v3(r0.y, r0.z, r0.w);
     m.lagrange = r[1];
@@ -313,9 +315,9 @@ __device__ __forceinline__ S4 eval_bsdf_pdf(const RenderScene &sc, const Spectra
     const EnergyTables E{sc.energy_metal, sc.energy_luts};
     Pbr p;
     pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, m.coat, m.anisotropy,
-                m.anisotropy > 0.0f ? surface_tangent(sc, s) : v3(0.0f, 0.0f, 0.0f), s.n, wo, !s.front,
+                m.anisotropy > 0.0f ? surface_tangent(sc, s) : v3(0.0f, 0.0f, 0.0f), m.fuzz, s.n, wo, !s.front,
                 reflectance(sp, lanes, m.lagrange));
-    pdf = pbr_pdf(p, wi);
+    pdf = pbr_pdf(p, E, wi);
     if (!(m.transmission > 0.0f) && dot(wi, s.ng) <= 0.0f) return s4(0.0f);
     return scale(pbr_eval(p, E, wi), 1.0f / cosine);
 }
@@ -373,7 +375,7 @@ __device__ __forceinline__ bool sample_bsdf(const RenderScene &sc, const Spectra
     const EnergyTables E{sc.energy_metal, sc.energy_luts};
     Pbr p;
     pbr_prepare(p, E, m.type, m.roughness, m.ior, m.metalness, m.transmission, m.coat, m.anisotropy,
-                m.anisotropy > 0.0f ? surface_tangent(sc, s) : v3(0.0f, 0.0f, 0.0f), s.n, -in, !s.front,
+                m.anisotropy > 0.0f ? surface_tangent(sc, s) : v3(0.0f, 0.0f, 0.0f), m.fuzz, s.n, -in, !s.front,
                 reflectance(sp, lanes, m.lagrange));
     float r0, r1, r2;
     rand_f3(seed, r0, r1, r2);

@@ -17,8 +17,9 @@ namespace vt {
 //   [1] reflectance Lagrange multipliers (L0, L1, L2, w): w = 1 MESE, w = 2 flat grey in x
 //   [2] emission rgb, light-selection weight (luminance; 0 = not an emitter)
 //   [3] roughness, ior, metalness, transmission (OpenPBR inputs)
-//   [4] coat weight, specular anisotropy, 2 unused
+//   [4] coat weight, specular anisotropy, fuzz weight, 1 unused
 constexpr uint32_t kMaterialFloats = 20;
+constexpr uint32_t kLtcOffset = 5 * 32768;  // floats before the LTC sheen table in the energy buffer
 enum MaterialType : uint32_t { kDiffuse = 0, kGlossy = 1, kDielectric = 2, kEmissive = 3, kOpenPbr = 4 };
 
 struct RenderScene {
@@ -45,7 +46,7 @@ struct RenderScene {
     float tree_lo[3], tree_scale[3];
     // Energy-compensation tables of the layered BSDF (129 x 129 + 129 ideal metal; 5 x 32768 others).
     const float *energy_metal;
-    const float *energy_luts;
+    const float *energy_luts;  // 5 x 32768 dielectric tables, then the LTC sheen table at kLtcOffset
     uint32_t layered;  // some slot is GLOSSY or OPENPBR (the shading kernels then carry the layered BSDF)
     // Environment map (null: none): env_w x env_h rgba32f texels, row 0 at the zenith; its sampling
     // tables, rows[2 y + table] (marginal CDFs) and cols[(table env_h + y) env_w + x] (conditional

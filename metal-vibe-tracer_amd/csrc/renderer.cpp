@@ -240,9 +240,10 @@ vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, 
                     return fail(VT_ERROR_INVALID_ARGUMENT, "material colours must be finite, emission >= 0");
             if (!(m.roughness >= 0.0f && m.roughness <= 1.0f) || !(m.ior >= 1.0f && m.ior <= 4.0f) ||
                 !(m.metalness >= 0.0f && m.metalness <= 1.0f) || !(m.transmission >= 0.0f && m.transmission <= 1.0f) ||
-                !(m.coat >= 0.0f && m.coat <= 1.0f) || !(m.anisotropy >= 0.0f && m.anisotropy <= 1.0f))
+                !(m.coat >= 0.0f && m.coat <= 1.0f) || !(m.anisotropy >= 0.0f && m.anisotropy <= 1.0f) ||
+                !(m.fuzz >= 0.0f && m.fuzz <= 1.0f))
                 return fail(VT_ERROR_INVALID_ARGUMENT,
-                            "roughness, metalness, transmission, coat, anisotropy must be in [0, 1], ior in [1, 4]");
+                            "roughness, metalness, transmission, coat, anisotropy, fuzz must be in [0, 1], ior in [1, 4]");
             if (m.type == 2 && !(m.ior > 1.0f)) return fail(VT_ERROR_INVALID_ARGUMENT, "a dielectric needs ior > 1");
             float *r = rec.data() + vt::kMaterialFloats * (size_t)k;
             uint32_t type = m.type;
@@ -258,6 +259,7 @@ vt_status vt_scene_set_materials(vt_scene *scene, const vt_material *materials, 
             r[12] = m.roughness; r[13] = m.ior; r[14] = m.metalness; r[15] = m.transmission;
             r[16] = m.coat;
             r[17] = m.anisotropy;
+            r[18] = m.fuzz;
             if (m.type == 1 || m.type == 4) layered = true;
             if (m.type == 2 || (m.type == 1 && m.roughness < 0.02f)) delta = true;
             if (m.type == 0 || m.type == 2 || (m.type == 1 && m.roughness < 0.02f)) lean = true;
@@ -370,7 +372,6 @@ vt_status vt_scene_set_surface_settings(vt_scene *scene, const vt_surface_settin
             const vt_surface_settings &s = settings[k];
             if (s.map_mask != 0) return fail(VT_ERROR_UNSUPPORTED, "texture maps are outside this path (map_mask must be 0)");
             if (!s.enabled) continue;
-            if (s.detail[0] != 0.0f) return fail(VT_ERROR_UNSUPPORTED, "fuzz is outside the layered BSDF subset");
             m.type = 4;
             for (int c = 0; c < 3; ++c) m.albedo[c] = s.color[c];
             m.roughness = s.surface[0];
@@ -378,6 +379,7 @@ vt_status vt_scene_set_surface_settings(vt_scene *scene, const vt_surface_settin
             // weights are fractions: clamped as the reference clamps its graph inputs (main.swift:2354-2355)
             m.coat = std::min(std::max(s.surface[2], 0.0f), 1.0f);
             m.anisotropy = std::min(std::max(s.surface[3], 0.0f), 1.0f);
+            m.fuzz = std::min(std::max(s.detail[0], 0.0f), 1.0f);
             m.ior = s.detail[1];
             m.transmission = s.detail[2];
         }
@@ -425,7 +427,7 @@ vt_status vt_energy_tables(vt_context *ctx, float *metal, float *luts) {
         vt::ensure_energy_tables(ctx);
         check(hipSetDevice(ctx->device), "hipSetDevice");
         if (metal) check(hipMemcpy(metal, ctx->energy_metal.ptr, ctx->energy_metal.bytes, hipMemcpyDeviceToHost), "hipMemcpy");
-        if (luts) check(hipMemcpy(luts, ctx->energy_luts.ptr, ctx->energy_luts.bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (luts) check(hipMemcpy(luts, ctx->energy_luts.ptr, (size_t)vt::kLtcOffset * 4, hipMemcpyDeviceToHost), "hipMemcpy");
         return VT_OK;
     });
 }

@@ -86,13 +86,13 @@ class Material(ctypes.Structure):
     _fields_ = [("type", ctypes.c_uint32), ("albedo", ctypes.c_float * 3), ("emission", ctypes.c_float * 3),
                 ("roughness", ctypes.c_float), ("ior", ctypes.c_float), ("metalness", ctypes.c_float),
                 ("specular_weight", ctypes.c_float), ("transmission", ctypes.c_float), ("coat", ctypes.c_float),
-                ("anisotropy", ctypes.c_float), ("reserved", ctypes.c_float * 2)]
+                ("anisotropy", ctypes.c_float), ("fuzz", ctypes.c_float), ("reserved", ctypes.c_float)]
 
 
 def material(type=0, albedo=(0.7, 0.7, 0.7), emission=(0.0, 0.0, 0.0), roughness=0.5, ior=1.5, metalness=0.0,
-             specular_weight=1.0, transmission=0.0, coat=0.0, anisotropy=0.0) -> Material:
+             specular_weight=1.0, transmission=0.0, coat=0.0, anisotropy=0.0, fuzz=0.0) -> Material:
     return Material(type, (ctypes.c_float * 3)(*albedo), (ctypes.c_float * 3)(*emission), roughness, ior, metalness,
-                    specular_weight, transmission, coat, anisotropy)
+                    specular_weight, transmission, coat, anisotropy, fuzz)
 
 
 class BatchJob(ctypes.Structure):

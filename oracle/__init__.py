@@ -85,6 +85,7 @@ def load(path=None):
         "vo_bsdf_eval_f64": (None, [vp, vp, vp, vp, vp, c.c_int, vp, vp]),
         "vo_bsdf_sample_f64": (c.c_int, [vp, vp, vp, vp, c.c_int, vp, vp, vp, vp]),
         "vo_energy_set_dielectric": (None, [vp]),
+        "vo_ltc_set": (None, [vp]),
         "vo_energy_table": (c.POINTER(c.c_uint16), [c.c_int]),
         "vo_metal_energy_table": (c.POINTER(c.c_float), []),
         "vo_energy_ior_node": (c.c_float, [c.c_int]),
@@ -120,6 +121,8 @@ def load(path=None):
     tables = reference_energy_tables()
     flat = np.concatenate([tables[k].ravel() for k in DIELECTRIC_KEYS]).astype(np.uint16)
     lib.vo_energy_set_dielectric(flat.ctypes.data_as(ctypes.c_void_p))
+    ltc = np.ascontiguousarray(tables["ltc_sheen"], np.float32)
+    lib.vo_ltc_set(ltc.ctypes.data_as(ctypes.c_void_p))
     _lib, _lib_path = lib, path
     return lib
 
@@ -349,7 +352,7 @@ def spectral_data():
 
 def material_records(materials):
     """materials: list of dicts with type, albedo, emission, and for the layered types roughness, ior,
-    metalness, transmission, coat, anisotropy -> (records (n, 20) float32, weights)."""
+    metalness, transmission, coat, anisotropy, fuzz -> (records (n, 20) float32, weights)."""
     d, _ = spectral_data()
     rec = np.zeros((len(materials), 20), np.float32)
     weights = np.zeros(len(materials))
@@ -366,6 +369,7 @@ def material_records(materials):
         rec[k, 12:16] = (m.get("roughness", 0.5), m.get("ior", 1.5), m.get("metalness", 0.0), m.get("transmission", 0.0))
         rec[k, 16] = m.get("coat", 0.0)
         rec[k, 17] = m.get("anisotropy", 0.0)
+        rec[k, 18] = m.get("fuzz", 0.0)
     return rec, weights
 
 
@@ -407,12 +411,13 @@ def build_scene_tree(vertices, faces, ids, weights):
 
 class BSDF:
     """The oracle's layered BSDF subset for one material: type 1 (GLOSSY: metal) or 4 (OPENPBR), with a
-    coat of weight `coat` (roughness 0.12, IOR 1.6) and a specular lobe of anisotropy `anisotropy` along
-    `tangent`."""
+    coat of weight `coat` (roughness 0.12, IOR 1.6), a specular lobe of anisotropy `anisotropy` along
+    `tangent`, and a fuzz (sheen) layer of weight `fuzz` (roughness 0.5, white)."""
 
     def __init__(self, type, roughness, ior=1.5, metalness=0.0, transmission=0.0, rho=(0.8, 0.8, 0.8, 0.8),
-                 precision="f32", coat=0.0, anisotropy=0.0, tangent=(1.0, 0.0, 0.0)):
-        self.params = np.array([type, roughness, ior, metalness, transmission, coat, anisotropy, *tangent], np.float32)
+                 precision="f32", coat=0.0, anisotropy=0.0, tangent=(1.0, 0.0, 0.0), fuzz=0.0):
+        self.params = np.array([type, roughness, ior, metalness, transmission, coat, anisotropy, *tangent, fuzz],
+                               np.float32)
         self.rho = np.asarray(rho, np.float32)
         self.sfx = precision
 

@@ -113,6 +113,19 @@ void vo_energy_set_dielectric(const uint16_t *tables) {
     }
     g_dielectric_set = 1;
 }
+static float g_ltc[3072];
+static int g_ltc_set;
+void vo_ltc_set(const float *table) {
+    memcpy(g_ltc, table, sizeof g_ltc);
+    g_ltc_set = 1;
+}
+const float *vo_ltc_table(void) {
+    if (!g_ltc_set) {
+        fprintf(stderr, "oracle: fuzz LTC table not set (vo_ltc_set)\n");
+        abort();
+    }
+    return g_ltc;
+}
 const uint16_t *vo_energy_table(int id) {
     if (!g_dielectric_set) {
         fprintf(stderr, "oracle: dielectric energy tables not set (vo_energy_set_dielectric)\n");

@@ -166,7 +166,7 @@ typedef struct {
     uint32_t type;
     const float *lagrange;
     v3 emission;
-    real weight, roughness, ior, metalness, transmission, coat, anisotropy;
+    real weight, roughness, ior, metalness, transmission, coat, anisotropy, fuzz;
 } mat_t;
 static mat_t mat_of(const vo_render_scene *sc, uint32_t slot) {
     static const float grey[4] = {0.7f, 0, 0, 2.0f};
@@ -174,6 +174,7 @@ static mat_t mat_of(const vo_render_scene *sc, uint32_t slot) {
     if (slot >= sc->material_count) {
         m.type = 0; m.lagrange = grey; m.emission = V(0, 0, 0); m.weight = 0;
         m.roughness = (real)0.5; m.ior = (real)1.5; m.metalness = 0; m.transmission = 0; m.coat = 0; m.anisotropy = 0;
+        m.fuzz = 0;
         return m;
     }
     const float *r = sc->materials + VO_MATERIAL_FLOATS * (size_t)slot;
@@ -187,6 +188,7 @@ static mat_t mat_of(const vo_render_scene *sc, uint32_t slot) {
     m.transmission = (real)r[15];
     m.coat = (real)r[16];
     m.anisotropy = (real)r[17];
+    m.fuzz = (real)r[18];
     if (r[8] > 0 || r[9] > 0 || r[10] > 0) m.type = 3;
     return m;
 }
@@ -485,7 +487,8 @@ static s4 eval_bsdf_pdf(const vo_render_scene *sc, const lanes_t *l, const surf_
     real cosine = RFABS(dot(s->n, wi));
     if (cosine < (real)1e-7 || dot(s->n, wo) <= 0) return S4(0);
     pbr_t p;
-    pbr_inputs in = {(real)m->type, m->roughness, m->ior, m->metalness, m->transmission, m->coat, m->anisotropy, s->tan};
+    pbr_inputs in = {(real)m->type, m->roughness, m->ior, m->metalness, m->transmission, m->coat, m->anisotropy, s->tan,
+                     m->fuzz};
     pbr_prepare(&p, &in, s->n, wo, !s->front, reflectance(sc, l, m->lagrange));
     *pdf = pbr_pdf(&p, wi);
     if (!(m->transmission > 0) && dot(wi, s->ng) <= 0) return S4(0);
@@ -541,7 +544,7 @@ static int sample_bsdf(const vo_render_scene *sc, const lanes_t *l, const surf_t
     }
     pbr_t p;
     pbr_inputs pin = {(real)m->type, m->roughness, m->ior, m->metalness, m->transmission, m->coat, m->anisotropy,
-                      s->tan};
+                      s->tan, m->fuzz};
     pbr_prepare(&p, &pin, s->n, neg(in), !s->front, reflectance(sc, l, m->lagrange));
     real r0, r1, r2;
     rr3(seed, &r0, &r1, &r2);
@@ -569,13 +572,14 @@ static v3 nee(const vo_render_scene *sc, const lanes_t *l, const surf_t *s, cons
 }
 
 /* The layered BSDF on its own, for tests: params = type, roughness, ior, metalness, transmission, coat,
- * anisotropy, tangent xyz;
+ * anisotropy, tangent xyz, fuzz;
  * n is the shading normal on the arriving side, wo the direction toward the viewer, rho the base
  * colour per lane.  eval: f |cos| per lane and the aggregate pdf of wi. */
 void SUFFIX(vo_bsdf_eval)(const float *params, const float *n, const float *wo, const float *wi, const float *rho,
                           int inside, float *fcos, float *pdf) {
     pbr_t p;
-    pbr_inputs in = {params[0], params[1], params[2], params[3], params[4], params[5], params[6], ld(params + 7)};
+    pbr_inputs in = {params[0], params[1], params[2], params[3], params[4], params[5], params[6], ld(params + 7),
+                     params[10]};
     s4 r = {{rho[0], rho[1], rho[2], rho[3]}};
     pbr_prepare(&p, &in, ld(n), ld(wo), inside, r);
     s4 f = pbr_eval(&p, ld(wi));
@@ -586,7 +590,8 @@ void SUFFIX(vo_bsdf_eval)(const float *params, const float *n, const float *wo, 
 int SUFFIX(vo_bsdf_sample)(const float *params, const float *n, const float *wo, const float *rho, int inside,
                            const float *u3, float *dir, float *weight, float *pdf) {
     pbr_t p;
-    pbr_inputs in = {params[0], params[1], params[2], params[3], params[4], params[5], params[6], ld(params + 7)};
+    pbr_inputs in = {params[0], params[1], params[2], params[3], params[4], params[5], params[6], ld(params + 7),
+                     params[10]};
     s4 r = {{rho[0], rho[1], rho[2], rho[3]}};
     pbr_prepare(&p, &in, ld(n), ld(wo), inside, r);
     v3 d;

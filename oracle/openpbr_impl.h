@@ -213,6 +213,40 @@ static real clamp_avg(real v) { return RFMAX(v, (real)1e-12); }
 static real mixr(real x, real y, real a) { return x + (y - x) * a; }
 #define COAT_IOR ((real)1.6)
 #define COAT_ROUGHNESS ((real)0.12)
+/* fuzz_color white and fuzz_roughness 0.5 (OpenPBR's defaults, openpbr_resolved_inputs.h:131-133) */
+#define FUZZ_ROUGHNESS ((real)0.5)
+
+/* ------------------------------------------------------------------ fuzz (Disney sheen LTC) */
+/* openpbr_disney_sheen_fetch_coeffs, array mode (openpbr_fuzz_lobe.h:147-176): bilinear in the 32 x 32
+ * table over (roughness, cosine). */
+static void ltc_fetch(real cos_t, real out[3]) {
+    const float *T = vo_ltc_table();
+    const real row = RFMAX((real)0, RFMIN(FUZZ_ROUGHNESS, LARGEST_BELOW_ONE)) * (real)31;
+    const real col = RFMAX((real)0, RFMIN(cos_t, LARGEST_BELOW_ONE)) * (real)31;
+    const real r = RFLOOR(row), c = RFLOOR(col), rf = row - r, cf = col - c;
+    const int ri = (int)r, ci = (int)c;
+    for (int k = 0; k < 3; ++k) {
+        const real a = mixr((real)T[(ri * 32 + ci) * 3 + k], (real)T[(ri * 32 + ci + 1) * 3 + k], cf);
+        const real b = mixr((real)T[((ri + 1) * 32 + ci) * 3 + k], (real)T[((ri + 1) * 32 + ci + 1) * 3 + k], cf);
+        out[k] = mixr(a, b, rf);
+    }
+}
+/* openpbr_disney_sheen_rotate_vector: about z by the azimuth of (x, y). */
+static v3 ltc_rotate(v3 v, real x, real y) {
+    const real r2 = x * x + y * y;
+    if (r2 == 0) return v;
+    const real inv = (real)1 / RSQRT(r2), sn = y * inv, cs = x * inv;
+    return V(cs * v.x + sn * -v.y, cs * v.y + sn * v.x, v.z);
+}
+/* openpbr_disney_sheen_eval_ltc: the LTC's density in the aligned frame. */
+static real ltc_eval(v3 wi, const real coef[3]) {
+    const real ai = coef[0], bi = coef[1];
+    const v3 o = V(ai * wi.x + bi * wi.z, ai * wi.y, wi.z);
+    const real z = RFMAX((real)0, wi.z), l2 = dot(o, o);
+    if (l2 == 0 || ai == 0 || z == 0) return 0;
+    const real k = ai / l2;
+    return (real)0.318309886 * (z * k) * k;
+}
 
 /* ------------------------------------------------------------------ prepared BSDF */
 typedef struct {
@@ -240,11 +274,19 @@ typedef struct {
     int c_on;
     real coat, c_alpha, c_prob;  /* presence, coat alpha (clamped), coat reflection probability */
     s4 c_scale;              /* in_base_layer_scale: (1 - in_reflected) x darkening, per lane */
+    real c_refl;             /* in_reflected */
+    /* fuzz lobe (openpbr_fuzz_lobe.h), used when a fuzzy surface is hit from outside */
+    int f_on;
+    real fuzz, f_refl, f_prob;   /* presence, view_reflected, sheen probability */
+    basis_t f_basis;
+    v3 f_wo;                 /* the view direction in the fuzz basis */
+    real f_coef[3];          /* LTC coefficients (a_inv, b_inv, R) at the view's cosine */
 } pbr_t;
 
 typedef struct {
     real type, roughness, ior, metalness, transmission, coat, anisotropy;
     v3 tangent;              /* the hit's tangent (dP/du); used by an anisotropic specular lobe */
+    real fuzz;
 } pbr_inputs;
 
 /* all_coefficients at cosine c: reflection and transmission per lane. */
@@ -309,8 +351,13 @@ static void pbr_prepare(pbr_t *p, const pbr_inputs *in, v3 n, v3 wo, int inside_
     }
     if (r_refr >= 1) r_refr = RFMAX(r_refr, (real)1.0001);
     else r_refr = RFMIN(r_refr, (real)1 / (real)1.0001);
-    /* Roughness under the coat (openpbr_bsdf.h:424-460; fuzz 0 leaves the coat roughness as given). */
-    const real c_rough = COAT_ROUGHNESS;
+    /* Roughnesses (openpbr_bsdf.h:424-460): the coat's adjusted for the fuzz over it, the base's for
+     * the coat. */
+    const real fuzz = in->fuzz;
+    const real fuzz_adjust = ((real)3 * ((real)1 / (real)3)) * FUZZ_ROUGHNESS * (real)0.005;
+    const real c_rough = mixr(COAT_ROUGHNESS,
+                              RSQRT(RSQRT(RFMIN((real)1, sq(sq(COAT_ROUGHNESS)) + fuzz_adjust * sq(sq(FUZZ_ROUGHNESS))))),
+                              fuzz);
     const real adjust = mixr(1, 0, rel_coat >= 1 ? 1 / rel_coat : rel_coat);
     const real adj_rough =
         mixr(rough, RSQRT(RSQRT(RFMIN((real)1, sq(sq(rough)) + adjust * sq(sq(c_rough))))), coat);
@@ -394,6 +441,24 @@ static void pbr_prepare(pbr_t *p, const pbr_inputs *in, v3 n, v3 wo, int inside_
             p->c_scale.v[k] = (1 - refl) * dark;
         }
         p->c_prob = safe_div(refl, refl + p->total * max3(p->c_scale), (real)0.5);
+        p->c_refl = refl;
+    } else {
+        p->c_refl = 0;
+    }
+    /* The fuzz lobe (openpbr_bsdf.h:842-858, openpbr_fuzz_lobe.h:395-430): presence `fuzz` outside,
+     * 0 inside; its normal blends the base and coat normals, which coincide here. */
+    p->f_on = fuzz > 0 && !inside;
+    p->fuzz = p->f_on ? fuzz : 0;
+    p->f_refl = 0;
+    p->f_prob = 0;
+    if (p->f_on) {
+        p->f_basis = make_basis(p->n);
+        p->f_wo = to_local(&p->f_basis, wo);
+        ltc_fetch(p->f_wo.z, p->f_coef);
+        p->f_refl = p->f_wo.z <= 0 ? 0 : p->fuzz * p->f_coef[2];
+        /* the coating lobe's estimate: its reflectance plus the base's weight through it */
+        const real below = p->c_on ? p->c_refl + p->total * max3(p->c_scale) : p->total;
+        p->f_prob = safe_div(p->f_refl, p->f_refl + below * (1 - p->f_refl), (real)0.5);
     }
 }
 
@@ -669,9 +734,9 @@ static int coat_sample(const pbr_t *p, real r0, real r1, v3 *wi, real *weight, r
     return 1;
 }
 
-/* openpbr_eval: the coat (scaled by its presence) over the base (scaled by in_base_layer_scale; the
- * light side's scale is 1 with OpenPBR's default view-side attenuation), f |cos| per lane. */
-static s4 pbr_eval(const pbr_t *p, v3 wi) {
+/* The coating lobe: the coat (scaled by its presence) over the base (scaled by in_base_layer_scale;
+ * the light side's scale is 1 with OpenPBR's default view-side attenuation), f |cos| per lane. */
+static s4 coated_eval(const pbr_t *p, v3 wi) {
     s4 b = base_eval(p, wi);
     if (!p->c_on) return b;
     real c = coat_value(p, wi);
@@ -679,14 +744,14 @@ static s4 pbr_eval(const pbr_t *p, v3 wi) {
     for (int k = 0; k < 4; ++k) r.v[k] = c * p->coat + b.v[k] * p->c_scale.v[k];
     return r;
 }
-/* openpbr_pdf: mix(base pdf, coat pdf, coat reflection probability). */
-static real pbr_pdf(const pbr_t *p, v3 wi) {
+/* The coating lobe's pdf: mix(base pdf, coat pdf, coat reflection probability). */
+static real coated_pdf(const pbr_t *p, v3 wi) {
     if (!p->c_on) return base_pdf(p, wi);
     return mixr(base_pdf(p, wi), coat_pdf(p, wi), p->c_prob);
 }
-/* openpbr_sample through the coating lobe (openpbr_coating_lobe.h:258-331): the first random number
- * chooses the coat or the base and is remapped; the other layer's value and pdf are folded in. */
-static int pbr_sample(const pbr_t *p, real r0, real r1, real r2, v3 *wi, s4 *weight, real *pdf) {
+/* The coating lobe's sample (openpbr_coating_lobe.h:258-331): the first random number chooses the coat
+ * or the base and is remapped; the other layer's value and pdf are folded in. */
+static int coated_sample(const pbr_t *p, real r0, real r1, real r2, v3 *wi, s4 *weight, real *pdf) {
     if (!p->c_on) return base_sample(p, r0, r1, r2, wi, weight, pdf);
     const real cp = p->c_prob;
     v3 d;
@@ -710,6 +775,67 @@ static int pbr_sample(const pbr_t *p, real r0, real r1, real r2, v3 *wi, s4 *wei
     *wi = d;
     *pdf = ptot;
     *weight = sscale(comb, (real)1 / ptot);
+    return 1;
+}
+
+/* ---- the fuzz lobe (openpbr_fuzz_lobe.h:179-306, 437-545) */
+static real sheen_pdf(const pbr_t *p, v3 wi) {
+    const v3 wl = to_local(&p->f_basis, wi);
+    if (p->f_wo.z <= 0 || wl.z <= 0) return 0;
+    return ltc_eval(ltc_rotate(wl, p->f_wo.x, vo_get_fault() == 15 ? p->f_wo.y : -p->f_wo.y), p->f_coef);  /* 15: planted */
+}
+static real sheen_value(const pbr_t *p, v3 wi) { return sheen_pdf(p, wi) * (p->fuzz * p->f_coef[2] * (real)1); }
+static int sheen_sample(const pbr_t *p, real u0, real u1, v3 *wi) {
+    if (p->f_wo.z <= 0) return 0;
+    const real phi = (real)6.28318530717958647692 * u0, z = RSQRT(u1), r = RSQRT(1 - u1);
+    const real a = 1 / p->f_coef[0], bi = p->f_coef[1];
+    const v3 o = V(RCOS(phi) * r, RSIN(phi) * r, z);
+    const v3 std_dir = norm(V(o.x * a - o.z * bi * a, o.y * a, o.z));
+    const v3 wl = ltc_rotate(std_dir, p->f_wo.x, p->f_wo.y);
+    if (!(p->f_wo.z * wl.z > 0)) return 0;
+    *wi = norm(to_world(&p->f_basis, wl));
+    return 1;
+}
+
+/* openpbr_eval: the sheen over the coating lobe, which is scaled by 1 - view_reflected; f |cos|. */
+static s4 pbr_eval(const pbr_t *p, v3 wi) {
+    s4 b = coated_eval(p, wi);
+    if (!p->f_on) return b;
+    const real sh = sheen_value(p, wi);
+    s4 r;
+    for (int k = 0; k < 4; ++k) r.v[k] = b.v[k] * ((1 - p->f_refl) * (real)1) + sh;
+    return r;
+}
+/* openpbr_pdf: mix(coating pdf, sheen pdf, sheen probability). */
+static real pbr_pdf(const pbr_t *p, v3 wi) {
+    if (!p->f_on) return coated_pdf(p, wi);
+    return mixr(coated_pdf(p, wi), sheen_pdf(p, wi), p->f_prob);
+}
+/* openpbr_sample through the fuzz lobe: the first random number chooses the sheen or the coating lobe;
+ * the sheen samples its LTC with the other two. */
+static int pbr_sample(const pbr_t *p, real r0, real r1, real r2, v3 *wi, s4 *weight, real *pdf) {
+    if (!p->f_on) return coated_sample(p, r0, r1, r2, wi, weight, pdf);
+    const real sp = p->f_prob, bp = 1 - sp;
+    v3 d;
+    if (r0 < sp) {
+        if (!sheen_sample(p, r1, r2, &d)) return 0;
+        real pd = sheen_pdf(p, d) * sp;
+        pd += coated_pdf(p, d) * bp;
+        *weight = sscale(pbr_eval(p, d), (real)1 / pd);
+        *pdf = pd;
+    } else {
+        real pd;
+        s4 w;
+        if (!coated_sample(p, RFMIN((r0 - sp) * ((real)1 / bp), LARGEST_BELOW_ONE), r1, r2, &d, &w, &pd)) return 0;
+        const real sh = sheen_value(p, d);
+        s4 bc;
+        for (int k = 0; k < 4; ++k) bc.v[k] = (w.v[k] * pd) * ((1 - p->f_refl) * (real)1) + sh;
+        pd *= bp;
+        pd += sheen_pdf(p, d) * sp;
+        *weight = sscale(bc, (real)1 / pd);
+        *pdf = pd;
+    }
+    *wi = d;
     return 1;
 }
 

@@ -135,7 +135,8 @@ float vo_light_tree_pmf(const vo_light_view *t, const float *p, uint32_t positio
 
 /* Scene as the integrators read it; material records are VO_MATERIAL_FLOATS (20) floats per slot:
  * type (u32 bits), albedo rgb | Lagrange multipliers (L0, L1, L2, w: 1 MESE, 2 flat grey) | emission
- * rgb, selection weight | roughness, ior, metalness, transmission | coat weight, specular anisotropy, 2 unused. */
+ * rgb, selection weight | roughness, ior, metalness, transmission | coat weight, specular anisotropy,
+ * fuzz weight, 1 unused. */
 #define VO_MATERIAL_FLOATS 20
 typedef struct vo_render_scene {
     const vo_scene *geo;
@@ -209,12 +210,16 @@ void vo_render_restir_f64(const vo_render_scene *sc, const vo_frame *fr, uint32_
 enum { VO_LUT_DIELECTRIC = 0, VO_LUT_DIELECTRIC_AVG, VO_LUT_DIELECTRIC_RATIO, VO_LUT_OPAQUE, VO_LUT_OPAQUE_AVG,
        VO_LUT_COUNT };
 void vo_energy_set_dielectric(const uint16_t *tables);
+/* The fuzz lobe's LTC table (OpenPBR's openpbr_ltc_data.h, committed as data): 32 x 32 entries
+ * (a_inv, b_inv, R), row = sheen roughness, column = cosine; copied. */
+void vo_ltc_set(const float *table);
+const float *vo_ltc_table(void);
 const uint16_t *vo_energy_table(int id);
 const float *vo_metal_energy_table(void);
 float vo_energy_ior_node(int i);
 
 /* The layered BSDF subset (openpbr_impl.h) on its own: params = type (1 GLOSSY, 4 OPENPBR),
- * roughness, ior, metalness, transmission, coat, anisotropy, tangent xyz; n shading normal on the arriving side, wo toward the
+ * roughness, ior, metalness, transmission, coat, anisotropy, tangent xyz, fuzz; n shading normal on the arriving side, wo toward the
  * viewer, rho base colour per lane, inside = hit from the back of the surface. */
 void vo_bsdf_eval_f32(const float *params, const float *n, const float *wo, const float *wi, const float *rho,
                       int inside, float *fcos, float *pdf);
@@ -242,8 +247,8 @@ void vo_render_nrc_f64(const vo_render_scene *sc, const vo_frame *fr, uint32_t s
  * solid-angle measure, 11 = the environment's cell density drops its 1 / sin(theta), 12 = every
  * environment light sample uses the compensated table (also where no BSDF continuation covers its
  * zeros), 13 = the coat's darkening factor left out (the base under a coat scaled by 1 - in_reflected
- * alone), 14 = the anisotropic alphas swapped (the lobe stretched along the bitangent).  9 is a test
- * switch, not a mistake: ReSTIR GI without its spatial taps. */
+ * alone), 14 = the anisotropic alphas swapped (the lobe stretched along the bitangent), 15 = the sheen
+ * evaluated in a frame turned the wrong way about the normal.  9 is a test switch, not a mistake: ReSTIR GI without its spatial taps. */
 void vo_set_fault(int fault);
 int vo_get_fault(void);
 /* Nearest half-precision value of x (ties to even): what an rgba16f texel store keeps. */

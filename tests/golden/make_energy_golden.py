@@ -7,7 +7,10 @@ layered BSDF, as data, for the oracle and the parity tests.
     (relative IOR, sqrt(alpha), cosine) or 32 x 32, indexed x * 1024 + y * 32 + z
     (openpbr_energy_array_access.h:76-88);
   * the ideal-metal table the reference splices over OpenPBR's metal lookups
-    (Shaders/MetalEnergy.metal: vibeMetalEnergy[129 * 129], vibeMetalAverageEnergy[129]).
+    (Shaders/MetalEnergy.metal: vibeMetalEnergy[129 * 129], vibeMetalAverageEnergy[129]);
+  * the fuzz lobe's LTC table vendored with OpenPBR (Vendor/OpenPBR/impl/data/openpbr_ltc_data.h, from
+    Zeltner, Burley and Chiang's ltc-sheen, Apache-2.0): 32 x 32 (sheen roughness, cosine) entries of
+    (a_inv, b_inv, R), float32.
 Reads the reference checkout in this container only; the npz travels, the reference does not:
     python3 tests/golden/make_energy_golden.py /root/reference
 """
@@ -43,6 +46,14 @@ def metal_arrays(path):
     return out
 
 
+def ltc_array(path):
+    """The 32 x 32 vec3 entries of openpbr_ltc_data.h, row (roughness) major."""
+    text = strip_comments(open(path).read())
+    vals = [float(v) for v in re.findall(r"vec3\(([^)]*)\)", text) for v in v.split(",")]
+    assert len(vals) == 3072, len(vals)
+    return np.array(vals, np.float32).reshape(32, 32, 3)
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     data = os.path.join(ref, "Vendor", "OpenPBR", "impl", "data")
@@ -54,6 +65,7 @@ def main():
     m = metal_arrays(os.path.join(ref, "Shaders", "MetalEnergy.metal"))
     arrays["metal_complement"] = m["vibeMetalEnergy"].reshape(129, 129)
     arrays["metal_avg_complement"] = m["vibeMetalAverageEnergy"]
+    arrays["ltc_sheen"] = ltc_array(os.path.join(data, "openpbr_ltc_data.h"))
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "openpbr_energy_tables.npz")
     np.savez_compressed(out, **arrays)
     print("wrote", out, {k: v.shape for k, v in arrays.items()})

@@ -173,7 +173,8 @@ def materials_cornell(variant="layered", light_size=0.25):
     'opaque': the metal, a half-metallic blue OPENPBR box (roughness 0.25, IOR 1.6) and the plastic floor;
     'coated': coats (roughness 0.12, IOR 1.6) over them -- a fully coated metal, a half-coated rough glass,
     and a red plastic floor under a 0.7 coat; 'anisotropic': a brushed metal, an anisotropic rough glass
-    and a half-metallic floor with a half coat, their specular lobes stretched along each face's tangent."""
+    and a half-metallic floor with a half coat, their specular lobes stretched along each face's tangent;
+    'fuzzy': sheen (roughness 0.5) over a metal, a half-coated blue plastic and a rough glass."""
     import vt
     base = vt.cornell(light_size)
     faces = base.faces.copy()
@@ -195,6 +196,10 @@ def materials_cornell(variant="layered", light_size=0.25):
         mats += [dict(type=1, albedo=(0.95, 0.64, 0.54), roughness=0.35, anisotropy=0.8),
                  dict(type=4, albedo=(1.0, 1.0, 1.0), roughness=0.3, ior=1.5, transmission=1.0, anisotropy=0.5),
                  dict(type=4, albedo=(0.2, 0.4, 0.8), roughness=0.4, ior=1.5, metalness=0.6, anisotropy=0.7, coat=0.5)]
+    elif variant == "fuzzy":
+        mats += [dict(type=1, albedo=(0.95, 0.64, 0.54), roughness=0.4, fuzz=0.8),
+                 dict(type=4, albedo=(0.3, 0.3, 0.8), roughness=0.5, ior=1.5, fuzz=1.0, coat=0.5),
+                 dict(type=4, albedo=(1.0, 1.0, 1.0), roughness=0.3, ior=1.5, transmission=1.0, fuzz=0.6)]
     elif variant == "layered":
         mats += [dict(type=1, albedo=(0.95, 0.64, 0.54), roughness=0.3),
                  dict(type=4, albedo=(1.0, 1.0, 1.0), roughness=0.2, ior=1.5, transmission=1.0),
@@ -216,7 +221,7 @@ def vt_materials(mats):
     import vt
     return [vt.material(m["type"], m["albedo"], m.get("emission", (0, 0, 0)), m.get("roughness", 0.5),
                         m.get("ior", 1.5), m.get("metalness", 0.0), 1.0, m.get("transmission", 0.0), m.get("coat", 0.0),
-                        m.get("anisotropy", 0.0)) for m in mats]
+                        m.get("anisotropy", 0.0), m.get("fuzz", 0.0)) for m in mats]
 
 
 def sky_map(width=64, height=32, sun=(0.3, 0.28), sun_radiance=300.0, seed=2):

@@ -57,7 +57,7 @@ def _compare(got, ref, label, tol):
 
 
 @pytest.mark.parametrize("variant,tol,min_close", [("opaque", 1e-3, 0.97), ("layered", 5e-3, 0.9), ("delta", 5e-3, 0.9),
-                                                  ("coated", 5e-3, 0.9), ("anisotropic", 5e-3, 0.9)])
+                                                  ("coated", 5e-3, 0.9), ("anisotropic", 5e-3, 0.9), ("fuzzy", 5e-3, 0.9)])
 @pytest.mark.parametrize("depth", [3, 8])
 def test_materials_match_oracle(gpu_ctx, variant, tol, min_close, depth):
     # Measured: opaque 8e-6, glass variants 1e-3 to 2.7e-3 (a refraction whose last bit differs sends
@@ -79,7 +79,7 @@ def test_materials_in_z_sampler_mode_match_oracle(gpu_ctx, variant, tol, min_clo
     assert _compare(got, ref, f"z {variant}", tol) > min_close
 
 
-@pytest.mark.parametrize("variant", ["opaque", "coated", "anisotropic"])
+@pytest.mark.parametrize("variant", ["opaque", "coated", "anisotropic", "fuzzy"])
 @pytest.mark.parametrize("mode", [2, 3])
 def test_nee_only_and_bsdf_only_match_oracle(gpu_ctx, mode, variant):
     mesh, mats = scenes.materials_cornell(variant, 0.5)

@@ -54,14 +54,14 @@ def test_triangle_upload_traces_like_the_indexed_upload(gpu_ctx):
 
 def test_surface_settings_render_as_their_materials(gpu_ctx):
     # SurfaceSettings + emissions map onto the materials resolve_material gives them: an enabled slot
-    # is OPENPBR with its colour / roughness / metalness / coat / anisotropy / IOR / transmission, an emission makes an
-    # emitter, a disabled slot is the imported-mesh default (Lambert 0.7).  Same image as the explicit
-    # materials; unsupported inputs are refused.
+    # is OPENPBR with its colour / roughness / metalness / coat / anisotropy / fuzz / IOR / transmission,
+    # an emission makes an emitter, a disabled slot is the imported-mesh default (Lambert 0.7).  Same
+    # image as the explicit materials; texture maps are refused.
     mesh = vt.cornell()
-    settings = [vt.SurfaceSettings(color=(0.73, 0.73, 0.73), roughness=0.4, ior=1.45, coat=0.6),
+    settings = [vt.SurfaceSettings(color=(0.73, 0.73, 0.73), roughness=0.4, ior=1.45, coat=0.6, fuzz=0.5),
                 vt.SurfaceSettings(color=(0.65, 0.05, 0.05), roughness=0.2, metalness=1.0, anisotropy=0.6),
                 vt.SurfaceSettings(enabled=0)]
-    explicit = [vt.material(4, (0.73, 0.73, 0.73), roughness=0.4, ior=1.45, coat=0.6),
+    explicit = [vt.material(4, (0.73, 0.73, 0.73), roughness=0.4, ior=1.45, coat=0.6, fuzz=0.5),
                 vt.material(4, (0.65, 0.05, 0.05), roughness=0.2, ior=1.5, metalness=1.0, anisotropy=0.6),
                 vt.material(0, (0.7, 0.7, 0.7)),
                 vt.material(3, (0.7, 0.7, 0.7), emission=(17.0, 17.0, 17.0))]
@@ -79,7 +79,7 @@ def test_surface_settings_render_as_their_materials(gpu_ctx):
         scene.close()
     assert imgs[0][..., :3].mean() > 0 and np.array_equal(imgs[0], imgs[1])
     scene = gpu_ctx.set_mesh(mesh)
-    for bad in (vt.SurfaceSettings(fuzz=0.3), vt.SurfaceSettings(map_mask=1)):
+    for bad in (vt.SurfaceSettings(map_mask=1),):
         with pytest.raises(vt.VTError):
             scene.set_surface_settings([bad])
     scene.close()
