@@ -384,6 +384,9 @@ __global__ __launch_bounds__(refill_block<kStack>()) void trace_refill_kernel(co
 // software branch).  The instance hierarchy is walked as a one-level one whose leaves name instances,
 // with the boxes grown by enlarge (|origin|max + reach); a leaf's instance traces its asset with the
 // object-space ray, up to the closest hit so far.  Each level keeps its node-group stack in LDS.
+// Both levels run in one loop: each iteration a lane steps either the instance hierarchy or its
+// current instance's asset, so lanes between instances need not wait for another lane's whole asset
+// traversal, as they would with the asset walk nested inside the instance walk.
 // Ties: the closest-hit rule compares (t, instance, face), so the result does not depend on the order
 // instances are visited in.
 template <bool ANY, int kTop, int kBot>
@@ -397,45 +400,64 @@ __global__ __launch_bounds__(kBlock) void trace_tlas_kernel(TlasScene sc, RayArg
     const TraceRay r{rays.ox[i], rays.oy[i], rays.oz[i], rays.dx[i], rays.dy[i], rays.dz[i], rays.tmin[i], rays.tmax[i]};
     LdsStack<kTop> top{top_base, top_bits, lane, 0, kTop};
     LdsStack<kBot> bot{bot_base, bot_bits, lane, 0, kBot};
-    TravState ts;
+    TravState ts, bs;
     trav_init_e(ts, r, sc.enlarge * (max_abs3(r.ox, r.oy, r.oz) + sc.reach));
     float bt = r.tmax, bb1 = 0.0f, bb2 = 0.0f;
-    int32_t bface = -1;
-    uint32_t binst = 0xffffffffu;
-    bool occ = false, overflow = false;
+    int32_t bface = -1, sentinel = -1;
+    uint32_t binst = 0xffffffffu, cur = 0u, items = 0u, base = 0u;
+    bool occ = false, overflow = false, top_done = false, in_asset = false;
     const U4 *tn = reinterpret_cast<const U4 *>(sc.nodes);
+    const U4 *bn = nullptr;
+    const F4 *btr = nullptr;
     for (;;) {
-        uint32_t items, base;
-        const bool done = trav_nodes<false>(ts, tn, top, items, base);
-        overflow |= ts.res.overflow;
-        while (items && !occ) {
+        const bool want_top = !in_asset && items == 0u;
+        if (want_top && top_done) break;
+        // The instance hierarchy is stepped once at least a quarter of the wave's remaining lanes wait
+        // for it; until then those lanes idle while the others walk their assets, so lanes that enter
+        // their instances together stay together.  Instanced icosphere field (1024 instances, 2M
+        // rays): camera 0.718 -> 0.664 ms, bounce 1.858 -> 1.229 ms against the nested walk; stepping
+        // for any waiting lane gave 0.794 / 1.454 ms, waiting for half 0.674 / 1.245, three quarters
+        // 0.713 / 1.432.
+        const uint32_t waiting = (uint32_t)__popcll(__ballot(want_top)), live = (uint32_t)__popcll(__ballot(1));
+        if (want_top) {
+            if (4u * waiting >= live) {
+                top_done = trav_nodes<false>(ts, tn, top, items, base);
+                overflow |= ts.res.overflow;
+            }
+            continue;
+        }
+        if (!in_asset) {
             const uint32_t b = __builtin_ctz(items);
             items &= items - 1u;
             const InstanceRecord &R = sc.instances[base + b];
             if (R.faces == 0) continue;
-            TravState bs;
             trav_init(bs, instance_ray(R.local, r, bt), R.root_max);
             // Equal t: a hit of this instance wins over the current one only from a smaller index.
-            const int32_t sentinel = (bface >= 0 && R.index < binst) ? 0x7fffffff : -1;
+            sentinel = (bface >= 0 && R.index < binst) ? 0x7fffffff : -1;
             bs.res.face = sentinel;
             bot.sp = 0;
-            const U4 *bn = reinterpret_cast<const U4 *>(R.nodes);
-            const F4 *btr = reinterpret_cast<const F4 *>(R.tris);
-            while (!trav_step<ANY, false>(bs, bn, btr, bot)) {
-            }
+            bn = reinterpret_cast<const U4 *>(R.nodes);
+            btr = reinterpret_cast<const F4 *>(R.tris);
+            cur = R.index;
+            in_asset = true;
+        }
+        if (trav_step<ANY, false>(bs, bn, btr, bot)) {
+            in_asset = false;
             overflow |= bs.res.overflow;
             if (ANY) {
-                occ = bs.res.occluded;
+                if (bs.res.occluded) {
+                    occ = true;
+                    break;
+                }
             } else if (bs.res.face != sentinel) {
                 bt = bs.res.t;
                 bb1 = bs.res.b1;
                 bb2 = bs.res.b2;
                 bface = bs.res.face;
-                binst = R.index;
+                binst = cur;
                 ts.res.t = bt;
             }
         }
-        if (occ || done) break;
     }
     if (overflow && sc.fault) atomicOr(sc.fault, 1u);
     if (ANY) {
