@@ -633,9 +633,9 @@ def bench_batch(args, ctx):
             frame = dict(width=w, height=h, eye=eye, target=target, fov=fov, max_depth=6)
             if k == 0:
                 oracle.render(osc, mats, frame, 1, 1, pixels=cpix[:1])   # the first call's one-time costs
-            dt, img = timed_render(oracle.render, osc, mats, frame, 1, 2, pixels=cpix)
+            cpu_dt, img = timed_render(oracle.render, osc, mats, frame, 1, 2, pixels=cpix)
             setup, _ = timed_render(oracle.render, osc, mats, frame, 1, 1, pixels=cpix[:1])
-            spent += dt - setup
+            spent += cpu_dt - setup
             paths += len(cpix) * 2
             acc += float(img.sum()) * 2
             del osc
