@@ -1,7 +1,8 @@
 """CPU tests that pin the oracle's layered-BSDF subset (oracle/openpbr_impl.h) and its energy tables
 (oracle/energy.c): the tables against the reference's own data (OpenPBR's vendored dielectric arrays,
 the reference's MetalEnergy.metal), then by what the mathematics fixes: the GGX single-scattering
-albedo by independent quadrature, Fresnel at zero roughness, sampler-evaluator consistency, pdf
+albedo by independent quadrature, Fresnel at zero roughness, the black dielectric and the smooth metal
+as textbook GGX reflections (value and pdf in closed form), sampler-evaluator consistency, pdf
 normalisation, reciprocity, white-furnace energy conservation of metal, glass and plastic with
 multiple-scattering compensation, and a planted mistake those checks must catch."""
 import math
@@ -227,3 +228,49 @@ def test_planted_shadowing_fault_is_caught():
         f, p = b.eval(N, wo, r[0])
         bad += not np.allclose(r[1], f / p, rtol=1e-4, atol=1e-7)
     assert bad > 10
+
+
+def _ggx_reflection(wo, wi, alpha, fresnel):
+    """Textbook GGX reflection (Walter et al. 2007): f |cos wi| = F D G1(wo) G1(wi) / (4 cos wo), and the
+    visible-normal density D G1(wo) / (4 cos wo) (Heitz 2018)."""
+    h = (wo + wi) / np.linalg.norm(wo + wi)
+    d = _ggx(h[2], alpha)
+    g = _g1(wo[2], alpha) * _g1(wi[2], alpha)
+    return fresnel(float(wo @ h)) * d * g / (4 * wo[2]), d * _g1(wo[2], alpha) / (4 * wo[2])
+
+
+@pytest.mark.parametrize("rough,ior", [(0.3, 1.5), (0.6, 1.33), (0.2, 2.0)])
+def test_black_dielectric_is_textbook_ggx_reflection(rough, ior):
+    # A black opaque dielectric has no diffuse, transmission or multiple-scattering term: its BSDF is
+    # the GGX reflection with dielectric Fresnel, and the aggregate pdf that lobe's visible-normal pdf.
+    b = oracle.BSDF(4, rough, ior, 0.0, 0.0, rho=(0, 0, 0, 0))
+    rng = np.random.default_rng(int(rough * 100 + ior * 10))
+    for _ in range(60):
+        wo = _wo(rng.uniform(0.15, 0.95), rng.uniform(0, 6.2))
+        wi = _wo(rng.uniform(0.1, 0.99), rng.uniform(0, 6.2))
+        f, pdf = b.eval(N, wo, wi)
+        want_f, want_pdf = _ggx_reflection(wo, wi, rough * rough, lambda c: _fresnel(ior, c))
+        assert np.allclose(f, want_f, rtol=1e-4, atol=1e-8), (f, want_f)
+        assert abs(pdf - want_pdf) <= 1e-4 * want_pdf + 1e-8, (pdf, want_pdf)
+
+
+def test_smooth_metal_is_textbook_ggx_with_schlick():
+    # Below roughness 0.04 the metal's multiple-scattering lobe is off: the BSDF is GGX with the F82
+    # Fresnel, which for a white tint is Schlick's f0 + (1 - f0) (1 - c)^5.
+    rho = np.array([0.95, 0.64, 0.54, 0.5])
+    rough = 0.038
+    b = oracle.BSDF(1, rough, rho=tuple(rho))
+    rng = np.random.default_rng(17)
+    n = 0
+    for _ in range(80):
+        wo = _wo(rng.uniform(0.2, 0.95), rng.uniform(0, 6.2))
+        wi = np.array([-wo[0], -wo[1], wo[2]]) + rng.normal(0, 0.01, 3)
+        wi /= np.linalg.norm(wi)
+        f, pdf = b.eval(N, wo, wi)
+        for k in range(4):
+            want, want_pdf = _ggx_reflection(wo, wi, rough * rough,
+                                             lambda c: rho[k] + (1 - rho[k]) * (1 - min(max(c, 0), 1)) ** 5)
+            assert abs(f[k] - want) <= 2e-4 * want + 1e-8, (k, f[k], want)
+        assert abs(pdf - want_pdf) <= 2e-4 * want_pdf + 1e-8
+        n += 1
+    assert n == 80
