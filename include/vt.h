@@ -360,6 +360,23 @@ vt_status vt_spectral_tables(float *icdf, float *cmf, float *basis);
  * (L0, L1, L2, 1) bounded-MESE multipliers, or (grey, 0, 0, 2). */
 vt_status vt_spectral_reflectance(const float rgb[3], float out[4]);
 
+/* The reference's reflectance-uplift grid built on the context's device, as PathTracerRenderer.
+ * refineSpectralGrid builds it once when the grid is loaded (spectral_refine_flags_kernel,
+ * spectral_refine_solve_kernel, main.swift:8187-8278): the exact moments of the 86^3 coarse nodes
+ * (8-bit codes 0, 3, ..., 255: the generator's Levenberg-Marquardt solve and 1024-node moments,
+ * scripts/generate_spectral_tables.py:381-386, in double -- the FourierSRGB86 grid), the cells of the
+ * 85^3 whose 8-bit codes the coarse conversion misses by more than 0.35 steps (every code tested, in
+ * float as the reference's kernel), and the exact moments of the 4 x 4 x 4 codes of each such cell
+ * (float Levenberg-Marquardt from the interpolated start, as the reference's kernel).  Outputs (host;
+ * any may be NULL, cells and refined when capacity is 0): coarse = 86^3 x 3 floats, node (r, g, b) at
+ * (r 86 + g) 86 + b; flags = 85^3 bytes, cell (i, j, k) at (i 85 + j) 85 + k; cells = the first
+ * `capacity` flagged cells in increasing order and refined = their 64 x 3 floats each, code
+ * (3i + a, 3j + b, 3k + c) at entry 16 a + 4 b + c (codes clamped to 255); *count = the number of
+ * flagged cells.  Allocates its scratch (about 17 MB) and synchronises.  The materials' own uplift
+ * (vt_spectral_reflectance) solves the codes it needs on the host in double. */
+vt_status vt_uplift_grid_build(vt_context *ctx, float *coarse, uint8_t *flags, uint32_t capacity, uint32_t *cells,
+                               float *refined, uint32_t *count);
+
 /* The layered BSDF's energy-compensation tables as the context bakes them on its device (baking them
  * first if needed): metal = 129 x 129 ideal-metal complements then 129 averages (the table
  * scripts/generate_metal_energy.py writes into Shaders/MetalEnergy.metal), luts = 5 x 32 x 32 x 32

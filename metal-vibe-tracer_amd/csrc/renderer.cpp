@@ -13,6 +13,7 @@
 #include "render_kernels.h"
 #include "scene.h"
 #include "spectral.h"
+#include "uplift.h"
 #include "vt.h"
 
 using vt::check;
@@ -417,6 +418,73 @@ vt_status vt_spectral_reflectance(const float rgb[3], float out[4]) {
     return vt::guarded_call([&]() -> vt_status {
         if (!rgb || !out) return fail(VT_ERROR_INVALID_ARGUMENT, "rgb and out must not be NULL");
         vt::reflectance_lagrange(host_tables(), rgb, out);
+        return VT_OK;
+    });
+}
+
+vt_status vt_uplift_grid_build(vt_context *ctx, float *coarse, uint8_t *flags, uint32_t capacity, uint32_t *cells,
+                               float *refined, uint32_t *count) {
+    return vt::guarded_call([&]() -> vt_status {
+        if (!ctx) return fail(VT_ERROR_INVALID_ARGUMENT, "context must not be NULL");
+        if (capacity && (!cells || !refined))
+            return fail(VT_ERROR_INVALID_ARGUMENT, "cells and refined must not be NULL when capacity > 0");
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        constexpr size_t kS = vt::kSpectralSamples, kNodes = 86, kCells = 85;
+        std::vector<double> w(3 * kS), c1(kS), c2(kS);
+        std::vector<float> wf(3 * kS), c1f(kS), c2f(kS), nodes(kNodes);
+        vt::uplift_solve_tables(host_tables(), w.data(), c1.data(), c2.data(), nodes.data());
+        for (size_t i = 0; i < 3 * kS; ++i) wf[i] = (float)w[i];
+        for (size_t i = 0; i < kS; ++i) {
+            c1f[i] = (float)c1[i];
+            c2f[i] = (float)c2[i];
+        }
+        vt::DeviceBuffer dw, dc1, dc2, dwf, dc1f, dc2f, dnodes, dgrid, dflags, dcells, drefined;
+        dw.upload(w.data(), w.size() * 8);
+        dc1.upload(c1.data(), kS * 8);
+        dc2.upload(c2.data(), kS * 8);
+        dwf.upload(wf.data(), wf.size() * 4);
+        dc1f.upload(c1f.data(), kS * 4);
+        dc2f.upload(c2f.data(), kS * 4);
+        dnodes.upload(nodes.data(), kNodes * 4);
+        const vt::UpliftTables tb{dw.as<const double>(), dc1.as<const double>(), dc2.as<const double>(),
+                                  dwf.as<const float>(), dc1f.as<const float>(), dc2f.as<const float>()};
+        const size_t ncells = kCells * kCells * kCells;
+        dgrid.alloc(kNodes * kNodes * kNodes * 16);
+        dflags.alloc(ncells);
+        check(hipMemset(dflags.ptr, 0, ncells), "hipMemset");
+        check(vt::launch_uplift_coarse(tb, dgrid.as<float4>(), nullptr), "uplift coarse nodes");
+        check(vt::launch_uplift_flags(tb, dgrid.as<const float4>(), dnodes.as<const float>(), dflags.as<uint8_t>(), nullptr),
+              "uplift refinement flags");
+        std::vector<uint8_t> hflags(ncells);
+        check(hipMemcpy(hflags.data(), dflags.ptr, ncells, hipMemcpyDeviceToHost), "hipMemcpy");
+        std::vector<uint32_t> list;
+        for (size_t c = 0; c < ncells; ++c)
+            if (hflags[c]) list.push_back((uint32_t)c);
+        const uint32_t n = (uint32_t)list.size();
+        if (n) {
+            dcells.upload(list.data(), list.size() * 4);
+            drefined.alloc((size_t)n * 64 * 16);
+            check(vt::launch_uplift_refine(tb, dgrid.as<const float4>(), dnodes.as<const float>(), dcells.as<const uint32_t>(),
+                                           n, drefined.as<float4>(), nullptr),
+                  "uplift refined cells");
+        }
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        if (coarse) {
+            std::vector<float> g(kNodes * kNodes * kNodes * 4);
+            check(hipMemcpy(g.data(), dgrid.ptr, g.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+            for (size_t i = 0; i < kNodes * kNodes * kNodes; ++i)
+                for (int k = 0; k < 3; ++k) coarse[3 * i + k] = g[4 * i + k];
+        }
+        if (flags) std::memcpy(flags, hflags.data(), ncells);
+        const uint32_t keep = std::min(capacity, n);
+        if (keep) {
+            std::vector<float> r((size_t)keep * 64 * 4);
+            check(hipMemcpy(r.data(), drefined.ptr, r.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+            std::memcpy(cells, list.data(), keep * 4);
+            for (size_t i = 0; i < (size_t)keep * 64; ++i)
+                for (int k = 0; k < 3; ++k) refined[3 * i + k] = r[4 * i + k];
+        }
+        if (count) *count = n;
         return VT_OK;
     });
 }

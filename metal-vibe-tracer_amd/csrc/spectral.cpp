@@ -338,6 +338,17 @@ UpliftGrid &uplift_grid(const SpectralTables &t) {
 
 }  // namespace
 
+void uplift_solve_tables(const SpectralTables &t, double *w, double *c1, double *c2, float *nodes) {
+    SolveData s;
+    solve_data(t, s);
+    for (int i = 0; i < kSpectralSamples; ++i) {
+        for (int k = 0; k < 3; ++k) w[k * kSpectralSamples + i] = s.w[k][i];
+        c1[i] = s.c1[i];
+        c2[i] = s.c2[i];
+    }
+    for (int i = 0; i < 86; ++i) nodes[i] = (float)srgb_eotf(3.0 * i / 255.0);
+}
+
 double solve_lagrange(const SpectralTables &t, const double target[3], double lam[3]) {
     SolveData s;
     solve_data(t, s);

@@ -38,4 +38,9 @@ void reflectance_lagrange(const SpectralTables &t, const float rgb[3], float out
 // D65: Levenberg-Marquardt in multiplier space.  Returns the max channel error.
 double solve_lagrange(const SpectralTables &t, const double target[3], double lagrange[3]);
 
+// The uplift solve's per-nanometre data: w = 3 rows of kSpectralSamples reflectance -> linear sRGB
+// weights under D65, c1 = 2 cos(phi), c2 = 2 cos(2 phi) of the warped phase; nodes = the 86 coarse
+// nodes' linear values (codes 0, 3, ..., 255).
+void uplift_solve_tables(const SpectralTables &t, double *w, double *c1, double *c2, float *nodes);
+
 }  // namespace vt

@@ -39,6 +39,7 @@ EXPORTED = [
     "vt_scene_upload_triangles", "vt_scene_set_surface_settings", "vt_renderer_set_callbacks",
     "vt_trace_stats_noalloc", "vt_tlas_create", "vt_tlas_destroy", "vt_tlas_get_info", "vt_tlas_trace_closest",
     "vt_tlas_trace_any", "vt_scene_set_environment", "vt_environment_tables", "vt_hierarchy_build_host",
+    "vt_uplift_grid_build",
 ]
 
 
@@ -194,6 +195,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "vt_render_frame": (i32, [vp, vp, P(_FrameParams), u32, vp]),
         "vt_reset_accumulation": (i32, [vp]),
         "vt_energy_tables": (i32, [vp, vp, vp]),
+        "vt_uplift_grid_build": (i32, [vp, vp, vp, u32, vp, vp, P(u32)]),
         "vt_nrc_create": (i32, [vp, u32, P(vp)]),
         "vt_nrc_destroy": (None, [vp]),
         "vt_nrc_get_params": (i32, [vp, vp]),
@@ -619,6 +621,20 @@ class Context:
         luts = np.empty((5, 32, 32, 32), np.float32)
         _check(load().vt_energy_tables(self._h, _ptr(metal), _ptr(luts)))
         return metal[:129 * 129].reshape(129, 129), metal[129 * 129:], luts
+
+    def uplift_grid(self, capacity=8192):
+        """The reflectance-uplift grid built on the device (vt_uplift_grid_build): coarse moments
+        (86, 86, 86, 3), refinement flags (85, 85, 85) bool, flagged cell indices (n,) and their codes'
+        moments (n, 4, 4, 4, 3), n = min(count, capacity), and the flagged-cell count."""
+        coarse = np.empty((86, 86, 86, 3), np.float32)
+        flags = np.empty((85, 85, 85), np.uint8)
+        cells = np.empty(capacity, np.uint32)
+        refined = np.empty((capacity, 4, 4, 4, 3), np.float32)
+        count = ctypes.c_uint32(0)
+        _check(load().vt_uplift_grid_build(self._h, _ptr(coarse), _ptr(flags), capacity, _ptr(cells), _ptr(refined),
+                                           ctypes.byref(count)))
+        n = min(count.value, capacity)
+        return coarse, flags.astype(bool), cells[:n].copy(), refined[:n].copy(), count.value
 
     def set_mesh(self, mesh: Mesh, leaf_size=0, bins=0, max_depth=0, collapse=0) -> "Scene":
         """MaterialLibrary.setMesh for an indexed mesh: builds and uploads the BVH."""
