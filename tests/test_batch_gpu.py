@@ -195,3 +195,26 @@ def test_autotune_and_errors(gpu_ctx):
     delta.close()
     for s, _, _ in sc:
         s.close()
+
+
+def test_batch_with_material_layers_matches_single_renders(gpu_ctx):
+    # Coats, anisotropic specular lobes and fuzz go through the batch kernels' layered shading too: each
+    # job equals its single-image render (the anisotropic scene's tangents come from its vertices' uv).
+    made = []
+    for variant in ("coated", "anisotropic", "fuzzy"):
+        mesh, mats = scenes.materials_cornell(variant)
+        s = gpu_ctx.set_mesh(mesh)
+        s.set_materials(scenes.vt_materials(mats))
+        made.append(s)
+    jobs = [(s, 64, 64, vt.orbit_eye(), (0.0, -0.1, 0.0), 42.0) for s in made]
+    b = vt.Batch(gpu_ctx, jobs, max_depth=DEPTH)
+    b.render(4)
+    for k, job in enumerate(jobs):
+        got, ref = b.readback(k), _single(gpu_ctx, job, 4)
+        close = np.isclose(got, ref, rtol=1e-4, atol=1e-6).all(axis=2).mean()
+        print(f"layers job {k}: batch mean {got[..., :3].mean():.6f} single {ref[..., :3].mean():.6f} close {close:.4f}",
+              flush=True)
+        assert got[..., :3].mean() > 0 and close > 0.999
+    b.close()
+    for s in made:
+        s.close()

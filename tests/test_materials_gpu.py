@@ -88,11 +88,12 @@ def test_nee_only_and_bsdf_only_match_oracle(gpu_ctx, mode, variant):
     assert _compare(got, ref, f"{variant} mode {mode}", 2e-3) > 0.95
 
 
-def test_restir_with_layered_primaries_matches_oracle(gpu_ctx):
-    mesh, mats = scenes.materials_cornell("opaque")
+@pytest.mark.parametrize("variant", ["opaque", "fuzzy"])
+def test_restir_with_layered_primaries_matches_oracle(gpu_ctx, variant):
+    mesh, mats = scenes.materials_cornell(variant)
     got = _render(gpu_ctx, mesh, mats, 96, 8, 3, 0)
     ref = _oracle(mesh, mats, 96, 8, 3, 0)
-    assert _compare(got, ref, "restir opaque", 2e-3) > 0.95
+    assert _compare(got, ref, f"restir {variant}", 2e-3) > 0.95
 
 
 def test_material_sort_gives_the_same_image(gpu_ctx):
