@@ -1,5 +1,6 @@
 """CPU tests of the integrator oracle with the reference's other material types (GLOSSY metal and
-mirror, DIELECTRIC glass, OPENPBR plastic and rough glass): the three sampling strategies estimate
+mirror, DIELECTRIC glass, OPENPBR plastic and rough glass, with coats, anisotropic specular lobes and
+fuzz): the three sampling strategies estimate
 the same image (the reference's cross-strategy check, tests/Fix_integrator.swift:188-215), and the
 float integrator agrees with the FP64 one."""
 import numpy as np
@@ -15,7 +16,7 @@ def _frame(w, depth, mode=1):
                 sampling_mode=mode)
 
 
-@pytest.mark.parametrize("variant", ["layered", "delta", "opaque"])
+@pytest.mark.parametrize("variant", ["layered", "delta", "opaque", "coated", "anisotropic", "fuzzy"])
 @pytest.mark.parametrize("depth", [2, 3])
 def test_strategies_agree(variant, depth):
     mesh, mats = scenes.materials_cornell(variant, 0.5)
