@@ -69,7 +69,7 @@ def test_materials_match_oracle(gpu_ctx, variant, tol, min_close, depth):
 
 
 @pytest.mark.parametrize("variant,tol,min_close", [("opaque", 1e-3, 0.97), ("layered", 5e-3, 0.9), ("delta", 5e-3, 0.9),
-                                                  ("coated", 5e-3, 0.9)])
+                                                  ("coated", 5e-3, 0.9), ("fuzzy", 5e-3, 0.9)])
 def test_materials_in_z_sampler_mode_match_oracle(gpu_ctx, variant, tol, min_close):
     """Z mode through every BSDF sampler: the layered OpenPBR lobes (one 3D constituent per scatter),
     Lambert, and the dielectric's 1D draws."""
